@@ -1,0 +1,123 @@
+"""Loader of libwindhip.so (the HIP solver) and ctypes mirrors of include/windhip.h.
+
+The library is built in-tree by ``__graft_entry__.build()`` (hicar_amd/csrc/Makefile).
+There is no fallback: if the library is missing this module raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libwindhip.so")
+
+WINDHIP_OK = 0
+WINDHIP_ERR_ARG = 1
+WINDHIP_ERR_HIP = 2
+WINDHIP_ERR_NAN = 3
+WINDHIP_ERR_DIVERGED = 4
+WINDHIP_ERR_NOMEM = 5
+
+WINDHIP_F32 = 4
+WINDHIP_F64 = 8
+WINDHIP_HOST = 0
+WINDHIP_DEVICE = 1
+
+START_WARM, START_ZERO, START_GIVEN, START_LEARNED = 0, 1, 2, 3
+PLAIN_F64, MIXED = 0, 1
+
+# every entry point declared in include/windhip.h
+EXPORTS = (
+    "windhip_default_opts", "windhip_create", "windhip_destroy", "windhip_last_error",
+    "windhip_set_metrics", "windhip_solve", "windhip_apply_correction", "windhip_calc_iter_winds",
+    "windhip_divergence", "windhip_get_lambda", "windhip_set_lambda", "windhip_probe_operator",
+    "windhip_time_kernel", "windhip_memory", "windhip_set_alpha", "windhip_apply_operator",
+    "windhip_get_rows",
+)
+
+
+class Desc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("dx", C.c_double),
+                ("ims", C.c_int), ("ime", C.c_int), ("kms", C.c_int), ("kme", C.c_int),
+                ("jms", C.c_int), ("jme", C.c_int), ("field_dtype", C.c_int), ("field_mem", C.c_int)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("tol_rel", C.c_double), ("tol_abs", C.c_double), ("max_iters", C.c_int),
+                ("start", C.c_int), ("precision", C.c_int), ("inner_tol", C.c_double),
+                ("inner_max_iters", C.c_int), ("max_outer", C.c_int), ("iters_per_sync", C.c_int),
+                ("use_graph", C.c_int), ("retry", C.c_int)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("status", C.c_int), ("iters", C.c_int), ("nsweeps", C.c_int), ("outer_iters", C.c_int),
+                ("res0", C.c_double), ("res_final", C.c_double), ("bnorm", C.c_double),
+                ("true_rel_res", C.c_double), ("ms", C.c_double), ("start_used", C.c_int),
+                ("reserved", C.c_int)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class WindHipError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"windhip error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load libwindhip.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() (hicar_amd/csrc/Makefile)")
+    lib = C.CDLL(LIB_PATH)
+    vp, i, d = C.c_void_p, C.c_int, C.c_double
+    pp = C.POINTER(C.c_void_p)
+    dp = C.POINTER(C.c_double)
+    sig = {
+        "windhip_default_opts": (None, [C.POINTER(SolveOpts)]),
+        "windhip_create": (i, [C.POINTER(Desc), pp]),
+        "windhip_destroy": (i, [vp]),
+        "windhip_last_error": (C.c_char_p, []),
+        "windhip_set_metrics": (i, [vp] + [vp] * 15),
+        "windhip_solve": (i, [vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
+        "windhip_apply_correction": (i, [vp, vp, vp, vp, vp, i]),
+        "windhip_calc_iter_winds": (i, [vp, vp, vp, vp, vp, vp, vp, i, C.POINTER(SolveOpts),
+                                        C.POINTER(SolveInfo)]),
+        "windhip_divergence": (i, [vp, vp, vp, vp, vp, i, i, i, vp]),
+        "windhip_get_lambda": (i, [vp, dp]),
+        "windhip_set_lambda": (i, [vp, dp]),
+        "windhip_probe_operator": (i, [vp, vp, vp, i, dp]),
+        "windhip_time_kernel": (i, [vp, i, i, dp]),
+        "windhip_memory": (i, [vp, dp, dp]),
+        "windhip_set_alpha": (i, [vp, vp]),
+        "windhip_apply_operator": (i, [vp, i, dp, dp]),
+        "windhip_get_rows": (i, [vp, i, dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    if rc != WINDHIP_OK:
+        msg = load().windhip_last_error()
+        raise WindHipError(rc, msg.decode() if msg else "")
+
+
+def default_opts(**kw) -> SolveOpts:
+    o = SolveOpts()
+    load().windhip_default_opts(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o

@@ -1,0 +1,1041 @@
+// windhip.hip -- host side of the MI355X wind solver: the C-ABI of
+// include/windhip.h, device state of one nest, and the orchestration of
+// calc_iter_winds (src/physics/wind_iterative.F90:295-532).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "windhip.h"
+#include "windhip_kernels.hpp"
+#include "windhip_setup.hpp"
+
+using namespace windhip;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(WINDHIP_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                \
+    } while (0)
+
+#define CHK(expr)                 \
+    do {                          \
+        int rc_ = (expr);         \
+        if (rc_ != WINDHIP_OK)    \
+            return rc_;           \
+    } while (0)
+
+int flat_grid(long n)
+{
+    long g = (n + kBlock - 1) / kBlock;
+    return (int)std::max(1L, std::min(g, 16384L));
+}
+
+}  // namespace
+
+struct windhip_solver {
+    windhip_desc desc{};
+    Dims d{};
+    Geo g{};
+    long N = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    size_t bytes = 0;
+    std::vector<void *> allocs;
+
+    // metrics on the row grid (FP64) and planes
+    double *m_dzdx, *m_dzdy, *m_jaco, *m_jaco_w, *m_alpha, *m_adv_dz, *m_dz_if;
+    double *m_dzdx_u, *m_jaco_u, *m_dzdy_v, *m_jaco_v;
+    double *p_terrain, *p_sx, *p_sy;
+    double *mf[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // mx_u, my_u, my_v, mx_v, mxy
+    // operator
+    double *B, *C, *H, *I, *L, *M;
+    float *B32, *C32, *H32, *I32, *L32, *M32;
+    double *bcx, *bcy, *bcz, *dztop;
+    float *bcx32, *bcy32, *bcz32, *dztop32;
+    double *P15[15] = {};
+    float *P15_32[15] = {};
+    double *dinv;
+    float *dinv32;
+    // Krylov FP64
+    double *x, *r, *rh, *p, *ph[3], *v, *s, *sh[3], *t, *b;
+    // Krylov FP32 (inner solve of the refinement)
+    float *x32 = nullptr, *r32, *rh32, *p32, *ph32[3], *v32, *s32, *sh32[3], *t32;
+    // reductions and state
+    double *partials;
+    double *scan_part, *scan_out;
+    unsigned *scan_cnt;
+    KState *st;       // [0] FP64 solve / outer, [1] FP32 inner
+    KState *st_host;  // pinned mirror
+    double *scan_host;
+    // staging for host-memory fields (caller layout, caller dtype)
+    void *stage[6] = {};
+    size_t stage_bytes[6] = {};
+    double *scratch_uvw = nullptr;  // probing: u, v, w, T in natural layout
+    unsigned long long *leak = nullptr;
+    double **cf_dev = nullptr;
+
+    bool metrics_set = false, structure_uploaded = false, probed = false;
+    int nsweeps = 2;  // precond_n_sweeps, BASE_PREC_SWEEPS (wind_iterative.F90:76)
+    std::map<long, hipGraphExec_t> graphs;
+
+    // ---------------------------------------------------------------- helpers
+    template <typename T>
+    int alloc(T **ptr, long count, bool zero = true)
+    {
+        size_t nb = sizeof(T) * (size_t)std::max(count, 1L);
+        void *q = nullptr;
+        hipError_t e = hipMalloc(&q, nb);
+        if (e != hipSuccess) return fail(WINDHIP_ERR_NOMEM, "hipMalloc(%zu) failed: %s", nb, hipGetErrorString(e));
+        allocs.push_back(q);
+        bytes += nb;
+        if (zero) HIPCHK(hipMemsetAsync(q, 0, nb, stream));
+        *ptr = (T *)q;
+        return WINDHIP_OK;
+    }
+
+    size_t esize() const { return (size_t)desc.field_dtype; }
+    int ni_mass() const { return desc.ime - desc.ims + 1; }
+    int nk_mass() const { return desc.kme - desc.kms + 1; }
+    int nj_mass() const { return desc.jme - desc.jms + 1; }
+    // element counts of the caller's arrays
+    long n_mass() const { return (long)ni_mass() * nk_mass() * nj_mass(); }
+    long n_u() const { return (long)(ni_mass() + 1) * nk_mass() * nj_mass(); }
+    long n_v() const { return (long)ni_mass() * nk_mass() * (nj_mass() + 1); }
+    long n_2d(int kind) const
+    {
+        return kind == 1 ? (long)(ni_mass() + 1) * nj_mass() : kind == 2 ? (long)ni_mass() * (nj_mass() + 1)
+                                                                      : (long)ni_mass() * nj_mass();
+    }
+
+    // Device view of a caller array. For host memory it is copied into staging
+    // slot `slot`; `copy_in` false only sizes the slot (outputs).
+    int view(const void *host_or_dev, long count, int slot, bool copy_in, void **dev)
+    {
+        if (desc.field_mem == WINDHIP_DEVICE) {
+            *dev = const_cast<void *>(host_or_dev);
+            return WINDHIP_OK;
+        }
+        size_t nb = (size_t)count * esize();
+        if (stage_bytes[slot] < nb) {
+            if (stage[slot]) {
+                HIPCHK(hipStreamSynchronize(stream));
+                HIPCHK(hipFree(stage[slot]));
+                bytes -= stage_bytes[slot];
+            }
+            HIPCHK(hipMalloc(&stage[slot], nb));
+            stage_bytes[slot] = nb;
+            bytes += nb;
+        }
+        if (copy_in) HIPCHK(hipMemcpyAsync(stage[slot], host_or_dev, nb, hipMemcpyHostToDevice, stream));
+        *dev = stage[slot];
+        return WINDHIP_OK;
+    }
+
+    int copy_out(void *host_or_dev, long count, int slot)
+    {
+        if (desc.field_mem == WINDHIP_DEVICE) return WINDHIP_OK;
+        HIPCHK(hipMemcpyAsync(host_or_dev, stage[slot], (size_t)count * esize(), hipMemcpyDeviceToHost, stream));
+        return WINDHIP_OK;
+    }
+
+    template <typename F>
+    Fld<F> fld(void *p, int kind) const
+    {
+        // kind 0 mass, 1 u-grid, 2 v-grid
+        Fld<F> f;
+        f.p = (F *)p;
+        f.i0 = desc.ims; f.k0 = desc.kms; f.j0 = desc.jms;
+        f.ni = ni_mass() + (kind == 1 ? 1 : 0);
+        f.nk = nk_mass();
+        return f;
+    }
+
+    Metrics metrics() const
+    {
+        Metrics m;
+        m.dzdx = m_dzdx; m.dzdy = m_dzdy; m.jaco = m_jaco; m.jaco_w = m_jaco_w; m.alpha = m_alpha;
+        m.adv_dz = m_adv_dz; m.dz_if = m_dz_if; m.dzdx_u = m_dzdx_u; m.jaco_u = m_jaco_u;
+        m.dzdy_v = m_dzdy_v; m.jaco_v = m_jaco_v; m.sx = p_sx; m.sy = p_sy;
+        m.mf_mx_u = mf[0]; m.mf_my_u = mf[1]; m.mf_my_v = mf[2]; m.mf_mx_v = mf[3]; m.mf_mxy = mf[4];
+        return m;
+    }
+
+    template <typename T>
+    OpA<T> opa() const;
+    template <typename T>
+    OpP<T> opp() const;
+
+    // gather a caller field into the row grid (FP64)
+    int gather3(const void *src, int kind, int ia, int ib, int ka, int kb, int ja, int jb, double *dst,
+                int slot, double scale = 1.0)
+    {
+        long cnt = kind == 1 ? n_u() : kind == 2 ? n_v() : n_mass();
+        void *dv = nullptr;
+        CHK(view(src, cnt, slot, true, &dv));
+        long tot = (long)(ib - ia + 1) * (kb - ka + 1) * (jb - ja + 1);
+        if (desc.field_dtype == WINDHIP_F32)
+            k_gather3<float><<<flat_grid(tot), kBlock, 0, stream>>>(d, fld<float>(dv, kind), dst, ia, ib, ka, kb, ja, jb, scale);
+        else
+            k_gather3<double><<<flat_grid(tot), kBlock, 0, stream>>>(d, fld<double>(dv, kind), dst, ia, ib, ka, kb, ja, jb, scale);
+        HIPCHK(hipGetLastError());
+        return WINDHIP_OK;
+    }
+
+    int gather2(const void *src, int kind, int ia, int ib, int ja, int jb, double *dst, int slot)
+    {
+        void *dv = nullptr;
+        CHK(view(src, n_2d(kind), slot, true, &dv));
+        long tot = (long)(ib - ia + 1) * (jb - ja + 1);
+        Fld<float> ff;
+        Fld<double> fd;
+        ff.p = (float *)dv; fd.p = (double *)dv;
+        ff.i0 = fd.i0 = desc.ims; ff.j0 = fd.j0 = desc.jms; ff.k0 = fd.k0 = 0;
+        ff.ni = fd.ni = ni_mass() + (kind == 1 ? 1 : 0);
+        ff.nk = fd.nk = 1;
+        if (desc.field_dtype == WINDHIP_F32)
+            k_gather2<float><<<flat_grid(tot), kBlock, 0, stream>>>(d, ff, dst, ia, ib, ja, jb);
+        else
+            k_gather2<double><<<flat_grid(tot), kBlock, 0, stream>>>(d, fd, dst, ia, ib, ja, jb);
+        HIPCHK(hipGetLastError());
+        return WINDHIP_OK;
+    }
+
+    int init();
+    int assemble(bool full);
+    int prepare_call(const void *alpha, const void *div);
+    int refresh_operator(double amin, double amax);
+    int scan(double *amin, double *amax, double *nans);
+    int set_alpha(const void *alpha);
+    int read_state(int idx);
+    int run_plain(int max_iters, double tol_rel, double tol_abs, int batch, bool use_graph, int *status,
+                  int *iters, double *res0, double *resf);
+    int run_mixed(const windhip_solve_opts &o, windhip_solve_info *info);
+    int true_residual(double *rel);
+    int enqueue_iterations_f64(int count);
+    int enqueue_iterations_f32(int count);
+    int launch_batch(int precision, int count, bool use_graph);
+    int alloc_f32_krylov();
+    int correction(void *u, void *v, void *w, const void *density, int adv_den);
+    int divergence(const void *u, const void *v, const void *w, const void *density, int adv_den,
+                   int horz_only, int use_dqdt, void *div_out);
+    int probe(const void *alpha, const void *density, int adv_den, double *max_leak);
+    void release();
+};
+
+template <>
+OpA<double> windhip_solver::opa<double>() const
+{
+    OpA<double> o;
+    o.B = B; o.C = C; o.H = H; o.I = I; o.L = L; o.M = M;
+    o.bcx = bcx; o.bcy = bcy; o.bcz = bcz; o.dztop = dztop;
+    o.inv = 1.0 / (desc.dx * desc.dx);
+    o.neg4 = -4.0 * o.inv;
+    return o;
+}
+template <>
+OpA<float> windhip_solver::opa<float>() const
+{
+    OpA<float> o;
+    o.B = B32; o.C = C32; o.H = H32; o.I = I32; o.L = L32; o.M = M32;
+    o.bcx = bcx32; o.bcy = bcy32; o.bcz = bcz32; o.dztop = dztop32;
+    double inv = 1.0 / (desc.dx * desc.dx);
+    o.inv = (float)inv;
+    o.neg4 = (float)(-4.0 * inv);
+    return o;
+}
+template <>
+OpP<double> windhip_solver::opp<double>() const
+{
+    OpP<double> o;
+    for (int q = 0; q < 15; ++q) o.cf[q] = P15[q];
+    return o;
+}
+template <>
+OpP<float> windhip_solver::opp<float>() const
+{
+    OpP<float> o;
+    for (int q = 0; q < 15; ++q) o.cf[q] = P15_32[q];
+    return o;
+}
+
+int windhip_solver::init()
+{
+    d.nx = desc.nx; d.ny = desc.ny; d.nz = desc.nz;
+    d.mx = desc.nx + 2; d.my = desc.ny + 2; d.mz = desc.nz + 2;
+    d.P = (long)d.mx * d.my;
+    d.dx = desc.dx;
+    g.mx = d.mx; g.my = d.my; g.mz = d.mz; g.P = d.P;
+    g.nb = (int)((d.P + kBlock - 1) / kBlock);
+    N = d.P * d.mz;
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&ev0));
+    HIPCHK(hipEventCreate(&ev1));
+    double **rows64[] = {&m_dzdx, &m_dzdy, &m_jaco, &m_jaco_w, &m_alpha, &m_adv_dz, &m_dz_if,
+                         &m_dzdx_u, &m_jaco_u, &m_dzdy_v, &m_jaco_v, &B, &C, &H, &I, &L, &M,
+                         &dinv, &x, &r, &rh, &p, &ph[0], &ph[1], &ph[2], &v, &s, &sh[0], &sh[1], &sh[2],
+                         &t, &b};
+    for (double **q : rows64) CHK(alloc(q, N));
+    float **rows32[] = {&B32, &C32, &H32, &I32, &L32, &M32, &dinv32};
+    for (float **q : rows32) CHK(alloc(q, N));
+    double **pl64[] = {&p_terrain, &p_sx, &p_sy, &bcx, &bcy, &bcz, &dztop};
+    for (double **q : pl64) CHK(alloc(q, d.P));
+    float **pl32[] = {&bcx32, &bcy32, &bcz32, &dztop32};
+    for (float **q : pl32) CHK(alloc(q, d.P));
+    CHK(alloc(&partials, (long)g.nb * 8));
+    CHK(alloc(&scan_part, 3L * 1024));
+    CHK(alloc(&scan_out, 4));
+    CHK(alloc(&scan_cnt, 4));
+    CHK(alloc(&st, 2));
+    HIPCHK(hipHostMalloc((void **)&st_host, 2 * sizeof(KState), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&scan_host, 4 * sizeof(double), hipHostMallocDefault));
+    HIPCHK(hipStreamSynchronize(stream));
+    return WINDHIP_OK;
+}
+
+void windhip_solver::release()
+{
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
+    for (void *q : allocs) (void)hipFree(q);
+    allocs.clear();
+    for (int q = 0; q < 6; ++q)
+        if (stage[q]) (void)hipFree(stage[q]);
+    if (st_host) (void)hipHostFree(st_host);
+    if (scan_host) (void)hipHostFree(scan_host);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+int windhip_solver::alloc_f32_krylov()
+{
+    if (x32) return WINDHIP_OK;
+    float **rows32[] = {&x32, &r32, &rh32, &p32, &ph32[0], &ph32[1], &ph32[2], &v32, &s32,
+                        &sh32[0], &sh32[1], &sh32[2], &t32};
+    for (float **q : rows32) CHK(alloc(q, N));
+    return WINDHIP_OK;
+}
+
+int windhip_solver::assemble(bool full)
+{
+    long tot = (long)d.nx * d.ny * d.nz;
+    k_assemble<<<flat_grid(tot), kBlock, 0, stream>>>(d, metrics(), full ? 1 : 0, B, C, H, I, L, M, B32, C32,
+                                                       H32, I32, L32, M32);
+    HIPCHK(hipGetLastError());
+    return WINDHIP_OK;
+}
+
+// Operator for the current alpha: first-call assembly (initialize_coefs) or
+// the alpha-range refresh (update_coefs_gpu, wind_iterative.F90:410-427), then
+// BC planes and the diagonal (extract_diagonal, :430). alpha must be gathered.
+int windhip_solver::refresh_operator(double amin, double amax)
+{
+    if (!structure_uploaded) {
+        CHK(assemble(true));
+        structure_uploaded = true;
+    } else if (amax > amin && !probed) {
+        CHK(assemble(false));
+    }
+    long pl = (long)d.nx * d.ny;
+    k_bc_planes<<<flat_grid(pl), kBlock, 0, stream>>>(d, metrics(), bcx, bcy, bcz, dztop, bcx32, bcy32,
+                                                       bcz32, dztop32);
+    HIPCHK(hipGetLastError());
+    k_diagonal<<<flat_grid(N), kBlock, 0, stream>>>(d, probed ? 1 : 0, P15[0], B, C, B32, C32, P15_32[0], bcz,
+                                                    dztop, bcz32, dztop32, dinv, dinv32);
+    HIPCHK(hipGetLastError());
+    return WINDHIP_OK;
+}
+
+int windhip_solver::scan(double *amin, double *amax, double *nans)
+{
+    const int nblk = 1024;
+    k_scan_inputs<<<nblk, kBlock, 0, stream>>>(d, m_alpha, b, scan_part, scan_cnt, scan_out, nblk);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(scan_host, scan_out, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    *amin = scan_host[0];
+    *amax = scan_host[1];
+    *nans = scan_host[2];
+    return WINDHIP_OK;
+}
+
+int windhip_solver::set_alpha(const void *alpha)
+{
+    if (!metrics_set) return fail(WINDHIP_ERR_ARG, "windhip_set_alpha: windhip_set_metrics has not been called");
+    CHK(gather3(alpha, 0, 1, d.nx, 1, d.nz, 1, d.ny, m_alpha, 0));
+    double amin, amax, nans;
+    CHK(scan(&amin, &amax, &nans));
+    return refresh_operator(amin, amax);
+}
+
+// calc_iter_winds up to the solve (wind_iterative.F90:309-433)
+int windhip_solver::prepare_call(const void *alpha, const void *div)
+{
+    if (!metrics_set) return fail(WINDHIP_ERR_ARG, "windhip_solve: windhip_set_metrics has not been called");
+    CHK(gather3(alpha, 0, 1, d.nx, 1, d.nz, 1, d.ny, m_alpha, 0));
+    void *dv = nullptr;
+    CHK(view(div, n_mass(), 1, true, &dv));
+    long tot = (long)d.nx * d.ny * d.nz;
+    if (desc.field_dtype == WINDHIP_F32)
+        k_rhs<float><<<flat_grid(tot), kBlock, 0, stream>>>(d, fld<const float>(dv, 0), b);
+    else
+        k_rhs<double><<<flat_grid(tot), kBlock, 0, stream>>>(d, fld<const double>(dv, 0), b);
+    HIPCHK(hipGetLastError());
+    double amin, amax, nans;
+    CHK(scan(&amin, &amax, &nans));
+    if (nans > 0) return fail(WINDHIP_ERR_NAN, "NaN in input divergence (count=%.0f)", nans);
+    return refresh_operator(amin, amax);
+}
+
+int windhip_solver::read_state(int idx)
+{
+    (void)idx;
+    HIPCHK(hipMemcpyAsync(st_host, st, 2 * sizeof(KState), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return WINDHIP_OK;
+}
+
+// One BiCGStab iteration per repetition (wind_iterative.F90:645-786).
+int windhip_solver::enqueue_iterations_f64(int count)
+{
+    const OpA<double> a = opa<double>();
+    const OpP<double> pp = opp<double>();
+    const int pr = probed ? 1 : 0;
+    KState *s0 = st;
+    const int fg = flat_grid(N);
+    for (int it = 0; it < count; ++it) {
+        double *cur = ph[0];
+        for (int sw = 2, q = 1; sw <= nsweeps; ++sw) {
+            double *nxt = ph[q];
+            k_sweep<double><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, s0, cur, p, nxt);
+            cur = nxt;
+            q = (q == 1) ? 2 : 1;
+        }
+        k_sigma<double><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, s0, partials, cur, rh, v);
+        k_update_s<double><<<fg, kBlock, 0, stream>>>(N, s0, r, v, dinv, s, sh[0]);
+        double *cs = sh[0];
+        for (int sw = 2, q = 1; sw <= nsweeps; ++sw) {
+            double *nxt = sh[q];
+            k_sweep<double><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, s0, cs, s, nxt);
+            cs = nxt;
+            q = (q == 1) ? 2 : 1;
+        }
+        k_omega<double><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, s0, partials, cs, s, rh, t);
+        k_update_xrp<double><<<fg, kBlock, 0, stream>>>(N, s0, x, cur, cs, s, t, r, p, v, dinv, ph[0]);
+    }
+    HIPCHK(hipGetLastError());
+    return WINDHIP_OK;
+}
+
+int windhip_solver::enqueue_iterations_f32(int count)
+{
+    const OpA<float> a = opa<float>();
+    const OpP<float> pp = opp<float>();
+    const int pr = probed ? 1 : 0;
+    KState *s1 = st + 1;
+    const int fg = flat_grid(N);
+    for (int it = 0; it < count; ++it) {
+        float *cur = ph32[0];
+        for (int sw = 2, q = 1; sw <= nsweeps; ++sw) {
+            float *nxt = ph32[q];
+            k_sweep<float><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, s1, cur, p32, nxt);
+            cur = nxt;
+            q = (q == 1) ? 2 : 1;
+        }
+        k_sigma<float><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, s1, partials, cur, rh32, v32);
+        k_update_s<float><<<fg, kBlock, 0, stream>>>(N, s1, r32, v32, dinv32, s32, sh32[0]);
+        float *cs = sh32[0];
+        for (int sw = 2, q = 1; sw <= nsweeps; ++sw) {
+            float *nxt = sh32[q];
+            k_sweep<float><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, s1, cs, s32, nxt);
+            cs = nxt;
+            q = (q == 1) ? 2 : 1;
+        }
+        k_omega<float><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, s1, partials, cs, s32, rh32, t32);
+        k_update_xrp<float><<<fg, kBlock, 0, stream>>>(N, s1, x32, cur, cs, s32, t32, r32, p32, v32, dinv32,
+                                                       ph32[0]);
+    }
+    HIPCHK(hipGetLastError());
+    return WINDHIP_OK;
+}
+
+// Launch `count` iterations, from a cached hipGraph when use_graph.
+int windhip_solver::launch_batch(int precision, int count, bool use_graph)
+{
+    if (!use_graph) return precision == 0 ? enqueue_iterations_f64(count) : enqueue_iterations_f32(count);
+    const long key = ((long)count << 8) | ((long)nsweeps << 4) | ((long)precision << 1) | (probed ? 1 : 0);
+    auto it = graphs.find(key);
+    if (it == graphs.end()) {
+        hipGraph_t graph;
+        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        int rc = precision == 0 ? enqueue_iterations_f64(count) : enqueue_iterations_f32(count);
+        hipError_t e = hipStreamEndCapture(stream, &graph);
+        if (rc != WINDHIP_OK) return rc;
+        if (e != hipSuccess) return fail(WINDHIP_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+        hipGraphExec_t exec;
+        HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(graph));
+        it = graphs.emplace(key, exec).first;
+    }
+    HIPCHK(hipGraphLaunch(it->second, stream));
+    return WINDHIP_OK;
+}
+
+// bicgstab_solve (wind_iterative.F90:555-791) on the device. The host only
+// launches batches of iterations and reads the state after each batch.
+int windhip_solver::run_plain(int max_iters, double tol_rel, double tol_abs, int batch, bool use_graph,
+                              int *status, int *iters, double *res0, double *resf)
+{
+    k_state_reset<<<1, 1, 0, stream>>>(st, max_iters, tol_rel, tol_abs, 0.0);
+    const OpA<double> a = opa<double>();
+    const OpP<double> pp = opp<double>();
+    k_init<double><<<g.nb, kBlock, 0, stream>>>(a, pp, probed ? 1 : 0, g, st, partials, x, b, dinv, r, rh, p,
+                                                ph[0]);
+    HIPCHK(hipGetLastError());
+    CHK(read_state(0));
+    int done_iters = 0;
+    while (!st_host[0].done) {
+        int n = std::min(batch, std::max(1, max_iters - done_iters));
+        CHK(launch_batch(0, n, use_graph));
+        CHK(read_state(0));
+        done_iters += n;
+        if (done_iters > max_iters + batch) break;  // defensive: the device sets done at max_iters
+    }
+    *status = st_host[0].status;
+    *iters = st_host[0].iter;
+    *res0 = st_host[0].res0;
+    *resf = st_host[0].rnorm;
+    return WINDHIP_OK;
+}
+
+int windhip_solver::true_residual(double *rel)
+{
+    k_true_resid<<<g.nb, kBlock, 0, stream>>>(opa<double>(), opp<double>(), probed ? 1 : 0, g, st, partials, x, b);
+    HIPCHK(hipGetLastError());
+    CHK(read_state(0));
+    double bn = st_host[0].outer_bnorm;
+    *rel = bn > 0 ? st_host[0].outer_rnorm / bn : st_host[0].outer_rnorm;
+    return WINDHIP_OK;
+}
+
+// FP32 inner BiCGStab inside FP64 iterative refinement.
+int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *info)
+{
+    CHK(alloc_f32_krylov());
+    HIPCHK(hipMemsetAsync(x32, 0, N * sizeof(float), stream));
+    k_state_reset<<<1, 1, 0, stream>>>(st, 0, o.tol_rel, o.tol_abs, 0.0);
+    k_state_reset<<<1, 1, 0, stream>>>(st + 1, o.inner_max_iters, 0.0, 0.0, o.inner_tol);
+    const OpA<double> a = opa<double>();
+    const OpP<double> pp = opp<double>();
+    const int pr = probed ? 1 : 0;
+    auto resid = [&]() -> int {
+        k_ir_resid<<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, st, st + 1, partials, x, b, dinv32, r32, rh32,
+                                                p32, ph32[0]);
+        HIPCHK(hipGetLastError());
+        return read_state(0);
+    };
+    CHK(resid());
+    info->res0 = st_host[0].outer_rnorm;
+    info->bnorm = st_host[0].outer_bnorm;
+    int total_inner = 0, outer = 0, status = 0;
+    const int batch = o.iters_per_sync;
+    double prev = st_host[0].outer_rnorm;
+    int stalls = 0;
+    while (!st_host[0].outer_done && outer < o.max_outer) {
+        ++outer;
+        int done_iters = 0;
+        while (!st_host[1].done) {
+            int n = std::min(batch, std::max(1, o.inner_max_iters - done_iters));
+            CHK(launch_batch(1, n, o.use_graph != 0));
+            CHK(read_state(1));
+            done_iters += n;
+            if (done_iters > o.inner_max_iters + batch) break;
+        }
+        total_inner += st_host[1].iter;
+        k_ir_accumulate<<<flat_grid(N), kBlock, 0, stream>>>(N, x, x32);
+        HIPCHK(hipGetLastError());
+        CHK(resid());
+        const double now = st_host[0].outer_rnorm;
+        if (!(now < prev)) {  // no progress (or NaN): stop refining
+            if (++stalls >= 2 || now != now) { status = 2; break; }
+        } else {
+            stalls = 0;
+        }
+        prev = now;
+    }
+    if (!st_host[0].outer_done && status == 0) status = 1;
+    info->status = st_host[0].outer_done ? 0 : status;
+    info->iters = total_inner;
+    info->outer_iters = outer;
+    info->res_final = st_host[0].outer_rnorm;
+    info->true_rel_res = info->bnorm > 0 ? st_host[0].outer_rnorm / info->bnorm : st_host[0].outer_rnorm;
+    return WINDHIP_OK;
+}
+
+int windhip_solver::correction(void *u, void *v, void *w, const void *density, int adv_den)
+{
+    void *du, *dv, *dw, *dr = nullptr;
+    CHK(view(u, n_u(), 2, true, &du));
+    CHK(view(v, n_v(), 3, true, &dv));
+    CHK(view(w, n_mass(), 4, true, &dw));
+    const int use_rho = (adv_den && density) ? 1 : 0;
+    if (use_rho) CHK(view(density, n_mass(), 5, true, &dr));
+    const Metrics m = metrics();
+    long tu = (long)(d.nx + 1) * d.ny, tv = (long)d.nx * (d.ny + 1), tw = (long)d.nx * d.ny;
+    if (desc.field_dtype == WINDHIP_F32) {
+        k_correct_u<float><<<flat_grid(tu), kBlock, 0, stream>>>(d, m, x, fld<float>(du, 1), fld<const float>(dr, 0), use_rho);
+        k_correct_v<float><<<flat_grid(tv), kBlock, 0, stream>>>(d, m, x, fld<float>(dv, 2), fld<const float>(dr, 0), use_rho);
+        k_correct_w<float><<<flat_grid(tw), kBlock, 0, stream>>>(d, m, x, fld<float>(dw, 0), fld<const float>(dr, 0), use_rho);
+    } else {
+        k_correct_u<double><<<flat_grid(tu), kBlock, 0, stream>>>(d, m, x, fld<double>(du, 1), fld<const double>(dr, 0), use_rho);
+        k_correct_v<double><<<flat_grid(tv), kBlock, 0, stream>>>(d, m, x, fld<double>(dv, 2), fld<const double>(dr, 0), use_rho);
+        k_correct_w<double><<<flat_grid(tw), kBlock, 0, stream>>>(d, m, x, fld<double>(dw, 0), fld<const double>(dr, 0), use_rho);
+    }
+    HIPCHK(hipGetLastError());
+    CHK(copy_out(u, n_u(), 2));
+    CHK(copy_out(v, n_v(), 3));
+    CHK(copy_out(w, n_mass(), 4));
+    HIPCHK(hipStreamSynchronize(stream));
+    return WINDHIP_OK;
+}
+
+int windhip_solver::divergence(const void *u, const void *v, const void *w, const void *density, int adv_den,
+                               int horz_only, int use_dqdt, void *div_out)
+{
+    void *du, *dv, *dw, *dr = nullptr, *dd;
+    CHK(view(u, n_u(), 2, true, &du));
+    CHK(view(v, n_v(), 3, true, &dv));
+    CHK(view(w, n_mass(), 4, true, &dw));
+    const int use_rho = (adv_den && density) ? 1 : 0;
+    if (use_rho) CHK(view(density, n_mass(), 5, true, &dr));
+    CHK(view(div_out, n_mass(), 1, desc.field_mem == WINDHIP_HOST, &dd));
+    long tw = (long)d.nx * d.ny;
+    if (desc.field_dtype == WINDHIP_F32)
+        k_divergence<float><<<flat_grid(tw), kBlock, 0, stream>>>(d, metrics(), fld<const float>(du, 1),
+            fld<const float>(dv, 2), fld<const float>(dw, 0), fld<const float>(dr, 0), use_rho, adv_den,
+            horz_only, use_dqdt, fld<float>(dd, 0));
+    else
+        k_divergence<double><<<flat_grid(tw), kBlock, 0, stream>>>(d, metrics(), fld<const double>(du, 1),
+            fld<const double>(dv, 2), fld<const double>(dw, 0), fld<const double>(dr, 0), use_rho, adv_den,
+            horz_only, use_dqdt, fld<double>(dd, 0));
+    HIPCHK(hipGetLastError());
+    CHK(copy_out(div_out, n_mass(), 1));
+    HIPCHK(hipStreamSynchronize(stream));
+    return WINDHIP_OK;
+}
+
+// calibrate_projection_operator (wind.F90:1010-1099) with the probe hooks of
+// wind_iterative.F90:1096-1248, on the device and on internal FP64 scratch
+// (u, v, w, the response T and the density in the natural (i,k,j) layout).
+int windhip_solver::probe(const void *alpha, const void *density, int adv_den, double *max_leak)
+{
+    if (!metrics_set) return fail(WINDHIP_ERR_ARG, "windhip_probe_operator: metrics not set");
+    CHK(gather3(alpha, 0, 1, d.nx, 1, d.nz, 1, d.ny, m_alpha, 0));
+    if (!P15[0]) {
+        for (int q = 0; q < 15; ++q) CHK(alloc(&P15[q], N));
+        for (int q = 0; q < 15; ++q) CHK(alloc(&P15_32[q], N));
+        CHK(alloc(&cf_dev, 15));
+        HIPCHK(hipMemcpyAsync(cf_dev, P15, 15 * sizeof(double *), hipMemcpyHostToDevice, stream));
+        CHK(alloc(&leak, 1));
+    }
+    const long nu = (long)(d.nx + 1) * d.nz * d.ny, nv = (long)d.nx * d.nz * (d.ny + 1);
+    const long nw = (long)d.nx * d.nz * d.ny;
+    if (!scratch_uvw) CHK(alloc(&scratch_uvw, nu + nv + 3 * nw));
+    double *su = scratch_uvw, *sv = su + nu, *sw = sv + nv, *sT = sw + nw, *srho = sT + nw;
+    const int use_rho = (adv_den && density) ? 1 : 0;
+    if (use_rho) {
+        void *dr = nullptr;
+        CHK(view(density, n_mass(), 5, true, &dr));
+        if (desc.field_dtype == WINDHIP_F32)
+            k_to_natural<float><<<flat_grid(nw), kBlock, 0, stream>>>(d, fld<float>(dr, 0), srho);
+        else
+            k_to_natural<double><<<flat_grid(nw), kBlock, 0, stream>>>(d, fld<double>(dr, 0), srho);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipMemsetAsync(leak, 0, sizeof(unsigned long long), stream));
+    const Fld<double> fu{su, 1, 1, 1, d.nx + 1, d.nz}, fv{sv, 1, 1, 1, d.nx, d.nz}, fw{sw, 1, 1, 1, d.nx, d.nz};
+    const Fld<const double> cu{su, 1, 1, 1, d.nx + 1, d.nz}, cv{sv, 1, 1, 1, d.nx, d.nz}, cw{sw, 1, 1, 1, d.nx, d.nz};
+    const Fld<const double> fr{srho, 1, 1, 1, d.nx, d.nz};
+    const Fld<double> fT{sT, 1, 1, 1, d.nx, d.nz};
+    const Metrics m = metrics();
+    const long tu = (long)(d.nx + 1) * d.ny, tv = (long)d.nx * (d.ny + 1), tw = (long)d.nx * d.ny;
+    for (int cc = 0; cc < 3; ++cc)
+        for (int cb = 0; cb < 3; ++cb)
+            for (int ca = 0; ca < 3; ++ca) {
+                k_probe_pattern<<<flat_grid(N), kBlock, 0, stream>>>(d, ca, cb, cc, x);
+                HIPCHK(hipMemsetAsync(su, 0, (nu + nv + nw) * sizeof(double), stream));
+                k_correct_u<double><<<flat_grid(tu), kBlock, 0, stream>>>(d, m, x, fu, fr, use_rho);
+                k_correct_v<double><<<flat_grid(tv), kBlock, 0, stream>>>(d, m, x, fv, fr, use_rho);
+                k_correct_w<double><<<flat_grid(tw), kBlock, 0, stream>>>(d, m, x, fw, fr, use_rho);
+                k_divergence<double><<<flat_grid(tw), kBlock, 0, stream>>>(d, m, cu, cv, cw, fr, use_rho, adv_den,
+                                                                            0, 1, fT);
+                k_probe_record<<<flat_grid(nw), kBlock, 0, stream>>>(d, ca, cb, cc, sT, cf_dev, leak);
+                HIPCHK(hipGetLastError());
+            }
+    for (int q = 0; q < 15; ++q) k_round_f32<<<flat_grid(N), kBlock, 0, stream>>>(N, P15[q], P15_32[q]);
+    HIPCHK(hipGetLastError());
+    unsigned long long lk = 0;
+    HIPCHK(hipMemcpyAsync(&lk, leak, sizeof(lk), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemsetAsync(x, 0, N * sizeof(double), stream));  // probe_finalize resets x_sol (:1241)
+    HIPCHK(hipStreamSynchronize(stream));
+    double lv;
+    memcpy(&lv, &lk, sizeof(lv));
+    if (max_leak) *max_leak = lv;
+    probed = true;
+    structure_uploaded = true;
+    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
+    return WINDHIP_OK;
+}
+
+// =================================================================== C-ABI
+extern "C" {
+
+const char *windhip_last_error(void) { return g_err.c_str(); }
+
+void windhip_default_opts(windhip_solve_opts *o)
+{
+    if (!o) return;
+    o->tol_rel = 1.0e-5;   // bicg_tol_rel (wind_iterative.F90:60)
+    o->tol_abs = 1.0e-10;  // bicg_tol_abs (:59)
+    o->max_iters = 1500;   // wind_solver_max_iters default (:57)
+    o->start = WINDHIP_START_WARM;
+    o->precision = WINDHIP_PLAIN_F64;
+    o->inner_tol = 1.0e-4;
+    o->inner_max_iters = 4000;
+    o->max_outer = 30;
+    o->iters_per_sync = 0;
+    o->use_graph = 1;
+    o->retry = 1;
+}
+
+int windhip_create(const windhip_desc *desc, windhip_solver **out)
+{
+    g_err.clear();
+    if (!desc || !out) return fail(WINDHIP_ERR_ARG, "windhip_create: null argument");
+    *out = nullptr;
+    const windhip_desc &s = *desc;
+    if (s.nx < 3 || s.ny < 3 || s.nz < 2) return fail(WINDHIP_ERR_ARG, "windhip_create: need nx, ny >= 3 and nz >= 2");
+    if (!(s.dx > 0)) return fail(WINDHIP_ERR_ARG, "windhip_create: dx must be positive");
+    if (s.ims > 1 || s.ime < s.nx || s.kms > 1 || s.kme < s.nz || s.jms > 1 || s.jme < s.ny)
+        return fail(WINDHIP_ERR_ARG, "windhip_create: memory bounds must contain 1..nx, 1..nz, 1..ny");
+    if (s.field_dtype != WINDHIP_F32 && s.field_dtype != WINDHIP_F64)
+        return fail(WINDHIP_ERR_ARG, "windhip_create: field_dtype must be WINDHIP_F32 or WINDHIP_F64");
+    if (s.field_mem != WINDHIP_HOST && s.field_mem != WINDHIP_DEVICE)
+        return fail(WINDHIP_ERR_ARG, "windhip_create: field_mem must be WINDHIP_HOST or WINDHIP_DEVICE");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(WINDHIP_ERR_HIP, "windhip_create: no HIP device available");
+    windhip_solver *h = new windhip_solver();
+    h->desc = s;
+    int rc = h->init();
+    if (rc != WINDHIP_OK) {
+        std::string keep = g_err;
+        h->release();
+        delete h;
+        g_err = keep;
+        return rc;
+    }
+    *out = h;
+    return WINDHIP_OK;
+}
+
+int windhip_destroy(windhip_solver *s)
+{
+    if (!s) return WINDHIP_OK;
+    s->release();
+    delete s;
+    return WINDHIP_OK;
+}
+
+int windhip_set_metrics(windhip_solver *s, const void *dzdx, const void *dzdy, const void *jacobian,
+                        const void *jacobian_u, const void *jacobian_v, const void *jacobian_w,
+                        const void *dzdx_u, const void *dzdy_v, const void *advection_dz, const void *terrain,
+                        const void *mapfac_mx_u, const void *mapfac_my_u, const void *mapfac_my_v,
+                        const void *mapfac_mx_v, const void *mapfac_mxy)
+{
+    if (!s) return fail(WINDHIP_ERR_ARG, "null handle");
+    const void *req[] = {dzdx, dzdy, jacobian, jacobian_u, jacobian_v, jacobian_w, dzdx_u, dzdy_v, advection_dz, terrain};
+    for (const void *q : req)
+        if (!q) return fail(WINDHIP_ERR_ARG, "windhip_set_metrics: metric arrays must not be NULL");
+    const int nx = s->d.nx, ny = s->d.ny, nz = s->d.nz;
+    CHK(s->gather3(dzdx, 0, 1, nx, 1, nz, 1, ny, s->m_dzdx, 0));
+    CHK(s->gather3(dzdy, 0, 1, nx, 1, nz, 1, ny, s->m_dzdy, 1));
+    CHK(s->gather3(jacobian, 0, 1, nx, 1, nz, 1, ny, s->m_jaco, 2));
+    CHK(s->gather3(jacobian_w, 0, 1, nx, 1, nz, 1, ny, s->m_jaco_w, 3));
+    CHK(s->gather3(advection_dz, 0, 1, nx, 1, nz, 1, ny, s->m_adv_dz, 4));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    CHK(s->gather3(jacobian_u, 1, 1, nx + 1, 1, nz, 1, ny, s->m_jaco_u, 0));
+    CHK(s->gather3(dzdx_u, 1, 1, nx + 1, 1, nz, 1, ny, s->m_dzdx_u, 1));
+    CHK(s->gather3(jacobian_v, 2, 1, nx, 1, nz, 1, ny + 1, s->m_jaco_v, 2));
+    CHK(s->gather3(dzdy_v, 2, 1, nx, 1, nz, 1, ny + 1, s->m_dzdy_v, 3));
+    CHK(s->gather2(terrain, 0, 1, nx, 1, ny, s->p_terrain, 4));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const void *mfs[5] = {mapfac_mx_u, mapfac_my_u, mapfac_my_v, mapfac_mx_v, mapfac_mxy};
+    const int kinds[5] = {1, 1, 2, 2, 0};
+    for (int q = 0; q < 5; ++q) {
+        if (!mfs[q]) continue;
+        if (!s->mf[q]) CHK(s->alloc(&s->mf[q], s->d.P));
+        int ib = kinds[q] == 1 ? nx + 1 : nx, jb = kinds[q] == 2 ? ny + 1 : ny;
+        CHK(s->gather2(mfs[q], kinds[q], 1, ib, 1, jb, s->mf[q], q));
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
+    long pl = (long)nx * ny;
+    k_module_vars<<<flat_grid(pl), kBlock, 0, s->stream>>>(s->d, s->m_adv_dz, s->m_dzdx, s->m_dzdy, s->p_terrain,
+                                                          s->m_dz_if, s->p_sx, s->p_sy);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->metrics_set = true;
+    s->structure_uploaded = false;
+    s->probed = false;
+    return WINDHIP_OK;
+}
+
+int windhip_solve(windhip_solver *s, const void *alpha, const void *div, const windhip_solve_opts *opts,
+                  windhip_solve_info *info)
+{
+    if (!s || !alpha || !div) return fail(WINDHIP_ERR_ARG, "windhip_solve: null argument");
+    windhip_solve_opts o;
+    if (opts) o = *opts;
+    else windhip_default_opts(&o);
+    if (o.max_iters < 0 || o.inner_max_iters < 1 || o.max_outer < 1)
+        return fail(WINDHIP_ERR_ARG, "windhip_solve: iteration caps must be positive");
+    if (o.start == WINDHIP_START_LEARNED)
+        return fail(WINDHIP_ERR_ARG, "windhip_solve: the learned start needs windhip_net_* (not loaded)");
+    windhip_solve_info inf{};
+    if (o.iters_per_sync <= 0) {
+        long per = 64000000L / std::max(1L, s->N);
+        o.iters_per_sync = (int)std::max(4L, std::min(64L, per));
+    }
+    HIPCHK(hipEventRecord(s->ev0, s->stream));
+    CHK(s->prepare_call(alpha, div));
+    if (o.start == WINDHIP_START_ZERO) HIPCHK(hipMemsetAsync(s->x, 0, s->N * sizeof(double), s->stream));
+    inf.start_used = o.start;
+    if (o.precision == WINDHIP_MIXED) {
+        CHK(s->run_mixed(o, &inf));
+        inf.nsweeps = s->nsweeps;
+    } else {
+        int status = 1, iters = 0;
+        double res0 = 0, resf = 0;
+        CHK(s->run_plain(o.max_iters, o.tol_rel, o.tol_abs, o.iters_per_sync, o.use_graph != 0, &status, &iters,
+                         &res0, &resf));
+        inf.nsweeps = s->nsweeps;
+        int rc = WINDHIP_OK;
+        // Adaptive preconditioner retry (wind_iterative.F90:474-504)
+        if (o.retry && status != 0 && iters > 0 && resf > 0.01 * res0) {
+            while (status != 0 && resf > 0.01 * res0 && s->nsweeps < 4) {
+                s->nsweeps += 1;
+                HIPCHK(hipMemsetAsync(s->x, 0, s->N * sizeof(double), s->stream));
+                CHK(s->run_plain(o.max_iters, o.tol_rel, o.tol_abs, o.iters_per_sync, o.use_graph != 0, &status,
+                                 &iters, &res0, &resf));
+                inf.nsweeps = s->nsweeps;
+            }
+            s->nsweeps = 2;
+            if (status != 0 && resf > res0)
+                rc = fail(WINDHIP_ERR_DIVERGED, "BiCGStab diverged after the retry ladder (res0=%g, res=%g)", res0, resf);
+        }
+        inf.status = status;
+        inf.iters = iters;
+        inf.res0 = res0;
+        inf.res_final = resf;
+        CHK(s->true_residual(&inf.true_rel_res));
+        inf.bnorm = s->st_host[0].outer_bnorm;
+        if (rc != WINDHIP_OK) {
+            if (info) *info = inf;
+            return rc;
+        }
+    }
+    HIPCHK(hipEventRecord(s->ev1, s->stream));
+    HIPCHK(hipEventSynchronize(s->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    inf.ms = ms;
+    if (info) *info = inf;
+    return WINDHIP_OK;
+}
+
+int windhip_apply_correction(windhip_solver *s, void *u, void *v, void *w, const void *density, int adv_den)
+{
+    if (!s || !u || !v || !w) return fail(WINDHIP_ERR_ARG, "windhip_apply_correction: null argument");
+    if (!s->metrics_set) return fail(WINDHIP_ERR_ARG, "windhip_apply_correction: metrics not set");
+    return s->correction(u, v, w, density, adv_den);
+}
+
+int windhip_calc_iter_winds(windhip_solver *s, const void *alpha, const void *div, void *u, void *v, void *w,
+                            const void *density, int adv_den, const windhip_solve_opts *opts,
+                            windhip_solve_info *info)
+{
+    CHK(windhip_solve(s, alpha, div, opts, info));
+    return windhip_apply_correction(s, u, v, w, density, adv_den);
+}
+
+int windhip_divergence(windhip_solver *s, const void *u, const void *v, const void *w, const void *density,
+                       int adv_den, int horz_only, int use_dqdt, void *div_out)
+{
+    if (!s || !u || !v || !w || !div_out) return fail(WINDHIP_ERR_ARG, "windhip_divergence: null argument");
+    if (!s->metrics_set) return fail(WINDHIP_ERR_ARG, "windhip_divergence: metrics not set");
+    return s->divergence(u, v, w, density, adv_den, horz_only, use_dqdt, div_out);
+}
+
+// lambda rows (0:nx+1, 0:nz+1, 0:ny+1) Fortran order <-> [k][j][i]
+int windhip_get_lambda(windhip_solver *s, double *lam)
+{
+    if (!s || !lam) return fail(WINDHIP_ERR_ARG, "windhip_get_lambda: null argument");
+    std::vector<double> tmp(s->N);
+    HIPCHK(hipMemcpyAsync(tmp.data(), s->x, s->N * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const int mx = s->d.mx, my = s->d.my, mz = s->d.mz;
+    for (int j = 0; j < my; ++j)
+        for (int k = 0; k < mz; ++k)
+            for (int i = 0; i < mx; ++i)
+                lam[i + (long)mx * (k + (long)mz * j)] = tmp[(long)k * s->d.P + (long)j * mx + i];
+    return WINDHIP_OK;
+}
+
+int windhip_set_lambda(windhip_solver *s, const double *lam)
+{
+    if (!s || !lam) return fail(WINDHIP_ERR_ARG, "windhip_set_lambda: null argument");
+    std::vector<double> tmp(s->N);
+    const int mx = s->d.mx, my = s->d.my, mz = s->d.mz;
+    for (int j = 0; j < my; ++j)
+        for (int k = 0; k < mz; ++k)
+            for (int i = 0; i < mx; ++i)
+                tmp[(long)k * s->d.P + (long)j * mx + i] = lam[i + (long)mx * (k + (long)mz * j)];
+    HIPCHK(hipMemcpyAsync(s->x, tmp.data(), s->N * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return WINDHIP_OK;
+}
+
+int windhip_probe_operator(windhip_solver *s, const void *alpha, const void *density, int adv_den, double *max_leak)
+{
+    if (!s || !alpha) return fail(WINDHIP_ERR_ARG, "windhip_probe_operator: null argument");
+    return s->probe(alpha, density, adv_den, max_leak);
+}
+
+int windhip_time_kernel(windhip_solver *s, int which, int reps, double *avg_ms)
+{
+    if (!s || !avg_ms || reps < 1) return fail(WINDHIP_ERR_ARG, "windhip_time_kernel: bad argument");
+    if (!s->structure_uploaded) return fail(WINDHIP_ERR_ARG, "windhip_time_kernel: solve once first");
+    const int pr = s->probed ? 1 : 0;
+    auto launch = [&]() {
+        if (which == 0)
+            k_spmv<double><<<s->g.nb, kBlock, 0, s->stream>>>(s->opa<double>(), s->opp<double>(), pr, s->g, s->ph[1], s->v);
+        else
+            k_spmv<float><<<s->g.nb, kBlock, 0, s->stream>>>(s->opa<float>(), s->opp<float>(), pr, s->g, s->ph32[1] ? s->ph32[1] : s->B32, s->v32 ? s->v32 : s->C32);
+    };
+    if (which == 1 && !s->x32) CHK(s->alloc_f32_krylov());
+    launch();
+    HIPCHK(hipEventRecord(s->ev0, s->stream));
+    for (int q = 0; q < reps; ++q) launch();
+    HIPCHK(hipEventRecord(s->ev1, s->stream));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventSynchronize(s->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    *avg_ms = ms / reps;
+    return WINDHIP_OK;
+}
+
+int windhip_set_alpha(windhip_solver *s, const void *alpha)
+{
+    if (!s || !alpha) return fail(WINDHIP_ERR_ARG, "windhip_set_alpha: null argument");
+    return s->set_alpha(alpha);
+}
+
+static void rows_to_dev_order(const windhip_solver *s, const double *f, double *dv)
+{
+    const int mx = s->d.mx, my = s->d.my, mz = s->d.mz;
+    for (int j = 0; j < my; ++j)
+        for (int k = 0; k < mz; ++k)
+            for (int i = 0; i < mx; ++i) dv[(long)k * s->d.P + (long)j * mx + i] = f[i + (long)mx * (k + (long)mz * j)];
+}
+
+static void rows_from_dev_order(const windhip_solver *s, const double *dv, double *f)
+{
+    const int mx = s->d.mx, my = s->d.my, mz = s->d.mz;
+    for (int j = 0; j < my; ++j)
+        for (int k = 0; k < mz; ++k)
+            for (int i = 0; i < mx; ++i) f[i + (long)mx * (k + (long)mz * j)] = dv[(long)k * s->d.P + (long)j * mx + i];
+}
+
+int windhip_apply_operator(windhip_solver *s, int precision, const double *x_rows, double *y_rows)
+{
+    if (!s || !x_rows || !y_rows) return fail(WINDHIP_ERR_ARG, "windhip_apply_operator: null argument");
+    if (!s->structure_uploaded) return fail(WINDHIP_ERR_ARG, "windhip_apply_operator: no operator yet (set_alpha or solve first)");
+    std::vector<double> tmp(s->N);
+    rows_to_dev_order(s, x_rows, tmp.data());
+    double *dx_ = nullptr, *dy_ = nullptr;
+    HIPCHK(hipMalloc(&dx_, s->N * sizeof(double)));
+    HIPCHK(hipMalloc(&dy_, s->N * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(dx_, tmp.data(), s->N * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    const int pr = s->probed ? 1 : 0;
+    if (precision == WINDHIP_MIXED) {
+        float *fx = nullptr, *fy = nullptr;
+        HIPCHK(hipMalloc(&fx, s->N * sizeof(float)));
+        HIPCHK(hipMalloc(&fy, s->N * sizeof(float)));
+        k_round_f32<<<flat_grid(s->N), kBlock, 0, s->stream>>>(s->N, dx_, fx);
+        k_spmv<float><<<s->g.nb, kBlock, 0, s->stream>>>(s->opa<float>(), s->opp<float>(), pr, s->g, fx, fy);
+        k_widen_f64<<<flat_grid(s->N), kBlock, 0, s->stream>>>(s->N, fy, dy_);
+        HIPCHK(hipStreamSynchronize(s->stream));
+        (void)hipFree(fx);
+        (void)hipFree(fy);
+    } else {
+        k_spmv<double><<<s->g.nb, kBlock, 0, s->stream>>>(s->opa<double>(), s->opp<double>(), pr, s->g, dx_, dy_);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(tmp.data(), dy_, s->N * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    (void)hipFree(dx_);
+    (void)hipFree(dy_);
+    rows_from_dev_order(s, tmp.data(), y_rows);
+    return WINDHIP_OK;
+}
+
+int windhip_get_rows(windhip_solver *s, int which, double *out)
+{
+    if (!s || !out) return fail(WINDHIP_ERR_ARG, "windhip_get_rows: null argument");
+    const double *src[] = {s->b, s->dinv, s->B, s->C, s->H, s->I, s->L, s->M, s->x};
+    if (which < 0 || which > 8) return fail(WINDHIP_ERR_ARG, "windhip_get_rows: which must be 0..8");
+    std::vector<double> tmp(s->N);
+    HIPCHK(hipMemcpyAsync(tmp.data(), src[which], s->N * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    rows_from_dev_order(s, tmp.data(), out);
+    return WINDHIP_OK;
+}
+
+int windhip_memory(windhip_solver *s, double *bytes_handle, double *bytes_device_used)
+{
+    if (!s) return fail(WINDHIP_ERR_ARG, "null handle");
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    if (bytes_handle) *bytes_handle = (double)s->bytes;
+    if (bytes_device_used) *bytes_device_used = (double)(tot - fr);
+    return WINDHIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,230 @@
+"""Host-side mirror of HICAR's ``wind_iterative`` module over the HIP solver.
+
+Same operations as the reference module (src/physics/wind_iterative.F90:39-41):
+``init_iter_winds``, ``calc_iter_winds``, ``finalize_iter_winds`` and the operator
+probing hooks, with the same argument meaning; the arithmetic runs in
+libwindhip.so (hicar_amd/csrc). Arrays are numpy, C order == Fortran (i,k,j):
+mass (ny, nz, nx), u (ny, nz, nx+1), v (ny+1, nz, nx), 2-D (ny, nx).
+
+Differences from the reference, by design:
+* errors raise ``WindHipError`` instead of printing and calling ``stop``
+  (wind_iterative.F90:506-528);
+* the plain path keeps lambda in FP64 through the correction (the reference
+  rounds it to FP32, :1770-1778).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native as nat
+
+_METRIC_KEYS = ("dzdx", "dzdy", "jacobian", "jacobian_u", "jacobian_v", "jacobian_w",
+                "dzdx_u", "dzdy_v", "advection_dz", "terrain")
+_MAPFAC_KEYS = ("mapfac_mx_u", "mapfac_my_u", "mapfac_my_v", "mapfac_mx_v", "mapfac_mxy")
+
+
+def _ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+class WindSolver:
+    """One nest's solver state on the GPU (the reference's module state,
+    wind_iterative.F90:42-205, held per nest as in hicar_cache_t :142-181)."""
+
+    def __init__(self, nx: int, ny: int, nz: int, dx: float, dtype=np.float64):
+        self.lib = nat.load()
+        self.nx, self.ny, self.nz, self.dx = nx, ny, nz, float(dx)
+        self.dtype = np.dtype(dtype)
+        if self.dtype not in (np.dtype(np.float32), np.dtype(np.float64)):
+            raise ValueError("dtype must be float32 (HICAR real) or float64")
+        d = nat.Desc(nx=nx, ny=ny, nz=nz, dx=float(dx), ims=1, ime=nx, kms=1, kme=nz, jms=1, jme=ny,
+                     field_dtype=self.dtype.itemsize, field_mem=nat.WINDHIP_HOST)
+        h = C.c_void_p()
+        nat.check(self.lib.windhip_create(C.byref(d), C.byref(h)))
+        self.h = h
+        self._keep = []
+
+    # ----------------------------------------------------------------- helpers
+    def _arr(self, a, shape, name):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.shape != shape:
+            raise ValueError(f"{name}: expected shape {shape}, got {a.shape}")
+        return a
+
+    @property
+    def mass_shape(self):
+        return (self.ny, self.nz, self.nx)
+
+    @property
+    def u_shape(self):
+        return (self.ny, self.nz, self.nx + 1)
+
+    @property
+    def v_shape(self):
+        return (self.ny + 1, self.nz, self.nx)
+
+    @property
+    def rows_shape(self):
+        return (self.ny + 2, self.nz + 2, self.nx + 2)
+
+    # ------------------------------------------------------------------- API
+    def set_metrics(self, m: dict) -> None:
+        """init_module_vars (wind_iterative.F90:2274-2421)."""
+        shapes = dict(dzdx=self.mass_shape, dzdy=self.mass_shape, jacobian=self.mass_shape,
+                      jacobian_u=self.u_shape, jacobian_v=self.v_shape, jacobian_w=self.mass_shape,
+                      dzdx_u=self.u_shape, dzdy_v=self.v_shape, advection_dz=self.mass_shape,
+                      terrain=(self.ny, self.nx))
+        arrs = [self._arr(m[k], shapes[k], k) for k in _METRIC_KEYS]
+        mf_shapes = dict(mapfac_mx_u=(self.ny, self.nx + 1), mapfac_my_u=(self.ny, self.nx + 1),
+                         mapfac_my_v=(self.ny + 1, self.nx), mapfac_mx_v=(self.ny + 1, self.nx),
+                         mapfac_mxy=(self.ny, self.nx))
+        mfs = [self._arr(m.get(k), mf_shapes[k], k) for k in _MAPFAC_KEYS]
+        nat.check(self.lib.windhip_set_metrics(self.h, *[_ptr(a) for a in arrs + mfs]))
+
+    def solve(self, alpha, div, **opts) -> dict:
+        """The solve of calc_iter_winds (wind_iterative.F90:295-528)."""
+        a = self._arr(alpha, self.mass_shape, "alpha")
+        d = self._arr(div, self.mass_shape, "div")
+        o = nat.default_opts(**opts)
+        info = nat.SolveInfo()
+        nat.check(self.lib.windhip_solve(self.h, _ptr(a), _ptr(d), C.byref(o), C.byref(info)))
+        return info.as_dict()
+
+    def apply_correction(self, u, v, w, density=None, adv_den=True):
+        """calc_updated_winds (wind_iterative.F90:1742-2071); u, v, w updated in place."""
+        for arr, shp, nm in ((u, self.u_shape, "u"), (v, self.v_shape, "v"), (w, self.mass_shape, "w")):
+            if not (isinstance(arr, np.ndarray) and arr.dtype == self.dtype and arr.flags.c_contiguous
+                    and arr.shape == shp):
+                raise ValueError(f"{nm}: need a C-contiguous {self.dtype} array of shape {shp} (updated in place)")
+        rho = self._arr(density, self.mass_shape, "density")
+        nat.check(self.lib.windhip_apply_correction(self.h, _ptr(u), _ptr(v), _ptr(w), _ptr(rho),
+                                                    int(bool(adv_den))))
+
+    def calc_iter_winds(self, alpha, div, u, v, w, density=None, adv_den=True, **opts) -> dict:
+        """calc_iter_winds(domain, alpha, div, adv_den) (wind_iterative.F90:295)."""
+        info = self.solve(alpha, div, **opts)
+        self.apply_correction(u, v, w, density, adv_den)
+        return info
+
+    def divergence(self, u, v, w, density=None, adv_den=True, horz_only=False, use_dqdt=True):
+        """calc_divergence (wind.F90:250-523)."""
+        uu = self._arr(u, self.u_shape, "u")
+        vv = self._arr(v, self.v_shape, "v")
+        ww = self._arr(w, self.mass_shape, "w")
+        rho = self._arr(density, self.mass_shape, "density")
+        out = np.zeros(self.mass_shape, dtype=self.dtype)
+        nat.check(self.lib.windhip_divergence(self.h, _ptr(uu), _ptr(vv), _ptr(ww), _ptr(rho), int(bool(adv_den)),
+                                              int(bool(horz_only)), int(bool(use_dqdt)), _ptr(out)))
+        return out
+
+    def get_lambda(self) -> np.ndarray:
+        out = np.zeros(self.rows_shape)
+        nat.check(self.lib.windhip_get_lambda(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def set_lambda(self, lam) -> None:
+        lam = np.ascontiguousarray(lam, dtype=np.float64).reshape(self.rows_shape)
+        nat.check(self.lib.windhip_set_lambda(self.h, lam.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def probe_operator(self, alpha, density=None, adv_den=True) -> float:
+        """calibrate_projection_operator (wind.F90:1010-1099): A = 2 D(G(.))."""
+        a = self._arr(alpha, self.mass_shape, "alpha")
+        rho = self._arr(density, self.mass_shape, "density")
+        leak = C.c_double()
+        nat.check(self.lib.windhip_probe_operator(self.h, _ptr(a), _ptr(rho), int(bool(adv_den)), C.byref(leak)))
+        return leak.value
+
+    # ------------------------------------------------------------ diagnostics
+    ROWS = {"rhs": 0, "dinv": 1, "B": 2, "C": 3, "H": 4, "I": 5, "L": 6, "M": 7, "lambda": 8}
+
+    def set_alpha(self, alpha) -> None:
+        a = self._arr(alpha, self.mass_shape, "alpha")
+        nat.check(self.lib.windhip_set_alpha(self.h, _ptr(a)))
+
+    def apply_operator(self, x, precision: int = nat.PLAIN_F64) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(self.rows_shape)
+        y = np.zeros(self.rows_shape)
+        dp = C.POINTER(C.c_double)
+        nat.check(self.lib.windhip_apply_operator(self.h, precision, x.ctypes.data_as(dp), y.ctypes.data_as(dp)))
+        return y
+
+    def get_rows(self, which: str) -> np.ndarray:
+        out = np.zeros(self.rows_shape)
+        nat.check(self.lib.windhip_get_rows(self.h, self.ROWS[which], out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def time_kernel(self, which: int = 0, reps: int = 20) -> float:
+        ms = C.c_double()
+        nat.check(self.lib.windhip_time_kernel(self.h, which, reps, C.byref(ms)))
+        return ms.value
+
+    def memory(self):
+        a, b = C.c_double(), C.c_double()
+        nat.check(self.lib.windhip_memory(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.windhip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ----------------------------------------------------- module-style interface
+@dataclass
+class Domain:
+    """The parts of HICAR's domain_t the wind solve reads: the tile size, dx,
+    nest index and the metric fields (numpy, layout as above)."""
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    metrics: dict
+    nest_indx: int = 1
+
+
+MAX_NESTS = 4  # wind_iterative.F90:139
+_cache: dict = {}
+_active = {"nest": -1}
+
+
+def init_iter_winds(domain: Domain, dtype=np.float64) -> WindSolver:
+    """init_iter_winds(domain, options) (wind_iterative.F90:215-291): one solver
+    state per nest, switched in O(1) when the nest changes."""
+    n = domain.nest_indx
+    if n in _cache:
+        _active["nest"] = n
+        return _cache[n]
+    if len(_cache) >= MAX_NESTS:
+        raise ValueError(f"at most {MAX_NESTS} nests")
+    s = WindSolver(domain.nx, domain.ny, domain.nz, domain.dx, dtype)
+    s.set_metrics(domain.metrics)
+    _cache[n] = s
+    _active["nest"] = n
+    return s
+
+
+def calc_iter_winds(domain: Domain, alpha, div, u, v, w, density=None, adv_den=True, **opts) -> dict:
+    """calc_iter_winds(domain, alpha_in, div_in, adv_den) (wind_iterative.F90:295)."""
+    s = _cache.get(domain.nest_indx)
+    if s is None:
+        raise RuntimeError("calc_iter_winds: init_iter_winds has not been called for this nest")
+    return s.calc_iter_winds(alpha, div, u, v, w, density, adv_den, **opts)
+
+
+def finalize_iter_winds() -> None:
+    """finalize_iter_winds (wind_iterative.F90:2142-2267): release every nest."""
+    for s in _cache.values():
+        s.close()
+    _cache.clear()
+    _active["nest"] = -1

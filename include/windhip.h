@@ -1,0 +1,174 @@
+/*
+ * windhip.h -- C-ABI of the MI355X mass-conserving wind solver.
+ *
+ * Drop-in for the Krylov solve of HICAR's `wind_iterative` module
+ * (src/physics/wind_iterative.F90). Fortran host code binds it through
+ * ISO_C_BINDING (hicar_amd/fortran/windhip_interface.F90); Python through
+ * ctypes (hicar_amd/_native.py). Plain pointers and sizes only.
+ *
+ * Conventions
+ *  - Every entry point returns int: WINDHIP_OK (0) or a nonzero code; the text
+ *    of the last error is in windhip_last_error(). (The reference prints and
+ *    `stop`s instead, wind_iterative.F90:506-528.)
+ *  - One opaque handle per nest (the reference keeps one module state per nest,
+ *    wind_iterative.F90:134-181). A handle owns its HIP stream and device state
+ *    and is not thread-safe; different handles are independent.
+ *  - Field arrays (metrics, alpha, div, u, v, w, density) are the caller's:
+ *    Fortran column-major (i,k,j) with the memory bounds of windhip_desc,
+ *    element type desc.field_dtype (HICAR's `real` is WINDHIP_F32), in host or
+ *    device memory per desc.field_mem. Mass-grid arrays span
+ *    (ims:ime, kms:kme, jms:jme); u-grid arrays (ims:ime+1, ...); v-grid arrays
+ *    (..., jms:jme+1); 2-D arrays (ims:ime, jms:jme), u-grid 2-D (ims:ime+1,
+ *    jms:jme), v-grid 2-D (ims:ime, jms:jme+1).
+ *  - Solver rows (lambda) are (0:nx+1, 0:nz+1, 0:ny+1), Fortran order, FP64,
+ *    as x_sol of wind_iterative.F90:359-370.
+ *  - One GPU: the domain is a single tile, ids = its = 1, ide = ite = nx, etc.
+ */
+#ifndef WINDHIP_H
+#define WINDHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct windhip_solver windhip_solver;
+
+enum {
+    WINDHIP_OK = 0,
+    WINDHIP_ERR_ARG = 1,      /* invalid argument or call order */
+    WINDHIP_ERR_HIP = 2,      /* HIP runtime error (message names the call) */
+    WINDHIP_ERR_NAN = 3,      /* NaN in the input divergence (wind_iterative.F90:319-349) */
+    WINDHIP_ERR_DIVERGED = 4, /* still diverged after the retry ladder (:506-528) */
+    WINDHIP_ERR_NOMEM = 5
+};
+
+enum { WINDHIP_F32 = 4, WINDHIP_F64 = 8 };     /* field element type */
+enum { WINDHIP_HOST = 0, WINDHIP_DEVICE = 1 }; /* field memory */
+
+/* Start value of lambda for a solve. WARM is the reference behaviour: x_sol
+ * persists between calls (wind_iterative.F90:435-442). */
+enum { WINDHIP_START_WARM = 0, WINDHIP_START_ZERO = 1, WINDHIP_START_GIVEN = 2,
+       WINDHIP_START_LEARNED = 3 };
+
+/* PLAIN_F64: BiCGStab in FP64 throughout. MIXED: FP32 inner BiCGStab
+ * (matvec, vectors, coefficients) inside FP64 iterative refinement; the true
+ * residual, its norm, the rho/omega dot products and the lambda accumulation
+ * stay FP64. */
+enum { WINDHIP_PLAIN_F64 = 0, WINDHIP_MIXED = 1 };
+
+typedef struct windhip_desc {
+    int nx, ny, nz;              /* physical cells (ide, jde, kde) */
+    double dx;                   /* domain%dx */
+    int ims, ime, kms, kme, jms, jme; /* memory bounds of mass-grid field arrays */
+    int field_dtype;             /* WINDHIP_F32 or WINDHIP_F64 */
+    int field_mem;               /* WINDHIP_HOST or WINDHIP_DEVICE */
+} windhip_desc;
+
+typedef struct windhip_solve_opts {
+    double tol_rel;        /* bicg_tol_rel (reference 1e-5, wind_iterative.F90:60) */
+    double tol_abs;        /* bicg_tol_abs (1e-10, :59) */
+    int max_iters;         /* wind_solver_iterations (1500, :57) */
+    int start;             /* WINDHIP_START_* */
+    int precision;         /* WINDHIP_PLAIN_F64 or WINDHIP_MIXED */
+    double inner_tol;      /* MIXED: relative reduction asked of each FP32 inner solve */
+    int inner_max_iters;   /* MIXED: cap per inner solve */
+    int max_outer;         /* MIXED: cap on refinement steps */
+    int iters_per_sync;    /* iterations launched between host convergence checks; 0 = auto */
+    int use_graph;         /* replay the iteration batch as a captured hipGraph */
+    int retry;             /* apply the reference's precond retry ladder (:474-504) */
+} windhip_solve_opts;
+
+typedef struct windhip_solve_info {
+    int status;            /* 0 converged, 1 iteration cap, 2 breakdown (bicgstab_solve) */
+    int iters;             /* BiCGStab iterations (MIXED: total inner iterations) */
+    int nsweeps;           /* preconditioner sweeps of the last solve */
+    int outer_iters;       /* MIXED: refinement steps */
+    double res0;           /* ||r0|| (recurrence norm of the reference, absolute) */
+    double res_final;      /* final recurrence residual norm (MIXED: true residual) */
+    double bnorm;          /* ||b|| */
+    double true_rel_res;   /* ||b - A lambda|| / ||b||, FP64, after the solve */
+    double ms;             /* device time of the solve (hipEvents on the handle's stream) */
+    int start_used;        /* WINDHIP_START_* actually used (the learned start falls back) */
+    int reserved;
+} windhip_solve_info;
+
+/* Library / handle life cycle. create ~ init_iter_winds (wind_iterative.F90:215-291):
+ * validates bounds, allocates device state, creates the stream. */
+void windhip_default_opts(windhip_solve_opts *opts);
+int windhip_create(const windhip_desc *desc, windhip_solver **out);
+/* finalize_iter_winds (wind_iterative.F90:2142-2267) */
+int windhip_destroy(windhip_solver *s);
+const char *windhip_last_error(void);
+
+/* init_module_vars (wind_iterative.F90:2274-2421): terrain metrics of the tile.
+ * terrain is neighbor_terrain (2-D). Map factors may be NULL (1.0):
+ * mapfac_mx_u, mapfac_my_u on the u grid; mapfac_my_v, mapfac_mx_v on the v
+ * grid; mapfac_mxy on the mass grid (calc_divergence, wind.F90:288-290, and
+ * calc_updated_winds, wind_iterative.F90:1797-1798). */
+int windhip_set_metrics(windhip_solver *s, const void *dzdx, const void *dzdy,
+                        const void *jacobian, const void *jacobian_u, const void *jacobian_v,
+                        const void *jacobian_w, const void *dzdx_u, const void *dzdy_v,
+                        const void *advection_dz, const void *terrain,
+                        const void *mapfac_mx_u, const void *mapfac_my_u,
+                        const void *mapfac_my_v, const void *mapfac_mx_v,
+                        const void *mapfac_mxy);
+
+/* The solve of calc_iter_winds (wind_iterative.F90:295-528): alpha/div copy,
+ * NaN scan, first-call assembly or alpha-range refresh, diagonal, rhs,
+ * BiCGStab (bicgstab_solve :555-791), retry ladder. Lambda stays on the device
+ * for windhip_apply_correction and as the next warm start. opts may be NULL. */
+int windhip_solve(windhip_solver *s, const void *alpha, const void *div,
+                  const windhip_solve_opts *opts, windhip_solve_info *info);
+
+/* calc_updated_winds (wind_iterative.F90:1742-2071): u, v (dqdt) and the
+ * grid-relative w (dqdt) corrected in place from grad(lambda). density may be
+ * NULL (1.0); it is read only when adv_den is nonzero. */
+int windhip_apply_correction(windhip_solver *s, void *u, void *v, void *w,
+                             const void *density, int adv_den);
+
+/* calc_iter_winds(domain, alpha, div, adv_den) (wind_iterative.F90:295): solve
+ * followed by the correction. opts and info may be NULL. */
+int windhip_calc_iter_winds(windhip_solver *s, const void *alpha, const void *div,
+                            void *u, void *v, void *w, const void *density, int adv_den,
+                            const windhip_solve_opts *opts, windhip_solve_info *info);
+
+/* calc_divergence (wind.F90:250-523), the operator D: div_out on the mass grid.
+ * density may be NULL (1.0). */
+int windhip_divergence(windhip_solver *s, const void *u, const void *v, const void *w,
+                       const void *density, int adv_den, int horz_only, int use_dqdt,
+                       void *div_out);
+
+/* Warm start / learned guess: lambda over the solver rows, host FP64. */
+int windhip_get_lambda(windhip_solver *s, double *lambda_rows);
+int windhip_set_lambda(windhip_solver *s, const double *lambda_rows);
+
+/* Operator probing (wind_iterative.F90:1078-1248, wind.F90:1010-1099): build
+ * A = 2 D(G(.)) from 27 lattice colourings on the device and switch the solver
+ * to the probed 15-array operator. alpha is the wind_alpha field in use. */
+int windhip_probe_operator(windhip_solver *s, const void *alpha, const void *density,
+                           int adv_den, double *max_leak);
+
+/* Per-kernel timing of the solve path: average device time (ms) of `reps`
+ * back-to-back launches of one kernel on the handle's stream, bracketed by
+ * hipEvents. which: 0 = FP64 matvec, 1 = FP32 matvec. */
+int windhip_time_kernel(windhip_solver *s, int which, int reps, double *avg_ms);
+
+/* ---- Diagnostics (tests, benchmarks) ----
+ * set_alpha: the alpha copy, first-call assembly or alpha-range refresh, BC
+ * planes and diagonal of calc_iter_winds (wind_iterative.F90:309-430) without
+ * the solve. apply_operator: y = A x with the current operator (spmv,
+ * :813-1027), precision WINDHIP_PLAIN_F64 or WINDHIP_MIXED (the FP32 operator);
+ * rows in Fortran order as for windhip_get_lambda. get_rows: which = 0 rhs b,
+ * 1 D^-1, 2..7 the stored coefficients B, C, H, I, L, M, 8 lambda. */
+int windhip_set_alpha(windhip_solver *s, const void *alpha);
+int windhip_apply_operator(windhip_solver *s, int precision, const double *x_rows, double *y_rows);
+int windhip_get_rows(windhip_solver *s, int which, double *out_rows);
+
+/* Device memory held by the handle (bytes) and the process high-water mark. */
+int windhip_memory(windhip_solver *s, double *bytes_handle, double *bytes_device_used);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WINDHIP_H */
