@@ -1,0 +1,216 @@
+"""Benchmark of the wind solve on one MI355X.
+
+Workload: a 512x512x40 wind solve to relative residual 1e-8, dx = dy = 50 m,
+seeded fractal terrain (k^-2 spectrum, 0-2500 m relief), 40 stretched levels
+10 m -> 12 km, power-law winds 8 m/s at 10 m with 25 deg direction shear plus
+N(0, 1 m/s) noise, alpha = 1, rho = 1. Inputs (metrics, alpha, divergence) are
+resident in HBM before the timed region. One step = one complete solve
+(operator refresh, diagonal, rhs, Krylov iterations, true residual) from a zero
+start.
+
+Arms: the headline path (FP32 inner BiCGStab inside FP64 iterative
+refinement) gives `value`; the plain FP64 BiCGStab is timed beside it. The
+per-kernel device times come from hipEvents recorded around every hot kernel
+launch on the solver's stream inside the timed region. The CPU baseline is the
+oracle (the reference algorithm restated in C, OpenMP) timed on this host on a
+bounded sample and scaled to a full solve.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ms per wind solve to rel. residual 1e-8 at 512²×40: learned+FP32 vs plain FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes(kind: str, nx: int, ny: int, nz: int) -> float:
+    """Algorithmic HBM bytes of one launch: every array the kernel must touch,
+    once. Coefficients (B, C, H, I, L, M) are read on interior cells, BC planes
+    (4 per column) once, vectors over all rows."""
+    cells = nx * ny * nz
+    P = (nx + 2) * (ny + 2)
+    N = P * (nz + 2)
+    t = 8 if kind.endswith("f64") or kind in ("ir_resid", "true_resid") else 4
+    coef = 6 * t * cells + 4 * t * P
+    table = {
+        "sweep": 3 * N * t + coef,       # y (stencil), in, coefficients -> out
+        "sigma": 3 * N * t + coef,       # p_hat (stencil), r_hat, coefficients -> v
+        "omega": 4 * N * t + coef,       # s_hat (stencil), s, r_hat, coefficients -> t
+        "update_s": 5 * N * t,           # r, v, D^-1 -> s, s_hat
+        "update_xrp": 12 * N * t,        # x, p_hat, s_hat, s, t, p, v, D^-1 -> x, r, p, p_hat
+        "init": 7 * N * t + coef,        # x (stencil), b, D^-1, coefficients -> r, r_hat, p, p_hat
+    }
+    if kind == "ir_resid":   # x, b (FP64), D32^-1, FP64 coefficients -> r32, rh32, p32, ph32
+        return N * (8 + 8 + 4 + 16) + 6 * 8 * cells + 4 * 8 * P
+    if kind == "ir_accumulate":
+        return N * 24
+    if kind == "true_resid":
+        return 2 * N * 8 + 6 * 8 * cells + 4 * 8 * P
+    base = kind.rsplit("_", 1)[0]
+    return float(table[base])
+
+
+def build_case(nx, ny, nz):
+    from hicar_amd import synthetic as S
+    return S.fractal_case(nx=nx, ny=ny, nz=nz, dx=50.0, relief=2500.0, speed=8.0, seed=1)
+
+
+def cpu_baseline(case, div_host, iters_full: int, sample_iters: int):
+    """The oracle timed on this host's cores: set-up plus `sample_iters`
+    BiCGStab iterations at full size, scaled to `iters_full` iterations."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    o = orc.Oracle(case.nx, case.ny, case.nz, case.dx)
+    o.set_metrics(case.metrics)
+    t0 = time.perf_counter()
+    o.set_alpha(case.alpha)
+    o.assemble()
+    o.extract_diagonal()
+    o.compute_rhs(div_host)
+    t1 = time.perf_counter()
+    o.bicgstab(max_iters=sample_iters, tol_rel=1e-8)
+    t2 = time.perf_counter()
+    per_iter = (t2 - t1) / sample_iters
+    ms = 1e3 * ((t1 - t0) + per_iter * iters_full)
+    o.close()
+    return dict(value=ms, unit="ms/solve", cores=orc.num_threads(), kind="port",
+                sample=f"oracle set-up + {sample_iters} BiCGStab iterations at {case.nx}x{case.ny}x{case.nz} "
+                       f"({t2 - t0:.1f} s), scaled to the {iters_full} iterations of the plain FP64 solve",
+                ms_per_iter=1e3 * per_iter)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nx", type=int, default=512)
+    ap.add_argument("--ny", type=int, default=512)
+    ap.add_argument("--nz", type=int, default=40)
+    ap.add_argument("--inner-tol", type=float, default=0.1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-iters", type=int, default=6)
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 arm")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is described in DESIGN.md, not implemented).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from hicar_amd.wind_iterative import WindSolver
+
+    dev = torch.device("cuda", 0)
+    t_gen = time.perf_counter()
+    case = build_case(args.nx, args.ny, args.nz)
+    t_gen = time.perf_counter() - t_gen
+    print(f"[bench] case {case.nx}x{case.ny}x{case.nz} generated in {t_gen:.1f}s", file=sys.stderr, flush=True)
+
+    def D(a):
+        return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+
+    s = WindSolver(case.nx, case.ny, case.nz, case.dx, device_fields=True)
+    s.set_metrics({k: D(v) for k, v in case.metrics.items()
+                   if k in ("dzdx", "dzdy", "jacobian", "jacobian_u", "jacobian_v", "jacobian_w", "dzdx_u",
+                            "dzdy_v", "advection_dz", "terrain")})
+    u, v, w, rho, alpha = D(case.u), D(case.v), D(case.w), D(case.rho), D(case.alpha)
+    div = torch.empty(s.mass_shape, dtype=torch.float64, device=dev)
+    s.divergence(u, v, w, density=rho, adv_den=True, use_dqdt=True, out=div)
+    torch.cuda.synchronize()
+
+    arms = {
+        "mixed": dict(precision=1, start=1, tol_rel=1e-8, inner_tol=args.inner_tol, max_iters=50000, profile=1),
+        "plain": dict(precision=0, start=1, tol_rel=1e-8, max_iters=50000, profile=1),
+    }
+
+    def timed(arm):
+        for _ in range(args.warmup):
+            s.solve(alpha, div, **arms[arm])
+        s.profile(reset=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        info = None
+        for _ in range(args.steps):
+            info = s.solve(alpha, div, **arms[arm])
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / args.steps
+        prof = s.profile(reset=True)
+        lam = s.get_lambda()
+        print(f"[bench] {arm}: {ms:.1f} ms/solve iters={info['iters']} outer={info['outer_iters']} "
+              f"status={info['status']} true_rel_res={info['true_rel_res']:.3e}", file=sys.stderr, flush=True)
+        return ms, info, prof, lam
+
+    ms_m, info_m, prof_m, lam_m = timed("mixed")
+    res = {}
+    if not args.no_plain:
+        ms_p, info_p, prof_p, lam_p = timed("plain")
+        err = float(np.linalg.norm(lam_m - lam_p) / np.linalg.norm(lam_p))
+        res.update(plain_f64_ms=ms_p, plain_iters=info_p["iters"], plain_true_rel_res=info_p["true_rel_res"],
+                   speedup_vs_plain=ms_p / ms_m, lambda_rel_err_vs_plain=err)
+    else:
+        prof_p, info_p = {}, None
+
+    def kernel_table(prof):
+        out = {}
+        for kind, (calls, tot) in prof.items():
+            avg = tot / calls
+            gbs = kernel_bytes(kind, case.nx, case.ny, case.nz) / (avg * 1e-3) / 1e9
+            out[kind] = dict(calls=int(calls), avg_ms=round(avg, 5), total_ms=round(tot, 2), GBs=round(gbs, 1),
+                             frac=round(gbs / HBM_PEAK_GBS, 3))
+        return out
+
+    km = kernel_table(prof_m)
+    dom = max(km, key=lambda k: km[k]["total_ms"])
+    d = km[dom]
+    roof = dict(bound="hbm", achieved=d["GBs"], peak=HBM_PEAK_GBS, unit="GB/s", frac=d["frac"], traffic=None,
+                kernel=dom, bytes_per_launch=kernel_bytes(dom, case.nx, case.ny, case.nz), avg_ms=d["avg_ms"])
+
+    cpu = None
+    if not args.no_cpu:
+        div_host = div.cpu().numpy()
+        iters_full = info_p["iters"] if info_p else info_m["iters"]
+        cpu = cpu_baseline(case, div_host, iters_full, args.cpu_iters)
+        print(f"[bench] cpu baseline: {cpu['value']:.0f} ms/solve ({cpu['cores']} threads)", file=sys.stderr,
+              flush=True)
+
+    line = {
+        "metric": METRIC,
+        "value": round(ms_m, 3),
+        "unit": "ms/solve",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_m, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"wind solve {case.nx}x{case.ny}x{case.nz} fractal terrain, rel. residual 1e-8, "
+                               "FP32 inner BiCGStab in FP64 refinement, zero start (learned start not built yet)",
+                   "grid": [case.nx, case.ny, case.nz], "dx_m": case.dx, "tol_rel": 1e-8,
+                   "inner_tol": args.inner_tol, "cells": case.cells},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "iters": info_m["iters"],
+        "outer_iters": info_m["outer_iters"],
+        "true_rel_res": info_m["true_rel_res"],
+        "cells_per_s": case.cells / (ms_m * 1e-3),
+        "kernels_mixed": km,
+        "kernels_plain": kernel_table(prof_p) if prof_p else None,
+    }
+    line.update(res)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

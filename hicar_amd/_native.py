@@ -32,8 +32,12 @@ EXPORTS = (
     "windhip_set_metrics", "windhip_solve", "windhip_apply_correction", "windhip_calc_iter_winds",
     "windhip_divergence", "windhip_get_lambda", "windhip_set_lambda", "windhip_probe_operator",
     "windhip_time_kernel", "windhip_memory", "windhip_set_alpha", "windhip_apply_operator",
-    "windhip_get_rows",
+    "windhip_get_rows", "windhip_profile",
 )
+
+PROFILE_KINDS = {0: "sweep_f64", 1: "sigma_f64", 2: "update_s_f64", 3: "omega_f64", 4: "update_xrp_f64",
+                 5: "sweep_f32", 6: "sigma_f32", 7: "update_s_f32", 8: "omega_f32", 9: "update_xrp_f32",
+                 10: "init_f64", 11: "ir_resid", 12: "ir_accumulate", 13: "true_resid"}
 
 
 class Desc(C.Structure):
@@ -46,7 +50,7 @@ class SolveOpts(C.Structure):
     _fields_ = [("tol_rel", C.c_double), ("tol_abs", C.c_double), ("max_iters", C.c_int),
                 ("start", C.c_int), ("precision", C.c_int), ("inner_tol", C.c_double),
                 ("inner_max_iters", C.c_int), ("max_outer", C.c_int), ("iters_per_sync", C.c_int),
-                ("use_graph", C.c_int), ("retry", C.c_int)]
+                ("use_graph", C.c_int), ("retry", C.c_int), ("profile", C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -98,6 +102,7 @@ def load():
         "windhip_set_alpha": (i, [vp, vp]),
         "windhip_apply_operator": (i, [vp, i, dp, dp]),
         "windhip_get_rows": (i, [vp, i, dp]),
+        "windhip_profile": (i, [vp, i, dp, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -235,8 +235,18 @@ struct windhip_solver {
                   int *iters, double *res0, double *resf);
     int run_mixed(const windhip_solve_opts &o, windhip_solve_info *info);
     int true_residual(double *rel);
-    int enqueue_iterations_f64(int count);
-    int enqueue_iterations_f32(int count);
+    template <typename T>
+    int enqueue_iterations(int count);
+    template <class Launch>
+    int timed(int kind, Launch &&launch);
+    int harvest();
+    template <typename T>
+    void kvec(T **X, T **R, T **RH, T **P, T **V, T **S, T **Tv, T **PH, T **SH, const T **DI);
+    bool profiling = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
+    std::vector<int> ev_kind;
+    int ev_next = 0;
+    double prof_ms[16] = {}, prof_calls[16] = {};
     int launch_batch(int precision, int count, bool use_graph);
     int alloc_f32_krylov();
     int correction(void *u, void *v, void *w, const void *density, int adv_den);
@@ -280,6 +290,21 @@ OpP<float> windhip_solver::opp<float>() const
     OpP<float> o;
     for (int q = 0; q < 15; ++q) o.cf[q] = P15_32[q];
     return o;
+}
+
+template <>
+void windhip_solver::kvec<double>(double **X, double **R, double **RH, double **P, double **V, double **S,
+                                  double **Tv, double **PH, double **SH, const double **DI)
+{
+    *X = x; *R = r; *RH = rh; *P = p; *V = v; *S = s; *Tv = t; *DI = dinv;
+    for (int q = 0; q < 3; ++q) { PH[q] = ph[q]; SH[q] = sh[q]; }
+}
+template <>
+void windhip_solver::kvec<float>(float **X, float **R, float **RH, float **P, float **V, float **S,
+                                 float **Tv, float **PH, float **SH, const float **DI)
+{
+    *X = x32; *R = r32; *RH = rh32; *P = p32; *V = v32; *S = s32; *Tv = t32; *DI = dinv32;
+    for (int q = 0; q < 3; ++q) { PH[q] = ph32[q]; SH[q] = sh32[q]; }
 }
 
 int windhip_solver::init()
@@ -417,68 +442,88 @@ int windhip_solver::read_state(int idx)
     (void)idx;
     HIPCHK(hipMemcpyAsync(st_host, st, 2 * sizeof(KState), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
+    if (profiling) CHK(harvest());
     return WINDHIP_OK;
 }
 
-// One BiCGStab iteration per repetition (wind_iterative.F90:645-786).
-int windhip_solver::enqueue_iterations_f64(int count)
+// Kernel kinds timed by the profiling mode (hipEvents around each launch on
+// the handle's stream): base + 5 * precision (0 FP64, 1 FP32).
+enum { PK_SWEEP = 0, PK_SIGMA = 1, PK_UPD_S = 2, PK_OMEGA = 3, PK_UPD_XRP = 4, PK_INIT = 10,
+       PK_IR_RESID = 11, PK_IR_ACC = 12, PK_TRUE_RESID = 13, PK_COUNT = 16 };
+
+template <class Launch>
+int windhip_solver::timed(int kind, Launch &&launch)
 {
-    const OpA<double> a = opa<double>();
-    const OpP<double> pp = opp<double>();
-    const int pr = probed ? 1 : 0;
-    KState *s0 = st;
-    const int fg = flat_grid(N);
-    for (int it = 0; it < count; ++it) {
-        double *cur = ph[0];
-        for (int sw = 2, q = 1; sw <= nsweeps; ++sw) {
-            double *nxt = ph[q];
-            k_sweep<double><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, s0, cur, p, nxt);
-            cur = nxt;
-            q = (q == 1) ? 2 : 1;
-        }
-        k_sigma<double><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, s0, partials, cur, rh, v);
-        k_update_s<double><<<fg, kBlock, 0, stream>>>(N, s0, r, v, dinv, s, sh[0]);
-        double *cs = sh[0];
-        for (int sw = 2, q = 1; sw <= nsweeps; ++sw) {
-            double *nxt = sh[q];
-            k_sweep<double><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, s0, cs, s, nxt);
-            cs = nxt;
-            q = (q == 1) ? 2 : 1;
-        }
-        k_omega<double><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, s0, partials, cs, s, rh, t);
-        k_update_xrp<double><<<fg, kBlock, 0, stream>>>(N, s0, x, cur, cs, s, t, r, p, v, dinv, ph[0]);
+    if (!profiling) {
+        launch();
+        return WINDHIP_OK;
     }
-    HIPCHK(hipGetLastError());
+    if (ev_next >= (int)ev_pool.size()) {
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        ev_pool.push_back({e0, e1});
+    }
+    auto &pr = ev_pool[ev_next];
+    HIPCHK(hipEventRecord(pr.first, stream));
+    launch();
+    HIPCHK(hipEventRecord(pr.second, stream));
+    ev_kind.push_back(kind);
+    ++ev_next;
     return WINDHIP_OK;
 }
 
-int windhip_solver::enqueue_iterations_f32(int count)
+// Accumulate the timed launches after the stream has been synchronized.
+int windhip_solver::harvest()
 {
-    const OpA<float> a = opa<float>();
-    const OpP<float> pp = opp<float>();
+    for (int q = 0; q < ev_next; ++q) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev_pool[q].first, ev_pool[q].second));
+        prof_ms[ev_kind[q]] += ms;
+        prof_calls[ev_kind[q]] += 1;
+    }
+    ev_next = 0;
+    ev_kind.clear();
+    return WINDHIP_OK;
+}
+
+// One BiCGStab iteration per repetition (wind_iterative.F90:645-786):
+// sweeps 2..n of the preconditioner on p, v = A p_hat with sigma, s and its
+// first sweep, sweeps on s, t = A s_hat with the five dots, the x/r/p update.
+template <typename T>
+int windhip_solver::enqueue_iterations(int count)
+{
+    const OpA<T> a = opa<T>();
+    const OpP<T> pp = opp<T>();
     const int pr = probed ? 1 : 0;
-    KState *s1 = st + 1;
+    const int prec = sizeof(T) == 8 ? 0 : 1;
+    KState *sp = st + prec;
+    T *X, *R, *RH, *Pv, *V, *S, *Tv, *PH[3], *SH[3];
+    const T *DI;
+    kvec<T>(&X, &R, &RH, &Pv, &V, &S, &Tv, PH, SH, &DI);
     const int fg = flat_grid(N);
+    const int base = 5 * prec;
     for (int it = 0; it < count; ++it) {
-        float *cur = ph32[0];
+        T *cur = PH[0];
         for (int sw = 2, q = 1; sw <= nsweeps; ++sw) {
-            float *nxt = ph32[q];
-            k_sweep<float><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, s1, cur, p32, nxt);
+            T *nxt = PH[q];
+            CHK(timed(base + PK_SWEEP, [&] { k_sweep<T><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, sp, cur, Pv, nxt); }));
             cur = nxt;
             q = (q == 1) ? 2 : 1;
         }
-        k_sigma<float><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, s1, partials, cur, rh32, v32);
-        k_update_s<float><<<fg, kBlock, 0, stream>>>(N, s1, r32, v32, dinv32, s32, sh32[0]);
-        float *cs = sh32[0];
+        CHK(timed(base + PK_SIGMA, [&] { k_sigma<T><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, sp, partials, cur, RH, V); }));
+        CHK(timed(base + PK_UPD_S, [&] { k_update_s<T><<<fg, kBlock, 0, stream>>>(N, sp, R, V, DI, S, SH[0]); }));
+        T *cs = SH[0];
         for (int sw = 2, q = 1; sw <= nsweeps; ++sw) {
-            float *nxt = sh32[q];
-            k_sweep<float><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, s1, cs, s32, nxt);
+            T *nxt = SH[q];
+            CHK(timed(base + PK_SWEEP, [&] { k_sweep<T><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, sp, cs, S, nxt); }));
             cs = nxt;
             q = (q == 1) ? 2 : 1;
         }
-        k_omega<float><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, s1, partials, cs, s32, rh32, t32);
-        k_update_xrp<float><<<fg, kBlock, 0, stream>>>(N, s1, x32, cur, cs, s32, t32, r32, p32, v32, dinv32,
-                                                       ph32[0]);
+        CHK(timed(base + PK_OMEGA, [&] { k_omega<T><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, sp, partials, cs, S, RH, Tv); }));
+        CHK(timed(base + PK_UPD_XRP, [&] {
+            k_update_xrp<T><<<fg, kBlock, 0, stream>>>(N, sp, X, cur, cs, S, Tv, R, Pv, V, DI, PH[0]);
+        }));
     }
     HIPCHK(hipGetLastError());
     return WINDHIP_OK;
@@ -487,13 +532,14 @@ int windhip_solver::enqueue_iterations_f32(int count)
 // Launch `count` iterations, from a cached hipGraph when use_graph.
 int windhip_solver::launch_batch(int precision, int count, bool use_graph)
 {
-    if (!use_graph) return precision == 0 ? enqueue_iterations_f64(count) : enqueue_iterations_f32(count);
+    if (!use_graph || profiling)
+        return precision == 0 ? enqueue_iterations<double>(count) : enqueue_iterations<float>(count);
     const long key = ((long)count << 8) | ((long)nsweeps << 4) | ((long)precision << 1) | (probed ? 1 : 0);
     auto it = graphs.find(key);
     if (it == graphs.end()) {
         hipGraph_t graph;
         HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        int rc = precision == 0 ? enqueue_iterations_f64(count) : enqueue_iterations_f32(count);
+        int rc = precision == 0 ? enqueue_iterations<double>(count) : enqueue_iterations<float>(count);
         hipError_t e = hipStreamEndCapture(stream, &graph);
         if (rc != WINDHIP_OK) return rc;
         if (e != hipSuccess) return fail(WINDHIP_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
@@ -514,8 +560,10 @@ int windhip_solver::run_plain(int max_iters, double tol_rel, double tol_abs, int
     k_state_reset<<<1, 1, 0, stream>>>(st, max_iters, tol_rel, tol_abs, 0.0);
     const OpA<double> a = opa<double>();
     const OpP<double> pp = opp<double>();
-    k_init<double><<<g.nb, kBlock, 0, stream>>>(a, pp, probed ? 1 : 0, g, st, partials, x, b, dinv, r, rh, p,
-                                                ph[0]);
+    CHK(timed(PK_INIT, [&] {
+        k_init<double><<<g.nb, kBlock, 0, stream>>>(a, pp, probed ? 1 : 0, g, st, partials, x, b, dinv, r, rh, p,
+                                                    ph[0]);
+    }));
     HIPCHK(hipGetLastError());
     CHK(read_state(0));
     int done_iters = 0;
@@ -535,7 +583,9 @@ int windhip_solver::run_plain(int max_iters, double tol_rel, double tol_abs, int
 
 int windhip_solver::true_residual(double *rel)
 {
-    k_true_resid<<<g.nb, kBlock, 0, stream>>>(opa<double>(), opp<double>(), probed ? 1 : 0, g, st, partials, x, b);
+    CHK(timed(PK_TRUE_RESID, [&] {
+        k_true_resid<<<g.nb, kBlock, 0, stream>>>(opa<double>(), opp<double>(), probed ? 1 : 0, g, st, partials, x, b);
+    }));
     HIPCHK(hipGetLastError());
     CHK(read_state(0));
     double bn = st_host[0].outer_bnorm;
@@ -543,55 +593,66 @@ int windhip_solver::true_residual(double *rel)
     return WINDHIP_OK;
 }
 
-// FP32 inner BiCGStab inside FP64 iterative refinement.
+// FP32 BiCGStab (matvec, vectors, coefficients in FP32; dots, scalars and
+// the solution in FP64) inside FP64 iterative refinement by reliable updates:
+// the FP32 iterate is flushed into the FP64 lambda and the FP32 residual
+// replaced by the FP64 true residual b - A lambda whenever the recursive norm
+// has fallen by the factor opts.inner_tol, and at the end; the solve stops
+// when the true residual meets the FP64 target.
 int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *info)
 {
     CHK(alloc_f32_krylov());
     HIPCHK(hipMemsetAsync(x32, 0, N * sizeof(float), stream));
     k_state_reset<<<1, 1, 0, stream>>>(st, 0, o.tol_rel, o.tol_abs, 0.0);
-    k_state_reset<<<1, 1, 0, stream>>>(st + 1, o.inner_max_iters, 0.0, 0.0, o.inner_tol);
+    k_state_reset<<<1, 1, 0, stream>>>(st + 1, o.max_iters, 0.0, 0.0, o.inner_tol);
     const OpA<double> a = opa<double>();
     const OpP<double> pp = opp<double>();
     const int pr = probed ? 1 : 0;
-    auto resid = [&]() -> int {
-        k_ir_resid<<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, st, st + 1, partials, x, b, dinv32, r32, rh32,
-                                                p32, ph32[0]);
+    auto resid = [&](int restart) -> int {
+        CHK(timed(PK_IR_RESID, [&] {
+            k_ir_resid<<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, st, st + 1, partials, restart, x, b, dinv32, r32,
+                                                    rh32, p32, ph32[0]);
+        }));
         HIPCHK(hipGetLastError());
         return read_state(0);
     };
-    CHK(resid());
+    auto flush = [&]() -> int {
+        CHK(timed(PK_IR_ACC, [&] { k_ir_accumulate<<<flat_grid(N), kBlock, 0, stream>>>(N, x, x32); }));
+        HIPCHK(hipGetLastError());
+        return WINDHIP_OK;
+    };
+    CHK(resid(1));
     info->res0 = st_host[0].outer_rnorm;
     info->bnorm = st_host[0].outer_bnorm;
-    int total_inner = 0, outer = 0, status = 0;
-    const int batch = o.iters_per_sync;
-    double prev = st_host[0].outer_rnorm;
-    int stalls = 0;
-    while (!st_host[0].outer_done && outer < o.max_outer) {
-        ++outer;
-        int done_iters = 0;
-        while (!st_host[1].done) {
-            int n = std::min(batch, std::max(1, o.inner_max_iters - done_iters));
-            CHK(launch_batch(1, n, o.use_graph != 0));
-            CHK(read_state(1));
-            done_iters += n;
-            if (done_iters > o.inner_max_iters + batch) break;
+    const double delta = (o.inner_tol > 0 && o.inner_tol < 1) ? o.inner_tol : 0.1;
+    double rmax = st_host[0].outer_rnorm;
+    int updates = 0, stalls = 0, status = 0;
+    double best = st_host[0].outer_rnorm;
+    while (!st_host[0].outer_done) {
+        const int left = o.max_iters - st_host[1].iter;
+        if (left <= 0) { status = 1; break; }
+        CHK(launch_batch(1, std::min(o.iters_per_sync, left), o.use_graph != 0));
+        CHK(read_state(1));
+        const KState &in = st_host[1];
+        const bool bad = !(in.rnorm == in.rnorm) || in.status == 2;
+        if (in.done || bad || in.rnorm < delta * rmax) {
+            CHK(flush());
+            // a finished or broken-down recursion restarts from the true residual
+            CHK(resid((in.done || bad) ? 1 : 0));
+            ++updates;
+            const double now = st_host[0].outer_rnorm;
+            if (now < best) { best = now; stalls = 0; }
+            else if (++stalls >= 4 || !(now == now)) { status = 2; break; }
+            rmax = now;
         }
-        total_inner += st_host[1].iter;
-        k_ir_accumulate<<<flat_grid(N), kBlock, 0, stream>>>(N, x, x32);
-        HIPCHK(hipGetLastError());
-        CHK(resid());
-        const double now = st_host[0].outer_rnorm;
-        if (!(now < prev)) {  // no progress (or NaN): stop refining
-            if (++stalls >= 2 || now != now) { status = 2; break; }
-        } else {
-            stalls = 0;
-        }
-        prev = now;
     }
-    if (!st_host[0].outer_done && status == 0) status = 1;
-    info->status = st_host[0].outer_done ? 0 : status;
-    info->iters = total_inner;
-    info->outer_iters = outer;
+    if (!st_host[0].outer_done && st_host[1].iter > 0) {
+        CHK(flush());
+        CHK(resid(1));
+    }
+    info->status = st_host[0].outer_done ? 0 : (status ? status : 1);
+    info->iters = st_host[1].iter;
+    info->outer_iters = updates;
     info->res_final = st_host[0].outer_rnorm;
     info->true_rel_res = info->bnorm > 0 ? st_host[0].outer_rnorm / info->bnorm : st_host[0].outer_rnorm;
     return WINDHIP_OK;
@@ -726,12 +787,13 @@ void windhip_default_opts(windhip_solve_opts *o)
     o->max_iters = 1500;   // wind_solver_max_iters default (:57)
     o->start = WINDHIP_START_WARM;
     o->precision = WINDHIP_PLAIN_F64;
-    o->inner_tol = 1.0e-4;
+    o->inner_tol = 0.1;
     o->inner_max_iters = 4000;
     o->max_outer = 30;
     o->iters_per_sync = 0;
     o->use_graph = 1;
     o->retry = 1;
+    o->profile = 0;
 }
 
 int windhip_create(const windhip_desc *desc, windhip_solver **out)
@@ -832,6 +894,7 @@ int windhip_solve(windhip_solver *s, const void *alpha, const void *div, const w
         long per = 64000000L / std::max(1L, s->N);
         o.iters_per_sync = (int)std::max(4L, std::min(64L, per));
     }
+    s->profiling = o.profile != 0;
     HIPCHK(hipEventRecord(s->ev0, s->stream));
     CHK(s->prepare_call(alpha, div));
     if (o.start == WINDHIP_START_ZERO) HIPCHK(hipMemsetAsync(s->x, 0, s->N * sizeof(double), s->stream));
@@ -1025,6 +1088,17 @@ int windhip_get_rows(windhip_solver *s, int which, double *out)
     HIPCHK(hipMemcpyAsync(tmp.data(), src[which], s->N * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     rows_from_dev_order(s, tmp.data(), out);
+    return WINDHIP_OK;
+}
+
+int windhip_profile(windhip_solver *s, int reset, double *calls, double *ms)
+{
+    if (!s) return fail(WINDHIP_ERR_ARG, "null handle");
+    for (int q = 0; q < PK_COUNT; ++q) {
+        if (calls) calls[q] = s->prof_calls[q];
+        if (ms) ms[q] = s->prof_ms[q];
+        if (reset) s->prof_calls[q] = s->prof_ms[q] = 0.0;
+    }
     return WINDHIP_OK;
 }
 

@@ -405,6 +405,7 @@ __global__ __launch_bounds__(kBlock) void k_omega(OpA<T> opa, OpP<T> opp, int pr
         st->rho = rho_new;
         st->fmode = 3;
         if (st->iter >= st->max_iters) { st->status = 1; st->done = 1; }
+        if (!(st->rnorm == st->rnorm) || st->rnorm > 1.0e300) { st->status = 2; st->done = 1; }
     });
 }
 
@@ -436,48 +437,60 @@ __global__ __launch_bounds__(kBlock) void k_update_xrp(long N, const KState *st,
 }
 
 // ----------------------------------------------- FP64 refinement around FP32
-// r = b - A x in FP64, ||r||^2 (and ||b||^2); the FP32 inner solve of
-// A d = r starts from d = 0: r32 = rh32 = p32 = float(r), ph32 = D32^-1 r32,
-// rho0 = <r32, r32> in FP64.
+// The FP32 BiCGStab runs on the residual equation of the FP64 iterate x64:
+// its own iterate y32 is flushed into x64 (k_ir_accumulate) and its recursive
+// residual r32 is replaced by the true residual b - A x64, computed here in
+// FP64, whenever the recursive norm has fallen by the refinement factor
+// (a reliable update). `restart` also resets the shadow residual and the search
+// direction (r_hat = p = r), which a breakdown or a finished inner solve needs;
+// otherwise p and r_hat are kept and rho = <r_hat, r> is recomputed.
+// Sums: 0 ||r64||^2, 1 ||b||^2, 2 <r32, r32>, 3 <r_hat32, r32>.
 __global__ __launch_bounds__(kBlock) void k_ir_resid(OpA<double> opa, OpP<double> opp, int probed, Geo g,
-                                                     KState *st_out, KState *st_in, double *partials,
+                                                     KState *st_out, KState *st_in, double *partials, int restart,
                                                      const double *__restrict__ x, const double *__restrict__ b,
                                                      const float *__restrict__ dinv32, float *__restrict__ r32,
                                                      float *__restrict__ rh32, float *__restrict__ p32,
                                                      float *__restrict__ ph32)
 {
     WINDHIP_COLUMN_PROLOGUE
-    double acc[3] = {0.0, 0.0, 0.0};
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
     if (active) {
         auto epi = [&](int, long n, double t, double, double) {
             const double bv = b[n];
             const double rv = bv - t;
             const float rf = (float)rv;
-            r32[n] = rf; rh32[n] = rf; p32[n] = rf;
-            ph32[n] = dinv32[n] * rf;
+            r32[n] = rf;
+            float rhv;
+            if (restart) {
+                rh32[n] = rf; p32[n] = rf;
+                ph32[n] = dinv32[n] * rf;
+                rhv = rf;
+            } else {
+                rhv = rh32[n];
+            }
             acc[0] = acc[0] + rv * rv;
-            acc[1] = acc[1] + (double)rf * (double)rf;
-            acc[2] = acc[2] + bv * bv;
+            acc[1] = acc[1] + bv * bv;
+            acc[2] = acc[2] + (double)rf * (double)rf;
+            acc[3] = acc[3] + (double)rhv * (double)rf;
         };
         march<double, false>(opa, opp, probed, g, x, c, i, j, epi);
     }
-    reduce_finalize<3>(acc, partials, &st_out->cnt[3], L, g.nb, [&](const double *tot) {
+    reduce_finalize<4>(acc, partials, &st_out->cnt[3], L, g.nb, [&](const double *tot) {
         st_out->outer_rnorm = sqrt(tot[0]);
-        st_out->outer_bnorm = sqrt(tot[2]);
+        st_out->outer_bnorm = sqrt(tot[1]);
         st_out->outer_target = fmax(st_out->tol_abs, st_out->tol_rel * st_out->outer_bnorm);
         st_out->outer_done = st_out->outer_rnorm <= st_out->outer_target;
-        // inner solve state (bicgstab set-up with x0 = 0, its own rhs r32)
-        st_in->rho = tot[1];
-        st_in->rnorm = sqrt(tot[1]);
+        st_in->rho = tot[3];
+        st_in->rnorm = sqrt(tot[2]);
         st_in->res0 = st_in->rnorm;
-        st_in->bnorm = st_in->rnorm;
-        st_in->target = fmax(1.0e-300, st_in->inner_tol * st_in->rnorm);
-        st_in->iter = 0;
-        st_in->omega = 1.0;
-        st_in->alpha = 1.0;
+        st_in->bnorm = st_out->outer_bnorm;
+        // the recursion aims a little below the FP64 target so that the true
+        // residual passes at the check that follows
+        st_in->target = 0.5 * st_out->outer_target;
+        if (restart) { st_in->omega = 1.0; st_in->alpha = 1.0; }
         st_in->fmode = 0;
-        if (st_out->outer_done || st_in->rnorm <= st_in->target) { st_in->status = 0; st_in->done = 1; }
-        else { st_in->status = 1; st_in->done = 0; }
+        if (st_out->outer_done) { st_in->status = 0; st_in->done = 1; }
+        else { st_in->status = 1; st_in->done = (st_in->iter >= st_in->max_iters); }
     });
 }
 

@@ -26,22 +26,46 @@ _METRIC_KEYS = ("dzdx", "dzdy", "jacobian", "jacobian_u", "jacobian_v", "jacobia
 _MAPFAC_KEYS = ("mapfac_mx_u", "mapfac_my_u", "mapfac_my_v", "mapfac_mx_v", "mapfac_mxy")
 
 
+class _DevArr:
+    """A device field argument: checks shape, dtype and contiguity of a tensor-like
+    object and exposes its device address."""
+
+    def __init__(self, t, shape, dtype, name):
+        if not hasattr(t, "data_ptr"):
+            raise TypeError(f"{name}: device fields need an object with data_ptr() (e.g. a GPU tensor)")
+        if tuple(t.shape) != tuple(shape):
+            raise ValueError(f"{name}: expected shape {shape}, got {tuple(t.shape)}")
+        if getattr(t, "element_size", lambda: dtype.itemsize)() != dtype.itemsize:
+            raise ValueError(f"{name}: expected {dtype} elements")
+        if hasattr(t, "is_contiguous") and not t.is_contiguous():
+            raise ValueError(f"{name}: must be contiguous")
+        self.addr = int(t.data_ptr())
+
+
 def _ptr(a):
-    return None if a is None else C.c_void_p(a.ctypes.data)
+    if a is None:
+        return None
+    if isinstance(a, _DevArr):
+        return C.c_void_p(a.addr)
+    return C.c_void_p(a.ctypes.data)
 
 
 class WindSolver:
     """One nest's solver state on the GPU (the reference's module state,
     wind_iterative.F90:42-205, held per nest as in hicar_cache_t :142-181)."""
 
-    def __init__(self, nx: int, ny: int, nz: int, dx: float, dtype=np.float64):
+    def __init__(self, nx: int, ny: int, nz: int, dx: float, dtype=np.float64, device_fields: bool = False):
+        """device_fields: field arguments are device arrays (objects with
+        ``data_ptr()``, e.g. torch tensors on the GPU) instead of numpy arrays."""
         self.lib = nat.load()
         self.nx, self.ny, self.nz, self.dx = nx, ny, nz, float(dx)
         self.dtype = np.dtype(dtype)
+        self.device_fields = bool(device_fields)
         if self.dtype not in (np.dtype(np.float32), np.dtype(np.float64)):
             raise ValueError("dtype must be float32 (HICAR real) or float64")
         d = nat.Desc(nx=nx, ny=ny, nz=nz, dx=float(dx), ims=1, ime=nx, kms=1, kme=nz, jms=1, jme=ny,
-                     field_dtype=self.dtype.itemsize, field_mem=nat.WINDHIP_HOST)
+                     field_dtype=self.dtype.itemsize,
+                     field_mem=nat.WINDHIP_DEVICE if device_fields else nat.WINDHIP_HOST)
         h = C.c_void_p()
         nat.check(self.lib.windhip_create(C.byref(d), C.byref(h)))
         self.h = h
@@ -51,6 +75,8 @@ class WindSolver:
     def _arr(self, a, shape, name):
         if a is None:
             return None
+        if self.device_fields:
+            return _DevArr(a, shape, self.dtype, name)
         a = np.ascontiguousarray(a, dtype=self.dtype)
         if a.shape != shape:
             raise ValueError(f"{name}: expected shape {shape}, got {a.shape}")
@@ -97,6 +123,13 @@ class WindSolver:
 
     def apply_correction(self, u, v, w, density=None, adv_den=True):
         """calc_updated_winds (wind_iterative.F90:1742-2071); u, v, w updated in place."""
+        if self.device_fields:
+            uu, vv, ww = (self._arr(u, self.u_shape, "u"), self._arr(v, self.v_shape, "v"),
+                          self._arr(w, self.mass_shape, "w"))
+            rho = self._arr(density, self.mass_shape, "density")
+            nat.check(self.lib.windhip_apply_correction(self.h, _ptr(uu), _ptr(vv), _ptr(ww), _ptr(rho),
+                                                        int(bool(adv_den))))
+            return
         for arr, shp, nm in ((u, self.u_shape, "u"), (v, self.v_shape, "v"), (w, self.mass_shape, "w")):
             if not (isinstance(arr, np.ndarray) and arr.dtype == self.dtype and arr.flags.c_contiguous
                     and arr.shape == shp):
@@ -111,12 +144,19 @@ class WindSolver:
         self.apply_correction(u, v, w, density, adv_den)
         return info
 
-    def divergence(self, u, v, w, density=None, adv_den=True, horz_only=False, use_dqdt=True):
-        """calc_divergence (wind.F90:250-523)."""
+    def divergence(self, u, v, w, density=None, adv_den=True, horz_only=False, use_dqdt=True, out=None):
+        """calc_divergence (wind.F90:250-523). With device fields `out` must be given."""
         uu = self._arr(u, self.u_shape, "u")
         vv = self._arr(v, self.v_shape, "v")
         ww = self._arr(w, self.mass_shape, "w")
         rho = self._arr(density, self.mass_shape, "density")
+        if self.device_fields:
+            if out is None:
+                raise ValueError("divergence: pass `out` (a device array) when fields live on the device")
+            nat.check(self.lib.windhip_divergence(self.h, _ptr(uu), _ptr(vv), _ptr(ww), _ptr(rho),
+                                                  int(bool(adv_den)), int(bool(horz_only)), int(bool(use_dqdt)),
+                                                  _ptr(self._arr(out, self.mass_shape, "out"))))
+            return out
         out = np.zeros(self.mass_shape, dtype=self.dtype)
         nat.check(self.lib.windhip_divergence(self.h, _ptr(uu), _ptr(vv), _ptr(ww), _ptr(rho), int(bool(adv_den)),
                                               int(bool(horz_only)), int(bool(use_dqdt)), _ptr(out)))
@@ -157,6 +197,13 @@ class WindSolver:
         out = np.zeros(self.rows_shape)
         nat.check(self.lib.windhip_get_rows(self.h, self.ROWS[which], out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
+
+    def profile(self, reset: bool = True) -> dict:
+        """Per-kernel (calls, total ms) of the solves run with profile=1."""
+        calls = (C.c_double * 16)()
+        ms = (C.c_double * 16)()
+        nat.check(self.lib.windhip_profile(self.h, int(reset), calls, ms))
+        return {name: (calls[k], ms[k]) for k, name in nat.PROFILE_KINDS.items() if calls[k] > 0}
 
     def time_kernel(self, which: int = 0, reps: int = 20) -> float:
         ms = C.c_double()

@@ -185,7 +185,7 @@ def test_mixed_precision_small():
     o = oracle_for(c)
     div = o.divergence(c.u, c.v, c.w, rho=c.rho, adv_den=True, use_dqdt=True)
     o.solve(c.alpha, div, max_iters=5000, tol_rel=1e-12)
-    info = s.solve(c.alpha, div, precision=1, tol_rel=1e-8, start=1, inner_tol=1e-3)
+    info = s.solve(c.alpha, div, precision=1, tol_rel=1e-8, start=1, max_iters=20000)
     assert info["status"] == 0
     assert info["true_rel_res"] <= 1e-8
     assert _rel(s.get_lambda(), o.get_lambda()) <= 1e-6

@@ -1,0 +1,427 @@
+"""Pins of the CPU oracle (oracle/wind_oracle.c) against mathematics that does
+not come from the oracle itself: closed forms, invariants, a sparse-matrix
+brute force, a direct solve, a textbook BiCGStab, a manufactured solution,
+the reference's own composition identity A = 2 D(G(.)) and the reference's
+unit-test behaviour. The reference ships no golden vectors for this path.
+
+Several tests plant a plausible mistake in the oracle's coefficients (on the
+CPU) and check that the pin then fails.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from conftest import oracle_for, small_case
+from hicar_amd import synthetic as S
+
+import oracle as orc
+
+
+# ----------------------------------------------------------------- helpers
+def brute_matrix(o) -> sp.csr_matrix:
+    """A as an explicit sparse matrix, column by column from o.spmv(e_n)."""
+    n = int(np.prod(o.rows_shape))
+    cols = []
+    e = np.zeros(n)
+    for q in range(n):
+        e[q] = 1.0
+        cols.append(sp.csr_matrix(o.spmv(e).ravel()[:, None]))
+        e[q] = 0.0
+    return sp.hstack(cols).tocsr()
+
+
+def stencil_matrix(o, c) -> sp.csr_matrix:
+    """A from the stored coefficient arrays and the BC rows, written as the
+    sparse matrix of wind_iterative.F90:912-1025 (independent of spmv's loops)."""
+    nx, ny, nz = o.nx, o.ny, o.nz
+    mx, my, mz = nx + 2, ny + 2, nz + 2
+    idx = lambda i, k, j: i + mx * (k + mz * j)  # noqa: E731
+    rows, cols, vals = [], [], []
+    cf = {q: o.coef(q) for q in "BCHIJKLMNO"}
+    dz_if, sx, sy = o.planes()
+    inv = 1.0 / (o.dx * o.dx)
+    for j in range(my):
+        for k in range(mz):
+            for i in range(mx):
+                r = idx(i, k, j)
+                if i in (0, mx - 1) or j in (0, my - 1):
+                    rows.append(r); cols.append(r); vals.append(1.0)
+                    continue
+                if k == mz - 1:
+                    d = dz_if[j - 1, nz, i - 1]
+                    rows += [r, r]; cols += [idx(i, k, j), idx(i, k - 1, j)]; vals += [1 / d, -1 / d]
+                    continue
+                if k == 0:
+                    al = c.alpha[j - 1, 0, i - 1]
+                    den = 2.0 * (sx[j - 1, i - 1] ** 2 + sy[j - 1, i - 1] ** 2 + al ** 2) / c.metrics["jacobian"][j - 1, 0, i - 1]
+                    cx = sx[j - 1, i - 1] / (den * 2 * o.dx)
+                    cy = sy[j - 1, i - 1] / (den * 2 * o.dx)
+                    cz = 1.0 / dz_if[j - 1, 0, i - 1]
+                    for (di, dk, dj), v in {(0, 0, -1): cy, (0, 1, -1): cy, (-1, 0, 0): cx, (0, 0, 0): -cz,
+                                            (1, 0, 0): -cx, (-1, 1, 0): cx, (0, 1, 0): cz, (1, 1, 0): -cx,
+                                            (0, 0, 1): -cy, (0, 1, 1): -cy}.items():
+                        rows.append(r); cols.append(idx(i + di, k + dk, j + dj)); vals.append(v)
+                    continue
+                g = lambda q: cf[q][j - 1, k - 1, i - 1]  # noqa: E731
+                b, cc = g("B"), g("C")
+                ent = {(0, 0, 0): -4 * inv - b - cc, (0, 1, 0): b, (0, -1, 0): cc, (1, 0, 0): inv,
+                       (-1, 0, 0): inv, (0, 0, 1): inv, (0, 0, -1): inv, (1, 1, 0): g("H"), (-1, 1, 0): g("I"),
+                       (1, -1, 0): g("J"), (-1, -1, 0): g("K"), (0, 1, 1): g("L"), (0, 1, -1): g("M"),
+                       (0, -1, 1): g("N"), (0, -1, -1): g("O")}
+                for (di, dk, dj), v in ent.items():
+                    rows.append(r); cols.append(idx(i + di, k + dk, j + dj)); vals.append(v)
+    n = mx * my * mz
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+
+
+def tiny_case(seed=1):
+    return S.gaussian_hill_case(nx=7, ny=6, nz=4, dx=400.0, seed=seed)
+
+
+def assembled(c, alpha=None):
+    o = oracle_for(c)
+    o.set_alpha(c.alpha if alpha is None else alpha)
+    o.assemble()
+    o.extract_diagonal()
+    return o
+
+
+# -------------------------------------------------------------- reductions
+def _butterfly_tree(vals):
+    """The canonical order written as a recursive pairing, independently of the C code."""
+    v = list(vals)
+    for off in (32, 16, 8, 4, 2, 1):
+        v = [v[l] + v[l ^ off] for l in range(64)]
+    return v[0]
+
+
+@pytest.mark.parametrize("P", [1, 255, 256, 1000, 70000])
+def test_canonical_reduce(P):
+    rng = np.random.default_rng(P)
+    x = rng.standard_normal(P)
+    got = orc.canon_reduce(x)
+    assert abs(got - math.fsum(x)) <= 1e-13 * np.abs(x).sum()
+    ints = rng.integers(-1000, 1000, size=P).astype(np.float64)  # exact in any order
+    assert orc.canon_reduce(ints) == ints.sum()
+    if P <= 1000:  # the exact tree
+        nb = (P + 255) // 256
+        pad = np.zeros(nb * 256)
+        pad[:P] = x
+        parts = []
+        for b in range(nb):
+            w = [_butterfly_tree(pad[256 * b + 64 * q: 256 * b + 64 * q + 64]) for q in range(4)]
+            parts.append(((w[0] + w[1]) + w[2]) + w[3])
+        lanes = []
+        for t in range(256):
+            acc = 0.0
+            for q in range(t, nb, 256):
+                acc = acc + parts[q]
+            lanes.append(acc)
+        w = [_butterfly_tree(lanes[64 * q: 64 * q + 64]) for q in range(4)]
+        assert got == ((w[0] + w[1]) + w[2]) + w[3]
+
+
+# ---------------------------------------------------------------- assembly
+def test_flat_terrain_closed_form():
+    """Flat terrain, alpha = 1: H..O = 0 and B, C, A are the textbook
+    variable-dz 7-point Laplacian: B = 1/(dz_k dz_if(k+1)), C = 1/(dz_k dz_if(k))."""
+    c = S.flat_plane_case(nx=8, ny=7, nz=9, dx=300.0)
+    dz = c.metrics["dz_levels"]
+    o = assembled(c)
+    dzif = np.concatenate([[dz[0]], (dz[1:] + dz[:-1]) / 2, [dz[-1]]])
+    Bx = 1.0 / (dz * dzif[1:])
+    Cx = 1.0 / (dz * dzif[:-1])
+    B, C, A = o.coef("B"), o.coef("C"), o.coef("A")
+    np.testing.assert_allclose(B, np.broadcast_to(Bx[None, :, None], B.shape), rtol=1e-14)
+    np.testing.assert_allclose(C, np.broadcast_to(Cx[None, :, None], C.shape), rtol=1e-14)
+    np.testing.assert_allclose(A, -4.0 / 300.0 ** 2 - B - C, rtol=1e-14)
+    for q in "HIJKLMNO":
+        assert np.all(o.coef(q) == 0.0)
+    for q in "DEFG":
+        assert np.all(o.coef(q) == 1.0 / 300.0 ** 2)
+
+
+@pytest.mark.parametrize("name", ["hill", "fractal_alpha"])
+def test_coefficient_invariants(name):
+    c = small_case(name)
+    o = assembled(c)
+    for a, b in (("J", "H"), ("K", "I"), ("N", "L"), ("O", "M")):
+        assert np.array_equal(o.coef(a), -o.coef(b))
+    # every interior row sums to zero (A x 1 = 0 away from the BC rows)
+    tot = sum(o.coef(q) for q in "ABCDEFGHIJKLMNO")
+    assert np.abs(tot).max() <= 1e-15 * np.abs(o.coef("A")).max() * 8
+
+
+def test_mirror_symmetry_of_slope_terms():
+    """Mirroring the terrain in x swaps H and I (and J, K) cell by cell and leaves
+    B, C unchanged; a wrong face index in H or I breaks this."""
+    c = S.gaussian_hill_case(nx=20, ny=17, nz=6, dx=250.0)
+    m = {k: v for k, v in c.metrics.items() if isinstance(v, np.ndarray)}
+    mir = dict(m)
+    for k in ("dzdx", "dzdy", "jacobian", "jacobian_v", "jacobian_w", "dzdy_v", "advection_dz"):
+        mir[k] = m[k][..., ::-1].copy()
+    mir["dzdx"] = -mir["dzdx"]
+    mir["jacobian_u"] = m["jacobian_u"][..., ::-1].copy()
+    mir["dzdx_u"] = -m["dzdx_u"][..., ::-1].copy()
+    mir["terrain"] = m["terrain"][:, ::-1].copy()
+    o1 = orc.Oracle(c.nx, c.ny, c.nz, c.dx); o1.set_metrics(m); o1.set_alpha(c.alpha); o1.assemble()
+    o2 = orc.Oracle(c.nx, c.ny, c.nz, c.dx); o2.set_metrics(mir); o2.set_alpha(c.alpha[..., ::-1].copy()); o2.assemble()
+    f = lambda a: a[..., ::-1]  # noqa: E731
+    np.testing.assert_allclose(f(o2.coef("H")), o1.coef("I"), rtol=1e-13, atol=1e-25)
+    np.testing.assert_allclose(f(o2.coef("I")), o1.coef("H"), rtol=1e-13, atol=1e-25)
+    np.testing.assert_allclose(f(o2.coef("B")), o1.coef("B"), rtol=1e-12)
+    np.testing.assert_allclose(f(o2.coef("C")), o1.coef("C"), rtol=1e-12)
+    np.testing.assert_allclose(f(o2.coef("L")), o1.coef("L"), rtol=1e-13, atol=1e-25)
+
+
+def _height_residual(o, c):
+    z = c.metrics["z"]
+    lam = np.zeros(o.rows_shape)
+    lam[1:-1, 1:-1, 1:-1] = z
+    lam[:, 0, :] = 2 * lam[:, 1, :] - lam[:, 2, :]
+    lam[:, -1, :] = 2 * lam[:, -2, :] - lam[:, -3, :]
+    y = o.spmv(lam)[1:-1, 1:-1, 1:-1][4:-4, 3:-3, 4:-4]
+    return np.abs(y).mean()
+
+
+def test_operator_annihilates_physical_height():
+    """The terrain-following operator discretises a Laplacian-type operator with
+    alpha = 1; the physical height z(x, y, zeta) is harmonic, so A z is only
+    truncation error in the interior. Flipping the sign of the slope couplings
+    H..O (a plausible mistake) makes it ~20x larger."""
+    c = S.gaussian_hill_case(nx=48, ny=44, nz=16, dx=250.0)
+    o = assembled(c)
+    good = _height_residual(o, c)
+    for q in "HIJKLMNO":
+        o.set_coef(q, -o.coef(q))
+    bad = _height_residual(o, c)
+    assert good < 5e-6
+    assert bad > 10 * good
+
+
+# -------------------------------------------------------------------- spmv
+@pytest.mark.parametrize("seed", [1, 2])
+def test_spmv_matches_sparse_matrix(seed):
+    c = tiny_case(seed)
+    o = assembled(c)
+    Am = stencil_matrix(o, c)
+    x = np.random.default_rng(seed).standard_normal(o.rows_shape)
+    np.testing.assert_allclose(o.spmv(x).ravel(), Am @ x.ravel(), rtol=1e-13, atol=1e-13 * np.abs(x).max())
+    # and the brute-force matrix (columns of spmv) is the same matrix
+    Ab = brute_matrix(o)
+    assert abs(Ab - Am).max() <= 1e-15 * abs(Am).max() * 10
+
+
+def test_diagonal_and_preconditioner_closed_form():
+    c = tiny_case(3)
+    o = assembled(c)
+    Am = brute_matrix(o)
+    d = Am.diagonal()
+    np.testing.assert_allclose(o.rows("dinv").ravel(), 1.0 / d, rtol=1e-15)
+    x = np.random.default_rng(0).standard_normal(o.rows_shape).ravel()
+    Dinv = sp.diags(1.0 / d)
+    I = sp.identity(Am.shape[0])
+    np.testing.assert_allclose(o.apply_precond(1, x).ravel(), Dinv @ x, rtol=1e-14)
+    M2 = Dinv @ (2 * I - Am @ Dinv)
+    np.testing.assert_allclose(o.apply_precond(2, x).ravel(), M2 @ x, rtol=1e-12, atol=1e-12 * np.abs(M2 @ x).max())
+    M3 = Dinv + (I - Dinv @ Am) @ M2
+    np.testing.assert_allclose(o.apply_precond(3, x).ravel(), M3 @ x, rtol=1e-11, atol=1e-11 * np.abs(M3 @ x).max())
+
+
+# ---------------------------------------------------------------- BiCGStab
+def _textbook_bicgstab(A, M, b, iters):
+    """Right-preconditioned BiCGStab (van der Vorst 1992) from x0 = 0."""
+    x = np.zeros_like(b)
+    r = b.copy(); rh = r.copy(); p = r.copy()
+    rho = rh @ r
+    out = []
+    for _ in range(iters):
+        ph = M @ p
+        v = A @ ph
+        alpha = rho / (rh @ v)
+        s = r - alpha * v
+        sh = M @ s
+        t = A @ sh
+        omega = (t @ s) / (t @ t)
+        x = x + alpha * ph + omega * sh
+        r = s - omega * t
+        rho_new = rh @ r
+        beta = (rho_new / rho) * (alpha / omega)
+        p = r + beta * (p - omega * v)
+        rho = rho_new
+        out.append((alpha, omega, np.linalg.norm(r)))
+    return x, out
+
+
+def test_bicgstab_scalars_match_textbook():
+    c = tiny_case(4)
+    o = assembled(c)
+    rng = np.random.default_rng(4)
+    div = rng.standard_normal((c.ny, c.nz, c.nx)) * 1e-3
+    o.compute_rhs(div)
+    Am = brute_matrix(o)
+    d = Am.diagonal()
+    Dinv = sp.diags(1.0 / d)
+    I = sp.identity(Am.shape[0])
+    M2 = (Dinv @ (2 * I - Am @ Dinv)).tocsr()
+    b = o.rows("rhs").ravel()
+    _, ref = _textbook_bicgstab(Am, M2, b, 8)
+    res = o.bicgstab(max_iters=8, tol_rel=1e-30, tol_abs=0.0, trace_cap=8)
+    tr = res["trace"]
+    for it in range(8):
+        a, w, rn = ref[it]
+        # rounding differences grow along the trajectory; a wrong formula is O(1)
+        assert abs(tr[it, 1] - a) <= 1e-6 * abs(a)
+        assert abs(tr[it, 2] - w) <= 1e-6 * abs(w)
+        assert abs(tr[it, 4] - rn) <= 1e-6 * rn
+
+
+@pytest.mark.parametrize("name", ["hill", "fractal_alpha"])
+def test_solve_matches_direct_solve(name):
+    c = small_case(name)
+    o = oracle_for(c)
+    div = o.divergence(c.u, c.v, c.w, rho=c.rho, adv_den=True)
+    r = o.solve(c.alpha, div, max_iters=20000, tol_rel=1e-13, tol_abs=0.0)
+    assert r["status"] == 0
+    Am = stencil_matrix(o, c)
+    lam_direct = spla.spsolve(Am.tocsc(), o.rows("rhs").ravel())
+    lam = o.get_lambda().ravel()
+    assert np.linalg.norm(lam - lam_direct) <= 1e-9 * np.linalg.norm(lam_direct)
+
+
+def test_manufactured_solution():
+    c = small_case("hill")
+    o = assembled(c)
+    rng = np.random.default_rng(9)
+    lam_star = np.zeros(o.rows_shape)
+    lam_star[1:-1, :, 1:-1] = rng.standard_normal(lam_star[1:-1, :, 1:-1].shape)
+    b = o.spmv(lam_star)
+    # drive the solve with this b: rhs = -2 div on interior rows, so feed the
+    # rows through a div that reproduces b there and keep b's BC rows by
+    # solving the same system from the explicit matrix
+    Am = stencil_matrix(o, c)
+    lam = spla.spsolve(Am.tocsc(), b.ravel())
+    np.testing.assert_allclose(lam, lam_star.ravel(), rtol=1e-8, atol=1e-8)
+    # and the oracle's Krylov solve on an interior-only right-hand side
+    div = -0.5 * b[1:-1, 1:-1, 1:-1]
+    lam_int = lam_star.copy()
+    o.set_lambda(np.zeros(o.rows_shape))
+    o.compute_rhs(div)
+    r = o.bicgstab(max_iters=20000, tol_rel=1e-11, tol_abs=0.0)
+    assert r["status"] == 0
+    x = o.get_lambda()
+    np.testing.assert_allclose(o.spmv(x)[1:-1, 1:-1, 1:-1], b[1:-1, 1:-1, 1:-1], rtol=0,
+                               atol=1e-10 * np.abs(b).max())
+    del lam_int
+
+
+def test_retry_ladder_and_status_codes():
+    """Status 1 at the iteration cap triggers the sweep ladder 2 -> 3 -> 4 from
+    x = 0 (wind_iterative.F90:474-504); a zero rhs converges at iteration 0."""
+    c = small_case("hill")
+    o = oracle_for(c)
+    div = o.divergence(c.u, c.v, c.w)
+    r = o.solve(c.alpha, div, max_iters=2, tol_rel=1e-12)
+    assert r["status"] == 1 and r["nsweeps"] == 4
+    o.set_lambda(np.zeros(o.rows_shape))  # the warm start would otherwise carry lambda over
+    r = o.solve(c.alpha, np.zeros_like(div), max_iters=100, tol_rel=1e-8)
+    assert r["status"] == 0 and r["iters"] == 0
+
+
+# ---------------------------------------------------- divergence and correction
+def test_divergence_closed_form_flat():
+    """Flat terrain, adv_den false, dqdt: div = (u(i+1)-u(i))/dx + (v(j+1)-v(j))/dx
+    + (w(k) - w(k-1))/dz(k) with w(0) = 0."""
+    c = S.flat_plane_case(nx=9, ny=8, nz=5, dx=200.0)
+    o = oracle_for(c)
+    rng = np.random.default_rng(1)
+    u = rng.standard_normal(c.u.shape); v = rng.standard_normal(c.v.shape); w = rng.standard_normal(c.w.shape)
+    div = o.divergence(u, v, w, adv_den=False, use_dqdt=True)
+    dz = c.metrics["dz_levels"]
+    wpad = np.concatenate([np.zeros((c.ny, 1, c.nx)), w], axis=1)
+    ref = (u[:, :, 1:] - u[:, :, :-1]) / 200.0 + (v[1:] - v[:-1]) / 200.0 + (wpad[:, 1:] - wpad[:, :-1]) / dz[None, :, None]
+    np.testing.assert_allclose(div, ref, rtol=1e-12, atol=1e-15)
+
+
+def test_correction_closed_form_flat():
+    """Flat terrain, rho = 1, lambda linear in x: du = 0.5 a / dx on every face
+    between interior cells; lambda linear in k: dw = 0.5 alpha^2 b / dz_if."""
+    c = S.flat_plane_case(nx=9, ny=8, nz=5, dx=200.0)
+    o = oracle_for(c)
+    o.set_alpha(c.alpha)
+    lam = np.zeros(o.rows_shape)
+    i = np.arange(c.nx + 2)
+    lam[1:-1, 1:-1, 1:-1] = 3.0 * i[1:-1][None, None, :]
+    o.set_lambda(lam)
+    u, v, w = o.correct(np.zeros_like(c.u), np.zeros_like(c.v), np.zeros_like(c.w))
+    np.testing.assert_allclose(u[:, :, 1:-1], 0.5 * 3.0 / 200.0, rtol=1e-14)
+    assert np.all(v[1:-1] == 0.0)
+    kk = np.arange(c.nz + 2)
+    lam[1:-1, 1:-1, 1:-1] = 2.0 * kk[1:-1][None, :, None]
+    o.set_lambda(lam)
+    u, v, w = o.correct(np.zeros_like(c.u), np.zeros_like(c.v), np.zeros_like(c.w))
+    dz = c.metrics["dz_levels"]
+    dzif = (dz[1:] + dz[:-1]) / 2
+    np.testing.assert_allclose(w[:, :-1, :], (0.5 * 2.0 / dzif)[None, :, None] * np.ones_like(w[:, :-1, :]), rtol=1e-14)
+    assert np.all(w[:, -1, :] == 0.0)
+
+
+def test_probed_operator_is_composition():
+    """A_probed = 2 D(G(.)) (wind_iterative.F90:1081-1089): zero leakage, the probed
+    operator reproduces 2 D(G(x)) on random x, and after the probed solve the
+    corrected winds are divergence free to the solve tolerance."""
+    c = small_case("hill")
+    o = oracle_for(c)
+    o.set_alpha(c.alpha)
+    assert o.probe(rho=c.rho, adv_den=True) == 0.0
+    o.extract_diagonal()
+    rng = np.random.default_rng(3)
+    x = np.zeros(o.rows_shape)
+    x[1:-1, 1:-1, 1:-1] = rng.standard_normal(x[1:-1, 1:-1, 1:-1].shape)
+    y = o.spmv(x)
+    o.set_lambda(x)
+    u, v, w = o.correct(np.zeros_like(c.u), np.zeros_like(c.v), np.zeros_like(c.w), rho=c.rho, adv_den=True)
+    dg = o.divergence(u, v, w, rho=c.rho, adv_den=True, use_dqdt=True)
+    np.testing.assert_allclose(y[1:-1, 1:-1, 1:-1], 2.0 * dg, rtol=1e-11, atol=1e-12 * np.abs(2 * dg).max())
+    o.set_lambda(np.zeros(o.rows_shape))
+    div = o.divergence(c.u, c.v, c.w, rho=c.rho, adv_den=True)
+    r = o.solve(c.alpha, div, max_iters=5000, tol_rel=1e-11)
+    assert r["status"] == 0
+    u, v, w = o.correct(c.u, c.v, c.w, rho=c.rho, adv_den=True)
+    d1 = o.divergence(u, v, w, rho=c.rho, adv_den=True)
+    assert np.linalg.norm(d1) <= 1e-9 * np.linalg.norm(div)
+
+
+def test_reference_unit_behaviour_on_oracle():
+    """tests/unit/test_wind_iterative.F90:139-152: finite and max|div| decreases."""
+    c = S.flat_plane_case(nx=24, ny=22, nz=20)
+    o = oracle_for(c)
+    div0 = o.divergence(c.u, c.v, c.w, adv_den=False, use_dqdt=True)
+    r = o.solve(c.alpha, div0)
+    u, v, w = o.correct(c.u, c.v, c.w, adv_den=False)
+    div1 = o.divergence(u, v, w, adv_den=False, use_dqdt=True)
+    assert r["status"] == 0
+    assert np.all(np.isfinite(div1)) and np.abs(div1).max() < np.abs(div0).max()
+
+
+# ------------------------------------------------------------ input generator
+def test_auto_dz_closed_form():
+    dz = S.auto_dz(40, 10.0, 12000.0)
+    assert abs(dz[0] - 10.0) < 1e-9 and abs(dz.sum() - 12000.0) < 1e-6 and np.all(np.diff(dz) > 0)
+    # second differences of a cubic are linear in the level index
+    d3 = np.diff(dz, 2)
+    assert np.allclose(np.diff(d3), np.diff(d3)[0], rtol=1e-6, atol=1e-9)
+
+
+def test_smooth_array_matches_brute_force():
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((9, 13))
+    w = 2
+    ref = np.empty_like(a)
+    for j in range(9):
+        for i in range(13):
+            ref[j, i] = a[max(0, j - w):j + w + 1, max(0, i - w):i + w + 1].mean()
+    np.testing.assert_allclose(S.smooth_array_2d(a, w, 1), ref, rtol=1e-13)
