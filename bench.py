@@ -42,9 +42,9 @@ def kernel_bytes(kind: str, nx: int, ny: int, nz: int) -> float:
     t = 8 if kind.endswith("f64") or kind in ("ir_resid", "true_resid") else 4
     coef = 6 * t * cells + 4 * t * P
     table = {
-        "sweep": 3 * N * t + coef,       # y (stencil), in, coefficients -> out
-        "sigma": 3 * N * t + coef,       # p_hat (stencil), r_hat, coefficients -> v
-        "omega": 4 * N * t + coef,       # s_hat (stencil), s, r_hat, coefficients -> t
+        "sweep": 3 * N * t + coef,        # y (stencil), in, coefficients -> out
+        "sweep_sigma": 5 * N * t + coef,  # p_hat1 (stencil), p, r_hat, coefficients -> p_hat, v
+        "sweep_omega": 5 * N * t + coef,  # s_hat1 (stencil), s, r_hat, coefficients -> s_hat, t
         "update_s": 5 * N * t,           # r, v, D^-1 -> s, s_hat
         "update_xrp": 12 * N * t,        # x, p_hat, s_hat, s, t, p, v, D^-1 -> x, r, p, p_hat
         "init": 7 * N * t + coef,        # x (stencil), b, D^-1, coefficients -> r, r_hat, p, p_hat
@@ -100,6 +100,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=6)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 arm")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy a 1 GiB device buffer three times (known traffic for counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -126,6 +128,14 @@ def main():
     div = torch.empty(s.mass_shape, dtype=torch.float64, device=dev)
     s.divergence(u, v, w, density=rho, adv_den=True, use_dqdt=True, out=div)
     torch.cuda.synchronize()
+    if args.copy_calibration:
+        src = torch.ones(2 ** 27, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        print("[bench] copy calibration: 3 x 1 GiB device copies", file=sys.stderr, flush=True)
 
     arms = {
         "mixed": dict(precision=1, start=1, tol_rel=1e-8, inner_tol=args.inner_tol, max_iters=50000, profile=1),

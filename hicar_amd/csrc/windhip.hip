@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -99,6 +100,7 @@ struct windhip_solver {
     double **cf_dev = nullptr;
 
     bool metrics_set = false, structure_uploaded = false, probed = false;
+    bool verbose = false;  // WINDHIP_VERBOSE=1: per-batch progress on stderr (verbose_solver, :274-275)
     int nsweeps = 2;  // precond_n_sweeps, BASE_PREC_SWEEPS (wind_iterative.F90:76)
     std::map<long, hipGraphExec_t> graphs;
 
@@ -188,6 +190,13 @@ struct windhip_solver {
     OpA<T> opa() const;
     template <typename T>
     OpP<T> opp() const;
+    // call f with the operator in use (analytic or probed), chosen at compile time in the kernels
+    template <typename T, class F>
+    int with_op(F &&f) const
+    {
+        if (probed) return f(opp<T>());
+        return f(opa<T>());
+    }
 
     // gather a caller field into the row grid (FP64)
     int gather3(const void *src, int kind, int ia, int ib, int ka, int kb, int ja, int jb, double *dst,
@@ -314,7 +323,9 @@ int windhip_solver::init()
     d.P = (long)d.mx * d.my;
     d.dx = desc.dx;
     g.mx = d.mx; g.my = d.my; g.mz = d.mz; g.P = d.P;
-    g.nb = (int)((d.P + kBlock - 1) / kBlock);
+    g.nti = (d.mx + kTileI - 1) / kTileI;
+    g.ntj = (d.my + kTileJ - 1) / kTileJ;
+    g.nt = g.nti * g.ntj;
     N = d.P * d.mz;
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ev0));
@@ -330,7 +341,7 @@ int windhip_solver::init()
     for (double **q : pl64) CHK(alloc(q, d.P));
     float **pl32[] = {&bcx32, &bcy32, &bcz32, &dztop32};
     for (float **q : pl32) CHK(alloc(q, d.P));
-    CHK(alloc(&partials, (long)g.nb * 8));
+    CHK(alloc(&partials, (long)g.nt * 8));
     CHK(alloc(&scan_part, 3L * 1024));
     CHK(alloc(&scan_out, 4));
     CHK(alloc(&scan_cnt, 4));
@@ -493,9 +504,6 @@ int windhip_solver::harvest()
 template <typename T>
 int windhip_solver::enqueue_iterations(int count)
 {
-    const OpA<T> a = opa<T>();
-    const OpP<T> pp = opp<T>();
-    const int pr = probed ? 1 : 0;
     const int prec = sizeof(T) == 8 ? 0 : 1;
     KState *sp = st + prec;
     T *X, *R, *RH, *Pv, *V, *S, *Tv, *PH[3], *SH[3];
@@ -503,30 +511,42 @@ int windhip_solver::enqueue_iterations(int count)
     kvec<T>(&X, &R, &RH, &Pv, &V, &S, &Tv, PH, SH, &DI);
     const int fg = flat_grid(N);
     const int base = 5 * prec;
-    for (int it = 0; it < count; ++it) {
-        T *cur = PH[0];
-        for (int sw = 2, q = 1; sw <= nsweeps; ++sw) {
-            T *nxt = PH[q];
-            CHK(timed(base + PK_SWEEP, [&] { k_sweep<T><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, sp, cur, Pv, nxt); }));
-            cur = nxt;
-            q = (q == 1) ? 2 : 1;
+    return with_op<T>([&](auto op) -> int {
+        using Op = decltype(op);
+        for (int it = 0; it < count; ++it) {
+            // sweeps 2..n-1 alone, the last sweep fused with v = A p_hat and sigma
+            T *cur = PH[0];
+            int q = 1;
+            for (int sw = 2; sw < nsweeps; ++sw) {
+                T *nxt = PH[q];
+                CHK(timed(base + PK_SWEEP, [&] { k_sweep<Op><<<g.nt, kThreads, 0, stream>>>(op, g, sp, cur, Pv, nxt); }));
+                cur = nxt;
+                q = (q == 1) ? 2 : 1;
+            }
+            T *phat = PH[q];
+            CHK(timed(base + PK_SIGMA, [&] {
+                k_sweep_sigma<Op><<<g.nt, kThreads, 0, stream>>>(op, g, sp, partials, cur, Pv, phat, RH, V);
+            }));
+            CHK(timed(base + PK_UPD_S, [&] { k_update_s<T><<<fg, kBlock, 0, stream>>>(N, sp, R, V, DI, S, SH[0]); }));
+            T *cs = SH[0];
+            q = 1;
+            for (int sw = 2; sw < nsweeps; ++sw) {
+                T *nxt = SH[q];
+                CHK(timed(base + PK_SWEEP, [&] { k_sweep<Op><<<g.nt, kThreads, 0, stream>>>(op, g, sp, cs, S, nxt); }));
+                cs = nxt;
+                q = (q == 1) ? 2 : 1;
+            }
+            T *shat = SH[q];
+            CHK(timed(base + PK_OMEGA, [&] {
+                k_sweep_omega<Op><<<g.nt, kThreads, 0, stream>>>(op, g, sp, partials, cs, S, shat, RH, Tv);
+            }));
+            CHK(timed(base + PK_UPD_XRP, [&] {
+                k_update_xrp<T><<<fg, kBlock, 0, stream>>>(N, sp, X, phat, shat, S, Tv, R, Pv, V, DI, PH[0]);
+            }));
         }
-        CHK(timed(base + PK_SIGMA, [&] { k_sigma<T><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, sp, partials, cur, RH, V); }));
-        CHK(timed(base + PK_UPD_S, [&] { k_update_s<T><<<fg, kBlock, 0, stream>>>(N, sp, R, V, DI, S, SH[0]); }));
-        T *cs = SH[0];
-        for (int sw = 2, q = 1; sw <= nsweeps; ++sw) {
-            T *nxt = SH[q];
-            CHK(timed(base + PK_SWEEP, [&] { k_sweep<T><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, sp, cs, S, nxt); }));
-            cs = nxt;
-            q = (q == 1) ? 2 : 1;
-        }
-        CHK(timed(base + PK_OMEGA, [&] { k_omega<T><<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, sp, partials, cs, S, RH, Tv); }));
-        CHK(timed(base + PK_UPD_XRP, [&] {
-            k_update_xrp<T><<<fg, kBlock, 0, stream>>>(N, sp, X, cur, cs, S, Tv, R, Pv, V, DI, PH[0]);
-        }));
-    }
-    HIPCHK(hipGetLastError());
-    return WINDHIP_OK;
+        HIPCHK(hipGetLastError());
+        return WINDHIP_OK;
+    });
 }
 
 // Launch `count` iterations, from a cached hipGraph when use_graph.
@@ -558,11 +578,10 @@ int windhip_solver::run_plain(int max_iters, double tol_rel, double tol_abs, int
                               int *status, int *iters, double *res0, double *resf)
 {
     k_state_reset<<<1, 1, 0, stream>>>(st, max_iters, tol_rel, tol_abs, 0.0);
-    const OpA<double> a = opa<double>();
-    const OpP<double> pp = opp<double>();
-    CHK(timed(PK_INIT, [&] {
-        k_init<double><<<g.nb, kBlock, 0, stream>>>(a, pp, probed ? 1 : 0, g, st, partials, x, b, dinv, r, rh, p,
-                                                    ph[0]);
+    CHK(with_op<double>([&](auto op) {
+        return timed(PK_INIT, [&] {
+            k_init<decltype(op)><<<g.nt, kThreads, 0, stream>>>(op, g, st, partials, x, b, dinv, r, rh, p, ph[0]);
+        });
     }));
     HIPCHK(hipGetLastError());
     CHK(read_state(0));
@@ -583,8 +602,10 @@ int windhip_solver::run_plain(int max_iters, double tol_rel, double tol_abs, int
 
 int windhip_solver::true_residual(double *rel)
 {
-    CHK(timed(PK_TRUE_RESID, [&] {
-        k_true_resid<<<g.nb, kBlock, 0, stream>>>(opa<double>(), opp<double>(), probed ? 1 : 0, g, st, partials, x, b);
+    CHK(with_op<double>([&](auto op) {
+        return timed(PK_TRUE_RESID, [&] {
+            k_true_resid<decltype(op)><<<g.nt, kThreads, 0, stream>>>(op, g, st, partials, x, b);
+        });
     }));
     HIPCHK(hipGetLastError());
     CHK(read_state(0));
@@ -593,66 +614,60 @@ int windhip_solver::true_residual(double *rel)
     return WINDHIP_OK;
 }
 
-// FP32 BiCGStab (matvec, vectors, coefficients in FP32; dots, scalars and
-// the solution in FP64) inside FP64 iterative refinement by reliable updates:
-// the FP32 iterate is flushed into the FP64 lambda and the FP32 residual
-// replaced by the FP64 true residual b - A lambda whenever the recursive norm
-// has fallen by the factor opts.inner_tol, and at the end; the solve stops
-// when the true residual meets the FP64 target.
+// FP32 BiCGStab (matvec, vectors, coefficients in FP32; dots and scalars in
+// FP64) inside FP64 iterative refinement: each refinement step solves
+// A d = r for the current FP64 true residual r = b - A lambda with a fresh FP32
+// BiCGStab from d = 0 down to opts.inner_tol * ||r||, then lambda += d in FP64.
+// The inner solve asks for a modest reduction on purpose: an FP32 recursion on
+// this operator cannot get far below kappa * eps32 of its own starting
+// residual, while a fresh start per step keeps every step well inside that.
 int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *info)
 {
     CHK(alloc_f32_krylov());
     HIPCHK(hipMemsetAsync(x32, 0, N * sizeof(float), stream));
+    const double inner_tol = (o.inner_tol > 0 && o.inner_tol < 1) ? o.inner_tol : 0.1;
     k_state_reset<<<1, 1, 0, stream>>>(st, 0, o.tol_rel, o.tol_abs, 0.0);
-    k_state_reset<<<1, 1, 0, stream>>>(st + 1, o.max_iters, 0.0, 0.0, o.inner_tol);
-    const OpA<double> a = opa<double>();
-    const OpP<double> pp = opp<double>();
-    const int pr = probed ? 1 : 0;
-    auto resid = [&](int restart) -> int {
-        CHK(timed(PK_IR_RESID, [&] {
-            k_ir_resid<<<g.nb, kBlock, 0, stream>>>(a, pp, pr, g, st, st + 1, partials, restart, x, b, dinv32, r32,
-                                                    rh32, p32, ph32[0]);
+    k_state_reset<<<1, 1, 0, stream>>>(st + 1, o.inner_max_iters, 0.0, 0.0, inner_tol);
+    auto resid = [&]() -> int {
+        CHK(with_op<double>([&](auto op) {
+            return timed(PK_IR_RESID, [&] {
+                k_ir_resid<decltype(op)><<<g.nt, kThreads, 0, stream>>>(op, g, st, st + 1, partials, x, b, dinv32,
+                                                                        r32, rh32, p32, ph32[0]);
+            });
         }));
         HIPCHK(hipGetLastError());
         return read_state(0);
     };
-    auto flush = [&]() -> int {
-        CHK(timed(PK_IR_ACC, [&] { k_ir_accumulate<<<flat_grid(N), kBlock, 0, stream>>>(N, x, x32); }));
-        HIPCHK(hipGetLastError());
-        return WINDHIP_OK;
-    };
-    CHK(resid(1));
+    CHK(resid());
     info->res0 = st_host[0].outer_rnorm;
     info->bnorm = st_host[0].outer_bnorm;
-    const double delta = (o.inner_tol > 0 && o.inner_tol < 1) ? o.inner_tol : 0.1;
-    double rmax = st_host[0].outer_rnorm;
-    int updates = 0, stalls = 0, status = 0;
+    int total_inner = 0, outer = 0, status = 0, stalls = 0;
     double best = st_host[0].outer_rnorm;
-    while (!st_host[0].outer_done) {
-        const int left = o.max_iters - st_host[1].iter;
-        if (left <= 0) { status = 1; break; }
-        CHK(launch_batch(1, std::min(o.iters_per_sync, left), o.use_graph != 0));
-        CHK(read_state(1));
-        const KState &in = st_host[1];
-        const bool bad = !(in.rnorm == in.rnorm) || in.status == 2;
-        if (in.done || bad || in.rnorm < delta * rmax) {
-            CHK(flush());
-            // a finished or broken-down recursion restarts from the true residual
-            CHK(resid((in.done || bad) ? 1 : 0));
-            ++updates;
-            const double now = st_host[0].outer_rnorm;
-            if (now < best) { best = now; stalls = 0; }
-            else if (++stalls >= 4 || !(now == now)) { status = 2; break; }
-            rmax = now;
+    while (!st_host[0].outer_done && outer < o.max_outer && total_inner < o.max_iters) {
+        ++outer;
+        int done_iters = 0;
+        while (!st_host[1].done) {
+            const int n = std::min(o.iters_per_sync, std::max(1, o.inner_max_iters - done_iters));
+            CHK(launch_batch(1, n, o.use_graph != 0));
+            CHK(read_state(1));
+            done_iters += n;
+            if (done_iters > o.inner_max_iters + o.iters_per_sync) break;
         }
-    }
-    if (!st_host[0].outer_done && st_host[1].iter > 0) {
-        CHK(flush());
-        CHK(resid(1));
+        total_inner += st_host[1].iter;
+        CHK(timed(PK_IR_ACC, [&] { k_ir_accumulate<<<flat_grid(N), kBlock, 0, stream>>>(N, x, x32); }));
+        HIPCHK(hipGetLastError());
+        const int inner_status = st_host[1].status, inner_iters = st_host[1].iter;
+        CHK(resid());
+        const double now = st_host[0].outer_rnorm;
+        if (verbose)
+            fprintf(stderr, "[windhip] refinement %d: %d FP32 iterations (status %d), true residual %.3e (target %.3e)\n",
+                    outer, inner_iters, inner_status, now, st_host[0].outer_target);
+        if (now < best) { best = now; stalls = 0; }
+        else if (++stalls >= 3 || !(now == now)) { status = 2; break; }
     }
     info->status = st_host[0].outer_done ? 0 : (status ? status : 1);
-    info->iters = st_host[1].iter;
-    info->outer_iters = updates;
+    info->iters = total_inner;
+    info->outer_iters = outer;
     info->res_final = st_host[0].outer_rnorm;
     info->true_rel_res = info->bnorm > 0 ? st_host[0].outer_rnorm / info->bnorm : st_host[0].outer_rnorm;
     return WINDHIP_OK;
@@ -788,8 +803,8 @@ void windhip_default_opts(windhip_solve_opts *o)
     o->start = WINDHIP_START_WARM;
     o->precision = WINDHIP_PLAIN_F64;
     o->inner_tol = 0.1;
-    o->inner_max_iters = 4000;
-    o->max_outer = 30;
+    o->inner_max_iters = 2000;
+    o->max_outer = 60;
     o->iters_per_sync = 0;
     o->use_graph = 1;
     o->retry = 1;
@@ -815,6 +830,8 @@ int windhip_create(const windhip_desc *desc, windhip_solver **out)
         return fail(WINDHIP_ERR_HIP, "windhip_create: no HIP device available");
     windhip_solver *h = new windhip_solver();
     h->desc = s;
+    const char *vb = getenv("WINDHIP_VERBOSE");
+    h->verbose = vb && vb[0] && vb[0] != '0';
     int rc = h->init();
     if (rc != WINDHIP_OK) {
         std::string keep = g_err;
@@ -1004,12 +1021,17 @@ int windhip_time_kernel(windhip_solver *s, int which, int reps, double *avg_ms)
 {
     if (!s || !avg_ms || reps < 1) return fail(WINDHIP_ERR_ARG, "windhip_time_kernel: bad argument");
     if (!s->structure_uploaded) return fail(WINDHIP_ERR_ARG, "windhip_time_kernel: solve once first");
-    const int pr = s->probed ? 1 : 0;
     auto launch = [&]() {
         if (which == 0)
-            k_spmv<double><<<s->g.nb, kBlock, 0, s->stream>>>(s->opa<double>(), s->opp<double>(), pr, s->g, s->ph[1], s->v);
+            s->with_op<double>([&](auto op) {
+                k_spmv<decltype(op)><<<s->g.nt, kThreads, 0, s->stream>>>(op, s->g, s->ph[1], s->v);
+                return 0;
+            });
         else
-            k_spmv<float><<<s->g.nb, kBlock, 0, s->stream>>>(s->opa<float>(), s->opp<float>(), pr, s->g, s->ph32[1] ? s->ph32[1] : s->B32, s->v32 ? s->v32 : s->C32);
+            s->with_op<float>([&](auto op) {
+                k_spmv<decltype(op)><<<s->g.nt, kThreads, 0, s->stream>>>(op, s->g, s->ph32[1], s->v32);
+                return 0;
+            });
     };
     if (which == 1 && !s->x32) CHK(s->alloc_f32_krylov());
     launch();
@@ -1056,19 +1078,24 @@ int windhip_apply_operator(windhip_solver *s, int precision, const double *x_row
     HIPCHK(hipMalloc(&dx_, s->N * sizeof(double)));
     HIPCHK(hipMalloc(&dy_, s->N * sizeof(double)));
     HIPCHK(hipMemcpyAsync(dx_, tmp.data(), s->N * sizeof(double), hipMemcpyHostToDevice, s->stream));
-    const int pr = s->probed ? 1 : 0;
     if (precision == WINDHIP_MIXED) {
         float *fx = nullptr, *fy = nullptr;
         HIPCHK(hipMalloc(&fx, s->N * sizeof(float)));
         HIPCHK(hipMalloc(&fy, s->N * sizeof(float)));
         k_round_f32<<<flat_grid(s->N), kBlock, 0, s->stream>>>(s->N, dx_, fx);
-        k_spmv<float><<<s->g.nb, kBlock, 0, s->stream>>>(s->opa<float>(), s->opp<float>(), pr, s->g, fx, fy);
+        s->with_op<float>([&](auto op) {
+            k_spmv<decltype(op)><<<s->g.nt, kThreads, 0, s->stream>>>(op, s->g, fx, fy);
+            return 0;
+        });
         k_widen_f64<<<flat_grid(s->N), kBlock, 0, s->stream>>>(s->N, fy, dy_);
         HIPCHK(hipStreamSynchronize(s->stream));
         (void)hipFree(fx);
         (void)hipFree(fy);
     } else {
-        k_spmv<double><<<s->g.nb, kBlock, 0, s->stream>>>(s->opa<double>(), s->opp<double>(), pr, s->g, dx_, dy_);
+        s->with_op<double>([&](auto op) {
+            k_spmv<decltype(op)><<<s->g.nt, kThreads, 0, s->stream>>>(op, s->g, dx_, dy_);
+            return 0;
+        });
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(tmp.data(), dy_, s->N * sizeof(double), hipMemcpyDeviceToHost, s->stream));

@@ -32,13 +32,22 @@
 
 namespace windhip {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;          // elementwise kernels
+constexpr int kWaves = 8;            // stencil kernels: wavefronts per block = tile rows incl. halo
+constexpr int kThreads = 64 * kWaves;
+constexpr int kTileI = 62;           // inner columns of a tile row (64 lanes less the halo pair)
+constexpr int kTileJ = kWaves - 2;   // inner rows of a tile
 constexpr double kBreakdownEps = 1.0e-30;  // wind_iterative.F90:61
 
+// Tiling of the (i,j) columns: tile (ti,tj) covers lanes l = 0..63 of wave w =
+// 0..7 at column i = 62 ti + l - 1, j = 6 tj + w - 1; lanes 1..62 of waves 1..6
+// are its inner columns (every column is inner to exactly one tile), the rest
+// its halo.
 struct Geo {
     int mx, my, mz;
-    long P;   // columns
-    int nb;   // column blocks = ceil(P / 256)
+    long P;        // columns
+    int nti, ntj;  // tiles in i and j
+    int nt;        // tiles
 };
 
 // Device-resident scalars of one BiCGStab (bicgstab_solve, wind_iterative.F90:563-569).
@@ -53,36 +62,117 @@ struct KState {
     unsigned cnt[4];
 };
 
+// Three-level window of the five columns C, W, E, S, N: levels k-1, k, k+1.
+template <typename T>
+struct Win {
+    T c0, c1, c2, w0, w1, w2, e0, e1, e2, s0, s1, s2, n0, n1, n2;
+    __device__ __forceinline__ void shift()
+    {
+        c0 = c1; w0 = w1; e0 = e1; s0 = s1; n0 = n1;
+        c1 = c2; w1 = w2; e1 = e2; s1 = s2; n1 = n2;
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ T clamp_d(T d)
+{
+    if (fabs((double)d) < kBreakdownEps) d = copysign(T(kBreakdownEps), d);
+    return d;
+}
+
 // Analytic operator (spmv analytic branch, wind_iterative.F90:912-1025) in
 // compressed form: J=-H, K=-I, N=-L, O=-M, D=E=F=G=1/dx^2, A=-4/dx^2-B-C.
 template <typename T>
 struct OpA {
+    using type = T;
     const T *B, *C, *H, *I, *L, *M;   // rows
     const T *bcx, *bcy, *bcz, *dztop; // planes: bottom-row dzdx/(denom 2dx), dzdy/(...), 1/dz_if(1); top dz_if(nz+1)
     T inv, neg4;                      // 1/dx^2, -4/dx^2
+    struct Cf { T B, C, H, I, L, M; };
+    struct Bc { T bx, by, bz, dzt; };
+    __device__ __forceinline__ Cf load(long n) const { return Cf{B[n], C[n], H[n], I[n], L[n], M[n]}; }
+    __device__ __forceinline__ Bc bc(long c) const { return Bc{bcx[c], bcy[c], bcz[c], dztop[c]}; }
+    // (A x)(row) at level k of a column. FP64 evaluates the reference's sum
+    // term by term in its order (bit-identical to the oracle). FP32 evaluates
+    // the same operator in difference form: every row of A sums to zero
+    // (A = -4/dx^2 - B - C, J = -H, ...), so A x is a combination of
+    // neighbour differences, whose rounding is relative to the differences
+    // rather than to |x|; the smooth, slowly converging modes keep their
+    // accuracy in FP32.
+    __device__ __forceinline__ T row(int k, int mz, bool lat, const Win<T> &x, const Cf &f, const Bc &b) const
+    {
+        if (lat) return x.c1;
+        if (sizeof(T) == 8) {
+            if (k == 0)  // bottom BC row, wind_iterative.F90:962-988
+                return b.by * x.s1 + b.by * x.s2 + b.bx * x.w1 + (-b.bz) * x.c1 + (-b.bx) * x.e1 + b.bx * x.w2
+                       + b.bz * x.c2 + (-b.bx) * x.e2 + (-b.by) * x.n1 + (-b.by) * x.n2;
+            if (k == mz - 1) return (x.c1 - x.c0) / b.dzt;  // top BC row, :950-960
+            const T a = (neg4 - f.B) - f.C;
+            // interior 15-point row, :1005-1020
+            return (-f.M) * x.s0 + inv * x.s1 + f.M * x.s2 + (-f.I) * x.w0 + f.C * x.c0 + (-f.H) * x.e0
+                   + inv * x.w1 + a * x.c1 + inv * x.e1 + f.I * x.w2 + f.B * x.c2 + f.H * x.e2 + (-f.L) * x.n0
+                   + inv * x.n1 + f.L * x.n2;
+        }
+        if (k == 0)
+            return b.bz * (x.c2 - x.c1) + b.bx * ((x.w1 + x.w2) - (x.e1 + x.e2))
+                   + b.by * ((x.s1 + x.s2) - (x.n1 + x.n2));
+        if (k == mz - 1) return (x.c1 - x.c0) / b.dzt;
+        return inv * (((x.w1 - x.c1) + (x.e1 - x.c1)) + ((x.s1 - x.c1) + (x.n1 - x.c1)))
+               + f.B * (x.c2 - x.c1) + f.C * (x.c0 - x.c1) + f.H * (x.e2 - x.e0) + f.I * (x.w2 - x.w0)
+               + f.L * (x.n2 - x.n0) + f.M * (x.s2 - x.s0);
+    }
+    // 1/diag(A)(row) as extract_diagonal (wind_iterative.F90:1041-1075) computes it
+    __device__ __forceinline__ T dinv(int k, int mz, bool lat, const Cf &f, const Bc &b) const
+    {
+        T d;
+        if (lat) d = T(1);
+        else if (k == mz - 1) d = clamp_d<T>(T(1) / b.dzt);
+        else if (k == 0) d = clamp_d<T>(-b.bz);
+        else d = clamp_d<T>((neg4 - f.B) - f.C);
+        return T(1) / d;
+    }
 };
 
-// Probed 15-array operator (spmv probed branch, wind_iterative.F90:837-910).
+// Probed 15-array operator (spmv probed branch, wind_iterative.F90:837-910):
+// lateral and ghost-plane identity rows.
 template <typename T>
 struct OpP {
+    using type = T;
     const T *cf[15];  // A..O, rows
+    struct Cf { T a[15]; };
+    struct Bc {};
+    __device__ __forceinline__ Cf load(long n) const
+    {
+        Cf f;
+#pragma unroll
+        for (int q = 0; q < 15; ++q) f.a[q] = cf[q][n];
+        return f;
+    }
+    __device__ __forceinline__ Bc bc(long) const { return Bc{}; }
+    __device__ __forceinline__ T row(int k, int mz, bool lat, const Win<T> &x, const Cf &f, const Bc &) const
+    {
+        if (lat || k == 0 || k == mz - 1) return x.c1;
+        // wind_iterative.F90:888-904 (A..O = 0..14)
+        return f.a[14] * x.s0 + f.a[6] * x.s1 + f.a[12] * x.s2 + f.a[10] * x.w0 + f.a[2] * x.c0 + f.a[9] * x.e0
+               + f.a[4] * x.w1 + f.a[0] * x.c1 + f.a[3] * x.e1 + f.a[8] * x.w2 + f.a[1] * x.c2 + f.a[7] * x.e2
+               + f.a[13] * x.n0 + f.a[5] * x.n1 + f.a[11] * x.n2;
+    }
+    __device__ __forceinline__ T dinv(int k, int mz, bool lat, const Cf &f, const Bc &) const
+    {
+        if (lat || k == 0 || k == mz - 1) return T(1);
+        return T(1) / clamp_d<T>(f.a[0]);
+    }
 };
 
 __device__ __forceinline__ int logical_block(int nb)
 {
     // Workgroups are dispatched round-robin over the 8 XCDs; give XCD x the
-    // contiguous logical range [start(x), start(x) + count(x)).
+    // contiguous logical range [start(x), start(x) + count(x)) so neighbouring
+    // tiles share an L2.
     int b = blockIdx.x;
     int x = b & 7, r = b >> 3;
     int q = nb >> 3, rem = nb & 7;
     return x * q + (x < rem ? x : rem) + r;
-}
-
-template <typename T>
-__device__ __forceinline__ T clamp_d(T d)
-{
-    if (fabs((double)d) < kBreakdownEps) d = d < T(0) ? T(-kBreakdownEps) : T(kBreakdownEps);
-    return d;
 }
 
 __device__ __forceinline__ double wave_butterfly(double v)
@@ -97,13 +187,15 @@ __device__ __forceinline__ double ld_coherent(const double *p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Canonical block reduction of NV per-thread values; the last block to finish
-// folds all partials and thread 0 of it calls fin(total). Returns nothing.
+// Canonical reduction of NV per-thread values over a block of kThreads: 64-lane
+// xor butterfly per wave, waves summed in order; block partials are folded by
+// the last block to finish (lane t sums partials t, t+kThreads, ... from 0),
+// reduced the same way, and thread 0 of that block calls fin(total).
 template <int NV, class Fin>
-__device__ __forceinline__ void reduce_finalize(double (&acc)[NV], double *partials,
-                                                unsigned *counter, int L, int nb, Fin fin)
+__device__ __forceinline__ void reduce_finalize(double (&acc)[NV], double *partials, unsigned *counter, int L,
+                                                int nb, Fin fin)
 {
-    __shared__ double red[4][NV];
+    __shared__ double red[kWaves][NV];
     __shared__ int is_last;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
@@ -114,8 +206,11 @@ __device__ __forceinline__ void reduce_finalize(double (&acc)[NV], double *parti
     __syncthreads();
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v)
-            partials[(long)L * NV + v] = ((red[0][v] + red[1][v]) + red[2][v]) + red[3][v];
+        for (int v = 0; v < NV; ++v) {
+            double s = red[0][v];
+            for (int q = 1; q < kWaves; ++q) s = s + red[q][v];
+            partials[(long)L * NV + v] = s;
+        }
         __threadfence();
         unsigned old = atomicAdd(counter, 1u);
         is_last = (old == (unsigned)(nb - 1));
@@ -126,7 +221,7 @@ __device__ __forceinline__ void reduce_finalize(double (&acc)[NV], double *parti
     double fold[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) fold[v] = 0.0;
-    for (int q = threadIdx.x; q < nb; q += kBlock) {
+    for (int q = threadIdx.x; q < nb; q += kThreads) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) fold[v] = fold[v] + ld_coherent(&partials[(long)q * NV + v]);
     }
@@ -140,172 +235,230 @@ __device__ __forceinline__ void reduce_finalize(double (&acc)[NV], double *parti
     if (threadIdx.x == 0) {
         double tot[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) tot[v] = ((red[0][v] + red[1][v]) + red[2][v]) + red[3][v];
+        for (int v = 0; v < NV; ++v) {
+            double s = red[0][v];
+            for (int q = 1; q < kWaves; ++q) s = s + red[q][v];
+            tot[v] = s;
+        }
         __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         fin(tot);
     }
 }
 
+// Position of this thread in its tile.
+struct TilePos {
+    int L, lane, w, i, j;
+    bool valid, inner, lat;
+    long c, cw, ce, cs, cn;
+};
+
+__device__ __forceinline__ TilePos tile_pos(const Geo &g)
+{
+    TilePos t;
+    t.L = logical_block(g.nt);
+    const int ti = t.L % g.nti, tj = t.L / g.nti;
+    t.lane = threadIdx.x & 63;
+    t.w = threadIdx.x >> 6;
+    t.i = kTileI * ti + t.lane - 1;
+    t.j = kTileJ * tj + t.w - 1;
+    t.valid = t.i >= 0 && t.i < g.mx && t.j >= 0 && t.j < g.my;
+    t.inner = t.valid && t.lane >= 1 && t.lane <= kTileI && t.w >= 1 && t.w <= kTileJ;
+    t.lat = t.valid && (t.i == 0 || t.i == g.mx - 1 || t.j == 0 || t.j == g.my - 1);
+    t.c = t.valid ? (long)t.j * g.mx + t.i : 0;
+    const bool clampn = !t.valid || t.lat;
+    t.cw = clampn ? t.c : t.c - 1;
+    t.ce = clampn ? t.c : t.c + 1;
+    t.cs = clampn ? t.c : t.c - g.mx;
+    t.cn = clampn ? t.c : t.c + g.mx;
+    return t;
+}
+
+template <typename T>
+__device__ __forceinline__ void load_level(const T *__restrict__ x, long o, const TilePos &t, T &c, T &w, T &e,
+                                           T &s, T &n)
+{
+    c = x[o + t.c]; w = x[o + t.cw]; e = x[o + t.ce]; s = x[o + t.cs]; n = x[o + t.cn];
+}
+
 // ------------------------------------------------------------------ the march
-// Visits the rows of column c bottom to top and calls epi(k, n, y, xc, dinv)
-// with y = (A x)(row), xc = x(row) and dinv = 1/diag(A)(row) (extract_diagonal,
-// wind_iterative.F90:1041-1075) when NEED_DINV.
-template <typename T, bool NEED_DINV, class Epi>
-__device__ __forceinline__ void march_analytic(const OpA<T> &op, const Geo &g, const T *__restrict__ x,
-                                               long c, int i, int j, Epi &epi)
+// Visits the rows of the thread's column bottom to top and calls
+// epi(k, n, y, xc, dinv) with y = (A x)(row), xc = x(row) and dinv = D^-1(row)
+// when NEED_DINV. x is read from global memory; the loads of level k+2 are
+// issued before level k is evaluated.
+template <class Op, bool NEED_DINV, class Epi>
+__device__ __forceinline__ void march(const Op &op, const Geo &g, const typename Op::type *__restrict__ x,
+                                      const TilePos &t, Epi &epi)
 {
+    using T = typename Op::type;
     const long P = g.P;
-    const bool lat = (i == 0) | (i == g.mx - 1) | (j == 0) | (j == g.my - 1);
-    const long cw = lat ? c : c - 1, ce = lat ? c : c + 1;
-    const long cs = lat ? c : c - g.mx, cn = lat ? c : c + g.mx;
-    T c0 = x[c], w0 = x[cw], e0 = x[ce], s0 = x[cs], n0 = x[cn];
-    T c1 = x[P + c], w1 = x[P + cw], e1 = x[P + ce], s1 = x[P + cs], n1 = x[P + cn];
-    {   // bottom BC row, wind_iterative.F90:962-988
-        const T bx = op.bcx[c], by = op.bcy[c], bz = op.bcz[c];
-        T y = by * s0 + by * s1 + bx * w0 + (-bz) * c0 + (-bx) * e0 + bx * w1 + bz * c1
-              + (-bx) * e1 + (-by) * n0 + (-by) * n1;
-        T dv = T(1);
-        if (lat) y = c0;
-        if (NEED_DINV) {
-            T d = lat ? T(1) : clamp_d<T>(-bz);
-            dv = T(1) / d;
-        }
-        epi(0, c, y, c0, dv);
-    }
-    const T inv = op.inv, neg4 = op.neg4;
-    for (int k = 1; k <= g.mz - 2; ++k) {
-        const long o2 = (long)(k + 1) * P;
-        const T c2 = x[o2 + c], w2 = x[o2 + cw], e2 = x[o2 + ce], s2 = x[o2 + cs], n2 = x[o2 + cn];
-        const long n = (long)k * P + c;
-        const T B = op.B[n], Cc = op.C[n], H = op.H[n], I = op.I[n], L = op.L[n], M = op.M[n];
-        const T a = (neg4 - B) - Cc;
-        // interior 15-point row, wind_iterative.F90:1005-1020
-        T y = (-M) * s0 + inv * s1 + M * s2 + (-I) * w0 + Cc * c0 + (-H) * e0 + inv * w1 + a * c1
-              + inv * e1 + I * w2 + B * c2 + H * e2 + (-L) * n0 + inv * n1 + L * n2;
-        T dv = T(1);
-        if (lat) y = c1;
-        if (NEED_DINV) {
-            T d = lat ? T(1) : clamp_d<T>(a);
-            dv = T(1) / d;
-        }
-        epi(k, n, y, c1, dv);
-        c0 = c1; w0 = w1; e0 = e1; s0 = s1; n0 = n1;
-        c1 = c2; w1 = w2; e1 = e2; s1 = s2; n1 = n2;
-    }
-    {   // top BC row, wind_iterative.F90:950-960
-        const long n = (long)(g.mz - 1) * P + c;
-        const T dzt = op.dztop[c];
-        T y = (c1 - c0) / dzt;
-        T dv = T(1);
-        if (lat) y = c1;
-        if (NEED_DINV) {
-            T d = lat ? T(1) : clamp_d<T>(T(1) / dzt);
-            dv = T(1) / d;
-        }
-        epi(g.mz - 1, n, y, c1, dv);
+    const int mz = g.mz;
+    Win<T> win;
+    load_level(x, 0, t, win.c1, win.w1, win.e1, win.s1, win.n1);
+    load_level(x, P, t, win.c2, win.w2, win.e2, win.s2, win.n2);
+    const typename Op::Bc bc = op.bc(t.c);
+    typename Op::Cf cf = op.load(P + t.c);  // level 1; the ghost planes use the BC planes only
+    for (int k = 0; k < mz; ++k) {
+        T nc, nw, ne, ns, nn;
+        if (k + 2 <= mz - 1) load_level(x, (long)(k + 2) * P, t, nc, nw, ne, ns, nn);
+        typename Op::Cf cn;
+        if (k >= 1 && k + 1 <= mz - 2) cn = op.load((long)(k + 1) * P + t.c);
+        const T yv = op.row(k, mz, t.lat, win, cf, bc);
+        const T dv = NEED_DINV ? op.dinv(k, mz, t.lat, cf, bc) : T(1);
+        epi(k, (long)k * P + t.c, yv, win.c1, dv);
+        win.shift();
+        if (k + 2 <= mz - 1) { win.c2 = nc; win.w2 = nw; win.e2 = ne; win.s2 = ns; win.n2 = nn; }
+        if (k >= 1 && k + 1 <= mz - 2) cf = cn;
     }
 }
 
-// Probed operator: lateral and ghost-plane identity rows, 15 stored arrays.
-template <typename T, bool NEED_DINV, class Epi>
-__device__ __forceinline__ void march_probed(const OpP<T> &op, const Geo &g, const T *__restrict__ x,
-                                             long c, int i, int j, Epi &epi)
+// Barrier for the LDS ring: waits for this wave's LDS writes only, so global
+// loads issued for later levels stay in flight across it (a full
+// __syncthreads() would drain them at every level).
+__device__ __forceinline__ void lds_barrier()
 {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Two stages in one pass: stage 1 is a Richardson sweep out1 = y + D^-1 (in - A y)
+// over every column of the tile (halo included), staged plane by plane in an
+// LDS ring; stage 2 evaluates (A out1)(row) one level behind on the inner
+// columns from that ring and hands it to epi2. Each coefficient is read from
+// HBM once for both stages. Loads run one level ahead of their use: x two
+// levels, coefficients and `in` one level, the epilogue operands before the
+// barrier of their step.
+template <class Op, class Epi2>
+__device__ __forceinline__ void fused_sweep_matvec(const Op &op, const Geo &g, const TilePos &t,
+                                                   const typename Op::type *__restrict__ y,
+                                                   const typename Op::type *__restrict__ in,
+                                                   typename Op::type *__restrict__ out1, Epi2 &epi2)
+{
+    using T = typename Op::type;
+    __shared__ T ring[4][kWaves][64];
     const long P = g.P;
-    const bool lat = (i == 0) | (i == g.mx - 1) | (j == 0) | (j == g.my - 1);
-    const long cw = lat ? c : c - 1, ce = lat ? c : c + 1;
-    const long cs = lat ? c : c - g.mx, cn = lat ? c : c + g.mx;
-    T c0 = x[c], w0 = x[cw], e0 = x[ce], s0 = x[cs], n0 = x[cn];
-    T c1 = x[P + c], w1 = x[P + cw], e1 = x[P + ce], s1 = x[P + cs], n1 = x[P + cn];
-    epi(0, c, c0, c0, T(1));
-    for (int k = 1; k <= g.mz - 2; ++k) {
-        const long o2 = (long)(k + 1) * P;
-        const T c2 = x[o2 + c], w2 = x[o2 + cw], e2 = x[o2 + ce], s2 = x[o2 + cs], n2 = x[o2 + cn];
-        const long n = (long)k * P + c;
-        // wind_iterative.F90:888-904
-        T y = op.cf[14][n] * s0 + op.cf[6][n] * s1 + op.cf[12][n] * s2 + op.cf[10][n] * w0
-              + op.cf[2][n] * c0 + op.cf[9][n] * e0 + op.cf[4][n] * w1 + op.cf[0][n] * c1
-              + op.cf[3][n] * e1 + op.cf[8][n] * w2 + op.cf[1][n] * c2 + op.cf[7][n] * e2
-              + op.cf[13][n] * n0 + op.cf[5][n] * n1 + op.cf[11][n] * n2;
-        T dv = T(1);
-        if (lat) y = c1;
-        if (NEED_DINV) {
-            T d = lat ? T(1) : clamp_d<T>(op.cf[0][n]);
-            dv = T(1) / d;
-        }
-        epi(k, n, y, c1, dv);
-        c0 = c1; w0 = w1; e0 = e1; s0 = s1; n0 = n1;
-        c1 = c2; w1 = w2; e1 = e2; s1 = s2; n1 = n2;
+    const int mz = g.mz;
+    Win<T> win = {};
+    typename Op::Bc bc = {};
+    typename Op::Cf cf_cur = {}, cf_nxt = {}, cf_prev = {};
+    T in_nxt = T(0);
+    if (t.valid) {
+        load_level(y, 0, t, win.c1, win.w1, win.e1, win.s1, win.n1);
+        load_level(y, P, t, win.c2, win.w2, win.e2, win.s2, win.n2);
+        bc = op.bc(t.c);
+        in_nxt = in[t.c];
     }
-    epi(g.mz - 1, (long)(g.mz - 1) * P + c, c1, c1, T(1));
+    for (int m = 0; m <= mz; ++m) {
+        T nc = T(0), nw = T(0), ne = T(0), ns = T(0), nn = T(0), in_cur = T(0);
+        if (m < mz && t.valid) {
+            cf_cur = cf_nxt;
+            in_cur = in_nxt;
+            if (m + 2 <= mz - 1) load_level(y, (long)(m + 2) * P, t, nc, nw, ne, ns, nn);
+            if (m + 1 >= 1 && m + 1 <= mz - 2) cf_nxt = op.load((long)(m + 1) * P + t.c);
+            if (m + 1 <= mz - 1) in_nxt = in[(long)(m + 1) * P + t.c];
+        }
+        if (m >= 1 && t.inner) epi2.prefetch((long)(m - 1) * P + t.c);
+        if (m < mz) {
+            T v1 = T(0);
+            if (t.valid) {
+                const long n = (long)m * P + t.c;
+                const T ay = op.row(m, mz, t.lat, win, cf_cur, bc);
+                const T dv = op.dinv(m, mz, t.lat, cf_cur, bc);
+                v1 = win.c1 + dv * (in_cur - ay);
+                if (t.inner) out1[n] = v1;
+                win.shift();
+                if (m + 2 <= mz - 1) { win.c2 = nc; win.w2 = nw; win.e2 = ne; win.s2 = ns; win.n2 = nn; }
+            }
+            ring[m & 3][t.w][t.lane] = v1;
+        }
+        lds_barrier();
+        if (m >= 1 && t.inner) {
+            const int k = m - 1;
+            const long n = (long)k * P + t.c;
+            Win<T> x2;
+            const int p0 = (k + 3) & 3, p1 = k & 3, p2 = (k + 1) & 3;
+            x2.c1 = ring[p1][t.w][t.lane];
+            x2.w1 = ring[p1][t.w][t.lane - 1]; x2.e1 = ring[p1][t.w][t.lane + 1];
+            x2.s1 = ring[p1][t.w - 1][t.lane]; x2.n1 = ring[p1][t.w + 1][t.lane];
+            if (k >= 1) {
+                x2.c0 = ring[p0][t.w][t.lane];
+                x2.w0 = ring[p0][t.w][t.lane - 1]; x2.e0 = ring[p0][t.w][t.lane + 1];
+                x2.s0 = ring[p0][t.w - 1][t.lane]; x2.n0 = ring[p0][t.w + 1][t.lane];
+            } else {
+                x2.c0 = x2.w0 = x2.e0 = x2.s0 = x2.n0 = T(0);
+            }
+            if (k + 1 <= mz - 1) {
+                x2.c2 = ring[p2][t.w][t.lane];
+                x2.w2 = ring[p2][t.w][t.lane - 1]; x2.e2 = ring[p2][t.w][t.lane + 1];
+                x2.s2 = ring[p2][t.w - 1][t.lane]; x2.n2 = ring[p2][t.w + 1][t.lane];
+            } else {
+                x2.c2 = x2.w2 = x2.e2 = x2.s2 = x2.n2 = T(0);
+            }
+            const T y2 = op.row(k, mz, t.lat, x2, cf_prev, bc);
+            epi2.apply(k, n, y2, x2.c1);
+        }
+        cf_prev = cf_cur;  // stage 2 of the next step evaluates this level
+    }
 }
-
-template <typename T, bool NEED_DINV, class Epi>
-__device__ __forceinline__ void march(const OpA<T> &opa, const OpP<T> &opp, bool probed, const Geo &g,
-                                      const T *__restrict__ x, long c, int i, int j, Epi &epi)
-{
-    if (probed) march_probed<T, NEED_DINV>(opp, g, x, c, i, j, epi);
-    else march_analytic<T, NEED_DINV>(opa, g, x, c, i, j, epi);
-}
-
-#define WINDHIP_COLUMN_PROLOGUE                                   \
-    const int L = logical_block(g.nb);                            \
-    const long c = (long)L * kBlock + threadIdx.x;                \
-    const bool active = c < g.P;                                  \
-    const int i = active ? (int)(c % g.mx) : 0;                   \
-    const int j = active ? (int)(c / g.mx) : 0;
 
 // ------------------------------------------------------------------ kernels
 // y = A x (spmv)
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_spmv(OpA<T> opa, OpP<T> opp, int probed, Geo g,
-                                                 const T *__restrict__ x, T *__restrict__ y)
+template <class Op>
+__global__ __launch_bounds__(kThreads) void k_spmv(Op op, Geo g, const typename Op::type *__restrict__ x,
+                                                   typename Op::type *__restrict__ y)
 {
-    WINDHIP_COLUMN_PROLOGUE
-    if (!active) return;
+    using T = typename Op::type;
+    const TilePos t = tile_pos(g);
+    if (!t.inner) return;
     auto epi = [&](int, long n, T v, T, T) { y[n] = v; };
-    march<T, false>(opa, opp, probed, g, x, c, i, j, epi);
+    march<Op, false>(op, g, x, t, epi);
 }
 
 // One Richardson sweep of apply_precond (wind_iterative.F90:1300-1314):
 // out = y + D^-1 (in - A y), y the current iterate.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_sweep(OpA<T> opa, OpP<T> opp, int probed, Geo g,
-                                                  const KState *st, const T *__restrict__ y,
-                                                  const T *__restrict__ in, T *__restrict__ out)
+template <class Op>
+__global__ __launch_bounds__(kThreads) void k_sweep(Op op, Geo g, const KState *st,
+                                                    const typename Op::type *__restrict__ y,
+                                                    const typename Op::type *__restrict__ in,
+                                                    typename Op::type *__restrict__ out)
 {
+    using T = typename Op::type;
     if (st->done) return;
-    WINDHIP_COLUMN_PROLOGUE
-    if (!active) return;
+    const TilePos t = tile_pos(g);
+    if (!t.inner) return;
     auto epi = [&](int, long n, T ay, T yc, T dv) { out[n] = yc + dv * (in[n] - ay); };
-    march<T, true>(opa, opp, probed, g, y, c, i, j, epi);
+    march<Op, true>(op, g, y, t, epi);
 }
 
 // Set-up of bicgstab_solve (wind_iterative.F90:586-640): t = A x0,
 // r = b - t, r_hat = r, p = r, p_hat(sweep 1) = D^-1 r, rho0 = <r_hat, r>,
 // ||r0||^2, ||b||^2.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_init(OpA<T> opa, OpP<T> opp, int probed, Geo g, KState *st,
-                                                 double *partials, const T *__restrict__ x,
-                                                 const T *__restrict__ b, const T *__restrict__ dinv,
-                                                 T *__restrict__ r, T *__restrict__ rh,
-                                                 T *__restrict__ p, T *__restrict__ ph)
+template <class Op>
+__global__ __launch_bounds__(kThreads) void k_init(Op op, Geo g, KState *st, double *partials,
+                                                   const typename Op::type *__restrict__ x,
+                                                   const typename Op::type *__restrict__ b,
+                                                   const typename Op::type *__restrict__ dinv,
+                                                   typename Op::type *__restrict__ r,
+                                                   typename Op::type *__restrict__ rh,
+                                                   typename Op::type *__restrict__ p,
+                                                   typename Op::type *__restrict__ ph)
 {
-    WINDHIP_COLUMN_PROLOGUE
+    using T = typename Op::type;
+    const TilePos t = tile_pos(g);
     double acc[3] = {0.0, 0.0, 0.0};
-    if (active) {
-        auto epi = [&](int, long n, T t, T, T) {
+    if (t.inner) {
+        auto epi = [&](int, long n, T tv, T, T) {
             const T bv = b[n];
-            const T rv = T(1) * bv + T(-1) * t;
+            const T rv = T(1) * bv + T(-1) * tv;
             r[n] = rv; rh[n] = rv; p[n] = rv;
             ph[n] = dinv[n] * rv;
             acc[0] = acc[0] + (double)rv * (double)rv;
             acc[1] = acc[1] + (double)rv * (double)rv;
             acc[2] = acc[2] + (double)bv * (double)bv;
         };
-        march<T, false>(opa, opp, probed, g, x, c, i, j, epi);
+        march<Op, false>(op, g, x, t, epi);
     }
-    reduce_finalize<3>(acc, partials, &st->cnt[0], L, g.nb, [&](const double *tot) {
+    reduce_finalize<3>(acc, partials, &st->cnt[0], t.L, g.nt, [&](const double *tot) {
         st->rho = tot[0];
         st->rnorm = sqrt(tot[1]);
         st->res0 = st->rnorm;
@@ -320,27 +473,38 @@ __global__ __launch_bounds__(kBlock) void k_init(OpA<T> opa, OpP<T> opp, int pro
     });
 }
 
-// v = A p_hat and sigma = <r_hat, v> (wind_iterative.F90:657-688).
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_sigma(OpA<T> opa, OpP<T> opp, int probed, Geo g, KState *st,
-                                                  double *partials, const T *__restrict__ ph,
-                                                  const T *__restrict__ rh, T *__restrict__ v)
+// Last preconditioner sweep on p fused with v = A p_hat and sigma = <r_hat, v>
+// (wind_iterative.F90:649-688).
+template <class Op>
+__global__ __launch_bounds__(kThreads) void k_sweep_sigma(Op op, Geo g, KState *st, double *partials,
+                                                          const typename Op::type *__restrict__ ph_in,
+                                                          const typename Op::type *__restrict__ p,
+                                                          typename Op::type *__restrict__ ph,
+                                                          const typename Op::type *__restrict__ rh,
+                                                          typename Op::type *__restrict__ v)
 {
+    using T = typename Op::type;
     if (st->done) {
         // clear the update mode left by the iteration that finished the solve
         if (blockIdx.x == 0 && threadIdx.x == 0) st->fmode = 0;
         return;
     }
-    WINDHIP_COLUMN_PROLOGUE
-    double acc[1] = {0.0};
-    if (active) {
-        auto epi = [&](int, long n, T y, T, T) {
-            v[n] = y;
-            acc[0] = acc[0] + (double)rh[n] * (double)y;
-        };
-        march<T, false>(opa, opp, probed, g, ph, c, i, j, epi);
-    }
-    reduce_finalize<1>(acc, partials, &st->cnt[1], L, g.nb, [&](const double *tot) {
+    const TilePos t = tile_pos(g);
+    struct Epi {
+        const T *rh;
+        T *v;
+        double acc[1];
+        T rhv;
+        __device__ void prefetch(long n) { rhv = rh[n]; }
+        __device__ void apply(int, long n, T yv, T)
+        {
+            v[n] = yv;
+            acc[0] = acc[0] + (double)rhv * (double)yv;
+        }
+    } epi{rh, v, {0.0}, T(0)};
+    fused_sweep_matvec(op, g, t, ph_in, p, ph, epi);
+    double (&acc)[1] = epi.acc;
+    reduce_finalize<1>(acc, partials, &st->cnt[1], t.L, g.nt, [&](const double *tot) {
         const int it = st->iter + 1;
         st->iter = it;
         st->sigma = tot[0];
@@ -365,30 +529,39 @@ __global__ __launch_bounds__(kBlock) void k_update_s(long N, const KState *st, c
     }
 }
 
-// t = A s_hat and the five fused dots <t,s>, <t,t>, <r_hat,s>, <r_hat,t>, <s,s>
-// (wind_iterative.F90:703-785 up to the update of x).
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_omega(OpA<T> opa, OpP<T> opp, int probed, Geo g, KState *st,
-                                                  double *partials, const T *__restrict__ sh,
-                                                  const T *__restrict__ s, const T *__restrict__ rh,
-                                                  T *__restrict__ t)
+// Last sweep on s fused with t = A s_hat and the five dots <t,s>, <t,t>,
+// <r_hat,s>, <r_hat,t>, <s,s> (wind_iterative.F90:695-785 up to the update of x).
+template <class Op>
+__global__ __launch_bounds__(kThreads) void k_sweep_omega(Op op, Geo g, KState *st, double *partials,
+                                                          const typename Op::type *__restrict__ sh_in,
+                                                          const typename Op::type *__restrict__ s,
+                                                          typename Op::type *__restrict__ sh,
+                                                          const typename Op::type *__restrict__ rh,
+                                                          typename Op::type *__restrict__ tt_out)
 {
+    using T = typename Op::type;
     if (st->done) return;
-    WINDHIP_COLUMN_PROLOGUE
-    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (active) {
-        auto epi = [&](int, long n, T tv, T, T) {
-            t[n] = tv;
-            const double td = tv, sd = s[n], rd = rh[n];
+    const TilePos t = tile_pos(g);
+    struct Epi {
+        const T *s, *rh;
+        T *out;
+        double acc[5];
+        T sv, rv;
+        __device__ void prefetch(long n) { sv = s[n]; rv = rh[n]; }
+        __device__ void apply(int, long n, T tv, T)
+        {
+            out[n] = tv;
+            const double td = tv, sd = sv, rd = rv;
             acc[0] = acc[0] + td * sd;
             acc[1] = acc[1] + td * td;
             acc[2] = acc[2] + rd * sd;
             acc[3] = acc[3] + rd * td;
             acc[4] = acc[4] + sd * sd;
-        };
-        march<T, false>(opa, opp, probed, g, sh, c, i, j, epi);
-    }
-    reduce_finalize<5>(acc, partials, &st->cnt[2], L, g.nb, [&](const double *tot) {
+        }
+    } epi{s, rh, tt_out, {0.0, 0.0, 0.0, 0.0, 0.0}, T(0), T(0)};
+    fused_sweep_matvec(op, g, t, sh_in, s, sh, epi);
+    double (&acc)[5] = epi.acc;
+    reduce_finalize<5>(acc, partials, &st->cnt[2], t.L, g.nt, [&](const double *tot) {
         const double ts = tot[0], tt = tot[1], rs = tot[2], rt = tot[3], ss = tot[4];
         st->ts = ts; st->tt = tt; st->rs = rs; st->rt = rt; st->ss = ss;
         if (tt < kBreakdownEps) { st->status = 2; st->done = 1; st->fmode = 0; return; }
@@ -437,60 +610,49 @@ __global__ __launch_bounds__(kBlock) void k_update_xrp(long N, const KState *st,
 }
 
 // ----------------------------------------------- FP64 refinement around FP32
-// The FP32 BiCGStab runs on the residual equation of the FP64 iterate x64:
-// its own iterate y32 is flushed into x64 (k_ir_accumulate) and its recursive
-// residual r32 is replaced by the true residual b - A x64, computed here in
-// FP64, whenever the recursive norm has fallen by the refinement factor
-// (a reliable update). `restart` also resets the shadow residual and the search
-// direction (r_hat = p = r), which a breakdown or a finished inner solve needs;
-// otherwise p and r_hat are kept and rho = <r_hat, r> is recomputed.
-// Sums: 0 ||r64||^2, 1 ||b||^2, 2 <r32, r32>, 3 <r_hat32, r32>.
-__global__ __launch_bounds__(kBlock) void k_ir_resid(OpA<double> opa, OpP<double> opp, int probed, Geo g,
-                                                     KState *st_out, KState *st_in, double *partials, int restart,
-                                                     const double *__restrict__ x, const double *__restrict__ b,
-                                                     const float *__restrict__ dinv32, float *__restrict__ r32,
-                                                     float *__restrict__ rh32, float *__restrict__ p32,
-                                                     float *__restrict__ ph32)
+// r = b - A x in FP64 (true residual), ||r||^2 and ||b||^2; the next FP32
+// inner solve of A d = r starts from d = 0: r32 = r_hat32 = p32 = float(r),
+// p_hat32(sweep 1) = D32^-1 r32, rho0 = <r32, r32> in FP64, target
+// inner_tol * ||r32||.
+template <class Op>
+__global__ __launch_bounds__(kThreads) void k_ir_resid(Op op, Geo g, KState *st_out, KState *st_in,
+                                                       double *partials, const double *__restrict__ x,
+                                                       const double *__restrict__ b,
+                                                       const float *__restrict__ dinv32, float *__restrict__ r32,
+                                                       float *__restrict__ rh32, float *__restrict__ p32,
+                                                       float *__restrict__ ph32)
 {
-    WINDHIP_COLUMN_PROLOGUE
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    if (active) {
-        auto epi = [&](int, long n, double t, double, double) {
+    const TilePos t = tile_pos(g);
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (t.inner) {
+        auto epi = [&](int, long n, double tv, double, double) {
             const double bv = b[n];
-            const double rv = bv - t;
+            const double rv = bv - tv;
             const float rf = (float)rv;
-            r32[n] = rf;
-            float rhv;
-            if (restart) {
-                rh32[n] = rf; p32[n] = rf;
-                ph32[n] = dinv32[n] * rf;
-                rhv = rf;
-            } else {
-                rhv = rh32[n];
-            }
+            r32[n] = rf; rh32[n] = rf; p32[n] = rf;
+            ph32[n] = dinv32[n] * rf;
             acc[0] = acc[0] + rv * rv;
             acc[1] = acc[1] + bv * bv;
             acc[2] = acc[2] + (double)rf * (double)rf;
-            acc[3] = acc[3] + (double)rhv * (double)rf;
         };
-        march<double, false>(opa, opp, probed, g, x, c, i, j, epi);
+        march<Op, false>(op, g, x, t, epi);
     }
-    reduce_finalize<4>(acc, partials, &st_out->cnt[3], L, g.nb, [&](const double *tot) {
+    reduce_finalize<3>(acc, partials, &st_out->cnt[3], t.L, g.nt, [&](const double *tot) {
         st_out->outer_rnorm = sqrt(tot[0]);
         st_out->outer_bnorm = sqrt(tot[1]);
         st_out->outer_target = fmax(st_out->tol_abs, st_out->tol_rel * st_out->outer_bnorm);
         st_out->outer_done = st_out->outer_rnorm <= st_out->outer_target;
-        st_in->rho = tot[3];
+        st_in->rho = tot[2];
         st_in->rnorm = sqrt(tot[2]);
         st_in->res0 = st_in->rnorm;
-        st_in->bnorm = st_out->outer_bnorm;
-        // the recursion aims a little below the FP64 target so that the true
-        // residual passes at the check that follows
-        st_in->target = 0.5 * st_out->outer_target;
-        if (restart) { st_in->omega = 1.0; st_in->alpha = 1.0; }
+        st_in->bnorm = st_in->rnorm;
+        st_in->target = fmax(1.0e-300, st_in->inner_tol * st_in->rnorm);
+        st_in->iter = 0;
+        st_in->omega = 1.0;
+        st_in->alpha = 1.0;
         st_in->fmode = 0;
-        if (st_out->outer_done) { st_in->status = 0; st_in->done = 1; }
-        else { st_in->status = 1; st_in->done = (st_in->iter >= st_in->max_iters); }
+        if (st_out->outer_done || st_in->rnorm <= st_in->target) { st_in->status = 0; st_in->done = 1; }
+        else { st_in->status = 1; st_in->done = 0; }
     });
 }
 
@@ -504,23 +666,23 @@ __global__ __launch_bounds__(kBlock) void k_ir_accumulate(long N, double *__rest
 }
 
 // ||b - A x||^2 in FP64 (true residual), stored as st->outer_rnorm.
-__global__ __launch_bounds__(kBlock) void k_true_resid(OpA<double> opa, OpP<double> opp, int probed, Geo g,
-                                                       KState *st, double *partials,
+template <class Op>
+__global__ __launch_bounds__(kThreads) void k_true_resid(Op op, Geo g, KState *st, double *partials,
                                                        const double *__restrict__ x,
                                                        const double *__restrict__ b)
 {
-    WINDHIP_COLUMN_PROLOGUE
+    const TilePos t = tile_pos(g);
     double acc[2] = {0.0, 0.0};
-    if (active) {
-        auto epi = [&](int, long n, double t, double, double) {
+    if (t.inner) {
+        auto epi = [&](int, long n, double tv, double, double) {
             const double bv = b[n];
-            const double rv = bv - t;
+            const double rv = bv - tv;
             acc[0] = acc[0] + rv * rv;
             acc[1] = acc[1] + bv * bv;
         };
-        march<double, false>(opa, opp, probed, g, x, c, i, j, epi);
+        march<Op, false>(op, g, x, t, epi);
     }
-    reduce_finalize<2>(acc, partials, &st->cnt[3], L, g.nb, [&](const double *tot) {
+    reduce_finalize<2>(acc, partials, &st->cnt[3], t.L, g.nt, [&](const double *tot) {
         st->outer_rnorm = sqrt(tot[0]);
         st->outer_bnorm = sqrt(tot[1]);
     });

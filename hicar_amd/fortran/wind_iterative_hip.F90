@@ -1,0 +1,189 @@
+!>----------------------------------------------------------------------------
+!! Drop-in for HICAR's `wind_iterative` module (src/physics/wind_iterative.F90)
+!! on one MI355X, through the C-ABI of libwindhip.so.
+!!
+!! Same public operations and argument meaning as the reference module
+!! (:39-41): init_iter_winds, calc_iter_winds, finalize_iter_winds, and the
+!! probing entry point used by calibrate_projection_operator. The reference
+!! takes `type(domain_t)`; this module takes `wind_tile_t`, which holds exactly
+!! the domain fields the solve reads (pointers into the caller's arrays, no
+!! copies). A HICAR adapter fills it from domain_t (INTEGRATION.md shows how).
+!!
+!! Fields are HICAR's `real` (FP32), host memory, Fortran (i,k,j) with the
+!! tile's memory bounds; the library promotes them to FP64 on the GPU.
+!! Not thread-safe (module state per nest, as in the reference, :134-181).
+!!----------------------------------------------------------------------------
+module wind_iterative_hip
+    use iso_c_binding
+    use windhip_interface
+    implicit none
+    private
+
+    public :: wind_tile_t, wind_solver_options_t
+    public :: init_iter_winds, calc_iter_winds, finalize_iter_winds, probe_operator, last_solve_info
+    public :: calc_divergence
+
+    integer, parameter :: MAX_NESTS = 4   ! wind_iterative.F90:139
+
+    !> The parts of domain_t the wind solve reads (one tile, one GPU).
+    type :: wind_tile_t
+        integer :: nest_indx = 1
+        integer :: nx, ny, nz
+        real    :: dx
+        integer :: ims, ime, kms, kme, jms, jme
+        ! metrics (ims:ime[+1], kms:kme, jms:jme[+1]) and terrain (ims:ime, jms:jme)
+        real, pointer, contiguous :: dzdx(:,:,:) => null(), dzdy(:,:,:) => null()
+        real, pointer, contiguous :: jacobian(:,:,:) => null(), jacobian_u(:,:,:) => null()
+        real, pointer, contiguous :: jacobian_v(:,:,:) => null(), jacobian_w(:,:,:) => null()
+        real, pointer, contiguous :: dzdx_u(:,:,:) => null(), dzdy_v(:,:,:) => null()
+        real, pointer, contiguous :: advection_dz(:,:,:) => null(), terrain(:,:) => null()
+        real, pointer, contiguous :: density(:,:,:) => null()
+        ! the forcing-tendency winds the solve corrects in place (u%dqdt_3d, ...)
+        real, pointer, contiguous :: u(:,:,:) => null(), v(:,:,:) => null(), w(:,:,:) => null()
+    end type wind_tile_t
+
+    !> options%wind%wind_solver_iterations and the solver choices of this build.
+    type :: wind_solver_options_t
+        integer :: wind_solver_iterations = 1500     ! namelist_utilities.F90:3871-3876
+        real(c_double) :: tol_rel = 1.0e-5_c_double   ! wind_iterative.F90:60
+        real(c_double) :: tol_abs = 1.0e-10_c_double  ! wind_iterative.F90:59
+        integer :: precision = WINDHIP_PLAIN_F64
+        logical :: verbose = .false.
+    end type wind_solver_options_t
+
+    type :: nest_slot_t
+        type(c_ptr) :: handle = c_null_ptr
+        type(windhip_solve_opts) :: opts
+        logical :: verbose = .false.
+    end type nest_slot_t
+
+    type(nest_slot_t), save :: slots(MAX_NESTS)
+    type(windhip_solve_info), save :: info_last
+
+contains
+
+    !> init_iter_winds(domain, options) (wind_iterative.F90:215-291): creates the
+    !! nest's device state on first use and uploads the terrain metrics
+    !! (init_module_vars, :2274-2421); later calls for a known nest are O(1).
+    subroutine init_iter_winds(tile, options)
+        type(wind_tile_t), intent(in) :: tile
+        type(wind_solver_options_t), intent(in) :: options
+        type(windhip_desc) :: d
+        integer(c_int) :: rc
+        integer :: n
+
+        n = tile%nest_indx
+        if (n < 1 .or. n > MAX_NESTS) error stop "wind_iterative_hip: nest index out of range"
+        if (c_associated(slots(n)%handle)) return
+
+        d%nx = tile%nx; d%ny = tile%ny; d%nz = tile%nz
+        d%dx = real(tile%dx, c_double)
+        d%ims = tile%ims; d%ime = tile%ime; d%kms = tile%kms; d%kme = tile%kme
+        d%jms = tile%jms; d%jme = tile%jme
+        d%field_dtype = WINDHIP_F32
+        d%field_mem = WINDHIP_HOST
+        rc = windhip_create(d, slots(n)%handle)
+        call check(rc, "windhip_create")
+        rc = windhip_set_metrics(slots(n)%handle, c_loc(tile%dzdx), c_loc(tile%dzdy), c_loc(tile%jacobian), &
+                                 c_loc(tile%jacobian_u), c_loc(tile%jacobian_v), c_loc(tile%jacobian_w), &
+                                 c_loc(tile%dzdx_u), c_loc(tile%dzdy_v), c_loc(tile%advection_dz), &
+                                 c_loc(tile%terrain), c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr)
+        call check(rc, "windhip_set_metrics")
+        call windhip_default_opts(slots(n)%opts)
+        slots(n)%opts%max_iters = options%wind_solver_iterations
+        slots(n)%opts%tol_rel = options%tol_rel
+        slots(n)%opts%tol_abs = options%tol_abs
+        slots(n)%opts%precision = options%precision
+        slots(n)%verbose = options%verbose
+    end subroutine init_iter_winds
+
+    !> calc_iter_winds(domain, alpha_in, div_in, adv_den) (wind_iterative.F90:295-532):
+    !! solve for lambda and correct tile%u, tile%v, tile%w in place.
+    subroutine calc_iter_winds(tile, alpha_in, div_in, adv_den)
+        type(wind_tile_t), intent(inout) :: tile
+        real, dimension(tile%ims:tile%ime, tile%kms:tile%kme, tile%jms:tile%jme), intent(in), target :: &
+            alpha_in, div_in
+        logical, intent(in) :: adv_den
+        integer(c_int) :: rc, ad
+        type(c_ptr) :: rho, h
+
+        h = slots(tile%nest_indx)%handle
+        if (.not. c_associated(h)) error stop "calc_iter_winds: init_iter_winds has not been called"
+        ad = merge(1_c_int, 0_c_int, adv_den)
+        rho = c_null_ptr
+        if (adv_den .and. associated(tile%density)) rho = c_loc(tile%density)
+        rc = windhip_calc_iter_winds(h, c_loc(alpha_in), c_loc(div_in), c_loc(tile%u), c_loc(tile%v), &
+                                     c_loc(tile%w), rho, ad, slots(tile%nest_indx)%opts, info_last)
+        if (slots(tile%nest_indx)%verbose) then
+            write(*,*) ' windhip BiCGStab status=', info_last%status, ' iterations=', info_last%iters, &
+                       ' precond_n_sweeps=', info_last%nsweeps
+            write(*,*) '   Residual at iter 0:    ', info_last%res0
+            write(*,*) '   Residual at final iter:', info_last%res_final
+        end if
+        if (rc == WINDHIP_ERR_DIVERGED) then
+            ! the reference stops the model here (wind_iterative.F90:506-528)
+            write(*,*) '---------------windhip BiCGStab DIVERGED: ', windhip_error_message()
+            error stop
+        end if
+        call check(rc, "windhip_calc_iter_winds")
+    end subroutine calc_iter_winds
+
+    !> calibrate_projection_operator (wind.F90:1010-1099) on the device: switch the
+    !! nest to the probed operator A = 2 D(G(.)).
+    subroutine probe_operator(tile, alpha_in, adv_den, max_leak)
+        type(wind_tile_t), intent(inout) :: tile
+        real, dimension(tile%ims:tile%ime, tile%kms:tile%kme, tile%jms:tile%jme), intent(in), target :: alpha_in
+        logical, intent(in) :: adv_den
+        real(c_double), intent(out) :: max_leak
+        integer(c_int) :: rc
+        type(c_ptr) :: rho
+        rho = c_null_ptr
+        if (adv_den .and. associated(tile%density)) rho = c_loc(tile%density)
+        rc = windhip_probe_operator(slots(tile%nest_indx)%handle, c_loc(alpha_in), rho, &
+                                    merge(1_c_int, 0_c_int, adv_den), max_leak)
+        call check(rc, "windhip_probe_operator")
+    end subroutine probe_operator
+
+    !> calc_divergence(div, domain, advect_density, horz_only, use_dqdt)
+    !! (wind.F90:250-523) of the tile's u, v, w on the device.
+    subroutine calc_divergence(tile, div, advect_density, horz_only, use_dqdt)
+        type(wind_tile_t), intent(in) :: tile
+        real, dimension(tile%ims:tile%ime, tile%kms:tile%kme, tile%jms:tile%jme), intent(out), target :: div
+        logical, intent(in) :: advect_density, horz_only, use_dqdt
+        integer(c_int) :: rc
+        type(c_ptr) :: rho
+        rho = c_null_ptr
+        if (advect_density .and. associated(tile%density)) rho = c_loc(tile%density)
+        rc = windhip_divergence(slots(tile%nest_indx)%handle, c_loc(tile%u), c_loc(tile%v), c_loc(tile%w), rho, &
+                                merge(1_c_int, 0_c_int, advect_density), merge(1_c_int, 0_c_int, horz_only), &
+                                merge(1_c_int, 0_c_int, use_dqdt), c_loc(div))
+        call check(rc, "windhip_divergence")
+    end subroutine calc_divergence
+
+    !> finalize_iter_winds (wind_iterative.F90:2142-2267): release every nest.
+    subroutine finalize_iter_winds()
+        integer :: n
+        integer(c_int) :: rc
+        do n = 1, MAX_NESTS
+            if (c_associated(slots(n)%handle)) then
+                rc = windhip_destroy(slots(n)%handle)
+                slots(n)%handle = c_null_ptr
+            end if
+        end do
+    end subroutine finalize_iter_winds
+
+    function last_solve_info() result(info)
+        type(windhip_solve_info) :: info
+        info = info_last
+    end function last_solve_info
+
+    subroutine check(rc, what)
+        integer(c_int), intent(in) :: rc
+        character(len=*), intent(in) :: what
+        if (rc /= WINDHIP_OK) then
+            write(*,*) 'wind_iterative_hip: ', what, ' failed (', rc, '): ', windhip_error_message()
+            error stop
+        end if
+    end subroutine check
+
+end module wind_iterative_hip

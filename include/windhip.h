@@ -67,13 +67,12 @@ typedef struct windhip_desc {
 typedef struct windhip_solve_opts {
     double tol_rel;        /* bicg_tol_rel (reference 1e-5, wind_iterative.F90:60) */
     double tol_abs;        /* bicg_tol_abs (1e-10, :59) */
-    int max_iters;         /* wind_solver_iterations (1500, :57); MIXED: total FP32 iterations */
+    int max_iters;         /* wind_solver_iterations (1500, :57); MIXED: cap on all FP32 iterations */
     int start;             /* WINDHIP_START_* */
     int precision;         /* WINDHIP_PLAIN_F64 or WINDHIP_MIXED */
-    double inner_tol;      /* MIXED: fall of the FP32 recursive residual that triggers a
-                              refinement step (FP64 true-residual replacement), 0 < . < 1 */
-    int inner_max_iters;   /* reserved */
-    int max_outer;         /* reserved */
+    double inner_tol;      /* MIXED: relative reduction asked of each FP32 inner solve (0.1) */
+    int inner_max_iters;   /* MIXED: cap per inner solve */
+    int max_outer;         /* MIXED: cap on refinement steps */
     int iters_per_sync;    /* iterations launched between host convergence checks; 0 = auto */
     int use_graph;         /* replay the iteration batch as a captured hipGraph */
     int retry;             /* apply the reference's precond retry ladder (:474-504) */
@@ -84,7 +83,7 @@ typedef struct windhip_solve_info {
     int status;            /* 0 converged, 1 iteration cap, 2 breakdown (bicgstab_solve) */
     int iters;             /* BiCGStab iterations (MIXED: total inner iterations) */
     int nsweeps;           /* preconditioner sweeps of the last solve */
-    int outer_iters;       /* MIXED: refinement steps (FP64 residual replacements) */
+    int outer_iters;       /* MIXED: refinement steps */
     double res0;           /* ||r0|| (recurrence norm of the reference, absolute) */
     double res_final;      /* final recurrence residual norm (MIXED: true residual) */
     double bnorm;          /* ||b|| */
@@ -167,8 +166,9 @@ int windhip_apply_operator(windhip_solver *s, int precision, const double *x_row
 int windhip_get_rows(windhip_solver *s, int which, double *out_rows);
 
 /* Per-kernel device time accumulated by solves run with opts.profile: arrays of
- * 16 (calls, total ms) indexed by kernel kind: 0..4 FP64 sweep, sigma (v = A p),
- * s-update, omega (t = A s + dots), x/r/p-update; 5..9 the same in FP32;
+ * 16 (calls, total ms) indexed by kernel kind: 0..4 FP64 lone sweep (3rd and 4th
+ * sweeps of the retry ladder), last sweep + v = A p + sigma, s-update, last sweep
+ * + t = A s + 5 dots, x/r/p-update; 5..9 the same in FP32;
  * 10 set-up, 11 FP64 refinement residual, 12 refinement accumulate,
  * 13 true residual. reset != 0 clears the counters after reading. */
 int windhip_profile(windhip_solver *s, int reset, double *calls, double *ms);

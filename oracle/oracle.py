@@ -65,7 +65,7 @@ def _load():
         "orc_correct": (None, [v, _dp, _dp, _dp, _dp, _dp, _dp, i]),
         "orc_divergence": (None, [v, _dp, _dp, _dp, _dp, _dp, _dp, _dp, i, i, i, _dp]),
         "orc_probe": (d, [v, _dp, _dp, _dp, _dp, _dp, _dp, i]),
-        "orc_canon_reduce": (d, [_dp, C.c_long]),
+        "orc_canon_reduce": (d, [_dp, i, i]),
         "orc_dot": (d, [v, _dp, _dp]),
         "orc_num_threads": (i, []),
     }
@@ -89,8 +89,10 @@ def _f64(a):
 
 
 def canon_reduce(colsum: np.ndarray) -> float:
-    cs = _f64(colsum.ravel())
-    return _load().orc_canon_reduce(_p(cs), cs.size)
+    """Canonical sum of per-column values given as a (my, mx) array."""
+    cs = _f64(colsum)
+    my, mx = cs.shape
+    return _load().orc_canon_reduce(_p(cs), mx, my)
 
 
 def num_threads() -> int:

@@ -78,14 +78,21 @@ typedef struct orc {
 static double *zalloc(long n) { return (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double)); }
 
 /* ------------------------------------------------------ canonical reduction
- * The order every dot product is summed in (shared with the HIP kernels):
- *  1. per column c = j*mx + i: acc = 0; acc = acc + f(k) for k = 0..mz-1;
- *  2. columns are grouped in blocks of 256 = four waves of 64 consecutive c;
- *     a wave is reduced by the xor butterfly v[l] = v[l] + v[l^off],
- *     off = 32,16,8,4,2,1, and gives v[0]; block = ((w0 + w1) + w2) + w3;
- *  3. the NB block partials are folded by 256 lanes, lane t summing
- *     q = t, t+256, ... in order from 0, then reduced like one block.
+ * The order every dot product is summed in (shared with the HIP kernels).
+ * Columns c = j*mx + i are grouped in tiles: tile (ti,tj) has 8 waves w of 64
+ * lanes l at column i = 62 ti + l - 1, j = 6 tj + w - 1; lanes 1..62 of waves
+ * 1..6 that fall on the grid are the tile's own (inner) columns, all other
+ * lanes hold 0.
+ *  1. per column: acc = 0; acc = acc + f(k) for k = 0..mz-1;
+ *  2. per wave: xor butterfly v[l] = v[l] + v[l^off], off = 32,16,8,4,2,1,
+ *     giving v[0]; per tile: waves summed in order from wave 0;
+ *  3. tiles t = tj*nti + ti are folded by 512 lanes, lane q summing tiles
+ *     q, q+512, ... in order from 0, then reduced like one tile.
  */
+#define CANON_WAVES 8
+#define CANON_TI 62
+#define CANON_TJ (CANON_WAVES - 2)
+
 static void butterfly64(double *v)
 {
     double tmp[64];
@@ -95,38 +102,42 @@ static void butterfly64(double *v)
     }
 }
 
-static double block256(const double *vals /* 256 */)
+static double block_of_waves(const double *vals /* CANON_WAVES*64 */)
 {
-    double w[4];
-    for (int q = 0; q < 4; ++q) {
+    double s = 0.0;
+    for (int w = 0; w < CANON_WAVES; ++w) {
         double v[64];
-        memcpy(v, vals + 64 * q, sizeof(v));
+        memcpy(v, vals + 64 * w, sizeof(v));
         butterfly64(v);
-        w[q] = v[0];
+        s = (w == 0) ? v[0] : s + v[0];
     }
-    return ((w[0] + w[1]) + w[2]) + w[3];
+    return s;
 }
 
-double orc_canon_reduce(const double *colsum, long P)
+double orc_canon_reduce(const double *colsum, int mx, int my)
 {
-    long NB = (P + 255) / 256;
-    double *part = zalloc(NB);
-    for (long b = 0; b < NB; ++b) {
-        double vals[256];
-        for (int t = 0; t < 256; ++t) {
-            long c = 256 * b + t;
-            vals[t] = c < P ? colsum[c] : 0.0;
-        }
-        part[b] = block256(vals);
+    const int nti = (mx + CANON_TI - 1) / CANON_TI, ntj = (my + CANON_TJ - 1) / CANON_TJ;
+    const long nt = (long)nti * ntj;
+    double *part = zalloc(nt);
+    for (long t = 0; t < nt; ++t) {
+        const int ti = (int)(t % nti), tj = (int)(t / nti);
+        double vals[CANON_WAVES * 64];
+        for (int w = 0; w < CANON_WAVES; ++w)
+            for (int l = 0; l < 64; ++l) {
+                const int i = CANON_TI * ti + l - 1, j = CANON_TJ * tj + w - 1;
+                const int inner = l >= 1 && l <= CANON_TI && w >= 1 && w <= CANON_TJ && i < mx && j < my;
+                vals[64 * w + l] = inner ? colsum[(long)j * mx + i] : 0.0;
+            }
+        part[t] = block_of_waves(vals);
     }
-    double lanes[256];
-    for (int t = 0; t < 256; ++t) {
+    double lanes[CANON_WAVES * 64];
+    for (int q = 0; q < CANON_WAVES * 64; ++q) {
         double acc = 0.0;
-        for (long q = t; q < NB; q += 256) acc = acc + part[q];
-        lanes[t] = acc;
+        for (long t = q; t < nt; t += CANON_WAVES * 64) acc = acc + part[t];
+        lanes[q] = acc;
     }
     free(part);
-    return block256(lanes);
+    return block_of_waves(lanes);
 }
 
 /* colsum of a*b over the rows of one column; k ascending from 0 */
@@ -149,7 +160,7 @@ double orc_dot(const orc_t *o, const double *a, const double *b)
 {
     double *cs = zalloc(o->P);
     colsums_dot(o, a, b, cs);
-    double s = orc_canon_reduce(cs, o->P);
+    double s = orc_canon_reduce(cs, o->mx, o->my);
     free(cs);
     return s;
 }
@@ -525,9 +536,9 @@ int orc_bicgstab(orc_t *o, int max_iters, double tol_rel, double tol_abs,
                     css[c] = css[c] + sv * sv;
                 }
         }
-        double ts = orc_canon_reduce(cts, o->P), tt = orc_canon_reduce(ctt, o->P);
-        double rs = orc_canon_reduce(crs, o->P), rt = orc_canon_reduce(crt, o->P);
-        double ss = orc_canon_reduce(css, o->P);
+        double ts = orc_canon_reduce(cts, o->mx, o->my), tt = orc_canon_reduce(ctt, o->mx, o->my);
+        double rs = orc_canon_reduce(crs, o->mx, o->my), rt = orc_canon_reduce(crt, o->mx, o->my);
+        double ss = orc_canon_reduce(css, o->mx, o->my);
         if (tt < breakdown_eps) { *status_out = 2; *iters_out = it; break; }
         omega = ts / tt;
         double rho_new = rs - omega * rt;

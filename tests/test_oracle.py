@@ -98,30 +98,45 @@ def _butterfly_tree(vals):
     return v[0]
 
 
-@pytest.mark.parametrize("P", [1, 255, 256, 1000, 70000])
-def test_canonical_reduce(P):
-    rng = np.random.default_rng(P)
-    x = rng.standard_normal(P)
+def _tile_tree(cs):
+    my, mx = cs.shape
+    nti, ntj = -(-mx // 62), -(-my // 6)
+    parts = []
+    for t in range(nti * ntj):
+        ti, tj = t % nti, t // nti
+        s = None
+        for w in range(8):
+            lanes = []
+            for l in range(64):
+                i, j = 62 * ti + l - 1, 6 * tj + w - 1
+                inner = 1 <= l <= 62 and 1 <= w <= 6 and i < mx and j < my
+                lanes.append(cs[j, i] if inner else 0.0)
+            b = _butterfly_tree(lanes)
+            s = b if s is None else s + b
+        parts.append(s)
+    lanes = []
+    for q in range(512):
+        acc = 0.0
+        for t in range(q, len(parts), 512):
+            acc = acc + parts[t]
+        lanes.append(acc)
+    s = None
+    for w in range(8):
+        b = _butterfly_tree(lanes[64 * w: 64 * w + 64])
+        s = b if s is None else s + b
+    return s
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (3, 5), (7, 70), (13, 130), (300, 257)])
+def test_canonical_reduce(shape):
+    rng = np.random.default_rng(shape[0] * 1000 + shape[1])
+    x = rng.standard_normal(shape)
     got = orc.canon_reduce(x)
-    assert abs(got - math.fsum(x)) <= 1e-13 * np.abs(x).sum()
-    ints = rng.integers(-1000, 1000, size=P).astype(np.float64)  # exact in any order
+    assert abs(got - math.fsum(x.ravel())) <= 1e-13 * np.abs(x).sum()
+    ints = rng.integers(-1000, 1000, size=shape).astype(np.float64)  # exact in any order
     assert orc.canon_reduce(ints) == ints.sum()
-    if P <= 1000:  # the exact tree
-        nb = (P + 255) // 256
-        pad = np.zeros(nb * 256)
-        pad[:P] = x
-        parts = []
-        for b in range(nb):
-            w = [_butterfly_tree(pad[256 * b + 64 * q: 256 * b + 64 * q + 64]) for q in range(4)]
-            parts.append(((w[0] + w[1]) + w[2]) + w[3])
-        lanes = []
-        for t in range(256):
-            acc = 0.0
-            for q in range(t, nb, 256):
-                acc = acc + parts[q]
-            lanes.append(acc)
-        w = [_butterfly_tree(lanes[64 * q: 64 * q + 64]) for q in range(4)]
-        assert got == ((w[0] + w[1]) + w[2]) + w[3]
+    if x.size <= 2000:  # the exact tree
+        assert got == _tile_tree(x)
 
 
 # ---------------------------------------------------------------- assembly
