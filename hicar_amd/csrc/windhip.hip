@@ -62,6 +62,7 @@ struct windhip_solver {
     Dims d{};
     Geo g{};
     long N = 0;
+    long guard = 0;  // elements before and after every row array (mx + 1)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     size_t bytes = 0;
@@ -108,14 +109,19 @@ struct windhip_solver {
     template <typename T>
     int alloc(T **ptr, long count, bool zero = true)
     {
-        size_t nb = sizeof(T) * (size_t)std::max(count, 1L);
+        // Arrays of N rows get `guard` zeroed elements before and after them: the
+        // stencil kernels address the five columns C, W, E, S, N of every lane
+        // from one shifted base (guard = mx + 1), so lateral and out-of-grid
+        // lanes touch at most guard elements beyond either end.
+        const long gd = count == N ? guard : 0;
+        size_t nb = sizeof(T) * (size_t)std::max(count + 2 * gd, 1L);
         void *q = nullptr;
         hipError_t e = hipMalloc(&q, nb);
         if (e != hipSuccess) return fail(WINDHIP_ERR_NOMEM, "hipMalloc(%zu) failed: %s", nb, hipGetErrorString(e));
         allocs.push_back(q);
         bytes += nb;
-        if (zero) HIPCHK(hipMemsetAsync(q, 0, nb, stream));
-        *ptr = (T *)q;
+        if (zero || gd) HIPCHK(hipMemsetAsync(q, 0, nb, stream));
+        *ptr = (T *)q + gd;
         return WINDHIP_OK;
     }
 
@@ -327,6 +333,9 @@ int windhip_solver::init()
     g.ntj = (d.my + kTileJ - 1) / kTileJ;
     g.nt = g.nti * g.ntj;
     N = d.P * d.mz;
+    guard = d.mx + 1;
+    if ((N + 2 * guard) * (long)sizeof(double) >= (1L << 32))
+        return fail(WINDHIP_ERR_ARG, "windhip_create: %ld rows exceed the 4 GiB span of one buffer resource", N);
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ev0));
     HIPCHK(hipEventCreate(&ev1));

@@ -10,25 +10,29 @@
 //   planes [j][i] (P values) for 2-D fields (surface slopes, BC coefficients,
 //         map factors).
 //
-// Work decomposition of the stencil kernels: one thread per (i,j) column,
-// marching k from the bottom ghost plane to the top one with a three-plane
-// register window of the five columns C, W, E, S, N; 256 consecutive columns
-// per block. Each column's loads are unit-stride across a wavefront.
-// Blocks are remapped so that each XCD processes a contiguous range of
-// columns (the j +- 1 rows of a block then sit in the same XCD's L2).
+// Work decomposition of the stencil kernels: a block is a tile of 8
+// wavefronts x 64 lanes, one (i,j) column per thread; each thread marches its
+// column bottom to top through the levels k. Lanes 1..62 of waves 1..6 are the
+// tile's inner columns, the rest its halo (columns of the neighbouring tiles,
+// recomputed where a fused kernel needs them). Unit-stride loads across a
+// wavefront; blocks are remapped so each XCD processes a contiguous range of
+// tiles (the j +- 1 rows of a tile then sit in the same XCD's L2). The fused
+// sweep + matvec kernels stage the swept plane in an LDS ring and software-
+// pipeline their global loads one level ahead through buffer instructions.
 //
 // Arithmetic contract: every expression rounds exactly as the reference writes
 // it (left-to-right sums, no FMA: the file is compiled with -ffp-contract=off),
 // and every dot product is summed in the canonical order
 //   per column sum over k (from 0) -> 64-lane xor butterfly per wave ->
-//   ((w0 + w1) + w2) + w3 per block -> lane t folds partials t, t+256, ...
-//   -> the same block reduction.
+//   waves summed in order per tile -> lane t of the last block folds tile
+//   partials t, t+512, ... -> the same block reduction.
 // The CPU oracle sums in this same order, so the FP64 solve is bit-identical to
 // it iteration by iteration.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 namespace windhip {
 
@@ -80,6 +84,66 @@ __device__ __forceinline__ T clamp_d(T d)
     return d;
 }
 
+// Element e of the plane that starts at base, addressed as a uniform 64-bit
+// base plus a 32-bit byte offset, which maps onto the saddr + voffset form of
+// the global load/store instructions (no per-lane 64-bit address arithmetic).
+template <typename T>
+__device__ __forceinline__ const T &at(const T *base, unsigned e)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + e * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ T &at(T *base, unsigned e)
+{
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + e * (unsigned)sizeof(T));
+}
+
+// Buffer-resource access for the streaming kernels: one SGPR resource per
+// array, a 32-bit per-lane byte offset (voffset, shared by every array of the
+// same layout) and a uniform 32-bit byte offset of the level (soffset). The
+// host guarantees that every row array spans less than 4 GiB.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+constexpr int kRsrcWord3 = 0x00020000;  // gfx9 family: 32-bit data format, raw (stride 0) buffer
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void *p, long bytes)
+{
+    // base and size are uniform by construction; readfirstlane states it, so a
+    // value the compiler happened to keep in VGPRs cannot turn every buffer
+    // access into a per-lane waterfall loop
+    const unsigned long long a = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+    const int nr = __builtin_amdgcn_readfirstlane((int)(unsigned)bytes);
+    void *base = (void *)(((unsigned long long)hi << 32) | lo);
+    return __builtin_amdgcn_make_buffer_rsrc(base, (short)0, nr, kRsrcWord3);
+}
+// the level offset is uniform by construction; state it (free when the value
+// already sits in an SGPR)
+__device__ __forceinline__ int uni(unsigned so) { return __builtin_amdgcn_readfirstlane((int)so); }
+template <typename T>
+__device__ __forceinline__ T bload(rsrc_t r, unsigned vo, unsigned so);
+template <>
+__device__ __forceinline__ float bload<float>(rsrc_t r, unsigned vo, unsigned so)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, uni(so), 0));
+}
+template <>
+__device__ __forceinline__ double bload<double>(rsrc_t r, unsigned vo, unsigned so)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, uni(so), 0));
+}
+__device__ __forceinline__ void bstore(float v, rsrc_t r, unsigned vo, unsigned so)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)vo, uni(so), 0);
+}
+__device__ __forceinline__ void bstore(double v, rsrc_t r, unsigned vo, unsigned so)
+{
+    using u2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, (int)vo, uni(so), 0);
+}
+
+enum RowClass { kRowBottom, kRowInterior, kRowTop };
+
 // Analytic operator (spmv analytic branch, wind_iterative.F90:912-1025) in
 // compressed form: J=-H, K=-I, N=-L, O=-M, D=E=F=G=1/dx^2, A=-4/dx^2-B-C.
 template <typename T>
@@ -91,45 +155,84 @@ struct OpA {
     struct Cf { T B, C, H, I, L, M; };
     struct Bc { T bx, by, bz, dzt; };
     __device__ __forceinline__ Cf load(long n) const { return Cf{B[n], C[n], H[n], I[n], L[n], M[n]}; }
+    // buffer resources of the six arrays; row = level offset lo (so bytes) + column e (vo bytes)
+    struct Src { rsrc_t B, C, H, I, L, M; };
+    __device__ __forceinline__ Src src(long N) const
+    {
+        const long nb = N * (long)sizeof(T);
+        return Src{make_rsrc(B, nb), make_rsrc(C, nb), make_rsrc(H, nb),
+                   make_rsrc(I, nb), make_rsrc(L, nb), make_rsrc(M, nb)};
+    }
+    __device__ __forceinline__ Cf load_src(const Src &r, long, unsigned so, unsigned, unsigned vo) const
+    {
+        return Cf{bload<T>(r.B, vo, so), bload<T>(r.C, vo, so), bload<T>(r.H, vo, so),
+                  bload<T>(r.I, vo, so), bload<T>(r.L, vo, so), bload<T>(r.M, vo, so)};
+    }
     __device__ __forceinline__ Bc bc(long c) const { return Bc{bcx[c], bcy[c], bcz[c], dztop[c]}; }
-    // (A x)(row) at level k of a column. FP64 evaluates the reference's sum
-    // term by term in its order (bit-identical to the oracle). FP32 evaluates
-    // the same operator in difference form: every row of A sums to zero
-    // (A = -4/dx^2 - B - C, J = -H, ...), so A x is a combination of
+    // (A x)(row) of a row of class RC (kRowBottom: ghost plane k = 0,
+    // kRowInterior, kRowTop: ghost plane k = mz-1). FP64 evaluates the
+    // reference's sum term by term in its order (bit-identical to the oracle).
+    // FP32 evaluates the same operator in difference form: every row of A sums
+    // to zero (A = -4/dx^2 - B - C, J = -H, ...), so A x is a combination of
     // neighbour differences, whose rounding is relative to the differences
     // rather than to |x|; the smooth, slowly converging modes keep their
     // accuracy in FP32.
-    __device__ __forceinline__ T row(int k, int mz, bool lat, const Win<T> &x, const Cf &f, const Bc &b) const
+    template <int RC>
+    __device__ __forceinline__ T row_c(bool lat, const Win<T> &x, const Cf &f, const Bc &b) const
     {
         if (lat) return x.c1;
-        if (sizeof(T) == 8) {
-            if (k == 0)  // bottom BC row, wind_iterative.F90:962-988
+        if constexpr (sizeof(T) == 8) {
+            if constexpr (RC == kRowBottom)  // bottom BC row, wind_iterative.F90:962-988
                 return b.by * x.s1 + b.by * x.s2 + b.bx * x.w1 + (-b.bz) * x.c1 + (-b.bx) * x.e1 + b.bx * x.w2
                        + b.bz * x.c2 + (-b.bx) * x.e2 + (-b.by) * x.n1 + (-b.by) * x.n2;
-            if (k == mz - 1) return (x.c1 - x.c0) / b.dzt;  // top BC row, :950-960
-            const T a = (neg4 - f.B) - f.C;
-            // interior 15-point row, :1005-1020
-            return (-f.M) * x.s0 + inv * x.s1 + f.M * x.s2 + (-f.I) * x.w0 + f.C * x.c0 + (-f.H) * x.e0
-                   + inv * x.w1 + a * x.c1 + inv * x.e1 + f.I * x.w2 + f.B * x.c2 + f.H * x.e2 + (-f.L) * x.n0
-                   + inv * x.n1 + f.L * x.n2;
+            else if constexpr (RC == kRowTop)
+                return (x.c1 - x.c0) / b.dzt;  // top BC row, :950-960
+            else {
+                const T a = (neg4 - f.B) - f.C;
+                // interior 15-point row, :1005-1020
+                return (-f.M) * x.s0 + inv * x.s1 + f.M * x.s2 + (-f.I) * x.w0 + f.C * x.c0 + (-f.H) * x.e0
+                       + inv * x.w1 + a * x.c1 + inv * x.e1 + f.I * x.w2 + f.B * x.c2 + f.H * x.e2
+                       + (-f.L) * x.n0 + inv * x.n1 + f.L * x.n2;
+            }
+        } else {
+            if constexpr (RC == kRowBottom)
+                return b.bz * (x.c2 - x.c1) + b.bx * ((x.w1 + x.w2) - (x.e1 + x.e2))
+                       + b.by * ((x.s1 + x.s2) - (x.n1 + x.n2));
+            else if constexpr (RC == kRowTop)
+                return (x.c1 - x.c0) / b.dzt;
+            else
+                return inv * (((x.w1 - x.c1) + (x.e1 - x.c1)) + ((x.s1 - x.c1) + (x.n1 - x.c1)))
+                       + f.B * (x.c2 - x.c1) + f.C * (x.c0 - x.c1) + f.H * (x.e2 - x.e0)
+                       + f.I * (x.w2 - x.w0) + f.L * (x.n2 - x.n0) + f.M * (x.s2 - x.s0);
         }
-        if (k == 0)
-            return b.bz * (x.c2 - x.c1) + b.bx * ((x.w1 + x.w2) - (x.e1 + x.e2))
-                   + b.by * ((x.s1 + x.s2) - (x.n1 + x.n2));
-        if (k == mz - 1) return (x.c1 - x.c0) / b.dzt;
-        return inv * (((x.w1 - x.c1) + (x.e1 - x.c1)) + ((x.s1 - x.c1) + (x.n1 - x.c1)))
-               + f.B * (x.c2 - x.c1) + f.C * (x.c0 - x.c1) + f.H * (x.e2 - x.e0) + f.I * (x.w2 - x.w0)
-               + f.L * (x.n2 - x.n0) + f.M * (x.s2 - x.s0);
+    }
+    __device__ __forceinline__ T row(int k, int mz, bool lat, const Win<T> &x, const Cf &f, const Bc &b) const
+    {
+        if (k == 0) return row_c<kRowBottom>(lat, x, f, b);
+        if (k == mz - 1) return row_c<kRowTop>(lat, x, f, b);
+        return row_c<kRowInterior>(lat, x, f, b);
     }
     // 1/diag(A)(row) as extract_diagonal (wind_iterative.F90:1041-1075) computes it
-    __device__ __forceinline__ T dinv(int k, int mz, bool lat, const Cf &f, const Bc &b) const
+    template <int RC>
+    __device__ __forceinline__ T dinv_c(bool lat, const Cf &f, const Bc &b) const
     {
         T d;
         if (lat) d = T(1);
-        else if (k == mz - 1) d = clamp_d<T>(T(1) / b.dzt);
-        else if (k == 0) d = clamp_d<T>(-b.bz);
+        else if constexpr (RC == kRowTop) d = clamp_d<T>(T(1) / b.dzt);
+        else if constexpr (RC == kRowBottom) d = clamp_d<T>(-b.bz);
         else d = clamp_d<T>((neg4 - f.B) - f.C);
         return T(1) / d;
+    }
+    __device__ __forceinline__ T dinv(int k, int mz, bool lat, const Cf &f, const Bc &b) const
+    {
+        if (k == 0) return dinv_c<kRowBottom>(lat, f, b);
+        if (k == mz - 1) return dinv_c<kRowTop>(lat, f, b);
+        return dinv_c<kRowInterior>(lat, f, b);
+    }
+    // one Richardson update y + D^-1 (in - A y) (apply_precond, :1300-1314)
+    __device__ __forceinline__ T relax(T yc, T dv, T inv_, T ay) const
+    {
+        return yc + dv * (inv_ - ay);
     }
 };
 
@@ -148,7 +251,29 @@ struct OpP {
         for (int q = 0; q < 15; ++q) f.a[q] = cf[q][n];
         return f;
     }
+    struct Src {};
+    __device__ __forceinline__ Src src(long) const { return Src{}; }
+    __device__ __forceinline__ Cf load_src(const Src &, long lo, unsigned, unsigned e, unsigned) const
+    {
+        Cf f;
+#pragma unroll
+        for (int q = 0; q < 15; ++q) f.a[q] = at(cf[q] + lo, e);
+        return f;
+    }
     __device__ __forceinline__ Bc bc(long) const { return Bc{}; }
+    template <int RC>
+    __device__ __forceinline__ T row_c(bool lat, const Win<T> &x, const Cf &f, const Bc &bc) const
+    {
+        if constexpr (RC != kRowInterior) return x.c1;
+        else return row(1, 3, lat, x, f, bc);
+    }
+    template <int RC>
+    __device__ __forceinline__ T dinv_c(bool lat, const Cf &f, const Bc &bc) const
+    {
+        if constexpr (RC != kRowInterior) return T(1);
+        else return dinv(1, 3, lat, f, bc);
+    }
+    __device__ __forceinline__ T relax(T yc, T dv, T inv_, T ay) const { return yc + dv * (inv_ - ay); }
     __device__ __forceinline__ T row(int k, int mz, bool lat, const Win<T> &x, const Cf &f, const Bc &) const
     {
         if (lat || k == 0 || k == mz - 1) return x.c1;
@@ -250,6 +375,7 @@ struct TilePos {
     int L, lane, w, i, j;
     bool valid, inner, lat;
     long c, cw, ce, cs, cn;
+    unsigned oc, ow, oe, os, on;  // the same column offsets in 32 bits (P < 2^31)
 };
 
 __device__ __forceinline__ TilePos tile_pos(const Geo &g)
@@ -270,8 +396,11 @@ __device__ __forceinline__ TilePos tile_pos(const Geo &g)
     t.ce = clampn ? t.c : t.c + 1;
     t.cs = clampn ? t.c : t.c - g.mx;
     t.cn = clampn ? t.c : t.c + g.mx;
+    t.oc = (unsigned)t.c; t.ow = (unsigned)t.cw; t.oe = (unsigned)t.ce;
+    t.os = (unsigned)t.cs; t.on = (unsigned)t.cn;
     return t;
 }
+
 
 template <typename T>
 __device__ __forceinline__ void load_level(const T *__restrict__ x, long o, const TilePos &t, T &c, T &w, T &e,
@@ -325,9 +454,56 @@ __device__ __forceinline__ void lds_barrier()
 // over every column of the tile (halo included), staged plane by plane in an
 // LDS ring; stage 2 evaluates (A out1)(row) one level behind on the inner
 // columns from that ring and hands it to epi2. Each coefficient is read from
-// HBM once for both stages. Loads run one level ahead of their use: x two
-// levels, coefficients and `in` one level, the epilogue operands before the
-// barrier of their step.
+// HBM once for both stages.
+//
+// Step m evaluates stage 1 at level m and stage 2 at level m-1, m = 0..mz.
+// Its operands (x at level m+1, coefficients and `in` at level m, epilogue
+// operands of level m-1) form one Slot, loaded one step ahead into one of two
+// register sets that alternate with the step, so no loaded register is copied
+// while its load is in flight. The steps that touch a ghost plane (0, 1, mz-1,
+// mz) are peeled out of the loop; the loop body is the interior stencil only.
+// All global accesses are buffer loads/stores: one resource per array, the
+// lane's column as the 32-bit voffset, the level (and for x the neighbour
+// direction) as the uniform soffset.
+enum StepPos { kFirst, kSecond, kMid, kTop, kLast };
+
+// Register copy the register allocator cannot coalesce away. Used for values
+// taken out of a pipeline slot that outlive the slot's refill: if such a copy
+// were merged with the slot register, the refill would land in another
+// register and be copied back at the loop back-edge while its load is still
+// in flight, which waits for every outstanding load.
+__device__ __forceinline__ float keep(float v)
+{
+    float r;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+__device__ __forceinline__ double keep(double v)
+{
+    double r;
+    asm volatile("v_mov_b64 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+template <typename T, class Cf>
+__device__ __forceinline__ Cf keep_cf(const Cf &c)
+{
+    Cf r;
+    constexpr int n = sizeof(Cf) / sizeof(T);
+    const T *src = reinterpret_cast<const T *>(&c);
+    T *dst = reinterpret_cast<T *>(&r);
+#pragma unroll
+    for (int q = 0; q < n; ++q) dst[q] = keep(src[q]);
+    return r;
+}
+
+template <typename T>
+__device__ __forceinline__ void load_x5(rsrc_t r, unsigned vo, unsigned sS, unsigned sW, unsigned sC, unsigned sE,
+                                        unsigned sN, T &c, T &w, T &e, T &s, T &n)
+{
+    s = bload<T>(r, vo, sS); w = bload<T>(r, vo, sW); c = bload<T>(r, vo, sC);
+    e = bload<T>(r, vo, sE); n = bload<T>(r, vo, sN);
+}
+
 template <class Op, class Epi2>
 __device__ __forceinline__ void fused_sweep_matvec(const Op &op, const Geo &g, const TilePos &t,
                                                    const typename Op::type *__restrict__ y,
@@ -335,69 +511,140 @@ __device__ __forceinline__ void fused_sweep_matvec(const Op &op, const Geo &g, c
                                                    typename Op::type *__restrict__ out1, Epi2 &epi2)
 {
     using T = typename Op::type;
+    using Cf = typename Op::Cf;
+    using Pre = typename Epi2::Pre;
+    struct Slot {
+        T c, w, e, s, n;
+        Cf cf;
+        T in;
+        Pre ep;
+    };
     __shared__ T ring[4][kWaves][64];
-    const long P = g.P;
-    const int mz = g.mz;
-    Win<T> win = {};
-    typename Op::Bc bc = {};
-    typename Op::Cf cf_cur = {}, cf_nxt = {}, cf_prev = {};
-    T in_nxt = T(0);
-    if (t.valid) {
-        load_level(y, 0, t, win.c1, win.w1, win.e1, win.s1, win.n1);
-        load_level(y, P, t, win.c2, win.w2, win.e2, win.s2, win.n2);
-        bc = op.bc(t.c);
-        in_nxt = in[t.c];
-    }
-    for (int m = 0; m <= mz; ++m) {
-        T nc = T(0), nw = T(0), ne = T(0), ns = T(0), nn = T(0), in_cur = T(0);
-        if (m < mz && t.valid) {
-            cf_cur = cf_nxt;
-            in_cur = in_nxt;
-            if (m + 2 <= mz - 1) load_level(y, (long)(m + 2) * P, t, nc, nw, ne, ns, nn);
-            if (m + 1 >= 1 && m + 1 <= mz - 2) cf_nxt = op.load((long)(m + 1) * P + t.c);
-            if (m + 1 <= mz - 1) in_nxt = in[(long)(m + 1) * P + t.c];
+    // grid scalars pinned to SGPRs (they are uniform; see make_rsrc)
+    const int mx = __builtin_amdgcn_readfirstlane(g.mx);
+    const int mz = __builtin_amdgcn_readfirstlane(g.mz);
+    const long P = (long)mx * __builtin_amdgcn_readfirstlane(g.my);
+    const long N = P * mz;
+    constexpr unsigned sz = sizeof(T);
+    const unsigned pb = (unsigned)(P * (long)sz);  // bytes per level
+    const unsigned voc = t.oc * sz;
+    // x is addressed from y - (mx + 1): neighbour d of column c sits at byte
+    // (c + d + mx + 1) * sz; the row arrays carry mx + 1 guard elements on
+    // both sides, so every lane stays inside the allocation.
+    const long gd = mx + 1;
+    const unsigned oS = sz, oW = (unsigned)mx * sz, oC = (unsigned)(mx + 1) * sz,
+                   oE = (unsigned)(mx + 2) * sz, oN = (unsigned)(2 * mx + 1) * sz;
+    const rsrc_t ry = make_rsrc(y - gd, (N + 2 * gd) * (long)sz), rin = make_rsrc(in, N * (long)sz),
+                 rout = make_rsrc(out1, N * (long)sz);
+    const typename Op::Src rcf = op.src(N);
+    epi2.setup(N);
+    // operands of step m; levels are clamped into [0, mz-1]
+    auto fill = [&](Slot &sl, int m, auto pos) {
+        constexpr int ps = decltype(pos)::value;
+        if (ps != kTop && ps != kLast) {
+            const int ly = m + 1 < mz - 1 ? m + 1 : mz - 1;
+            const unsigned so = (unsigned)ly * pb;
+            load_x5(ry, voc, so + oS, so + oW, so + oC, so + oE, so + oN, sl.c, sl.w, sl.e, sl.s, sl.n);
         }
-        if (m >= 1 && t.inner) epi2.prefetch((long)(m - 1) * P + t.c);
-        if (m < mz) {
+        if (ps == kSecond || ps == kMid) {
+            const int lv = m < mz - 2 ? m : mz - 2;
+            sl.cf = op.load_src(rcf, (long)lv * P, (unsigned)lv * pb, t.oc, voc);
+        }
+        if (ps != kLast) {
+            const int lv = m < mz - 1 ? m : mz - 1;
+            sl.in = bload<T>(rin, voc, (unsigned)lv * pb);
+        }
+        if (ps != kFirst) {
+            const int ke = m - 1 < mz - 1 ? m - 1 : mz - 1;
+            sl.ep = epi2.load((unsigned)ke * pb, voc);
+        }
+    };
+    Win<T> win = {};
+    Cf cf_prev = {};
+    // boundary-condition planes of the column (ghost-plane steps only)
+    typename Op::Bc bc = op.bc(t.c);
+    // stage 1 at level m (ps: kFirst bottom row, kMid interior, kTop top row)
+    // then stage 2 at level m-1 (kSecond bottom row, kMid/kTop interior, kLast top row)
+    auto step = [&](const Slot &cur, int m, auto pos) {
+        constexpr int ps = decltype(pos)::value;
+        if (ps != kLast) {
+            if (ps != kFirst) win.shift();
+            win.c2 = keep(cur.c); win.w2 = keep(cur.w); win.e2 = keep(cur.e);
+            win.s2 = keep(cur.s); win.n2 = keep(cur.n);
             T v1 = T(0);
             if (t.valid) {
-                const long n = (long)m * P + t.c;
-                const T ay = op.row(m, mz, t.lat, win, cf_cur, bc);
-                const T dv = op.dinv(m, mz, t.lat, cf_cur, bc);
-                v1 = win.c1 + dv * (in_cur - ay);
-                if (t.inner) out1[n] = v1;
-                win.shift();
-                if (m + 2 <= mz - 1) { win.c2 = nc; win.w2 = nw; win.e2 = ne; win.s2 = ns; win.n2 = nn; }
+                constexpr int rc = ps == kFirst ? kRowBottom : (ps == kTop ? kRowTop : kRowInterior);
+                const T ay = op.template row_c<rc>(t.lat, win, cur.cf, bc);
+                const T dv = op.template dinv_c<rc>(t.lat, cur.cf, bc);
+                v1 = op.relax(win.c1, dv, cur.in, ay);
+                if (t.inner) bstore(v1, rout, voc, (unsigned)m * pb);
             }
             ring[m & 3][t.w][t.lane] = v1;
         }
         lds_barrier();
-        if (m >= 1 && t.inner) {
+        if (ps != kFirst && t.inner) {
             const int k = m - 1;
-            const long n = (long)k * P + t.c;
             Win<T> x2;
             const int p0 = (k + 3) & 3, p1 = k & 3, p2 = (k + 1) & 3;
             x2.c1 = ring[p1][t.w][t.lane];
             x2.w1 = ring[p1][t.w][t.lane - 1]; x2.e1 = ring[p1][t.w][t.lane + 1];
             x2.s1 = ring[p1][t.w - 1][t.lane]; x2.n1 = ring[p1][t.w + 1][t.lane];
-            if (k >= 1) {
+            if (ps != kSecond) {
                 x2.c0 = ring[p0][t.w][t.lane];
                 x2.w0 = ring[p0][t.w][t.lane - 1]; x2.e0 = ring[p0][t.w][t.lane + 1];
                 x2.s0 = ring[p0][t.w - 1][t.lane]; x2.n0 = ring[p0][t.w + 1][t.lane];
             } else {
                 x2.c0 = x2.w0 = x2.e0 = x2.s0 = x2.n0 = T(0);
             }
-            if (k + 1 <= mz - 1) {
+            if (ps != kLast) {
                 x2.c2 = ring[p2][t.w][t.lane];
                 x2.w2 = ring[p2][t.w][t.lane - 1]; x2.e2 = ring[p2][t.w][t.lane + 1];
                 x2.s2 = ring[p2][t.w - 1][t.lane]; x2.n2 = ring[p2][t.w + 1][t.lane];
             } else {
                 x2.c2 = x2.w2 = x2.e2 = x2.s2 = x2.n2 = T(0);
             }
-            const T y2 = op.row(k, mz, t.lat, x2, cf_prev, bc);
-            epi2.apply(k, n, y2, x2.c1);
+            constexpr int rc = ps == kSecond ? kRowBottom : (ps == kLast ? kRowTop : kRowInterior);
+            const T y2 = op.template row_c<rc>(t.lat, x2, cf_prev, bc);
+            epi2.apply(k, (unsigned)k * pb, voc, y2, x2.c1, cur.ep);
         }
-        cf_prev = cf_cur;  // stage 2 of the next step evaluates this level
+        if (ps == kSecond || ps == kMid) cf_prev = keep_cf<T>(cur.cf);  // stage 2 of the next step evaluates this level
+    };
+    using PF = std::integral_constant<int, kFirst>;
+    using PS = std::integral_constant<int, kSecond>;
+    using PM = std::integral_constant<int, kMid>;
+    using PT = std::integral_constant<int, kTop>;
+    using PL = std::integral_constant<int, kLast>;
+    Slot sa, sb;
+    load_x5(ry, voc, oS, oW, oC, oE, oN, win.c1, win.w1, win.e1, win.s1, win.n1);
+    fill(sa, 0, PF{});
+    fill(sb, 1, PS{});
+    step(sa, 0, PF{});
+    // interior steps m = 2..mz-2 in pairs; the set holding step m alternates
+    const int nmid = mz - 3;
+    fill(sa, 2, PM{});
+    step(sb, 1, PS{});
+    int m = 2;
+    // the scheduling barriers keep each set's refill after the last use of its
+    // previous contents, so both live in the same registers
+    for (; m + 1 <= mz - 2; m += 2) {
+        __builtin_amdgcn_sched_barrier(0);
+        fill(sb, m + 1, PM{});
+        step(sa, m, PM{});
+        __builtin_amdgcn_sched_barrier(0);
+        fill(sa, m + 2, PM{});
+        step(sb, m + 1, PM{});
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (nmid & 1) {  // one interior step left: step mz-1's operands go to sb
+        fill(sb, m + 1, PT{});
+        step(sa, m, PM{});
+        fill(sa, mz, PL{});
+        step(sb, mz - 1, PT{});
+        step(sa, mz, PL{});
+    } else {  // step mz-1's operands are in sa
+        fill(sb, mz, PL{});
+        step(sa, mz - 1, PT{});
+        step(sb, mz, PL{});
     }
 }
 
@@ -494,14 +741,16 @@ __global__ __launch_bounds__(kThreads) void k_sweep_sigma(Op op, Geo g, KState *
         const T *rh;
         T *v;
         double acc[1];
-        T rhv;
-        __device__ void prefetch(long n) { rhv = rh[n]; }
-        __device__ void apply(int, long n, T yv, T)
+        rsrc_t rrh, rv;
+        struct Pre { T rh; };
+        __device__ void setup(long N) { rrh = make_rsrc(rh, N * (long)sizeof(T)); rv = make_rsrc(v, N * (long)sizeof(T)); }
+        __device__ Pre load(unsigned so, unsigned vo) const { return Pre{bload<T>(rrh, vo, so)}; }
+        __device__ void apply(int, unsigned so, unsigned vo, T yv, T, const Pre &p)
         {
-            v[n] = yv;
-            acc[0] = acc[0] + (double)rhv * (double)yv;
+            bstore(yv, rv, vo, so);
+            acc[0] = acc[0] + (double)p.rh * (double)yv;
         }
-    } epi{rh, v, {0.0}, T(0)};
+    } epi{rh, v, {0.0}};
     fused_sweep_matvec(op, g, t, ph_in, p, ph, epi);
     double (&acc)[1] = epi.acc;
     reduce_finalize<1>(acc, partials, &st->cnt[1], t.L, g.nt, [&](const double *tot) {
@@ -546,19 +795,25 @@ __global__ __launch_bounds__(kThreads) void k_sweep_omega(Op op, Geo g, KState *
         const T *s, *rh;
         T *out;
         double acc[5];
-        T sv, rv;
-        __device__ void prefetch(long n) { sv = s[n]; rv = rh[n]; }
-        __device__ void apply(int, long n, T tv, T)
+        rsrc_t rs, rrh, rout;
+        struct Pre { T s, rh; };
+        __device__ void setup(long N)
         {
-            out[n] = tv;
-            const double td = tv, sd = sv, rd = rv;
+            const long nb = N * (long)sizeof(T);
+            rs = make_rsrc(s, nb); rrh = make_rsrc(rh, nb); rout = make_rsrc(out, nb);
+        }
+        __device__ Pre load(unsigned so, unsigned vo) const { return Pre{bload<T>(rs, vo, so), bload<T>(rrh, vo, so)}; }
+        __device__ void apply(int, unsigned so, unsigned vo, T tv, T, const Pre &p)
+        {
+            bstore(tv, rout, vo, so);
+            const double td = tv, sd = p.s, rd = p.rh;
             acc[0] = acc[0] + td * sd;
             acc[1] = acc[1] + td * td;
             acc[2] = acc[2] + rd * sd;
             acc[3] = acc[3] + rd * td;
             acc[4] = acc[4] + sd * sd;
         }
-    } epi{s, rh, tt_out, {0.0, 0.0, 0.0, 0.0, 0.0}, T(0), T(0)};
+    } epi{s, rh, tt_out, {0.0, 0.0, 0.0, 0.0, 0.0}};
     fused_sweep_matvec(op, g, t, sh_in, s, sh, epi);
     double (&acc)[5] = epi.acc;
     reduce_finalize<5>(acc, partials, &st->cnt[2], t.L, g.nt, [&](const double *tot) {
