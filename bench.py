@@ -96,7 +96,7 @@ def main():
     ap.add_argument("--nx", type=int, default=512)
     ap.add_argument("--ny", type=int, default=512)
     ap.add_argument("--nz", type=int, default=40)
-    ap.add_argument("--inner-tol", type=float, default=0.1)
+    ap.add_argument("--inner-tol", type=float, default=0.5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=6)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 arm")

@@ -38,7 +38,9 @@ EXPORTS = (
 PROFILE_KINDS = {0: "sweep_f64", 1: "sweep_sigma_f64", 2: "update_s_f64", 3: "sweep_omega_f64",
                  4: "update_xrp_f64", 5: "sweep_f32", 6: "sweep_sigma_f32", 7: "update_s_f32",
                  8: "sweep_omega_f32", 9: "update_xrp_f32",
-                 10: "init_f64", 11: "ir_resid", 12: "ir_accumulate", 13: "true_resid"}
+                 10: "init_f64", 11: "ir_resid", 12: "ir_accumulate", 13: "true_resid",
+                 14: "fg_setup", 15: "fg_matvec", 16: "fg_axpy_dot"}
+PROFILE_SLOTS = 24  # WINDHIP_PROFILE_KINDS
 
 
 class Desc(C.Structure):

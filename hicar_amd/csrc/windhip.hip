@@ -86,6 +86,14 @@ struct windhip_solver {
     double *x, *r, *rh, *p, *ph[3], *v, *s, *sh[3], *t, *b;
     // Krylov FP32 (inner solve of the refinement)
     float *x32 = nullptr, *r32, *rh32, *p32, *ph32[3], *v32, *s32, *sh32[3], *t32;
+    // FGMRES-IR (mixed precision): basis, FP32 directions, Hessenberg columns
+    static constexpr int kMaxRestart = 40;
+    static constexpr int kRestart = 32;  // FGMRES cycle length (directions kept per cycle)
+    int fg_m = 0;
+    double *fg_v[kMaxRestart + 1] = {};
+    float *fg_z[kMaxRestart] = {};
+    double *fg_z64 = nullptr, *fg_part = nullptr, *fg_h = nullptr, *fg_y = nullptr;
+    float **fg_zs = nullptr;
     // reductions and state
     double *partials;
     double *scan_part, *scan_out;
@@ -261,9 +269,10 @@ struct windhip_solver {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
     std::vector<int> ev_kind;
     int ev_next = 0;
-    double prof_ms[16] = {}, prof_calls[16] = {};
+    double prof_ms[WINDHIP_PROFILE_KINDS] = {}, prof_calls[WINDHIP_PROFILE_KINDS] = {};
     int launch_batch(int precision, int count, bool use_graph);
     int alloc_f32_krylov();
+    int alloc_fgmres(int m);
     int correction(void *u, void *v, void *w, const void *density, int adv_den);
     int divergence(const void *u, const void *v, const void *w, const void *density, int adv_den,
                    int horz_only, int use_dqdt, void *div_out);
@@ -469,7 +478,8 @@ int windhip_solver::read_state(int idx)
 // Kernel kinds timed by the profiling mode (hipEvents around each launch on
 // the handle's stream): base + 5 * precision (0 FP64, 1 FP32).
 enum { PK_SWEEP = 0, PK_SIGMA = 1, PK_UPD_S = 2, PK_OMEGA = 3, PK_UPD_XRP = 4, PK_INIT = 10,
-       PK_IR_RESID = 11, PK_IR_ACC = 12, PK_TRUE_RESID = 13, PK_COUNT = 16 };
+       PK_IR_RESID = 11, PK_IR_ACC = 12, PK_TRUE_RESID = 13, PK_FG_SETUP = 14, PK_FG_MATVEC = 15,
+       PK_FG_AXPY_DOT = 16, PK_COUNT = 17 };
 
 template <class Launch>
 int windhip_solver::timed(int kind, Launch &&launch)
@@ -623,25 +633,28 @@ int windhip_solver::true_residual(double *rel)
     return WINDHIP_OK;
 }
 
-// FP32 BiCGStab (matvec, vectors, coefficients in FP32; dots and scalars in
-// FP64) inside FP64 iterative refinement: each refinement step solves
-// A d = r for the current FP64 true residual r = b - A lambda with a fresh FP32
-// BiCGStab from d = 0 down to opts.inner_tol * ||r||, then lambda += d in FP64.
-// The inner solve asks for a modest reduction on purpose: an FP32 recursion on
-// this operator cannot get far below kappa * eps32 of its own starting
-// residual, while a fresh start per step keeps every step well inside that.
+// Mixed precision: FGMRES-IR. Flexible GMRES(m) in FP64 on A lambda = b whose
+// preconditioner is an FP32 BiCGStab solve of A z = v_j to inner_tol (the
+// reference's iteration, wind_iterative.F90:555-791, on the FP32 operator and
+// vectors). The Arnoldi basis, the Hessenberg least squares, the lambda update
+// and the true residual are FP64; FGMRES minimises the FP64 residual over all
+// directions of a cycle, so an inaccurate inner solve slows it down but cannot
+// make the residual grow.
 int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *info)
 {
     CHK(alloc_f32_krylov());
-    HIPCHK(hipMemsetAsync(x32, 0, N * sizeof(float), stream));
-    const double inner_tol = (o.inner_tol > 0 && o.inner_tol < 1) ? o.inner_tol : 0.1;
+    const double inner_tol = (o.inner_tol > 0 && o.inner_tol < 1) ? o.inner_tol : 0.5;
+    const int m = std::max(1, std::min(o.max_outer, kRestart));
+    CHK(alloc_fgmres(m));
+    const int H = m + 2;  // stride of a Hessenberg column in fg_h
+    const int fb = (int)std::min<long>(kFlatBlocks, (N + kThreads - 1) / kThreads);
     k_state_reset<<<1, 1, 0, stream>>>(st, 0, o.tol_rel, o.tol_abs, 0.0);
     k_state_reset<<<1, 1, 0, stream>>>(st + 1, o.inner_max_iters, 0.0, 0.0, inner_tol);
+    double *beta2 = fg_h + (long)m * H;  // ||r||^2 of the cycle's residual
     auto resid = [&]() -> int {
         CHK(with_op<double>([&](auto op) {
             return timed(PK_IR_RESID, [&] {
-                k_ir_resid<decltype(op)><<<g.nt, kThreads, 0, stream>>>(op, g, st, st + 1, partials, x, b, dinv32,
-                                                                        r32, rh32, p32, ph32[0]);
+                k_fg_resid<decltype(op)><<<g.nt, kThreads, 0, stream>>>(op, g, st, partials, x, b, fg_v[0], beta2);
             });
         }));
         HIPCHK(hipGetLastError());
@@ -650,35 +663,136 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
     CHK(resid());
     info->res0 = st_host[0].outer_rnorm;
     info->bnorm = st_host[0].outer_bnorm;
-    int total_inner = 0, outer = 0, status = 0, stalls = 0;
-    double best = st_host[0].outer_rnorm;
-    while (!st_host[0].outer_done && outer < o.max_outer && total_inner < o.max_iters) {
-        ++outer;
-        int done_iters = 0;
-        while (!st_host[1].done) {
-            const int n = std::min(o.iters_per_sync, std::max(1, o.inner_max_iters - done_iters));
-            CHK(launch_batch(1, n, o.use_graph != 0));
-            CHK(read_state(1));
-            done_iters += n;
-            if (done_iters > o.inner_max_iters + o.iters_per_sync) break;
+    int total_inner = 0, steps = 0, status = 0;
+    std::vector<double> Hm((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), y(m), col(H);
+    while (!st_host[0].outer_done && steps < o.max_outer && total_inner < o.max_iters) {
+        std::fill(gv.begin(), gv.end(), 0.0);
+        gv[0] = st_host[0].outer_rnorm;
+        int k = 0;
+        bool breakdown = false;
+        for (int j = 0; j < m && steps < o.max_outer && total_inner < o.max_iters; ++j) {
+            // z_j ~ A^-1 v_j in FP32 (v_j normalised in place first). An inner
+            // solve whose FP32 recurrence overflowed is run again from z = 0 with
+            // half the iterations it had reached, keeping a finite iterate.
+            const double *vsq = j == 0 ? beta2 : fg_h + (long)(j - 1) * H + j;
+            int cap = o.inner_max_iters, spent = 0;
+            for (int attempt = 0;; ++attempt) {
+                if (attempt || j > 0) k_state_reset<<<1, 1, 0, stream>>>(st + 1, cap, 0.0, 0.0, inner_tol);
+                CHK(timed(PK_FG_SETUP, [&] {
+                    k_fg_setup<<<fb, kThreads, 0, stream>>>(N, st + 1, fg_part, vsq, attempt == 0, fg_v[j], dinv32,
+                                                            x32, r32, rh32, p32, ph32[0]);
+                }));
+                HIPCHK(hipGetLastError());
+                CHK(read_state(1));
+                int done_iters = 0;
+                while (!st_host[1].done) {
+                    const int n = std::min(o.iters_per_sync, std::max(1, cap - done_iters));
+                    CHK(launch_batch(1, n, o.use_graph != 0));
+                    CHK(read_state(1));
+                    done_iters += n;
+                    if (done_iters > cap + o.iters_per_sync) break;
+                }
+                spent += st_host[1].iter;
+                const double rn = st_host[1].rnorm;
+                if (rn == rn && rn < 1e300) break;
+                if (verbose)
+                    fprintf(stderr, "[windhip] FP32 inner solve overflowed after %d iterations; rerun with %d\n",
+                            st_host[1].iter, st_host[1].iter / 2);
+                cap = st_host[1].iter / 2;
+                if (cap < 1 || attempt >= 3) { breakdown = true; break; }
+            }
+            if (breakdown) { total_inner += spent; ++steps; break; }
+            total_inner += spent;
+            ++steps;
+            double *hc = fg_h + (long)j * H;
+            k_fg_take<<<flat_grid(N), kBlock, 0, stream>>>(N, x32, fg_z[j], fg_z64);
+            // w = A z_j, orthogonalised against v_0..v_j (modified Gram-Schmidt)
+            CHK(with_op<double>([&](auto op) {
+                return timed(PK_FG_MATVEC, [&] {
+                    k_fg_matvec<decltype(op)><<<g.nt, kThreads, 0, stream>>>(op, g, st, partials, fg_z64, fg_v[0],
+                                                                             fg_v[j + 1], hc);
+                });
+            }));
+            for (int i = 0; i <= j; ++i)
+                CHK(timed(PK_FG_AXPY_DOT, [&] {
+                    k_fg_axpy_dot<<<fb, kThreads, 0, stream>>>(N, st, fg_part, fg_v[j + 1], fg_v[i],
+                                                               i < j ? fg_v[i + 1] : nullptr, hc, i);
+                }));
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(col.data(), hc, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            CHK(harvest());
+            // Hessenberg column j, previous Givens rotations, new rotation
+            for (int i = 0; i <= j; ++i) Hm[(size_t)i * m + j] = col[i];
+            const double hn = sqrt(std::max(col[j + 1], 0.0));
+            for (int i = 0; i < j; ++i) {
+                const double a = Hm[(size_t)i * m + j], c2 = Hm[(size_t)(i + 1) * m + j];
+                Hm[(size_t)i * m + j] = cs[i] * a + sn[i] * c2;
+                Hm[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * c2;
+            }
+            const double a = Hm[(size_t)j * m + j];
+            const double rr = std::hypot(a, hn);
+            cs[j] = rr > 0 ? a / rr : 1.0;
+            sn[j] = rr > 0 ? hn / rr : 0.0;
+            Hm[(size_t)j * m + j] = rr;
+            gv[j + 1] = -sn[j] * gv[j];
+            gv[j] = cs[j] * gv[j];
+            k = j + 1;
+            if (verbose)
+                fprintf(stderr, "[windhip] FGMRES step %d: %d FP32 iterations (status %d), residual estimate %.3e "
+                                "(target %.3e)\n", steps, st_host[1].iter, st_host[1].status, fabs(gv[j + 1]),
+                        st_host[0].outer_target);
+            if (!(rr > 0) || !(hn == hn) || !(rr == rr)) { k = j; breakdown = true; break; }  // drop column j
+            if (fabs(gv[j + 1]) <= st_host[0].outer_target) break;
+            if (hn <= 1e-14 * rr) break;  // happy breakdown: the basis spans the solution
         }
-        total_inner += st_host[1].iter;
-        CHK(timed(PK_IR_ACC, [&] { k_ir_accumulate<<<flat_grid(N), kBlock, 0, stream>>>(N, x, x32); }));
+        if (k == 0) break;
+        // y = R^-1 g, lambda += sum y_q z_q (FP64)
+        for (int i = k - 1; i >= 0; --i) {
+            double acc = gv[i];
+            for (int q = i + 1; q < k; ++q) acc -= Hm[(size_t)i * m + q] * y[q];
+            y[i] = Hm[(size_t)i * m + i] != 0 ? acc / Hm[(size_t)i * m + i] : 0.0;
+        }
+        HIPCHK(hipMemcpyAsync(fg_y, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, stream));
+        CHK(timed(PK_IR_ACC, [&] {
+            k_fg_update<<<flat_grid(N), kBlock, 0, stream>>>(N, x, (const float *const *)fg_zs, fg_y, k);
+        }));
         HIPCHK(hipGetLastError());
-        const int inner_status = st_host[1].status, inner_iters = st_host[1].iter;
+        const double before = st_host[0].outer_rnorm;
         CHK(resid());
-        const double now = st_host[0].outer_rnorm;
         if (verbose)
-            fprintf(stderr, "[windhip] refinement %d: %d FP32 iterations (status %d), true residual %.3e (target %.3e)\n",
-                    outer, inner_iters, inner_status, now, st_host[0].outer_target);
-        if (now < best) { best = now; stalls = 0; }
-        else if (++stalls >= 3 || !(now == now)) { status = 2; break; }
+            fprintf(stderr, "[windhip] FGMRES cycle of %d steps: true residual %.3e -> %.3e\n", k, before,
+                    st_host[0].outer_rnorm);
+        // a cycle that made no progress (or broke down without progress) ends the solve
+        if (!(st_host[0].outer_rnorm < before)) { status = 2; break; }
+        (void)breakdown;
     }
     info->status = st_host[0].outer_done ? 0 : (status ? status : 1);
     info->iters = total_inner;
-    info->outer_iters = outer;
+    info->outer_iters = steps;
     info->res_final = st_host[0].outer_rnorm;
     info->true_rel_res = info->bnorm > 0 ? st_host[0].outer_rnorm / info->bnorm : st_host[0].outer_rnorm;
+    return WINDHIP_OK;
+}
+
+// Arnoldi basis (m + 1 FP64 vectors), m FP32 directions and the small arrays of
+// FGMRES(m); kept for later solves.
+int windhip_solver::alloc_fgmres(int m)
+{
+    if (fg_m >= m) return WINDHIP_OK;
+    for (int q = fg_m + 1; q <= m; ++q) CHK(alloc(&fg_v[q], N));
+    if (fg_m == 0) CHK(alloc(&fg_v[0], N));
+    for (int q = fg_m; q < m; ++q) CHK(alloc(&fg_z[q], N));
+    if (!fg_z64) CHK(alloc(&fg_z64, N));
+    if (!fg_part) {
+        CHK(alloc(&fg_part, (long)kFlatBlocks * 2));
+        CHK(alloc(&fg_h, (long)(kMaxRestart + 1) * (kMaxRestart + 2)));
+        CHK(alloc(&fg_y, kMaxRestart));
+        CHK(alloc(&fg_zs, kMaxRestart));
+    }
+    HIPCHK(hipMemcpyAsync(fg_zs, fg_z, sizeof(float *) * m, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    fg_m = m;
     return WINDHIP_OK;
 }
 
@@ -811,7 +925,7 @@ void windhip_default_opts(windhip_solve_opts *o)
     o->max_iters = 1500;   // wind_solver_max_iters default (:57)
     o->start = WINDHIP_START_WARM;
     o->precision = WINDHIP_PLAIN_F64;
-    o->inner_tol = 0.1;
+    o->inner_tol = 0.5;
     o->inner_max_iters = 2000;
     o->max_outer = 60;
     o->iters_per_sync = 0;
@@ -1130,7 +1244,7 @@ int windhip_get_rows(windhip_solver *s, int which, double *out)
 int windhip_profile(windhip_solver *s, int reset, double *calls, double *ms)
 {
     if (!s) return fail(WINDHIP_ERR_ARG, "null handle");
-    for (int q = 0; q < PK_COUNT; ++q) {
+    for (int q = 0; q < WINDHIP_PROFILE_KINDS; ++q) {
         if (calls) calls[q] = s->prof_calls[q];
         if (ms) ms[q] = s->prof_ms[q];
         if (reset) s->prof_calls[q] = s->prof_ms[q] = 0.0;

@@ -864,62 +864,6 @@ __global__ __launch_bounds__(kBlock) void k_update_xrp(long N, const KState *st,
     }
 }
 
-// ----------------------------------------------- FP64 refinement around FP32
-// r = b - A x in FP64 (true residual), ||r||^2 and ||b||^2; the next FP32
-// inner solve of A d = r starts from d = 0: r32 = r_hat32 = p32 = float(r),
-// p_hat32(sweep 1) = D32^-1 r32, rho0 = <r32, r32> in FP64, target
-// inner_tol * ||r32||.
-template <class Op>
-__global__ __launch_bounds__(kThreads) void k_ir_resid(Op op, Geo g, KState *st_out, KState *st_in,
-                                                       double *partials, const double *__restrict__ x,
-                                                       const double *__restrict__ b,
-                                                       const float *__restrict__ dinv32, float *__restrict__ r32,
-                                                       float *__restrict__ rh32, float *__restrict__ p32,
-                                                       float *__restrict__ ph32)
-{
-    const TilePos t = tile_pos(g);
-    double acc[3] = {0.0, 0.0, 0.0};
-    if (t.inner) {
-        auto epi = [&](int, long n, double tv, double, double) {
-            const double bv = b[n];
-            const double rv = bv - tv;
-            const float rf = (float)rv;
-            r32[n] = rf; rh32[n] = rf; p32[n] = rf;
-            ph32[n] = dinv32[n] * rf;
-            acc[0] = acc[0] + rv * rv;
-            acc[1] = acc[1] + bv * bv;
-            acc[2] = acc[2] + (double)rf * (double)rf;
-        };
-        march<Op, false>(op, g, x, t, epi);
-    }
-    reduce_finalize<3>(acc, partials, &st_out->cnt[3], t.L, g.nt, [&](const double *tot) {
-        st_out->outer_rnorm = sqrt(tot[0]);
-        st_out->outer_bnorm = sqrt(tot[1]);
-        st_out->outer_target = fmax(st_out->tol_abs, st_out->tol_rel * st_out->outer_bnorm);
-        st_out->outer_done = st_out->outer_rnorm <= st_out->outer_target;
-        st_in->rho = tot[2];
-        st_in->rnorm = sqrt(tot[2]);
-        st_in->res0 = st_in->rnorm;
-        st_in->bnorm = st_in->rnorm;
-        st_in->target = fmax(1.0e-300, st_in->inner_tol * st_in->rnorm);
-        st_in->iter = 0;
-        st_in->omega = 1.0;
-        st_in->alpha = 1.0;
-        st_in->fmode = 0;
-        if (st_out->outer_done || st_in->rnorm <= st_in->target) { st_in->status = 0; st_in->done = 1; }
-        else { st_in->status = 1; st_in->done = 0; }
-    });
-}
-
-// x += d (FP64 accumulation of the FP32 correction), d = 0 for the next inner solve.
-__global__ __launch_bounds__(kBlock) void k_ir_accumulate(long N, double *__restrict__ x, float *__restrict__ d)
-{
-    for (long n = (long)blockIdx.x * kBlock + threadIdx.x; n < N; n += (long)gridDim.x * kBlock) {
-        x[n] = x[n] + (double)d[n];
-        d[n] = 0.0f;
-    }
-}
-
 // ||b - A x||^2 in FP64 (true residual), stored as st->outer_rnorm.
 template <class Op>
 __global__ __launch_bounds__(kThreads) void k_true_resid(Op op, Geo g, KState *st, double *partials,
@@ -941,6 +885,139 @@ __global__ __launch_bounds__(kThreads) void k_true_resid(Op op, Geo g, KState *s
         st->outer_rnorm = sqrt(tot[0]);
         st->outer_bnorm = sqrt(tot[1]);
     });
+}
+
+// ------------------------------------------------------------------ FGMRES-IR
+// Outer solve of the mixed-precision path: flexible GMRES in FP64 whose
+// preconditioner is an FP32 BiCGStab solve (GMRES-based iterative refinement).
+// V holds the FP64 Arnoldi basis, Z the FP32 preconditioned directions; the
+// small Hessenberg least-squares problem is solved on the host.
+
+constexpr int kFlatBlocks = 1024;  // grid of the flat (row-parallel) reductions
+
+// r = b - A x (FP64) into v0, ||r||^2 and ||b||^2 -> st (outer_rnorm, outer_bnorm,
+// outer_target, outer_done); h[0] = ||r||^2 for the first basis vector's scale.
+template <class Op>
+__global__ __launch_bounds__(kThreads) void k_fg_resid(Op op, Geo g, KState *st, double *partials,
+                                                       const double *__restrict__ x,
+                                                       const double *__restrict__ b, double *__restrict__ v0,
+                                                       double *h)
+{
+    const TilePos t = tile_pos(g);
+    double acc[2] = {0.0, 0.0};
+    if (t.inner) {
+        auto epi = [&](int, long n, double tv, double, double) {
+            const double bv = b[n];
+            const double rv = bv - tv;
+            v0[n] = rv;
+            acc[0] = acc[0] + rv * rv;
+            acc[1] = acc[1] + bv * bv;
+        };
+        march<Op, false>(op, g, x, t, epi);
+    }
+    reduce_finalize<2>(acc, partials, &st->cnt[3], t.L, g.nt, [&](const double *tot) {
+        st->outer_rnorm = sqrt(tot[0]);
+        st->outer_bnorm = sqrt(tot[1]);
+        st->outer_target = fmax(st->tol_abs, st->tol_rel * st->outer_bnorm);
+        st->outer_done = st->outer_rnorm <= st->outer_target;
+        h[0] = tot[0];
+    });
+}
+
+// Start of one preconditioner application: scale the basis vector v by
+// 1/sqrt(*vsq) in place (FP64) when normalise is set, and set up the FP32
+// BiCGStab for A z = v from z = 0 (bicgstab_solve set-up,
+// wind_iterative.F90:586-640, with r0 = v).
+__global__ __launch_bounds__(kThreads) void k_fg_setup(long N, KState *st_in, double *partials,
+                                                       const double *vsq, int normalise, double *__restrict__ v,
+                                                       const float *__restrict__ dinv32, float *__restrict__ x32,
+                                                       float *__restrict__ r32, float *__restrict__ rh32,
+                                                       float *__restrict__ p32, float *__restrict__ ph32)
+{
+    const double sc = normalise ? 1.0 / sqrt(*vsq) : 1.0;
+    double acc[1] = {0.0};
+    for (long n = (long)blockIdx.x * kThreads + threadIdx.x; n < N; n += (long)gridDim.x * kThreads) {
+        const double vv = v[n] * sc;
+        if (normalise) v[n] = vv;
+        const float rf = (float)vv;
+        x32[n] = 0.0f;
+        r32[n] = rf; rh32[n] = rf; p32[n] = rf;
+        ph32[n] = dinv32[n] * rf;
+        acc[0] = acc[0] + (double)rf * (double)rf;
+    }
+    reduce_finalize<1>(acc, partials, &st_in->cnt[3], blockIdx.x, gridDim.x, [&](const double *tot) {
+        st_in->rho = tot[0];
+        st_in->rnorm = sqrt(tot[0]);
+        st_in->res0 = st_in->rnorm;
+        st_in->bnorm = st_in->rnorm;
+        st_in->target = fmax(1.0e-300, st_in->inner_tol * st_in->rnorm);
+        st_in->iter = 0;
+        st_in->omega = 1.0;
+        st_in->alpha = 1.0;
+        st_in->fmode = 0;
+        if (st_in->rnorm <= st_in->target) { st_in->status = 0; st_in->done = 1; }
+        else { st_in->status = 1; st_in->done = 0; }
+    });
+}
+
+// Keep the inner solution: z (FP32 copy for the final update) and its FP64
+// widening for the FP64 matvec.
+__global__ __launch_bounds__(kBlock) void k_fg_take(long N, const float *__restrict__ x32, float *__restrict__ z,
+                                                    double *__restrict__ z64)
+{
+    for (long n = (long)blockIdx.x * kBlock + threadIdx.x; n < N; n += (long)gridDim.x * kBlock) {
+        const float v = x32[n];
+        z[n] = v;
+        z64[n] = (double)v;
+    }
+}
+
+// w = A z (FP64) and h[0] = <w, v0>.
+template <class Op>
+__global__ __launch_bounds__(kThreads) void k_fg_matvec(Op op, Geo g, KState *st, double *partials,
+                                                        const double *__restrict__ z64,
+                                                        const double *__restrict__ v0, double *__restrict__ w,
+                                                        double *h)
+{
+    const TilePos t = tile_pos(g);
+    double acc[1] = {0.0};
+    if (t.inner) {
+        auto epi = [&](int, long n, double tv, double, double) {
+            w[n] = tv;
+            acc[0] = acc[0] + tv * v0[n];
+        };
+        march<Op, false>(op, g, z64, t, epi);
+    }
+    reduce_finalize<1>(acc, partials, &st->cnt[3], t.L, g.nt, [&](const double *tot) { h[0] = tot[0]; });
+}
+
+// One modified Gram-Schmidt step fused with the next inner product:
+// w -= h[i] v_i, then h[i+1] = <w, u> (u = v_{i+1}; u = nullptr gives ||w||^2).
+__global__ __launch_bounds__(kThreads) void k_fg_axpy_dot(long N, KState *st, double *partials,
+                                                          double *__restrict__ w, const double *__restrict__ vi,
+                                                          const double *__restrict__ u, double *h, int i)
+{
+    const double c = h[i];
+    double acc[1] = {0.0};
+    for (long n = (long)blockIdx.x * kThreads + threadIdx.x; n < N; n += (long)gridDim.x * kThreads) {
+        const double wv = w[n] - c * vi[n];
+        w[n] = wv;
+        const double uv = u ? u[n] : wv;  // u == nullptr: ||w||^2
+        acc[0] = acc[0] + wv * uv;
+    }
+    reduce_finalize<1>(acc, partials, &st->cnt[3], blockIdx.x, gridDim.x,
+                       [&](const double *tot) { h[i + 1] = tot[0]; });
+}
+
+// x += sum_q y[q] z_q (FP64 accumulation of the FP32 directions).
+__global__ __launch_bounds__(kBlock) void k_fg_update(long N, double *__restrict__ x, const float *const *zs,
+                                                      const double *y, int k)
+{
+    for (long n = (long)blockIdx.x * kBlock + threadIdx.x; n < N; n += (long)gridDim.x * kBlock) {
+        double acc = x[n];
+        for (int q = 0; q < k; ++q) acc = acc + y[q] * (double)zs[q][n];
+        x[n] = acc;
+    }
 }
 
 __global__ void k_state_reset(KState *st, int max_iters, double tol_rel, double tol_abs, double inner_tol)

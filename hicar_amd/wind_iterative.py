@@ -200,8 +200,8 @@ class WindSolver:
 
     def profile(self, reset: bool = True) -> dict:
         """Per-kernel (calls, total ms) of the solves run with profile=1."""
-        calls = (C.c_double * 16)()
-        ms = (C.c_double * 16)()
+        calls = (C.c_double * nat.PROFILE_SLOTS)()
+        ms = (C.c_double * nat.PROFILE_SLOTS)()
         nat.check(self.lib.windhip_profile(self.h, int(reset), calls, ms))
         return {name: (calls[k], ms[k]) for k, name in nat.PROFILE_KINDS.items() if calls[k] > 0}
 

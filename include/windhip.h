@@ -70,9 +70,9 @@ typedef struct windhip_solve_opts {
     int max_iters;         /* wind_solver_iterations (1500, :57); MIXED: cap on all FP32 iterations */
     int start;             /* WINDHIP_START_* */
     int precision;         /* WINDHIP_PLAIN_F64 or WINDHIP_MIXED */
-    double inner_tol;      /* MIXED: relative reduction asked of each FP32 inner solve (0.1) */
+    double inner_tol;      /* MIXED: relative reduction asked of each FP32 inner solve (0.5) */
     int inner_max_iters;   /* MIXED: cap per inner solve */
-    int max_outer;         /* MIXED: cap on refinement steps */
+    int max_outer;         /* MIXED: cap on FGMRES steps (= inner solves); cycles restart every 32 */
     int iters_per_sync;    /* iterations launched between host convergence checks; 0 = auto */
     int use_graph;         /* replay the iteration batch as a captured hipGraph */
     int retry;             /* apply the reference's precond retry ladder (:474-504) */
@@ -166,11 +166,14 @@ int windhip_apply_operator(windhip_solver *s, int precision, const double *x_row
 int windhip_get_rows(windhip_solver *s, int which, double *out_rows);
 
 /* Per-kernel device time accumulated by solves run with opts.profile: arrays of
- * 16 (calls, total ms) indexed by kernel kind: 0..4 FP64 lone sweep (3rd and 4th
- * sweeps of the retry ladder), last sweep + v = A p + sigma, s-update, last sweep
- * + t = A s + 5 dots, x/r/p-update; 5..9 the same in FP32;
- * 10 set-up, 11 FP64 refinement residual, 12 refinement accumulate,
- * 13 true residual. reset != 0 clears the counters after reading. */
+ * WINDHIP_PROFILE_KINDS (calls, total ms) indexed by kernel kind: 0..4 FP64 lone
+ * sweep (3rd and 4th sweeps of the retry ladder), last sweep + v = A p + sigma,
+ * s-update, last sweep + t = A s + 5 dots, x/r/p-update; 5..9 the same in FP32;
+ * 10 set-up, 11 FP64 residual of the mixed solve, 12 mixed-solve lambda update,
+ * 13 true residual, 14 FGMRES inner set-up, 15 FGMRES FP64 matvec,
+ * 16 FGMRES Gram-Schmidt step; the rest unused. reset != 0 clears the
+ * counters after reading. */
+#define WINDHIP_PROFILE_KINDS 24
 int windhip_profile(windhip_solver *s, int reset, double *calls, double *ms);
 
 /* Device memory held by the handle (bytes) and the process high-water mark. */
