@@ -33,6 +33,9 @@ EXPORTS = (
     "windhip_divergence", "windhip_get_lambda", "windhip_set_lambda", "windhip_probe_operator",
     "windhip_time_kernel", "windhip_memory", "windhip_set_alpha", "windhip_apply_operator",
     "windhip_get_rows", "windhip_profile",
+    "windhip_net_param_count", "windhip_net_create", "windhip_net_destroy", "windhip_net_last_error",
+    "windhip_net_set_params", "windhip_net_get_params", "windhip_net_forward", "windhip_net_loss_grad",
+    "windhip_net_adam_step", "windhip_net_time_forward", "windhip_set_net", "windhip_learned_input",
 )
 
 PROFILE_KINDS = {0: "sweep_f64", 1: "sweep_sigma_f64", 2: "update_s_f64", 3: "sweep_omega_f64",
@@ -60,7 +63,7 @@ class SolveInfo(C.Structure):
     _fields_ = [("status", C.c_int), ("iters", C.c_int), ("nsweeps", C.c_int), ("outer_iters", C.c_int),
                 ("res0", C.c_double), ("res_final", C.c_double), ("bnorm", C.c_double),
                 ("true_rel_res", C.c_double), ("ms", C.c_double), ("start_used", C.c_int),
-                ("reserved", C.c_int)]
+                ("reserved", C.c_int), ("guess_res", C.c_double), ("plain_res", C.c_double)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -86,6 +89,7 @@ def load():
     vp, i, d = C.c_void_p, C.c_int, C.c_double
     pp = C.POINTER(C.c_void_p)
     dp = C.POINTER(C.c_double)
+    fp = C.POINTER(C.c_float)
     sig = {
         "windhip_default_opts": (None, [C.POINTER(SolveOpts)]),
         "windhip_create": (i, [C.POINTER(Desc), pp]),
@@ -106,6 +110,18 @@ def load():
         "windhip_apply_operator": (i, [vp, i, dp, dp]),
         "windhip_get_rows": (i, [vp, i, dp]),
         "windhip_profile": (i, [vp, i, dp, dp]),
+        "windhip_net_param_count": (C.c_long, []),
+        "windhip_net_create": (i, [pp]),
+        "windhip_net_destroy": (i, [vp]),
+        "windhip_net_last_error": (C.c_char_p, [vp]),
+        "windhip_net_set_params": (i, [vp, fp]),
+        "windhip_net_get_params": (i, [vp, fp]),
+        "windhip_net_forward": (i, [vp, vp, i, i, i, i, vp]),
+        "windhip_net_loss_grad": (i, [vp, vp, vp, i, i, i, i, dp, fp]),
+        "windhip_net_adam_step": (i, [vp, C.c_float, C.c_float, C.c_float, C.c_float]),
+        "windhip_net_time_forward": (i, [vp, i, i, i, i, i, dp]),
+        "windhip_set_net": (i, [vp, vp]),
+        "windhip_learned_input": (i, [vp, vp, C.POINTER(C.c_int), dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -16,6 +16,7 @@
 #include "windhip.h"
 #include "windhip_kernels.hpp"
 #include "windhip_setup.hpp"
+#include "windhip_net.hpp"
 
 using namespace windhip;
 
@@ -108,6 +109,10 @@ struct windhip_solver {
     unsigned long long *leak = nullptr;
     double **cf_dev = nullptr;
 
+    // learned initial guess: attached network, its padded input/output volumes
+    windhip_net *net = nullptr;
+    float *net_in = nullptr, *net_out = nullptr;
+    long net_vox = 0;
     bool metrics_set = false, structure_uploaded = false, probed = false;
     bool verbose = false;  // WINDHIP_VERBOSE=1: per-batch progress on stderr (verbose_solver, :274-275)
     int nsweeps = 2;  // precond_n_sweeps, BASE_PREC_SWEEPS (wind_iterative.F90:76)
@@ -257,7 +262,10 @@ struct windhip_solver {
     int run_plain(int max_iters, double tol_rel, double tol_abs, int batch, bool use_graph, int *status,
                   int *iters, double *res0, double *resf);
     int run_mixed(const windhip_solve_opts &o, windhip_solve_info *info);
-    int true_residual(double *rel);
+    int true_residual(double *rel, const double *xv = nullptr);
+    int net_dims(int *Dp, int *Hp, int *Wp) const;
+    int net_input(float *in, double *scale);
+    int learned_start(double *res_guess, double *res_plain, bool *used);
     template <typename T>
     int enqueue_iterations(int count);
     template <class Launch>
@@ -619,17 +627,74 @@ int windhip_solver::run_plain(int max_iters, double tol_rel, double tol_abs, int
     return WINDHIP_OK;
 }
 
-int windhip_solver::true_residual(double *rel)
+int windhip_solver::true_residual(double *rel, const double *xv)
 {
+    const double *xx = xv ? xv : x;
     CHK(with_op<double>([&](auto op) {
         return timed(PK_TRUE_RESID, [&] {
-            k_true_resid<decltype(op)><<<g.nt, kThreads, 0, stream>>>(op, g, st, partials, x, b);
+            k_true_resid<decltype(op)><<<g.nt, kThreads, 0, stream>>>(op, g, st, partials, xx, b);
         });
     }));
     HIPCHK(hipGetLastError());
     CHK(read_state(0));
     double bn = st_host[0].outer_bnorm;
     *rel = bn > 0 ? st_host[0].outer_rnorm / bn : st_host[0].outer_rnorm;
+    return WINDHIP_OK;
+}
+
+// ------------------------------------------------------------------ learned start
+// Padded network volume of the grid (multiples of 8).
+int windhip_solver::net_dims(int *Dp, int *Hp, int *Wp) const
+{
+    *Dp = (d.nz + 7) / 8 * 8;
+    *Hp = (d.ny + 7) / 8 * 8;
+    *Wp = (d.nx + 7) / 8 * 8;
+    return WINDHIP_OK;
+}
+
+// Network input from the current rhs b, metrics and alpha. The rhs is scaled
+// by 1 / rms(b) over the cells (the solution is linear in b); *scale returns
+// the factor that turns the network output back into lambda, rms(b) dx^2.
+int windhip_solver::net_input(float *in, double *scale)
+{
+    int Dp, Hp, Wp;
+    net_dims(&Dp, &Hp, &Wp);
+    double rel;
+    CHK(true_residual(&rel, nullptr));  // also ||b||
+    const double bn = st_host[0].outer_bnorm;
+    const double rms = bn / sqrt((double)d.nx * d.ny * d.nz);
+    const double bs = rms > 0 ? 1.0 / rms : 0.0;
+    k_net_input<<<flat_grid((long)Hp * Wp), kBlock, 0, stream>>>(d, Dp, Hp, Wp, b, bs, m_dzdx, m_dzdy, m_jaco,
+                                                                 m_alpha, m_adv_dz, in);
+    HIPCHK(hipGetLastError());
+    *scale = rms * d.dx * d.dx;
+    return WINDHIP_OK;
+}
+
+// Start selection: the network's guess and the plain start (the current
+// lambda: the previous solution, or zero) are both evaluated by their FP64
+// true residual; the lower one becomes the starting lambda.
+int windhip_solver::learned_start(double *res_guess, double *res_plain, bool *used)
+{
+    int Dp, Hp, Wp;
+    net_dims(&Dp, &Hp, &Wp);
+    const long vox = (long)Dp * Hp * Wp;
+    if (vox > net_vox) {
+        CHK(alloc(&net_in, vox * wnet::kInCh));
+        CHK(alloc(&net_out, vox));
+        net_vox = vox;
+    }
+    double scale = 0.0, rel = 0.0;
+    CHK(net_input(net_in, &scale));
+    *res_plain = st_host[0].outer_rnorm;  // of the current lambda, from net_input's residual
+    const int rc = wnet_forward_dev(net, stream, net_in, wnet::Shape{1, Dp, Hp, Wp}, net_out);
+    if (rc) return fail(rc, "learned start: network forward failed");
+    k_net_output<<<flat_grid(d.P), kBlock, 0, stream>>>(d, Dp, Hp, Wp, net_out, scale, t);
+    HIPCHK(hipGetLastError());
+    CHK(true_residual(&rel, t));
+    *res_guess = st_host[0].outer_rnorm;
+    *used = *res_guess < *res_plain;
+    if (*used) HIPCHK(hipMemcpyAsync(x, t, N * sizeof(double), hipMemcpyDeviceToDevice, stream));
     return WINDHIP_OK;
 }
 
@@ -1027,8 +1092,8 @@ int windhip_solve(windhip_solver *s, const void *alpha, const void *div, const w
     else windhip_default_opts(&o);
     if (o.max_iters < 0 || o.inner_max_iters < 1 || o.max_outer < 1)
         return fail(WINDHIP_ERR_ARG, "windhip_solve: iteration caps must be positive");
-    if (o.start == WINDHIP_START_LEARNED)
-        return fail(WINDHIP_ERR_ARG, "windhip_solve: the learned start needs windhip_net_* (not loaded)");
+    if (o.start == WINDHIP_START_LEARNED && !s->net)
+        return fail(WINDHIP_ERR_ARG, "windhip_solve: the learned start needs a network (windhip_set_net)");
     windhip_solve_info inf{};
     if (o.iters_per_sync <= 0) {
         long per = 64000000L / std::max(1L, s->N);
@@ -1039,6 +1104,17 @@ int windhip_solve(windhip_solver *s, const void *alpha, const void *div, const w
     CHK(s->prepare_call(alpha, div));
     if (o.start == WINDHIP_START_ZERO) HIPCHK(hipMemsetAsync(s->x, 0, s->N * sizeof(double), s->stream));
     inf.start_used = o.start;
+    if (o.start == WINDHIP_START_LEARNED) {
+        double rg = 0, rp = 0;
+        bool used = false;
+        CHK(s->learned_start(&rg, &rp, &used));
+        inf.start_used = used ? WINDHIP_START_LEARNED : WINDHIP_START_WARM;
+        inf.guess_res = rg;
+        inf.plain_res = rp;
+        if (s->verbose)
+            fprintf(stderr, "[windhip] learned start: residual %.3e vs plain start %.3e -> %s\n", rg, rp,
+                    used ? "network" : "plain");
+    }
     if (o.precision == WINDHIP_MIXED) {
         CHK(s->run_mixed(o, &inf));
         inf.nsweeps = s->nsweeps;
@@ -1238,6 +1314,28 @@ int windhip_get_rows(windhip_solver *s, int which, double *out)
     HIPCHK(hipMemcpyAsync(tmp.data(), src[which], s->N * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     rows_from_dev_order(s, tmp.data(), out);
+    return WINDHIP_OK;
+}
+
+int windhip_set_net(windhip_solver *s, windhip_net *net)
+{
+    if (!s) return WINDHIP_ERR_ARG;
+    s->net = net;
+    return WINDHIP_OK;
+}
+
+int windhip_learned_input(windhip_solver *s, float *in_dev, int *dims_out, double *scale)
+{
+    if (!s || !dims_out) return WINDHIP_ERR_ARG;
+    int Dp, Hp, Wp;
+    s->net_dims(&Dp, &Hp, &Wp);
+    dims_out[0] = Dp; dims_out[1] = Hp; dims_out[2] = Wp;
+    if (!in_dev) return WINDHIP_OK;
+    if (!s->metrics_set) return fail(WINDHIP_ERR_ARG, "windhip_learned_input: metrics not set");
+    double sc = 0.0;
+    CHK(s->net_input(in_dev, &sc));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (scale) *scale = sc;
     return WINDHIP_OK;
 }
 

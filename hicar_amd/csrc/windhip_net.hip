@@ -1,0 +1,808 @@
+// windhip_net.hip -- the learned initial guess: a 3-D convolutional
+// encoder-decoder trained and evaluated with hand-written gfx950 kernels.
+//
+// Architecture (fixed): 4 resolution levels with 16/32/64/64 channels, two
+// 3x3x3 convolutions + ReLU per level, 2x2x2 average pooling down, nearest
+// 2x upsampling + skip concatenation up, a 1x1x1 head to one channel.
+// Activations and gradients are bf16, channels-last ([voxel][channel], every
+// channel count a multiple of 8 so one lane moves 8 channels in 16 bytes);
+// weights are bf16 copies of FP32 master weights; every product accumulates
+// in FP32.
+//
+// Convolutions are implicit GEMMs on v_mfma_f32_16x16x32_bf16:
+//   forward     Y[v][n] = sum_k A[v][k] W[n][k],  A[v][t*C + c] = X[v + off(t)][c]
+//   input grad  the same kernel on the ReLU-masked output gradient with the
+//               tap-flipped, transposed weights
+//   weight grad dW[t][co][ci] = sum_v G[v][co] X[v + off(t)][ci], K = voxels,
+//               staged transposed through LDS, per-slab partials reduced in
+//               a fixed order (deterministic).
+// MFMA operand layout (16x16x32): lane l holds row/column l % 16 and the 8
+// consecutive k of group l / 16 for both A and B; the result lane l holds
+// rows 4 (l / 16) + e, e = 0..3, of column l % 16.
+#include "windhip_net.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "windhip.h"
+
+using bf16 = __bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace wnet {
+
+const ConvSpec kConvs[kNumConv] = {
+    {8, 16, 0},   {16, 16, 0}, {16, 32, 1}, {32, 32, 1}, {32, 64, 2},  {64, 64, 2}, {64, 64, 3},
+    {64, 64, 3},  {128, 64, 2}, {64, 64, 2}, {96, 32, 1}, {32, 32, 1}, {48, 16, 0}, {16, 16, 0}};
+enum { C0A, C0B, C1A, C1B, C2A, C2B, C3A, C3B, D2A, D2B, D1A, D1B, D0A, D0B };
+
+long param_count()
+{
+    long n = 0;
+    for (const auto &c : kConvs) n += (long)c.co * 27 * c.ci + c.co;
+    return n + 16 + 1;
+}
+
+constexpr int kT = 256;  // threads of the conv / elementwise kernels
+
+__device__ __forceinline__ int kpad(int k) { return (k + 31) / 32 * 32; }
+
+// ------------------------------------------------------------------ kernels
+// Implicit-GEMM 3x3x3 convolution, zero padding, stride 1. A wave computes 64
+// voxels x 64 output channels (4 x 4 MFMA tiles); blockIdx.y selects the
+// 64-channel chunk of N.
+__global__ __launch_bounds__(kT) void k_conv3(Shape s, const bf16 *__restrict__ X, int ldx, int C,
+                                             const bf16 *__restrict__ Wt, int Kp, const float *__restrict__ bias,
+                                             int N, bf16 *__restrict__ Y, int ldy, int relu)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, g = lane >> 4;
+    const long M = s.voxels();
+    const long m0 = ((long)blockIdx.x * 4 + wave) * 64;
+    const int n0 = blockIdx.y * 64;
+    if (m0 >= M) return;
+    const int nbl = min(4, (N - n0 + 15) / 16);
+    const long HW = (long)s.H * s.W;
+    int cz[4], cy[4], cx[4];
+    long cv[4];
+    bool ok[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const long v = m0 + 16 * r + r16;
+        ok[r] = v < M;
+        const long vv = ok[r] ? v : 0;
+        cv[r] = vv;
+        cx[r] = (int)(vv % s.W);
+        const long q = vv / s.W;
+        cy[r] = (int)(q % s.H);
+        cz[r] = (int)((q / s.H) % s.D);
+    }
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) acc[r][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const bf16x8 zero8 = {};
+    for (int k0 = 0; k0 < Kp; k0 += 32) {
+        const int k = k0 + 8 * g;
+        const int t = k / C, c = k - t * C;
+        const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+        bf16x8 a[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int z = cz[r] + dz, y = cy[r] + dy, x = cx[r] + dx;
+            const bool in = t < 27 && ok[r] && z >= 0 && z < s.D && y >= 0 && y < s.H && x >= 0 && x < s.W;
+            a[r] = in ? *reinterpret_cast<const bf16x8 *>(X + (cv[r] + dz * HW + dy * s.W + dx) * ldx + c) : zero8;
+        }
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+            if (nb < nbl) {
+                const bf16x8 b = *reinterpret_cast<const bf16x8 *>(Wt + (long)(n0 + 16 * nb + r16) * Kp + k);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], b, acc[r][nb], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+            if (nb >= nbl) continue;
+            const int n = n0 + 16 * nb + r16;
+            const float bv = bias ? bias[n] : 0.f;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const long v = m0 + 16 * r + 4 * g + e;
+                if (v < M && n < N) {
+                    float y = acc[r][nb][e] + bv;
+                    if (relu) y = fmaxf(y, 0.f);
+                    Y[v * ldy + n] = (bf16)y;
+                }
+            }
+        }
+}
+
+// Weight-gradient partials of one tap over one slab of voxels:
+// part[slab][t][co][ci] = sum_{v in slab} G[v][co] X[v + off(t)][ci].
+constexpr int kWgRow = 40;  // LDS row: 32 voxels + 8 pad (80 bytes, 16-byte aligned)
+__global__ __launch_bounds__(kT) void k_wgrad(Shape s, const bf16 *__restrict__ G, int ldg, int Co,
+                                             const bf16 *__restrict__ X, int ldx, int Ci, long slab,
+                                             float *__restrict__ part)
+{
+    __shared__ __attribute__((aligned(16))) bf16 tg[64 * kWgRow];
+    __shared__ __attribute__((aligned(16))) bf16 tx[128 * kWgRow];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, g = lane >> 4;
+    const int t = blockIdx.y;
+    const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+    const long M = s.voxels();
+    const long v0 = (long)blockIdx.x * slab, v1 = min(M, v0 + slab);
+    const long HW = (long)s.H * s.W;
+    const int nbo = (Co + 15) / 16, nbi = (Ci + 15) / 16, nblk = nbo * nbi;
+    // rows between the channel count and the next multiple of 16 stay zero
+    for (int q = threadIdx.x; q < 64 * kWgRow; q += kT) tg[q] = (bf16)0.f;
+    for (int q = threadIdx.x; q < 128 * kWgRow; q += kT) tx[q] = (bf16)0.f;
+    f32x4 acc[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) acc[b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const bf16x8 zero8 = {};
+    __syncthreads();
+    for (long vb = v0; vb < v1; vb += 32) {
+        const int cg = Co / 8, xg = Ci / 8;
+        for (int q = threadIdx.x; q < 32 * cg; q += kT) {
+            const int vox = q / cg, c8 = (q - vox * cg) * 8;
+            const long v = vb + vox;
+            const bf16x8 val = v < v1 ? *reinterpret_cast<const bf16x8 *>(G + v * ldg + c8) : zero8;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) tg[(c8 + e) * kWgRow + vox] = val[e];
+        }
+        for (int q = threadIdx.x; q < 32 * xg; q += kT) {
+            const int vox = q / xg, c8 = (q - vox * xg) * 8;
+            const long v = vb + vox;
+            bf16x8 val = zero8;
+            if (v < v1) {
+                const int x = (int)(v % s.W) + dx;
+                const long qq = v / s.W;
+                const int y = (int)(qq % s.H) + dy, z = (int)((qq / s.H) % s.D) + dz;
+                if (z >= 0 && z < s.D && y >= 0 && y < s.H && x >= 0 && x < s.W)
+                    val = *reinterpret_cast<const bf16x8 *>(X + (v + dz * HW + dy * s.W + dx) * ldx + c8);
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) tx[(c8 + e) * kWgRow + vox] = val[e];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const int q = wave + 4 * b;
+            if (q < nblk) {
+                const int bo = q / nbi, bi = q - bo * nbi;
+                const bf16x8 A = *reinterpret_cast<const bf16x8 *>(&tg[(16 * bo + r16) * kWgRow + 8 * g]);
+                const bf16x8 B = *reinterpret_cast<const bf16x8 *>(&tx[(16 * bi + r16) * kWgRow + 8 * g]);
+                acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[b], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    float *out = part + ((long)blockIdx.x * 27 + t) * Co * Ci;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const int q = wave + 4 * b;
+        if (q >= nblk) continue;
+        const int bo = q / nbi, bi = q - bo * nbi;
+        const int ci = 16 * bi + r16;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int co = 16 * bo + 4 * g + e;
+            if (co < Co && ci < Ci) out[(long)co * Ci + ci] = acc[b][e];
+        }
+    }
+}
+
+// dW[co][t][ci] (canonical layout) = sum over slabs in slab order.
+__global__ void k_wgrad_reduce(int nslab, int Co, int Ci, const float *__restrict__ part, float *__restrict__ dW)
+{
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long tot = 27L * Co * Ci;
+    if (n >= tot) return;
+    const int co = (int)(n / (27L * Ci));
+    const int rem = (int)(n - (long)co * 27 * Ci);
+    const int t = rem / Ci, ci = rem - t * Ci;
+    float acc = 0.f;
+    for (int q = 0; q < nslab; ++q) acc += part[(((long)q * 27 + t) * Co + co) * Ci + ci];
+    dW[n] = acc;
+}
+
+// Per-slab channel sums of G (bias gradient): part[slab][c].
+__global__ __launch_bounds__(kT) void k_colsum(long M, const bf16 *__restrict__ G, int ldg, int C, long slab,
+                                              float *__restrict__ part)
+{
+    __shared__ float red[kT];
+    const long v0 = (long)blockIdx.x * slab, v1 = min(M, v0 + slab);
+    const int c = threadIdx.x % C, lanes = (kT / C) * C;
+    const int row = threadIdx.x / C;
+    float acc = 0.f;
+    if (threadIdx.x < lanes)
+        for (long v = v0 + row; v < v1; v += kT / C) acc += (float)G[v * ldg + c];
+    red[threadIdx.x] = threadIdx.x < lanes ? acc : 0.f;
+    __syncthreads();
+    if (threadIdx.x < C) {
+        float s = 0.f;
+        for (int q = threadIdx.x; q < lanes; q += C) s += red[q];
+        part[(long)blockIdx.x * C + threadIdx.x] = s;
+    }
+}
+
+__global__ void k_slab_reduce(int nslab, int C, const float *__restrict__ part, float *__restrict__ out)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    float acc = 0.f;
+    for (int q = 0; q < nslab; ++q) acc += part[(long)q * C + c];
+    out[c] = acc;
+}
+
+// bf16 weight copies from the FP32 master: forward [co][Kp] with k = t*ci + c,
+// input-gradient [ci][Kpb] with k = t'*co + c holding W[c][26 - t'][ci].
+__global__ void k_wt_prepare(const float *__restrict__ Wm, int ci, int co, bf16 *__restrict__ Wf, int Kp,
+                             bf16 *__restrict__ Wb, int Kpb)
+{
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nf = (long)co * Kp, nbk = (long)ci * Kpb;
+    if (n < nf) {
+        const int o = (int)(n / Kp), k = (int)(n - (long)o * Kp);
+        Wf[n] = k < 27 * ci ? (bf16)Wm[(long)o * 27 * ci + k] : (bf16)0.f;
+    }
+    if (n < nbk) {
+        const int i = (int)(n / Kpb), k = (int)(n - (long)i * Kpb);
+        float w = 0.f;
+        if (k < 27 * co) {
+            const int tp = k / co, c = k - tp * co;
+            w = Wm[(long)c * 27 * ci + (26 - tp) * ci + i];
+        }
+        Wb[n] = (bf16)w;
+    }
+}
+
+__global__ void k_cast_in(long M, const float *__restrict__ in, bf16 *__restrict__ x)
+{
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < M * kInCh) x[n] = (bf16)in[n];
+}
+
+// G masked by the ReLU of the layer output Y: out = Y > 0 ? G : 0.
+__global__ void k_relu_mask(long M, int C, const bf16 *__restrict__ G, int ldg, const bf16 *__restrict__ Y, int ldy,
+                            bf16 *__restrict__ out, int ldo)
+{
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= M * C) return;
+    const long v = n / C;
+    const int c = (int)(n - v * C);
+    out[v * ldo + c] = (float)Y[v * ldy + c] > 0.f ? G[v * ldg + c] : (bf16)0.f;
+}
+
+// 2x2x2 average pooling: s is the fine shape.
+__global__ void k_pool(Shape s, int C, const bf16 *__restrict__ X, int ldx, bf16 *__restrict__ Y, int ldy)
+{
+    const Shape h = s.half();
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= h.voxels() * C) return;
+    const long v = n / C;
+    const int c = (int)(n - v * C);
+    const int x = (int)(v % h.W);
+    long q = v / h.W;
+    const int y = (int)(q % h.H);
+    q /= h.H;
+    const int z = (int)(q % h.D), b = (int)(q / h.D);
+    float acc = 0.f;
+    for (int e = 0; e < 8; ++e) {
+        const long f = (((long)b * s.D + 2 * z + (e >> 2)) * s.H + 2 * y + ((e >> 1) & 1)) * s.W + 2 * x + (e & 1);
+        acc += (float)X[f * ldx + c];
+    }
+    Y[v * ldy + c] = (bf16)(acc * 0.125f);
+}
+
+// Input gradient of the pooling plus an optional skip gradient:
+// out[v] = G[parent(v)] / 8 + S[v] (s is the fine shape).
+__global__ void k_pool_bwd(Shape s, int C, const bf16 *__restrict__ G, int ldg, const bf16 *__restrict__ S, int lds,
+                           bf16 *__restrict__ out, int ldo)
+{
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= s.voxels() * C) return;
+    const long v = n / C;
+    const int c = (int)(n - v * C);
+    const int x = (int)(v % s.W);
+    long q = v / s.W;
+    const int y = (int)(q % s.H);
+    q /= s.H;
+    const int z = (int)(q % s.D), b = (int)(q / s.D);
+    const Shape h = s.half();
+    const long p = (((long)b * h.D + z / 2) * h.H + y / 2) * h.W + x / 2;
+    float val = (float)G[p * ldg + c] * 0.125f;
+    if (S) val += (float)S[v * lds + c];
+    out[v * ldo + c] = (bf16)val;
+}
+
+// Nearest 2x upsampling: Y (fine shape s) <- X (coarse).
+__global__ void k_up(Shape s, int C, const bf16 *__restrict__ X, int ldx, bf16 *__restrict__ Y, int ldy)
+{
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= s.voxels() * C) return;
+    const long v = n / C;
+    const int c = (int)(n - v * C);
+    const int x = (int)(v % s.W);
+    long q = v / s.W;
+    const int y = (int)(q % s.H);
+    q /= s.H;
+    const int z = (int)(q % s.D), b = (int)(q / s.D);
+    const Shape h = s.half();
+    const long p = (((long)b * h.D + z / 2) * h.H + y / 2) * h.W + x / 2;
+    Y[v * ldy + c] = X[p * ldx + c];
+}
+
+// Input gradient of the upsampling: out (coarse) = sum of the 8 children of G.
+__global__ void k_up_bwd(Shape s, int C, const bf16 *__restrict__ G, int ldg, bf16 *__restrict__ out, int ldo)
+{
+    const Shape h = s.half();
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= h.voxels() * C) return;
+    const long v = n / C;
+    const int c = (int)(n - v * C);
+    const int x = (int)(v % h.W);
+    long q = v / h.W;
+    const int y = (int)(q % h.H);
+    q /= h.H;
+    const int z = (int)(q % h.D), b = (int)(q / h.D);
+    float acc = 0.f;
+    for (int e = 0; e < 8; ++e) {
+        const long f = (((long)b * s.D + 2 * z + (e >> 2)) * s.H + 2 * y + ((e >> 1) & 1)) * s.W + 2 * x + (e & 1);
+        acc += (float)G[f * ldg + c];
+    }
+    out[v * ldo + c] = (bf16)acc;
+}
+
+// 1x1x1 head: out[v] = sum_c w[c] X[v][c] + b (FP32 output).
+__global__ void k_head(long M, const bf16 *__restrict__ X, const float *__restrict__ w, float *__restrict__ out)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= M) return;
+    const bf16x8 a = *reinterpret_cast<const bf16x8 *>(X + v * 16);
+    const bf16x8 b = *reinterpret_cast<const bf16x8 *>(X + v * 16 + 8);
+    float acc = w[16];
+    for (int c = 0; c < 8; ++c) acc += w[c] * (float)a[c];
+    for (int c = 0; c < 8; ++c) acc += w[8 + c] * (float)b[c];
+    out[v] = acc;
+}
+
+// Head input gradient dX[v][c] = w[c] dout[v], and per-slab partials of
+// dw[c] = sum_v X[v][c] dout[v] (c < 16) and db = sum_v dout[v] (c = 16).
+__global__ __launch_bounds__(kT) void k_head_bwd(long M, const bf16 *__restrict__ X, const float *__restrict__ w,
+                                                const float *__restrict__ dout, bf16 *__restrict__ dX, long slab,
+                                                float *__restrict__ part)
+{
+    __shared__ float red[17][kT];
+    const long v0 = (long)blockIdx.x * slab, v1 = min(M, v0 + slab);
+    float acc[17];
+    for (int c = 0; c < 17; ++c) acc[c] = 0.f;
+    for (long v = v0 + threadIdx.x; v < v1; v += kT) {
+        const float d = dout[v];
+        for (int c = 0; c < 16; ++c) {
+            dX[v * 16 + c] = (bf16)(w[c] * d);
+            acc[c] += (float)X[v * 16 + c] * d;
+        }
+        acc[16] += d;
+    }
+    for (int c = 0; c < 17; ++c) red[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    if (threadIdx.x < 17) {
+        float s = 0.f;
+        for (int q = 0; q < kT; ++q) s += red[threadIdx.x][q];
+        part[(long)blockIdx.x * 17 + threadIdx.x] = s;
+    }
+}
+
+// Mean squared error and its gradient; per-block partial sums of the squares.
+__global__ __launch_bounds__(kT) void k_mse(long M, const float *__restrict__ out, const float *__restrict__ tgt,
+                                           float *__restrict__ dout, double *__restrict__ part)
+{
+    __shared__ double red[kT];
+    double acc = 0.0;
+    for (long v = (long)blockIdx.x * kT + threadIdx.x; v < M; v += (long)gridDim.x * kT) {
+        const float e = out[v] - tgt[v];
+        dout[v] = 2.f * e / (float)M;
+        acc += (double)e * e;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int q = 0; q < kT; ++q) s += red[q];
+        part[blockIdx.x] = s;
+    }
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master parameters.
+__global__ void k_adam(long n, float *__restrict__ p, const float *__restrict__ gr, float *__restrict__ m,
+                       float *__restrict__ v, float lr, float b1, float b2, float eps, float bc1, float bc2)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const float g = gr[q];
+    const float mq = b1 * m[q] + (1.f - b1) * g;
+    const float vq = b2 * v[q] + (1.f - b2) * g * g;
+    m[q] = mq;
+    v[q] = vq;
+    p[q] -= lr * (mq / bc1) / (sqrtf(vq / bc2) + eps);
+}
+
+inline int grid1(long n) { return (int)std::max<long>(1, (n + kT - 1) / kT); }
+
+}  // namespace wnet
+
+using namespace wnet;
+
+// ------------------------------------------------------------------ the network
+struct windhip_net {
+    hipStream_t stream = nullptr;
+    std::string err;
+    float *params = nullptr, *grads = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+    long np = 0;
+    long woff[kNumConv] = {}, boff[kNumConv] = {}, hoff = 0;
+    bf16 *wf[kNumConv] = {}, *wb[kNumConv] = {};
+    int kpf[kNumConv] = {}, kpb[kNumConv] = {};
+    bool weights_dirty = true;
+    int adam_t = 0;
+    // activations / gradients for the current shape
+    Shape shape{0, 0, 0, 0};
+    std::vector<void *> bufs;
+    bf16 *x0, *a0, *cat0, *p1, *a1, *cat1, *p2, *a2, *cat2, *p3, *a3, *b3, *e2a, *e2b, *e1a, *e1b, *e0a, *e0b;
+    bf16 *gtmp, *g0, *g1, *gcat0, *gcat1, *gcat2, *gl1a, *gl1b, *gl2a, *gl2b, *gl3a, *gl3b;
+    float *out = nullptr, *dout = nullptr, *part = nullptr;
+    double *lpart = nullptr;
+    long part_floats = 0;
+
+    int fail(const char *fmt, ...)
+    {
+        char b[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(b, sizeof b, fmt, ap);
+        va_end(ap);
+        err = b;
+        return WINDHIP_ERR_ARG;
+    }
+    int hip(hipError_t e, const char *what)
+    {
+        if (e == hipSuccess) return WINDHIP_OK;
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return WINDHIP_ERR_HIP;
+    }
+    template <typename T>
+    int dalloc(T **p, long n)
+    {
+        void *q = nullptr;
+        if (hipMalloc(&q, sizeof(T) * (size_t)std::max(n, 1L)) != hipSuccess) {
+            err = "hipMalloc failed";
+            return WINDHIP_ERR_NOMEM;
+        }
+        bufs.push_back(q);
+        *p = (T *)q;
+        return WINDHIP_OK;
+    }
+    void free_shape()
+    {
+        for (void *q : bufs) (void)hipFree(q);
+        bufs.clear();
+        shape = Shape{0, 0, 0, 0};
+    }
+    int ensure(Shape s);
+    int prepare_weights(hipStream_t st);
+    void conv(hipStream_t st, int l, Shape s, const bf16 *X, int ldx, bf16 *Y, int ldy);
+    void conv_back(hipStream_t st, int l, Shape s, const bf16 *G, int ldg, const bf16 *Yout, int ldy,
+                   const bf16 *X, int ldx, bf16 *dX, int lddx);
+    int forward(hipStream_t st, const float *in, Shape s, float *o);
+    int backward(hipStream_t st, Shape s);
+};
+
+#define NCHK(x)                   \
+    do {                          \
+        int rc_ = (x);            \
+        if (rc_) return rc_;      \
+    } while (0)
+#define NHIP(x) NCHK(hip((x), #x))
+
+int windhip_net::ensure(Shape s)
+{
+    if (s.B == shape.B && s.D == shape.D && s.H == shape.H && s.W == shape.W) return WINDHIP_OK;
+    if (s.D % 8 || s.H % 8 || s.W % 8 || s.B < 1 || s.D < 8)
+        return fail("windhip_net: volume %dx%dx%d must be multiples of 8", s.D, s.H, s.W);
+    free_shape();
+    const long m0 = s.voxels(), m1 = m0 / 8, m2 = m0 / 64, m3 = m0 / 512;
+    struct { bf16 **p; long n; } tab[] = {
+        {&x0, m0 * 8},   {&a0, m0 * 16},  {&cat0, m0 * 48}, {&p1, m1 * 16},  {&a1, m1 * 32},  {&cat1, m1 * 96},
+        {&p2, m2 * 32},  {&a2, m2 * 64},  {&cat2, m2 * 128}, {&p3, m3 * 64}, {&a3, m3 * 64},  {&b3, m3 * 64},
+        {&e2a, m2 * 64}, {&e2b, m2 * 64}, {&e1a, m1 * 32},  {&e1b, m1 * 32}, {&e0a, m0 * 16}, {&e0b, m0 * 16},
+        {&gtmp, m0 * 48}, {&g0, m0 * 48}, {&g1, m0 * 48},   {&gcat0, m0 * 48}, {&gcat1, m1 * 96},
+        {&gcat2, m2 * 128}, {&gl1a, m1 * 96}, {&gl1b, m1 * 96}, {&gl2a, m2 * 128}, {&gl2b, m2 * 128},
+        {&gl3a, m3 * 64}, {&gl3b, m3 * 64}};
+    for (auto &e : tab) NCHK(dalloc(e.p, e.n));
+    NCHK(dalloc(&out, m0));
+    NCHK(dalloc(&dout, m0));
+    // weight-gradient partials: <= 128 slabs x 27 x co x ci
+    long mx = 0;
+    for (const auto &c : kConvs) mx = std::max(mx, 27L * c.co * c.ci);
+    part_floats = 128 * mx + 128 * 64;
+    NCHK(dalloc(&part, part_floats));
+    NCHK(dalloc(&lpart, 1024));
+    shape = s;
+    return WINDHIP_OK;
+}
+
+int windhip_net::prepare_weights(hipStream_t st)
+{
+    if (!weights_dirty) return WINDHIP_OK;
+    for (int l = 0; l < kNumConv; ++l) {
+        const auto &c = kConvs[l];
+        const long n = std::max((long)c.co * kpf[l], (long)c.ci * kpb[l]);
+        k_wt_prepare<<<grid1(n), kT, 0, st>>>(params + woff[l], c.ci, c.co, wf[l], kpf[l], wb[l], kpb[l]);
+    }
+    NHIP(hipGetLastError());
+    weights_dirty = false;
+    return WINDHIP_OK;
+}
+
+void windhip_net::conv(hipStream_t st, int l, Shape s, const bf16 *X, int ldx, bf16 *Y, int ldy)
+{
+    const auto &c = kConvs[l];
+    const long M = s.voxels();
+    dim3 grid((unsigned)((M + 255) / 256), (unsigned)((c.co + 63) / 64));
+    k_conv3<<<grid, kT, 0, st>>>(s, X, ldx, c.ci, wf[l], kpf[l], params + boff[l], c.co, Y, ldy, 1);
+}
+
+// Backward of conv l (output Yout after ReLU, gradient G w.r.t. Yout): weight
+// and bias gradients into grads; input gradient into dX unless dX is null.
+void windhip_net::conv_back(hipStream_t st, int l, Shape s, const bf16 *G, int ldg, const bf16 *Yout, int ldy,
+                            const bf16 *X, int ldx, bf16 *dX, int lddx)
+{
+    const auto &c = kConvs[l];
+    const long M = s.voxels();
+    k_relu_mask<<<grid1(M * c.co), kT, 0, st>>>(M, c.co, G, ldg, Yout, ldy, gtmp, c.co);
+    long slab = std::max<long>(32, ((M + 127) / 128 + 31) / 32 * 32);
+    const int nslab = (int)((M + slab - 1) / slab);
+    k_wgrad<<<dim3(nslab, 27), kT, 0, st>>>(s, gtmp, c.co, c.co, X, ldx, c.ci, slab, part);
+    k_wgrad_reduce<<<grid1(27L * c.co * c.ci), kT, 0, st>>>(nslab, c.co, c.ci, part, grads + woff[l]);
+    k_colsum<<<nslab, kT, 0, st>>>(M, gtmp, c.co, c.co, slab, part);
+    k_slab_reduce<<<1, 64, 0, st>>>(nslab, c.co, part, grads + boff[l]);
+    if (dX) {
+        dim3 grid((unsigned)((M + 255) / 256), (unsigned)((c.ci + 63) / 64));
+        k_conv3<<<grid, kT, 0, st>>>(s, gtmp, c.co, c.co, wb[l], kpb[l], nullptr, c.ci, dX, lddx, 0);
+    }
+}
+
+int windhip_net::forward(hipStream_t st, const float *in, Shape s, float *o)
+{
+    NCHK(ensure(s));
+    NCHK(prepare_weights(st));
+    const Shape s1 = s.half(), s2 = s1.half(), s3 = s2.half();
+    const long m0 = s.voxels();
+    k_cast_in<<<grid1(m0 * kInCh), kT, 0, st>>>(m0, in, x0);
+    conv(st, C0A, s, x0, 8, a0, 16);
+    conv(st, C0B, s, a0, 16, cat0 + 32, 48);
+    k_pool<<<grid1(s1.voxels() * 16), kT, 0, st>>>(s, 16, cat0 + 32, 48, p1, 16);
+    conv(st, C1A, s1, p1, 16, a1, 32);
+    conv(st, C1B, s1, a1, 32, cat1 + 64, 96);
+    k_pool<<<grid1(s2.voxels() * 32), kT, 0, st>>>(s1, 32, cat1 + 64, 96, p2, 32);
+    conv(st, C2A, s2, p2, 32, a2, 64);
+    conv(st, C2B, s2, a2, 64, cat2 + 64, 128);
+    k_pool<<<grid1(s3.voxels() * 64), kT, 0, st>>>(s2, 64, cat2 + 64, 128, p3, 64);
+    conv(st, C3A, s3, p3, 64, a3, 64);
+    conv(st, C3B, s3, a3, 64, b3, 64);
+    k_up<<<grid1(s2.voxels() * 64), kT, 0, st>>>(s2, 64, b3, 64, cat2, 128);
+    conv(st, D2A, s2, cat2, 128, e2a, 64);
+    conv(st, D2B, s2, e2a, 64, e2b, 64);
+    k_up<<<grid1(s1.voxels() * 64), kT, 0, st>>>(s1, 64, e2b, 64, cat1, 96);
+    conv(st, D1A, s1, cat1, 96, e1a, 32);
+    conv(st, D1B, s1, e1a, 32, e1b, 32);
+    k_up<<<grid1(m0 * 32), kT, 0, st>>>(s, 32, e1b, 32, cat0, 48);
+    conv(st, D0A, s, cat0, 48, e0a, 16);
+    conv(st, D0B, s, e0a, 16, e0b, 16);
+    k_head<<<grid1(m0), kT, 0, st>>>(m0, e0b, params + hoff, o ? o : out);
+    NHIP(hipGetLastError());
+    return WINDHIP_OK;
+}
+
+// Gradients of the loss w.r.t. every parameter from dout (gradient w.r.t. the
+// head output of the last forward pass).
+int windhip_net::backward(hipStream_t st, Shape s)
+{
+    const Shape s1 = s.half(), s2 = s1.half(), s3 = s2.half();
+    const long m0 = s.voxels();
+    long slab = std::max<long>(256, ((m0 + 127) / 128 + 255) / 256 * 256);
+    const int nslab = (int)((m0 + slab - 1) / slab);
+    // head
+    k_head_bwd<<<nslab, kT, 0, st>>>(m0, e0b, params + hoff, dout, g0, slab, part);
+    k_slab_reduce<<<1, 64, 0, st>>>(nslab, 17, part, grads + hoff);
+    // level 0 decoder
+    conv_back(st, D0B, s, g0, 16, e0b, 16, e0a, 16, g1, 16);
+    conv_back(st, D0A, s, g1, 16, e0a, 16, cat0, 48, gcat0, 48);
+    k_up_bwd<<<grid1(s1.voxels() * 32), kT, 0, st>>>(s, 32, gcat0, 48, gl1a, 32);  // d e1b
+    conv_back(st, D1B, s1, gl1a, 32, e1b, 32, e1a, 32, gl1b, 32);
+    conv_back(st, D1A, s1, gl1b, 32, e1a, 32, cat1, 96, gcat1, 96);
+    k_up_bwd<<<grid1(s2.voxels() * 64), kT, 0, st>>>(s1, 64, gcat1, 96, gl2a, 64);  // d e2b
+    conv_back(st, D2B, s2, gl2a, 64, e2b, 64, e2a, 64, gl2b, 64);
+    conv_back(st, D2A, s2, gl2b, 64, e2a, 64, cat2, 128, gcat2, 128);
+    k_up_bwd<<<grid1(s3.voxels() * 64), kT, 0, st>>>(s2, 64, gcat2, 128, gl3a, 64);  // d b3
+    // bottleneck
+    conv_back(st, C3B, s3, gl3a, 64, b3, 64, a3, 64, gl3b, 64);
+    conv_back(st, C3A, s3, gl3b, 64, a3, 64, p3, 64, gl3a, 64);  // d p3
+    // encoder level 2: d s2 = pool_bwd(d p3) + skip part of d cat2
+    k_pool_bwd<<<grid1(s2.voxels() * 64), kT, 0, st>>>(s2, 64, gl3a, 64, gcat2 + 64, 128, gl2a, 64);
+    conv_back(st, C2B, s2, gl2a, 64, cat2 + 64, 128, a2, 64, gl2b, 64);
+    conv_back(st, C2A, s2, gl2b, 64, a2, 64, p2, 32, gl2a, 32);  // d p2
+    k_pool_bwd<<<grid1(s1.voxels() * 32), kT, 0, st>>>(s1, 32, gl2a, 32, gcat1 + 64, 96, gl1a, 32);
+    conv_back(st, C1B, s1, gl1a, 32, cat1 + 64, 96, a1, 32, gl1b, 32);
+    conv_back(st, C1A, s1, gl1b, 32, a1, 32, p1, 16, gl1a, 16);  // d p1
+    k_pool_bwd<<<grid1(m0 * 16), kT, 0, st>>>(s, 16, gl1a, 16, gcat0 + 32, 48, g0, 16);
+    conv_back(st, C0B, s, g0, 16, cat0 + 32, 48, a0, 16, g1, 16);
+    conv_back(st, C0A, s, g1, 16, a0, 16, x0, 8, nullptr, 0);
+    NHIP(hipGetLastError());
+    return WINDHIP_OK;
+}
+
+int wnet_forward_dev(windhip_net *net, hipStream_t stream, const float *in, wnet::Shape s, float *out)
+{
+    return net->forward(stream, in, s, out);
+}
+
+// ------------------------------------------------------------------ C-ABI
+extern "C" {
+
+long windhip_net_param_count(void) { return param_count(); }
+
+int windhip_net_create(windhip_net **out)
+{
+    if (!out) return WINDHIP_ERR_ARG;
+    *out = nullptr;
+    windhip_net *n = new windhip_net();
+    auto bad = [&](int rc) { n->free_shape(); delete n; return rc; };
+    if (hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess) return bad(WINDHIP_ERR_HIP);
+    long off = 0;
+    for (int l = 0; l < kNumConv; ++l) {
+        const auto &c = kConvs[l];
+        n->woff[l] = off;
+        off += 27L * c.co * c.ci;
+        n->boff[l] = off;
+        off += c.co;
+        n->kpf[l] = (27 * c.ci + 31) / 32 * 32;
+        n->kpb[l] = (27 * c.co + 31) / 32 * 32;
+    }
+    n->hoff = off;
+    n->np = off + 17;
+    // parameters and their Adam state live outside the per-shape buffers
+    std::vector<void *> keep;
+    auto palloc = [&](void **p, size_t nb) {
+        if (hipMalloc(p, nb) != hipSuccess) return false;
+        return hipMemset(*p, 0, nb) == hipSuccess;
+    };
+    if (!palloc((void **)&n->params, n->np * 4) || !palloc((void **)&n->grads, n->np * 4) ||
+        !palloc((void **)&n->adam_m, n->np * 4) || !palloc((void **)&n->adam_v, n->np * 4))
+        return bad(WINDHIP_ERR_NOMEM);
+    for (int l = 0; l < kNumConv; ++l) {
+        const auto &c = kConvs[l];
+        if (!palloc((void **)&n->wf[l], (size_t)c.co * n->kpf[l] * 2) ||
+            !palloc((void **)&n->wb[l], (size_t)c.ci * n->kpb[l] * 2))
+            return bad(WINDHIP_ERR_NOMEM);
+    }
+    *out = n;
+    return WINDHIP_OK;
+}
+
+int windhip_net_destroy(windhip_net *n)
+{
+    if (!n) return WINDHIP_OK;
+    if (n->stream) (void)hipStreamSynchronize(n->stream);
+    n->free_shape();
+    for (float *p : {n->params, n->grads, n->adam_m, n->adam_v})
+        if (p) (void)hipFree(p);
+    for (int l = 0; l < kNumConv; ++l) {
+        if (n->wf[l]) (void)hipFree(n->wf[l]);
+        if (n->wb[l]) (void)hipFree(n->wb[l]);
+    }
+    if (n->stream) (void)hipStreamDestroy(n->stream);
+    delete n;
+    return WINDHIP_OK;
+}
+
+const char *windhip_net_last_error(windhip_net *n) { return n ? n->err.c_str() : "null network handle"; }
+
+int windhip_net_set_params(windhip_net *n, const float *params)
+{
+    if (!n || !params) return WINDHIP_ERR_ARG;
+    if (hipMemcpy(n->params, params, n->np * 4, hipMemcpyHostToDevice) != hipSuccess) return WINDHIP_ERR_HIP;
+    if (hipMemset(n->adam_m, 0, n->np * 4) != hipSuccess || hipMemset(n->adam_v, 0, n->np * 4) != hipSuccess)
+        return WINDHIP_ERR_HIP;
+    n->adam_t = 0;
+    n->weights_dirty = true;
+    return WINDHIP_OK;
+}
+
+int windhip_net_get_params(windhip_net *n, float *params)
+{
+    if (!n || !params) return WINDHIP_ERR_ARG;
+    if (hipStreamSynchronize(n->stream) != hipSuccess) return WINDHIP_ERR_HIP;
+    return hipMemcpy(params, n->params, n->np * 4, hipMemcpyDeviceToHost) == hipSuccess ? WINDHIP_OK
+                                                                                       : WINDHIP_ERR_HIP;
+}
+
+int windhip_net_forward(windhip_net *n, const float *in_dev, int B, int D, int H, int W, float *out_dev)
+{
+    if (!n || !in_dev || !out_dev) return WINDHIP_ERR_ARG;
+    NCHK(n->forward(n->stream, in_dev, Shape{B, D, H, W}, out_dev));
+    return n->hip(hipStreamSynchronize(n->stream), "forward");
+}
+
+int windhip_net_loss_grad(windhip_net *n, const float *in_dev, const float *target_dev, int B, int D, int H, int W,
+                          double *loss, float *grad_host)
+{
+    if (!n || !in_dev || !target_dev) return WINDHIP_ERR_ARG;
+    const Shape s{B, D, H, W};
+    NCHK(n->forward(n->stream, in_dev, s, nullptr));
+    const long m0 = s.voxels();
+    const int nb = (int)std::min<long>(1024, (m0 + kT - 1) / kT);
+    k_mse<<<nb, kT, 0, n->stream>>>(m0, n->out, target_dev, n->dout, n->lpart);
+    NCHK(n->backward(n->stream, s));
+    std::vector<double> lp(nb);
+    NCHK(n->hip(hipMemcpyAsync(lp.data(), n->lpart, nb * sizeof(double), hipMemcpyDeviceToHost, n->stream), "loss"));
+    NCHK(n->hip(hipStreamSynchronize(n->stream), "loss_grad"));
+    double acc = 0.0;
+    for (double x : lp) acc += x;
+    if (loss) *loss = acc / (double)m0;
+    if (grad_host)
+        NCHK(n->hip(hipMemcpy(grad_host, n->grads, n->np * 4, hipMemcpyDeviceToHost), "grads"));
+    return WINDHIP_OK;
+}
+
+int windhip_net_adam_step(windhip_net *n, float lr, float beta1, float beta2, float eps)
+{
+    if (!n) return WINDHIP_ERR_ARG;
+    n->adam_t += 1;
+    const float bc1 = 1.f - powf(beta1, (float)n->adam_t), bc2 = 1.f - powf(beta2, (float)n->adam_t);
+    k_adam<<<grid1(n->np), kT, 0, n->stream>>>(n->np, n->params, n->grads, n->adam_m, n->adam_v, lr, beta1, beta2,
+                                              eps, bc1, bc2);
+    n->weights_dirty = true;
+    return n->hip(hipGetLastError(), "adam");
+}
+
+int windhip_net_time_forward(windhip_net *n, int B, int D, int H, int W, int reps, double *ms)
+{
+    if (!n || reps < 1) return WINDHIP_ERR_ARG;
+    const Shape s{B, D, H, W};
+    NCHK(n->ensure(s));
+    float *in = nullptr;
+    NCHK(n->hip(hipMalloc(&in, s.voxels() * kInCh * 4), "time_forward"));
+    (void)hipMemset(in, 0, s.voxels() * kInCh * 4);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    int rc = n->forward(n->stream, in, s, nullptr);
+    (void)hipEventRecord(e0, n->stream);
+    for (int q = 0; q < reps && !rc; ++q) rc = n->forward(n->stream, in, s, nullptr);
+    (void)hipEventRecord(e1, n->stream);
+    (void)hipEventSynchronize(e1);
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    if (ms) *ms = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(in);
+    return rc;
+}
+
+}  // extern "C"

@@ -517,4 +517,56 @@ __global__ void k_widen_f64(long n, const float *__restrict__ a, double *__restr
         b[e] = (double)a[e];
 }
 
+// ------------------------------------------------------------------ learned guess
+// Network input from the solver's fields, one thread per (x, y) column of the
+// padded volume (Dp, Hp, Wp multiples of 8, channels-last, 8 channels):
+// 0 rhs b * bscale, 1 dz/dx, 2 dz/dy, 3 Jacobian, 4 alpha, 5 height of the
+// cell centre above the surface / column height, 6..7 zero; zero outside the
+// grid.
+__global__ void k_net_input(Dims d, int Dp, int Hp, int Wp, const double *__restrict__ b, double bscale,
+                            const double *__restrict__ dzdx, const double *__restrict__ dzdy,
+                            const double *__restrict__ jaco, const double *__restrict__ alpha,
+                            const double *__restrict__ adv_dz, float *__restrict__ in)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long)Hp * Wp) return;
+    const int x = (int)(q % Wp), y = (int)(q / Wp);
+    const bool col = x < d.nx && y < d.ny;
+    double top = 0.0;
+    if (col)
+        for (int k = 1; k <= d.nz; ++k) top += adv_dz[d.r(x + 1, k, y + 1)];
+    double h = 0.0;
+    for (int z = 0; z < Dp; ++z) {
+        float *o = in + (((long)z * Hp + y) * Wp + x) * 8;
+        float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (col && z < d.nz) {
+            const long n = d.r(x + 1, z + 1, y + 1);
+            const double dz = adv_dz[n];
+            v[0] = (float)(b[n] * bscale);
+            v[1] = (float)dzdx[n];
+            v[2] = (float)dzdy[n];
+            v[3] = (float)jaco[n];
+            v[4] = (float)alpha[n];
+            v[5] = (float)((h + 0.5 * dz) / top);
+            h += dz;
+        }
+        for (int c = 0; c < 8; ++c) o[c] = v[c];
+    }
+}
+
+// lambda rows from the network output: interior = scale * out, lateral rows
+// 0 (their rhs), ghost planes copied from the adjacent interior level.
+__global__ void k_net_output(Dims d, int Dp, int Hp, int Wp, const float *__restrict__ out, double scale,
+                             double *__restrict__ x)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= d.P) return;
+    const int i = (int)(q % d.mx), j = (int)(q / d.mx);
+    const bool lat = i == 0 || i == d.mx - 1 || j == 0 || j == d.my - 1;
+    for (int k = 1; k <= d.nz; ++k)
+        x[d.r(i, k, j)] = lat ? 0.0 : scale * (double)out[((long)(k - 1) * Hp + (j - 1)) * Wp + (i - 1)];
+    x[d.r(i, 0, j)] = x[d.r(i, 1, j)];
+    x[d.r(i, d.mz - 1, j)] = x[d.r(i, d.nz, j)];
+}
+
 }  // namespace windhip

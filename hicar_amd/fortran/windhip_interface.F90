@@ -38,12 +38,14 @@ module windhip_interface
         integer(c_int) :: status, iters, nsweeps, outer_iters
         real(c_double) :: res0, res_final, bnorm, true_rel_res, ms
         integer(c_int) :: start_used, reserved
+        real(c_double) :: guess_res, plain_res
     end type windhip_solve_info
 
     public :: windhip_default_opts, windhip_create, windhip_destroy, windhip_last_error, &
               windhip_set_metrics, windhip_solve, windhip_apply_correction, windhip_calc_iter_winds, &
               windhip_divergence, windhip_get_lambda, windhip_set_lambda, windhip_probe_operator, &
-              windhip_error_message
+              windhip_net_param_count, windhip_net_create, windhip_net_destroy, windhip_net_set_params, &
+              windhip_set_net, windhip_error_message
 
     interface
         subroutine windhip_default_opts(opts) bind(C, name="windhip_default_opts")
@@ -121,6 +123,31 @@ module windhip_interface
             type(c_ptr), value :: handle, alpha, density
             integer(c_int), value :: adv_den
             real(c_double), intent(out) :: max_leak
+        end function
+
+        ! learned initial guess: load trained parameters and attach the network
+        integer(c_long) function windhip_net_param_count() bind(C, name="windhip_net_param_count")
+            import :: c_long
+        end function
+
+        integer(c_int) function windhip_net_create(net) bind(C, name="windhip_net_create")
+            import :: c_int, c_ptr
+            type(c_ptr), intent(out) :: net
+        end function
+
+        integer(c_int) function windhip_net_destroy(net) bind(C, name="windhip_net_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: net
+        end function
+
+        integer(c_int) function windhip_net_set_params(net, params) bind(C, name="windhip_net_set_params")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: net, params
+        end function
+
+        integer(c_int) function windhip_set_net(handle, net) bind(C, name="windhip_set_net")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle, net
         end function
     end interface
 

@@ -83,7 +83,7 @@ typedef struct windhip_solve_info {
     int status;            /* 0 converged, 1 iteration cap, 2 breakdown (bicgstab_solve) */
     int iters;             /* BiCGStab iterations (MIXED: total inner iterations) */
     int nsweeps;           /* preconditioner sweeps of the last solve */
-    int outer_iters;       /* MIXED: refinement steps */
+    int outer_iters;       /* MIXED: FGMRES steps (FP32 inner solves) */
     double res0;           /* ||r0|| (recurrence norm of the reference, absolute) */
     double res_final;      /* final recurrence residual norm (MIXED: true residual) */
     double bnorm;          /* ||b|| */
@@ -91,6 +91,8 @@ typedef struct windhip_solve_info {
     double ms;             /* device time of the solve (hipEvents on the handle's stream) */
     int start_used;        /* WINDHIP_START_* actually used (the learned start falls back) */
     int reserved;
+    double guess_res;      /* LEARNED: ||b - A lambda_net|| of the network's guess */
+    double plain_res;      /* LEARNED: ||b - A lambda|| of the plain start it was compared with */
 } windhip_solve_info;
 
 /* Library / handle life cycle. create ~ init_iter_winds (wind_iterative.F90:215-291):
@@ -164,6 +166,44 @@ int windhip_time_kernel(windhip_solver *s, int which, int reps, double *avg_ms);
 int windhip_set_alpha(windhip_solver *s, const void *alpha);
 int windhip_apply_operator(windhip_solver *s, int precision, const double *x_rows, double *y_rows);
 int windhip_get_rows(windhip_solver *s, int which, double *out_rows);
+
+/* ------------------------------------------------------------------ learned guess
+ * No counterpart in the reference: the learned initial guess for lambda. A
+ * 3-D convolutional encoder-decoder (4 scales, 16/32/64/64 channels, 3x3x3
+ * kernels, bf16 weights, FP32 accumulation on MFMA) maps the rhs, dz/dx, dz/dy,
+ * the Jacobian, alpha and the height above ground to lambda / (rms(b) dx^2).
+ * Volumes are channels-last FP32 device arrays [B][D][H][W][8] (channels 6, 7
+ * zero), D, H, W multiples of 8. Parameters are one flat FP32 array, per
+ * convolution W[co][27][ci] (tap = 9 (dz+1) + 3 (dy+1) + (dx+1)) then bias[co],
+ * in the order c0a c0b c1a c1b c2a c2b c3a c3b d2a d2b d1a d1b d0a d0b, then
+ * the 1x1x1 head (16 weights, 1 bias). */
+typedef struct windhip_net windhip_net;
+long windhip_net_param_count(void);
+int windhip_net_create(windhip_net **out);
+int windhip_net_destroy(windhip_net *net);
+const char *windhip_net_last_error(windhip_net *net);
+int windhip_net_set_params(windhip_net *net, const float *params_host);
+int windhip_net_get_params(windhip_net *net, float *params_host);
+/* out[B][D][H][W] = network(in) */
+int windhip_net_forward(windhip_net *net, const float *in_dev, int B, int D, int H, int W, float *out_dev);
+/* Mean squared error against target [B][D][H][W] and the gradient of every
+ * parameter (copied to grad_host when non-null) */
+int windhip_net_loss_grad(windhip_net *net, const float *in_dev, const float *target_dev, int B, int D, int H,
+                          int W, double *loss, float *grad_host);
+/* One Adam step (Kingma & Ba) with the gradient of the last loss_grad */
+int windhip_net_adam_step(windhip_net *net, float lr, float beta1, float beta2, float eps);
+/* Mean device time of the forward pass over a B x D x H x W volume */
+int windhip_net_time_forward(windhip_net *net, int B, int D, int H, int W, int reps, double *ms);
+/* Attach a network to a solver: opts.start = WINDHIP_START_LEARNED then
+ * starts from the network's guess when its FP64 residual is below the plain
+ * start's (the current lambda), and from the plain start otherwise. */
+int windhip_set_net(windhip_solver *s, windhip_net *net);
+/* The network input for the solver's current rhs (that of its last solve),
+ * metrics and alpha, written to in_dev (1 x dims[0] x dims[1] x dims[2] x 8);
+ * *scale turns the network output into lambda. dims_out receives the padded
+ * volume (nz, ny, nx rounded up to multiples of 8); in_dev may be null to
+ * query it. */
+int windhip_learned_input(windhip_solver *s, float *in_dev, int *dims_out, double *scale);
 
 /* Per-kernel device time accumulated by solves run with opts.profile: arrays of
  * WINDHIP_PROFILE_KINDS (calls, total ms) indexed by kernel kind: 0..4 FP64 lone
