@@ -35,7 +35,7 @@ EXPORTS = (
     "windhip_get_rows", "windhip_profile",
     "windhip_net_param_count", "windhip_net_create", "windhip_net_destroy", "windhip_net_last_error",
     "windhip_net_set_params", "windhip_net_get_params", "windhip_net_forward", "windhip_net_loss_grad",
-    "windhip_net_adam_step", "windhip_net_time_forward", "windhip_set_net", "windhip_learned_input",
+    "windhip_net_adam_step", "windhip_net_time_forward", "windhip_set_net", "windhip_learned_input", "windhip_set_guess_smoothing",
 )
 
 PROFILE_KINDS = {0: "sweep_f64", 1: "sweep_sigma_f64", 2: "update_s_f64", 3: "sweep_omega_f64",
@@ -78,6 +78,23 @@ class WindHipError(RuntimeError):
 _lib = None
 
 
+def _init_torch_runtime_first() -> None:
+    """PyTorch's ROCm wheel carries its own HIP/HSA runtime next to the system
+    one this library links. In one process torch only finds the GPU if its
+    runtime initialises before the system one, so when torch is present it is
+    initialised before the library is loaded. Hosts without torch see a
+    single runtime and skip this."""
+    try:
+        import torch
+    except ImportError:
+        return
+    try:
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except RuntimeError:
+        pass
+
+
 def load():
     """Load libwindhip.so; raise if it has not been built."""
     global _lib
@@ -85,6 +102,7 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() (hicar_amd/csrc/Makefile)")
+    _init_torch_runtime_first()
     lib = C.CDLL(LIB_PATH)
     vp, i, d = C.c_void_p, C.c_int, C.c_double
     pp = C.POINTER(C.c_void_p)
@@ -121,6 +139,7 @@ def load():
         "windhip_net_adam_step": (i, [vp, C.c_float, C.c_float, C.c_float, C.c_float]),
         "windhip_net_time_forward": (i, [vp, i, i, i, i, i, dp]),
         "windhip_set_net": (i, [vp, vp]),
+        "windhip_set_guess_smoothing": (i, [vp, i]),
         "windhip_learned_input": (i, [vp, vp, C.POINTER(C.c_int), dp]),
     }
     for name, (res, args) in sig.items():

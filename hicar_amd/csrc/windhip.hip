@@ -113,6 +113,8 @@ struct windhip_solver {
     windhip_net *net = nullptr;
     float *net_in = nullptr, *net_out = nullptr;
     long net_vox = 0;
+    int net_smooth = 4;                        // line-relaxation steps applied to the network's guess
+    double *lr_c = nullptr, *lr_d = nullptr;   // line relaxation: Thomas coefficients
     bool metrics_set = false, structure_uploaded = false, probed = false;
     bool verbose = false;  // WINDHIP_VERBOSE=1: per-batch progress on stderr (verbose_solver, :274-275)
     int nsweeps = 2;  // precond_n_sweeps, BASE_PREC_SWEEPS (wind_iterative.F90:76)
@@ -266,6 +268,7 @@ struct windhip_solver {
     int net_dims(int *Dp, int *Hp, int *Wp) const;
     int net_input(float *in, double *scale);
     int learned_start(double *res_guess, double *res_plain, bool *used);
+    int line_relax(double **y, double **scratch, int steps);
     template <typename T>
     int enqueue_iterations(int count);
     template <class Launch>
@@ -671,9 +674,30 @@ int windhip_solver::net_input(float *in, double *scale)
     return WINDHIP_OK;
 }
 
-// Start selection: the network's guess and the plain start (the current
-// lambda: the previous solution, or zero) are both evaluated by their FP64
-// true residual; the lower one becomes the starting lambda.
+// `steps` line-relaxation steps on *y against the current rhs (FP64 operator),
+// alternating between *y and *scratch; *y points at the result.
+int windhip_solver::line_relax(double **y, double **scratch, int steps)
+{
+    if (steps <= 0) return WINDHIP_OK;
+    if (!lr_c) {
+        CHK(alloc(&lr_c, N));
+        CHK(alloc(&lr_d, N));
+    }
+    for (int q = 0; q < steps; ++q) {
+        CHK(with_op<double>([&](auto op) {
+            k_line_relax<decltype(op)><<<g.nt, kThreads, 0, stream>>>(op, g, *y, b, lr_c, lr_d, *scratch);
+            return WINDHIP_OK;
+        }));
+        HIPCHK(hipGetLastError());
+        std::swap(*y, *scratch);
+    }
+    return WINDHIP_OK;
+}
+
+// Start selection: the network's guess, after net_smooth vertical
+// line-relaxation steps, and the plain start (the current lambda: the previous
+// solution, or zero) are both evaluated by their FP64 true residual; the lower
+// one becomes the starting lambda.
 int windhip_solver::learned_start(double *res_guess, double *res_plain, bool *used)
 {
     int Dp, Hp, Wp;
@@ -691,10 +715,12 @@ int windhip_solver::learned_start(double *res_guess, double *res_plain, bool *us
     if (rc) return fail(rc, "learned start: network forward failed");
     k_net_output<<<flat_grid(d.P), kBlock, 0, stream>>>(d, Dp, Hp, Wp, net_out, scale, t);
     HIPCHK(hipGetLastError());
-    CHK(true_residual(&rel, t));
+    double *gy = t, *gs = s;
+    CHK(line_relax(&gy, &gs, net_smooth));
+    CHK(true_residual(&rel, gy));
     *res_guess = st_host[0].outer_rnorm;
     *used = *res_guess < *res_plain;
-    if (*used) HIPCHK(hipMemcpyAsync(x, t, N * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    if (*used) HIPCHK(hipMemcpyAsync(x, gy, N * sizeof(double), hipMemcpyDeviceToDevice, stream));
     return WINDHIP_OK;
 }
 
@@ -1321,6 +1347,13 @@ int windhip_set_net(windhip_solver *s, windhip_net *net)
 {
     if (!s) return WINDHIP_ERR_ARG;
     s->net = net;
+    return WINDHIP_OK;
+}
+
+int windhip_set_guess_smoothing(windhip_solver *s, int steps)
+{
+    if (!s || steps < 0 || steps > 100) return fail(WINDHIP_ERR_ARG, "windhip_set_guess_smoothing: 0..100 steps");
+    s->net_smooth = steps;
     return WINDHIP_OK;
 }
 

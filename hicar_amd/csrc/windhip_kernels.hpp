@@ -229,6 +229,15 @@ struct OpA {
         if (k == mz - 1) return dinv_c<kRowTop>(lat, f, b);
         return dinv_c<kRowInterior>(lat, f, b);
     }
+    // The vertical line of row k (coefficients of x(k-1), x(k), x(k+1)): the
+    // tridiagonal part of A used by the line relaxation.
+    __device__ __forceinline__ void tri(int k, int mz, bool lat, const Cf &f, const Bc &b, T &l, T &dd, T &u) const
+    {
+        if (lat) { l = T(0); dd = T(1); u = T(0); return; }
+        if (k == 0) { l = T(0); dd = -b.bz; u = b.bz; return; }
+        if (k == mz - 1) { l = T(-1) / b.dzt; dd = T(1) / b.dzt; u = T(0); return; }
+        l = f.C; dd = (neg4 - f.B) - f.C; u = f.B;
+    }
     // one Richardson update y + D^-1 (in - A y) (apply_precond, :1300-1314)
     __device__ __forceinline__ T relax(T yc, T dv, T inv_, T ay) const
     {
@@ -274,6 +283,12 @@ struct OpP {
         else return dinv(1, 3, lat, f, bc);
     }
     __device__ __forceinline__ T relax(T yc, T dv, T inv_, T ay) const { return yc + dv * (inv_ - ay); }
+    __device__ __forceinline__ void tri(int k, int mz, bool lat, const Cf &f, const Bc &, T &l, T &dd, T &u) const
+    {
+        if (lat || k == 0 || k == mz - 1) { l = T(0); dd = T(1); u = T(0); return; }
+        l = f.a[2]; dd = f.a[0]; u = f.a[1];  // C, A, B
+    }
+
     __device__ __forceinline__ T row(int k, int mz, bool lat, const Win<T> &x, const Cf &f, const Bc &) const
     {
         if (lat || k == 0 || k == mz - 1) return x.c1;
@@ -885,6 +900,55 @@ __global__ __launch_bounds__(kThreads) void k_true_resid(Op op, Geo g, KState *s
         st->outer_rnorm = sqrt(tot[0]);
         st->outer_bnorm = sqrt(tot[1]);
     });
+}
+
+// ------------------------------------------------------------------ line relaxation
+// One vertical line-relaxation step out = y + M^-1 (in - A y), M the
+// tridiagonal (vertical) part of A in every column: the Thomas algorithm, the
+// elimination on the way up (its modified coefficients stored in cq, dq) and
+// the back substitution on the way down. The terrain-following grid couples
+// the levels of a column far more strongly than neighbouring columns
+// (dz << dx near the ground), so this removes the stiff vertical error
+// components that point relaxation leaves behind. out must not alias y.
+template <class Op>
+__global__ __launch_bounds__(kThreads) void k_line_relax(Op op, Geo g, const typename Op::type *__restrict__ y,
+                                                         const typename Op::type *__restrict__ in,
+                                                         typename Op::type *__restrict__ cq,
+                                                         typename Op::type *__restrict__ dq,
+                                                         typename Op::type *__restrict__ out)
+{
+    using T = typename Op::type;
+    const TilePos t = tile_pos(g);
+    if (!t.inner) return;
+    const long P = g.P;
+    const int mz = g.mz;
+    Win<T> win;
+    load_level(y, 0, t, win.c1, win.w1, win.e1, win.s1, win.n1);
+    load_level(y, P, t, win.c2, win.w2, win.e2, win.s2, win.n2);
+    win.c0 = win.w0 = win.e0 = win.s0 = win.n0 = T(0);
+    const typename Op::Bc bc = op.bc(t.c);
+    T cprev = T(0), dprev = T(0);
+    for (int k = 0; k < mz; ++k) {
+        const long n = (long)k * P + t.c;
+        typename Op::Cf cf = {};
+        if (k >= 1 && k <= mz - 2) cf = op.load(n);
+        const T r = in[n] - op.row(k, mz, t.lat, win, cf, bc);
+        T l, dd, u;
+        op.tri(k, mz, t.lat, cf, bc, l, dd, u);
+        const T m = dd - l * cprev;
+        cprev = u / m;
+        dprev = (r - l * dprev) / m;
+        cq[n] = cprev;
+        dq[n] = dprev;
+        win.shift();
+        if (k + 2 <= mz - 1) load_level(y, (long)(k + 2) * P, t, win.c2, win.w2, win.e2, win.s2, win.n2);
+    }
+    T delta = T(0);
+    for (int k = mz - 1; k >= 0; --k) {
+        const long n = (long)k * P + t.c;
+        delta = dq[n] - cq[n] * delta;
+        out[n] = y[n] + delta;
+    }
 }
 
 // ------------------------------------------------------------------ FGMRES-IR

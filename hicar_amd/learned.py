@@ -114,9 +114,12 @@ class Net:
         return ms.value
 
 
-def attach(solver, net: Net | None) -> None:
-    """Use `net` for solver.solve(..., start=nat.START_LEARNED)."""
+def attach(solver, net: Net | None, smooth: int | None = None) -> None:
+    """Use `net` for solver.solve(..., start=nat.START_LEARNED); `smooth`
+    sets the line-relaxation steps applied to its guess (default 4)."""
     nat.check(solver.lib.windhip_set_net(solver.h, net.h if net is not None else None))
+    if smooth is not None:
+        nat.check(solver.lib.windhip_set_guess_smoothing(solver.h, int(smooth)))
     solver._net = net  # keep it alive while attached
 
 
@@ -143,23 +146,50 @@ def lambda_target(solver, inp_shape, scale: float):
     return t
 
 
-def make_pair(case, solver_cls, device: str = "cuda", tol_rel: float = 1e-8):
+def load_case(solver, case, device: str = "cuda"):
+    """Upload a synthetic case's metrics into `solver` (device fields); returns
+    (alpha, div) device tensors, div = D(u, v, w) of the case's winds."""
+    import torch
+
+    D_ = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(device)  # noqa: E731
+    solver.set_metrics({k: D_(v) for k, v in case.metrics.items()
+                        if k in ("dzdx", "dzdy", "jacobian", "jacobian_u", "jacobian_v", "jacobian_w", "dzdx_u",
+                                 "dzdy_v", "advection_dz", "terrain")})
+    div = torch.empty(solver.mass_shape, dtype=torch.float64, device=device)
+    solver.divergence(D_(case.u), D_(case.v), D_(case.w), density=D_(case.rho), adv_den=True, use_dqdt=True,
+                      out=div)
+    return D_(case.alpha), div
+
+
+def make_pair(case, solver, device: str = "cuda", tol_rel: float = 1e-8):
     """One training pair from a synthetic case: the plain FP64 solve of its
     divergence, the network input assembled by the solver, the scaled target."""
     import torch
 
-    D_ = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(device)  # noqa: E731
-    s = solver_cls(case.nx, case.ny, case.nz, case.dx, device_fields=True)
-    s.set_metrics({k: D_(v) for k, v in case.metrics.items()
-                   if k in ("dzdx", "dzdy", "jacobian", "jacobian_u", "jacobian_v", "jacobian_w", "dzdx_u",
-                            "dzdy_v", "advection_dz", "terrain")})
-    div = torch.empty(s.mass_shape, dtype=torch.float64, device=device)
-    s.divergence(D_(case.u), D_(case.v), D_(case.w), density=D_(case.rho), adv_den=True, use_dqdt=True, out=div)
-    info = s.solve(D_(case.alpha), div, precision=0, start=1, tol_rel=tol_rel, max_iters=20000)
-    inp, scale = learned_input(s, device)
-    tgt = torch.from_numpy(lambda_target(s, tuple(inp.shape), scale)).to(device)
-    s.close()
+    alpha, div = load_case(solver, case, device)
+    info = solver.solve(alpha, div, precision=0, start=nat.START_ZERO, tol_rel=tol_rel, max_iters=20000)
+    inp, scale = learned_input(solver, device)
+    tgt = torch.from_numpy(lambda_target(solver, tuple(inp.shape), scale)).to(device)
     return inp, tgt, info
+
+
+def evaluate(solver, alpha, div, net: Net, tol_rel: float = 1e-8, precisions=(0, 1), smooth: int = 4) -> dict:
+    """Iterations and time of the same system solved from three starts, for
+    each precision (0 plain FP64, 1 mixed): zero; the zero start after `smooth`
+    line-relaxation steps (a network with zero parameters: what the relaxation
+    alone gives); the network's guess after the same steps."""
+    zero_net = Net(np.zeros(net.n, dtype=np.float32))
+    out = {}
+    for prec in precisions:
+        for name, start, nn in (("zero", nat.START_ZERO, None), ("relax", nat.START_LEARNED, zero_net),
+                                ("learned", nat.START_LEARNED, net)):
+            attach(solver, nn, smooth)
+            solver.set_lambda(np.zeros(solver.rows_shape))
+            info = solver.solve(alpha, div, precision=prec, start=start, tol_rel=tol_rel, max_iters=50000)
+            out[f"{'plain' if prec == 0 else 'mixed'}_{name}"] = info
+    attach(solver, None)
+    zero_net.close()
+    return out
 
 
 def train(net: Net, inputs, targets, epochs: int, batch: int = 8, lr: float = 1e-3, seed: int = 0,
@@ -188,3 +218,78 @@ def train(net: Net, inputs, targets, epochs: int, batch: int = 8, lr: float = 1e
         if time_budget_s is not None and time.time() - t0 > time_budget_s:
             break
     return hist
+
+
+# ------------------------------------------------------------------ training CLI
+def _patch(args):
+    """A seeded training case (runs in a worker process; numpy only)."""
+    from . import synthetic
+
+    seed, n, nz = args
+    rng = np.random.default_rng(seed)
+    return synthetic.fractal_case(n, n, nz, dx=50.0, relief=float(rng.uniform(500.0, 3000.0)),
+                                  speed=float(rng.uniform(2.0, 20.0)), seed=seed,
+                                  varying_alpha=bool(rng.integers(0, 2)),
+                                  direction_deg=float(rng.uniform(0.0, 360.0)))
+
+
+def main(argv=None) -> int:
+    """python -m hicar_amd.learned --patches N --out params.npy
+
+    Generates seeded 128x128x40 fractal-terrain patches (relief 500-3000 m,
+    wind 2-20 m/s, half of them with a varying stability weight), solves each
+    with the plain FP64 GPU solver to rel. residual 1e-8, and trains the
+    network with Adam (lr 1e-3, batch 8) on (input, lambda) pairs."""
+    import argparse
+    import multiprocessing as mp
+    import sys
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--patches", type=int, default=2000)
+    ap.add_argument("--size", type=int, default=128)
+    ap.add_argument("--nz", type=int, default=40)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=1000)
+    ap.add_argument("--workers", type=int, default=12)
+    ap.add_argument("--time-budget", type=float, default=None, help="stop training after this many seconds")
+    ap.add_argument("--init", default=None, help="start from these parameters (.npy)")
+    ap.add_argument("--out", required=True)
+    args = ap.parse_args(argv)
+    log = lambda m: print(f"[learned] {m}", flush=True)  # noqa: E731
+    # worker processes are forked before this process touches the GPU
+    pool = mp.get_context("fork").Pool(args.workers)
+    import torch
+
+    from .wind_iterative import WindSolver
+
+    t0 = time.time()
+    inputs, targets, iters = [], [], []
+    jobs = [(args.seed + q, args.size, args.nz) for q in range(args.patches)]
+    solver = WindSolver(args.size, args.size, args.nz, 50.0, device_fields=True)
+    for q, case in enumerate(pool.imap(_patch, jobs, chunksize=2)):
+        inp, tgt, info = make_pair(case, solver, "cuda")
+        if info["status"] != 0:
+            log(f"patch {q}: plain solve status {info['status']}, skipped")
+            continue
+        inputs.append(inp)
+        targets.append(tgt)
+        iters.append(info["iters"])
+        if (q + 1) % 100 == 0:
+            log(f"{q + 1}/{args.patches} pairs ({time.time() - t0:.0f}s, mean {np.mean(iters):.0f} FP64 iterations)")
+    pool.close()
+    solver.close()
+    log(f"{len(inputs)} pairs in {time.time() - t0:.0f}s")
+    net = Net(np.load(args.init, allow_pickle=False) if args.init else None)
+    hist = train(net, inputs, targets, args.epochs, args.batch, args.lr, seed=args.seed, log=log,
+                 time_budget_s=args.time_budget)
+    net.save(args.out)
+    log(f"saved {args.out}; loss {hist[0]:.4e} -> {hist[-1]:.4e}")
+    del inputs, targets
+    torch.cuda.empty_cache()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

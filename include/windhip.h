@@ -195,9 +195,13 @@ int windhip_net_adam_step(windhip_net *net, float lr, float beta1, float beta2, 
 /* Mean device time of the forward pass over a B x D x H x W volume */
 int windhip_net_time_forward(windhip_net *net, int B, int D, int H, int W, int reps, double *ms);
 /* Attach a network to a solver: opts.start = WINDHIP_START_LEARNED then
- * starts from the network's guess when its FP64 residual is below the plain
- * start's (the current lambda), and from the plain start otherwise. */
+ * starts from the network's guess, after `steps` (default 4) vertical
+ * line-relaxation steps lambda += M^-1 (b - A lambda) with M the tridiagonal
+ * column part of A, when its FP64 residual is below the plain start's (the
+ * current lambda), and from the plain start otherwise. A network whose
+ * parameters are all zero gives the relaxation of the zero start alone. */
 int windhip_set_net(windhip_solver *s, windhip_net *net);
+int windhip_set_guess_smoothing(windhip_solver *s, int steps);
 /* The network input for the solver's current rhs (that of its last solve),
  * metrics and alpha, written to in_dev (1 x dims[0] x dims[1] x dims[2] x 8);
  * *scale turns the network output into lambda. dims_out receives the padded
