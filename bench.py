@@ -5,15 +5,22 @@ seeded fractal terrain (k^-2 spectrum, 0-2500 m relief), 40 stretched levels
 10 m -> 12 km, power-law winds 8 m/s at 10 m with 25 deg direction shear plus
 N(0, 1 m/s) noise, alpha = 1, rho = 1. Inputs (metrics, alpha, divergence) are
 resident in HBM before the timed region. One step = one complete solve
-(operator refresh, diagonal, rhs, Krylov iterations, true residual) from a zero
-start.
+(operator refresh, diagonal, rhs, start selection, Krylov iterations, true
+residual) from a zero lambda.
 
-Arms: the headline path (FP32 inner BiCGStab inside FP64 iterative
-refinement) gives `value`; the plain FP64 BiCGStab is timed beside it. The
-per-kernel device times come from hipEvents recorded around every hot kernel
-launch on the solver's stream inside the timed region. The CPU baseline is the
-oracle (the reference algorithm restated in C, OpenMP) timed on this host on a
-bounded sample and scaled to a full solve.
+Arms (each solves the same system to the same tolerance):
+  plain          the reference's BiCGStab in FP64 (2 Jacobi-Richardson sweeps)
+  mixed          FP64 FGMRES around that BiCGStab in FP32
+  line           FP64 FGMRES around an FP32 BiCGStab preconditioned by two
+                 vertical line-relaxation steps
+  line_learned   `line` started from the learned guess (the headline `value`)
+  mixed_learned  `mixed` started from the learned guess
+Errors of every arm against the plain FP64 solution are reported for lambda,
+the corrected u, v, w and the divergence after the correction. Per-kernel
+device times come from hipEvents recorded around every hot launch on the
+solver's stream (one extra profiled solve per arm). The CPU baseline is the
+oracle (the reference algorithm restated in C, OpenMP) timed on this host on
+a bounded sample and scaled to a full solve.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -30,6 +37,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ms per wind solve to rel. residual 1e-8 at 512²×40: learned+FP32 vs plain FP64"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+WEIGHTS = os.path.join(ROOT, "hicar_amd", "data", "lnet_params.npy")
+TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic.json")
 
 
 def kernel_bytes(kind: str, nx: int, ny: int, nz: int) -> float:
@@ -39,22 +49,34 @@ def kernel_bytes(kind: str, nx: int, ny: int, nz: int) -> float:
     cells = nx * ny * nz
     P = (nx + 2) * (ny + 2)
     N = P * (nz + 2)
-    t = 8 if kind.endswith("f64") or kind in ("ir_resid", "true_resid") else 4
+    t = 8 if kind.endswith("f64") or kind in ("ir_resid", "true_resid", "fg_matvec", "fg_axpy_dot") else 4
     coef = 6 * t * cells + 4 * t * P
     table = {
         "sweep": 3 * N * t + coef,        # y (stencil), in, coefficients -> out
         "sweep_sigma": 5 * N * t + coef,  # p_hat1 (stencil), p, r_hat, coefficients -> p_hat, v
         "sweep_omega": 5 * N * t + coef,  # s_hat1 (stencil), s, r_hat, coefficients -> s_hat, t
-        "update_s": 5 * N * t,           # r, v, D^-1 -> s, s_hat
-        "update_xrp": 12 * N * t,        # x, p_hat, s_hat, s, t, p, v, D^-1 -> x, r, p, p_hat
-        "init": 7 * N * t + coef,        # x (stencil), b, D^-1, coefficients -> r, r_hat, p, p_hat
+        "update_s": 5 * N * t,            # r, v, D^-1 -> s, s_hat
+        "update_xrp": 12 * N * t,         # x, p_hat, s_hat, s, t, p, v, D^-1 -> x, r, p, p_hat
+        "init": 7 * N * t + coef,         # x (stencil), b, D^-1, coefficients -> r, r_hat, p, p_hat
+        "line_solve": 7 * N * t,          # in, lo, 1/pivot -> dq; dq, cp -> out
+        "line_step": 9 * N * t + coef,    # y (stencil), in, lo, 1/pivot, coefficients -> dq; dq, cp, y -> out
+        "mv_sigma": 3 * N * t + coef,     # p_hat (stencil), r_hat, coefficients -> v
+        "mv_omega": 4 * N * t + coef,     # s_hat (stencil), s, r_hat, coefficients -> t
     }
-    if kind == "ir_resid":   # x, b (FP64), D32^-1, FP64 coefficients -> r32, rh32, p32, ph32
-        return N * (8 + 8 + 4 + 16) + 6 * 8 * cells + 4 * 8 * P
-    if kind == "ir_accumulate":
-        return N * 24
+    if kind == "ir_resid":    # lambda (stencil), b, FP64 coefficients -> v0
+        return 3 * N * 8 + 6 * 8 * cells + 4 * 8 * P
+    if kind == "ir_accumulate":  # lambda += sum of y_q z_q: lambda in/out + the FP32 directions (counted once)
+        return N * 16 + N * 4
     if kind == "true_resid":
         return 2 * N * 8 + 6 * 8 * cells + 4 * 8 * P
+    if kind == "fg_setup":    # v (FP64 in/out) -> x32, r32, rh32, p32, ph32 and D32^-1
+        return N * (16 + 4 * 6)
+    if kind == "fg_matvec":   # z (FP64 widened), v0, coefficients -> w
+        return 3 * N * 8 + 6 * 8 * cells + 4 * 8 * P
+    if kind == "fg_axpy_dot":  # w (in/out), v_i, v_{i+1}
+        return 4 * N * 8
+    if kind == "update_s_line_f32":  # r, v -> s
+        return 3 * N * 4
     base = kind.rsplit("_", 1)[0]
     return float(table[base])
 
@@ -97,9 +119,9 @@ def main():
     ap.add_argument("--ny", type=int, default=512)
     ap.add_argument("--nz", type=int, default=40)
     ap.add_argument("--inner-tol", type=float, default=0.5)
+    ap.add_argument("--arms", default="plain,mixed,line,line_learned,mixed_learned")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=6)
-    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 arm")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy a 1 GiB device buffer three times (known traffic for counter calibration)")
     args = ap.parse_args()
@@ -109,13 +131,15 @@ def main():
         sys.exit(2)
 
     import torch
+    from hicar_amd import _native as nat
+    from hicar_amd import learned
     from hicar_amd.wind_iterative import WindSolver
 
+    log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
     dev = torch.device("cuda", 0)
     t_gen = time.perf_counter()
     case = build_case(args.nx, args.ny, args.nz)
-    t_gen = time.perf_counter() - t_gen
-    print(f"[bench] case {case.nx}x{case.ny}x{case.nz} generated in {t_gen:.1f}s", file=sys.stderr, flush=True)
+    log(f"case {case.nx}x{case.ny}x{case.nz} generated in {time.perf_counter() - t_gen:.1f}s")
 
     def D(a):
         return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
@@ -135,90 +159,129 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-        print("[bench] copy calibration: 3 x 1 GiB device copies", file=sys.stderr, flush=True)
+        log("copy calibration: 3 x 1 GiB device copies")
 
+    net = learned.Net.load(WEIGHTS) if os.path.exists(WEIGHTS) else None
     arms = {
-        "mixed": dict(precision=1, start=1, tol_rel=1e-8, inner_tol=args.inner_tol, max_iters=50000, profile=1),
-        "plain": dict(precision=0, start=1, tol_rel=1e-8, max_iters=50000, profile=1),
+        "plain": dict(precision=nat.PLAIN_F64, start=nat.START_ZERO),
+        "mixed": dict(precision=nat.MIXED, start=nat.START_ZERO),
+        "line": dict(precision=nat.MIXED_LINE, start=nat.START_ZERO),
+        "line_learned": dict(precision=nat.MIXED_LINE, start=nat.START_LEARNED),
+        "mixed_learned": dict(precision=nat.MIXED, start=nat.START_LEARNED),
     }
+    want = [a for a in args.arms.split(",") if a in arms and (net is not None or "learned" not in a)]
+    zeros = np.zeros(s.rows_shape)
 
-    def timed(arm):
-        for _ in range(args.warmup):
-            s.solve(alpha, div, **arms[arm])
-        s.profile(reset=True)
+    def solve(arm, profile=0):
+        o = arms[arm]
+        learned.attach(s, net if o["start"] == nat.START_LEARNED else None)
+        s.set_lambda(zeros)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        info = None
-        for _ in range(args.steps):
-            info = s.solve(alpha, div, **arms[arm])
+        info = s.solve(alpha, div, tol_rel=1e-8, max_iters=50000, inner_tol=args.inner_tol, profile=profile, **o)
         torch.cuda.synchronize()
-        ms = 1e3 * (time.perf_counter() - t0) / args.steps
-        prof = s.profile(reset=True)
-        lam = s.get_lambda()
-        print(f"[bench] {arm}: {ms:.1f} ms/solve iters={info['iters']} outer={info['outer_iters']} "
-              f"status={info['status']} true_rel_res={info['true_rel_res']:.3e}", file=sys.stderr, flush=True)
-        return ms, info, prof, lam
+        return 1e3 * (time.perf_counter() - t0), info
 
-    ms_m, info_m, prof_m, lam_m = timed("mixed")
+    def winds():
+        uu, vv, ww = u.clone(), v.clone(), w.clone()
+        s.apply_correction(uu, vv, ww, density=rho, adv_den=True)
+        dd = torch.empty_like(div)
+        s.divergence(uu, vv, ww, density=rho, adv_den=True, use_dqdt=True, out=dd)
+        return [t.cpu().numpy() for t in (uu, vv, ww, dd)]
+
     res = {}
-    if not args.no_plain:
-        ms_p, info_p, prof_p, lam_p = timed("plain")
-        err = float(np.linalg.norm(lam_m - lam_p) / np.linalg.norm(lam_p))
-        res.update(plain_f64_ms=ms_p, plain_iters=info_p["iters"], plain_true_rel_res=info_p["true_rel_res"],
-                   speedup_vs_plain=ms_p / ms_m, lambda_rel_err_vs_plain=err)
-    else:
-        prof_p, info_p = {}, None
+    ref = None
+    for arm in want:
+        for _ in range(args.warmup):
+            solve(arm)
+        times, info = [], None
+        for _ in range(args.steps):
+            ms, info = solve(arm)
+            times.append(ms)
+        lam = s.get_lambda()
+        wnd = winds()
+        s.profile(reset=True)
+        solve(arm, profile=1)
+        prof = s.profile(reset=True)
+        r = dict(ms=float(np.mean(times)), iters=info["iters"], outer=info["outer_iters"], status=info["status"],
+                 true_rel_res=info["true_rel_res"], start_used=info["start_used"], prof=prof)
+        if arm == "plain":
+            ref = (lam, wnd)
+        elif ref is not None:
+            rel = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))  # noqa: E731
+            r["err_vs_plain"] = dict(lam=rel(lam, ref[0]), u=rel(wnd[0], ref[1][0]), v=rel(wnd[1], ref[1][1]),
+                                     w=rel(wnd[2], ref[1][2]), div=rel(wnd[3], ref[1][3]))
+        if "learned" in arm:
+            r.update(guess_res=info["guess_res"], plain_res=info["plain_res"])
+        res[arm] = r
+        log(f"{arm}: {r['ms']:.1f} ms/solve iters={r['iters']} outer={r['outer']} status={r['status']} "
+            f"true_rel_res={r['true_rel_res']:.3e} start_used={r['start_used']}")
 
-    def kernel_table(prof):
+    head = "line_learned" if "line_learned" in res else ("line" if "line" in res else next(iter(res)))
+    ms_h = res[head]["ms"]
+
+    def kernel_table(prof, arm):
         out = {}
         for kind, (calls, tot) in prof.items():
+            if arm.startswith("line") and kind == "update_s_f32":
+                kind = "update_s_line_f32"  # s = r - alpha v only, without the Jacobi sweep
             avg = tot / calls
             gbs = kernel_bytes(kind, case.nx, case.ny, case.nz) / (avg * 1e-3) / 1e9
             out[kind] = dict(calls=int(calls), avg_ms=round(avg, 5), total_ms=round(tot, 2), GBs=round(gbs, 1),
                              frac=round(gbs / HBM_PEAK_GBS, 3))
         return out
 
-    km = kernel_table(prof_m)
-    dom = max(km, key=lambda k: km[k]["total_ms"])
-    d = km[dom]
-    roof = dict(bound="hbm", achieved=d["GBs"], peak=HBM_PEAK_GBS, unit="GB/s", frac=d["frac"], traffic=None,
+    tables = {arm: kernel_table(r.pop("prof"), arm) for arm, r in res.items()}
+    kh = tables[head]
+    dom = max(kh, key=lambda k: kh[k]["total_ms"])
+    d = kh[dom]
+    traffic = None
+    if os.path.exists(TRAFFIC):
+        with open(TRAFFIC) as f:
+            tr = json.load(f).get("kernels", {}).get(dom)
+        if tr:
+            traffic = tr["bytes_per_launch"]
+    roof = dict(bound="hbm", achieved=d["GBs"], peak=HBM_PEAK_GBS, unit="GB/s", frac=d["frac"], traffic=traffic,
                 kernel=dom, bytes_per_launch=kernel_bytes(dom, case.nx, case.ny, case.nz), avg_ms=d["avg_ms"])
 
-    cpu = None
-    if not args.no_cpu:
-        div_host = div.cpu().numpy()
-        iters_full = info_p["iters"] if info_p else info_m["iters"]
-        cpu = cpu_baseline(case, div_host, iters_full, args.cpu_iters)
-        print(f"[bench] cpu baseline: {cpu['value']:.0f} ms/solve ({cpu['cores']} threads)", file=sys.stderr,
-              flush=True)
+    netinfo = None
+    if net is not None:
+        fms = net.time_forward(case.nz, case.ny, case.nx, reps=3)
+        fl = learned.conv_flops(case.nz, case.ny, case.nx)
+        netinfo = dict(forward_ms=round(fms, 3), forward_tflop=round(fl / 1e12, 3),
+                       tflops=round(fl / fms / 1e9, 1), mfma_util=round(fl / fms / 1e9 / BF16_PEAK_TFLOPS, 4))
 
+    cpu = None
+    if not args.no_cpu and "plain" in res:
+        cpu = cpu_baseline(case, div.cpu().numpy(), res["plain"]["iters"], args.cpu_iters)
+        log(f"cpu baseline: {cpu['value']:.0f} ms/solve ({cpu['cores']} threads)")
+
+    plain_ms = res["plain"]["ms"] if "plain" in res else None
     line = {
         "metric": METRIC,
-        "value": round(ms_m, 3),
+        "value": round(ms_h, 3),
         "unit": "ms/solve",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_m, 3),
+        "ms_per_step": round(ms_h, 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"wind solve {case.nx}x{case.ny}x{case.nz} fractal terrain, rel. residual 1e-8, "
-                               "FP32 inner BiCGStab in FP64 refinement, zero start (learned start not built yet)",
-                   "grid": [case.nx, case.ny, case.nz], "dx_m": case.dx, "tol_rel": 1e-8,
-                   "inner_tol": args.inner_tol, "cells": case.cells},
+                               f"arm '{head}'", "grid": [case.nx, case.ny, case.nz], "dx_m": case.dx,
+                   "tol_rel": 1e-8, "inner_tol": args.inner_tol, "cells": case.cells,
+                   "learned_weights": os.path.relpath(WEIGHTS, ROOT) if net is not None else None},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "iters": info_m["iters"],
-        "outer_iters": info_m["outer_iters"],
-        "true_rel_res": info_m["true_rel_res"],
-        "cells_per_s": case.cells / (ms_m * 1e-3),
-        "kernels_mixed": km,
-        "kernels_plain": kernel_table(prof_p) if prof_p else None,
+        "speedup_vs_plain": round(plain_ms / ms_h, 3) if plain_ms else None,
+        "cells_per_s": case.cells / (ms_h * 1e-3),
+        "arms": res,
+        "network": netinfo,
+        "kernels": tables,
     }
-    line.update(res)
     print(json.dumps(line))
 
 

@@ -115,6 +115,8 @@ struct windhip_solver {
     long net_vox = 0;
     int net_smooth = 4;                        // line-relaxation steps applied to the network's guess
     double *lr_c = nullptr, *lr_d = nullptr;   // line relaxation: Thomas coefficients
+    // MIXED_LINE: FP32 Thomas factors of the column tridiagonal and scratch
+    float *lf_lo = nullptr, *lf_cp = nullptr, *lf_ip = nullptr, *lf_dq = nullptr;
     bool metrics_set = false, structure_uploaded = false, probed = false;
     bool verbose = false;  // WINDHIP_VERBOSE=1: per-batch progress on stderr (verbose_solver, :274-275)
     int nsweeps = 2;  // precond_n_sweeps, BASE_PREC_SWEEPS (wind_iterative.F90:76)
@@ -263,11 +265,11 @@ struct windhip_solver {
     int read_state(int idx);
     int run_plain(int max_iters, double tol_rel, double tol_abs, int batch, bool use_graph, int *status,
                   int *iters, double *res0, double *resf);
-    int run_mixed(const windhip_solve_opts &o, windhip_solve_info *info);
+    int run_mixed(const windhip_solve_opts &o, windhip_solve_info *info, const double *first_dir = nullptr);
     int true_residual(double *rel, const double *xv = nullptr);
     int net_dims(int *Dp, int *Hp, int *Wp) const;
     int net_input(float *in, double *scale);
-    int learned_start(double *res_guess, double *res_plain, bool *used);
+    int learned_start(double *res_guess, double *res_plain, double **guess);
     int line_relax(double **y, double **scratch, int steps);
     template <typename T>
     int enqueue_iterations(int count);
@@ -282,6 +284,7 @@ struct windhip_solver {
     int ev_next = 0;
     double prof_ms[WINDHIP_PROFILE_KINDS] = {}, prof_calls[WINDHIP_PROFILE_KINDS] = {};
     int launch_batch(int precision, int count, bool use_graph);
+    int enqueue_iterations_line(int count);
     int alloc_f32_krylov();
     int alloc_fgmres(int m);
     int correction(void *u, void *v, void *w, const void *density, int adv_den);
@@ -490,7 +493,8 @@ int windhip_solver::read_state(int idx)
 // the handle's stream): base + 5 * precision (0 FP64, 1 FP32).
 enum { PK_SWEEP = 0, PK_SIGMA = 1, PK_UPD_S = 2, PK_OMEGA = 3, PK_UPD_XRP = 4, PK_INIT = 10,
        PK_IR_RESID = 11, PK_IR_ACC = 12, PK_TRUE_RESID = 13, PK_FG_SETUP = 14, PK_FG_MATVEC = 15,
-       PK_FG_AXPY_DOT = 16, PK_COUNT = 17 };
+       PK_FG_AXPY_DOT = 16, PK_LINE_SOLVE = 17, PK_LINE_STEP = 18, PK_MV_SIGMA = 19, PK_MV_OMEGA = 20,
+       PK_COUNT = 21 };
 
 template <class Launch>
 int windhip_solver::timed(int kind, Launch &&launch)
@@ -579,17 +583,64 @@ int windhip_solver::enqueue_iterations(int count)
     });
 }
 
+// One line-preconditioned FP32 BiCGStab iteration per repetition: p_hat =
+// M^-1 p (line solve, line step), v = A p_hat with sigma, s = r - alpha v,
+// s_hat = M^-1 s, t = A s_hat with the five dots, the x/r/p update.
+int windhip_solver::enqueue_iterations_line(int count)
+{
+    KState *sp = st + 1;
+    float *X, *R, *RH, *Pv, *V, *S, *Tv, *PH[3], *SH[3];
+    const float *DI;
+    kvec<float>(&X, &R, &RH, &Pv, &V, &S, &Tv, PH, SH, &DI);
+    const int fg = flat_grid(N);
+    return with_op<float>([&](auto op) -> int {
+        using Op = decltype(op);
+        for (int it = 0; it < count; ++it) {
+            CHK(timed(PK_LINE_SOLVE, [&] {
+                k_line_solve<<<g.nt, kThreads, 0, stream>>>(g, sp, Pv, lf_lo, lf_cp, lf_ip, lf_dq, PH[1]);
+            }));
+            CHK(timed(PK_LINE_STEP, [&] {
+                k_line_step<Op><<<g.nt, kThreads, 0, stream>>>(op, g, sp, PH[1], Pv, lf_lo, lf_cp, lf_ip, lf_dq,
+                                                               PH[2]);
+            }));
+            CHK(timed(PK_MV_SIGMA, [&] {
+                k_mv_sigma<Op><<<g.nt, kThreads, 0, stream>>>(op, g, sp, partials, PH[2], RH, V);
+            }));
+            CHK(timed(5 + PK_UPD_S, [&] { k_update_s_only<float><<<fg, kBlock, 0, stream>>>(N, sp, R, V, S); }));
+            CHK(timed(PK_LINE_SOLVE, [&] {
+                k_line_solve<<<g.nt, kThreads, 0, stream>>>(g, sp, S, lf_lo, lf_cp, lf_ip, lf_dq, SH[1]);
+            }));
+            CHK(timed(PK_LINE_STEP, [&] {
+                k_line_step<Op><<<g.nt, kThreads, 0, stream>>>(op, g, sp, SH[1], S, lf_lo, lf_cp, lf_ip, lf_dq,
+                                                               SH[2]);
+            }));
+            CHK(timed(PK_MV_OMEGA, [&] {
+                k_mv_omega<Op><<<g.nt, kThreads, 0, stream>>>(op, g, sp, partials, SH[2], S, RH, Tv);
+            }));
+            CHK(timed(5 + PK_UPD_XRP, [&] {
+                k_update_xrp<float><<<fg, kBlock, 0, stream>>>(N, sp, X, PH[2], SH[2], S, Tv, R, Pv, V, DI, PH[0]);
+            }));
+        }
+        HIPCHK(hipGetLastError());
+        return WINDHIP_OK;
+    });
+}
+
 // Launch `count` iterations, from a cached hipGraph when use_graph.
 int windhip_solver::launch_batch(int precision, int count, bool use_graph)
 {
-    if (!use_graph || profiling)
-        return precision == 0 ? enqueue_iterations<double>(count) : enqueue_iterations<float>(count);
+    auto enqueue = [&]() {
+        if (precision == 0) return enqueue_iterations<double>(count);
+        if (precision == 2) return enqueue_iterations_line(count);
+        return enqueue_iterations<float>(count);
+    };
+    if (!use_graph || profiling) return enqueue();
     const long key = ((long)count << 8) | ((long)nsweeps << 4) | ((long)precision << 1) | (probed ? 1 : 0);
     auto it = graphs.find(key);
     if (it == graphs.end()) {
         hipGraph_t graph;
         HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        int rc = precision == 0 ? enqueue_iterations<double>(count) : enqueue_iterations<float>(count);
+        int rc = enqueue();
         hipError_t e = hipStreamEndCapture(stream, &graph);
         if (rc != WINDHIP_OK) return rc;
         if (e != hipSuccess) return fail(WINDHIP_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
@@ -694,11 +745,12 @@ int windhip_solver::line_relax(double **y, double **scratch, int steps)
     return WINDHIP_OK;
 }
 
-// Start selection: the network's guess, after net_smooth vertical
-// line-relaxation steps, and the plain start (the current lambda: the previous
-// solution, or zero) are both evaluated by their FP64 true residual; the lower
-// one becomes the starting lambda.
-int windhip_solver::learned_start(double *res_guess, double *res_plain, bool *used)
+// The learned guess: the network's lambda after net_smooth vertical
+// line-relaxation steps (FP64), and the FP64 true residuals of that guess and
+// of the plain start (the current lambda: the previous solution, or zero).
+// The plain solve starts from the lower of the two; the mixed solves take
+// guess - lambda as the first direction of their first FGMRES cycle.
+int windhip_solver::learned_start(double *res_guess, double *res_plain, double **guess)
 {
     int Dp, Hp, Wp;
     net_dims(&Dp, &Hp, &Wp);
@@ -719,8 +771,7 @@ int windhip_solver::learned_start(double *res_guess, double *res_plain, bool *us
     CHK(line_relax(&gy, &gs, net_smooth));
     CHK(true_residual(&rel, gy));
     *res_guess = st_host[0].outer_rnorm;
-    *used = *res_guess < *res_plain;
-    if (*used) HIPCHK(hipMemcpyAsync(x, gy, N * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    *guess = gy;
     return WINDHIP_OK;
 }
 
@@ -731,9 +782,23 @@ int windhip_solver::learned_start(double *res_guess, double *res_plain, bool *us
 // and the true residual are FP64; FGMRES minimises the FP64 residual over all
 // directions of a cycle, so an inaccurate inner solve slows it down but cannot
 // make the residual grow.
-int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *info)
+int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *info, const double *first_dir)
 {
     CHK(alloc_f32_krylov());
+    const bool line = o.precision == WINDHIP_MIXED_LINE;
+    if (line) {
+        if (!lf_lo) {
+            CHK(alloc(&lf_lo, N));
+            CHK(alloc(&lf_cp, N));
+            CHK(alloc(&lf_ip, N));
+            CHK(alloc(&lf_dq, N));
+        }
+        CHK(with_op<double>([&](auto op) {
+            k_line_factor<decltype(op)><<<g.nt, kThreads, 0, stream>>>(op, g, lf_lo, lf_cp, lf_ip);
+            return WINDHIP_OK;
+        }));
+        HIPCHK(hipGetLastError());
+    }
     const double inner_tol = (o.inner_tol > 0 && o.inner_tol < 1) ? o.inner_tol : 0.5;
     const int m = std::max(1, std::min(o.max_outer, kRestart));
     CHK(alloc_fgmres(m));
@@ -762,6 +827,18 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
         int k = 0;
         bool breakdown = false;
         for (int j = 0; j < m && steps < o.max_outer && total_inner < o.max_iters; ++j) {
+            if (j == 0 && first_dir) {
+                // augmented first direction (the learned correction): no inner
+                // solve; the set-up still normalises v_0 in place
+                k_fg_setup<<<fb, kThreads, 0, stream>>>(N, st + 1, fg_part, beta2, 1, fg_v[0], dinv32, x32, r32,
+                                                        rh32, p32, ph32[0]);
+                k_fg_take64<<<flat_grid(N), kBlock, 0, stream>>>(N, first_dir, fg_z[0], fg_z64);
+                HIPCHK(hipGetLastError());
+                first_dir = nullptr;
+                ++steps;
+                goto have_direction;
+            }
+            {
             // z_j ~ A^-1 v_j in FP32 (v_j normalised in place first). An inner
             // solve whose FP32 recurrence overflowed is run again from z = 0 with
             // half the iterations it had reached, keeping a finite iterate.
@@ -778,7 +855,7 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
                 int done_iters = 0;
                 while (!st_host[1].done) {
                     const int n = std::min(o.iters_per_sync, std::max(1, cap - done_iters));
-                    CHK(launch_batch(1, n, o.use_graph != 0));
+                    CHK(launch_batch(line ? 2 : 1, n, o.use_graph != 0));
                     CHK(read_state(1));
                     done_iters += n;
                     if (done_iters > cap + o.iters_per_sync) break;
@@ -795,8 +872,10 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
             if (breakdown) { total_inner += spent; ++steps; break; }
             total_inner += spent;
             ++steps;
-            double *hc = fg_h + (long)j * H;
             k_fg_take<<<flat_grid(N), kBlock, 0, stream>>>(N, x32, fg_z[j], fg_z64);
+            }
+        have_direction:
+            double *hc = fg_h + (long)j * H;
             // w = A z_j, orthogonalised against v_0..v_j (modified Gram-Schmidt)
             CHK(with_op<double>([&](auto op) {
                 return timed(PK_FG_MATVEC, [&] {
@@ -1130,19 +1209,29 @@ int windhip_solve(windhip_solver *s, const void *alpha, const void *div, const w
     CHK(s->prepare_call(alpha, div));
     if (o.start == WINDHIP_START_ZERO) HIPCHK(hipMemsetAsync(s->x, 0, s->N * sizeof(double), s->stream));
     inf.start_used = o.start;
+    const bool mixed = o.precision == WINDHIP_MIXED || o.precision == WINDHIP_MIXED_LINE;
+    double *aug = nullptr;  // mixed: the learned correction as the first FGMRES direction
     if (o.start == WINDHIP_START_LEARNED) {
-        double rg = 0, rp = 0;
-        bool used = false;
-        CHK(s->learned_start(&rg, &rp, &used));
-        inf.start_used = used ? WINDHIP_START_LEARNED : WINDHIP_START_WARM;
+        double rg = 0, rp = 0, *guess = nullptr;
+        CHK(s->learned_start(&rg, &rp, &guess));
         inf.guess_res = rg;
         inf.plain_res = rp;
+        if (mixed) {
+            k_axpby<<<flat_grid(s->N), kBlock, 0, s->stream>>>(s->N, 1.0, guess, -1.0, s->x, guess);
+            HIPCHK(hipGetLastError());
+            aug = guess;
+            inf.start_used = WINDHIP_START_LEARNED;
+        } else {
+            const bool used = rg < rp;
+            if (used) HIPCHK(hipMemcpyAsync(s->x, guess, s->N * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+            inf.start_used = used ? WINDHIP_START_LEARNED : WINDHIP_START_WARM;
+        }
         if (s->verbose)
             fprintf(stderr, "[windhip] learned start: residual %.3e vs plain start %.3e -> %s\n", rg, rp,
-                    used ? "network" : "plain");
+                    mixed ? "first FGMRES direction" : (inf.start_used == WINDHIP_START_LEARNED ? "network" : "plain"));
     }
-    if (o.precision == WINDHIP_MIXED) {
-        CHK(s->run_mixed(o, &inf));
+    if (mixed) {
+        CHK(s->run_mixed(o, &inf, aug));
         inf.nsweeps = s->nsweeps;
     } else {
         int status = 1, iters = 0;

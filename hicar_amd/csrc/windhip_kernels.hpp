@@ -663,6 +663,41 @@ __device__ __forceinline__ void fused_sweep_matvec(const Op &op, const Geo &g, c
     }
 }
 
+// ------------------------------------------------------------------ iteration scalars
+// sigma = <r_hat, v>, alpha = rho / sigma (wind_iterative.F90:649-688)
+__device__ __forceinline__ void finish_sigma(KState *st, const double *tot)
+{
+    const int it = st->iter + 1;
+    st->iter = it;
+    st->sigma = tot[0];
+    st->fmode = 0;
+    if (fabs(tot[0]) < kBreakdownEps) { st->status = 2; st->done = 1; }
+    else st->alpha = st->rho / tot[0];
+}
+
+// omega, rho_new, ||r|| from <t,s>, <t,t>, <r_hat,s>, <r_hat,t>, <s,s>; the
+// stop rule and breakdown guards (wind_iterative.F90:695-785)
+__device__ __forceinline__ void finish_omega(KState *st, const double *tot)
+{
+    const double ts = tot[0], tt = tot[1], rs = tot[2], rt = tot[3], ss = tot[4];
+    st->ts = ts; st->tt = tt; st->rs = rs; st->rt = rt; st->ss = ss;
+    if (tt < kBreakdownEps) { st->status = 2; st->done = 1; st->fmode = 0; return; }
+    const double omega = ts / tt;
+    const double rho_new = rs - omega * rt;
+    double rn2 = (ss - (2.0 * omega) * ts) + (omega * omega) * tt;
+    if (rn2 < 0.0) rn2 = 0.0;
+    st->omega = omega;
+    st->rnorm = sqrt(rn2);
+    if (st->rnorm <= st->target) { st->status = 0; st->done = 1; st->fmode = 1; return; }
+    if (fabs(omega) < kBreakdownEps) { st->status = 2; st->done = 1; st->fmode = 1; return; }
+    if (fabs(st->rho) < kBreakdownEps) { st->status = 2; st->done = 1; st->fmode = 2; return; }
+    st->beta = (rho_new / st->rho) * (st->alpha / omega);
+    st->rho = rho_new;
+    st->fmode = 3;
+    if (st->iter >= st->max_iters) { st->status = 1; st->done = 1; }
+    if (!(st->rnorm == st->rnorm) || st->rnorm > 1.0e300) { st->status = 2; st->done = 1; }
+}
+
 // ------------------------------------------------------------------ kernels
 // y = A x (spmv)
 template <class Op>
@@ -768,14 +803,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep_sigma(Op op, Geo g, KState *
     } epi{rh, v, {0.0}};
     fused_sweep_matvec(op, g, t, ph_in, p, ph, epi);
     double (&acc)[1] = epi.acc;
-    reduce_finalize<1>(acc, partials, &st->cnt[1], t.L, g.nt, [&](const double *tot) {
-        const int it = st->iter + 1;
-        st->iter = it;
-        st->sigma = tot[0];
-        st->fmode = 0;
-        if (fabs(tot[0]) < kBreakdownEps) { st->status = 2; st->done = 1; }
-        else st->alpha = st->rho / tot[0];
-    });
+    reduce_finalize<1>(acc, partials, &st->cnt[1], t.L, g.nt, [&](const double *tot) { finish_sigma(st, tot); });
 }
 
 // s = r - alpha v (vec_axpby_into, :690) and the first sweep s_hat = D^-1 s.
@@ -831,25 +859,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep_omega(Op op, Geo g, KState *
     } epi{s, rh, tt_out, {0.0, 0.0, 0.0, 0.0, 0.0}};
     fused_sweep_matvec(op, g, t, sh_in, s, sh, epi);
     double (&acc)[5] = epi.acc;
-    reduce_finalize<5>(acc, partials, &st->cnt[2], t.L, g.nt, [&](const double *tot) {
-        const double ts = tot[0], tt = tot[1], rs = tot[2], rt = tot[3], ss = tot[4];
-        st->ts = ts; st->tt = tt; st->rs = rs; st->rt = rt; st->ss = ss;
-        if (tt < kBreakdownEps) { st->status = 2; st->done = 1; st->fmode = 0; return; }
-        const double omega = ts / tt;
-        const double rho_new = rs - omega * rt;
-        double rn2 = (ss - (2.0 * omega) * ts) + (omega * omega) * tt;
-        if (rn2 < 0.0) rn2 = 0.0;
-        st->omega = omega;
-        st->rnorm = sqrt(rn2);
-        if (st->rnorm <= st->target) { st->status = 0; st->done = 1; st->fmode = 1; return; }
-        if (fabs(omega) < kBreakdownEps) { st->status = 2; st->done = 1; st->fmode = 1; return; }
-        if (fabs(st->rho) < kBreakdownEps) { st->status = 2; st->done = 1; st->fmode = 2; return; }
-        st->beta = (rho_new / st->rho) * (st->alpha / omega);
-        st->rho = rho_new;
-        st->fmode = 3;
-        if (st->iter >= st->max_iters) { st->status = 1; st->done = 1; }
-        if (!(st->rnorm == st->rnorm) || st->rnorm > 1.0e300) { st->status = 2; st->done = 1; }
-    });
+    reduce_finalize<5>(acc, partials, &st->cnt[2], t.L, g.nt, [&](const double *tot) { finish_omega(st, tot); });
 }
 
 // x += alpha p_hat + omega s_hat (vec_axpy2, :755); r = s - omega t (:771);
@@ -951,6 +961,147 @@ __global__ __launch_bounds__(kThreads) void k_line_relax(Op op, Geo g, const typ
     }
 }
 
+// ---------------------------------------------- line-preconditioned FP32 BiCGStab
+// Inner solver of WINDHIP_MIXED_LINE: the reference's BiCGStab iteration with
+// the preconditioner M^-1 = two vertical line-relaxation steps
+// (y1 = L^-1 in, y2 = y1 + L^-1 (in - A y1), L the tridiagonal column part of
+// A) instead of the two Jacobi-Richardson sweeps.
+
+// Thomas factors of L per column, computed in FP64 from the FP64 operator and
+// stored in FP32: lo = coefficient of x(k-1), ip = 1 / pivot, cp = upper
+// coefficient / pivot.
+template <class Op>
+__global__ __launch_bounds__(kThreads) void k_line_factor(Op op, Geo g, float *__restrict__ lo,
+                                                          float *__restrict__ cp, float *__restrict__ ip)
+{
+    const TilePos t = tile_pos(g);
+    if (!t.inner) return;
+    const typename Op::Bc bc = op.bc(t.c);
+    double cprev = 0.0;
+    for (int k = 0; k < g.mz; ++k) {
+        const long n = (long)k * g.P + t.c;
+        typename Op::Cf cf = {};
+        if (k >= 1 && k <= g.mz - 2) cf = op.load(n);
+        double l, dd, u;
+        op.tri(k, g.mz, t.lat, cf, bc, l, dd, u);
+        const double m = dd - l * cprev;
+        cprev = u / m;
+        lo[n] = (float)l;
+        ip[n] = (float)(1.0 / m);
+        cp[n] = (float)cprev;
+    }
+}
+
+// out = L^-1 in
+__global__ __launch_bounds__(kThreads) void k_line_solve(Geo g, const KState *st, const float *__restrict__ in,
+                                                         const float *__restrict__ lo, const float *__restrict__ cp,
+                                                         const float *__restrict__ ip, float *__restrict__ dq,
+                                                         float *__restrict__ out)
+{
+    if (st->done) return;
+    const TilePos t = tile_pos(g);
+    if (!t.inner) return;
+    float dprev = 0.f;
+    for (int k = 0; k < g.mz; ++k) {
+        const long n = (long)k * g.P + t.c;
+        dprev = (in[n] - lo[n] * dprev) * ip[n];
+        dq[n] = dprev;
+    }
+    float delta = 0.f;
+    for (int k = g.mz - 1; k >= 0; --k) {
+        const long n = (long)k * g.P + t.c;
+        delta = dq[n] - cp[n] * delta;
+        out[n] = delta;
+    }
+}
+
+// out = y + L^-1 (in - A y); out must not alias y
+template <class Op>
+__global__ __launch_bounds__(kThreads) void k_line_step(Op op, Geo g, const KState *st,
+                                                        const float *__restrict__ y, const float *__restrict__ in,
+                                                        const float *__restrict__ lo, const float *__restrict__ cp,
+                                                        const float *__restrict__ ip, float *__restrict__ dq,
+                                                        float *__restrict__ out)
+{
+    if (st->done) return;
+    const TilePos t = tile_pos(g);
+    if (!t.inner) return;
+    float dprev = 0.f;
+    auto epi = [&](int, long n, float ay, float, float) {
+        dprev = ((in[n] - ay) - lo[n] * dprev) * ip[n];
+        dq[n] = dprev;
+    };
+    march<Op, false>(op, g, y, t, epi);
+    float delta = 0.f;
+    for (int k = g.mz - 1; k >= 0; --k) {
+        const long n = (long)k * g.P + t.c;
+        delta = dq[n] - cp[n] * delta;
+        out[n] = y[n] + delta;
+    }
+}
+
+// v = A p_hat and sigma = <r_hat, v>
+template <class Op>
+__global__ __launch_bounds__(kThreads) void k_mv_sigma(Op op, Geo g, KState *st, double *partials,
+                                                       const typename Op::type *__restrict__ ph,
+                                                       const typename Op::type *__restrict__ rh,
+                                                       typename Op::type *__restrict__ v)
+{
+    using T = typename Op::type;
+    if (st->done) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->fmode = 0;
+        return;
+    }
+    const TilePos t = tile_pos(g);
+    double acc[1] = {0.0};
+    if (t.inner) {
+        auto epi = [&](int, long n, T yv, T, T) {
+            v[n] = yv;
+            acc[0] = acc[0] + (double)rh[n] * (double)yv;
+        };
+        march<Op, false>(op, g, ph, t, epi);
+    }
+    reduce_finalize<1>(acc, partials, &st->cnt[1], t.L, g.nt, [&](const double *tot) { finish_sigma(st, tot); });
+}
+
+// s = r - alpha v
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_update_s_only(long N, const KState *st, const T *__restrict__ r,
+                                                          const T *__restrict__ v, T *__restrict__ s)
+{
+    if (st->done) return;
+    const T na = T(-st->alpha);
+    for (long n = (long)blockIdx.x * kBlock + threadIdx.x; n < N; n += (long)gridDim.x * kBlock)
+        s[n] = T(1) * r[n] + na * v[n];
+}
+
+// t = A s_hat and the five dots <t,s>, <t,t>, <r_hat,s>, <r_hat,t>, <s,s>
+template <class Op>
+__global__ __launch_bounds__(kThreads) void k_mv_omega(Op op, Geo g, KState *st, double *partials,
+                                                       const typename Op::type *__restrict__ sh,
+                                                       const typename Op::type *__restrict__ s,
+                                                       const typename Op::type *__restrict__ rh,
+                                                       typename Op::type *__restrict__ tt_out)
+{
+    using T = typename Op::type;
+    if (st->done) return;
+    const TilePos t = tile_pos(g);
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (t.inner) {
+        auto epi = [&](int, long n, T tv, T, T) {
+            tt_out[n] = tv;
+            const double td = tv, sd = s[n], rd = rh[n];
+            acc[0] = acc[0] + td * sd;
+            acc[1] = acc[1] + td * td;
+            acc[2] = acc[2] + rd * sd;
+            acc[3] = acc[3] + rd * td;
+            acc[4] = acc[4] + sd * sd;
+        };
+        march<Op, false>(op, g, sh, t, epi);
+    }
+    reduce_finalize<5>(acc, partials, &st->cnt[2], t.L, g.nt, [&](const double *tot) { finish_omega(st, tot); });
+}
+
 // ------------------------------------------------------------------ FGMRES-IR
 // Outer solve of the mixed-precision path: flexible GMRES in FP64 whose
 // preconditioner is an FP32 BiCGStab solve (GMRES-based iterative refinement).
@@ -1031,6 +1182,26 @@ __global__ __launch_bounds__(kBlock) void k_fg_take(long N, const float *__restr
 {
     for (long n = (long)blockIdx.x * kBlock + threadIdx.x; n < N; n += (long)gridDim.x * kBlock) {
         const float v = x32[n];
+        z[n] = v;
+        z64[n] = (double)v;
+    }
+}
+
+// out = a x + b y (out may alias x or y)
+__global__ __launch_bounds__(kBlock) void k_axpby(long N, double a, const double *x, double b, const double *y,
+                                                  double *out)
+{
+    for (long n = (long)blockIdx.x * kBlock + threadIdx.x; n < N; n += (long)gridDim.x * kBlock)
+        out[n] = a * x[n] + b * y[n];
+}
+
+// A given FP64 direction as an FGMRES direction: z (its FP32 rounding, kept
+// for the final update) and the FP64 widening of that rounding for the matvec.
+__global__ __launch_bounds__(kBlock) void k_fg_take64(long N, const double *__restrict__ src, float *__restrict__ z,
+                                                      double *__restrict__ z64)
+{
+    for (long n = (long)blockIdx.x * kBlock + threadIdx.x; n < N; n += (long)gridDim.x * kBlock) {
+        const float v = (float)src[n];
         z[n] = v;
         z64[n] = (double)v;
     }

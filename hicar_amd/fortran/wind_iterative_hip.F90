@@ -21,7 +21,7 @@ module wind_iterative_hip
 
     public :: wind_tile_t, wind_solver_options_t
     public :: init_iter_winds, calc_iter_winds, finalize_iter_winds, probe_operator, last_solve_info
-    public :: calc_divergence
+    public :: calc_divergence, load_learned_guess
 
     integer, parameter :: MAX_NESTS = 4   ! wind_iterative.F90:139
 
@@ -47,12 +47,14 @@ module wind_iterative_hip
         integer :: wind_solver_iterations = 1500     ! namelist_utilities.F90:3871-3876
         real(c_double) :: tol_rel = 1.0e-5_c_double   ! wind_iterative.F90:60
         real(c_double) :: tol_abs = 1.0e-10_c_double  ! wind_iterative.F90:59
-        integer :: precision = WINDHIP_PLAIN_F64
+        integer :: precision = WINDHIP_PLAIN_F64      ! or WINDHIP_MIXED, WINDHIP_MIXED_LINE
+        integer :: start = WINDHIP_START_WARM         ! WINDHIP_START_LEARNED once load_learned_guess ran
         logical :: verbose = .false.
     end type wind_solver_options_t
 
     type :: nest_slot_t
         type(c_ptr) :: handle = c_null_ptr
+        type(c_ptr) :: net = c_null_ptr
         type(windhip_solve_opts) :: opts
         logical :: verbose = .false.
     end type nest_slot_t
@@ -94,6 +96,7 @@ contains
         slots(n)%opts%tol_rel = options%tol_rel
         slots(n)%opts%tol_abs = options%tol_abs
         slots(n)%opts%precision = options%precision
+        slots(n)%opts%start = options%start
         slots(n)%verbose = options%verbose
     end subroutine init_iter_winds
 
@@ -160,6 +163,27 @@ contains
         call check(rc, "windhip_divergence")
     end subroutine calc_divergence
 
+    !> Load the trained parameters of the learned initial guess (the flat FP32
+    !! array of windhip.h, windhip_net_param_count() values) for nest
+    !! nest_indx, after its init_iter_winds; solves with options%start =
+    !! WINDHIP_START_LEARNED then start from the network's guess when its
+    !! residual beats the previous lambda's. No counterpart in the reference.
+    subroutine load_learned_guess(nest_indx, params)
+        integer, intent(in) :: nest_indx
+        real(c_float), intent(in), target :: params(:)
+        integer(c_int) :: rc
+        if (.not. c_associated(slots(nest_indx)%handle)) error stop "load_learned_guess: call init_iter_winds first"
+        if (size(params, kind=c_long) /= windhip_net_param_count()) error stop "load_learned_guess: wrong size"
+        if (.not. c_associated(slots(nest_indx)%net)) then
+            rc = windhip_net_create(slots(nest_indx)%net)
+            call check(rc, "windhip_net_create")
+        end if
+        rc = windhip_net_set_params(slots(nest_indx)%net, c_loc(params))
+        call check(rc, "windhip_net_set_params")
+        rc = windhip_set_net(slots(nest_indx)%handle, slots(nest_indx)%net)
+        call check(rc, "windhip_set_net")
+    end subroutine load_learned_guess
+
     !> finalize_iter_winds (wind_iterative.F90:2142-2267): release every nest.
     subroutine finalize_iter_winds()
         integer :: n
@@ -168,6 +192,10 @@ contains
             if (c_associated(slots(n)%handle)) then
                 rc = windhip_destroy(slots(n)%handle)
                 slots(n)%handle = c_null_ptr
+            end if
+            if (c_associated(slots(n)%net)) then
+                rc = windhip_net_destroy(slots(n)%net)
+                slots(n)%net = c_null_ptr
             end if
         end do
     end subroutine finalize_iter_winds

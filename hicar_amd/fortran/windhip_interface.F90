@@ -18,7 +18,7 @@ module windhip_interface
     integer(c_int), parameter, public :: WINDHIP_HOST = 0, WINDHIP_DEVICE = 1
     integer(c_int), parameter, public :: WINDHIP_START_WARM = 0, WINDHIP_START_ZERO = 1, &
                                          WINDHIP_START_GIVEN = 2, WINDHIP_START_LEARNED = 3
-    integer(c_int), parameter, public :: WINDHIP_PLAIN_F64 = 0, WINDHIP_MIXED = 1
+    integer(c_int), parameter, public :: WINDHIP_PLAIN_F64 = 0, WINDHIP_MIXED = 1, WINDHIP_MIXED_LINE = 2
 
     type, bind(C), public :: windhip_desc
         integer(c_int) :: nx, ny, nz

@@ -50,11 +50,14 @@ enum { WINDHIP_HOST = 0, WINDHIP_DEVICE = 1 }; /* field memory */
 enum { WINDHIP_START_WARM = 0, WINDHIP_START_ZERO = 1, WINDHIP_START_GIVEN = 2,
        WINDHIP_START_LEARNED = 3 };
 
-/* PLAIN_F64: BiCGStab in FP64 throughout. MIXED: FP32 inner BiCGStab
- * (matvec, vectors, coefficients) inside FP64 iterative refinement; the true
- * residual, its norm, the rho/omega dot products and the lambda accumulation
- * stay FP64. */
-enum { WINDHIP_PLAIN_F64 = 0, WINDHIP_MIXED = 1 };
+/* PLAIN_F64: the reference's BiCGStab in FP64 throughout. MIXED: FP32 inner
+ * BiCGStab (matvec, vectors, coefficients; the reference's iteration with its
+ * 2 Jacobi-Richardson sweeps) as the preconditioner of an FP64 flexible GMRES
+ * refinement; the true residual, its norm, the rho/omega dot products and the
+ * lambda accumulation stay FP64. MIXED_LINE: the same with the FP32 BiCGStab
+ * preconditioned by two vertical line-relaxation steps instead (no
+ * counterpart in the reference). */
+enum { WINDHIP_PLAIN_F64 = 0, WINDHIP_MIXED = 1, WINDHIP_MIXED_LINE = 2 };
 
 typedef struct windhip_desc {
     int nx, ny, nz;              /* physical cells (ide, jde, kde) */
@@ -69,7 +72,7 @@ typedef struct windhip_solve_opts {
     double tol_abs;        /* bicg_tol_abs (1e-10, :59) */
     int max_iters;         /* wind_solver_iterations (1500, :57); MIXED: cap on all FP32 iterations */
     int start;             /* WINDHIP_START_* */
-    int precision;         /* WINDHIP_PLAIN_F64 or WINDHIP_MIXED */
+    int precision;         /* WINDHIP_PLAIN_F64, WINDHIP_MIXED or WINDHIP_MIXED_LINE */
     double inner_tol;      /* MIXED: relative reduction asked of each FP32 inner solve (0.5) */
     int inner_max_iters;   /* MIXED: cap per inner solve */
     int max_outer;         /* MIXED: cap on FGMRES steps (= inner solves); cycles restart every 32 */
@@ -215,7 +218,9 @@ int windhip_learned_input(windhip_solver *s, float *in_dev, int *dims_out, doubl
  * s-update, last sweep + t = A s + 5 dots, x/r/p-update; 5..9 the same in FP32;
  * 10 set-up, 11 FP64 residual of the mixed solve, 12 mixed-solve lambda update,
  * 13 true residual, 14 FGMRES inner set-up, 15 FGMRES FP64 matvec,
- * 16 FGMRES Gram-Schmidt step; the rest unused. reset != 0 clears the
+ * 16 FGMRES Gram-Schmidt step; MIXED_LINE inner iteration: 17 line solve,
+ * 18 line step (y + L^-1 (in - A y)), 19 v = A p_hat + sigma, 20 t = A s_hat +
+ * 5 dots (its s- and x/r/p-updates count as 7 and 9); the rest unused. reset != 0 clears the
  * counters after reading. */
 #define WINDHIP_PROFILE_KINDS 24
 int windhip_profile(windhip_solver *s, int reset, double *calls, double *ms);

@@ -166,3 +166,77 @@ def test_learned_start_selection():
     learned.attach(s, None)
     with pytest.raises(nat.WindHipError):
         s.solve(c.alpha, div, start=nat.START_LEARNED)
+
+
+def test_line_relaxation_matches_tridiagonal_solve():
+    """One line-relaxation step from zero (a zero-parameter network, one
+    smoothing step, no iterations) returns L^-1 b, L the tridiagonal column part
+    of A, here solved directly from the independently assembled sparse matrix."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    from test_oracle import stencil_matrix
+    from hicar_amd import learned
+    from hicar_amd import _native as nat
+    from hicar_amd.wind_iterative import WindSolver
+    _dev()
+    c = small_case("fractal_alpha")
+    o = oracle_for(c)
+    o.set_alpha(c.alpha)
+    o.assemble()
+    div = o.divergence(c.u, c.v, c.w, rho=c.rho, adv_den=True, use_dqdt=True)
+    o.compute_rhs(div)
+    b = o.rows("rhs").ravel()
+    A = stencil_matrix(o, c).tocoo()
+    mx, mz = c.nx + 2, c.nz + 2
+    col_r, col_c = A.row % mx + mx * (A.row // (mx * mz)), A.col % mx + mx * (A.col // (mx * mz))
+    keep = (col_r == col_c) & (np.abs(A.row // mx % mz - A.col // mx % mz) <= 1)
+    T = sp.csr_matrix((A.data[keep], (A.row[keep], A.col[keep])), shape=A.shape)
+    ref = spl.spsolve(T.tocsc(), b).reshape(o.rows_shape)
+    s = WindSolver(c.nx, c.ny, c.nz, c.dx)
+    s.set_metrics(c.metrics)
+    zero = learned.Net(np.zeros(learned.Net().n, dtype=np.float32))
+    learned.attach(s, zero, smooth=1)
+    s.set_lambda(np.zeros(s.rows_shape))
+    info = s.solve(c.alpha, div, start=nat.START_LEARNED, max_iters=0, retry=0)
+    assert info["start_used"] == nat.START_LEARNED and info["guess_res"] < info["plain_res"]
+    assert _rel(s.get_lambda(), ref) <= 1e-11
+
+
+def test_mixed_line_matches_oracle():
+    from hicar_amd import _native as nat
+    from hicar_amd.wind_iterative import WindSolver
+    _dev()
+    c = small_case("fractal_alpha")
+    o = oracle_for(c)
+    div = o.divergence(c.u, c.v, c.w, rho=c.rho, adv_den=True, use_dqdt=True)
+    o.solve(c.alpha, div, max_iters=5000, tol_rel=1e-12)
+    s = WindSolver(c.nx, c.ny, c.nz, c.dx)
+    s.set_metrics(c.metrics)
+    mixed = s.solve(c.alpha, div, precision=nat.MIXED, tol_rel=1e-8, start=1, max_iters=20000)
+    s.set_lambda(np.zeros(s.rows_shape))
+    info = s.solve(c.alpha, div, precision=nat.MIXED_LINE, tol_rel=1e-8, start=1, max_iters=20000)
+    assert info["status"] == 0 and info["true_rel_res"] <= 1e-8
+    assert _rel(s.get_lambda(), o.get_lambda()) <= 1e-6
+    assert info["iters"] < mixed["iters"]
+
+
+def test_mixed_learned_direction_keeps_the_answer():
+    """In the mixed solves the learned correction is the first FGMRES direction:
+    even a random network leaves the converged answer within tolerance."""
+    from hicar_amd import learned
+    from hicar_amd import _native as nat
+    from hicar_amd.wind_iterative import WindSolver
+    _dev()
+    c = small_case("fractal_alpha")
+    o = oracle_for(c)
+    div = o.divergence(c.u, c.v, c.w, rho=c.rho, adv_den=True, use_dqdt=True)
+    o.solve(c.alpha, div, max_iters=5000, tol_rel=1e-12)
+    s = WindSolver(c.nx, c.ny, c.nz, c.dx)
+    s.set_metrics(c.metrics)
+    learned.attach(s, learned.Net(learned.init_params(5)))
+    for prec in (nat.MIXED, nat.MIXED_LINE):
+        s.set_lambda(np.zeros(s.rows_shape))
+        info = s.solve(c.alpha, div, precision=prec, start=nat.START_LEARNED, tol_rel=1e-8, max_iters=20000)
+        assert info["start_used"] == nat.START_LEARNED
+        assert info["status"] == 0 and info["true_rel_res"] <= 1e-8
+        assert _rel(s.get_lambda(), o.get_lambda()) <= 1e-6
