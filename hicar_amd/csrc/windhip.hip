@@ -116,7 +116,7 @@ struct windhip_solver {
     int net_smooth = 4;                        // line-relaxation steps applied to the network's guess
     double *lr_c = nullptr, *lr_d = nullptr;   // line relaxation: Thomas coefficients
     // MIXED_LINE: FP32 Thomas factors of the column tridiagonal and scratch
-    float *lf_lo = nullptr, *lf_cp = nullptr, *lf_ip = nullptr, *lf_dq = nullptr;
+    float *lf_lo = nullptr, *lf_cp = nullptr, *lf_ip = nullptr, *lf_dq = nullptr, *lf_r = nullptr;
     bool metrics_set = false, structure_uploaded = false, probed = false;
     bool verbose = false;  // WINDHIP_VERBOSE=1: per-batch progress on stderr (verbose_solver, :274-275)
     int nsweeps = 2;  // precond_n_sweeps, BASE_PREC_SWEEPS (wind_iterative.F90:76)
@@ -373,7 +373,7 @@ int windhip_solver::init()
     for (double **q : pl64) CHK(alloc(q, d.P));
     float **pl32[] = {&bcx32, &bcy32, &bcz32, &dztop32};
     for (float **q : pl32) CHK(alloc(q, d.P));
-    CHK(alloc(&partials, (long)g.nt * 8));
+    CHK(alloc(&partials, (long)std::max(g.nt, chunk_blocks(g)) * 8));
     CHK(alloc(&scan_part, 3L * 1024));
     CHK(alloc(&scan_out, 4));
     CHK(alloc(&scan_cnt, 4));
@@ -584,8 +584,9 @@ int windhip_solver::enqueue_iterations(int count)
 }
 
 // One line-preconditioned FP32 BiCGStab iteration per repetition: p_hat =
-// M^-1 p (line solve, line step), v = A p_hat with sigma, s = r - alpha v,
-// s_hat = M^-1 s, t = A s_hat with the five dots, the x/r/p update.
+// M^-1 p (line solve, residual, line solve), v = A p_hat with sigma,
+// s = r - alpha v, s_hat = M^-1 s, t = A s_hat with the five dots, the x/r/p
+// update.
 int windhip_solver::enqueue_iterations_line(int count)
 {
     KState *sp = st + 1;
@@ -593,29 +594,29 @@ int windhip_solver::enqueue_iterations_line(int count)
     const float *DI;
     kvec<float>(&X, &R, &RH, &Pv, &V, &S, &Tv, PH, SH, &DI);
     const int fg = flat_grid(N);
+    const int cb = col_blocks(g), kb = chunk_blocks(g);
     return with_op<float>([&](auto op) -> int {
         using Op = decltype(op);
-        for (int it = 0; it < count; ++it) {
+        // M^-1 in: y1 = L^-1 in, r = in - A y1, out = y1 + L^-1 r
+        auto precond = [&](const float *in, float *y1, float *out) -> int {
             CHK(timed(PK_LINE_SOLVE, [&] {
-                k_line_solve<<<g.nt, kThreads, 0, stream>>>(g, sp, Pv, lf_lo, lf_cp, lf_ip, lf_dq, PH[1]);
+                k_line_solve_c<<<cb, kColThreads, 0, stream>>>(g, sp, in, lf_lo, lf_cp, lf_ip, lf_dq, nullptr, y1);
             }));
-            CHK(timed(PK_LINE_STEP, [&] {
-                k_line_step<Op><<<g.nt, kThreads, 0, stream>>>(op, g, sp, PH[1], Pv, lf_lo, lf_cp, lf_ip, lf_dq,
-                                                               PH[2]);
+            CHK(timed(PK_LINE_STEP, [&] { k_resid_c<Op><<<kb, kColThreads, 0, stream>>>(op, g, sp, y1, in, lf_r); }));
+            CHK(timed(PK_LINE_SOLVE, [&] {
+                k_line_solve_c<<<cb, kColThreads, 0, stream>>>(g, sp, lf_r, lf_lo, lf_cp, lf_ip, lf_dq, y1, out);
             }));
+            return WINDHIP_OK;
+        };
+        for (int it = 0; it < count; ++it) {
+            CHK(precond(Pv, PH[1], PH[2]));
             CHK(timed(PK_MV_SIGMA, [&] {
-                k_mv_sigma<Op><<<g.nt, kThreads, 0, stream>>>(op, g, sp, partials, PH[2], RH, V);
+                k_mv_sigma_c<Op><<<kb, kColThreads, 0, stream>>>(op, g, sp, partials, PH[2], RH, V);
             }));
             CHK(timed(5 + PK_UPD_S, [&] { k_update_s_only<float><<<fg, kBlock, 0, stream>>>(N, sp, R, V, S); }));
-            CHK(timed(PK_LINE_SOLVE, [&] {
-                k_line_solve<<<g.nt, kThreads, 0, stream>>>(g, sp, S, lf_lo, lf_cp, lf_ip, lf_dq, SH[1]);
-            }));
-            CHK(timed(PK_LINE_STEP, [&] {
-                k_line_step<Op><<<g.nt, kThreads, 0, stream>>>(op, g, sp, SH[1], S, lf_lo, lf_cp, lf_ip, lf_dq,
-                                                               SH[2]);
-            }));
+            CHK(precond(S, SH[1], SH[2]));
             CHK(timed(PK_MV_OMEGA, [&] {
-                k_mv_omega<Op><<<g.nt, kThreads, 0, stream>>>(op, g, sp, partials, SH[2], S, RH, Tv);
+                k_mv_omega_c<Op><<<kb, kColThreads, 0, stream>>>(op, g, sp, partials, SH[2], S, RH, Tv);
             }));
             CHK(timed(5 + PK_UPD_XRP, [&] {
                 k_update_xrp<float><<<fg, kBlock, 0, stream>>>(N, sp, X, PH[2], SH[2], S, Tv, R, Pv, V, DI, PH[0]);
@@ -792,6 +793,7 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
             CHK(alloc(&lf_cp, N));
             CHK(alloc(&lf_ip, N));
             CHK(alloc(&lf_dq, N));
+            CHK(alloc(&lf_r, N));
         }
         CHK(with_op<double>([&](auto op) {
             k_line_factor<decltype(op)><<<g.nt, kThreads, 0, stream>>>(op, g, lf_lo, lf_cp, lf_ip);

@@ -327,15 +327,17 @@ __device__ __forceinline__ double ld_coherent(const double *p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Canonical reduction of NV per-thread values over a block of kThreads: 64-lane
+// Canonical reduction of NV per-thread values over a block of NW waves: 64-lane
 // xor butterfly per wave, waves summed in order; block partials are folded by
-// the last block to finish (lane t sums partials t, t+kThreads, ... from 0),
-// reduced the same way, and thread 0 of that block calls fin(total).
-template <int NV, class Fin>
+// the last block to finish (lane t sums partials t, t+64 NW, ... from 0),
+// reduced the same way, and thread 0 of that block calls fin(total). The
+// partials move through agent-scope atomic stores and loads, the arrival
+// through an agent-scope counter.
+template <int NV, int NW = kWaves, class Fin>
 __device__ __forceinline__ void reduce_finalize(double (&acc)[NV], double *partials, unsigned *counter, int L,
                                                 int nb, Fin fin)
 {
-    __shared__ double red[kWaves][NV];
+    __shared__ double red[NW][NV];
     __shared__ int is_last;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
@@ -348,20 +350,24 @@ __device__ __forceinline__ void reduce_finalize(double (&acc)[NV], double *parti
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             double s = red[0][v];
-            for (int q = 1; q < kWaves; ++q) s = s + red[q][v];
-            partials[(long)L * NV + v] = s;
+            for (int q = 1; q < NW; ++q) s = s + red[q][v];
+            // written through to the device-coherent level (an agent-scope
+            // store); a __threadfence() here would be a release of every
+            // store of the kernel, i.e. a write-back of the XCD's whole L2
+            __hip_atomic_store(&partials[(long)L * NV + v], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        __threadfence();
-        unsigned old = atomicAdd(counter, 1u);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);  // the partials have landed before the counter moves
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        unsigned old = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         is_last = (old == (unsigned)(nb - 1));
     }
     __syncthreads();
     if (!is_last) return;
-    __threadfence();
     double fold[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) fold[v] = 0.0;
-    for (int q = threadIdx.x; q < nb; q += kThreads) {
+    for (int q = threadIdx.x; q < nb; q += NW * 64) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) fold[v] = fold[v] + ld_coherent(&partials[(long)q * NV + v]);
     }
@@ -377,7 +383,7 @@ __device__ __forceinline__ void reduce_finalize(double (&acc)[NV], double *parti
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             double s = red[0][v];
-            for (int q = 1; q < kWaves; ++q) s = s + red[q][v];
+            for (int q = 1; q < NW; ++q) s = s + red[q][v];
             tot[v] = s;
         }
         __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1100,6 +1106,316 @@ __global__ __launch_bounds__(kThreads) void k_mv_omega(Op op, Geo g, KState *st,
         march<Op, false>(op, g, sh, t, epi);
     }
     reduce_finalize<5>(acc, partials, &st->cnt[2], t.L, g.nt, [&](const double *tot) { finish_omega(st, tot); });
+}
+
+// ---------------------------------------------- flat column march (FP32 line path)
+// Single-stage FP32 kernels of the line-preconditioned iteration carry no
+// bitwise contract, so they take one column per lane over all P columns
+// (256 consecutive columns per block, XCD-aware block order) instead of the
+// tiles, whose halo lanes would idle. Their reductions use the same
+// block-partial / last-block fold with one partial per block.
+constexpr int kColThreads = 256;  // flat column kernels: 4 wavefronts per block
+constexpr int kColWaves = kColThreads / 64;
+
+struct ColPos {
+    int L;
+    bool valid, lat;
+    long c;
+    unsigned oc;
+};
+
+__device__ __forceinline__ ColPos col_pos(const Geo &g, int nb)
+{
+    ColPos t;
+    t.L = logical_block(nb);
+    const long c = (long)t.L * kColThreads + threadIdx.x;
+    t.valid = c < g.P;
+    t.c = t.valid ? c : 0;
+    const int i = (int)(t.c % g.mx), j = (int)(t.c / g.mx);
+    t.lat = i == 0 || i == g.mx - 1 || j == 0 || j == g.my - 1;
+    t.oc = (unsigned)t.c;
+    return t;
+}
+
+// Pipelined march of (A x)(row) over levels [k0, k1) of one column, bottom to
+// top: the operands of level m (x at m+1, coefficients, the epilogue's
+// pre-loads) are loaded one level ahead into alternating register sets, all
+// through buffer resources; the ghost planes are peeled where they fall in the
+// range. epi.apply(k, so, vo, ax, xc, pre) per level.
+template <class Op, class Epi>
+__device__ __forceinline__ void march_pipe(const Op &op, const Geo &g, const ColPos &t,
+                                           const typename Op::type *__restrict__ x, int k0, int k1, Epi &epi)
+{
+    using T = typename Op::type;
+    using Cf = typename Op::Cf;
+    using Pre = typename Epi::Pre;
+    struct Slot {
+        T c, w, e, s, n;
+        Cf cf;
+        Pre ep;
+    };
+    const int mx = __builtin_amdgcn_readfirstlane(g.mx);
+    const int mz = __builtin_amdgcn_readfirstlane(g.mz);
+    const long P = (long)mx * __builtin_amdgcn_readfirstlane(g.my);
+    const long N = P * mz;
+    constexpr unsigned sz = sizeof(T);
+    const unsigned pb = (unsigned)(P * (long)sz);
+    const unsigned voc = t.oc * sz;
+    const long gd = mx + 1;
+    const unsigned oS = sz, oW = (unsigned)mx * sz, oC = (unsigned)(mx + 1) * sz, oE = (unsigned)(mx + 2) * sz,
+                   oN = (unsigned)(2 * mx + 1) * sz;
+    const rsrc_t rx = make_rsrc(x - gd, (N + 2 * gd) * (long)sz);
+    const typename Op::Src rcf = op.src(N);
+    epi.setup(N);
+    auto fill = [&](Slot &sl, int m, auto pos) {
+        constexpr int ps = decltype(pos)::value;
+        if (ps != kTop) {
+            const unsigned so = (unsigned)(m + 1 < mz - 1 ? m + 1 : mz - 1) * pb;
+            load_x5(rx, voc, so + oS, so + oW, so + oC, so + oE, so + oN, sl.c, sl.w, sl.e, sl.s, sl.n);
+        }
+        if (ps == kMid) {
+            const int lv = m < mz - 2 ? m : mz - 2;
+            sl.cf = op.load_src(rcf, (long)lv * P, (unsigned)lv * pb, t.oc, voc);
+        }
+        sl.ep = epi.load((unsigned)(m < mz - 1 ? m : mz - 1) * pb, voc);
+    };
+    Win<T> win = {};
+    const typename Op::Bc bc = op.bc(t.c);
+    auto step = [&](const Slot &cur, int m, auto pos) {
+        constexpr int ps = decltype(pos)::value;
+        if (ps != kFirst) win.shift();
+        win.c2 = keep(cur.c); win.w2 = keep(cur.w); win.e2 = keep(cur.e);
+        win.s2 = keep(cur.s); win.n2 = keep(cur.n);
+        constexpr int rc = ps == kFirst ? kRowBottom : (ps == kTop ? kRowTop : kRowInterior);
+        const T ax = op.template row_c<rc>(t.lat, win, cur.cf, bc);
+        epi.apply(m, (unsigned)m * pb, voc, ax, win.c1, cur.ep);
+    };
+    using PF = std::integral_constant<int, kFirst>;
+    using PM = std::integral_constant<int, kMid>;
+    using PT = std::integral_constant<int, kTop>;
+    Slot sa, sb;
+    int m = k0;
+    if (k0 == 0) {
+        load_x5(rx, voc, oS, oW, oC, oE, oN, win.c1, win.w1, win.e1, win.s1, win.n1);
+        fill(sa, 0, PF{});
+        fill(sb, 1, PM{});
+        step(sa, 0, PF{});
+        m = 1;
+    } else {
+        // window before the first step: level k0-1 as c1, level k0 as c2
+        const unsigned s1 = (unsigned)(k0 - 1) * pb, s2 = (unsigned)k0 * pb;
+        load_x5(rx, voc, s1 + oS, s1 + oW, s1 + oC, s1 + oE, s1 + oN, win.c1, win.w1, win.e1, win.s1, win.n1);
+        load_x5(rx, voc, s2 + oS, s2 + oW, s2 + oC, s2 + oE, s2 + oN, win.c2, win.w2, win.e2, win.s2, win.n2);
+        fill(sb, k0, PM{});
+    }
+    // sb holds the operands of step m; interior steps m .. mend-1
+    const int mend = k1 == mz ? mz - 1 : k1;
+    for (; m + 1 < mend; m += 2) {
+        fill(sa, m + 1, PM{});
+        step(sb, m, PM{});
+        fill(sb, m + 2, PM{});
+        step(sa, m + 1, PM{});
+    }
+    if (m < mend) {  // one interior step left
+        if (k1 == mz) {
+            fill(sa, mz - 1, PT{});
+            step(sb, m, PM{});
+            step(sa, mz - 1, PT{});
+        } else {
+            step(sb, m, PM{});
+        }
+    } else if (k1 == mz) {
+        step(sb, mz - 1, PT{});
+    }
+}
+
+constexpr int kChunksK = 3;  // level chunks per column of the flat stencil kernels
+
+// 1-D grid of col_blocks(g) * kChunksK blocks; consecutive logical blocks are
+// the level chunks of the same columns (same XCD, shared halo levels in L2).
+struct ChunkPos {
+    ColPos t;
+    int k0, k1, L, nb;
+};
+
+__device__ __forceinline__ ChunkPos chunk_pos(const Geo &g)
+{
+    ChunkPos c;
+    c.nb = gridDim.x;
+    c.L = logical_block(c.nb);
+    const int cb = c.L / kChunksK, q = c.L - cb * kChunksK;
+    const long col = (long)cb * kColThreads + threadIdx.x;
+    c.t.L = c.L;
+    c.t.valid = col < g.P;
+    c.t.c = c.t.valid ? col : 0;
+    const int i = (int)(c.t.c % g.mx), j = (int)(c.t.c / g.mx);
+    c.t.lat = i == 0 || i == g.mx - 1 || j == 0 || j == g.my - 1;
+    c.t.oc = (unsigned)c.t.c;
+    c.k0 = (int)((long)q * g.mz / kChunksK);
+    c.k1 = (int)((long)(q + 1) * g.mz / kChunksK);
+    return c;
+}
+
+inline int col_blocks(const Geo &g) { return (int)((g.P + kColThreads - 1) / kColThreads); }
+inline int chunk_blocks(const Geo &g) { return col_blocks(g) * kChunksK; }
+
+// v = A p_hat and sigma = <r_hat, v> (FP32 line path)
+template <class Op>
+__global__ __launch_bounds__(kColThreads) void k_mv_sigma_c(Op op, Geo g, KState *st, double *partials,
+                                                            const typename Op::type *__restrict__ ph,
+                                                            const typename Op::type *__restrict__ rh,
+                                                            typename Op::type *__restrict__ v)
+{
+    using T = typename Op::type;
+    if (st->done) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->fmode = 0;
+        return;
+    }
+    const ChunkPos c = chunk_pos(g);
+    struct Epi {
+        const T *rh;
+        T *v;
+        double acc[1];
+        bool on;
+        rsrc_t rr, rv;
+        struct Pre { T rh; };
+        __device__ void setup(long N) { rr = make_rsrc(rh, N * (long)sizeof(T)); rv = make_rsrc(v, N * (long)sizeof(T)); }
+        __device__ Pre load(unsigned so, unsigned vo) const { return Pre{bload<T>(rr, vo, so)}; }
+        __device__ void apply(int, unsigned so, unsigned vo, T ax, T, const Pre &p)
+        {
+            if (!on) return;
+            bstore(ax, rv, vo, so);
+            acc[0] = acc[0] + (double)p.rh * (double)ax;
+        }
+    } epi{rh, v, {0.0}, c.t.valid};
+    march_pipe(op, g, c.t, ph, c.k0, c.k1, epi);
+    reduce_finalize<1, kColWaves>(epi.acc, partials, &st->cnt[1], c.L, c.nb,
+                                  [&](const double *tot) { finish_sigma(st, tot); });
+}
+
+// t = A s_hat and the five dots (FP32 line path)
+template <class Op>
+__global__ __launch_bounds__(kColThreads) void k_mv_omega_c(Op op, Geo g, KState *st, double *partials,
+                                                            const typename Op::type *__restrict__ sh,
+                                                            const typename Op::type *__restrict__ s,
+                                                            const typename Op::type *__restrict__ rh,
+                                                            typename Op::type *__restrict__ tt_out)
+{
+    using T = typename Op::type;
+    if (st->done) return;
+    const ChunkPos c = chunk_pos(g);
+    struct Epi {
+        const T *s, *rh;
+        T *out;
+        double acc[5];
+        bool on;
+        rsrc_t rs, rr, ro;
+        struct Pre { T s, rh; };
+        __device__ void setup(long N)
+        {
+            const long nbt = N * (long)sizeof(T);
+            rs = make_rsrc(s, nbt); rr = make_rsrc(rh, nbt); ro = make_rsrc(out, nbt);
+        }
+        __device__ Pre load(unsigned so, unsigned vo) const { return Pre{bload<T>(rs, vo, so), bload<T>(rr, vo, so)}; }
+        __device__ void apply(int, unsigned so, unsigned vo, T tv, T, const Pre &p)
+        {
+            if (!on) return;
+            bstore(tv, ro, vo, so);
+            const double td = tv, sd = p.s, rd = p.rh;
+            acc[0] = acc[0] + td * sd;
+            acc[1] = acc[1] + td * td;
+            acc[2] = acc[2] + rd * sd;
+            acc[3] = acc[3] + rd * td;
+            acc[4] = acc[4] + sd * sd;
+        }
+    } epi{s, rh, tt_out, {0.0, 0.0, 0.0, 0.0, 0.0}, c.t.valid};
+    march_pipe(op, g, c.t, sh, c.k0, c.k1, epi);
+    reduce_finalize<5, kColWaves>(epi.acc, partials, &st->cnt[2], c.L, c.nb,
+                                  [&](const double *tot) { finish_omega(st, tot); });
+}
+
+// r = in - A y (FP32 line path)
+template <class Op>
+__global__ __launch_bounds__(kColThreads) void k_resid_c(Op op, Geo g, const KState *st,
+                                                         const typename Op::type *__restrict__ y,
+                                                         const typename Op::type *__restrict__ in,
+                                                         typename Op::type *__restrict__ r)
+{
+    using T = typename Op::type;
+    if (st->done) return;
+    const ChunkPos c = chunk_pos(g);
+    struct Epi {
+        const T *in;
+        T *r;
+        bool on;
+        rsrc_t ri, rr;
+        struct Pre { T in; };
+        __device__ void setup(long N) { ri = make_rsrc(in, N * (long)sizeof(T)); rr = make_rsrc(r, N * (long)sizeof(T)); }
+        __device__ Pre load(unsigned so, unsigned vo) const { return Pre{bload<T>(ri, vo, so)}; }
+        __device__ void apply(int, unsigned so, unsigned vo, T ay, T, const Pre &p)
+        {
+            if (on) bstore(p.in - ay, rr, vo, so);
+        }
+    } epi{in, r, c.t.valid};
+    march_pipe(op, g, c.t, y, c.k0, c.k1, epi);
+}
+
+// out = y + L^-1 in (y null: out = L^-1 in), the Thomas algorithm per column
+// with the loads of kLB levels issued together on each pass (the recursion
+// itself is a few FMAs per level; its loads are what costs).
+constexpr int kLB = 8;
+__global__ __launch_bounds__(kColThreads) void k_line_solve_c(Geo g, const KState *st, const float *__restrict__ in,
+                                                              const float *__restrict__ lo,
+                                                              const float *__restrict__ cp,
+                                                              const float *__restrict__ ip, float *__restrict__ dq,
+                                                              const float *__restrict__ y, float *__restrict__ out)
+{
+    if (st->done) return;
+    const ColPos t = col_pos(g, gridDim.x);
+    const int mz = g.mz;
+    const long N = g.P * mz;
+    const unsigned voc = t.oc * 4u, pb = (unsigned)(g.P * 4);
+    const long nbt = N * 4;
+    const rsrc_t rin = make_rsrc(in, nbt), rlo = make_rsrc(lo, nbt), rip = make_rsrc(ip, nbt),
+                 rdq = make_rsrc(dq, nbt), rcp = make_rsrc(cp, nbt), ro = make_rsrc(out, nbt);
+    const bool hy = y != nullptr;
+    const rsrc_t ry = make_rsrc(hy ? y : in, nbt);
+    float dprev = 0.f;
+    for (int k0 = 0; k0 < mz; k0 += kLB) {
+        float a[kLB], l[kLB], q[kLB];
+#pragma unroll
+        for (int e = 0; e < kLB; ++e) {
+            const unsigned so = (unsigned)(k0 + e < mz ? k0 + e : mz - 1) * pb;
+            a[e] = bload<float>(rin, voc, so);
+            l[e] = bload<float>(rlo, voc, so);
+            q[e] = bload<float>(rip, voc, so);
+        }
+#pragma unroll
+        for (int e = 0; e < kLB; ++e) {
+            if (k0 + e < mz) {
+                dprev = (a[e] - l[e] * dprev) * q[e];
+                if (t.valid) bstore(dprev, rdq, voc, (unsigned)(k0 + e) * pb);
+            }
+        }
+    }
+    float delta = 0.f;
+    for (int k1 = mz - 1; k1 >= 0; k1 -= kLB) {
+        float d[kLB], c[kLB], yv[kLB];
+#pragma unroll
+        for (int e = 0; e < kLB; ++e) {
+            const unsigned so = (unsigned)(k1 - e >= 0 ? k1 - e : 0) * pb;
+            d[e] = bload<float>(rdq, voc, so);
+            c[e] = bload<float>(rcp, voc, so);
+            yv[e] = hy ? bload<float>(ry, voc, so) : 0.f;
+        }
+#pragma unroll
+        for (int e = 0; e < kLB; ++e) {
+            if (k1 - e >= 0) {
+                delta = d[e] - c[e] * delta;
+                if (t.valid) bstore(yv[e] + delta, ro, voc, (unsigned)(k1 - e) * pb);
+            }
+        }
+    }
 }
 
 // ------------------------------------------------------------------ FGMRES-IR
