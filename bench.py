@@ -15,8 +15,10 @@ Arms (each solves the same system to the same tolerance):
                  vertical line-relaxation steps
   line_learned   `line` started from the learned guess (the headline `value`)
   mixed_learned  `mixed` started from the learned guess
-Errors of every arm against the plain FP64 solution are reported for lambda,
-the corrected u, v, w and the divergence after the correction. Per-kernel
+Errors (relative L2) of every arm are reported for lambda, the corrected u,
+v, w and the divergence after the correction, against the plain FP64 solve
+at the same tolerance and against a tight FP64 reference (the plain FP64
+solve to rel. residual 1e-11, untimed). Per-kernel
 device times come from hipEvents recorded around every hot launch on the
 solver's stream (one extra profiled solve per arm). The CPU baseline is the
 oracle (the reference algorithm restated in C, OpenMP) timed on this host on
@@ -58,8 +60,8 @@ def kernel_bytes(kind: str, nx: int, ny: int, nz: int) -> float:
         "update_s": 5 * N * t,            # r, v, D^-1 -> s, s_hat
         "update_xrp": 12 * N * t,         # x, p_hat, s_hat, s, t, p, v, D^-1 -> x, r, p, p_hat
         "init": 7 * N * t + coef,         # x (stencil), b, D^-1, coefficients -> r, r_hat, p, p_hat
-        "line_solve": 7 * N * t,          # in, lo, 1/pivot -> dq; dq, cp -> out
-        "line_step": 9 * N * t + coef,    # y (stencil), in, lo, 1/pivot, coefficients -> dq; dq, cp, y -> out
+        "line_solve": 5.5 * N * t,        # in, lo, 1/pivot, cp -> out (+ y on every second call)
+        "resid": 3 * N * t + coef,        # y (stencil), in, coefficients -> r
         "mv_sigma": 3 * N * t + coef,     # p_hat (stencil), r_hat, coefficients -> v
         "mv_omega": 4 * N * t + coef,     # s_hat (stencil), s, r_hat, coefficients -> t
     }
@@ -77,6 +79,8 @@ def kernel_bytes(kind: str, nx: int, ny: int, nz: int) -> float:
         return 4 * N * 8
     if kind == "update_s_line_f32":  # r, v -> s
         return 3 * N * 4
+    if kind == "update_xrp_line_f32":  # x, p_hat, s_hat, s, t, p, v -> x, r, p
+        return 10 * N * 4
     base = kind.rsplit("_", 1)[0]
     return float(table[base])
 
@@ -122,6 +126,9 @@ def main():
     ap.add_argument("--arms", default="plain,mixed,line,line_learned,mixed_learned")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=6)
+    ap.add_argument("--tight", type=float, default=1e-11,
+                    help="rel. residual of the FP64 reference solve the errors are stated against")
+    ap.add_argument("--no-tight", action="store_true", help="skip the tight FP64 reference solve")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy a 1 GiB device buffer three times (known traffic for counter calibration)")
     args = ap.parse_args()
@@ -172,13 +179,13 @@ def main():
     want = [a for a in args.arms.split(",") if a in arms and (net is not None or "learned" not in a)]
     zeros = np.zeros(s.rows_shape)
 
-    def solve(arm, profile=0):
+    def solve(arm, profile=0, tol=1e-8):
         o = arms[arm]
         learned.attach(s, net if o["start"] == nat.START_LEARNED else None)
         s.set_lambda(zeros)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        info = s.solve(alpha, div, tol_rel=1e-8, max_iters=50000, inner_tol=args.inner_tol, profile=profile, **o)
+        info = s.solve(alpha, div, tol_rel=tol, max_iters=50000, inner_tol=args.inner_tol, profile=profile, **o)
         torch.cuda.synchronize()
         return 1e3 * (time.perf_counter() - t0), info
 
@@ -188,6 +195,22 @@ def main():
         dd = torch.empty_like(div)
         s.divergence(uu, vv, ww, density=rho, adv_den=True, use_dqdt=True, out=dd)
         return [t.cpu().numpy() for t in (uu, vv, ww, dd)]
+
+    rel = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))  # noqa: E731
+
+    def errs(lam, wnd, ref):
+        return dict(lam=rel(lam, ref[0]), u=rel(wnd[0], ref[1][0]), v=rel(wnd[1], ref[1][1]),
+                    w=rel(wnd[2], ref[1][2]), div=rel(wnd[3], ref[1][3]))
+
+    # Tight FP64 reference (untimed): the plain FP64 solve to rel. residual
+    # TIGHT, against which every arm's solution error is stated.
+    tight = None
+    if not args.no_tight:
+        ms_t, info_t = solve("plain", tol=args.tight)
+        tight = (s.get_lambda(), winds())
+        tight_info = dict(tol_rel=args.tight, iters=info_t["iters"], status=info_t["status"],
+                          true_rel_res=info_t["true_rel_res"], ms=round(ms_t, 1))
+        log(f"tight FP64 reference: {tight_info}")
 
     res = {}
     ref = None
@@ -208,9 +231,9 @@ def main():
         if arm == "plain":
             ref = (lam, wnd)
         elif ref is not None:
-            rel = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))  # noqa: E731
-            r["err_vs_plain"] = dict(lam=rel(lam, ref[0]), u=rel(wnd[0], ref[1][0]), v=rel(wnd[1], ref[1][1]),
-                                     w=rel(wnd[2], ref[1][2]), div=rel(wnd[3], ref[1][3]))
+            r["err_vs_plain"] = errs(lam, wnd, ref)
+        if tight is not None:
+            r["err_vs_tight"] = errs(lam, wnd, tight)
         if "learned" in arm:
             r.update(guess_res=info["guess_res"], plain_res=info["plain_res"])
         res[arm] = r
@@ -223,8 +246,8 @@ def main():
     def kernel_table(prof, arm):
         out = {}
         for kind, (calls, tot) in prof.items():
-            if arm.startswith("line") and kind == "update_s_f32":
-                kind = "update_s_line_f32"  # s = r - alpha v only, without the Jacobi sweep
+            if arm.startswith("line") and kind in ("update_s_f32", "update_xrp_f32"):
+                kind = kind[:-4] + "_line_f32"  # without the Jacobi sweep's D^-1 terms
             avg = tot / calls
             gbs = kernel_bytes(kind, case.nx, case.ny, case.nz) / (avg * 1e-3) / 1e9
             out[kind] = dict(calls=int(calls), avg_ms=round(avg, 5), total_ms=round(tot, 2), GBs=round(gbs, 1),
@@ -278,6 +301,7 @@ def main():
         "cpu_baseline": cpu,
         "speedup_vs_plain": round(plain_ms / ms_h, 3) if plain_ms else None,
         "cells_per_s": case.cells / (ms_h * 1e-3),
+        "tight_reference": tight_info if tight is not None else None,
         "arms": res,
         "network": netinfo,
         "kernels": tables,

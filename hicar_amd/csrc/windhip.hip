@@ -493,7 +493,7 @@ int windhip_solver::read_state(int idx)
 // the handle's stream): base + 5 * precision (0 FP64, 1 FP32).
 enum { PK_SWEEP = 0, PK_SIGMA = 1, PK_UPD_S = 2, PK_OMEGA = 3, PK_UPD_XRP = 4, PK_INIT = 10,
        PK_IR_RESID = 11, PK_IR_ACC = 12, PK_TRUE_RESID = 13, PK_FG_SETUP = 14, PK_FG_MATVEC = 15,
-       PK_FG_AXPY_DOT = 16, PK_LINE_SOLVE = 17, PK_LINE_STEP = 18, PK_MV_SIGMA = 19, PK_MV_OMEGA = 20,
+       PK_FG_AXPY_DOT = 16, PK_LINE_SOLVE = 17, PK_RESID = 18, PK_MV_SIGMA = 19, PK_MV_OMEGA = 20,
        PK_COUNT = 21 };
 
 template <class Launch>
@@ -598,14 +598,18 @@ int windhip_solver::enqueue_iterations_line(int count)
     return with_op<float>([&](auto op) -> int {
         using Op = decltype(op);
         // M^-1 in: y1 = L^-1 in, r = in - A y1, out = y1 + L^-1 r
+        auto lsolve = [&](const float *in, const float *y, float *out) {
+            if (g.mz <= 48)
+                k_line_solve_r<48><<<cb, kColThreads, 0, stream>>>(g, sp, in, lf_lo, lf_cp, lf_ip, y, out);
+            else if (g.mz <= 64)
+                k_line_solve_r<64><<<cb, kColThreads, 0, stream>>>(g, sp, in, lf_lo, lf_cp, lf_ip, y, out);
+            else
+                k_line_solve_c<<<cb, kColThreads, 0, stream>>>(g, sp, in, lf_lo, lf_cp, lf_ip, lf_dq, y, out);
+        };
         auto precond = [&](const float *in, float *y1, float *out) -> int {
-            CHK(timed(PK_LINE_SOLVE, [&] {
-                k_line_solve_c<<<cb, kColThreads, 0, stream>>>(g, sp, in, lf_lo, lf_cp, lf_ip, lf_dq, nullptr, y1);
-            }));
-            CHK(timed(PK_LINE_STEP, [&] { k_resid_c<Op><<<kb, kColThreads, 0, stream>>>(op, g, sp, y1, in, lf_r); }));
-            CHK(timed(PK_LINE_SOLVE, [&] {
-                k_line_solve_c<<<cb, kColThreads, 0, stream>>>(g, sp, lf_r, lf_lo, lf_cp, lf_ip, lf_dq, y1, out);
-            }));
+            CHK(timed(PK_LINE_SOLVE, [&] { lsolve(in, nullptr, y1); }));
+            CHK(timed(PK_RESID, [&] { k_resid_c<Op><<<kb, kColThreads, 0, stream>>>(op, g, sp, y1, in, lf_r); }));
+            CHK(timed(PK_LINE_SOLVE, [&] { lsolve(lf_r, y1, out); }));
             return WINDHIP_OK;
         };
         for (int it = 0; it < count; ++it) {
@@ -619,7 +623,8 @@ int windhip_solver::enqueue_iterations_line(int count)
                 k_mv_omega_c<Op><<<kb, kColThreads, 0, stream>>>(op, g, sp, partials, SH[2], S, RH, Tv);
             }));
             CHK(timed(5 + PK_UPD_XRP, [&] {
-                k_update_xrp<float><<<fg, kBlock, 0, stream>>>(N, sp, X, PH[2], SH[2], S, Tv, R, Pv, V, DI, PH[0]);
+                k_update_xrp<float><<<fg, kBlock, 0, stream>>>(N, sp, X, PH[2], SH[2], S, Tv, R, Pv, V, nullptr,
+                                                                 nullptr);
             }));
         }
         HIPCHK(hipGetLastError());

@@ -869,7 +869,8 @@ __global__ __launch_bounds__(kThreads) void k_sweep_omega(Op op, Geo g, KState *
 }
 
 // x += alpha p_hat + omega s_hat (vec_axpy2, :755); r = s - omega t (:771);
-// p = r + beta (p - omega v) (vec_p_update, :783); next p_hat(sweep 1) = D^-1 p.
+// p = r + beta (p - omega v) (vec_p_update, :783); next p_hat(sweep 1) = D^-1 p
+// (skipped when dinv is null: the line-preconditioned iteration has no use for it).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_update_xrp(long N, const KState *st, T *__restrict__ x,
                                                        const T *__restrict__ ph, const T *__restrict__ sh,
@@ -889,7 +890,7 @@ __global__ __launch_bounds__(kBlock) void k_update_xrp(long N, const KState *st,
             if (mode >= 3) {
                 const T pv = rv + be * (p[n] - om * v[n]);
                 p[n] = pv;
-                ph_next[n] = dinv[n] * pv;
+                if (dinv) ph_next[n] = dinv[n] * pv;
             }
         }
     }
@@ -1414,6 +1415,65 @@ __global__ __launch_bounds__(kColThreads) void k_line_solve_c(Geo g, const KStat
                 delta = d[e] - c[e] * delta;
                 if (t.valid) bstore(yv[e] + delta, ro, voc, (unsigned)(k1 - e) * pb);
             }
+        }
+    }
+}
+
+// The same solve for columns of at most KM levels with the forward results kept
+// in registers: every level's loads are issued up front in batches, the
+// forward recursion fills dq[], the back substitution reads it back, so the
+// intermediate never goes to memory (5 arrays of traffic, 6 with y, instead
+// of 7 and 8).
+template <int KM>
+__global__ __launch_bounds__(kColThreads) void k_line_solve_r(Geo g, const KState *st, const float *__restrict__ in,
+                                                              const float *__restrict__ lo,
+                                                              const float *__restrict__ cp,
+                                                              const float *__restrict__ ip,
+                                                              const float *__restrict__ y, float *__restrict__ out)
+{
+    if (st->done) return;
+    const ColPos t = col_pos(g, gridDim.x);
+    const int mz = __builtin_amdgcn_readfirstlane(g.mz);
+    const long N = g.P * mz;
+    const unsigned voc = t.oc * 4u, pb = (unsigned)(g.P * 4);
+    const long nbt = N * 4;
+    const rsrc_t rin = make_rsrc(in, nbt), rlo = make_rsrc(lo, nbt), rip = make_rsrc(ip, nbt),
+                 rcp = make_rsrc(cp, nbt), ro = make_rsrc(out, nbt);
+    const bool hy = y != nullptr;
+    const rsrc_t ry = make_rsrc(hy ? y : in, nbt);
+    float dq[KM];
+    float dprev = 0.f;
+#pragma unroll
+    for (int k0 = 0; k0 < KM; k0 += kLB) {
+        float a[kLB], l[kLB], q[kLB];
+#pragma unroll
+        for (int e = 0; e < kLB; ++e) {
+            const unsigned so = (unsigned)(k0 + e < mz ? k0 + e : mz - 1) * pb;
+            a[e] = bload<float>(rin, voc, so);
+            l[e] = bload<float>(rlo, voc, so);
+            q[e] = bload<float>(rip, voc, so);
+        }
+#pragma unroll
+        for (int e = 0; e < kLB; ++e) {
+            dprev = (a[e] - l[e] * dprev) * q[e];
+            dq[k0 + e] = dprev;  // levels >= mz hold values that are never used
+        }
+    }
+    float delta = 0.f;
+#pragma unroll
+    for (int k1 = KM - 1; k1 >= 0; k1 -= kLB) {
+        float c[kLB], yv[kLB];
+#pragma unroll
+        for (int e = 0; e < kLB; ++e) {
+            const unsigned so = (unsigned)(k1 - e < mz ? k1 - e : mz - 1) * pb;
+            c[e] = bload<float>(rcp, voc, so);
+            yv[e] = hy ? bload<float>(ry, voc, so) : 0.f;
+        }
+#pragma unroll
+        for (int e = 0; e < kLB; ++e) {
+            const int k = k1 - e;
+            delta = k < mz ? dq[k] - c[e] * delta : 0.f;
+            if (t.valid && k < mz) bstore(yv[e] + delta, ro, voc, (unsigned)k * pb);
         }
     }
 }

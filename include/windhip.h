@@ -219,8 +219,8 @@ int windhip_learned_input(windhip_solver *s, float *in_dev, int *dims_out, doubl
  * 10 set-up, 11 FP64 residual of the mixed solve, 12 mixed-solve lambda update,
  * 13 true residual, 14 FGMRES inner set-up, 15 FGMRES FP64 matvec,
  * 16 FGMRES Gram-Schmidt step; MIXED_LINE inner iteration: 17 line solve,
- * 18 line step (y + L^-1 (in - A y)), 19 v = A p_hat + sigma, 20 t = A s_hat +
- * 5 dots (its s- and x/r/p-updates count as 7 and 9); the rest unused. reset != 0 clears the
+ * 18 residual in - A y, 19 v = A p_hat + sigma, 20 t = A s_hat + 5 dots (its
+ * s- and x/r/p-updates count as 7 and 9); the rest unused. reset != 0 clears the
  * counters after reading. */
 #define WINDHIP_PROFILE_KINDS 24
 int windhip_profile(windhip_solver *s, int reset, double *calls, double *ms);
