@@ -13,8 +13,12 @@ Arms (each solves the same system to the same tolerance):
   mixed          FP64 FGMRES around that BiCGStab in FP32
   line           FP64 FGMRES around an FP32 BiCGStab preconditioned by two
                  vertical line-relaxation steps
-  line_learned   `line` started from the learned guess (the headline `value`)
-  mixed_learned  `mixed` started from the learned guess
+  mg             FP64 FGMRES around an FP32 BiCGStab preconditioned by one
+                 multigrid V-cycle (horizontal aggregation, Galerkin coarse
+                 levels, vertical line smoothing)
+  mg_learned     `mg` with the learned guess as its first FGMRES direction
+                 (the headline `value`: learned guess + FP32/FP64 refinement)
+  line_learned, mixed_learned   `line`, `mixed` with the learned guess
 Errors (relative L2) of every arm are reported for lambda, the corrected u,
 v, w and the divergence after the correction, against the plain FP64 solve
 at the same tolerance and against a tight FP64 reference (the plain FP64
@@ -81,6 +85,10 @@ def kernel_bytes(kind: str, nx: int, ny: int, nz: int) -> float:
         return 3 * N * 4
     if kind == "update_xrp_line_f32":  # x, p_hat, s_hat, s, t, p, v -> x, r, p
         return 10 * N * 4
+    if kind == "mg_transfer_f32":  # restriction (r -> b_1) and prolongation (y += P e_1) alternate
+        return 0.5 * (1.25 + 2.25) * N * 4
+    if kind in ("mg_coarse_f32", "mg_setup"):  # every kernel of the coarse levels / the set-up: no single size
+        return None
     base = kind.rsplit("_", 1)[0]
     return float(table[base])
 
@@ -123,7 +131,7 @@ def main():
     ap.add_argument("--ny", type=int, default=512)
     ap.add_argument("--nz", type=int, default=40)
     ap.add_argument("--inner-tol", type=float, default=0.5)
-    ap.add_argument("--arms", default="plain,mixed,line,line_learned,mixed_learned")
+    ap.add_argument("--arms", default="plain,mixed,line,mg,mg_learned")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=6)
     ap.add_argument("--tight", type=float, default=1e-11,
@@ -175,17 +183,20 @@ def main():
         "line": dict(precision=nat.MIXED_LINE, start=nat.START_ZERO),
         "line_learned": dict(precision=nat.MIXED_LINE, start=nat.START_LEARNED),
         "mixed_learned": dict(precision=nat.MIXED, start=nat.START_LEARNED),
+        "mg": dict(precision=nat.MIXED_MG, start=nat.START_ZERO, inner_tol=0.1),
+        "mg_learned": dict(precision=nat.MIXED_MG, start=nat.START_LEARNED, inner_tol=0.1),
     }
     want = [a for a in args.arms.split(",") if a in arms and (net is not None or "learned" not in a)]
     zeros = np.zeros(s.rows_shape)
 
     def solve(arm, profile=0, tol=1e-8):
-        o = arms[arm]
+        o = dict(arms[arm])
+        inner = o.pop("inner_tol", args.inner_tol)
         learned.attach(s, net if o["start"] == nat.START_LEARNED else None)
         s.set_lambda(zeros)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        info = s.solve(alpha, div, tol_rel=tol, max_iters=50000, inner_tol=args.inner_tol, profile=profile, **o)
+        info = s.solve(alpha, div, tol_rel=tol, max_iters=50000, inner_tol=inner, profile=profile, **o)
         torch.cuda.synchronize()
         return 1e3 * (time.perf_counter() - t0), info
 
@@ -240,23 +251,25 @@ def main():
         log(f"{arm}: {r['ms']:.1f} ms/solve iters={r['iters']} outer={r['outer']} status={r['status']} "
             f"true_rel_res={r['true_rel_res']:.3e} start_used={r['start_used']}")
 
-    head = "line_learned" if "line_learned" in res else ("line" if "line" in res else next(iter(res)))
+    head = next((a for a in ("mg_learned", "mg", "line_learned", "line") if a in res), next(iter(res)))
     ms_h = res[head]["ms"]
 
     def kernel_table(prof, arm):
         out = {}
         for kind, (calls, tot) in prof.items():
-            if arm.startswith("line") and kind in ("update_s_f32", "update_xrp_f32"):
+            if arm.startswith(("line", "mg")) and kind in ("update_s_f32", "update_xrp_f32"):
                 kind = kind[:-4] + "_line_f32"  # without the Jacobi sweep's D^-1 terms
             avg = tot / calls
-            gbs = kernel_bytes(kind, case.nx, case.ny, case.nz) / (avg * 1e-3) / 1e9
-            out[kind] = dict(calls=int(calls), avg_ms=round(avg, 5), total_ms=round(tot, 2), GBs=round(gbs, 1),
-                             frac=round(gbs / HBM_PEAK_GBS, 3))
+            nb = kernel_bytes(kind, case.nx, case.ny, case.nz)
+            gbs = nb / (avg * 1e-3) / 1e9 if nb else None
+            out[kind] = dict(calls=int(calls), avg_ms=round(avg, 5), total_ms=round(tot, 2),
+                             GBs=round(gbs, 1) if gbs else None, frac=round(gbs / HBM_PEAK_GBS, 3) if gbs else None)
         return out
 
     tables = {arm: kernel_table(r.pop("prof"), arm) for arm, r in res.items()}
     kh = tables[head]
-    dom = max(kh, key=lambda k: kh[k]["total_ms"])
+    # the dominant single kernel (the coarse-level kind mixes every kernel of every coarse level)
+    dom = max((k for k in kh if kh[k]["GBs"]), key=lambda k: kh[k]["total_ms"])
     d = kh[dom]
     traffic = None
     if os.path.exists(TRAFFIC):
@@ -266,6 +279,13 @@ def main():
             traffic = tr["bytes_per_launch"]
     roof = dict(bound="hbm", achieved=d["GBs"], peak=HBM_PEAK_GBS, unit="GB/s", frac=d["frac"], traffic=traffic,
                 kernel=dom, bytes_per_launch=kernel_bytes(dom, case.nx, case.ny, case.nz), avg_ms=d["avg_ms"])
+    # the plain FP64 solve's matvec (fused with the last Jacobi sweep: v = A p_hat and sigma)
+    plain_mv = None
+    if "plain" in tables and "sweep_sigma_f64" in tables["plain"]:
+        pm = tables["plain"]["sweep_sigma_f64"]
+        plain_mv = dict(kernel="sweep_sigma_f64", achieved=pm["GBs"], peak=HBM_PEAK_GBS, unit="GB/s",
+                        frac=pm["frac"], avg_ms=pm["avg_ms"],
+                        bytes_per_launch=kernel_bytes("sweep_sigma_f64", case.nx, case.ny, case.nz))
 
     netinfo = None
     if net is not None:
@@ -298,6 +318,7 @@ def main():
                    "tol_rel": 1e-8, "inner_tol": args.inner_tol, "cells": case.cells,
                    "learned_weights": os.path.relpath(WEIGHTS, ROOT) if net is not None else None},
         "roofline": roof,
+        "plain_matvec_roofline": plain_mv,
         "cpu_baseline": cpu,
         "speedup_vs_plain": round(plain_ms / ms_h, 3) if plain_ms else None,
         "cells_per_s": case.cells / (ms_h * 1e-3),

@@ -24,7 +24,7 @@ WINDHIP_HOST = 0
 WINDHIP_DEVICE = 1
 
 START_WARM, START_ZERO, START_GIVEN, START_LEARNED = 0, 1, 2, 3
-PLAIN_F64, MIXED, MIXED_LINE = 0, 1, 2
+PLAIN_F64, MIXED, MIXED_LINE, MIXED_MG = 0, 1, 2, 3
 
 # every entry point declared in include/windhip.h
 EXPORTS = (
@@ -36,6 +36,7 @@ EXPORTS = (
     "windhip_net_param_count", "windhip_net_create", "windhip_net_destroy", "windhip_net_last_error",
     "windhip_net_set_params", "windhip_net_get_params", "windhip_net_forward", "windhip_net_loss_grad",
     "windhip_net_adam_step", "windhip_net_time_forward", "windhip_set_net", "windhip_learned_input", "windhip_set_guess_smoothing",
+    "windhip_set_multigrid", "windhip_mg_level",
 )
 
 PROFILE_KINDS = {0: "sweep_f64", 1: "sweep_sigma_f64", 2: "update_s_f64", 3: "sweep_omega_f64",
@@ -43,7 +44,8 @@ PROFILE_KINDS = {0: "sweep_f64", 1: "sweep_sigma_f64", 2: "update_s_f64", 3: "sw
                  8: "sweep_omega_f32", 9: "update_xrp_f32",
                  10: "init_f64", 11: "ir_resid", 12: "ir_accumulate", 13: "true_resid",
                  14: "fg_setup", 15: "fg_matvec", 16: "fg_axpy_dot", 17: "line_solve_f32",
-                 18: "resid_f32", 19: "mv_sigma_f32", 20: "mv_omega_f32"}
+                 18: "resid_f32", 19: "mv_sigma_f32", 20: "mv_omega_f32", 21: "mg_transfer_f32",
+                 22: "mg_coarse_f32", 23: "mg_setup"}
 PROFILE_SLOTS = 24  # WINDHIP_PROFILE_KINDS
 
 
@@ -141,6 +143,8 @@ def load():
         "windhip_net_time_forward": (i, [vp, i, i, i, i, i, dp]),
         "windhip_set_net": (i, [vp, vp]),
         "windhip_set_guess_smoothing": (i, [vp, i]),
+        "windhip_set_multigrid": (i, [vp, i, i, d, d]),
+        "windhip_mg_level": (i, [vp, i, C.POINTER(C.c_int), fp]),
         "windhip_learned_input": (i, [vp, vp, C.POINTER(C.c_int), dp]),
     }
     for name, (res, args) in sig.items():

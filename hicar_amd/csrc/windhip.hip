@@ -16,6 +16,7 @@
 #include "windhip.h"
 #include "windhip_kernels.hpp"
 #include "windhip_setup.hpp"
+#include "windhip_mg.hpp"
 #include "windhip_net.hpp"
 
 using namespace windhip;
@@ -117,6 +118,17 @@ struct windhip_solver {
     double *lr_c = nullptr, *lr_d = nullptr;   // line relaxation: Thomas coefficients
     // MIXED_LINE: FP32 Thomas factors of the column tridiagonal and scratch
     float *lf_lo = nullptr, *lf_cp = nullptr, *lf_ip = nullptr, *lf_dq = nullptr, *lf_r = nullptr;
+    // MIXED_MG: the coarse levels of the semi-coarsening hierarchy (the fine
+    // level is the solver's grid with the line factors lf_*): Galerkin
+    // coefficients (15 x N), line factors, correction, rhs, residual, spare
+    struct MgLevel {
+        MgGeo g;
+        float *A, *lo, *cp, *ip, *x, *b, *r, *t;
+    };
+    std::vector<MgLevel> mg;
+    int mg_pre = 2, mg_post = 2;  // line-smoothing steps before and after the coarse correction
+    float mg_omega = 0.8f;        // damping of the line smoother
+    double mg_hs = 0.7;           // scale of the coarse same-level horizontal couplings
     bool metrics_set = false, structure_uploaded = false, probed = false;
     bool verbose = false;  // WINDHIP_VERBOSE=1: per-batch progress on stderr (verbose_solver, :274-275)
     int nsweeps = 2;  // precond_n_sweeps, BASE_PREC_SWEEPS (wind_iterative.F90:76)
@@ -284,7 +296,9 @@ struct windhip_solver {
     int ev_next = 0;
     double prof_ms[WINDHIP_PROFILE_KINDS] = {}, prof_calls[WINDHIP_PROFILE_KINDS] = {};
     int launch_batch(int precision, int count, bool use_graph);
-    int enqueue_iterations_line(int count);
+    int enqueue_iterations_line(int count, bool multigrid);
+    int build_mg();
+    float *mg_coarse(size_t l);
     int alloc_f32_krylov();
     int alloc_fgmres(int m);
     int correction(void *u, void *v, void *w, const void *density, int adv_den);
@@ -494,7 +508,7 @@ int windhip_solver::read_state(int idx)
 enum { PK_SWEEP = 0, PK_SIGMA = 1, PK_UPD_S = 2, PK_OMEGA = 3, PK_UPD_XRP = 4, PK_INIT = 10,
        PK_IR_RESID = 11, PK_IR_ACC = 12, PK_TRUE_RESID = 13, PK_FG_SETUP = 14, PK_FG_MATVEC = 15,
        PK_FG_AXPY_DOT = 16, PK_LINE_SOLVE = 17, PK_RESID = 18, PK_MV_SIGMA = 19, PK_MV_OMEGA = 20,
-       PK_COUNT = 21 };
+       PK_MG_TRANSFER = 21, PK_MG_COARSE = 22, PK_MG_SETUP = 23, PK_COUNT = 24 };
 
 template <class Launch>
 int windhip_solver::timed(int kind, Launch &&launch)
@@ -583,11 +597,11 @@ int windhip_solver::enqueue_iterations(int count)
     });
 }
 
-// One line-preconditioned FP32 BiCGStab iteration per repetition: p_hat =
-// M^-1 p (line solve, residual, line solve), v = A p_hat with sigma,
-// s = r - alpha v, s_hat = M^-1 s, t = A s_hat with the five dots, the x/r/p
-// update.
-int windhip_solver::enqueue_iterations_line(int count)
+// One preconditioned FP32 BiCGStab iteration per repetition: p_hat = M^-1 p,
+// v = A p_hat with sigma, s = r - alpha v, s_hat = M^-1 s, t = A s_hat with
+// the five dots, the x/r/p update. M^-1 is two vertical line-relaxation steps
+// (line solve, residual, line solve) or, with multigrid, one V-cycle.
+int windhip_solver::enqueue_iterations_line(int count, bool multigrid)
 {
     KState *sp = st + 1;
     float *X, *R, *RH, *Pv, *V, *S, *Tv, *PH[3], *SH[3];
@@ -598,18 +612,52 @@ int windhip_solver::enqueue_iterations_line(int count)
     return with_op<float>([&](auto op) -> int {
         using Op = decltype(op);
         // M^-1 in: y1 = L^-1 in, r = in - A y1, out = y1 + L^-1 r
-        auto lsolve = [&](const float *in, const float *y, float *out) {
+        auto lsolve = [&](const float *in, const float *y, float *out, float w) {
             if (g.mz <= 48)
-                k_line_solve_r<48><<<cb, kColThreads, 0, stream>>>(g, sp, in, lf_lo, lf_cp, lf_ip, y, out);
+                k_line_solve_r<48><<<cb, kColThreads, 0, stream>>>(g, sp, in, lf_lo, lf_cp, lf_ip, y, out, w);
             else if (g.mz <= 64)
-                k_line_solve_r<64><<<cb, kColThreads, 0, stream>>>(g, sp, in, lf_lo, lf_cp, lf_ip, y, out);
+                k_line_solve_r<64><<<cb, kColThreads, 0, stream>>>(g, sp, in, lf_lo, lf_cp, lf_ip, y, out, w);
             else
-                k_line_solve_c<<<cb, kColThreads, 0, stream>>>(g, sp, in, lf_lo, lf_cp, lf_ip, lf_dq, y, out);
+                k_line_solve_c<<<cb, kColThreads, 0, stream>>>(g, sp, in, lf_lo, lf_cp, lf_ip, lf_dq, y, out, w);
         };
+        auto resid = [&](const float *y, const float *in, float *r) {
+            return timed(PK_RESID, [&] { k_resid_c<Op><<<kb, kColThreads, 0, stream>>>(op, g, sp, y, in, r); });
+        };
+        const MgGeo f0 = mg_geo(g.mx - 2, g.my - 2, g.mz);
+        // out = M^-1 in, y1 and lf_dq as scratch
         auto precond = [&](const float *in, float *y1, float *out) -> int {
-            CHK(timed(PK_LINE_SOLVE, [&] { lsolve(in, nullptr, y1); }));
-            CHK(timed(PK_RESID, [&] { k_resid_c<Op><<<kb, kColThreads, 0, stream>>>(op, g, sp, y1, in, lf_r); }));
-            CHK(timed(PK_LINE_SOLVE, [&] { lsolve(lf_r, y1, out); }));
+            if (!multigrid) {
+                CHK(timed(PK_LINE_SOLVE, [&] { lsolve(in, nullptr, y1, 1.0f); }));
+                CHK(resid(y1, in, lf_r));
+                CHK(timed(PK_LINE_SOLVE, [&] { lsolve(lf_r, y1, out, 1.0f); }));
+                return WINDHIP_OK;
+            }
+            // V-cycle from a zero guess: damped line smoothing, residual,
+            // restriction, coarse levels, prolongation, line smoothing
+            float *cur = y1, *alt = lf_dq;
+            CHK(timed(PK_LINE_SOLVE, [&] { lsolve(in, nullptr, cur, mg_omega); }));
+            for (int q = 1; q < mg_pre; ++q) {
+                CHK(resid(cur, in, lf_r));
+                CHK(timed(PK_LINE_SOLVE, [&] { lsolve(lf_r, cur, alt, mg_omega); }));
+                std::swap(cur, alt);
+            }
+            if (!mg.empty()) {
+                CHK(resid(cur, in, lf_r));
+                CHK(timed(PK_MG_TRANSFER, [&] {
+                    k_mg_restrict<<<mg_blocks(mg[0].g.N), kMgThreads, 0, stream>>>(f0, mg[0].g, sp, lf_r, mg[0].b);
+                }));
+                float *e = mg_coarse(0);
+                CHK(timed(PK_MG_TRANSFER, [&] {
+                    k_mg_prolong<<<mg_blocks(f0.N), kMgThreads, 0, stream>>>(f0, mg[0].g, sp, e, cur);
+                }));
+            }
+            for (int q = 0; q < mg_post; ++q) {
+                float *dst = q == mg_post - 1 ? out : alt;
+                CHK(resid(cur, in, lf_r));
+                CHK(timed(PK_LINE_SOLVE, [&] { lsolve(lf_r, cur, dst, mg_omega); }));
+                alt = cur;
+                cur = dst;
+            }
             return WINDHIP_OK;
         };
         for (int it = 0; it < count; ++it) {
@@ -632,12 +680,103 @@ int windhip_solver::enqueue_iterations_line(int count)
     });
 }
 
+// Enqueue the V-cycle below the fine level from coarse level l (rhs in
+// mg[l].b); returns the buffer that holds its correction.
+float *windhip_solver::mg_coarse(size_t l)
+{
+    MgLevel &L = mg[l];
+    const KState *sp = st + 1;
+    const int nr = mg_blocks(L.g.N);
+    // the level as a column grid for the register-resident line solve
+    Geo gl{};
+    gl.mx = L.g.mx; gl.my = L.g.my; gl.mz = L.g.mz; gl.P = L.g.P;
+    const int cb = col_blocks(gl);
+    auto line = [&](const float *in, const float *y, float *out, float w) {
+        timed(PK_MG_COARSE, [&] {
+            if (gl.mz <= 48)
+                k_line_solve_r<48><<<cb, kColThreads, 0, stream>>>(gl, sp, in, L.lo, L.cp, L.ip, y, out, w);
+            else if (gl.mz <= 64)
+                k_line_solve_r<64><<<cb, kColThreads, 0, stream>>>(gl, sp, in, L.lo, L.cp, L.ip, y, out, w);
+            else
+                k_mg_line<<<mg_blocks(L.g.P), kMgThreads, 0, stream>>>(L.g, sp, L.lo, L.cp, L.ip, in, y, out, w);
+        });
+    };
+    auto resid = [&](const float *xv) {
+        timed(PK_MG_COARSE, [&] { k_mg_resid<<<nr, kMgThreads, 0, stream>>>(L.g, sp, L.A, xv, L.b, L.r); });
+    };
+    if (l + 1 == mg.size()) {  // one interior column: its line solve is exact
+        line(L.b, nullptr, L.x, 1.0f);
+        return L.x;
+    }
+    float *cur = L.x, *alt = L.t;
+    line(L.b, nullptr, cur, mg_omega);
+    for (int q = 1; q < mg_pre; ++q) {
+        resid(cur);
+        line(L.r, cur, alt, mg_omega);
+        std::swap(cur, alt);
+    }
+    resid(cur);
+    const MgLevel &C = mg[l + 1];
+    timed(PK_MG_COARSE, [&] {
+        k_mg_restrict<<<mg_blocks(C.g.N), kMgThreads, 0, stream>>>(L.g, C.g, sp, L.r, C.b);
+    });
+    float *e = mg_coarse(l + 1);
+    timed(PK_MG_COARSE, [&] { k_mg_prolong<<<nr, kMgThreads, 0, stream>>>(L.g, C.g, sp, e, cur); });
+    for (int q = 0; q < mg_post; ++q) {
+        resid(cur);
+        line(L.r, cur, alt, mg_omega);
+        std::swap(cur, alt);
+    }
+    return cur;
+}
+
+// The coarse levels (allocated once) and their Galerkin operators, rebuilt
+// from the current FP64 operator.
+int windhip_solver::build_mg()
+{
+    const MgGeo f0 = mg_geo(g.mx - 2, g.my - 2, g.mz);
+    if (mg.empty()) {
+        MgGeo f = f0;
+        while (f.nx > 1 || f.ny > 1) {
+            MgLevel L{};
+            L.g = mg_geo((f.nx + 1) / 2, (f.ny + 1) / 2, g.mz);
+            CHK(alloc(&L.A, 15 * L.g.N));
+            CHK(alloc(&L.lo, L.g.N));
+            CHK(alloc(&L.cp, L.g.N));
+            CHK(alloc(&L.ip, L.g.N));
+            CHK(alloc(&L.x, L.g.N));
+            CHK(alloc(&L.b, L.g.N));
+            CHK(alloc(&L.r, L.g.N));
+            CHK(alloc(&L.t, L.g.N));
+            mg.push_back(L);
+            f = L.g;
+        }
+    }
+    if (mg.empty()) return WINDHIP_OK;
+    CHK(with_op<double>([&](auto op) {
+        return timed(PK_MG_SETUP, [&] {
+            MgExtractOp<decltype(op)> ex{op, g.P, g.mz};
+            k_mg_galerkin<<<mg_blocks(mg[0].g.P), kMgThreads, 0, stream>>>(ex, f0, mg[0].g, mg[0].A, mg[0].lo,
+                                                                          mg[0].cp, mg[0].ip, mg_hs);
+        });
+    }));
+    for (size_t l = 1; l < mg.size(); ++l) {
+        const MgLevel &F = mg[l - 1], &L = mg[l];
+        CHK(timed(PK_MG_SETUP, [&] {
+            MgExtractC ex{F.A, F.g.P, F.g.N};
+            k_mg_galerkin<<<mg_blocks(L.g.P), kMgThreads, 0, stream>>>(ex, F.g, L.g, L.A, L.lo, L.cp, L.ip, mg_hs);
+        }));
+    }
+    HIPCHK(hipGetLastError());
+    return WINDHIP_OK;
+}
+
 // Launch `count` iterations, from a cached hipGraph when use_graph.
 int windhip_solver::launch_batch(int precision, int count, bool use_graph)
 {
     auto enqueue = [&]() {
         if (precision == 0) return enqueue_iterations<double>(count);
-        if (precision == 2) return enqueue_iterations_line(count);
+        if (precision == 2 || precision == 3) return enqueue_iterations_line(count, precision == 3);
         return enqueue_iterations<float>(count);
     };
     if (!use_graph || profiling) return enqueue();
@@ -791,7 +930,8 @@ int windhip_solver::learned_start(double *res_guess, double *res_plain, double *
 int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *info, const double *first_dir)
 {
     CHK(alloc_f32_krylov());
-    const bool line = o.precision == WINDHIP_MIXED_LINE;
+    const bool multigrid = o.precision == WINDHIP_MIXED_MG;
+    const bool line = o.precision == WINDHIP_MIXED_LINE || multigrid;
     if (line) {
         if (!lf_lo) {
             CHK(alloc(&lf_lo, N));
@@ -805,6 +945,7 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
             return WINDHIP_OK;
         }));
         HIPCHK(hipGetLastError());
+        if (multigrid) CHK(build_mg());
     }
     const double inner_tol = (o.inner_tol > 0 && o.inner_tol < 1) ? o.inner_tol : 0.5;
     const int m = std::max(1, std::min(o.max_outer, kRestart));
@@ -862,7 +1003,7 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
                 int done_iters = 0;
                 while (!st_host[1].done) {
                     const int n = std::min(o.iters_per_sync, std::max(1, cap - done_iters));
-                    CHK(launch_batch(line ? 2 : 1, n, o.use_graph != 0));
+                    CHK(launch_batch(multigrid ? 3 : (line ? 2 : 1), n, o.use_graph != 0));
                     CHK(read_state(1));
                     done_iters += n;
                     if (done_iters > cap + o.iters_per_sync) break;
@@ -1210,13 +1351,15 @@ int windhip_solve(windhip_solver *s, const void *alpha, const void *div, const w
     if (o.iters_per_sync <= 0) {
         long per = 64000000L / std::max(1L, s->N);
         o.iters_per_sync = (int)std::max(4L, std::min(64L, per));
+        if (o.precision == WINDHIP_MIXED_MG) o.iters_per_sync = 2;  // a few V-cycle iterations per inner solve
     }
     s->profiling = o.profile != 0;
     HIPCHK(hipEventRecord(s->ev0, s->stream));
     CHK(s->prepare_call(alpha, div));
     if (o.start == WINDHIP_START_ZERO) HIPCHK(hipMemsetAsync(s->x, 0, s->N * sizeof(double), s->stream));
     inf.start_used = o.start;
-    const bool mixed = o.precision == WINDHIP_MIXED || o.precision == WINDHIP_MIXED_LINE;
+    const bool mixed = o.precision == WINDHIP_MIXED || o.precision == WINDHIP_MIXED_LINE ||
+                       o.precision == WINDHIP_MIXED_MG;
     double *aug = nullptr;  // mixed: the learned correction as the first FGMRES direction
     if (o.start == WINDHIP_START_LEARNED) {
         double rg = 0, rp = 0, *guess = nullptr;
@@ -1450,6 +1593,33 @@ int windhip_set_guess_smoothing(windhip_solver *s, int steps)
 {
     if (!s || steps < 0 || steps > 100) return fail(WINDHIP_ERR_ARG, "windhip_set_guess_smoothing: 0..100 steps");
     s->net_smooth = steps;
+    return WINDHIP_OK;
+}
+
+int windhip_set_multigrid(windhip_solver *s, int pre, int post, double omega, double hscale)
+{
+    if (!s || pre < 1 || pre > 8 || post < 1 || post > 8 || !(omega > 0 && omega <= 1.5) || !(hscale > 0 && hscale <= 1))
+        return fail(WINDHIP_ERR_ARG, "windhip_set_multigrid: pre, post in 1..8, omega in (0, 1.5], hscale in (0, 1]");
+    s->mg_pre = pre;
+    s->mg_post = post;
+    s->mg_omega = (float)omega;
+    s->mg_hs = hscale;  // the coarse operators are rebuilt by every MIXED_MG solve
+    for (auto &kv : s->graphs) (void)hipGraphExecDestroy(kv.second);  // the cached V-cycles change shape
+    s->graphs.clear();
+    return WINDHIP_OK;
+}
+
+int windhip_mg_level(windhip_solver *s, int level, int *dims_out, float *coef_host)
+{
+    if (!s || !dims_out) return fail(WINDHIP_ERR_ARG, "windhip_mg_level: null argument");
+    if (level < 1 || level > (int)s->mg.size())
+        return fail(WINDHIP_ERR_ARG, "windhip_mg_level: level %d outside 1..%d (run a MIXED_MG solve first)", level,
+                    (int)s->mg.size());
+    const auto &L = s->mg[level - 1];
+    dims_out[0] = L.g.nx; dims_out[1] = L.g.ny; dims_out[2] = L.g.mz;
+    if (!coef_host) return WINDHIP_OK;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipMemcpy(coef_host, L.A, sizeof(float) * 15 * L.g.N, hipMemcpyDeviceToHost));
     return WINDHIP_OK;
 }
 

@@ -1361,7 +1361,7 @@ __global__ __launch_bounds__(kColThreads) void k_resid_c(Op op, Geo g, const KSt
     march_pipe(op, g, c.t, y, c.k0, c.k1, epi);
 }
 
-// out = y + L^-1 in (y null: out = L^-1 in), the Thomas algorithm per column
+// out = y + w L^-1 in (y null: out = w L^-1 in), the Thomas algorithm per column
 // with the loads of kLB levels issued together on each pass (the recursion
 // itself is a few FMAs per level; its loads are what costs).
 constexpr int kLB = 8;
@@ -1369,7 +1369,8 @@ __global__ __launch_bounds__(kColThreads) void k_line_solve_c(Geo g, const KStat
                                                               const float *__restrict__ lo,
                                                               const float *__restrict__ cp,
                                                               const float *__restrict__ ip, float *__restrict__ dq,
-                                                              const float *__restrict__ y, float *__restrict__ out)
+                                                              const float *__restrict__ y, float *__restrict__ out,
+                                                              float w)
 {
     if (st->done) return;
     const ColPos t = col_pos(g, gridDim.x);
@@ -1413,7 +1414,7 @@ __global__ __launch_bounds__(kColThreads) void k_line_solve_c(Geo g, const KStat
         for (int e = 0; e < kLB; ++e) {
             if (k1 - e >= 0) {
                 delta = d[e] - c[e] * delta;
-                if (t.valid) bstore(yv[e] + delta, ro, voc, (unsigned)(k1 - e) * pb);
+                if (t.valid) bstore(yv[e] + w * delta, ro, voc, (unsigned)(k1 - e) * pb);
             }
         }
     }
@@ -1429,7 +1430,8 @@ __global__ __launch_bounds__(kColThreads) void k_line_solve_r(Geo g, const KStat
                                                               const float *__restrict__ lo,
                                                               const float *__restrict__ cp,
                                                               const float *__restrict__ ip,
-                                                              const float *__restrict__ y, float *__restrict__ out)
+                                                              const float *__restrict__ y, float *__restrict__ out,
+                                                              float w)
 {
     if (st->done) return;
     const ColPos t = col_pos(g, gridDim.x);
@@ -1473,7 +1475,7 @@ __global__ __launch_bounds__(kColThreads) void k_line_solve_r(Geo g, const KStat
         for (int e = 0; e < kLB; ++e) {
             const int k = k1 - e;
             delta = k < mz ? dq[k] - c[e] * delta : 0.f;
-            if (t.valid && k < mz) bstore(yv[e] + delta, ro, voc, (unsigned)k * pb);
+            if (t.valid && k < mz) bstore(yv[e] + w * delta, ro, voc, (unsigned)k * pb);
         }
     }
 }

@@ -171,6 +171,19 @@ class WindSolver:
         lam = np.ascontiguousarray(lam, dtype=np.float64).reshape(self.rows_shape)
         nat.check(self.lib.windhip_set_lambda(self.h, lam.ctypes.data_as(C.POINTER(C.c_double))))
 
+    def set_multigrid(self, pre: int = 2, post: int = 2, omega: float = 0.8, hscale: float = 0.7) -> None:
+        """V-cycle of the MIXED_MG inner preconditioner (no counterpart in the reference)."""
+        nat.check(self.lib.windhip_set_multigrid(self.h, int(pre), int(post), float(omega), float(hscale)))
+
+    def mg_level(self, level: int):
+        """Coarse level `level` of the last MIXED_MG solve: ((nx, ny, mz), coefficients [15, mz, ny+2, nx+2])."""
+        dims = (C.c_int * 3)()
+        nat.check(self.lib.windhip_mg_level(self.h, int(level), dims, None))
+        nx, ny, mz = dims[0], dims[1], dims[2]
+        out = np.zeros((15, mz, ny + 2, nx + 2), dtype=np.float32)
+        nat.check(self.lib.windhip_mg_level(self.h, int(level), dims, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return (nx, ny, mz), out
+
     def probe_operator(self, alpha, density=None, adv_den=True) -> float:
         """calibrate_projection_operator (wind.F90:1010-1099): A = 2 D(G(.))."""
         a = self._arr(alpha, self.mass_shape, "alpha")

@@ -55,9 +55,12 @@ enum { WINDHIP_START_WARM = 0, WINDHIP_START_ZERO = 1, WINDHIP_START_GIVEN = 2,
  * 2 Jacobi-Richardson sweeps) as the preconditioner of an FP64 flexible GMRES
  * refinement; the true residual, its norm, the rho/omega dot products and the
  * lambda accumulation stay FP64. MIXED_LINE: the same with the FP32 BiCGStab
- * preconditioned by two vertical line-relaxation steps instead (no
- * counterpart in the reference). */
-enum { WINDHIP_PLAIN_F64 = 0, WINDHIP_MIXED = 1, WINDHIP_MIXED_LINE = 2 };
+ * preconditioned by two vertical line-relaxation steps instead; MIXED_MG:
+ * preconditioned by one multigrid V-cycle (horizontal 2 x 2 aggregation down
+ * to one interior column, Galerkin coarse operators, damped vertical line
+ * smoothing; windhip_set_multigrid). Neither has a counterpart in the
+ * reference. */
+enum { WINDHIP_PLAIN_F64 = 0, WINDHIP_MIXED = 1, WINDHIP_MIXED_LINE = 2, WINDHIP_MIXED_MG = 3 };
 
 typedef struct windhip_desc {
     int nx, ny, nz;              /* physical cells (ide, jde, kde) */
@@ -72,7 +75,7 @@ typedef struct windhip_solve_opts {
     double tol_abs;        /* bicg_tol_abs (1e-10, :59) */
     int max_iters;         /* wind_solver_iterations (1500, :57); MIXED: cap on all FP32 iterations */
     int start;             /* WINDHIP_START_* */
-    int precision;         /* WINDHIP_PLAIN_F64, WINDHIP_MIXED or WINDHIP_MIXED_LINE */
+    int precision;         /* WINDHIP_PLAIN_F64, WINDHIP_MIXED, WINDHIP_MIXED_LINE or WINDHIP_MIXED_MG */
     double inner_tol;      /* MIXED: relative reduction asked of each FP32 inner solve (0.5) */
     int inner_max_iters;   /* MIXED: cap per inner solve */
     int max_outer;         /* MIXED: cap on FGMRES steps (= inner solves); cycles restart every 32 */
@@ -205,6 +208,19 @@ int windhip_net_time_forward(windhip_net *net, int B, int D, int H, int W, int r
  * parameters are all zero gives the relaxation of the zero start alone. */
 int windhip_set_net(windhip_solver *s, windhip_net *net);
 int windhip_set_guess_smoothing(windhip_solver *s, int steps);
+/* MIXED_MG V-cycle: `pre` and `post` damped line-smoothing steps (1..8, default
+ * 2 and 2) around each coarse correction, damping omega (default 0.8), and
+ * hscale (default 0.7), the factor on the coarse same-level horizontal
+ * couplings that keeps aggregated second differences at their coarse-grid
+ * size. No counterpart in the reference. */
+int windhip_set_multigrid(windhip_solver *s, int pre, int post, double omega, double hscale);
+/* Coarse level `level` (1 = first coarsening) of the MIXED_MG hierarchy built
+ * by the last MIXED_MG solve: dims_out = (nx, ny, mz) of the level (interior
+ * columns, rows per column); coef_host (may be null) receives its 15 x N
+ * FP32 coefficients, rows [k][j][i] of (nx+2) x (ny+2) columns, entry 0 the
+ * row sum and 1..14 the couplings B..O of the 15-point pattern. Inspection
+ * only; no counterpart in the reference. */
+int windhip_mg_level(windhip_solver *s, int level, int *dims_out, float *coef_host);
 /* The network input for the solver's current rhs (that of its last solve),
  * metrics and alpha, written to in_dev (1 x dims[0] x dims[1] x dims[2] x 8);
  * *scale turns the network output into lambda. dims_out receives the padded
@@ -220,7 +236,10 @@ int windhip_learned_input(windhip_solver *s, float *in_dev, int *dims_out, doubl
  * 13 true residual, 14 FGMRES inner set-up, 15 FGMRES FP64 matvec,
  * 16 FGMRES Gram-Schmidt step; MIXED_LINE inner iteration: 17 line solve,
  * 18 residual in - A y, 19 v = A p_hat + sigma, 20 t = A s_hat + 5 dots (its
- * s- and x/r/p-updates count as 7 and 9); the rest unused. reset != 0 clears the
+ * s- and x/r/p-updates count as 7 and 9); MIXED_MG adds 21 fine-level
+ * restriction and prolongation, 22 every kernel of the coarse levels, 23 the
+ * Galerkin set-up (its fine-level line solves and residuals count as 17 and
+ * 18). reset != 0 clears the
  * counters after reading. */
 #define WINDHIP_PROFILE_KINDS 24
 int windhip_profile(windhip_solver *s, int reset, double *calls, double *ms);
