@@ -126,8 +126,10 @@ struct windhip_solver {
         float *A, *lo, *cp, *ip, *x, *b, *r, *t;
     };
     std::vector<MgLevel> mg;
-    int mg_pre = 2, mg_post = 2;  // line-smoothing steps before and after the coarse correction
-    float mg_omega = 0.8f;        // damping of the line smoother
+    int mg_pre = 2, mg_post = 2;    // fine-level line-smoothing steps before and after the coarse correction
+    int mg_cpre = 1, mg_cpost = 1;  // the same on the coarse levels
+    int mg_depth = 0;               // coarse levels used (0: all, down to one interior column)
+    float mg_omega = 0.8f;          // damping of the line smoother
     double mg_hs = 0.7;           // scale of the coarse same-level horizontal couplings
     bool metrics_set = false, structure_uploaded = false, probed = false;
     bool verbose = false;  // WINDHIP_VERBOSE=1: per-batch progress on stderr (verbose_solver, :274-275)
@@ -704,13 +706,22 @@ float *windhip_solver::mg_coarse(size_t l)
     auto resid = [&](const float *xv) {
         timed(PK_MG_COARSE, [&] { k_mg_resid<<<nr, kMgThreads, 0, stream>>>(L.g, sp, L.A, xv, L.b, L.r); });
     };
+    const size_t used = mg_depth > 0 ? std::min(mg.size(), (size_t)mg_depth) : mg.size();
     if (l + 1 == mg.size()) {  // one interior column: its line solve is exact
         line(L.b, nullptr, L.x, 1.0f);
         return L.x;
     }
     float *cur = L.x, *alt = L.t;
     line(L.b, nullptr, cur, mg_omega);
-    for (int q = 1; q < mg_pre; ++q) {
+    if (l + 1 == used) {  // the coarsest level in use: smoothing only
+        for (int q = 1; q < mg_cpre + mg_cpost; ++q) {
+            resid(cur);
+            line(L.r, cur, alt, mg_omega);
+            std::swap(cur, alt);
+        }
+        return cur;
+    }
+    for (int q = 1; q < mg_cpre; ++q) {
         resid(cur);
         line(L.r, cur, alt, mg_omega);
         std::swap(cur, alt);
@@ -722,7 +733,7 @@ float *windhip_solver::mg_coarse(size_t l)
     });
     float *e = mg_coarse(l + 1);
     timed(PK_MG_COARSE, [&] { k_mg_prolong<<<nr, kMgThreads, 0, stream>>>(L.g, C.g, sp, e, cur); });
-    for (int q = 0; q < mg_post; ++q) {
+    for (int q = 0; q < mg_cpost; ++q) {
         resid(cur);
         line(L.r, cur, alt, mg_omega);
         std::swap(cur, alt);
@@ -753,19 +764,25 @@ int windhip_solver::build_mg()
         }
     }
     if (mg.empty()) return WINDHIP_OK;
+    auto factor = [&](const MgLevel &L) {
+        return timed(PK_MG_SETUP, [&] {
+            k_mg_factor<<<mg_blocks(L.g.P), kMgThreads, 0, stream>>>(L.g, L.A, L.lo, L.cp, L.ip);
+        });
+    };
     CHK(with_op<double>([&](auto op) {
         return timed(PK_MG_SETUP, [&] {
             MgExtractOp<decltype(op)> ex{op, g.P, g.mz};
-            k_mg_galerkin<<<mg_blocks(mg[0].g.P), kMgThreads, 0, stream>>>(ex, f0, mg[0].g, mg[0].A, mg[0].lo,
-                                                                          mg[0].cp, mg[0].ip, mg_hs);
+            k_mg_galerkin<<<mg_blocks(mg[0].g.N), kMgThreads, 0, stream>>>(ex, f0, mg[0].g, mg[0].A, mg_hs);
         });
     }));
+    CHK(factor(mg[0]));
     for (size_t l = 1; l < mg.size(); ++l) {
         const MgLevel &F = mg[l - 1], &L = mg[l];
         CHK(timed(PK_MG_SETUP, [&] {
             MgExtractC ex{F.A, F.g.P, F.g.N};
-            k_mg_galerkin<<<mg_blocks(L.g.P), kMgThreads, 0, stream>>>(ex, F.g, L.g, L.A, L.lo, L.cp, L.ip, mg_hs);
+            k_mg_galerkin<<<mg_blocks(L.g.N), kMgThreads, 0, stream>>>(ex, F.g, L.g, L.A, mg_hs);
         }));
+        CHK(factor(L));
     }
     HIPCHK(hipGetLastError());
     return WINDHIP_OK;
@@ -1596,12 +1613,19 @@ int windhip_set_guess_smoothing(windhip_solver *s, int steps)
     return WINDHIP_OK;
 }
 
-int windhip_set_multigrid(windhip_solver *s, int pre, int post, double omega, double hscale)
+int windhip_set_multigrid(windhip_solver *s, int pre, int post, int coarse_pre, int coarse_post, int depth,
+                          double omega, double hscale)
 {
-    if (!s || pre < 1 || pre > 8 || post < 1 || post > 8 || !(omega > 0 && omega <= 1.5) || !(hscale > 0 && hscale <= 1))
-        return fail(WINDHIP_ERR_ARG, "windhip_set_multigrid: pre, post in 1..8, omega in (0, 1.5], hscale in (0, 1]");
+    auto ok = [](int n) { return n >= 1 && n <= 8; };
+    if (!s || !ok(pre) || !ok(post) || !ok(coarse_pre) || !ok(coarse_post) || depth < 0 ||
+        !(omega > 0 && omega <= 1.5) || !(hscale > 0 && hscale <= 1))
+        return fail(WINDHIP_ERR_ARG, "windhip_set_multigrid: smoothing steps in 1..8, depth >= 0, omega in (0, 1.5], "
+                                     "hscale in (0, 1]");
     s->mg_pre = pre;
     s->mg_post = post;
+    s->mg_cpre = coarse_pre;
+    s->mg_cpost = coarse_post;
+    s->mg_depth = depth;
     s->mg_omega = (float)omega;
     s->mg_hs = hscale;  // the coarse operators are rebuilt by every MIXED_MG solve
     for (auto &kv : s->graphs) (void)hipGraphExecDestroy(kv.second);  // the cached V-cycles change shape

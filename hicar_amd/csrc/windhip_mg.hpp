@@ -138,57 +138,71 @@ __device__ __forceinline__ int mg_cmap(int i, int nf, int nc)
     return (i + 1) >> 1;
 }
 
-// Galerkin coarse operator of level c from level f, and the FP64 Thomas
-// factors (lo, cp, 1/pivot) of every coarse column's tridiagonal (C, A, B).
+// Galerkin coarse operator of level c from level f, one coarse row per thread.
 template <class Ext>
 __global__ __launch_bounds__(kMgThreads) void k_mg_galerkin(Ext ext, MgGeo f, MgGeo c, float *__restrict__ Ac,
-                                                            float *__restrict__ lo, float *__restrict__ cp,
-                                                            float *__restrict__ ip, double hs)
+                                                            double hs)
+{
+    const long n = (long)blockIdx.x * kMgThreads + threadIdx.x;
+    if (n >= c.N) return;
+    const int k = (int)(n / c.P);
+    const long cc = n - (long)k * c.P;
+    const int I = (int)(cc % c.mx), J = (int)(cc / c.mx);
+    double a[15];
+#pragma unroll
+    for (int q = 0; q < 15; ++q) a[q] = 0.0;
+    if (I == 0 || I == c.mx - 1 || J == 0 || J == c.my - 1) {
+        a[0] = 1.0;
+    } else {
+        for (int dj = 0; dj < 2; ++dj) {
+            const int j = 2 * J - 1 + dj;
+            if (j > f.ny) continue;
+            for (int di = 0; di < 2; ++di) {
+                const int i = 2 * I - 1 + di;
+                if (i > f.nx) continue;
+                double fa[15];
+                ext.coefs(k, (long)j * f.mx + i, fa);
+                for (int q = 0; q < 15; ++q) {
+                    if (fa[q] == 0.0) continue;
+                    const int NI = mg_cmap(i + off_i(q), f.nx, c.nx), NJ = mg_cmap(j + off_j(q), f.ny, c.ny);
+                    const int qc = stencil_index(NI - I, off_k(q), NJ - J);
+                    if (qc >= 0) a[qc] += fa[q];
+                }
+            }
+        }
+        if (k >= 1 && k <= c.mz - 2) {
+            const double hsum = ((a[3] + a[4]) + a[5]) + a[6];
+            a[3] *= hs; a[4] *= hs; a[5] *= hs; a[6] *= hs;
+            a[0] += (1.0 - hs) * hsum;
+        }
+    }
+    double off = 0.0;
+    for (int q = 1; q < 15; ++q) {
+        Ac[q * c.N + n] = (float)a[q];
+        off += a[q];
+    }
+    Ac[n] = (float)(a[0] + off);  // row sum (zero up to rounding on every non-lateral row)
+}
+
+// Thomas factors (lo, cp, 1/pivot) in FP64 of every column's tridiagonal
+// (C, A, B) of level c as stored, i.e. of the operator the residual kernel
+// applies: the diagonal is s - sum of the couplings, written as h - B - C with
+// h = s - (couplings outside the column), so that the pivots of nearly
+// singular columns (derivative conditions at both ends, weak horizontal
+// coupling on the coarse levels) carry no cancellation of B and C.
+__global__ __launch_bounds__(kMgThreads) void k_mg_factor(MgGeo c, const float *__restrict__ Ac,
+                                                          float *__restrict__ lo, float *__restrict__ cp,
+                                                          float *__restrict__ ip)
 {
     const long cc = (long)blockIdx.x * kMgThreads + threadIdx.x;
     if (cc >= c.P) return;
-    const int I = (int)(cc % c.mx), J = (int)(cc / c.mx);
-    const bool lat = I == 0 || I == c.mx - 1 || J == 0 || J == c.my - 1;
     double cprev = 0.0;
     for (int k = 0; k < c.mz; ++k) {
         const long n = (long)k * c.P + cc;
-        double a[15];
-#pragma unroll
-        for (int q = 0; q < 15; ++q) a[q] = 0.0;
-        if (lat) {
-            a[0] = 1.0;
-        } else {
-            for (int dj = 0; dj < 2; ++dj) {
-                const int j = 2 * J - 1 + dj;
-                if (j > f.ny) continue;
-                for (int di = 0; di < 2; ++di) {
-                    const int i = 2 * I - 1 + di;
-                    if (i > f.nx) continue;
-                    double fa[15];
-                    ext.coefs(k, (long)j * f.mx + i, fa);
-                    for (int q = 0; q < 15; ++q) {
-                        if (fa[q] == 0.0) continue;
-                        const int NI = mg_cmap(i + off_i(q), f.nx, c.nx), NJ = mg_cmap(j + off_j(q), f.ny, c.ny);
-                        const int qc = stencil_index(NI - I, off_k(q), NJ - J);
-                        if (qc >= 0) a[qc] += fa[q];
-                    }
-                }
-            }
-            if (k >= 1 && k <= c.mz - 2) {
-                const double hsum = ((a[3] + a[4]) + a[5]) + a[6];
-                a[3] *= hs; a[4] *= hs; a[5] *= hs; a[6] *= hs;
-                a[0] += (1.0 - hs) * hsum;
-            }
-        }
-        double off = 0.0;
-        for (int q = 1; q < 15; ++q) {
-            Ac[q * c.N + n] = (float)a[q];
-            off += a[q];
-        }
-        Ac[n] = (float)(a[0] + off);  // row sum (zero up to rounding on every non-lateral row)
-        // Thomas factors of (l, d, u) = (C, A, B)
-        const double l = k > 0 ? a[2] : 0.0, d = a[0], u = k < c.mz - 1 ? a[1] : 0.0;
-        double piv = d - l * cprev;
+        double h = (double)Ac[n];
+        for (int q = 3; q < 15; ++q) h -= (double)Ac[q * c.N + n];
+        const double l = (double)Ac[2 * c.N + n], u = (double)Ac[c.N + n];
+        double piv = (h - u) - l * (1.0 + cprev);  // d - l cp_{k-1} with d = h - l - u
         if (fabs(piv) < 1e-300) piv = copysign(1e-300, piv);
         const double inv = 1.0 / piv;
         cprev = u * inv;

@@ -208,12 +208,16 @@ int windhip_net_time_forward(windhip_net *net, int B, int D, int H, int W, int r
  * parameters are all zero gives the relaxation of the zero start alone. */
 int windhip_set_net(windhip_solver *s, windhip_net *net);
 int windhip_set_guess_smoothing(windhip_solver *s, int steps);
-/* MIXED_MG V-cycle: `pre` and `post` damped line-smoothing steps (1..8, default
- * 2 and 2) around each coarse correction, damping omega (default 0.8), and
- * hscale (default 0.7), the factor on the coarse same-level horizontal
- * couplings that keeps aggregated second differences at their coarse-grid
- * size. No counterpart in the reference. */
-int windhip_set_multigrid(windhip_solver *s, int pre, int post, double omega, double hscale);
+/* MIXED_MG V-cycle: `pre` and `post` damped line-smoothing steps on the fine
+ * level and `coarse_pre`, `coarse_post` on the coarse levels (1..8, defaults
+ * 2, 2, 1, 1) around each coarse correction; `depth` coarse levels used (0, the
+ * default: all of them, down to one interior column, solved exactly; a
+ * shallower cycle smooths its last level coarse_pre + coarse_post times);
+ * damping omega (default 0.8); hscale (default 0.7), the factor on the coarse
+ * same-level horizontal couplings that keeps aggregated second differences at
+ * their coarse-grid size. No counterpart in the reference. */
+int windhip_set_multigrid(windhip_solver *s, int pre, int post, int coarse_pre, int coarse_post, int depth,
+                          double omega, double hscale);
 /* Coarse level `level` (1 = first coarsening) of the MIXED_MG hierarchy built
  * by the last MIXED_MG solve: dims_out = (nx, ny, mz) of the level (interior
  * columns, rows per column); coef_host (may be null) receives its 15 x N
