@@ -47,7 +47,7 @@ module wind_iterative_hip
         integer :: wind_solver_iterations = 1500     ! namelist_utilities.F90:3871-3876
         real(c_double) :: tol_rel = 1.0e-5_c_double   ! wind_iterative.F90:60
         real(c_double) :: tol_abs = 1.0e-10_c_double  ! wind_iterative.F90:59
-        integer :: precision = WINDHIP_PLAIN_F64      ! or WINDHIP_MIXED, WINDHIP_MIXED_LINE
+        integer :: precision = WINDHIP_PLAIN_F64      ! or WINDHIP_MIXED, WINDHIP_MIXED_LINE, WINDHIP_MIXED_MG
         integer :: start = WINDHIP_START_WARM         ! WINDHIP_START_LEARNED once load_learned_guess ran
         logical :: verbose = .false.
     end type wind_solver_options_t

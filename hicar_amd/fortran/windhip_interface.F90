@@ -18,7 +18,8 @@ module windhip_interface
     integer(c_int), parameter, public :: WINDHIP_HOST = 0, WINDHIP_DEVICE = 1
     integer(c_int), parameter, public :: WINDHIP_START_WARM = 0, WINDHIP_START_ZERO = 1, &
                                          WINDHIP_START_GIVEN = 2, WINDHIP_START_LEARNED = 3
-    integer(c_int), parameter, public :: WINDHIP_PLAIN_F64 = 0, WINDHIP_MIXED = 1, WINDHIP_MIXED_LINE = 2
+    integer(c_int), parameter, public :: WINDHIP_PLAIN_F64 = 0, WINDHIP_MIXED = 1, WINDHIP_MIXED_LINE = 2, &
+                                         WINDHIP_MIXED_MG = 3
 
     type, bind(C), public :: windhip_desc
         integer(c_int) :: nx, ny, nz
@@ -45,7 +46,7 @@ module windhip_interface
               windhip_set_metrics, windhip_solve, windhip_apply_correction, windhip_calc_iter_winds, &
               windhip_divergence, windhip_get_lambda, windhip_set_lambda, windhip_probe_operator, &
               windhip_net_param_count, windhip_net_create, windhip_net_destroy, windhip_net_set_params, &
-              windhip_set_net, windhip_error_message
+              windhip_set_net, windhip_set_multigrid, windhip_error_message
 
     interface
         subroutine windhip_default_opts(opts) bind(C, name="windhip_default_opts")
@@ -148,6 +149,14 @@ module windhip_interface
         integer(c_int) function windhip_set_net(handle, net) bind(C, name="windhip_set_net")
             import :: c_int, c_ptr
             type(c_ptr), value :: handle, net
+        end function
+
+        integer(c_int) function windhip_set_multigrid(handle, pre, post, coarse_pre, coarse_post, depth, &
+                                                      omega, hscale) bind(C, name="windhip_set_multigrid")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: handle
+            integer(c_int), value :: pre, post, coarse_pre, coarse_post, depth
+            real(c_double), value :: omega, hscale
         end function
     end interface
 
