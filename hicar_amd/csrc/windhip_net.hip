@@ -56,78 +56,135 @@ constexpr int kT = 256;  // threads of the conv / elementwise kernels
 __device__ __forceinline__ int kpad(int k) { return (k + 31) / 32 * 32; }
 
 // ------------------------------------------------------------------ kernels
-// Implicit-GEMM 3x3x3 convolution, zero padding, stride 1. A wave computes 64
-// voxels x 64 output channels (4 x 4 MFMA tiles); blockIdx.y selects the
-// 64-channel chunk of N.
-__global__ __launch_bounds__(kT) void k_conv3(Shape s, const bf16 *__restrict__ X, int ldx, int C,
-                                             const bf16 *__restrict__ Wt, int Kp, const float *__restrict__ bias,
-                                             int N, bf16 *__restrict__ Y, int ldy, int relu)
+// Implicit-GEMM 3x3x3 convolution (zero padding, stride 1) on
+// v_mfma_f32_16x16x32_bf16, staged through LDS. A block computes an output tile of
+// 16 (x) x 4 (y) x 4 (z) voxels x up to 64 output channels (blockIdx.y picks
+// the 64-channel chunk of N): wave w takes z-slice w, MFMA row group r the
+// 16-voxel x-run of row y = r. The input channels go in chunks of CC: the
+// zero-padded 18 x 6 x 6 halo tile of a chunk is staged in LDS as
+// [channel group of 8][voxel][8], so the 16 lanes of a row group read 256
+// contiguous bytes (no bank conflicts) and every tap's A fragment is one
+// 16-byte LDS read per lane, with no bounds tests in the MFMA loop. A k-step
+// of 32 covers 32 / CC taps of the chunk; the weights' k index is t C + c.
+// NB (1, 2 or 4) is the number of 16-channel blocks of the chunk: a layer of
+// 16 outputs keeps 16 accumulators, not 64.
+constexpr int kTx = 16, kTy = 4, kTz = 4;
+constexpr int kHx = kTx + 2, kHy = kTy + 2, kHz = kTz + 2, kHv = kHx * kHy * kHz;
+template <int CC, int NB>
+__global__ __launch_bounds__(kT) void k_conv3_lds(Shape s, const bf16 *__restrict__ X, int ldx, int C,
+                                                 const bf16 *__restrict__ Wt, int Kp, const float *__restrict__ bias,
+                                                 int N, bf16 *__restrict__ Y, int ldy, int relu)
 {
+    __shared__ bf16x8 tile[(CC / 8) * kHv];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r16 = lane & 15, g = lane >> 4;
-    const long M = s.voxels();
-    const long m0 = ((long)blockIdx.x * 4 + wave) * 64;
-    const int n0 = blockIdx.y * 64;
-    if (m0 >= M) return;
-    const int nbl = min(4, (N - n0 + 15) / 16);
+    const int ntx = (s.W + kTx - 1) / kTx, nty = (s.H + kTy - 1) / kTy, ntz = (s.D + kTz - 1) / kTz;
+    int b = blockIdx.x;
+    const int tx = b % ntx;
+    b /= ntx;
+    const int ty = b % nty;
+    b /= nty;
+    const int tz = b % ntz;
+    const int bb = b / ntz;
+    const int x0 = tx * kTx, y0 = ty * kTy, z0 = tz * kTz;
+    const int n0 = blockIdx.y * 16 * NB;
+    const int nbl = min(NB, (N - n0 + 15) / 16);
     const long HW = (long)s.H * s.W;
-    int cz[4], cy[4], cx[4];
-    long cv[4];
-    bool ok[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const long v = m0 + 16 * r + r16;
-        ok[r] = v < M;
-        const long vv = ok[r] ? v : 0;
-        cv[r] = vv;
-        cx[r] = (int)(vv % s.W);
-        const long q = vv / s.W;
-        cy[r] = (int)(q % s.H);
-        cz[r] = (int)((q / s.H) % s.D);
-    }
-    f32x4 acc[4][4];
+    const long vb = (long)bb * s.D * HW;
+    constexpr int TPS = 32 / CC;           // taps per k-step
+    constexpr int KS = (27 + TPS - 1) / TPS;
+    const int gt = (8 * g) / CC;           // this lane group's tap within the k-step
+    const int gc = ((8 * g) % CC) / 8;     // and channel group within the chunk
+    const int abase = gc * kHv + wave * (kHx * kHy) + r16;
+    f32x4 acc[4][NB];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-        for (int nb = 0; nb < 4; ++nb) acc[r][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int nb = 0; nb < NB; ++nb) acc[r][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
     const bf16x8 zero8 = {};
-    for (int k0 = 0; k0 < Kp; k0 += 32) {
-        const int k = k0 + 8 * g;
-        const int t = k / C, c = k - t * C;
-        const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-        bf16x8 a[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int z = cz[r] + dz, y = cy[r] + dy, x = cx[r] + dx;
-            const bool in = t < 27 && ok[r] && z >= 0 && z < s.D && y >= 0 && y < s.H && x >= 0 && x < s.W;
-            a[r] = in ? *reinterpret_cast<const bf16x8 *>(X + (cv[r] + dz * HW + dy * s.W + dx) * ldx + c) : zero8;
+    for (int c0 = 0; c0 < C; c0 += CC) {
+        __syncthreads();  // the previous chunk's reads are done
+        for (int q = threadIdx.x; q < (CC / 8) * kHv; q += kT) {
+            const int cg = q / kHv, v = q - cg * kHv;
+            const int hx = v % kHx, hy = (v / kHx) % kHy, hz = v / (kHx * kHy);
+            const int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
+            bf16x8 val = zero8;
+            if (x >= 0 && x < s.W && y >= 0 && y < s.H && z >= 0 && z < s.D)
+                val = *reinterpret_cast<const bf16x8 *>(X + (vb + z * HW + (long)y * s.W + x) * ldx + c0 + cg * 8);
+            tile[q] = val;
         }
+        __syncthreads();
+#pragma unroll 2
+        for (int ks = 0; ks < KS; ++ks) {
+            const int t = ks * TPS + gt;
+            const bool tv = t < 27;
+            const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
+            const int off = abase + dz * (kHx * kHy) + dy * kHx + dx;
+            bf16x8 a[4];
 #pragma unroll
-        for (int nb = 0; nb < 4; ++nb) {
-            if (nb < nbl) {
-                const bf16x8 b = *reinterpret_cast<const bf16x8 *>(Wt + (long)(n0 + 16 * nb + r16) * Kp + k);
+            for (int r = 0; r < 4; ++r) a[r] = tv ? tile[off + r * kHx] : zero8;
+            const long kk = (long)t * C + c0 + gc * 8;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], b, acc[r][nb], 0, 0, 0);
-            }
-        }
-    }
+            for (int nb = 0; nb < NB; ++nb) {
+                if (nb < nbl) {
+                    const int row = n0 + 16 * nb + r16;  // weight rows beyond N read as zero
+                    const bf16x8 bw = tv && row < N ? *reinterpret_cast<const bf16x8 *>(Wt + (long)row * Kp + kk)
+                                                    : zero8;
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int nb = 0; nb < 4; ++nb) {
-            if (nb >= nbl) continue;
-            const int n = n0 + 16 * nb + r16;
-            const float bv = bias ? bias[n] : 0.f;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const long v = m0 + 16 * r + 4 * g + e;
-                if (v < M && n < N) {
-                    float y = acc[r][nb][e] + bv;
-                    if (relu) y = fmaxf(y, 0.f);
-                    Y[v * ldy + n] = (bf16)y;
+                    for (int r = 0; r < 4; ++r)
+                        acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], bw, acc[r][nb], 0, 0, 0);
                 }
             }
         }
+    }
+    const int z = z0 + wave;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int y = y0 + r;
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            if (nb >= nbl) continue;
+            const int n = n0 + 16 * nb + r16;
+            if (n >= N || y >= s.H || z >= s.D) continue;
+            const float bv = bias ? bias[n] : 0.f;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int x = x0 + 4 * g + e;
+                if (x < s.W) {
+                    float o = acc[r][nb][e] + bv;
+                    if (relu) o = fmaxf(o, 0.f);
+                    Y[(vb + z * HW + (long)y * s.W + x) * ldy + n] = (bf16)o;
+                }
+            }
+        }
+    }
+}
+
+// Launch the LDS-staged convolution with the widest channel chunk that divides
+// C and as many 16-channel output blocks per block as N needs (up to 4).
+template <int CC>
+static void conv3_cc(hipStream_t st, Shape s, const bf16 *X, int ldx, int C, const bf16 *Wt, int Kp,
+                     const float *bias, int N, bf16 *Y, int ldy, int relu)
+{
+    const int ntx = (s.W + kTx - 1) / kTx, nty = (s.H + kTy - 1) / kTy, ntz = (s.D + kTz - 1) / kTz;
+    const unsigned tiles = (unsigned)((long)ntx * nty * ntz * s.B);
+    if (N <= 16)
+        k_conv3_lds<CC, 1><<<dim3(tiles, 1), kT, 0, st>>>(s, X, ldx, C, Wt, Kp, bias, N, Y, ldy, relu);
+    else if (N <= 32)
+        k_conv3_lds<CC, 2><<<dim3(tiles, 1), kT, 0, st>>>(s, X, ldx, C, Wt, Kp, bias, N, Y, ldy, relu);
+    else
+        k_conv3_lds<CC, 4><<<dim3(tiles, (unsigned)((N + 63) / 64)), kT, 0, st>>>(s, X, ldx, C, Wt, Kp, bias, N, Y,
+                                                                                   ldy, relu);
+}
+static void conv3(hipStream_t st, Shape s, const bf16 *X, int ldx, int C, const bf16 *Wt, int Kp, const float *bias,
+                  int N, bf16 *Y, int ldy, int relu)
+{
+    if (C % 32 == 0)
+        conv3_cc<32>(st, s, X, ldx, C, Wt, Kp, bias, N, Y, ldy, relu);
+    else if (C % 16 == 0)
+        conv3_cc<16>(st, s, X, ldx, C, Wt, Kp, bias, N, Y, ldy, relu);
+    else
+        conv3_cc<8>(st, s, X, ldx, C, Wt, Kp, bias, N, Y, ldy, relu);
 }
 
 // Weight-gradient partials of one tap over one slab of voxels:
@@ -561,9 +618,7 @@ int windhip_net::prepare_weights(hipStream_t st)
 void windhip_net::conv(hipStream_t st, int l, Shape s, const bf16 *X, int ldx, bf16 *Y, int ldy)
 {
     const auto &c = kConvs[l];
-    const long M = s.voxels();
-    dim3 grid((unsigned)((M + 255) / 256), (unsigned)((c.co + 63) / 64));
-    k_conv3<<<grid, kT, 0, st>>>(s, X, ldx, c.ci, wf[l], kpf[l], params + boff[l], c.co, Y, ldy, 1);
+    conv3(st, s, X, ldx, c.ci, wf[l], kpf[l], params + boff[l], c.co, Y, ldy, 1);
 }
 
 // Backward of conv l (output Yout after ReLU, gradient G w.r.t. Yout): weight
@@ -580,10 +635,7 @@ void windhip_net::conv_back(hipStream_t st, int l, Shape s, const bf16 *G, int l
     k_wgrad_reduce<<<grid1(27L * c.co * c.ci), kT, 0, st>>>(nslab, c.co, c.ci, part, grads + woff[l]);
     k_colsum<<<nslab, kT, 0, st>>>(M, gtmp, c.co, c.co, slab, part);
     k_slab_reduce<<<1, 64, 0, st>>>(nslab, c.co, part, grads + boff[l]);
-    if (dX) {
-        dim3 grid((unsigned)((M + 255) / 256), (unsigned)((c.ci + 63) / 64));
-        k_conv3<<<grid, kT, 0, st>>>(s, gtmp, c.co, c.co, wb[l], kpb[l], nullptr, c.ci, dX, lddx, 0);
-    }
+    if (dX) conv3(st, s, gtmp, c.co, c.co, wb[l], kpb[l], nullptr, c.ci, dX, lddx, 0);
 }
 
 int windhip_net::forward(hipStream_t st, const float *in, Shape s, float *o)
