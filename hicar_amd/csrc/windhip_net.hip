@@ -328,12 +328,6 @@ __global__ void k_wt_prepare(const float *__restrict__ Wm, int ci, int co, bf16 
     }
 }
 
-__global__ void k_cast_in(long M, const float *__restrict__ in, bf16 *__restrict__ x)
-{
-    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n < M * kInCh) x[n] = (bf16)in[n];
-}
-
 // G masked by the ReLU of the layer output Y: out = Y > 0 ? G : 0.
 __global__ void k_relu_mask(long M, int C, const bf16 *__restrict__ G, int ldg, const bf16 *__restrict__ Y, int ldy,
                             bf16 *__restrict__ out, int ldo)
@@ -343,27 +337,6 @@ __global__ void k_relu_mask(long M, int C, const bf16 *__restrict__ G, int ldg, 
     const long v = n / C;
     const int c = (int)(n - v * C);
     out[v * ldo + c] = (float)Y[v * ldy + c] > 0.f ? G[v * ldg + c] : (bf16)0.f;
-}
-
-// 2x2x2 average pooling: s is the fine shape.
-__global__ void k_pool(Shape s, int C, const bf16 *__restrict__ X, int ldx, bf16 *__restrict__ Y, int ldy)
-{
-    const Shape h = s.half();
-    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= h.voxels() * C) return;
-    const long v = n / C;
-    const int c = (int)(n - v * C);
-    const int x = (int)(v % h.W);
-    long q = v / h.W;
-    const int y = (int)(q % h.H);
-    q /= h.H;
-    const int z = (int)(q % h.D), b = (int)(q / h.D);
-    float acc = 0.f;
-    for (int e = 0; e < 8; ++e) {
-        const long f = (((long)b * s.D + 2 * z + (e >> 2)) * s.H + 2 * y + ((e >> 1) & 1)) * s.W + 2 * x + (e & 1);
-        acc += (float)X[f * ldx + c];
-    }
-    Y[v * ldy + c] = (bf16)(acc * 0.125f);
 }
 
 // Input gradient of the pooling plus an optional skip gradient:
@@ -387,21 +360,65 @@ __global__ void k_pool_bwd(Shape s, int C, const bf16 *__restrict__ G, int ldg, 
     out[v * ldo + c] = (bf16)val;
 }
 
-// Nearest 2x upsampling: Y (fine shape s) <- X (coarse).
-__global__ void k_up(Shape s, int C, const bf16 *__restrict__ X, int ldx, bf16 *__restrict__ Y, int ldy)
+// Forward pooling, upsampling and input cast, 8 channels (16 bytes) per thread
+// with 32-bit index arithmetic (the host keeps voxels x C / 8 below 2^31).
+__device__ __forceinline__ void vox_coords(unsigned v, const Shape &s, int &x, int &y, int &z, int &b)
 {
-    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= s.voxels() * C) return;
-    const long v = n / C;
-    const int c = (int)(n - v * C);
-    const int x = (int)(v % s.W);
-    long q = v / s.W;
-    const int y = (int)(q % s.H);
-    q /= s.H;
-    const int z = (int)(q % s.D), b = (int)(q / s.D);
+    x = (int)(v % (unsigned)s.W);
+    unsigned q = v / (unsigned)s.W;
+    y = (int)(q % (unsigned)s.H);
+    q /= (unsigned)s.H;
+    z = (int)(q % (unsigned)s.D);
+    b = (int)(q / (unsigned)s.D);
+}
+
+// 2x2x2 average pooling (s the fine shape).
+__global__ void k_pool8(Shape s, int C, const bf16 *__restrict__ X, int ldx, bf16 *__restrict__ Y, int ldy)
+{
+    const Shape h = s.half();
+    const unsigned g8 = (unsigned)(C / 8);
+    const unsigned n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (unsigned)h.voxels() * g8) return;
+    const unsigned v = n / g8, c = (n - v * g8) * 8;
+    int x, y, z, b;
+    vox_coords(v, h, x, y, z, b);
+    float acc[8] = {};
+    for (int e = 0; e < 8; ++e) {
+        const long f = (((long)b * s.D + 2 * z + (e >> 2)) * s.H + 2 * y + ((e >> 1) & 1)) * s.W + 2 * x + (e & 1);
+        const bf16x8 xv = *reinterpret_cast<const bf16x8 *>(X + f * ldx + c);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[q] += (float)xv[q];
+    }
+    bf16x8 o;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) o[q] = (bf16)(acc[q] * 0.125f);
+    *reinterpret_cast<bf16x8 *>(Y + (long)v * ldy + c) = o;
+}
+
+// Nearest 2x upsampling: Y (fine shape s) <- X (coarse).
+__global__ void k_up8(Shape s, int C, const bf16 *__restrict__ X, int ldx, bf16 *__restrict__ Y, int ldy)
+{
+    const unsigned g8 = (unsigned)(C / 8);
+    const unsigned n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (unsigned)s.voxels() * g8) return;
+    const unsigned v = n / g8, c = (n - v * g8) * 8;
+    int x, y, z, b;
+    vox_coords(v, s, x, y, z, b);
     const Shape h = s.half();
     const long p = (((long)b * h.D + z / 2) * h.H + y / 2) * h.W + x / 2;
-    Y[v * ldy + c] = X[p * ldx + c];
+    *reinterpret_cast<bf16x8 *>(Y + (long)v * ldy + c) = *reinterpret_cast<const bf16x8 *>(X + p * ldx + c);
+}
+
+// the 8 input channels of a voxel, FP32 -> bf16
+__global__ void k_cast_in8(long M, const float *__restrict__ in, bf16 *__restrict__ x)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= M) return;
+    const float4 a = reinterpret_cast<const float4 *>(in)[2 * v], b = reinterpret_cast<const float4 *>(in)[2 * v + 1];
+    bf16x8 o;
+    o[0] = (bf16)a.x; o[1] = (bf16)a.y; o[2] = (bf16)a.z; o[3] = (bf16)a.w;
+    o[4] = (bf16)b.x; o[5] = (bf16)b.y; o[6] = (bf16)b.z; o[7] = (bf16)b.w;
+    reinterpret_cast<bf16x8 *>(x)[v] = o;
 }
 
 // Input gradient of the upsampling: out (coarse) = sum of the 8 children of G.
@@ -580,6 +597,8 @@ int windhip_net::ensure(Shape s)
     if (s.B == shape.B && s.D == shape.D && s.H == shape.H && s.W == shape.W) return WINDHIP_OK;
     if (s.D % 8 || s.H % 8 || s.W % 8 || s.B < 1 || s.D < 8)
         return fail("windhip_net: volume %dx%dx%d must be multiples of 8", s.D, s.H, s.W);
+    if (s.voxels() > (1L << 28))  // the vectorised elementwise kernels index voxels x C / 8 in 32 bits
+        return fail("windhip_net: %ld voxels exceed the limit of 2^28", s.voxels());
     free_shape();
     const long m0 = s.voxels(), m1 = m0 / 8, m2 = m0 / 64, m3 = m0 / 512;
     struct { bf16 **p; long n; } tab[] = {
@@ -644,25 +663,26 @@ int windhip_net::forward(hipStream_t st, const float *in, Shape s, float *o)
     NCHK(prepare_weights(st));
     const Shape s1 = s.half(), s2 = s1.half(), s3 = s2.half();
     const long m0 = s.voxels();
-    k_cast_in<<<grid1(m0 * kInCh), kT, 0, st>>>(m0, in, x0);
+    static_assert(kInCh == 8, "k_cast_in8 converts 8 channels per voxel");
+    k_cast_in8<<<grid1(m0), kT, 0, st>>>(m0, in, x0);
     conv(st, C0A, s, x0, 8, a0, 16);
     conv(st, C0B, s, a0, 16, cat0 + 32, 48);
-    k_pool<<<grid1(s1.voxels() * 16), kT, 0, st>>>(s, 16, cat0 + 32, 48, p1, 16);
+    k_pool8<<<grid1(s1.voxels() * 2), kT, 0, st>>>(s, 16, cat0 + 32, 48, p1, 16);
     conv(st, C1A, s1, p1, 16, a1, 32);
     conv(st, C1B, s1, a1, 32, cat1 + 64, 96);
-    k_pool<<<grid1(s2.voxels() * 32), kT, 0, st>>>(s1, 32, cat1 + 64, 96, p2, 32);
+    k_pool8<<<grid1(s2.voxels() * 4), kT, 0, st>>>(s1, 32, cat1 + 64, 96, p2, 32);
     conv(st, C2A, s2, p2, 32, a2, 64);
     conv(st, C2B, s2, a2, 64, cat2 + 64, 128);
-    k_pool<<<grid1(s3.voxels() * 64), kT, 0, st>>>(s2, 64, cat2 + 64, 128, p3, 64);
+    k_pool8<<<grid1(s3.voxels() * 8), kT, 0, st>>>(s2, 64, cat2 + 64, 128, p3, 64);
     conv(st, C3A, s3, p3, 64, a3, 64);
     conv(st, C3B, s3, a3, 64, b3, 64);
-    k_up<<<grid1(s2.voxels() * 64), kT, 0, st>>>(s2, 64, b3, 64, cat2, 128);
+    k_up8<<<grid1(s2.voxels() * 8), kT, 0, st>>>(s2, 64, b3, 64, cat2, 128);
     conv(st, D2A, s2, cat2, 128, e2a, 64);
     conv(st, D2B, s2, e2a, 64, e2b, 64);
-    k_up<<<grid1(s1.voxels() * 64), kT, 0, st>>>(s1, 64, e2b, 64, cat1, 96);
+    k_up8<<<grid1(s1.voxels() * 8), kT, 0, st>>>(s1, 64, e2b, 64, cat1, 96);
     conv(st, D1A, s1, cat1, 96, e1a, 32);
     conv(st, D1B, s1, e1a, 32, e1b, 32);
-    k_up<<<grid1(m0 * 32), kT, 0, st>>>(s, 32, e1b, 32, cat0, 48);
+    k_up8<<<grid1(m0 * 4), kT, 0, st>>>(s, 32, e1b, 32, cat0, 48);
     conv(st, D0A, s, cat0, 48, e0a, 16);
     conv(st, D0B, s, e0a, 16, e0b, 16);
     k_head<<<grid1(m0), kT, 0, st>>>(m0, e0b, params + hoff, o ? o : out);
