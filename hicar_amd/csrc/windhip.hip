@@ -129,7 +129,7 @@ struct windhip_solver {
     int mg_pre = 2, mg_post = 2;    // fine-level line-smoothing steps before and after the coarse correction
     int mg_cpre = 1, mg_cpost = 1;  // the same on the coarse levels
     int mg_depth = 0;               // coarse levels used (0: all, down to one interior column)
-    float mg_omega = 0.8f;          // damping of the line smoother
+    float mg_omega = 0.85f;         // damping of the line smoother
     double mg_hs = 0.7;           // scale of the coarse same-level horizontal couplings
     bool metrics_set = false, structure_uploaded = false, probed = false;
     bool verbose = false;  // WINDHIP_VERBOSE=1: per-batch progress on stderr (verbose_solver, :274-275)

@@ -172,7 +172,7 @@ class WindSolver:
         nat.check(self.lib.windhip_set_lambda(self.h, lam.ctypes.data_as(C.POINTER(C.c_double))))
 
     def set_multigrid(self, pre: int = 2, post: int = 2, coarse_pre: int = 1, coarse_post: int = 1, depth: int = 0,
-                      omega: float = 0.8, hscale: float = 0.7) -> None:
+                      omega: float = 0.85, hscale: float = 0.7) -> None:
         """V-cycle of the MIXED_MG inner preconditioner (no counterpart in the reference)."""
         nat.check(self.lib.windhip_set_multigrid(self.h, int(pre), int(post), int(coarse_pre), int(coarse_post),
                                                  int(depth), float(omega), float(hscale)))

@@ -213,7 +213,7 @@ int windhip_set_guess_smoothing(windhip_solver *s, int steps);
  * 2, 2, 1, 1) around each coarse correction; `depth` coarse levels used (0, the
  * default: all of them, down to one interior column, solved exactly; a
  * shallower cycle smooths its last level coarse_pre + coarse_post times);
- * damping omega (default 0.8); hscale (default 0.7), the factor on the coarse
+ * damping omega (default 0.85); hscale (default 0.7), the factor on the coarse
  * same-level horizontal couplings that keeps aggregated second differences at
  * their coarse-grid size. No counterpart in the reference. */
 int windhip_set_multigrid(windhip_solver *s, int pre, int post, int coarse_pre, int coarse_post, int depth,
