@@ -8,7 +8,11 @@ resident in HBM before the timed region. One step = one complete solve
 (operator refresh, diagonal, rhs, start selection, Krylov iterations, true
 residual) from a zero lambda.
 
-Arms (each solves the same system to the same tolerance):
+Arms (each solves the same system; the plain FP64 arm to rel. residual 1e-8,
+the mixed-precision arms to --mixed-tol, 2e-9 by default: with the multigrid
+the error in lambda is about 100x the final residual, so stopping just below
+1e-8 could leave lambda further than 1e-6 from the FP64 solution; at 2e-9 it
+is as accurate as the plain solve at 1e-8):
   plain          the reference's BiCGStab in FP64 (2 Jacobi-Richardson sweeps)
   mixed          FP64 FGMRES around that BiCGStab in FP32
   line           FP64 FGMRES around an FP32 BiCGStab preconditioned by two
@@ -134,6 +138,9 @@ def main():
     ap.add_argument("--arms", default="plain,mixed,line,mg,mg_learned")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=6)
+    ap.add_argument("--mixed-tol", type=float, default=2e-9,
+                    help="rel. residual the mixed-precision arms stop at (<= 1e-8; tight enough that lambda "
+                         "stays within 1e-6 of the FP64 solution, about 100x the residual for the multigrid arms)")
     ap.add_argument("--tight", type=float, default=1e-11,
                     help="rel. residual of the FP64 reference solve the errors are stated against")
     ap.add_argument("--no-tight", action="store_true", help="skip the tight FP64 reference solve")
@@ -189,9 +196,11 @@ def main():
     want = [a for a in args.arms.split(",") if a in arms and (net is not None or "learned" not in a)]
     zeros = np.zeros(s.rows_shape)
 
-    def solve(arm, profile=0, tol=1e-8):
+    def solve(arm, profile=0, tol=None):
         o = dict(arms[arm])
         inner = o.pop("inner_tol", args.inner_tol)
+        if tol is None:
+            tol = 1e-8 if o["precision"] == nat.PLAIN_F64 else args.mixed_tol
         learned.attach(s, net if o["start"] == nat.START_LEARNED else None)
         s.set_lambda(zeros)
         torch.cuda.synchronize()
@@ -238,6 +247,7 @@ def main():
         solve(arm, profile=1)
         prof = s.profile(reset=True)
         r = dict(ms=float(np.mean(times)), iters=info["iters"], outer=info["outer_iters"], status=info["status"],
+                 tol_rel=1e-8 if arms[arm]["precision"] == nat.PLAIN_F64 else args.mixed_tol,
                  true_rel_res=info["true_rel_res"], start_used=info["start_used"], prof=prof)
         if arm == "plain":
             ref = (lam, wnd)
@@ -315,7 +325,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"wind solve {case.nx}x{case.ny}x{case.nz} fractal terrain, rel. residual 1e-8, "
                                f"arm '{head}'", "grid": [case.nx, case.ny, case.nz], "dx_m": case.dx,
-                   "tol_rel": 1e-8, "inner_tol": args.inner_tol, "cells": case.cells,
+                   "tol_rel": 1e-8, "tol_rel_mixed_arms": args.mixed_tol, "inner_tol": args.inner_tol,
+                   "cells": case.cells,
                    "learned_weights": os.path.relpath(WEIGHTS, ROOT) if net is not None else None},
         "roofline": roof,
         "plain_matvec_roofline": plain_mv,
