@@ -8,11 +8,8 @@ resident in HBM before the timed region. One step = one complete solve
 (operator refresh, diagonal, rhs, start selection, Krylov iterations, true
 residual) from a zero lambda.
 
-Arms (each solves the same system; the plain FP64 arm to rel. residual 1e-8,
-the mixed-precision arms to --mixed-tol, 2e-9 by default: with the multigrid
-the error in lambda is about 100x the final residual, so stopping just below
-1e-8 could leave lambda further than 1e-6 from the FP64 solution; at 2e-9 it
-is as accurate as the plain solve at 1e-8):
+Arms (each solves the same system to rel. residual 1e-8; --mixed-tol sets
+another stop for the mixed-precision arms):
   plain          the reference's BiCGStab in FP64 (2 Jacobi-Richardson sweeps)
   mixed          FP64 FGMRES around that BiCGStab in FP32
   line           FP64 FGMRES around an FP32 BiCGStab preconditioned by two
@@ -33,6 +30,16 @@ oracle (the reference algorithm restated in C, OpenMP) timed on this host on
 a bounded sample and scaled to a full solve.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+       python bench.py --sequence S   (the 1024x1024x60 solve sequence instead)
+
+--sequence S: S consecutive solves on a 1024x1024x60 grid (dx = 50 m, 3000 m
+fractal relief, 60 levels 10 m -> 12 km) whose forcing wind turns 1 deg and is
+rescaled by a seeded U[0.95, 1.05] factor each step (the mean power-law flow is
+rotated on the device; its N(0, 1 m/s) noise stays fixed), each step solved
+by the learned + FP32/FP64 multigrid solve to the mixed tolerance from three
+starts: the network guess (as the first FGMRES direction on top of the
+previous lambda), the previous-step lambda, and zero. Prints one JSON line with
+ms and iterations per start and the peak device memory in use.
 """
 import argparse
 import json
@@ -138,12 +145,13 @@ def main():
     ap.add_argument("--arms", default="plain,mixed,line,mg,mg_learned")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=6)
-    ap.add_argument("--mixed-tol", type=float, default=2e-9,
-                    help="rel. residual the mixed-precision arms stop at (<= 1e-8; tight enough that lambda "
-                         "stays within 1e-6 of the FP64 solution, about 100x the residual for the multigrid arms)")
+    ap.add_argument("--mixed-tol", type=float, default=1e-8,
+                    help="rel. residual the mixed-precision arms stop at (default the metric's 1e-8)")
     ap.add_argument("--tight", type=float, default=1e-11,
                     help="rel. residual of the FP64 reference solve the errors are stated against")
     ap.add_argument("--no-tight", action="store_true", help="skip the tight FP64 reference solve")
+    ap.add_argument("--sequence", type=int, default=0,
+                    help="run S consecutive solves of the 1024x1024x60 sequence instead of the benchmark")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy a 1 GiB device buffer three times (known traffic for counter calibration)")
     args = ap.parse_args()
@@ -159,6 +167,9 @@ def main():
 
     log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
     dev = torch.device("cuda", 0)
+    if args.sequence > 0:
+        run_sequence(args, log, dev)
+        return
     t_gen = time.perf_counter()
     case = build_case(args.nx, args.ny, args.nz)
     log(f"case {case.nx}x{case.ny}x{case.nz} generated in {time.perf_counter() - t_gen:.1f}s")
@@ -338,6 +349,77 @@ def main():
         "network": netinfo,
         "kernels": tables,
     }
+    print(json.dumps(line))
+
+
+def run_sequence(args, log, dev):
+    """The solve sequence of the 1024x1024x60 case (see the module docstring)."""
+    import torch
+    from hicar_amd import _native as nat
+    from hicar_amd import learned
+    from hicar_amd import synthetic as S
+    from hicar_amd.wind_iterative import WindSolver
+
+    t_gen = time.perf_counter()
+    case = S.fractal_case(nx=1024, ny=1024, nz=60, dx=50.0, relief=3000.0, speed=8.0, seed=4)
+    log(f"case 1024x1024x60 generated in {time.perf_counter() - t_gen:.1f}s")
+    m = case.metrics
+    nz = case.nz
+
+    def D(a):
+        return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+
+    s = WindSolver(case.nx, case.ny, case.nz, case.dx, device_fields=True)
+    s.set_metrics({k: D(v) for k, v in m.items()
+                   if k in ("dzdx", "dzdy", "jacobian", "jacobian_u", "jacobian_v", "jacobian_w", "dzdx_u",
+                            "dzdy_v", "advection_dz", "terrain")})
+    # the mean power-law flow (25 deg shear over the column) and the fixed noise
+    theta0 = torch.tensor(np.deg2rad(25.0 * np.arange(nz) / (nz - 1)), device=dev)[None, :, None]
+    su = D(8.0 * (np.maximum(m["z_agl_u"], 0.1) / 10.0) ** (1.0 / 7.0))
+    sv = D(8.0 * (np.maximum(m["z_agl_v"], 0.1) / 10.0) ** (1.0 / 7.0))
+    nu = D(case.u) - su * torch.cos(theta0)
+    nv = D(case.v) - sv * torch.sin(theta0)
+    w, rho, alpha = D(case.w), D(case.rho), D(case.alpha)
+    div = torch.empty(s.mass_shape, dtype=torch.float64, device=dev)
+    scales = np.random.default_rng(7).uniform(0.95, 1.05, size=args.sequence)
+    net = learned.Net.load(WEIGHTS)
+    starts = {"learned": nat.START_LEARNED, "warm": nat.START_WARM, "zero": nat.START_ZERO}
+    peak = 0.0
+    out = {}
+    for name, start in starts.items():
+        learned.attach(s, net if start == nat.START_LEARNED else None)
+        s.set_lambda(np.zeros(s.rows_shape))
+        ms, its, worst = [], [], 0.0
+        scale = 1.0
+        for step in range(args.sequence):
+            scale *= scales[step]
+            th = theta0 + np.deg2rad(float(step))
+            u = scale * su * torch.cos(th) + nu
+            v = scale * sv * torch.sin(th) + nv
+            s.divergence(u, v, w, density=rho, adv_den=True, use_dqdt=True, out=div)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            info = s.solve(alpha, div, precision=nat.MIXED_MG, start=start, tol_rel=args.mixed_tol,
+                           max_iters=3000, inner_tol=0.1)
+            torch.cuda.synchronize()
+            ms.append(1e3 * (time.perf_counter() - t0))
+            its.append(info["iters"])
+            log(f"{name} step {step}: {ms[-1]:.1f} ms, {info['iters']} iterations, {info['outer_iters']} FGMRES steps, "
+                f"true rel. residual {info['true_rel_res']:.2e}")
+            worst = max(worst, info["true_rel_res"])
+            if info["status"] != 0:
+                raise RuntimeError(f"sequence step {step} ({name} start) did not converge: {info}")
+            peak = max(peak, s.memory()[1])
+        out[name] = dict(ms_total=round(sum(ms), 1), ms_mean=round(float(np.mean(ms)), 2),
+                         iters_total=int(sum(its)), iters_first=its[0], iters_mean=round(float(np.mean(its)), 2),
+                         worst_true_rel_res=worst)
+        log(f"{name}: {out[name]}")
+    line = {"metric": "ms per wind solve in a 1024x1024x60 sequence", "steps": args.sequence,
+            "config": {"workload": f"{args.sequence} wind solves 1024x1024x60 fractal terrain (3000 m), forcing "
+                                   "turned 1 deg and rescaled U[0.95, 1.05] per step", "tol_rel": args.mixed_tol,
+                       "method": "MIXED_MG (FP64 FGMRES, FP32 multigrid-preconditioned BiCGStab)"},
+            "starts": out, "peak_device_bytes_in_use": peak,
+            "torch_peak_bytes": int(torch.cuda.max_memory_allocated())}
     print(json.dumps(line))
 
 

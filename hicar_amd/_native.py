@@ -143,7 +143,7 @@ def load():
         "windhip_net_time_forward": (i, [vp, i, i, i, i, i, dp]),
         "windhip_set_net": (i, [vp, vp]),
         "windhip_set_guess_smoothing": (i, [vp, i]),
-        "windhip_set_multigrid": (i, [vp, i, i, i, i, i, d, d]),
+        "windhip_set_multigrid": (i, [vp, i, i, i, i, i, d]),
         "windhip_mg_level": (i, [vp, i, C.POINTER(C.c_int), fp]),
         "windhip_learned_input": (i, [vp, vp, C.POINTER(C.c_int), dp]),
     }

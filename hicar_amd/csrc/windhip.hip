@@ -119,18 +119,27 @@ struct windhip_solver {
     // MIXED_LINE: FP32 Thomas factors of the column tridiagonal and scratch
     float *lf_lo = nullptr, *lf_cp = nullptr, *lf_ip = nullptr, *lf_dq = nullptr, *lf_r = nullptr;
     // MIXED_MG: the coarse levels of the semi-coarsening hierarchy (the fine
-    // level is the solver's grid with the line factors lf_*): Galerkin
-    // coefficients (15 x N), line factors, correction, rhs, residual, spare
+    // level is the solver's grid with the line factors lf_*): averaged
+    // metrics, the rediscretised analytic operator (FP64 and FP32), its BC
+    // planes and line factors, and the level's correction, rhs, residual and
+    // spare vectors (x and t with stencil guards)
     struct MgLevel {
         MgGeo g;
-        float *A, *lo, *cp, *ip, *x, *b, *r, *t;
+        Geo gg;   // the level as a tile / column grid
+        Dims d;   // set-up dimensions (dx of the level)
+        double *dzdx, *dzdy, *jaco, *jaco_w, *alpha, *adv_dz, *dz_if, *dzdx_u, *jaco_u, *dzdy_v, *jaco_v;
+        double *terrain, *sx, *sy;
+        double *B, *C, *H, *I, *L, *M, *bcx, *bcy, *bcz, *dztop;
+        float *B32, *C32, *H32, *I32, *L32, *M32, *bcx32, *bcy32, *bcz32, *dztop32;
+        float *lo, *cp, *ip, *x, *b, *r, *t;
     };
     std::vector<MgLevel> mg;
     int mg_pre = 2, mg_post = 2;    // fine-level line-smoothing steps before and after the coarse correction
     int mg_cpre = 1, mg_cpost = 1;  // the same on the coarse levels
     int mg_depth = 0;               // coarse levels used (0: all, down to one interior column)
-    float mg_omega = 0.85f;         // damping of the line smoother
-    double mg_hs = 0.7;           // scale of the coarse same-level horizontal couplings
+    float mg_omega = 0.9f;          // damping of the line smoother
+    template <typename T>
+    OpA<T> level_op(const MgLevel &L) const;
     bool metrics_set = false, structure_uploaded = false, probed = false;
     bool verbose = false;  // WINDHIP_VERBOSE=1: per-batch progress on stderr (verbose_solver, :274-275)
     int nsweeps = 2;  // precond_n_sweeps, BASE_PREC_SWEEPS (wind_iterative.F90:76)
@@ -138,13 +147,13 @@ struct windhip_solver {
 
     // ---------------------------------------------------------------- helpers
     template <typename T>
-    int alloc(T **ptr, long count, bool zero = true)
+    int alloc(T **ptr, long count, bool zero = true, long guard_elems = -1)
     {
         // Arrays of N rows get `guard` zeroed elements before and after them: the
         // stencil kernels address the five columns C, W, E, S, N of every lane
         // from one shifted base (guard = mx + 1), so lateral and out-of-grid
         // lanes touch at most guard elements beyond either end.
-        const long gd = count == N ? guard : 0;
+        const long gd = guard_elems >= 0 ? guard_elems : (count == N ? guard : 0);
         size_t nb = sizeof(T) * (size_t)std::max(count + 2 * gd, 1L);
         void *q = nullptr;
         hipError_t e = hipMalloc(&q, nb);
@@ -327,6 +336,27 @@ OpA<float> windhip_solver::opa<float>() const
     o.B = B32; o.C = C32; o.H = H32; o.I = I32; o.L = L32; o.M = M32;
     o.bcx = bcx32; o.bcy = bcy32; o.bcz = bcz32; o.dztop = dztop32;
     double inv = 1.0 / (desc.dx * desc.dx);
+    o.inv = (float)inv;
+    o.neg4 = (float)(-4.0 * inv);
+    return o;
+}
+template <>
+OpA<double> windhip_solver::level_op<double>(const MgLevel &L) const
+{
+    OpA<double> o;
+    o.B = L.B; o.C = L.C; o.H = L.H; o.I = L.I; o.L = L.L; o.M = L.M;
+    o.bcx = L.bcx; o.bcy = L.bcy; o.bcz = L.bcz; o.dztop = L.dztop;
+    o.inv = 1.0 / (L.d.dx * L.d.dx);
+    o.neg4 = -4.0 * o.inv;
+    return o;
+}
+template <>
+OpA<float> windhip_solver::level_op<float>(const MgLevel &L) const
+{
+    OpA<float> o;
+    o.B = L.B32; o.C = L.C32; o.H = L.H32; o.I = L.I32; o.L = L.L32; o.M = L.M32;
+    o.bcx = L.bcx32; o.bcy = L.bcy32; o.bcz = L.bcz32; o.dztop = L.dztop32;
+    const double inv = 1.0 / (L.d.dx * L.d.dx);
     o.inv = (float)inv;
     o.neg4 = (float)(-4.0 * inv);
     return o;
@@ -689,10 +719,9 @@ float *windhip_solver::mg_coarse(size_t l)
     MgLevel &L = mg[l];
     const KState *sp = st + 1;
     const int nr = mg_blocks(L.g.N);
-    // the level as a column grid for the register-resident line solve
-    Geo gl{};
-    gl.mx = L.g.mx; gl.my = L.g.my; gl.mz = L.g.mz; gl.P = L.g.P;
-    const int cb = col_blocks(gl);
+    const Geo &gl = L.gg;
+    const int cb = col_blocks(gl), kb = chunk_blocks(gl);
+    const OpA<float> op = level_op<float>(L);
     auto line = [&](const float *in, const float *y, float *out, float w) {
         timed(PK_MG_COARSE, [&] {
             if (gl.mz <= 48)
@@ -704,7 +733,7 @@ float *windhip_solver::mg_coarse(size_t l)
         });
     };
     auto resid = [&](const float *xv) {
-        timed(PK_MG_COARSE, [&] { k_mg_resid<<<nr, kMgThreads, 0, stream>>>(L.g, sp, L.A, xv, L.b, L.r); });
+        timed(PK_MG_COARSE, [&] { k_resid_c<OpA<float>><<<kb, kColThreads, 0, stream>>>(op, gl, sp, xv, L.b, L.r); });
     };
     const size_t used = mg_depth > 0 ? std::min(mg.size(), (size_t)mg_depth) : mg.size();
     if (l + 1 == mg.size()) {  // one interior column: its line solve is exact
@@ -741,48 +770,72 @@ float *windhip_solver::mg_coarse(size_t l)
     return cur;
 }
 
-// The coarse levels (allocated once) and their Galerkin operators, rebuilt
-// from the current FP64 operator.
+// The coarse levels (allocated once) and their operators, rebuilt by every
+// solve: the metrics and alpha averaged level by level, then the set-up chain
+// of the fine operator at each level's size and dx (init_module_vars' dz_if
+// and surface slopes, the analytic assembly, the BC planes) and the line
+// factors.
 int windhip_solver::build_mg()
 {
     const MgGeo f0 = mg_geo(g.mx - 2, g.my - 2, g.mz);
     if (mg.empty()) {
         MgGeo f = f0;
+        double dx = desc.dx;
         while (f.nx > 1 || f.ny > 1) {
             MgLevel L{};
             L.g = mg_geo((f.nx + 1) / 2, (f.ny + 1) / 2, g.mz);
-            CHK(alloc(&L.A, 15 * L.g.N));
-            CHK(alloc(&L.lo, L.g.N));
-            CHK(alloc(&L.cp, L.g.N));
-            CHK(alloc(&L.ip, L.g.N));
-            CHK(alloc(&L.x, L.g.N));
-            CHK(alloc(&L.b, L.g.N));
-            CHK(alloc(&L.r, L.g.N));
-            CHK(alloc(&L.t, L.g.N));
+            dx *= 2.0;
+            L.gg.mx = L.g.mx; L.gg.my = L.g.my; L.gg.mz = L.g.mz; L.gg.P = L.g.P;
+            L.gg.nti = (L.g.mx + kTileI - 1) / kTileI;
+            L.gg.ntj = (L.g.my + kTileJ - 1) / kTileJ;
+            L.gg.nt = L.gg.nti * L.gg.ntj;
+            L.d.nx = L.g.nx; L.d.ny = L.g.ny; L.d.nz = L.g.mz - 2;
+            L.d.mx = L.g.mx; L.d.my = L.g.my; L.d.mz = L.g.mz; L.d.P = L.g.P; L.d.dx = dx;
+            const long n = L.g.N, pl = L.g.P, gd = L.g.mx + 1;
+            for (double **a : {&L.dzdx, &L.dzdy, &L.jaco, &L.jaco_w, &L.alpha, &L.adv_dz, &L.dz_if, &L.dzdx_u,
+                               &L.jaco_u, &L.dzdy_v, &L.jaco_v, &L.B, &L.C, &L.H, &L.I, &L.L, &L.M})
+                CHK(alloc(a, n));
+            for (double **a : {&L.terrain, &L.sx, &L.sy, &L.bcx, &L.bcy, &L.bcz, &L.dztop}) CHK(alloc(a, pl));
+            for (float **a : {&L.B32, &L.C32, &L.H32, &L.I32, &L.L32, &L.M32, &L.lo, &L.cp, &L.ip, &L.b, &L.r})
+                CHK(alloc(a, n));
+            for (float **a : {&L.bcx32, &L.bcy32, &L.bcz32, &L.dztop32}) CHK(alloc(a, pl));
+            CHK(alloc(&L.x, n, true, gd));  // stencil inputs of the level's residual
+            CHK(alloc(&L.t, n, true, gd));
             mg.push_back(L);
             f = L.g;
         }
     }
     if (mg.empty()) return WINDHIP_OK;
-    auto factor = [&](const MgLevel &L) {
-        return timed(PK_MG_SETUP, [&] {
-            k_mg_factor<<<mg_blocks(L.g.P), kMgThreads, 0, stream>>>(L.g, L.A, L.lo, L.cp, L.ip);
-        });
-    };
-    CHK(with_op<double>([&](auto op) {
-        return timed(PK_MG_SETUP, [&] {
-            MgExtractOp<decltype(op)> ex{op, g.P, g.mz};
-            k_mg_galerkin<<<mg_blocks(mg[0].g.N), kMgThreads, 0, stream>>>(ex, f0, mg[0].g, mg[0].A, mg_hs);
-        });
-    }));
-    CHK(factor(mg[0]));
-    for (size_t l = 1; l < mg.size(); ++l) {
-        const MgLevel &F = mg[l - 1], &L = mg[l];
+    for (size_t l = 0; l < mg.size(); ++l) {
+        MgLevel &L = mg[l];
+        MgMetricsIn in{};
+        if (l == 0) {
+            in = MgMetricsIn{m_dzdx, m_dzdy, m_jaco, m_jaco_w, m_alpha, m_adv_dz, m_dzdx_u, m_jaco_u,
+                             m_dzdy_v, m_jaco_v, p_terrain};
+        } else {
+            const MgLevel &F = mg[l - 1];
+            in = MgMetricsIn{F.dzdx, F.dzdy, F.jaco, F.jaco_w, F.alpha, F.adv_dz, F.dzdx_u, F.jaco_u,
+                             F.dzdy_v, F.jaco_v, F.terrain};
+        }
+        const MgMetricsOut out{L.dzdx, L.dzdy, L.jaco, L.jaco_w, L.alpha, L.adv_dz, L.dzdx_u, L.jaco_u,
+                               L.dzdy_v, L.jaco_v, L.terrain};
+        const MgGeo fg = l == 0 ? f0 : mg[l - 1].g;
+        Metrics m{};
+        m.dzdx = L.dzdx; m.dzdy = L.dzdy; m.jaco = L.jaco; m.jaco_w = L.jaco_w; m.alpha = L.alpha;
+        m.adv_dz = L.adv_dz; m.dz_if = L.dz_if; m.dzdx_u = L.dzdx_u; m.jaco_u = L.jaco_u;
+        m.dzdy_v = L.dzdy_v; m.jaco_v = L.jaco_v; m.sx = L.sx; m.sy = L.sy;
+        const long tot = (long)L.d.nx * L.d.ny * L.d.nz, pl = (long)L.d.nx * L.d.ny;
         CHK(timed(PK_MG_SETUP, [&] {
-            MgExtractC ex{F.A, F.g.P, F.g.N};
-            k_mg_galerkin<<<mg_blocks(L.g.N), kMgThreads, 0, stream>>>(ex, F.g, L.g, L.A, mg_hs);
+            k_mg_coarsen<<<mg_blocks(L.g.N), kMgThreads, 0, stream>>>(fg, L.g, in, out, 0);
+            k_module_vars<<<flat_grid(pl), kBlock, 0, stream>>>(L.d, L.adv_dz, L.dzdx, L.dzdy, L.terrain, L.dz_if,
+                                                                 L.sx, L.sy);
+            k_assemble<<<flat_grid(tot), kBlock, 0, stream>>>(L.d, m, 1, L.B, L.C, L.H, L.I, L.L, L.M, L.B32,
+                                                               L.C32, L.H32, L.I32, L.L32, L.M32);
+            k_bc_planes<<<flat_grid(pl), kBlock, 0, stream>>>(L.d, m, L.bcx, L.bcy, L.bcz, L.dztop, L.bcx32,
+                                                               L.bcy32, L.bcz32, L.dztop32);
+            k_line_factor<OpA<double>><<<L.gg.nt, kThreads, 0, stream>>>(level_op<double>(L), L.gg, L.lo, L.cp,
+                                                                         L.ip);
         }));
-        CHK(factor(L));
     }
     HIPCHK(hipGetLastError());
     return WINDHIP_OK;
@@ -1614,20 +1667,19 @@ int windhip_set_guess_smoothing(windhip_solver *s, int steps)
 }
 
 int windhip_set_multigrid(windhip_solver *s, int pre, int post, int coarse_pre, int coarse_post, int depth,
-                          double omega, double hscale)
+                          double omega)
 {
     auto ok = [](int n) { return n >= 1 && n <= 8; };
-    if (!s || !ok(pre) || !ok(post) || !ok(coarse_pre) || !ok(coarse_post) || depth < 0 ||
-        !(omega > 0 && omega <= 1.5) || !(hscale > 0 && hscale <= 1))
-        return fail(WINDHIP_ERR_ARG, "windhip_set_multigrid: smoothing steps in 1..8, depth >= 0, omega in (0, 1.5], "
-                                     "hscale in (0, 1]");
+    if (!s || !ok(pre) || !ok(post) || !ok(coarse_pre) || coarse_post < 0 || coarse_post > 8 || depth < 0 ||
+        !(omega > 0 && omega <= 1.5))
+        return fail(WINDHIP_ERR_ARG, "windhip_set_multigrid: smoothing steps in 1..8 (coarse_post 0..8), depth >= 0, "
+                                     "omega in (0, 1.5]");
     s->mg_pre = pre;
     s->mg_post = post;
     s->mg_cpre = coarse_pre;
     s->mg_cpost = coarse_post;
     s->mg_depth = depth;
     s->mg_omega = (float)omega;
-    s->mg_hs = hscale;  // the coarse operators are rebuilt by every MIXED_MG solve
     for (auto &kv : s->graphs) (void)hipGraphExecDestroy(kv.second);  // the cached V-cycles change shape
     s->graphs.clear();
     return WINDHIP_OK;
@@ -1643,7 +1695,9 @@ int windhip_mg_level(windhip_solver *s, int level, int *dims_out, float *coef_ho
     dims_out[0] = L.g.nx; dims_out[1] = L.g.ny; dims_out[2] = L.g.mz;
     if (!coef_host) return WINDHIP_OK;
     HIPCHK(hipStreamSynchronize(s->stream));
-    HIPCHK(hipMemcpy(coef_host, L.A, sizeof(float) * 15 * L.g.N, hipMemcpyDeviceToHost));
+    const float *arr[6] = {L.B32, L.C32, L.H32, L.I32, L.L32, L.M32};
+    for (int q = 0; q < 6; ++q)
+        HIPCHK(hipMemcpy(coef_host + (long)q * L.g.N, arr[q], sizeof(float) * L.g.N, hipMemcpyDeviceToHost));
     return WINDHIP_OK;
 }
 

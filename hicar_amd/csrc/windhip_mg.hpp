@@ -14,16 +14,17 @@
 //
 // Every level keeps the layout of the fine grid: rows [k][j][i], n = k P + c,
 // a ring of lateral identity columns around nx x ny interior columns. Coarse
-// column (I, J) aggregates the fine columns 2I-1, 2I and 2J-1, 2J (one of
-// each at an odd edge). Coarse operators are Galerkin products R A P with
-// piecewise-constant P and R = P^T. The stencil has no (i+-1, j+-1) entries,
-// so the product keeps the 15-point pattern of the probed operator
-// (spmv, wind_iterative.F90:888-904). Aggregation reproduces first-derivative
-// and cross terms exactly but doubles second differences along a coarsened
-// direction; the same-level horizontal couplings (D, E, F, G) of interior
-// rows are therefore scaled by hs (1/2) with the diagonal moved so that every
-// row still sums to zero. Coefficients are accumulated in FP64 and stored in
-// FP32; the column tridiagonals are factored in FP64 in the same pass.
+// column (I, J) covers the fine columns 2I-1, 2I and 2J-1, 2J (one of each at
+// an odd edge). A coarse operator is the analytic operator rediscretised at
+// 2 dx from the fine level's metrics averaged over these 2 x 2 columns (cell
+// values over the four cells, face values over the two fine faces that make
+// up the coarse face), assembled by the same set-up kernels as the fine
+// operator. (Galerkin products with piecewise-constant transfer were tried
+// first: on steep terrain the antisymmetric cross terms of the rows inside an
+// aggregate fold into its vertical couplings, and the coarse columns lost the
+// dominance the line smoother needs.) Restriction averages the rows of an
+// aggregate (a coarse row is a per-cell equation like a fine one);
+// prolongation interpolates bilinearly in x and y.
 #pragma once
 
 namespace windhip {
@@ -48,213 +49,77 @@ inline MgGeo mg_geo(int nx, int ny, int mz)
 constexpr int kMgThreads = 256;
 inline int mg_blocks(long cols) { return (int)((cols + kMgThreads - 1) / kMgThreads); }
 
-// Offsets (di, dk, dj) of the stencil entries A..O (spmv, wind_iterative.F90:888-904).
-__device__ __forceinline__ int off_i(int q) { return ((q == 3 || q == 7 || q == 9) ? 1 : 0) - ((q == 4 || q == 8 || q == 10) ? 1 : 0); }
-__device__ __forceinline__ int off_k(int q) { return ((q == 1 || q == 7 || q == 8 || q == 11 || q == 12) ? 1 : 0) - ((q == 2 || q == 9 || q == 10 || q == 13 || q == 14) ? 1 : 0); }
-__device__ __forceinline__ int off_j(int q) { return ((q == 5 || q == 11 || q == 13) ? 1 : 0) - ((q == 6 || q == 12 || q == 14) ? 1 : 0); }
-// entry index of offset (di, dk, dj); -1 outside the pattern
-__device__ __forceinline__ int stencil_index(int di, int dk, int dj)
+// Index set of fine columns under coarse column I of a level with nf
+// interior columns: the ring maps to the ring.
+__device__ __forceinline__ void mg_fine_range(int I, int nf, int nc, int &lo, int &hi)
 {
-    if (di == 0 && dj == 0) return dk == 0 ? 0 : (dk > 0 ? 1 : 2);
-    if (dj == 0) {
-        if (dk == 0) return di > 0 ? 3 : 4;
-        if (dk > 0) return di > 0 ? 7 : 8;
-        return di > 0 ? 9 : 10;
-    }
-    if (di == 0) {
-        if (dk == 0) return dj > 0 ? 5 : 6;
-        if (dk > 0) return dj > 0 ? 11 : 12;
-        return dj > 0 ? 13 : 14;
-    }
-    return -1;
+    if (I <= 0) { lo = hi = 0; return; }
+    if (I > nc) { lo = hi = nf + 1; return; }
+    lo = 2 * I - 1;
+    hi = min(2 * I, nf);
 }
+// fine face under coarse face I (the face left of coarse column I)
+__device__ __forceinline__ int mg_fine_face(int I, int nf) { return I <= 0 ? 0 : min(2 * I - 1, nf + 1); }
 
-// Window with 1 at stencil entry q, 0 elsewhere: (A x)(row) of it is the
-// row's coefficient of that entry.
-template <typename T>
-__device__ __forceinline__ Win<T> onehot(int q)
-{
-    Win<T> w = {};
-    switch (q) {
-    case 0: w.c1 = T(1); break;
-    case 1: w.c2 = T(1); break;
-    case 2: w.c0 = T(1); break;
-    case 3: w.e1 = T(1); break;
-    case 4: w.w1 = T(1); break;
-    case 5: w.n1 = T(1); break;
-    case 6: w.s1 = T(1); break;
-    case 7: w.e2 = T(1); break;
-    case 8: w.w2 = T(1); break;
-    case 9: w.e0 = T(1); break;
-    case 10: w.w0 = T(1); break;
-    case 11: w.n2 = T(1); break;
-    case 12: w.s2 = T(1); break;
-    case 13: w.n0 = T(1); break;
-    default: w.s0 = T(1); break;
-    }
-    return w;
-}
-
-// Coefficients of an interior-column row of the fine operator (analytic or
-// probed, FP64), read off its own row evaluation.
-template <class Op>
-struct MgExtractOp {
-    Op op;
-    long P;
-    int mz;
-    __device__ void coefs(int k, long c, double a[15]) const
-    {
-        const long n = (long)k * P + c;
-        const typename Op::Cf f = op.load(n);
-        const typename Op::Bc b = op.bc(c);
-#pragma unroll
-        for (int q = 0; q < 15; ++q) a[q] = (double)op.row(k, mz, false, onehot<typename Op::type>(q), f, b);
-    }
+// Inputs of the set-up chain at level c, averaged from level f: the cell
+// fields over the fine cells of a coarse cell, u-face fields over the fine
+// j-range at the fine face under the coarse face (v-faces alike), the terrain
+// plane over the fine columns. One coarse row per thread (plane values from
+// the threads of level 0).
+struct MgMetricsIn {
+    const double *dzdx, *dzdy, *jaco, *jaco_w, *alpha, *adv_dz, *dzdx_u, *jaco_u, *dzdy_v, *jaco_v, *terrain;
 };
-
-// Coefficients of a coarse level: entry 0 holds the row sum, 1..14 the
-// off-diagonal couplings; the diagonal is the sum minus the couplings.
-struct MgExtractC {
-    const float *A;
-    long P, N;
-    __device__ void coefs(int k, long c, double a[15]) const
-    {
-        const long n = (long)k * P + c;
-        double off = 0.0;
-#pragma unroll
-        for (int q = 1; q < 15; ++q) {
-            a[q] = (double)A[q * N + n];
-            off += a[q];
-        }
-        a[0] = (double)A[n] - off;
-    }
+struct MgMetricsOut {
+    double *dzdx, *dzdy, *jaco, *jaco_w, *alpha, *adv_dz, *dzdx_u, *jaco_u, *dzdy_v, *jaco_v, *terrain;
 };
-
-// fine column index -> coarse column index (lateral ring to lateral ring)
-__device__ __forceinline__ int mg_cmap(int i, int nf, int nc)
-{
-    if (i <= 0) return 0;
-    if (i > nf) return nc + 1;
-    return (i + 1) >> 1;
-}
-
-// Galerkin coarse operator of level c from level f, one coarse row per thread.
-template <class Ext>
-__global__ __launch_bounds__(kMgThreads) void k_mg_galerkin(Ext ext, MgGeo f, MgGeo c, float *__restrict__ Ac,
-                                                            double hs)
+__global__ __launch_bounds__(kMgThreads) void k_mg_coarsen(MgGeo f, MgGeo c, MgMetricsIn a, MgMetricsOut o,
+                                                           int alpha_only)
 {
     const long n = (long)blockIdx.x * kMgThreads + threadIdx.x;
     if (n >= c.N) return;
     const int k = (int)(n / c.P);
     const long cc = n - (long)k * c.P;
     const int I = (int)(cc % c.mx), J = (int)(cc / c.mx);
-    double a[15];
-#pragma unroll
-    for (int q = 0; q < 15; ++q) a[q] = 0.0;
-    if (I == 0 || I == c.mx - 1 || J == 0 || J == c.my - 1) {
-        a[0] = 1.0;
-    } else {
-        for (int dj = 0; dj < 2; ++dj) {
-            const int j = 2 * J - 1 + dj;
-            if (j > f.ny) continue;
-            for (int di = 0; di < 2; ++di) {
-                const int i = 2 * I - 1 + di;
-                if (i > f.nx) continue;
-                double fa[15];
-                ext.coefs(k, (long)j * f.mx + i, fa);
-                for (int q = 0; q < 15; ++q) {
-                    if (fa[q] == 0.0) continue;
-                    const int NI = mg_cmap(i + off_i(q), f.nx, c.nx), NJ = mg_cmap(j + off_j(q), f.ny, c.ny);
-                    const int qc = stencil_index(NI - I, off_k(q), NJ - J);
-                    if (qc >= 0) a[qc] += fa[q];
-                }
-            }
-        }
-        if (k >= 1 && k <= c.mz - 2) {
-            const double hsum = ((a[3] + a[4]) + a[5]) + a[6];
-            a[3] *= hs; a[4] *= hs; a[5] *= hs; a[6] *= hs;
-            a[0] += (1.0 - hs) * hsum;
-        }
+    int i0, i1, j0, j1;
+    mg_fine_range(I, f.nx, c.nx, i0, i1);
+    mg_fine_range(J, f.ny, c.ny, j0, j1);
+    const long fk = (long)k * f.P;
+    auto cell = [&](const double *p) {
+        double s = 0.0;
+        for (int j = j0; j <= j1; ++j)
+            for (int i = i0; i <= i1; ++i) s += p[fk + (long)j * f.mx + i];
+        return s / (double)((i1 - i0 + 1) * (j1 - j0 + 1));
+    };
+    o.alpha[n] = cell(a.alpha);
+    if (alpha_only) return;
+    o.dzdx[n] = cell(a.dzdx);
+    o.dzdy[n] = cell(a.dzdy);
+    o.jaco[n] = cell(a.jaco);
+    o.jaco_w[n] = cell(a.jaco_w);
+    o.adv_dz[n] = cell(a.adv_dz);
+    const int fi = mg_fine_face(I, f.nx), fj = mg_fine_face(J, f.ny);
+    double su = 0.0, sj = 0.0;
+    for (int j = j0; j <= j1; ++j) {
+        const long q = fk + (long)j * f.mx + fi;
+        su += a.dzdx_u[q];
+        sj += a.jaco_u[q];
     }
-    double off = 0.0;
-    for (int q = 1; q < 15; ++q) {
-        Ac[q * c.N + n] = (float)a[q];
-        off += a[q];
+    o.dzdx_u[n] = su / (double)(j1 - j0 + 1);
+    o.jaco_u[n] = sj / (double)(j1 - j0 + 1);
+    double sv = 0.0, sk = 0.0;
+    for (int i = i0; i <= i1; ++i) {
+        const long q = fk + (long)fj * f.mx + i;
+        sv += a.dzdy_v[q];
+        sk += a.jaco_v[q];
     }
-    Ac[n] = (float)(a[0] + off);  // row sum (zero up to rounding on every non-lateral row)
-}
-
-// Thomas factors (lo, cp, 1/pivot) in FP64 of every column's tridiagonal
-// (C, A, B) of level c as stored, i.e. of the operator the residual kernel
-// applies: the diagonal is s - sum of the couplings, written as h - B - C with
-// h = s - (couplings outside the column), so that the pivots of nearly
-// singular columns (derivative conditions at both ends, weak horizontal
-// coupling on the coarse levels) carry no cancellation of B and C.
-__global__ __launch_bounds__(kMgThreads) void k_mg_factor(MgGeo c, const float *__restrict__ Ac,
-                                                          float *__restrict__ lo, float *__restrict__ cp,
-                                                          float *__restrict__ ip)
-{
-    const long cc = (long)blockIdx.x * kMgThreads + threadIdx.x;
-    if (cc >= c.P) return;
-    double cprev = 0.0;
-    for (int k = 0; k < c.mz; ++k) {
-        const long n = (long)k * c.P + cc;
-        double h = (double)Ac[n];
-        for (int q = 3; q < 15; ++q) h -= (double)Ac[q * c.N + n];
-        const double l = (double)Ac[2 * c.N + n], u = (double)Ac[c.N + n];
-        double piv = (h - u) - l * (1.0 + cprev);  // d - l cp_{k-1} with d = h - l - u
-        if (fabs(piv) < 1e-300) piv = copysign(1e-300, piv);
-        const double inv = 1.0 / piv;
-        cprev = u * inv;
-        lo[n] = (float)l;
-        cp[n] = (float)cprev;
-        ip[n] = (float)inv;
+    o.dzdy_v[n] = sv / (double)(i1 - i0 + 1);
+    o.jaco_v[n] = sk / (double)(i1 - i0 + 1);
+    if (k == 0) {
+        double st = 0.0;
+        for (int j = j0; j <= j1; ++j)
+            for (int i = i0; i <= i1; ++i) st += a.terrain[(long)j * f.mx + i];
+        o.terrain[cc] = st / (double)((i1 - i0 + 1) * (j1 - j0 + 1));
     }
-}
-
-// r = b - A x on a coarse level, one row per thread: lateral rows are
-// identity; every other row sums to zero, so (A x)(row) = s x_c + sum_q a_q
-// (x_q - x_c) with s the stored row sum (differences keep the smooth
-// components accurate in FP32). (The V-cycle kernels do nothing once the
-// inner solve whose preconditioner they apply has converged.)
-__global__ __launch_bounds__(kMgThreads) void k_mg_resid(MgGeo g, const KState *st, const float *__restrict__ A,
-                                                         const float *__restrict__ x, const float *__restrict__ b,
-                                                         float *__restrict__ r)
-{
-    if (st->done) return;
-    const long n = (long)blockIdx.x * kMgThreads + threadIdx.x;
-    if (n >= g.N) return;
-    const int k = (int)(n / g.P);
-    const long c = n - (long)k * g.P;
-    const int i = (int)(c % g.mx), j = (int)(c / g.mx);
-    const float xc = x[n];
-    if (i == 0 || i == g.mx - 1 || j == 0 || j == g.my - 1) {
-        r[n] = b[n] - xc;
-        return;
-    }
-    const long N = g.N, P = g.P, mx = g.mx;
-    float ax = A[n] * xc;
-    ax += A[3 * N + n] * (x[n + 1] - xc);
-    ax += A[4 * N + n] * (x[n - 1] - xc);
-    ax += A[5 * N + n] * (x[n + mx] - xc);
-    ax += A[6 * N + n] * (x[n - mx] - xc);
-    if (k + 1 < g.mz) {
-        const long u = n + P;
-        ax += A[1 * N + n] * (x[u] - xc);
-        ax += A[7 * N + n] * (x[u + 1] - xc);
-        ax += A[8 * N + n] * (x[u - 1] - xc);
-        ax += A[11 * N + n] * (x[u + mx] - xc);
-        ax += A[12 * N + n] * (x[u - mx] - xc);
-    }
-    if (k > 0) {
-        const long d = n - P;
-        ax += A[2 * N + n] * (x[d] - xc);
-        ax += A[9 * N + n] * (x[d + 1] - xc);
-        ax += A[10 * N + n] * (x[d - 1] - xc);
-        ax += A[13 * N + n] * (x[d + mx] - xc);
-        ax += A[14 * N + n] * (x[d - mx] - xc);
-    }
-    r[n] = b[n] - ax;
 }
 
 // out = y + w L^-1 in on a level whose columns are too tall for the
@@ -282,8 +147,8 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_line(MgGeo g, const KState *s
     }
 }
 
-// b_c = R r_f, one coarse row per thread: every interior row sums the rows of
-// its aggregate; lateral rows get zero.
+// b_c = R r_f, one coarse row per thread: every interior row is the mean of
+// the rows of its aggregate; lateral rows get zero.
 __global__ __launch_bounds__(kMgThreads) void k_mg_restrict(MgGeo f, MgGeo c, const KState *st,
                                                             const float *__restrict__ rf, float *__restrict__ bc)
 {
@@ -307,11 +172,13 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_restrict(MgGeo f, MgGeo c, co
         if (hi) t += rf[m + f.mx + 1];
         s += t;
     }
-    bc[n] = s;
+    bc[n] = s / (float)((hi ? 2 : 1) * (hj ? 2 : 1));
 }
 
-// x_f += P e_c, one fine row per thread: every interior row takes its
-// aggregate's value.
+// x_f += P e_c, one fine row per thread: cell-centred bilinear interpolation
+// in x and y (a fine cell is the left or right, lower or upper half of its
+// coarse cell: weights 9/16 on its own coarse cell, 3/16 on the two nearer
+// neighbours, 1/16 on the diagonal one; ring values are zero), the level kept.
 __global__ __launch_bounds__(kMgThreads) void k_mg_prolong(MgGeo f, MgGeo c, const KState *st,
                                                            const float *__restrict__ ec, float *__restrict__ xf)
 {
@@ -322,7 +189,12 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_prolong(MgGeo f, MgGeo c, con
     const long fc = n - (long)k * f.P;
     const int i = (int)(fc % f.mx), j = (int)(fc / f.mx);
     if (i == 0 || i == f.mx - 1 || j == 0 || j == f.my - 1) return;
-    xf[n] += ec[(long)k * c.P + (long)((j + 1) >> 1) * c.mx + ((i + 1) >> 1)];
+    const int I = (i + 1) >> 1, J = (j + 1) >> 1;
+    const int In = (i & 1) ? I - 1 : I + 1, Jn = (j & 1) ? J - 1 : J + 1;
+    const float *e = ec + (long)k * c.P;
+    const float v = 0.5625f * e[(long)J * c.mx + I] + 0.1875f * (e[(long)J * c.mx + In] + e[(long)Jn * c.mx + I])
+                  + 0.0625f * e[(long)Jn * c.mx + In];
+    xf[n] += v;
 }
 
 }  // namespace windhip

@@ -152,11 +152,11 @@ module windhip_interface
         end function
 
         integer(c_int) function windhip_set_multigrid(handle, pre, post, coarse_pre, coarse_post, depth, &
-                                                      omega, hscale) bind(C, name="windhip_set_multigrid")
+                                                      omega) bind(C, name="windhip_set_multigrid")
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: handle
             integer(c_int), value :: pre, post, coarse_pre, coarse_post, depth
-            real(c_double), value :: omega, hscale
+            real(c_double), value :: omega
         end function
     end interface
 

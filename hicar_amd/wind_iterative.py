@@ -172,17 +172,18 @@ class WindSolver:
         nat.check(self.lib.windhip_set_lambda(self.h, lam.ctypes.data_as(C.POINTER(C.c_double))))
 
     def set_multigrid(self, pre: int = 2, post: int = 2, coarse_pre: int = 1, coarse_post: int = 1, depth: int = 0,
-                      omega: float = 0.85, hscale: float = 0.7) -> None:
+                      omega: float = 0.9) -> None:
         """V-cycle of the MIXED_MG inner preconditioner (no counterpart in the reference)."""
         nat.check(self.lib.windhip_set_multigrid(self.h, int(pre), int(post), int(coarse_pre), int(coarse_post),
-                                                 int(depth), float(omega), float(hscale)))
+                                                 int(depth), float(omega)))
 
     def mg_level(self, level: int):
-        """Coarse level `level` of the last MIXED_MG solve: ((nx, ny, mz), coefficients [15, mz, ny+2, nx+2])."""
+        """Coarse level `level` of the last MIXED_MG solve: ((nx, ny, mz), coefficients [6, mz, ny+2, nx+2]
+        = B, C, H, I, L, M of the rediscretised operator)."""
         dims = (C.c_int * 3)()
         nat.check(self.lib.windhip_mg_level(self.h, int(level), dims, None))
         nx, ny, mz = dims[0], dims[1], dims[2]
-        out = np.zeros((15, mz, ny + 2, nx + 2), dtype=np.float32)
+        out = np.zeros((6, mz, ny + 2, nx + 2), dtype=np.float32)
         nat.check(self.lib.windhip_mg_level(self.h, int(level), dims, out.ctypes.data_as(C.POINTER(C.c_float))))
         return (nx, ny, mz), out
 

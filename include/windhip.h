@@ -209,21 +209,19 @@ int windhip_net_time_forward(windhip_net *net, int B, int D, int H, int W, int r
 int windhip_set_net(windhip_solver *s, windhip_net *net);
 int windhip_set_guess_smoothing(windhip_solver *s, int steps);
 /* MIXED_MG V-cycle: `pre` and `post` damped line-smoothing steps on the fine
- * level and `coarse_pre`, `coarse_post` on the coarse levels (1..8, defaults
- * 2, 2, 1, 1) around each coarse correction; `depth` coarse levels used (0, the
- * default: all of them, down to one interior column, solved exactly; a
- * shallower cycle smooths its last level coarse_pre + coarse_post times);
- * damping omega (default 0.85); hscale (default 0.7), the factor on the coarse
- * same-level horizontal couplings that keeps aggregated second differences at
- * their coarse-grid size. No counterpart in the reference. */
+ * level and `coarse_pre` (1..8), `coarse_post` (0..8) on the coarse levels
+ * (defaults 2, 2, 1, 1) around each coarse correction; `depth` coarse levels
+ * used (0, the default: all of them, down to one interior column, solved
+ * exactly; a shallower cycle smooths its last level coarse_pre + coarse_post
+ * times); damping omega (default 0.9). No counterpart in the reference. */
 int windhip_set_multigrid(windhip_solver *s, int pre, int post, int coarse_pre, int coarse_post, int depth,
-                          double omega, double hscale);
+                          double omega);
 /* Coarse level `level` (1 = first coarsening) of the MIXED_MG hierarchy built
  * by the last MIXED_MG solve: dims_out = (nx, ny, mz) of the level (interior
- * columns, rows per column); coef_host (may be null) receives its 15 x N
- * FP32 coefficients, rows [k][j][i] of (nx+2) x (ny+2) columns, entry 0 the
- * row sum and 1..14 the couplings B..O of the 15-point pattern. Inspection
- * only; no counterpart in the reference. */
+ * columns, rows per column); coef_host (may be null) receives its 6 x N FP32
+ * coefficients B, C, H, I, L, M (the analytic operator rediscretised at the
+ * level's dx from its averaged metrics), rows [k][j][i] of (nx+2) x (ny+2)
+ * columns. Inspection only; no counterpart in the reference. */
 int windhip_mg_level(windhip_solver *s, int level, int *dims_out, float *coef_host);
 /* The network input for the solver's current rhs (that of its last solve),
  * metrics and alpha, written to in_dev (1 x dims[0] x dims[1] x dims[2] x 8);

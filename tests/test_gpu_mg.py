@@ -1,9 +1,9 @@
 """The multigrid inner preconditioner (MIXED_MG) through the C-ABI.
 
-Its coarse operators are checked against the Galerkin product R A P formed
-with the oracle's FP64 spmv and the aggregation transfers of
-oracle/mg_oracle.py; its solves against the oracle's FP64 lambda (1e-6
-relative L2, the mixed-precision bar).
+Its coarse operators are checked against the oracle's assembly at the
+coarse size and spacing from metrics averaged by oracle/mg_oracle.py; its
+solves against the oracle's FP64 lambda (1e-6 relative L2, the
+mixed-precision bar).
 """
 import numpy as np
 import pytest
@@ -35,60 +35,39 @@ def _setup(name="fractal_alpha"):
     return c, o, div, s
 
 
-def _interior(e):
-    e = e.copy()
-    e[0] = e[-1] = 0.0
-    e[:, :, 0] = e[:, :, -1] = 0.0
-    return e
+def _level_vs_oracle(s, level, metrics, alpha, nz, dx):
+    """Coefficients B, C, H, I, L, M of solver level `level` against the
+    oracle's assembly at that level's size and dx from the averaged metrics."""
+    from oracle import Oracle
+    (nx, ny, mz), got = s.mg_level(level)
+    assert mz == nz + 2
+    o = Oracle(nx, ny, nz, dx)
+    o.set_metrics(dict(metrics, terrain=metrics["terrain"]))
+    o.set_alpha(alpha)
+    o.assemble()
+    for q, name in enumerate("BCHILM"):
+        want = o.coef(name)                                   # (ny, nz, nx)
+        g = got[q][1:-1, 1:-1, 1:-1].transpose(1, 0, 2)      # rows [k][j][i] -> (ny, nz, nx)
+        scale = np.abs(want).max()
+        assert np.allclose(g, want, rtol=2e-6, atol=2e-7 * scale), name
+    o.close()
+    return nx, ny
 
 
-def test_coarse_levels_are_galerkin_products():
-    """With hscale = 1 level 1 is R A P of the fine operator and level 2 is
-    R A_1 P, i.e. the 4 x 4 aggregation of A (odd edges included: ny = 23)."""
+def test_coarse_levels_rediscretise_the_operator():
+    """Level 1 and 2 hold the analytic operator assembled at 2 dx and 4 dx from
+    the metrics and alpha averaged over 2 x 2 columns (odd edge: ny = 23)."""
     from hicar_amd import _native as nat
     c, o, div, s = _setup()
-    o.set_alpha(c.alpha)
-    o.assemble()
-    s.set_multigrid(hscale=1.0)
     info = s.solve(c.alpha, div, precision=nat.MIXED_MG, tol_rel=1e-6, start=1, max_iters=2000)
     assert info["status"] == 0
-    rng = np.random.default_rng(11)
-    (nx1, ny1, mz), a1 = s.mg_level(1)
-    assert (nx1, ny1, mz) == (mo.coarse_size(c.nx), mo.coarse_size(c.ny), c.nz + 2)
-    e1 = _interior(rng.standard_normal((ny1 + 2, mz, nx1 + 2)))
-    want = mo.restrict(o.spmv(mo.prolong(e1, c.nx, c.ny)), c.nx, c.ny)
-    got = mo.apply15(mo.solver_level_to_rows(a1), e1)
-    inner = (slice(1, -1), slice(None), slice(1, -1))
-    assert _rel(got[inner], want[inner]) <= 1e-5
-    (nx2, ny2, _), a2 = s.mg_level(2)
-    e2 = _interior(rng.standard_normal((ny2 + 2, mz, nx2 + 2)))
-    pe = mo.prolong(mo.prolong(e2, nx1, ny1), c.nx, c.ny)
-    want2 = mo.restrict(mo.restrict(o.spmv(pe), c.nx, c.ny), nx1, ny1)
-    got2 = mo.apply15(mo.solver_level_to_rows(a2), e2)
-    assert _rel(got2[inner], want2[inner]) <= 1e-5
-
-
-def test_hscale_scales_only_the_same_level_couplings():
-    from hicar_amd import _native as nat
-    c, o, div, s = _setup()
-    levels = {}
-    for hs in (1.0, 0.7):
-        s.set_multigrid(hscale=hs)
-        s.set_lambda(np.zeros(s.rows_shape))
-        s.solve(c.alpha, div, precision=nat.MIXED_MG, tol_rel=1e-6, start=1, max_iters=2000)
-        levels[hs] = s.mg_level(1)[1].astype(np.float64)
-    a, b = levels[1.0], levels[0.7]
-    mid = (slice(1, -1), slice(1, -1), slice(1, -1))  # interior rows, k = 1 .. mz-2
-    for q in range(1, 15):
-        if q in (3, 4, 5, 6):  # D, E, F, G
-            assert np.allclose(b[q][mid], 0.7 * a[q][mid], rtol=1e-6, atol=0)
-        else:
-            assert np.array_equal(b[q][mid], a[q][mid])
-    for k in (0, -1):  # ghost-plane rows keep the exact product
-        assert np.array_equal(b[:, k], a[:, k])
-    # every non-lateral row still sums to zero (the row sum is entry 0)
-    scale = np.abs(b[1:]).max()
-    assert np.abs(b[0][:, 1:-1, 1:-1]).max() <= 1e-6 * scale
+    keys = mo.CELL + mo.U_FACE + mo.V_FACE + ("terrain",)
+    m0 = {k: c.metrics[k] for k in keys}
+    m1, a1 = mo.coarsen_metrics(m0, c.alpha)
+    nx1, ny1 = _level_vs_oracle(s, 1, m1, a1, c.nz, 2 * c.dx)
+    assert (nx1, ny1) == (mo.coarse_size(c.nx), mo.coarse_size(c.ny))
+    m2, a2 = mo.coarsen_metrics(m1, a1)
+    _level_vs_oracle(s, 2, m2, a2, c.nz, 4 * c.dx)
 
 
 def test_mixed_mg_matches_oracle():

@@ -1,66 +1,68 @@
-"""Pins of the multigrid transfer restatement (oracle/mg_oracle.py), CPU only."""
+"""Pins of the multigrid metric coarsening (oracle/mg_oracle.py), CPU only."""
 import numpy as np
 import pytest
 
 import mg_oracle as mo  # oracle/mg_oracle.py
-from conftest import oracle_for, small_case
+
+
+def _fields(rng, nx, ny, nz):
+    m = {k: rng.standard_normal((ny, nz, nx)) for k in mo.CELL}
+    m.update({k: rng.standard_normal((ny, nz, nx + 1)) for k in mo.U_FACE})
+    m.update({k: rng.standard_normal((ny + 1, nz, nx)) for k in mo.V_FACE})
+    m["terrain"] = rng.standard_normal((ny, nx))
+    return m, rng.standard_normal((ny, nz, nx))
+
+
+def _cell_set(c, n):
+    """Fine cells under coarse cell c (natural indices)."""
+    return [i for i in (2 * c, 2 * c + 1) if i < n]
 
 
 @pytest.mark.parametrize("nx,ny", [(1, 1), (2, 3), (5, 4), (7, 7), (6, 1)])
-def test_prolong_matches_aggregate_definition(nx, ny):
+def test_coarsening_matches_the_index_sets(nx, ny):
     rng = np.random.default_rng(nx * 10 + ny)
-    nxc, nyc, mz = mo.coarse_size(nx), mo.coarse_size(ny), 3
-    ec = rng.standard_normal((nyc + 2, mz, nxc + 2))
-    want = np.zeros((ny + 2, mz, nx + 2))
-    for j in range(1, ny + 1):
-        for i in range(1, nx + 1):
-            # fine columns 2I-1, 2I belong to coarse column I
-            I = next(I for I in range(1, nxc + 1) if i in (2 * I - 1, 2 * I))
-            J = next(J for J in range(1, nyc + 1) if j in (2 * J - 1, 2 * J))
-            want[j, :, i] = ec[J, :, I]
-    assert np.array_equal(mo.prolong(ec, nx, ny), want)
-    # every coarse interior column receives at least one fine column, at most four
-    hits = np.zeros((nyc + 2, nxc + 2), dtype=int)
-    ci, cj = mo.cmap(nx), mo.cmap(ny)
-    for j in range(1, ny + 1):
-        for i in range(1, nx + 1):
-            hits[cj[j], ci[i]] += 1
-    assert hits[1:-1, 1:-1].min() >= 1 and hits.max() <= 4 and hits[0].sum() == 0
-
-
-@pytest.mark.parametrize("nx,ny", [(2, 3), (5, 4), (9, 6)])
-def test_restrict_is_the_transpose_of_prolong(nx, ny):
-    rng = np.random.default_rng(7)
-    mz = 4
+    nz = 3
+    m, alpha = _fields(rng, nx, ny, nz)
+    got, a_c = mo.coarsen_metrics(m, alpha)
     nxc, nyc = mo.coarse_size(nx), mo.coarse_size(ny)
-    y = rng.standard_normal((ny + 2, mz, nx + 2))
-    e = rng.standard_normal((nyc + 2, mz, nxc + 2))
-    e[0] = e[-1] = 0.0
-    e[:, :, 0] = e[:, :, -1] = 0.0
-    lhs = float(np.sum(mo.restrict(y, nx, ny) * e))
-    rhs = float(np.sum(y * mo.prolong(e, nx, ny)))
-    assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + 1.0)
+    for k in mo.CELL + ("alpha",):
+        src = alpha if k == "alpha" else m[k]
+        out = a_c if k == "alpha" else got[k]
+        assert out.shape == (nyc, nz, nxc)
+        for J in range(nyc):
+            for I in range(nxc):
+                cells = [(j, i) for j in _cell_set(J, ny) for i in _cell_set(I, nx)]
+                want = np.mean([src[j, :, i] for j, i in cells], axis=0)
+                assert np.allclose(out[J, :, I], want, rtol=1e-14, atol=1e-14)
+    for k in mo.U_FACE:
+        assert got[k].shape == (nyc, nz, nxc + 1)
+        for J in range(nyc):
+            for a in range(nxc + 1):
+                want = np.mean([m[k][j, :, min(2 * a, nx)] for j in _cell_set(J, ny)], axis=0)
+                assert np.allclose(got[k][J, :, a], want, rtol=1e-14, atol=1e-14)
+    for k in mo.V_FACE:
+        assert got[k].shape == (nyc + 1, nz, nxc)
+        for b in range(nyc + 1):
+            for I in range(nxc):
+                want = np.mean([m[k][min(2 * b, ny), :, i] for i in _cell_set(I, nx)], axis=0)
+                assert np.allclose(got[k][b, :, I], want, rtol=1e-14, atol=1e-14)
+    for J in range(nyc):
+        for I in range(nxc):
+            want = np.mean([m["terrain"][j, i] for j in _cell_set(J, ny) for i in _cell_set(I, nx)])
+            assert abs(got["terrain"][J, I] - want) <= 1e-14
 
 
-def test_apply15_matches_the_oracle_probed_spmv():
-    """apply15 on a random 15-array operator equals the oracle's probed spmv
-    (wind_iterative.F90:888-904) on every interior row."""
-    c = small_case("fractal")
-    o = oracle_for(c)
-    o.set_alpha(c.alpha)
-    o.assemble()
-    rng = np.random.default_rng(3)
-    a = [rng.standard_normal((c.ny, c.nz, c.nx)) for _ in range(15)]
-    o.set_probed(True)
-    for q, name in enumerate("ABCDEFGHIJKLMNO"):
-        o.set_coef(name, a[q])
-    x = rng.standard_normal(o.rows_shape)
-    want = o.spmv(x)
-    coef = np.zeros((15,) + o.rows_shape)
-    for q in range(1, 15):
-        coef[q][1:-1, 1:-1, 1:-1] = a[q]
-    coef[0][1:-1, 1:-1, 1:-1] = sum(a)  # row sum
-    got = mo.apply15(coef, x)
-    inner = (slice(1, -1), slice(1, -1), slice(1, -1))
-    assert np.allclose(got[inner], want[inner], rtol=1e-12, atol=1e-12)
-    o.close()
+def test_constants_stay_and_linear_fields_keep_their_mean_position():
+    nx, ny, nz = 9, 6, 4
+    m = {k: np.full((ny, nz, nx), 2.5) for k in mo.CELL}
+    m.update({k: np.full((ny, nz, nx + 1), -1.0) for k in mo.U_FACE})
+    m.update({k: np.full((ny + 1, nz, nx), 3.0) for k in mo.V_FACE})
+    xi = np.arange(nx, dtype=float)
+    m["terrain"] = np.tile(7.0 * xi, (ny, 1))
+    got, a_c = mo.coarsen_metrics(m, np.ones((ny, nz, nx)))
+    for k in mo.CELL:
+        assert np.all(got[k] == 2.5)
+    assert np.all(got["jacobian_u"] == -1.0) and np.all(got["dzdy_v"] == 3.0) and np.all(a_c == 1.0)
+    # cell-centred means of a linear field are its value at the mean position
+    pos = np.array([np.mean(_cell_set(I, nx)) for I in range(mo.coarse_size(nx))])
+    assert np.allclose(got["terrain"], np.tile(7.0 * pos, (mo.coarse_size(ny), 1)), rtol=0, atol=1e-12)
