@@ -135,8 +135,8 @@ struct windhip_solver {
     };
     std::vector<MgLevel> mg;
     int mg_pre = 2, mg_post = 2;    // fine-level line-smoothing steps before and after the coarse correction
-    int mg_cpre = 1, mg_cpost = 1;  // the same on the coarse levels
-    int mg_depth = 0;               // coarse levels used (0: all, down to one interior column)
+    int mg_cpre = 2, mg_cpost = 2;  // the same on the coarse levels
+    int mg_depth = 0;               // coarse levels used (0: down to the last level with >= 4 columns each way)
     float mg_omega = 0.9f;          // damping of the line smoother
     template <typename T>
     OpA<T> level_op(const MgLevel &L) const;
@@ -735,7 +735,17 @@ float *windhip_solver::mg_coarse(size_t l)
     auto resid = [&](const float *xv) {
         timed(PK_MG_COARSE, [&] { k_resid_c<OpA<float>><<<kb, kColThreads, 0, stream>>>(op, gl, sp, xv, L.b, L.r); });
     };
-    const size_t used = mg_depth > 0 ? std::min(mg.size(), (size_t)mg_depth) : mg.size();
+    size_t used = mg.size();
+    if (mg_depth > 0) {
+        used = std::min(mg.size(), (size_t)mg_depth);
+    } else {
+        // the deepest level that still has 4 interior columns each way: below
+        // it the cell-centred levels misplace the lateral boundary by half a
+        // coarse cell and their correction of the smoothest error hurts
+        size_t q = 0;
+        while (q < mg.size() && std::min(mg[q].g.nx, mg[q].g.ny) >= 4) ++q;
+        used = std::max<size_t>(q, 1);
+    }
     if (l + 1 == mg.size()) {  // one interior column: its line solve is exact
         line(L.b, nullptr, L.x, 1.0f);
         return L.x;

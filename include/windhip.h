@@ -210,10 +210,11 @@ int windhip_set_net(windhip_solver *s, windhip_net *net);
 int windhip_set_guess_smoothing(windhip_solver *s, int steps);
 /* MIXED_MG V-cycle: `pre` and `post` damped line-smoothing steps on the fine
  * level and `coarse_pre` (1..8), `coarse_post` (0..8) on the coarse levels
- * (defaults 2, 2, 1, 1) around each coarse correction; `depth` coarse levels
- * used (0, the default: all of them, down to one interior column, solved
- * exactly; a shallower cycle smooths its last level coarse_pre + coarse_post
- * times); damping omega (default 0.9). No counterpart in the reference. */
+ * (defaults 2 each) around each coarse correction; `depth` coarse levels used
+ * (0, the default: down to the deepest level with at least 4 interior columns
+ * each way; a level with one interior column is solved exactly, any other
+ * last level is smoothed coarse_pre + coarse_post times); damping omega
+ * (default 0.9). No counterpart in the reference. */
 int windhip_set_multigrid(windhip_solver *s, int pre, int post, int coarse_pre, int coarse_post, int depth,
                           double omega);
 /* Coarse level `level` (1 = first coarsening) of the MIXED_MG hierarchy built
