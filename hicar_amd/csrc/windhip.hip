@@ -66,7 +66,9 @@ struct windhip_solver {
     long N = 0;
     long guard = 0;  // elements before and after every row array (mx + 1)
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipStream_t net_stream = nullptr;  // the learned guess, concurrent with the first inner solve
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_net0 = nullptr, ev_net1 = nullptr;
+    double *partials2 = nullptr;       // reduction partials of the guess residual (second stream)
     size_t bytes = 0;
     std::vector<void *> allocs;
 
@@ -100,7 +102,7 @@ struct windhip_solver {
     double *partials;
     double *scan_part, *scan_out;
     unsigned *scan_cnt;
-    KState *st;       // [0] FP64 solve / outer, [1] FP32 inner
+    KState *st;       // [0] FP64 solve / outer, [1] FP32 inner, [2] learned guess residual (second stream)
     KState *st_host;  // pinned mirror
     double *scan_host;
     // staging for host-memory fields (caller layout, caller dtype)
@@ -288,12 +290,15 @@ struct windhip_solver {
     int read_state(int idx);
     int run_plain(int max_iters, double tol_rel, double tol_abs, int batch, bool use_graph, int *status,
                   int *iters, double *res0, double *resf);
-    int run_mixed(const windhip_solve_opts &o, windhip_solve_info *info, const double *first_dir = nullptr);
+    int run_mixed(const windhip_solve_opts &o, windhip_solve_info *info, double *learned_guess = nullptr,
+                  double plain_res = 0.0);
     int true_residual(double *rel, const double *xv = nullptr);
     int net_dims(int *Dp, int *Hp, int *Wp) const;
     int net_input(float *in, double *scale);
-    int learned_start(double *res_guess, double *res_plain, double **guess);
-    int line_relax(double **y, double **scratch, int steps);
+    int line_relax(double **y, double **scratch, int steps, hipStream_t strm);
+    int learned_launch(double *res_plain, double **guess);
+    int learned_join(double *res_guess);
+    bool net_pending = false;
     template <typename T>
     int enqueue_iterations(int count);
     template <class Launch>
@@ -406,8 +411,11 @@ int windhip_solver::init()
     if ((N + 2 * guard) * (long)sizeof(double) >= (1L << 32))
         return fail(WINDHIP_ERR_ARG, "windhip_create: %ld rows exceed the 4 GiB span of one buffer resource", N);
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&net_stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ev0));
     HIPCHK(hipEventCreate(&ev1));
+    HIPCHK(hipEventCreateWithFlags(&ev_net0, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev_net1, hipEventDisableTiming));
     double **rows64[] = {&m_dzdx, &m_dzdy, &m_jaco, &m_jaco_w, &m_alpha, &m_adv_dz, &m_dz_if,
                          &m_dzdx_u, &m_jaco_u, &m_dzdy_v, &m_jaco_v, &B, &C, &H, &I, &L, &M,
                          &dinv, &x, &r, &rh, &p, &ph[0], &ph[1], &ph[2], &v, &s, &sh[0], &sh[1], &sh[2],
@@ -420,11 +428,12 @@ int windhip_solver::init()
     float **pl32[] = {&bcx32, &bcy32, &bcz32, &dztop32};
     for (float **q : pl32) CHK(alloc(q, d.P));
     CHK(alloc(&partials, (long)std::max(g.nt, chunk_blocks(g)) * 8));
+    CHK(alloc(&partials2, (long)g.nt * 8));
     CHK(alloc(&scan_part, 3L * 1024));
     CHK(alloc(&scan_out, 4));
     CHK(alloc(&scan_cnt, 4));
-    CHK(alloc(&st, 2));
-    HIPCHK(hipHostMalloc((void **)&st_host, 2 * sizeof(KState), hipHostMallocDefault));
+    CHK(alloc(&st, 3));
+    HIPCHK(hipHostMalloc((void **)&st_host, 3 * sizeof(KState), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&scan_host, 4 * sizeof(double), hipHostMallocDefault));
     HIPCHK(hipStreamSynchronize(stream));
     return WINDHIP_OK;
@@ -441,8 +450,12 @@ void windhip_solver::release()
         if (stage[q]) (void)hipFree(stage[q]);
     if (st_host) (void)hipHostFree(st_host);
     if (scan_host) (void)hipHostFree(scan_host);
+    if (net_stream) (void)hipStreamSynchronize(net_stream);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (ev_net0) (void)hipEventDestroy(ev_net0);
+    if (ev_net1) (void)hipEventDestroy(ev_net1);
+    if (net_stream) (void)hipStreamDestroy(net_stream);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -529,7 +542,7 @@ int windhip_solver::prepare_call(const void *alpha, const void *div)
 int windhip_solver::read_state(int idx)
 {
     (void)idx;
-    HIPCHK(hipMemcpyAsync(st_host, st, 2 * sizeof(KState), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(st_host, st, 2 * sizeof(KState), hipMemcpyDeviceToHost, stream));  // [2] is read at the join
     HIPCHK(hipStreamSynchronize(stream));
     if (profiling) CHK(harvest());
     return WINDHIP_OK;
@@ -952,7 +965,7 @@ int windhip_solver::net_input(float *in, double *scale)
 
 // `steps` line-relaxation steps on *y against the current rhs (FP64 operator),
 // alternating between *y and *scratch; *y points at the result.
-int windhip_solver::line_relax(double **y, double **scratch, int steps)
+int windhip_solver::line_relax(double **y, double **scratch, int steps, hipStream_t strm)
 {
     if (steps <= 0) return WINDHIP_OK;
     if (!lr_c) {
@@ -961,7 +974,7 @@ int windhip_solver::line_relax(double **y, double **scratch, int steps)
     }
     for (int q = 0; q < steps; ++q) {
         CHK(with_op<double>([&](auto op) {
-            k_line_relax<decltype(op)><<<g.nt, kThreads, 0, stream>>>(op, g, *y, b, lr_c, lr_d, *scratch);
+            k_line_relax<decltype(op)><<<g.nt, kThreads, 0, strm>>>(op, g, *y, b, lr_c, lr_d, *scratch);
             return WINDHIP_OK;
         }));
         HIPCHK(hipGetLastError());
@@ -973,9 +986,13 @@ int windhip_solver::line_relax(double **y, double **scratch, int steps)
 // The learned guess: the network's lambda after net_smooth vertical
 // line-relaxation steps (FP64), and the FP64 true residuals of that guess and
 // of the plain start (the current lambda: the previous solution, or zero).
-// The plain solve starts from the lower of the two; the mixed solves take
-// guess - lambda as the first direction of their first FGMRES cycle.
-int windhip_solver::learned_start(double *res_guess, double *res_plain, double **guess)
+// learned_launch takes the plain start's residual on the solver's stream, then
+// queues the network, the relaxation and the guess's residual on a second
+// stream (its own state slot and partials), so the mixed solves can run their
+// first inner solve meanwhile; learned_join makes the solver's stream wait for
+// it and reads the guess's residual. The guess lands in t or s (FP64 vectors
+// of the plain solve, unused by the mixed solves).
+int windhip_solver::learned_launch(double *res_plain, double **guess)
 {
     int Dp, Hp, Wp;
     net_dims(&Dp, &Hp, &Wp);
@@ -985,18 +1002,40 @@ int windhip_solver::learned_start(double *res_guess, double *res_plain, double *
         CHK(alloc(&net_out, vox));
         net_vox = vox;
     }
-    double scale = 0.0, rel = 0.0;
+    if (!lr_c) {
+        CHK(alloc(&lr_c, N));
+        CHK(alloc(&lr_d, N));
+    }
+    double scale = 0.0;
     CHK(net_input(net_in, &scale));
     *res_plain = st_host[0].outer_rnorm;  // of the current lambda, from net_input's residual
-    const int rc = wnet_forward_dev(net, stream, net_in, wnet::Shape{1, Dp, Hp, Wp}, net_out);
+    HIPCHK(hipEventRecord(ev_net0, stream));
+    HIPCHK(hipStreamWaitEvent(net_stream, ev_net0, 0));
+    const int rc = wnet_forward_dev(net, net_stream, net_in, wnet::Shape{1, Dp, Hp, Wp}, net_out);
     if (rc) return fail(rc, "learned start: network forward failed");
-    k_net_output<<<flat_grid(d.P), kBlock, 0, stream>>>(d, Dp, Hp, Wp, net_out, scale, t);
+    k_net_output<<<flat_grid(d.P), kBlock, 0, net_stream>>>(d, Dp, Hp, Wp, net_out, scale, t);
     HIPCHK(hipGetLastError());
     double *gy = t, *gs = s;
-    CHK(line_relax(&gy, &gs, net_smooth));
-    CHK(true_residual(&rel, gy));
-    *res_guess = st_host[0].outer_rnorm;
+    CHK(line_relax(&gy, &gs, net_smooth, net_stream));
+    CHK(with_op<double>([&](auto op) {
+        k_true_resid<decltype(op)><<<g.nt, kThreads, 0, net_stream>>>(op, g, st + 2, partials2, gy, b);
+        return WINDHIP_OK;
+    }));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ev_net1, net_stream));
+    net_pending = true;
     *guess = gy;
+    return WINDHIP_OK;
+}
+
+int windhip_solver::learned_join(double *res_guess)
+{
+    if (!net_pending) return WINDHIP_OK;
+    HIPCHK(hipStreamWaitEvent(stream, ev_net1, 0));
+    HIPCHK(hipMemcpyAsync(st_host + 2, st + 2, sizeof(KState), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    net_pending = false;
+    if (res_guess) *res_guess = st_host[2].outer_rnorm;
     return WINDHIP_OK;
 }
 
@@ -1007,8 +1046,28 @@ int windhip_solver::learned_start(double *res_guess, double *res_plain, double *
 // and the true residual are FP64; FGMRES minimises the FP64 residual over all
 // directions of a cycle, so an inaccurate inner solve slows it down but cannot
 // make the residual grow.
-int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *info, const double *first_dir)
+int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *info, double *learned_guess,
+                              double plain_res)
 {
+    // the learned guess (computed on the second stream) joins as the direction
+    // of FGMRES step 1, after the first inner solve, if its residual is below
+    // the plain start's
+    const double *first_dir = nullptr;
+    auto join_learned = [&]() -> int {
+        if (!learned_guess) return WINDHIP_OK;
+        double rg = 0.0;
+        CHK(learned_join(&rg));
+        info->guess_res = rg;
+        info->plain_res = plain_res;
+        if (rg < plain_res) {
+            k_axpby<<<flat_grid(N), kBlock, 0, stream>>>(N, 1.0, learned_guess, -1.0, x, learned_guess);
+            HIPCHK(hipGetLastError());
+            first_dir = learned_guess;
+            info->start_used = WINDHIP_START_LEARNED;
+        }
+        learned_guess = nullptr;
+        return WINDHIP_OK;
+    };
     CHK(alloc_f32_krylov());
     const bool multigrid = o.precision == WINDHIP_MIXED_MG;
     const bool line = o.precision == WINDHIP_MIXED_LINE || multigrid;
@@ -1055,12 +1114,14 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
         int k = 0;
         bool breakdown = false;
         for (int j = 0; j < m && steps < o.max_outer && total_inner < o.max_iters; ++j) {
-            if (j == 0 && first_dir) {
-                // augmented first direction (the learned correction): no inner
-                // solve; the set-up still normalises v_0 in place
-                k_fg_setup<<<fb, kThreads, 0, stream>>>(N, st + 1, fg_part, beta2, 1, fg_v[0], dinv32, x32, r32,
+            if (j == 1 && learned_guess) CHK(join_learned());
+            if (first_dir) {
+                // the learned correction as direction j: no inner solve; the
+                // set-up still normalises v_j in place
+                const double *vsq = j == 0 ? beta2 : fg_h + (long)(j - 1) * H + j;
+                k_fg_setup<<<fb, kThreads, 0, stream>>>(N, st + 1, fg_part, vsq, 1, fg_v[j], dinv32, x32, r32,
                                                         rh32, p32, ph32[0]);
-                k_fg_take64<<<flat_grid(N), kBlock, 0, stream>>>(N, first_dir, fg_z[0], fg_z64);
+                k_fg_take64<<<flat_grid(N), kBlock, 0, stream>>>(N, first_dir, fg_z[j], fg_z64);
                 HIPCHK(hipGetLastError());
                 first_dir = nullptr;
                 ++steps;
@@ -1165,6 +1226,7 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
         if (!(st_host[0].outer_rnorm < before)) { status = 2; break; }
         (void)breakdown;
     }
+    CHK(join_learned());  // a solve that ended before step 1 still waits for the second stream
     info->status = st_host[0].outer_done ? 0 : (status ? status : 1);
     info->iters = total_inner;
     info->outer_iters = steps;
@@ -1440,28 +1502,27 @@ int windhip_solve(windhip_solver *s, const void *alpha, const void *div, const w
     inf.start_used = o.start;
     const bool mixed = o.precision == WINDHIP_MIXED || o.precision == WINDHIP_MIXED_LINE ||
                        o.precision == WINDHIP_MIXED_MG;
-    double *aug = nullptr;  // mixed: the learned correction as the first FGMRES direction
+    double *guess = nullptr, *mixed_guess = nullptr, rp = 0.0;
     if (o.start == WINDHIP_START_LEARNED) {
-        double rg = 0, rp = 0, *guess = nullptr;
-        CHK(s->learned_start(&rg, &rp, &guess));
-        inf.guess_res = rg;
-        inf.plain_res = rp;
+        CHK(s->learned_launch(&rp, &guess));
+        inf.start_used = WINDHIP_START_WARM;
         if (mixed) {
-            k_axpby<<<flat_grid(s->N), kBlock, 0, s->stream>>>(s->N, 1.0, guess, -1.0, s->x, guess);
-            HIPCHK(hipGetLastError());
-            aug = guess;
-            inf.start_used = WINDHIP_START_LEARNED;
+            mixed_guess = guess;  // joins the FGMRES after its first inner solve
         } else {
+            double rg = 0.0;
+            CHK(s->learned_join(&rg));
+            inf.guess_res = rg;
+            inf.plain_res = rp;
             const bool used = rg < rp;
             if (used) HIPCHK(hipMemcpyAsync(s->x, guess, s->N * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
             inf.start_used = used ? WINDHIP_START_LEARNED : WINDHIP_START_WARM;
+            if (s->verbose)
+                fprintf(stderr, "[windhip] learned start: residual %.3e vs plain start %.3e -> %s\n", rg, rp,
+                        used ? "network" : "plain");
         }
-        if (s->verbose)
-            fprintf(stderr, "[windhip] learned start: residual %.3e vs plain start %.3e -> %s\n", rg, rp,
-                    mixed ? "first FGMRES direction" : (inf.start_used == WINDHIP_START_LEARNED ? "network" : "plain"));
     }
     if (mixed) {
-        CHK(s->run_mixed(o, &inf, aug));
+        CHK(s->run_mixed(o, &inf, mixed_guess, rp));
         inf.nsweeps = s->nsweeps;
     } else {
         int status = 1, iters = 0;
