@@ -321,6 +321,10 @@ struct windhip_solver {
     int divergence(const void *u, const void *v, const void *w, const void *density, int adv_den,
                    int horz_only, int use_dqdt, void *div_out);
     int probe(const void *alpha, const void *density, int adv_den, double *max_leak);
+    int balance_uvw(const void *u, const void *v, void *w, const void *density, int adv_den, int use_dqdt);
+    int w_real(const void *u, const void *v, const void *w_grid, void *out);
+    int idealized_wgrid(const void *u, const void *v, const void *w_real_in, void *w_out);
+    double *div_scr = nullptr;  // balance_uvw: the horizontal divergence, FP64, caller mass layout
     void release();
 };
 
@@ -1308,6 +1312,83 @@ int windhip_solver::divergence(const void *u, const void *v, const void *w, cons
     return WINDHIP_OK;
 }
 
+// balance_uvw (wind.F90:123-191): the horizontal divergence of (u, v)
+// (calc_divergence with horz_only, :250-523; density-weighted whatever adv_den
+// is, since balance_uvw leaves advect_density at its default) into FP64
+// scratch, then calc_w (:193-234) writes w; adv_den selects calc_w's form.
+int windhip_solver::balance_uvw(const void *u, const void *v, void *w, const void *density, int adv_den,
+                                int use_dqdt)
+{
+    if (!div_scr) CHK(alloc(&div_scr, n_mass()));
+    void *du, *dv, *dw, *dr = nullptr;
+    CHK(view(u, n_u(), 2, true, &du));
+    CHK(view(v, n_v(), 3, true, &dv));
+    CHK(view(w, n_mass(), 4, true, &dw));
+    const int use_rho = density ? 1 : 0;
+    if (use_rho) CHK(view(density, n_mass(), 5, true, &dr));
+    const long tw = (long)d.nx * d.ny;
+    const Fld<double> dd = fld<double>(div_scr, 0);
+    const Fld<const double> ddc{div_scr, dd.i0, dd.k0, dd.j0, dd.ni, dd.nk};
+    auto run = [&](auto tag) {
+        using F = decltype(tag);
+        k_divergence<F, double><<<flat_grid(tw), kBlock, 0, stream>>>(d, metrics(), fld<const F>(du, 1),
+            fld<const F>(dv, 2), fld<const F>(dw, 0), fld<const F>(dr, 0), use_rho, 1, 1, use_dqdt, dd);
+        k_calc_w<F><<<flat_grid(tw), kBlock, 0, stream>>>(d, metrics(), ddc, fld<const F>(dr, 0), use_rho, adv_den,
+                                                          fld<F>(dw, 0));
+    };
+    if (desc.field_dtype == WINDHIP_F32) run(float{});
+    else run(double{});
+    HIPCHK(hipGetLastError());
+    CHK(copy_out(w, n_mass(), 4));
+    HIPCHK(hipStreamSynchronize(stream));
+    return WINDHIP_OK;
+}
+
+// calc_w_real (wind.F90:1175-1227); w_grid may be null (zero).
+int windhip_solver::w_real(const void *u, const void *v, const void *w_grid, void *out)
+{
+    void *du, *dv, *dg = nullptr, *dout;
+    CHK(view(u, n_u(), 2, true, &du));
+    CHK(view(v, n_v(), 3, true, &dv));
+    if (w_grid) CHK(view(w_grid, n_mass(), 4, true, &dg));
+    CHK(view(out, n_mass(), 1, desc.field_mem == WINDHIP_HOST, &dout));
+    const long tot = (long)d.nx * d.ny * d.nz;
+    auto run = [&](auto tag) {
+        using F = decltype(tag);
+        k_w_real<F><<<flat_grid(tot), kBlock, 0, stream>>>(d, metrics(), fld<const F>(du, 1), fld<const F>(dv, 2),
+                                                           fld<const F>(dg, 0), fld<F>(dout, 0));
+    };
+    if (desc.field_dtype == WINDHIP_F32) run(float{});
+    else run(double{});
+    HIPCHK(hipGetLastError());
+    CHK(copy_out(out, n_mass(), 1));
+    HIPCHK(hipStreamSynchronize(stream));
+    return WINDHIP_OK;
+}
+
+// calc_idealized_wgrid (wind.F90:940-998).
+int windhip_solver::idealized_wgrid(const void *u, const void *v, const void *w_real_in, void *w_out)
+{
+    void *du, *dv, *dwr, *dout;
+    CHK(view(u, n_u(), 2, true, &du));
+    CHK(view(v, n_v(), 3, true, &dv));
+    CHK(view(w_real_in, n_mass(), 4, true, &dwr));
+    CHK(view(w_out, n_mass(), 1, desc.field_mem == WINDHIP_HOST, &dout));
+    const long tw = (long)d.nx * d.ny;
+    auto run = [&](auto tag) {
+        using F = decltype(tag);
+        k_idealized_wgrid<F><<<flat_grid(tw), kBlock, 0, stream>>>(d, metrics(), fld<const F>(du, 1),
+                                                                   fld<const F>(dv, 2), fld<const F>(dwr, 0),
+                                                                   fld<F>(dout, 0));
+    };
+    if (desc.field_dtype == WINDHIP_F32) run(float{});
+    else run(double{});
+    HIPCHK(hipGetLastError());
+    CHK(copy_out(w_out, n_mass(), 1));
+    HIPCHK(hipStreamSynchronize(stream));
+    return WINDHIP_OK;
+}
+
 // calibrate_projection_operator (wind.F90:1010-1099) with the probe hooks of
 // wind_iterative.F90:1096-1248, on the device and on internal FP64 scratch
 // (u, v, w, the response T and the density in the natural (i,k,j) layout).
@@ -1585,6 +1666,28 @@ int windhip_divergence(windhip_solver *s, const void *u, const void *v, const vo
     if (!s || !u || !v || !w || !div_out) return fail(WINDHIP_ERR_ARG, "windhip_divergence: null argument");
     if (!s->metrics_set) return fail(WINDHIP_ERR_ARG, "windhip_divergence: metrics not set");
     return s->divergence(u, v, w, density, adv_den, horz_only, use_dqdt, div_out);
+}
+
+int windhip_balance_uvw(windhip_solver *s, const void *u, const void *v, void *w, const void *density, int adv_den,
+                        int use_dqdt)
+{
+    if (!s || !u || !v || !w) return fail(WINDHIP_ERR_ARG, "windhip_balance_uvw: null argument");
+    if (!s->metrics_set) return fail(WINDHIP_ERR_ARG, "windhip_balance_uvw: metrics not set");
+    return s->balance_uvw(u, v, w, density, adv_den, use_dqdt);
+}
+
+int windhip_calc_w_real(windhip_solver *s, const void *u, const void *v, const void *w_grid, void *w_real)
+{
+    if (!s || !u || !v || !w_real) return fail(WINDHIP_ERR_ARG, "windhip_calc_w_real: null argument");
+    if (!s->metrics_set) return fail(WINDHIP_ERR_ARG, "windhip_calc_w_real: metrics not set");
+    return s->w_real(u, v, w_grid, w_real);
+}
+
+int windhip_idealized_wgrid(windhip_solver *s, const void *u, const void *v, const void *w_real, void *w)
+{
+    if (!s || !u || !v || !w_real || !w) return fail(WINDHIP_ERR_ARG, "windhip_idealized_wgrid: null argument");
+    if (!s->metrics_set) return fail(WINDHIP_ERR_ARG, "windhip_idealized_wgrid: metrics not set");
+    return s->idealized_wgrid(u, v, w_real, w);
 }
 
 // lambda rows (0:nx+1, 0:nz+1, 0:ny+1) Fortran order <-> [k][j][i]

@@ -376,11 +376,12 @@ __global__ void k_correct_w(Dims d, Metrics m, const double *__restrict__ lam, F
 }
 
 // ------------------------------------------------------------- divergence D
-// calc_divergence (wind.F90:250-523) on one tile.
-template <typename F>
+// calc_divergence (wind.F90:250-523) on one tile (output type O, by default
+// the fields' type).
+template <typename F, typename O = F>
 __global__ void k_divergence(Dims d, Metrics m, Fld<const F> u, Fld<const F> v, Fld<const F> w,
                              Fld<const F> rho, int use_rho, int adv_den, int horz_only, int use_dqdt,
-                             Fld<F> out)
+                             Fld<O> out)
 {
     const long tot = (long)d.nx * d.ny;
     for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
@@ -435,8 +436,111 @@ __global__ void k_divergence(Dims d, Metrics m, Fld<const F> u, Fld<const F> v, 
                 dv = dv / m.jaco[n];
                 wm_prev = wm;
             }
-            out.at(i, k, j) = (F)dv;
+            out.at(i, k, j) = (O)dv;
         }
+    }
+}
+
+// ------------------------------------------------------- the caller's steps
+// These run in FP64; values the reference keeps in arrays of the caller's type
+// (the divergence, rho_i, w, the intermediate w of the restaggering) are
+// rounded to that type where it stores them.
+//
+// calc_w (wind.F90:193-234): w from the horizontal divergence by upward
+// integration of each column, with the interface density
+// rho_i(k) = (rho(k) dz(k+1) + rho(k+1) dz(k)) / (dz(k) + dz(k+1)), rho_i(top)
+// = rho(top), dz = advection_dz, when adv_den.
+template <typename F>
+__global__ void k_calc_w(Dims d, Metrics m, Fld<const double> div, Fld<const F> rho, int use_rho, int adv_den,
+                         Fld<F> w)
+{
+    const long tot = (long)d.nx * d.ny;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+        const int i = 1 + (int)(e % d.nx);
+        const int j = 1 + (int)(e / d.nx);
+        auto R = [&](int kk) -> double { return use_rho ? (double)rho.at(i, kk, j) : 1.0; };
+        auto rho_i = [&](int kk) -> double {
+            if (kk == d.nz) return R(kk);
+            const double dz0 = m.adv_dz[d.r(i, kk, j)], dz1 = m.adv_dz[d.r(i, kk + 1, j)];
+            return (double)(F)((R(kk) * dz1 + R(kk + 1) * dz0) / (dz0 + dz1));
+        };
+        double wprev = 0.0, riprev = 0.0;
+        for (int k = 1; k <= d.nz; ++k) {
+            const long n = d.r(i, k, j);
+            const double dv = (double)(F)div.at(i, k, j), dz = m.adv_dz[n], jw = m.jaco_w[n];
+            double wk;
+            if (adv_den) {
+                const double ri = rho_i(k);
+                if (k == 1) wk = 0.0 - (dv * dz) / (jw * ri);
+                else wk = ((wprev * riprev) * m.jaco_w[d.r(i, k - 1, j)] - dv * dz) / (jw * ri);
+                riprev = ri;
+            } else {
+                if (k == 1) wk = (0.0 - dv * dz) / jw;
+                else wk = (wprev * m.jaco_w[d.r(i, k - 1, j)] - dv * dz) / jw;
+            }
+            w.at(i, k, j) = (F)wk;
+            wprev = (double)(F)wk;  // the reference reads w(k-1) back from the array
+        }
+    }
+}
+
+// calc_w_real (wind.F90:1175-1227): the real vertical velocity on the mass
+// grid, (u dz/dx)_i,i+1 / 2 + (v dz/dy)_j,j+1 / 2 + (w_ext(k) + w_ext(k+1)) / 2
+// with w_ext(1) = 0 and w_ext(k+1) = w_grid(k) jaco_w(k); a null w_grid is 0.
+template <typename F>
+__device__ __forceinline__ double w_real_at(const Dims &d, const Metrics &m, const Fld<const F> &u,
+                                            const Fld<const F> &v, const F *wg_p, const Fld<const F> &wg, int i,
+                                            int k, int j)
+{
+    const long n = d.r(i, k, j);
+    const double uw = ((double)u.at(i, k, j) * m.dzdx_u[n] + (double)u.at(i + 1, k, j) * m.dzdx_u[d.r(i + 1, k, j)]) * 0.5;
+    const double vw = ((double)v.at(i, k, j) * m.dzdy_v[n] + (double)v.at(i, k, j + 1) * m.dzdy_v[d.r(i, k, j + 1)]) * 0.5;
+    double lo = 0.0, hi = 0.0;
+    if (wg_p) {
+        if (k > 1) lo = (double)wg.at(i, k - 1, j) * m.jaco_w[d.r(i, k - 1, j)];
+        hi = (double)wg.at(i, k, j) * m.jaco_w[n];
+    }
+    return (uw + vw) + (lo + hi) * 0.5;
+}
+
+template <typename F>
+__global__ void k_w_real(Dims d, Metrics m, Fld<const F> u, Fld<const F> v, Fld<const F> wg, Fld<F> out)
+{
+    const long tot = (long)d.nx * d.ny * d.nz;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+        const int i = 1 + (int)(e % d.nx);
+        const long t = e / d.nx;
+        const int j = 1 + (int)(t % d.ny);
+        const int k = 1 + (int)(t / d.ny);
+        out.at(i, k, j) = (F)w_real_at<F>(d, m, u, v, wg.p, wg, i, k, j);
+    }
+}
+
+// calc_idealized_wgrid (wind.F90:940-998): the grid-relative w that leaves a
+// prescribed w_real, t = w_real - calc_w_real(u, v, 0) on the mass grid,
+// restaggered to the level interfaces with advection_dz weights and divided
+// by jaco_w; the top level is 0.
+template <typename F>
+__global__ void k_idealized_wgrid(Dims d, Metrics m, Fld<const F> u, Fld<const F> v, Fld<const F> w_real,
+                                  Fld<F> w)
+{
+    const long tot = (long)d.nx * d.ny;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+        const int i = 1 + (int)(e % d.nx);
+        const int j = 1 + (int)(e / d.nx);
+        auto tm = [&](int k) -> double {
+            const double h = (double)(F)w_real_at<F>(d, m, u, v, nullptr, u, i, k, j);
+            return (double)(F)((double)w_real.at(i, k, j) - h);
+        };
+        double tk = tm(1);
+        for (int k = 1; k < d.nz; ++k) {
+            const double t1 = tm(k + 1);
+            const double a0 = m.adv_dz[d.r(i, k, j)], a1 = m.adv_dz[d.r(i, k + 1, j)];
+            const double st = (double)(F)((tk * a1 + t1 * a0) / (a0 + a1));
+            w.at(i, k, j) = (F)(st / m.jaco_w[d.r(i, k, j)]);
+            tk = t1;
+        }
+        w.at(i, d.nz, j) = (F)0.0;
     }
 }
 

@@ -21,7 +21,7 @@ module wind_iterative_hip
 
     public :: wind_tile_t, wind_solver_options_t
     public :: init_iter_winds, calc_iter_winds, finalize_iter_winds, probe_operator, last_solve_info
-    public :: calc_divergence, load_learned_guess
+    public :: calc_divergence, load_learned_guess, balance_uvw, calc_w_real, calc_idealized_wgrid
 
     integer, parameter :: MAX_NESTS = 4   ! wind_iterative.F90:139
 
@@ -162,6 +162,47 @@ contains
                                 merge(1_c_int, 0_c_int, use_dqdt), c_loc(div))
         call check(rc, "windhip_divergence")
     end subroutine calc_divergence
+
+    !> balance_uvw(domain, adv_den, update_in) (wind.F90:123-191): the tile's w
+    !! from the horizontal divergence of its u, v (calc_w, :193-234). The tile
+    !! holds the arrays update_in selects (data_3d or dqdt_3d).
+    subroutine balance_uvw(tile, adv_den)
+        type(wind_tile_t), intent(inout) :: tile
+        logical, intent(in) :: adv_den
+        integer(c_int) :: rc
+        type(c_ptr) :: rho
+        rho = c_null_ptr
+        if (associated(tile%density)) rho = c_loc(tile%density)
+        rc = windhip_balance_uvw(slots(tile%nest_indx)%handle, c_loc(tile%u), c_loc(tile%v), c_loc(tile%w), rho, &
+                                 merge(1_c_int, 0_c_int, adv_den), 1_c_int)
+        call check(rc, "windhip_balance_uvw")
+    end subroutine balance_uvw
+
+    !> calc_w_real (wind.F90:1175-1227) of the tile's u, v and a grid-relative
+    !! w_grid (absent: zero).
+    subroutine calc_w_real(tile, w_real, w_grid)
+        type(wind_tile_t), intent(in) :: tile
+        real, dimension(tile%ims:tile%ime, tile%kms:tile%kme, tile%jms:tile%jme), intent(out), target :: w_real
+        real, dimension(tile%ims:tile%ime, tile%kms:tile%kme, tile%jms:tile%jme), intent(in), target, &
+            optional :: w_grid
+        integer(c_int) :: rc
+        type(c_ptr) :: wg
+        wg = c_null_ptr
+        if (present(w_grid)) wg = c_loc(w_grid)
+        rc = windhip_calc_w_real(slots(tile%nest_indx)%handle, c_loc(tile%u), c_loc(tile%v), wg, c_loc(w_real))
+        call check(rc, "windhip_calc_w_real")
+    end subroutine calc_w_real
+
+    !> calc_idealized_wgrid(domain) (wind.F90:940-998): the tile's w from a
+    !! prescribed w_real and its u, v.
+    subroutine calc_idealized_wgrid(tile, w_real)
+        type(wind_tile_t), intent(inout) :: tile
+        real, dimension(tile%ims:tile%ime, tile%kms:tile%kme, tile%jms:tile%jme), intent(in), target :: w_real
+        integer(c_int) :: rc
+        rc = windhip_idealized_wgrid(slots(tile%nest_indx)%handle, c_loc(tile%u), c_loc(tile%v), c_loc(w_real), &
+                                     c_loc(tile%w))
+        call check(rc, "windhip_idealized_wgrid")
+    end subroutine calc_idealized_wgrid
 
     !> Load the trained parameters of the learned initial guess (the flat FP32
     !! array of windhip.h, windhip_net_param_count() values) for nest

@@ -46,7 +46,8 @@ module windhip_interface
               windhip_set_metrics, windhip_solve, windhip_apply_correction, windhip_calc_iter_winds, &
               windhip_divergence, windhip_get_lambda, windhip_set_lambda, windhip_probe_operator, &
               windhip_net_param_count, windhip_net_create, windhip_net_destroy, windhip_net_set_params, &
-              windhip_set_net, windhip_set_multigrid, windhip_error_message
+              windhip_set_net, windhip_set_multigrid, windhip_balance_uvw, windhip_calc_w_real, &
+              windhip_idealized_wgrid, windhip_error_message
 
     interface
         subroutine windhip_default_opts(opts) bind(C, name="windhip_default_opts")
@@ -106,6 +107,23 @@ module windhip_interface
             import :: c_int, c_ptr
             type(c_ptr), value :: handle, u, v, w, density, div_out
             integer(c_int), value :: adv_den, horz_only, use_dqdt
+        end function
+
+        integer(c_int) function windhip_balance_uvw(handle, u, v, w, density, adv_den, use_dqdt) &
+                bind(C, name="windhip_balance_uvw")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle, u, v, w, density
+            integer(c_int), value :: adv_den, use_dqdt
+        end function
+
+        integer(c_int) function windhip_calc_w_real(handle, u, v, w_grid, w_real) bind(C, name="windhip_calc_w_real")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle, u, v, w_grid, w_real
+        end function
+
+        integer(c_int) function windhip_idealized_wgrid(handle, u, v, w_real, w) bind(C, name="windhip_idealized_wgrid")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle, u, v, w_real, w
         end function
 
         integer(c_int) function windhip_get_lambda(handle, lambda_rows) bind(C, name="windhip_get_lambda")

@@ -162,6 +162,50 @@ class WindSolver:
                                               int(bool(horz_only)), int(bool(use_dqdt)), _ptr(out)))
         return out
 
+    def _out(self, out, name):
+        """A mass-grid output: given (device fields: required), or a new host array."""
+        if self.device_fields:
+            if out is None:
+                raise ValueError(f"{name}: pass `out` (a device array) when fields live on the device")
+            return self._arr(out, self.mass_shape, "out")
+        if out is None:
+            return np.zeros(self.mass_shape, dtype=self.dtype)
+        if not (isinstance(out, np.ndarray) and out.dtype == self.dtype and out.flags.c_contiguous
+                and out.shape == self.mass_shape):
+            raise ValueError(f"{name}: need a C-contiguous {self.dtype} array of shape {self.mass_shape}")
+        return out
+
+    def balance_uvw(self, u, v, w, density=None, adv_den=True, use_dqdt=False):
+        """balance_uvw(domain, adv_den, update_in) (wind.F90:123-191): w (updated in
+        place) from the horizontal divergence of u, v by calc_w (:193-234)."""
+        uu = self._arr(u, self.u_shape, "u")
+        vv = self._arr(v, self.v_shape, "v")
+        ww = self._out(w, "balance_uvw")
+        rho = self._arr(density, self.mass_shape, "density")
+        nat.check(self.lib.windhip_balance_uvw(self.h, _ptr(uu), _ptr(vv), _ptr(ww), _ptr(rho),
+                                               int(bool(adv_den)), int(bool(use_dqdt))))
+        return w if self.device_fields else ww
+
+    def calc_w_real(self, u, v, w_grid=None, out=None):
+        """calc_w_real (wind.F90:1175-1227): the real vertical velocity on the mass
+        grid; w_grid None is zero."""
+        uu = self._arr(u, self.u_shape, "u")
+        vv = self._arr(v, self.v_shape, "v")
+        wg = self._arr(w_grid, self.mass_shape, "w_grid")
+        o = self._out(out, "calc_w_real")
+        nat.check(self.lib.windhip_calc_w_real(self.h, _ptr(uu), _ptr(vv), _ptr(wg), _ptr(o)))
+        return out if self.device_fields else o
+
+    def idealized_wgrid(self, u, v, w_real, out=None):
+        """calc_idealized_wgrid (wind.F90:940-998): the grid-relative w that
+        reproduces w_real for the given u, v."""
+        uu = self._arr(u, self.u_shape, "u")
+        vv = self._arr(v, self.v_shape, "v")
+        wr = self._arr(w_real, self.mass_shape, "w_real")
+        o = self._out(out, "idealized_wgrid")
+        nat.check(self.lib.windhip_idealized_wgrid(self.h, _ptr(uu), _ptr(vv), _ptr(wr), _ptr(o)))
+        return out if self.device_fields else o
+
     def get_lambda(self) -> np.ndarray:
         out = np.zeros(self.rows_shape)
         nat.check(self.lib.windhip_get_lambda(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))

@@ -147,6 +147,24 @@ int windhip_divergence(windhip_solver *s, const void *u, const void *v, const vo
                        const void *density, int adv_den, int horz_only, int use_dqdt,
                        void *div_out);
 
+/* The caller's steps around the solve in update_winds (wind.F90:790-860), on
+ * the same fields and metrics (every field in the caller's layout, dtype and
+ * memory as for windhip_calc_iter_winds; domain cells only):
+ * windhip_balance_uvw = balance_uvw (wind.F90:123-191): w from the horizontal
+ *   divergence of (u, v) by calc_w's upward integration (:193-234); w is
+ *   written. The caller passes the data or the dqdt arrays, as update_in
+ *   selects in the reference (use_dqdt mirrors that flag; the divergence is
+ *   the density-weighted form of calc_divergence either way, :159/:164).
+ *   adv_den selects calc_w's form; density may be null (rho = 1).
+ * windhip_calc_w_real = calc_w_real (:1175-1227): the real vertical velocity
+ *   on the mass grid from u, v and the grid-relative w_grid (null: zero).
+ * windhip_idealized_wgrid = calc_idealized_wgrid (:940-998): the grid-relative
+ *   w that reproduces a prescribed w_real for the given u, v. */
+int windhip_balance_uvw(windhip_solver *s, const void *u, const void *v, void *w, const void *density, int adv_den,
+                        int use_dqdt);
+int windhip_calc_w_real(windhip_solver *s, const void *u, const void *v, const void *w_grid, void *w_real);
+int windhip_idealized_wgrid(windhip_solver *s, const void *u, const void *v, const void *w_real, void *w);
+
 /* Warm start / learned guess: lambda over the solver rows, host FP64. */
 int windhip_get_lambda(windhip_solver *s, double *lambda_rows);
 int windhip_set_lambda(windhip_solver *s, const double *lambda_rows);

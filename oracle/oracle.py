@@ -64,6 +64,9 @@ def _load():
         "orc_get_planes": (None, [v, _dp, _dp, _dp]),
         "orc_correct": (None, [v, _dp, _dp, _dp, _dp, _dp, _dp, i]),
         "orc_divergence": (None, [v, _dp, _dp, _dp, _dp, _dp, _dp, _dp, i, i, i, _dp]),
+        "orc_balance_uvw": (None, [v, _dp, _dp, _dp, _dp, i, i]),
+        "orc_w_real": (None, [v, _dp, _dp, _dp, _dp]),
+        "orc_idealized_wgrid": (None, [v, _dp, _dp, _dp, _dp]),
         "orc_probe": (d, [v, _dp, _dp, _dp, _dp, _dp, _dp, i]),
         "orc_canon_reduce": (d, [_dp, i, i]),
         "orc_dot": (d, [v, _dp, _dp]),
@@ -238,6 +241,27 @@ class Oracle:
         self.lib.orc_divergence(self.h, _p(u), _p(v), _p(w), _p(rho), _p(a), _p(b), _p(c),
                                 int(adv_den), int(horz_only), int(use_dqdt), _p(div))
         return div
+
+    def balance_uvw(self, u, v, w, rho=None, adv_den=True, use_dqdt=True):
+        """balance_uvw + calc_w (wind.F90:123-234): a new w (mass layout)."""
+        u, v, rho = _f64(u), _f64(v), _f64(rho)
+        out = np.array(_f64(w), dtype=np.float64, copy=True)
+        self.lib.orc_balance_uvw(self.h, _p(u), _p(v), _p(out), _p(rho), int(adv_den), int(use_dqdt))
+        return out
+
+    def w_real(self, u, v, w_grid=None):
+        """calc_w_real (wind.F90:1175-1227)."""
+        u, v, wg = _f64(u), _f64(v), _f64(w_grid)
+        out = np.zeros((self.ny, self.nz, self.nx))
+        self.lib.orc_w_real(self.h, _p(u), _p(v), _p(wg), _p(out))
+        return out
+
+    def idealized_wgrid(self, u, v, w_real):
+        """calc_idealized_wgrid (wind.F90:940-998)."""
+        u, v, wr = _f64(u), _f64(v), _f64(w_real)
+        out = np.zeros((self.ny, self.nz, self.nx))
+        self.lib.orc_idealized_wgrid(self.h, _p(u), _p(v), _p(wr), _p(out))
+        return out
 
     def probe(self, rho=None, adv_den=False):
         r = _f64(rho)

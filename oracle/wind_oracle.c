@@ -864,6 +864,90 @@ double orc_probe(orc_t *o, const double *rho, const double *mf_mx_u, const doubl
     return max_leak;
 }
 
+/* ------------------------------------------------------------------ the caller's steps
+ * balance_uvw (wind.F90:123-191): the horizontal divergence (calc_divergence
+ * with horz_only, :250-523) of (u, v), density-weighted whatever adv_den is
+ * (balance_uvw leaves advect_density at its default, true, :159/:164), then
+ * calc_w (:193-234): w from the
+ * bottom up, w(k) = (w(k-1) rho_i(k-1) jaco_w(k-1) - div(k) dz(k)) /
+ * (jaco_w(k) rho_i(k)) with rho_i the dz-weighted interface density (rho at
+ * the top), or without rho_i when not adv_den; dz = advection_dz. */
+void orc_balance_uvw(const orc_t *o, const double *u, const double *v, double *w, const double *rho_in,
+                     int adv_den, int use_dqdt)
+{
+    int nx = o->nx, ny = o->ny, nz = o->nz;
+    long n3 = (long)nx * nz * ny;
+    double *div = zalloc(n3);
+    orc_divergence(o, u, v, w, rho_in, NULL, NULL, NULL, 1, 1, use_dqdt, div);
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            double wprev = 0.0, riprev = 0.0;
+            for (int k = 1; k <= nz; ++k) {
+                double rk = rho_in ? M3(o, rho_in, i, k, j) : 1.0;
+                double dz = M3(o, o->adv_dz, i, k, j), jw = M3(o, o->jaco_w, i, k, j), dv = M3(o, div, i, k, j);
+                double wk;
+                if (adv_den) {
+                    double ri = rk;
+                    if (k < nz) {
+                        double rk1 = rho_in ? M3(o, rho_in, i, k + 1, j) : 1.0;
+                        double dz1 = M3(o, o->adv_dz, i, k + 1, j);
+                        ri = (rk * dz1 + rk1 * dz) / (dz + dz1);
+                    }
+                    if (k == 1) wk = 0.0 - (dv * dz) / (jw * ri);
+                    else wk = ((wprev * riprev) * M3(o, o->jaco_w, i, k - 1, j) - dv * dz) / (jw * ri);
+                    riprev = ri;
+                } else {
+                    if (k == 1) wk = (0.0 - dv * dz) / jw;
+                    else wk = (wprev * M3(o, o->jaco_w, i, k - 1, j) - dv * dz) / jw;
+                }
+                M3(o, w, i, k, j) = wk;
+                wprev = wk;
+            }
+        }
+    free(div);
+}
+
+/* calc_w_real (wind.F90:1175-1227): (u dzdx_u at i and i+1)/2 + (v dzdy_v at
+ * j and j+1)/2 + (w_ext(k) + w_ext(k+1))/2, w_ext(1) = 0, w_ext(k+1) =
+ * w_grid(k) jaco_w(k); a null w_grid is zero. */
+static double w_real_at(const orc_t *o, const double *u, const double *v, const double *wg, int i, int k, int j)
+{
+    double uw = (U3(o, u, i, k, j) * U3(o, o->dzdx_u, i, k, j) + U3(o, u, i + 1, k, j) * U3(o, o->dzdx_u, i + 1, k, j)) * 0.5;
+    double vw = (V3(o, v, i, k, j) * V3(o, o->dzdy_v, i, k, j) + V3(o, v, i, k, j + 1) * V3(o, o->dzdy_v, i, k, j + 1)) * 0.5;
+    double lo = 0.0, hi = 0.0;
+    if (wg) {
+        if (k > 1) lo = M3(o, wg, i, k - 1, j) * M3(o, o->jaco_w, i, k - 1, j);
+        hi = M3(o, wg, i, k, j) * M3(o, o->jaco_w, i, k, j);
+    }
+    return (uw + vw) + (lo + hi) * 0.5;
+}
+
+void orc_w_real(const orc_t *o, const double *u, const double *v, const double *w_grid, double *out)
+{
+    for (int j = 1; j <= o->ny; ++j)
+        for (int k = 1; k <= o->nz; ++k)
+            for (int i = 1; i <= o->nx; ++i) M3(o, out, i, k, j) = w_real_at(o, u, v, w_grid, i, k, j);
+}
+
+/* calc_idealized_wgrid (wind.F90:940-998): t = w_real - calc_w_real(u, v, 0),
+ * restaggered to the level interfaces (advection_dz weights), divided by
+ * jaco_w; the top level is 0. */
+void orc_idealized_wgrid(const orc_t *o, const double *u, const double *v, const double *w_real, double *w)
+{
+    int nz = o->nz;
+    for (int j = 1; j <= o->ny; ++j)
+        for (int i = 1; i <= o->nx; ++i) {
+            for (int k = 1; k < nz; ++k) {
+                double t0 = M3(o, w_real, i, k, j) - w_real_at(o, u, v, NULL, i, k, j);
+                double t1 = M3(o, w_real, i, k + 1, j) - w_real_at(o, u, v, NULL, i, k + 1, j);
+                double a0 = M3(o, o->adv_dz, i, k, j), a1 = M3(o, o->adv_dz, i, k + 1, j);
+                double st = (t0 * a1 + t1 * a0) / (a0 + a1);
+                M3(o, w, i, k, j) = st / M3(o, o->jaco_w, i, k, j);
+            }
+            M3(o, w, i, nz, j) = 0.0;
+        }
+}
+
 int orc_num_threads(void)
 {
 #ifdef _OPENMP

@@ -17,7 +17,8 @@ program test_drop_in
     real, allocatable, target :: dzdx(:,:,:), dzdy(:,:,:), jac(:,:,:), jac_u(:,:,:), jac_v(:,:,:), jac_w(:,:,:)
     real, allocatable, target :: dzdx_u(:,:,:), dzdy_v(:,:,:), adz(:,:,:), terrain(:,:), rho(:,:,:)
     real, allocatable, target :: u(:,:,:), v(:,:,:), w(:,:,:), alpha(:,:,:), div0(:,:,:), div1(:,:,:)
-    real :: dz(nz), lx, ly, d0, d1
+    real, allocatable, target :: div_h(:,:,:), wr(:,:,:)
+    real :: dz(nz), lx, ly, d0, d1, db
     integer :: i, j, k
 
     call auto_dz_icon(nz, 20.0, 8000.0, 0.7, dz)
@@ -50,7 +51,27 @@ program test_drop_in
     call calc_iter_winds(tile, alpha, div0, .false.)
     info = last_solve_info()
     call calc_divergence(tile, div1, .false., .false., .true.)
+
+    ! balance_uvw: w from the horizontal divergence of (u, v); the full
+    ! density-weighted divergence then vanishes to FP32 rounding
+    allocate(div_h(nx,nz,ny), wr(nx,nz,ny))
+    do k = 1, nz
+        rho(:,k,:) = 1.2 - 0.01 * real(k)
+    end do
+    u = u * 1.25
+    call balance_uvw(tile, .true.)
+    call calc_divergence(tile, div_h, .true., .true., .false.)
+    call calc_divergence(tile, div1, .true., .false., .false.)
+    db = maxval(abs(div1)) / maxval(abs(div_h))
+    ! calc_w_real then calc_idealized_wgrid: the top level of w is 0
+    call calc_w_real(tile, wr, w)
+    call calc_idealized_wgrid(tile, wr)
     call finalize_iter_winds()
+    write(*,'(A,ES10.3,A,ES10.3)') 'balanced: max|div| / max|div_h| ', db, ' top w ', maxval(abs(w(:,nz,:)))
+    if (db /= db .or. db > 1.0e-4 .or. maxval(abs(w(:,nz,:))) /= 0.0 .or. any(wr /= wr)) then
+        write(*,*) 'FAIL'
+        error stop 1
+    end if
 
     d0 = maxval(abs(div0)); d1 = maxval(abs(div1))
     write(*,'(A,ES12.4,A,ES12.4,A,I0,A,I0,A,ES10.3)') 'max|div| before ', d0, ' after ', d1, &
