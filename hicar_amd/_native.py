@@ -36,7 +36,7 @@ EXPORTS = (
     "windhip_net_param_count", "windhip_net_create", "windhip_net_destroy", "windhip_net_last_error",
     "windhip_net_set_params", "windhip_net_get_params", "windhip_net_forward", "windhip_net_loss_grad",
     "windhip_net_adam_step", "windhip_net_time_forward", "windhip_set_net", "windhip_learned_input", "windhip_set_guess_smoothing",
-    "windhip_set_multigrid", "windhip_mg_level",
+    "windhip_set_multigrid", "windhip_mg_level", "windhip_mg_smooth",
     "windhip_balance_uvw", "windhip_calc_w_real", "windhip_idealized_wgrid",
 )
 
@@ -146,6 +146,7 @@ def load():
         "windhip_set_guess_smoothing": (i, [vp, i]),
         "windhip_set_multigrid": (i, [vp, i, i, i, i, i, d]),
         "windhip_mg_level": (i, [vp, i, C.POINTER(C.c_int), fp]),
+        "windhip_mg_smooth": (i, [vp, i, i, fp, fp, fp]),
         "windhip_balance_uvw": (i, [vp, vp, vp, vp, vp, i, i]),
         "windhip_calc_w_real": (i, [vp, vp, vp, vp, vp]),
         "windhip_idealized_wgrid": (i, [vp, vp, vp, vp, vp]),

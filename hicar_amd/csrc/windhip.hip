@@ -140,6 +140,14 @@ struct windhip_solver {
     int mg_cpre = 2, mg_cpost = 2;  // the same on the coarse levels
     int mg_depth = 0;               // coarse levels used (0: down to the last level with >= 4 columns each way)
     float mg_omega = 0.9f;          // damping of the line smoother
+    // Coarse levels of at most this many columns (64 x 64 interior columns and
+    // their ring) smooth in one launch, a wavefront per column. Larger levels
+    // keep the residual + line-solve kernels: a wavefront reads 64 rows of one
+    // column per load instruction, a cache line per lane, which costs more
+    // than the two serial marches once a level has enough columns to hide
+    // their latency (512 x 512 x 40: 66 x 66 columns 4 us faster per step,
+    // 130 x 130 columns 17 us slower).
+    long mg_wave_cols = 4608;
     template <typename T>
     OpA<T> level_op(const MgLevel &L) const;
     bool metrics_set = false, structure_uploaded = false, probed = false;
@@ -315,6 +323,7 @@ struct windhip_solver {
     int enqueue_iterations_line(int count, bool multigrid);
     int build_mg();
     float *mg_coarse(size_t l);
+    void mg_smooth(size_t l, bool wave, const float *y, float *out);
     int alloc_f32_krylov();
     int alloc_fgmres(int m);
     int correction(void *u, void *v, void *w, const void *density, int adv_den);
@@ -729,6 +738,41 @@ int windhip_solver::enqueue_iterations_line(int count, bool multigrid)
     });
 }
 
+// One damped line-relaxation step on coarse level l, out = y + w L^-1 (b - A y)
+// with b = mg[l].b (y null: out = w L^-1 b): in one launch, a wavefront per
+// column (k_mg_smooth_w), when `wave`; else a residual into mg[l].r and a
+// line solve.
+void windhip_solver::mg_smooth(size_t l, bool wave, const float *y, float *out)
+{
+    MgLevel &L = mg[l];
+    const KState *sp = st + 1;
+    const Geo &gl = L.gg;
+    if (wave && gl.mz <= 64) {
+        timed(PK_MG_COARSE, [&] {
+            k_mg_smooth_w<OpA<float>><<<(int)((L.g.P + kWaveCols - 1) / kWaveCols), 64 * kWaveCols, 0, stream>>>(
+                level_op<float>(L), gl, sp, L.b, y, L.lo, L.cp, L.ip, out, mg_omega);
+        });
+        return;
+    }
+    const float *in = L.b;
+    if (y) {
+        timed(PK_MG_COARSE, [&] {
+            k_resid_c<OpA<float>><<<chunk_blocks(gl), kColThreads, 0, stream>>>(level_op<float>(L), gl, sp, y, L.b,
+                                                                               L.r);
+        });
+        in = L.r;
+    }
+    const int cb = col_blocks(gl);
+    timed(PK_MG_COARSE, [&] {
+        if (gl.mz <= 48)
+            k_line_solve_r<48><<<cb, kColThreads, 0, stream>>>(gl, sp, in, L.lo, L.cp, L.ip, y, out, mg_omega);
+        else if (gl.mz <= 64)
+            k_line_solve_r<64><<<cb, kColThreads, 0, stream>>>(gl, sp, in, L.lo, L.cp, L.ip, y, out, mg_omega);
+        else
+            k_mg_line<<<mg_blocks(L.g.P), kMgThreads, 0, stream>>>(L.g, sp, L.lo, L.cp, L.ip, in, y, out, mg_omega);
+    });
+}
+
 // Enqueue the V-cycle below the fine level from coarse level l (rhs in
 // mg[l].b); returns the buffer that holds its correction.
 float *windhip_solver::mg_coarse(size_t l)
@@ -739,6 +783,7 @@ float *windhip_solver::mg_coarse(size_t l)
     const Geo &gl = L.gg;
     const int cb = col_blocks(gl), kb = chunk_blocks(gl);
     const OpA<float> op = level_op<float>(L);
+    const bool wave = L.g.P <= mg_wave_cols;
     auto line = [&](const float *in, const float *y, float *out, float w) {
         timed(PK_MG_COARSE, [&] {
             if (gl.mz <= 48)
@@ -768,18 +813,16 @@ float *windhip_solver::mg_coarse(size_t l)
         return L.x;
     }
     float *cur = L.x, *alt = L.t;
-    line(L.b, nullptr, cur, mg_omega);
+    mg_smooth(l, wave, nullptr, cur);
     if (l + 1 == used) {  // the coarsest level in use: smoothing only
         for (int q = 1; q < mg_cpre + mg_cpost; ++q) {
-            resid(cur);
-            line(L.r, cur, alt, mg_omega);
+            mg_smooth(l, wave, cur, alt);
             std::swap(cur, alt);
         }
         return cur;
     }
     for (int q = 1; q < mg_cpre; ++q) {
-        resid(cur);
-        line(L.r, cur, alt, mg_omega);
+        mg_smooth(l, wave, cur, alt);
         std::swap(cur, alt);
     }
     resid(cur);
@@ -790,8 +833,7 @@ float *windhip_solver::mg_coarse(size_t l)
     float *e = mg_coarse(l + 1);
     timed(PK_MG_COARSE, [&] { k_mg_prolong<<<nr, kMgThreads, 0, stream>>>(L.g, C.g, sp, e, cur); });
     for (int q = 0; q < mg_cpost; ++q) {
-        resid(cur);
-        line(L.r, cur, alt, mg_omega);
+        mg_smooth(l, wave, cur, alt);
         std::swap(cur, alt);
     }
     return cur;
@@ -1872,6 +1914,26 @@ int windhip_mg_level(windhip_solver *s, int level, int *dims_out, float *coef_ho
     const float *arr[6] = {L.B32, L.C32, L.H32, L.I32, L.L32, L.M32};
     for (int q = 0; q < 6; ++q)
         HIPCHK(hipMemcpy(coef_host + (long)q * L.g.N, arr[q], sizeof(float) * L.g.N, hipMemcpyDeviceToHost));
+    return WINDHIP_OK;
+}
+
+int windhip_mg_smooth(windhip_solver *s, int level, int fused, const float *b, const float *y, float *out)
+{
+    if (!s || !b || !out) return fail(WINDHIP_ERR_ARG, "windhip_mg_smooth: null argument");
+    if (level < 1 || level > (int)s->mg.size())
+        return fail(WINDHIP_ERR_ARG, "windhip_mg_smooth: level %d outside 1..%d (run a MIXED_MG solve first)", level,
+                    (int)s->mg.size());
+    auto &L = s->mg[level - 1];
+    const size_t bytes = sizeof(float) * L.g.N;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipMemcpy(L.b, b, bytes, hipMemcpyHostToDevice));
+    if (y) HIPCHK(hipMemcpy(L.x, y, bytes, hipMemcpyHostToDevice));
+    windhip::k_state_reset<<<1, 1, 0, s->stream>>>(s->st + 1, 1, 0.0, 0.0, 0.0);  // a live state: done = 0
+    HIPCHK(hipGetLastError());
+    s->mg_smooth(level - 1, fused != 0, y ? L.x : nullptr, L.t);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipMemcpy(out, L.t, bytes, hipMemcpyDeviceToHost));
     return WINDHIP_OK;
 }
 

@@ -147,6 +147,78 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_line(MgGeo g, const KState *s
     }
 }
 
+// One damped line-relaxation step of a coarse level in one launch,
+// out = y + w L^-1 (b - A y) (y null: out = w L^-1 b), for columns of at most
+// 64 rows: one wavefront per column, lane k holding row k. On the small levels
+// the two-kernel step (a residual march, then a Thomas march per column) is
+// bound by the latency of its serial per-thread marches, not by bandwidth.
+// Here every load is independent (one memory round trip): the residual row is
+// evaluated from the lane's own five columns, with the k-1 and k+1 values
+// taken from the neighbouring lanes, and both Thomas recurrences,
+// d(k) = ip(k) (r(k) - lo(k) d(k-1)) and e(k) = d(k) - cp(k) e(k+1), are
+// inclusive scans of affine maps z -> a z + c over the lanes (6 shuffle steps
+// each way) on the factors of k_line_factor. y must be a level vector with
+// mx + 1 guard elements on both sides (the neighbour loads of the ring
+// columns fall on them); out must alias neither b nor y.
+constexpr int kWaveCols = 4;  // columns (wavefronts) per block
+template <class Op>
+__global__ __launch_bounds__(64 * kWaveCols) void k_mg_smooth_w(Op op, Geo g, const KState *st,
+                                                               const float *__restrict__ b,
+                                                               const float *__restrict__ y,
+                                                               const float *__restrict__ lo,
+                                                               const float *__restrict__ cp,
+                                                               const float *__restrict__ ip,
+                                                               float *__restrict__ out, float w)
+{
+    if (st->done) return;
+    const int lane = threadIdx.x & 63;
+    const long c = (long)blockIdx.x * kWaveCols + (threadIdx.x >> 6);
+    if (c >= g.P) return;  // uniform per wavefront
+    const int mz = g.mz;
+    const bool act = lane < mz;
+    const int k = act ? lane : mz - 1;
+    const long n = (long)k * g.P + c;
+    const int i = (int)(c % g.mx), j = (int)(c / g.mx);
+    const bool lat = i == 0 || i == g.mx - 1 || j == 0 || j == g.my - 1;
+    float r = b[n];
+    float yc = 0.f;
+    if (y) {
+        Win<float> x;
+        yc = y[n];
+        const float yw = y[n - 1], ye = y[n + 1], ys = y[n - g.mx], yn = y[n + g.mx];
+        x.c1 = yc; x.w1 = yw; x.e1 = ye; x.s1 = ys; x.n1 = yn;
+        x.c0 = __shfl_up(yc, 1, 64); x.w0 = __shfl_up(yw, 1, 64); x.e0 = __shfl_up(ye, 1, 64);
+        x.s0 = __shfl_up(ys, 1, 64); x.n0 = __shfl_up(yn, 1, 64);
+        x.c2 = __shfl_down(yc, 1, 64); x.w2 = __shfl_down(yw, 1, 64); x.e2 = __shfl_down(ye, 1, 64);
+        x.s2 = __shfl_down(ys, 1, 64); x.n2 = __shfl_down(yn, 1, 64);
+        const typename Op::Cf f = op.load(n);
+        const typename Op::Bc bc = op.bc(c);
+        r = r - op.row(k, mz, lat, x, f, bc);
+    }
+    // forward: d(k) = a(k) d(k-1) + c(k), a = -ip lo, c = ip r, d(-1) = 0
+    const float ipv = ip[n];
+    float a = act ? -(ipv * lo[n]) : 0.f, s = act ? ipv * r : 0.f;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const float pa = __shfl_up(a, off, 64), ps = __shfl_up(s, off, 64);
+        if (lane >= off) {
+            s = a * ps + s;
+            a = a * pa;
+        }
+    }
+    // backward: e(k) = -cp(k) e(k+1) + d(k), e(mz) = 0 (lanes >= mz are zero maps)
+    float a2 = act ? -cp[n] : 0.f, s2 = act ? s : 0.f;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const float pa = __shfl_down(a2, off, 64), ps = __shfl_down(s2, off, 64);
+        if (lane + off < 64) {
+            s2 = a2 * ps + s2;
+            a2 = a2 * pa;
+        }
+    }
+    if (act) out[n] = yc + w * s2;
+}
+
 // b_c = R r_f, one coarse row per thread: every interior row is the mean of
 // the rows of its aggregate; lateral rows get zero.
 __global__ __launch_bounds__(kMgThreads) void k_mg_restrict(MgGeo f, MgGeo c, const KState *st,

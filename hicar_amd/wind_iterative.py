@@ -231,6 +231,22 @@ class WindSolver:
         nat.check(self.lib.windhip_mg_level(self.h, int(level), dims, out.ctypes.data_as(C.POINTER(C.c_float))))
         return (nx, ny, mz), out
 
+    def mg_smooth(self, level: int, fused: bool, b, y=None):
+        """One damped line-smoothing step on coarse level `level` of the last MIXED_MG
+        solve: y + omega L^-1 (b - A y), FP32 [mz, ny+2, nx+2] (no counterpart in
+        the reference)."""
+        dims = (C.c_int * 3)()
+        nat.check(self.lib.windhip_mg_level(self.h, int(level), dims, None))
+        shape = (dims[2], dims[1] + 2, dims[0] + 2)
+        fpp = C.POINTER(C.c_float)
+        bb = np.ascontiguousarray(b, dtype=np.float32).reshape(shape)
+        yy = None if y is None else np.ascontiguousarray(y, dtype=np.float32).reshape(shape)
+        out = np.zeros(shape, dtype=np.float32)
+        nat.check(self.lib.windhip_mg_smooth(self.h, int(level), int(bool(fused)), bb.ctypes.data_as(fpp),
+                                             None if yy is None else yy.ctypes.data_as(fpp),
+                                             out.ctypes.data_as(fpp)))
+        return out
+
     def probe_operator(self, alpha, density=None, adv_den=True) -> float:
         """calibrate_projection_operator (wind.F90:1010-1099): A = 2 D(G(.))."""
         a = self._arr(alpha, self.mass_shape, "alpha")

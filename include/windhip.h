@@ -242,6 +242,14 @@ int windhip_set_multigrid(windhip_solver *s, int pre, int post, int coarse_pre, 
  * level's dx from its averaged metrics), rows [k][j][i] of (nx+2) x (ny+2)
  * columns. Inspection only; no counterpart in the reference. */
 int windhip_mg_level(windhip_solver *s, int level, int *dims_out, float *coef_host);
+/* One damped line-smoothing step of the V-cycle on coarse level `level`, out =
+ * y + omega L^-1 (b - A y) (y null: out = omega L^-1 b), L the vertical
+ * tridiagonal part of the level's operator; host FP32 arrays of the level's N
+ * rows (layout of windhip_mg_level). fused = 1: the one-launch kernel the
+ * V-cycle uses on small levels (a wavefront per column); 0: the residual and
+ * line-solve kernels of the larger levels. Overwrites the level's work
+ * vectors. Inspection only; no counterpart in the reference. */
+int windhip_mg_smooth(windhip_solver *s, int level, int fused, const float *b, const float *y, float *out);
 /* The network input for the solver's current rhs (that of its last solve),
  * metrics and alpha, written to in_dev (1 x dims[0] x dims[1] x dims[2] x 8);
  * *scale turns the network output into lambda. dims_out receives the padded

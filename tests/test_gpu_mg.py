@@ -97,3 +97,36 @@ def test_mixed_mg_on_the_probed_operator():
     info = s.solve(c.alpha, div, precision=nat.MIXED_MG, tol_rel=1e-8, start=1, max_iters=20000)
     assert info["status"] == 0 and info["true_rel_res"] <= 1e-8
     assert _rel(s.get_lambda(), lam) <= 1e-6
+
+
+@pytest.mark.parametrize("name", ["fractal_alpha", "hill"])
+def test_fused_coarse_smoothing_matches_the_two_kernel_step(name):
+    """The one-launch smoothing step of the small levels (a wavefront per
+    column, the Thomas recurrences as lane scans) against the residual + line
+    solve kernels, on every coarse level, from a zero and a random iterate; the
+    lateral identity rows give y + omega (b - y) exactly."""
+    from hicar_amd import _native as nat
+    c, o, div, s = _setup(name)
+    s.solve(c.alpha, div, precision=nat.MIXED_MG, tol_rel=1e-6, start=1, max_iters=2000)
+    rng = np.random.default_rng(21)
+    level = 1
+    while True:
+        try:
+            (nx, ny, mz), _ = s.mg_level(level)
+        except RuntimeError:
+            break
+        shape = (mz, ny + 2, nx + 2)
+        b = rng.standard_normal(shape).astype(np.float32)
+        y = rng.standard_normal(shape).astype(np.float32)
+        for yy in (None, y):
+            fused = s.mg_smooth(level, True, b, yy)
+            split = s.mg_smooth(level, False, b, yy)
+            assert np.all(np.isfinite(fused))
+            assert _rel(fused, split) <= 1e-5, (level, yy is None)
+            ring = np.zeros(shape, dtype=bool)
+            ring[:, 0, :] = ring[:, -1, :] = ring[:, :, 0] = ring[:, :, -1] = True
+            y0 = np.zeros(shape, np.float32) if yy is None else yy
+            want = y0 + np.float32(0.9) * (b - y0)
+            np.testing.assert_allclose(fused[ring], want[ring], rtol=1e-6, atol=1e-6)
+        level += 1
+    assert level > 2
