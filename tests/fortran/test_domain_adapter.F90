@@ -1,0 +1,168 @@
+!> The `wind_iterative` adapter (hicar_amd/fortran/wind_iterative_domain.F90)
+!! driven as HICAR drives the reference module, through domain_t and options_t
+!! (the stub modules of stub_hicar.F90), with two nests:
+!!   1. init_iter_winds(nest 1), calc_iter_winds: max|div| falls;
+!!   2. init_iter_winds(nest 2, another size), calc_iter_winds: max|div| falls;
+!!   3. init_iter_winds(nest 1) again, its winds reset, calc_iter_winds: the
+!!      nest's own lambda is still there (the multi-nest cache of
+!!      wind_iterative.F90:134-181), so the warm start converges in at most 2
+!!      iterations where the first solve needed many;
+!!   4. a second init_iter_winds of the active nest is a no-op.
+!! Flat terrain, alpha = 1, rho = 1, the reference unit test's winds
+!! (tests/unit/test_wind_iterative.F90:179-207).
+program test_domain_adapter
+    use iso_c_binding
+    use icar_constants, only : kVARS
+    use domain_interface, only : domain_t
+    use options_interface, only : options_t
+    use windhip_interface, only : windhip_solve_info
+    use wind_iterative_hip, only : wind_tile_t, calc_divergence
+    use wind_iterative
+    implicit none
+    real, parameter :: pi = 3.14159265358979
+    type(domain_t), target :: d1, d2
+    type(options_t) :: opts
+    type(windhip_solve_info) :: info
+    real, allocatable, target :: alpha1(:,:,:), div1(:,:,:), alpha2(:,:,:), div2(:,:,:), u0(:,:,:), v0(:,:,:)
+    integer :: iters1
+    logical :: ok
+
+    ok = .true.
+    opts%wind%wind_solver_iterations = 1500
+    call make_domain(d1, 1, 40, 36, 20, 250.0)
+    call make_domain(d2, 2, 24, 20, 12, 500.0)
+    allocate(alpha1(40, 20, 36), div1(40, 20, 36), alpha2(24, 12, 20), div2(24, 12, 20))
+    alpha1 = 1; alpha2 = 1
+    u0 = d1%vars_3d(d1%var_indx(kVARS%u)%v)%dqdt_3d
+    v0 = d1%vars_3d(d1%var_indx(kVARS%v)%v)%dqdt_3d
+
+    ! 1. nest 1
+    call init_iter_winds(d1, opts)
+    if (active_nest() /= 1) call fail("active nest after init(1)")
+    call solve_and_check(d1, alpha1, div1, "nest 1")
+    info = last_solve_info()
+    iters1 = info%iters
+
+    ! 2. nest 2
+    call init_iter_winds(d2, opts)
+    if (active_nest() /= 2) call fail("active nest after init(2)")
+    call solve_and_check(d2, alpha2, div2, "nest 2")
+
+    ! 3. back to nest 1: the same forcing, started from the nest's cached lambda
+    call init_iter_winds(d1, opts)
+    if (active_nest() /= 1) call fail("active nest after init(1) again")
+    d1%vars_3d(d1%var_indx(kVARS%u)%v)%dqdt_3d = u0
+    d1%vars_3d(d1%var_indx(kVARS%v)%v)%dqdt_3d = v0
+    d1%vars_3d(d1%var_indx(kVARS%w)%v)%dqdt_3d = 0
+    call solve_and_check(d1, alpha1, div1, "nest 1 again")
+    info = last_solve_info()
+    write(*,'(A,I0,A,I0)') 'nest 1: first solve ', iters1, ' iterations, warm re-solve ', info%iters
+    if (.not. (iters1 > 10 .and. info%iters <= 2)) call fail("nest 1 lost its lambda across the nest switch")
+
+    ! 4. init of the active nest returns at once
+    call init_iter_winds(d1, opts)
+    if (active_nest() /= 1) call fail("active nest after a repeated init")
+
+    call finalize_iter_winds()
+    if (active_nest() /= -1) call fail("active nest after finalize")
+    if (.not. ok) error stop 1
+    write(*,*) 'PASS'
+
+contains
+
+    subroutine fail(what)
+        character(len=*), intent(in) :: what
+        write(*,*) 'FAIL: ', what
+        ok = .false.
+    end subroutine fail
+
+    !> calc_divergence, calc_iter_winds, calc_divergence on the active nest
+    subroutine solve_and_check(d, alpha, div, name)
+        type(domain_t), intent(inout), target :: d
+        real, intent(in) :: alpha(:,:,:)
+        real, intent(inout), target :: div(:,:,:)
+        character(len=*), intent(in) :: name
+        type(wind_tile_t) :: tile
+        real :: d0, da
+        type(windhip_solve_info) :: inf
+        call wind_tile_of(d, tile)
+        call calc_divergence(tile, div, .false., .false., .true.)
+        d0 = maxval(abs(div))
+        call calc_iter_winds(d, alpha, div, .false.)
+        call calc_divergence(tile, div, .false., .false., .true.)
+        da = maxval(abs(div))
+        inf = last_solve_info()
+        write(*,'(A,A,ES12.4,A,ES12.4,A,I0,A,I0)') name, ': max|div| before ', d0, ' after ', da, &
+            ' status ', inf%status, ' iters ', inf%iters
+        if (da /= da .or. .not. (da < d0) .or. inf%status /= 0) call fail(name)
+    end subroutine solve_and_check
+
+    !> A flat nest of nx x ny x nz cells: the metrics, u, v, w (dqdt_3d) and
+    !! density in vars_3d, stored in an order unlike kVARS so that every
+    !! access has to go through var_indx.
+    subroutine make_domain(d, nest, nx, ny, nz, dx)
+        type(domain_t), intent(out) :: d
+        integer, intent(in) :: nest, nx, ny, nz
+        real, intent(in) :: dx
+        real :: dz(nz), lx, ly
+        integer :: q, i, j, k
+        integer, parameter :: order(13) = [13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2, 1]
+        d%nest_indx = nest
+        d%dx = dx
+        d%grid%ids = 1; d%grid%ide = nx; d%grid%jds = 1; d%grid%jde = ny; d%grid%kds = 1; d%grid%kde = nz
+        d%ids = 1; d%ide = nx; d%jds = 1; d%jde = ny; d%kds = 1; d%kde = nz
+        d%ims = 1; d%ime = nx; d%jms = 1; d%jme = ny; d%kms = 1; d%kme = nz
+        allocate(d%vars_3d(13), d%vars_2d(1))
+        ! kVARS u=1 .. advection_dz=13 -> vars_3d slot order(kVARS value)
+        do q = 1, 13
+            d%var_indx(q)%v = order(q)
+        end do
+        d%var_indx(kVARS%neighbor_terrain)%v = 1
+        allocate(d%vars_2d(1)%data_2d(nx, ny))
+        d%vars_2d(1)%data_2d = 0
+        call auto_dz_icon(nz, 20.0, 8000.0, 0.7, dz)
+        associate(v => d%vars_3d, x => d%var_indx)
+            allocate(v(x(kVARS%u)%v)%dqdt_3d(nx + 1, nz, ny), v(x(kVARS%v)%v)%dqdt_3d(nx, nz, ny + 1))
+            allocate(v(x(kVARS%w)%v)%dqdt_3d(nx, nz, ny))
+            allocate(v(x(kVARS%jacobian_u)%v)%data_3d(nx + 1, nz, ny), v(x(kVARS%dzdx_u)%v)%data_3d(nx + 1, nz, ny))
+            allocate(v(x(kVARS%jacobian_v)%v)%data_3d(nx, nz, ny + 1), v(x(kVARS%dzdy_v)%v)%data_3d(nx, nz, ny + 1))
+            allocate(v(x(kVARS%dzdx)%v)%data_3d(nx, nz, ny), v(x(kVARS%dzdy)%v)%data_3d(nx, nz, ny))
+            allocate(v(x(kVARS%jacobian)%v)%data_3d(nx, nz, ny), v(x(kVARS%jacobian_w)%v)%data_3d(nx, nz, ny))
+            allocate(v(x(kVARS%advection_dz)%v)%data_3d(nx, nz, ny), v(x(kVARS%density)%v)%data_3d(nx, nz, ny))
+            v(x(kVARS%dzdx)%v)%data_3d = 0; v(x(kVARS%dzdy)%v)%data_3d = 0
+            v(x(kVARS%dzdx_u)%v)%data_3d = 0; v(x(kVARS%dzdy_v)%v)%data_3d = 0
+            v(x(kVARS%jacobian)%v)%data_3d = 1; v(x(kVARS%jacobian_u)%v)%data_3d = 1
+            v(x(kVARS%jacobian_v)%v)%data_3d = 1; v(x(kVARS%jacobian_w)%v)%data_3d = 1
+            v(x(kVARS%density)%v)%data_3d = 1
+            do k = 1, nz
+                v(x(kVARS%advection_dz)%v)%data_3d(:, k, :) = dz(k)
+            end do
+            lx = real(nx - 1); ly = real(ny - 1)
+            do i = 1, nx + 1
+                v(x(kVARS%u)%v)%dqdt_3d(i, :, :) = 8.0 * cos(2.0 * pi * real(i - 1) / lx)
+            end do
+            do j = 1, ny + 1
+                v(x(kVARS%v)%v)%dqdt_3d(:, :, j) = 6.0 * sin(2.0 * pi * real(j - 1) / ly)
+            end do
+            v(x(kVARS%w)%v)%dqdt_3d = 0
+        end associate
+    end subroutine make_domain
+
+    !> auto_dz case 1 (src/objects/domain_obj.F90:1532-1544)
+    subroutine auto_dz_icon(n, lowest, top, stretch, dzo)
+        integer, intent(in) :: n
+        real, intent(in) :: lowest, top, stretch
+        real, intent(out) :: dzo(n)
+        real :: x1, a, b, c, vct(0:n)
+        integer :: jk
+        x1 = (2.0 * stretch - 1.0) * lowest
+        b = (top - (x1 / 6.0) * n**3 - (lowest - x1 / 6.0) * n) / (n**2 - 1.0 / 3.0 * n**3 - 2.0 / 3.0 * n)
+        a = (x1 - 2.0 * b) / 6.0
+        c = lowest - (a + b)
+        do jk = 0, n
+            vct(jk) = a * jk**3 + b * jk**2 + c * jk
+        end do
+        dzo = vct(1:n) - vct(0:n-1)
+    end subroutine auto_dz_icon
+
+end program test_domain_adapter

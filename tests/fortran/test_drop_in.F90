@@ -51,6 +51,7 @@ program test_drop_in
     call calc_iter_winds(tile, alpha, div0, .false.)
     info = last_solve_info()
     call calc_divergence(tile, div1, .false., .false., .true.)
+    d0 = maxval(abs(div0)); d1 = maxval(abs(div1))
 
     ! balance_uvw: w from the horizontal divergence of (u, v); the full
     ! density-weighted divergence then vanishes to FP32 rounding
@@ -73,7 +74,6 @@ program test_drop_in
         error stop 1
     end if
 
-    d0 = maxval(abs(div0)); d1 = maxval(abs(div1))
     write(*,'(A,ES12.4,A,ES12.4,A,I0,A,I0,A,ES10.3)') 'max|div| before ', d0, ' after ', d1, &
         ' status ', info%status, ' iters ', info%iters, ' true_rel_res ', info%true_rel_res
     if (d1 /= d1 .or. .not. (d1 < d0) .or. info%status /= 0) then
