@@ -130,3 +130,32 @@ def test_fused_coarse_smoothing_matches_the_two_kernel_step(name):
             np.testing.assert_allclose(fused[ring], want[ring], rtol=1e-6, atol=1e-6)
         level += 1
     assert level > 2
+
+
+@pytest.mark.parametrize("precision", ["MIXED_LINE", "MIXED_MG"])
+def test_tall_columns_take_the_general_line_kernels(precision):
+    """nz = 64 (66 rows per column): the line solves leave the register-resident
+    kernels (at most 64 rows) for the batched march with its forward results in
+    memory, and the small multigrid levels keep the two-kernel smoothing step;
+    the answer is the oracle's as for short columns."""
+    from hicar_amd import _native as nat
+    from hicar_amd import synthetic as S
+    from hicar_amd.wind_iterative import WindSolver
+    _dev()
+    c = S.gaussian_hill_case(nx=14, ny=12, nz=64, dx=250.0, seed=2)
+    o = oracle_for(c)
+    div = o.divergence(c.u, c.v, c.w, rho=c.rho, adv_den=True, use_dqdt=True)
+    o.solve(c.alpha, div, max_iters=20000, tol_rel=1e-12)
+    s = WindSolver(c.nx, c.ny, c.nz, c.dx)
+    s.set_metrics(c.metrics)
+    info = s.solve(c.alpha, div, precision=getattr(nat, precision), tol_rel=1e-8, start=1, max_iters=20000)
+    assert info["status"] == 0 and info["true_rel_res"] <= 1e-8
+    assert _rel(s.get_lambda(), o.get_lambda()) <= 1e-6
+    if precision == "MIXED_MG":
+        (nx, ny, mz), _ = s.mg_level(1)
+        assert mz == 66
+        rng = np.random.default_rng(3)
+        b = rng.standard_normal((mz, ny + 2, nx + 2)).astype(np.float32)
+        y = rng.standard_normal((mz, ny + 2, nx + 2)).astype(np.float32)
+        # no one-launch form above 64 rows: both requests take the two-kernel step
+        assert np.array_equal(s.mg_smooth(1, True, b, y), s.mg_smooth(1, False, b, y))
