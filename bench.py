@@ -150,6 +150,8 @@ def main():
     ap.add_argument("--tight", type=float, default=1e-11,
                     help="rel. residual of the FP64 reference solve the errors are stated against")
     ap.add_argument("--no-tight", action="store_true", help="skip the tight FP64 reference solve")
+    ap.add_argument("--no-varying-alpha", action="store_true",
+                    help="skip the mixed arms on the spatially varying alpha of the mixed-precision configuration")
     ap.add_argument("--sequence", type=int, default=0,
                     help="run S consecutive solves of the 1024x1024x60 sequence instead of the benchmark")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -207,7 +209,7 @@ def main():
     want = [a for a in args.arms.split(",") if a in arms and (net is not None or "learned" not in a)]
     zeros = np.zeros(s.rows_shape)
 
-    def solve(arm, profile=0, tol=None):
+    def solve(arm, profile=0, tol=None, a=None):
         o = dict(arms[arm])
         inner = o.pop("inner_tol", args.inner_tol)
         if tol is None:
@@ -216,7 +218,8 @@ def main():
         s.set_lambda(zeros)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        info = s.solve(alpha, div, tol_rel=tol, max_iters=50000, inner_tol=inner, profile=profile, **o)
+        info = s.solve(alpha if a is None else a, div, tol_rel=tol, max_iters=50000, inner_tol=inner,
+                       profile=profile, **o)
         torch.cuda.synchronize()
         return 1e3 * (time.perf_counter() - t0), info
 
@@ -271,6 +274,35 @@ def main():
         res[arm] = r
         log(f"{arm}: {r['ms']:.1f} ms/solve iters={r['iters']} outer={r['outer']} status={r['status']} "
             f"true_rel_res={r['true_rel_res']:.3e} start_used={r['start_used']}")
+
+    # The mixed-precision configuration with a spatially varying stability
+    # weight: the same terrain and winds, alpha in [0.3, 1] from a seeded
+    # lapse-rate field smoothed over 2 km; errors of the mixed arms against a
+    # tight FP64 solve and the plain FP64 solve of the same alpha.
+    vary = None
+    if not args.no_varying_alpha:
+        from hicar_amd import synthetic as S
+        a2 = D(S.fractal_case(nx=args.nx, ny=args.ny, nz=args.nz, dx=50.0, relief=2500.0, speed=8.0, seed=1,
+                              varying_alpha=True).alpha)
+        vary = dict(alpha_min=round(float(a2.min()), 4), alpha_max=round(float(a2.max()), 4), arms={})
+        ms_t, info_t = solve("plain", tol=args.tight, a=a2)
+        tight2 = (s.get_lambda(), winds())
+        vary["tight_reference"] = dict(tol_rel=args.tight, iters=info_t["iters"], status=info_t["status"],
+                                       true_rel_res=info_t["true_rel_res"])
+        ref2 = None
+        for arm in [x for x in ("plain", "mixed", "mg", "mg_learned") if x in want]:
+            solve(arm, a=a2)
+            ms2, info2 = solve(arm, a=a2)
+            lam2, wnd2 = s.get_lambda(), winds()
+            r2 = dict(ms=round(ms2, 2), iters=info2["iters"], outer=info2["outer_iters"], status=info2["status"],
+                      true_rel_res=info2["true_rel_res"], err_vs_tight=errs(lam2, wnd2, tight2))
+            if arm == "plain":
+                ref2 = (lam2, wnd2)
+            elif ref2 is not None:
+                r2["err_vs_plain"] = errs(lam2, wnd2, ref2)
+            vary["arms"][arm] = r2
+            log(f"varying alpha, {arm}: {r2['ms']:.1f} ms/solve iters={r2['iters']} "
+                f"lambda err vs tight {r2['err_vs_tight']['lam']:.2e}")
 
     head = next((a for a in ("mg_learned", "mg", "line_learned", "line") if a in res), next(iter(res)))
     ms_h = res[head]["ms"]
@@ -346,6 +378,7 @@ def main():
         "cells_per_s": case.cells / (ms_h * 1e-3),
         "tight_reference": tight_info if tight is not None else None,
         "arms": res,
+        "varying_alpha": vary,
         "network": netinfo,
         "kernels": tables,
     }

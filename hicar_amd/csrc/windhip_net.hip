@@ -102,16 +102,32 @@ __global__ __launch_bounds__(kT) void k_conv3_lds(Shape s, const bf16 *__restric
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) acc[r][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
     const bf16x8 zero8 = {};
+    // halo entries per thread, staged in batches of up to kQB: a batch's loads
+    // are all issued before its LDS stores, so the staging costs one memory
+    // round trip per batch instead of one per entry
+    constexpr int NQ = ((CC / 8) * kHv + kT - 1) / kT, kQB = NQ < 6 ? NQ : 6;
     for (int c0 = 0; c0 < C; c0 += CC) {
         __syncthreads();  // the previous chunk's reads are done
-        for (int q = threadIdx.x; q < (CC / 8) * kHv; q += kT) {
-            const int cg = q / kHv, v = q - cg * kHv;
-            const int hx = v % kHx, hy = (v / kHx) % kHy, hz = v / (kHx * kHy);
-            const int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
-            bf16x8 val = zero8;
-            if (x >= 0 && x < s.W && y >= 0 && y < s.H && z >= 0 && z < s.D)
-                val = *reinterpret_cast<const bf16x8 *>(X + (vb + z * HW + (long)y * s.W + x) * ldx + c0 + cg * 8);
-            tile[q] = val;
+#pragma unroll 1
+        for (int i0 = 0; i0 < NQ; i0 += kQB) {
+            bf16x8 val[kQB];
+#pragma unroll
+            for (int e = 0; e < kQB; ++e) {
+                const int q = threadIdx.x + (i0 + e) * kT;
+                const int cg = q / kHv, v = q - cg * kHv;
+                const int hx = v % kHx, hy = (v / kHx) % kHy, hz = v / (kHx * kHy);
+                const int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
+                val[e] = zero8;
+                if (i0 + e < NQ && q < (CC / 8) * kHv && x >= 0 && x < s.W && y >= 0 && y < s.H && z >= 0 &&
+                    z < s.D)
+                    val[e] = *reinterpret_cast<const bf16x8 *>(X + (vb + z * HW + (long)y * s.W + x) * ldx + c0 +
+                                                               cg * 8);
+            }
+#pragma unroll
+            for (int e = 0; e < kQB; ++e) {
+                const int q = threadIdx.x + (i0 + e) * kT;
+                if (i0 + e < NQ && q < (CC / 8) * kHv) tile[q] = val[e];
+            }
         }
         __syncthreads();
 #pragma unroll 2
