@@ -417,7 +417,7 @@ def run_sequence(args, log, dev):
     scales = np.random.default_rng(7).uniform(0.95, 1.05, size=args.sequence)
     net = learned.Net.load(WEIGHTS)
     starts = {"learned": nat.START_LEARNED, "warm": nat.START_WARM, "zero": nat.START_ZERO}
-    peak = 0.0
+    peak = handle_peak = 0.0
     out = {}
     for name, start in starts.items():
         learned.attach(s, net if start == nat.START_LEARNED else None)
@@ -443,6 +443,7 @@ def run_sequence(args, log, dev):
             if info["status"] != 0:
                 raise RuntimeError(f"sequence step {step} ({name} start) did not converge: {info}")
             peak = max(peak, s.memory()[1])
+            handle_peak = max(handle_peak, s.memory()[0])
         out[name] = dict(ms_total=round(sum(ms), 1), ms_mean=round(float(np.mean(ms)), 2),
                          iters_total=int(sum(its)), iters_first=its[0], iters_mean=round(float(np.mean(its)), 2),
                          worst_true_rel_res=worst)
@@ -451,7 +452,7 @@ def run_sequence(args, log, dev):
             "config": {"workload": f"{args.sequence} wind solves 1024x1024x60 fractal terrain (3000 m), forcing "
                                    "turned 1 deg and rescaled U[0.95, 1.05] per step", "tol_rel": args.mixed_tol,
                        "method": "MIXED_MG (FP64 FGMRES, FP32 multigrid-preconditioned BiCGStab)"},
-            "starts": out, "peak_device_bytes_in_use": peak,
+            "starts": out, "peak_device_bytes_in_use": peak, "solver_handle_bytes": handle_peak,
             "torch_peak_bytes": int(torch.cuda.max_memory_allocated())}
     print(json.dumps(line))
 
