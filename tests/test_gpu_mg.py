@@ -159,3 +159,25 @@ def test_tall_columns_take_the_general_line_kernels(precision):
         y = rng.standard_normal((mz, ny + 2, nx + 2)).astype(np.float32)
         # no one-launch form above 64 rows: both requests take the two-kernel step
         assert np.array_equal(s.mg_smooth(1, True, b, y), s.mg_smooth(1, False, b, y))
+
+
+def test_mixed_mg_with_f32_fields():
+    """HICAR's `real` (FP32) caller fields: the multigrid solve of the promoted
+    inputs reaches the oracle's lambda on the same FP32-rounded inputs."""
+    from hicar_amd import _native as nat
+    from hicar_amd.wind_iterative import WindSolver
+    from oracle import Oracle
+    _dev()
+    c = small_case("fractal_alpha")
+    m32 = {k: (np.asarray(x, dtype=np.float32) if isinstance(x, np.ndarray) else x) for k, x in c.metrics.items()}
+    m64 = {k: (x.astype(np.float64) if isinstance(x, np.ndarray) else x) for k, x in m32.items()}
+    s = WindSolver(c.nx, c.ny, c.nz, c.dx, dtype=np.float32)
+    s.set_metrics(m32)
+    o = Oracle(c.nx, c.ny, c.nz, c.dx)
+    o.set_metrics(m64)
+    div = o.divergence(c.u, c.v, c.w, rho=c.rho, adv_den=True).astype(np.float32)
+    a32 = c.alpha.astype(np.float32)
+    o.solve(a32.astype(np.float64), div.astype(np.float64), tol_rel=1e-12, max_iters=5000)
+    info = s.solve(a32, div, precision=nat.MIXED_MG, tol_rel=1e-8, start=1, max_iters=20000, inner_tol=0.1)
+    assert info["status"] == 0 and info["true_rel_res"] <= 1e-8
+    assert _rel(s.get_lambda(), o.get_lambda()) <= 1e-6
