@@ -454,7 +454,9 @@ int windhip_solver::init()
 
 void windhip_solver::release()
 {
+    // both streams drained before anything they may still read is freed
     if (stream) (void)hipStreamSynchronize(stream);
+    if (net_stream) (void)hipStreamSynchronize(net_stream);
     for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
     graphs.clear();
     for (void *q : allocs) (void)hipFree(q);
@@ -463,7 +465,6 @@ void windhip_solver::release()
         if (stage[q]) (void)hipFree(stage[q]);
     if (st_host) (void)hipHostFree(st_host);
     if (scan_host) (void)hipHostFree(scan_host);
-    if (net_stream) (void)hipStreamSynchronize(net_stream);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (ev_net0) (void)hipEventDestroy(ev_net0);
