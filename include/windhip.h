@@ -267,8 +267,8 @@ int windhip_learned_input(windhip_solver *s, float *in_dev, int *dims_out, doubl
  * 18 residual in - A y, 19 v = A p_hat + sigma, 20 t = A s_hat + 5 dots (its
  * s- and x/r/p-updates count as 7 and 9); MIXED_MG adds 21 fine-level
  * restriction and prolongation, 22 every kernel of the coarse levels, 23 the
- * Galerkin set-up (its fine-level line solves and residuals count as 17 and
- * 18). reset != 0 clears the
+ * coarse levels' set-up (its fine-level line solves and residuals count as 17
+ * and 18). reset != 0 clears the
  * counters after reading. */
 #define WINDHIP_PROFILE_KINDS 24
 int windhip_profile(windhip_solver *s, int reset, double *calls, double *ms);
