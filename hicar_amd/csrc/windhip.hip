@@ -77,6 +77,7 @@ struct windhip_solver {
     double *m_dzdx_u, *m_jaco_u, *m_dzdy_v, *m_jaco_v;
     double *p_terrain, *p_sx, *p_sy;
     double *mf[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // mx_u, my_u, my_v, mx_v, mxy
+    bool mf_on[5] = {false, false, false, false, false};             // given by the last set_metrics
     // operator
     double *B, *C, *H, *I, *L, *M;
     float *B32, *C32, *H32, *I32, *L32, *M32;
@@ -238,7 +239,9 @@ struct windhip_solver {
         m.dzdx = m_dzdx; m.dzdy = m_dzdy; m.jaco = m_jaco; m.jaco_w = m_jaco_w; m.alpha = m_alpha;
         m.adv_dz = m_adv_dz; m.dz_if = m_dz_if; m.dzdx_u = m_dzdx_u; m.jaco_u = m_jaco_u;
         m.dzdy_v = m_dzdy_v; m.jaco_v = m_jaco_v; m.sx = p_sx; m.sy = p_sy;
-        m.mf_mx_u = mf[0]; m.mf_my_u = mf[1]; m.mf_my_v = mf[2]; m.mf_mx_v = mf[3]; m.mf_mxy = mf[4];
+        m.mf_mx_u = mf_on[0] ? mf[0] : nullptr; m.mf_my_u = mf_on[1] ? mf[1] : nullptr;
+        m.mf_my_v = mf_on[2] ? mf[2] : nullptr; m.mf_mx_v = mf_on[3] ? mf[3] : nullptr;
+        m.mf_mxy = mf_on[4] ? mf[4] : nullptr;
         return m;
     }
 
@@ -1585,6 +1588,7 @@ int windhip_set_metrics(windhip_solver *s, const void *dzdx, const void *dzdy, c
     const void *mfs[5] = {mapfac_mx_u, mapfac_my_u, mapfac_my_v, mapfac_mx_v, mapfac_mxy};
     const int kinds[5] = {1, 1, 2, 2, 0};
     for (int q = 0; q < 5; ++q) {
+        s->mf_on[q] = mfs[q] != nullptr;  // a null map factor is 1, also after an earlier non-null one
         if (!mfs[q]) continue;
         if (!s->mf[q]) CHK(s->alloc(&s->mf[q], s->d.P));
         int ib = kinds[q] == 1 ? nx + 1 : nx, jb = kinds[q] == 2 ? ny + 1 : ny;

@@ -263,6 +263,9 @@ class Oracle:
         self.lib.orc_idealized_wgrid(self.h, _p(u), _p(v), _p(wr), _p(out))
         return out
 
-    def probe(self, rho=None, adv_den=False):
+    def probe(self, rho=None, adv_den=False, mf_mx_u=None, mf_my_v=None, mf_my_u=None, mf_mx_v=None, mf_mxy=None):
+        """calibrate_projection_operator (wind.F90:1010-1099); map factors as in
+        correct (G) and divergence (D), None = 1."""
         r = _f64(rho)
-        return self.lib.orc_probe(self.h, _p(r), None, None, None, None, None, int(adv_den))
+        mfs = [_f64(a) for a in (mf_mx_u, mf_my_v, mf_my_u, mf_mx_v, mf_mxy)]
+        return self.lib.orc_probe(self.h, _p(r), *[_p(a) for a in mfs], int(adv_den))

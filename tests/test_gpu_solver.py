@@ -209,3 +209,50 @@ def test_error_paths():
     d = o.divergence(c.u, c.v, c.w)
     info = s.solve(c.alpha, d, start=1, max_iters=3, retry=0, tol_rel=1e-12)
     assert info["status"] == 1 and info["iters"] == 3
+
+
+def test_map_factors_bitwise_and_cleared():
+    """Map factors (calc_divergence's mapfac_mxy, mapfac_my_u, mapfac_mx_v;
+    calc_updated_winds' mapfac_mx_u, mapfac_my_v; both in probing): the
+    divergence, the correction and the probed operator and its solve are
+    bit-identical to the oracle's; a later set_metrics without map factors
+    means factors of 1 again."""
+    c = small_case("hill")
+    rng = np.random.default_rng(11)
+    f = lambda shp: 1.0 + 0.1 * rng.uniform(-1.0, 1.0, shp)  # noqa: E731
+    mf = dict(mf_mx_u=f((c.ny, c.nx + 1)), mf_my_u=f((c.ny, c.nx + 1)), mf_my_v=f((c.ny + 1, c.nx)),
+              mf_mx_v=f((c.ny + 1, c.nx)), mf_mxy=f((c.ny, c.nx)))
+    m = dict(c.metrics)
+    m.update({"mapfac_" + k[3:]: a for k, a in mf.items()})
+    from hicar_amd.wind_iterative import WindSolver
+    s = WindSolver(c.nx, c.ny, c.nz, c.dx)
+    s.set_metrics(m)
+    o = oracle_for(c)
+    dkw = dict(mf_my_u=mf["mf_my_u"], mf_mx_v=mf["mf_mx_v"], mf_mxy=mf["mf_mxy"])
+    div_o = o.divergence(c.u, c.v, c.w, rho=c.rho, adv_den=True, use_dqdt=True, **dkw)
+    assert np.array_equal(s.divergence(c.u, c.v, c.w, density=c.rho, adv_den=True, use_dqdt=True), div_o)
+    o.solve(c.alpha, div_o, max_iters=3000, tol_rel=1e-8)
+    s.solve(c.alpha, div_o, max_iters=3000, tol_rel=1e-8)
+    assert np.array_equal(s.get_lambda(), o.get_lambda())
+    uo, vo, wo = o.correct(c.u, c.v, c.w, rho=c.rho, adv_den=True, mf_mx_u=mf["mf_mx_u"], mf_my_v=mf["mf_my_v"])
+    ug, vg, wg = c.u.copy(), c.v.copy(), c.w.copy()
+    s.apply_correction(ug, vg, wg, density=c.rho, adv_den=True)
+    assert np.array_equal(ug, uo) and np.array_equal(vg, vo) and np.array_equal(wg, wo)
+    # probing with map factors: the same operator and solve
+    o.set_alpha(c.alpha)
+    assert o.probe(rho=c.rho, adv_den=True, **mf) == 0.0
+    assert s.probe_operator(c.alpha, density=c.rho, adv_den=True) == 0.0
+    x = np.random.default_rng(12).standard_normal(s.rows_shape)
+    s.set_alpha(c.alpha)
+    assert np.array_equal(s.apply_operator(x), o.spmv(x))
+    ref = o.solve(c.alpha, div_o, max_iters=3000, tol_rel=1e-11)
+    got = s.solve(c.alpha, div_o, max_iters=3000, tol_rel=1e-11)
+    assert got["iters"] == ref["iters"] and np.array_equal(s.get_lambda(), o.get_lambda())
+    ug, vg, wg = c.u.copy(), c.v.copy(), c.w.copy()
+    s.apply_correction(ug, vg, wg, density=c.rho, adv_den=True)
+    d1 = s.divergence(ug, vg, wg, density=c.rho, adv_den=True, use_dqdt=True)
+    assert np.linalg.norm(d1) <= 1e-9 * np.linalg.norm(div_o)
+    # metrics again without map factors: factors of 1
+    s.set_metrics(c.metrics)
+    assert np.array_equal(s.divergence(c.u, c.v, c.w, density=c.rho, adv_den=True, use_dqdt=True),
+                          o.divergence(c.u, c.v, c.w, rho=c.rho, adv_den=True, use_dqdt=True))
