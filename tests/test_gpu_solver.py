@@ -256,3 +256,44 @@ def test_map_factors_bitwise_and_cleared():
     s.set_metrics(c.metrics)
     assert np.array_equal(s.divergence(c.u, c.v, c.w, density=c.rho, adv_den=True, use_dqdt=True),
                           o.divergence(c.u, c.v, c.w, rho=c.rho, adv_den=True, use_dqdt=True))
+
+
+@pytest.mark.parametrize("adv_den", [False, True])
+@pytest.mark.parametrize("horz_only", [False, True])
+@pytest.mark.parametrize("use_dqdt", [False, True])
+def test_every_divergence_branch_bitwise(adv_den, horz_only, use_dqdt):
+    """calc_divergence (wind.F90:250-523) in all eight branch combinations,
+    with density, bit for bit."""
+    c = small_case("fractal_alpha")
+    rho = 1.0 + 0.05 * np.random.default_rng(9).standard_normal(c.rho.shape)
+    s = _solver(c)
+    o = oracle_for(c)
+    want = o.divergence(c.u, c.v, c.w, rho=rho, adv_den=adv_den, horz_only=horz_only, use_dqdt=use_dqdt)
+    got = s.divergence(c.u, c.v, c.w, density=rho, adv_den=adv_den, horz_only=horz_only, use_dqdt=use_dqdt)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("max_iters", [5, 20, 40])
+def test_retry_ladder_bitwise(max_iters):
+    """At the iteration cap without a 100x residual drop the solve retries from
+    lambda = 0 with 3, then 4 preconditioner sweeps (wind_iterative.F90:474-504);
+    a residual still above the initial one after the ladder is the reference's
+    fatal divergence (:506-528), an error here. Cap 5: divergence; cap 20: the
+    whole ladder, status 1; cap 40: status 1 without the ladder. Status, sweep
+    count, iterations, residual and lambda as the oracle's."""
+    from hicar_amd import _native as nat
+    c = small_case("hill")
+    s = _solver(c)
+    o = oracle_for(c)
+    div = o.divergence(c.u, c.v, c.w, rho=c.rho, adv_den=True, use_dqdt=True)
+    ref = o.solve(c.alpha, div, max_iters=max_iters, tol_rel=1e-12)
+    assert ref["status"] == 1
+    if ref["res_final"] > ref["res0"]:
+        with pytest.raises(nat.WindHipError) as e:
+            s.solve(c.alpha, div, max_iters=max_iters, tol_rel=1e-12, start=1)
+        assert e.value.code == nat.WINDHIP_ERR_DIVERGED
+        return
+    got = s.solve(c.alpha, div, max_iters=max_iters, tol_rel=1e-12, start=1)
+    assert (got["status"], got["nsweeps"], got["iters"]) == (ref["status"], ref["nsweeps"], ref["iters"])
+    assert got["res0"] == ref["res0"] and got["res_final"] == ref["res_final"]
+    assert np.array_equal(s.get_lambda(), o.get_lambda())
