@@ -706,11 +706,11 @@ int windhip_solver::enqueue_iterations_line(int count, bool multigrid)
             if (!mg.empty()) {
                 CHK(resid(cur, in, lf_r));
                 CHK(timed(PK_MG_TRANSFER, [&] {
-                    k_mg_restrict<<<mg_blocks(mg[0].g.N), kMgThreads, 0, stream>>>(f0, mg[0].g, sp, lf_r, mg[0].b);
+                    k_mg_restrict<<<mg_plane_grid(mg[0].g), kMgThreads, 0, stream>>>(f0, mg[0].g, sp, lf_r, mg[0].b);
                 }));
                 float *e = mg_coarse(0);
                 CHK(timed(PK_MG_TRANSFER, [&] {
-                    k_mg_prolong<<<mg_blocks(f0.N), kMgThreads, 0, stream>>>(f0, mg[0].g, sp, e, cur);
+                    k_mg_prolong<<<mg_plane_grid(f0), kMgThreads, 0, stream>>>(f0, mg[0].g, sp, e, cur);
                 }));
             }
             for (int q = 0; q < mg_post; ++q) {
@@ -783,7 +783,6 @@ float *windhip_solver::mg_coarse(size_t l)
 {
     MgLevel &L = mg[l];
     const KState *sp = st + 1;
-    const int nr = mg_blocks(L.g.N);
     const Geo &gl = L.gg;
     const int cb = col_blocks(gl), kb = chunk_blocks(gl);
     const OpA<float> op = level_op<float>(L);
@@ -832,10 +831,10 @@ float *windhip_solver::mg_coarse(size_t l)
     resid(cur);
     const MgLevel &C = mg[l + 1];
     timed(PK_MG_COARSE, [&] {
-        k_mg_restrict<<<mg_blocks(C.g.N), kMgThreads, 0, stream>>>(L.g, C.g, sp, L.r, C.b);
+        k_mg_restrict<<<mg_plane_grid(C.g), kMgThreads, 0, stream>>>(L.g, C.g, sp, L.r, C.b);
     });
     float *e = mg_coarse(l + 1);
-    timed(PK_MG_COARSE, [&] { k_mg_prolong<<<nr, kMgThreads, 0, stream>>>(L.g, C.g, sp, e, cur); });
+    timed(PK_MG_COARSE, [&] { k_mg_prolong<<<mg_plane_grid(L.g), kMgThreads, 0, stream>>>(L.g, C.g, sp, e, cur); });
     for (int q = 0; q < mg_cpost; ++q) {
         mg_smooth(l, wave, cur, alt);
         std::swap(cur, alt);

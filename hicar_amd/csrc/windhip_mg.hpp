@@ -48,6 +48,14 @@ inline MgGeo mg_geo(int nx, int ny, int mz)
 
 constexpr int kMgThreads = 256;
 inline int mg_blocks(long cols) { return (int)((cols + kMgThreads - 1) / kMgThreads); }
+// the plane-by-level grid of the transfer kernels: kPlaneX x kPlaneY columns
+// per block, one grid layer per level
+constexpr int kPlaneX = 64, kPlaneY = kMgThreads / kPlaneX;
+inline dim3 mg_plane_grid(const MgGeo &g)
+{
+    return dim3((unsigned)((g.mx + kPlaneX - 1) / kPlaneX), (unsigned)((g.my + kPlaneY - 1) / kPlaneY),
+                (unsigned)g.mz);
+}
 
 // Index set of fine columns under coarse column I of a level with nf
 // interior columns: the ring maps to the ring.
@@ -225,11 +233,13 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_restrict(MgGeo f, MgGeo c, co
                                                             const float *__restrict__ rf, float *__restrict__ bc)
 {
     if (st->done) return;
-    const long n = (long)blockIdx.x * kMgThreads + threadIdx.x;
-    if (n >= c.N) return;
-    const int k = (int)(n / c.P);
-    const long cc = n - (long)k * c.P;
-    const int I = (int)(cc % c.mx), J = (int)(cc / c.mx);
+    // a 64 x 4 block of (I, J) per grid cell (x, y), the level as grid z: no
+    // index divisions
+    const int I = (int)(blockIdx.x * kPlaneX + (threadIdx.x & (kPlaneX - 1)));
+    const int J = (int)(blockIdx.y * kPlaneY + (threadIdx.x / kPlaneX));
+    const int k = (int)blockIdx.z;
+    if (I >= c.mx || J >= c.my) return;
+    const long n = (long)k * c.P + (long)J * c.mx + I;
     if (I == 0 || I == c.mx - 1 || J == 0 || J == c.my - 1) {
         bc[n] = 0.f;
         return;
@@ -255,18 +265,16 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_prolong(MgGeo f, MgGeo c, con
                                                            const float *__restrict__ ec, float *__restrict__ xf)
 {
     if (st->done) return;
-    const long n = (long)blockIdx.x * kMgThreads + threadIdx.x;
-    if (n >= f.N) return;
-    const int k = (int)(n / f.P);
-    const long fc = n - (long)k * f.P;
-    const int i = (int)(fc % f.mx), j = (int)(fc / f.mx);
-    if (i == 0 || i == f.mx - 1 || j == 0 || j == f.my - 1) return;
+    const int i = (int)(blockIdx.x * kPlaneX + (threadIdx.x & (kPlaneX - 1)));
+    const int j = (int)(blockIdx.y * kPlaneY + (threadIdx.x / kPlaneX));
+    const int k = (int)blockIdx.z;
+    if (i <= 0 || i >= f.mx - 1 || j <= 0 || j >= f.my - 1) return;
     const int I = (i + 1) >> 1, J = (j + 1) >> 1;
     const int In = (i & 1) ? I - 1 : I + 1, Jn = (j & 1) ? J - 1 : J + 1;
     const float *e = ec + (long)k * c.P;
     const float v = 0.5625f * e[(long)J * c.mx + I] + 0.1875f * (e[(long)J * c.mx + In] + e[(long)Jn * c.mx + I])
                   + 0.0625f * e[(long)Jn * c.mx + In];
-    xf[n] += v;
+    xf[(long)k * f.P + (long)j * f.mx + i] += v;
 }
 
 }  // namespace windhip
