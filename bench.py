@@ -59,15 +59,17 @@ WEIGHTS = os.path.join(ROOT, "hicar_amd", "data", "lnet_params.npy")
 TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic.json")
 
 
-def kernel_bytes(kind: str, nx: int, ny: int, nz: int) -> float:
+def kernel_bytes(kind: str, nx: int, ny: int, nz: int, coef_bytes: int = 0) -> float:
     """Algorithmic HBM bytes of one launch: every array the kernel must touch,
     once. Coefficients (B, C, H, I, L, M) are read on interior cells, BC planes
-    (4 per column) once, vectors over all rows."""
+    (4 per column) once, vectors over all rows. coef_bytes: bytes per
+    coefficient when they are stored narrower than the vectors (2: the bf16 rows
+    of the multigrid inner solve)."""
     cells = nx * ny * nz
     P = (nx + 2) * (ny + 2)
     N = P * (nz + 2)
     t = 8 if kind.endswith("f64") or kind in ("ir_resid", "true_resid", "fg_matvec", "fg_axpy_dot") else 4
-    coef = 6 * t * cells + 4 * t * P
+    coef = 6 * (coef_bytes or t) * cells + 4 * t * P
     table = {
         "sweep": 3 * N * t + coef,        # y (stencil), in, coefficients -> out
         "sweep_sigma": 5 * N * t + coef,  # p_hat1 (stencil), p, r_hat, coefficients -> p_hat, v
@@ -313,9 +315,11 @@ def main():
             if arm.startswith(("line", "mg")) and kind in ("update_s_f32", "update_xrp_f32"):
                 kind = kind[:-4] + "_line_f32"  # without the Jacobi sweep's D^-1 terms
             avg = tot / calls
-            nb = kernel_bytes(kind, case.nx, case.ny, case.nz)
+            # the multigrid inner solve reads bf16 coefficient rows
+            cb = 2 if arm.startswith("mg") and kind in ("resid_f32", "mv_sigma_f32", "mv_omega_f32") else 0
+            nb = kernel_bytes(kind, case.nx, case.ny, case.nz, cb)
             gbs = nb / (avg * 1e-3) / 1e9 if nb else None
-            out[kind] = dict(calls=int(calls), avg_ms=round(avg, 5), total_ms=round(tot, 2),
+            out[kind] = dict(calls=int(calls), avg_ms=round(avg, 5), total_ms=round(tot, 2), bytes=nb,
                              GBs=round(gbs, 1) if gbs else None, frac=round(gbs / HBM_PEAK_GBS, 3) if gbs else None)
         return out
 
@@ -331,7 +335,7 @@ def main():
         if tr:
             traffic = tr["bytes_per_launch"]
     roof = dict(bound="hbm", achieved=d["GBs"], peak=HBM_PEAK_GBS, unit="GB/s", frac=d["frac"], traffic=traffic,
-                kernel=dom, bytes_per_launch=kernel_bytes(dom, case.nx, case.ny, case.nz), avg_ms=d["avg_ms"])
+                kernel=dom, bytes_per_launch=d["bytes"], avg_ms=d["avg_ms"])
     # the plain FP64 solve's matvec (fused with the last Jacobi sweep: v = A p_hat and sigma)
     plain_mv = None
     if "plain" in tables and "sweep_sigma_f64" in tables["plain"]:

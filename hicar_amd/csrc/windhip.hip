@@ -81,6 +81,9 @@ struct windhip_solver {
     // operator
     double *B, *C, *H, *I, *L, *M;
     float *B32, *C32, *H32, *I32, *L32, *M32;
+    // bf16 copies of B32..M32 for the multigrid inner solve, refreshed after assembly
+    unsigned short *Bh = nullptr, *Ch = nullptr, *Hh = nullptr, *Ih = nullptr, *Lh = nullptr, *Mh = nullptr;
+    bool coef_bf16_stale = true;
     double *bcx, *bcy, *bcz, *dztop;
     float *bcx32, *bcy32, *bcz32, *dztop32;
     double *P15[15] = {};
@@ -247,6 +250,8 @@ struct windhip_solver {
 
     template <typename T>
     OpA<T> opa() const;
+    OpAB opab() const;
+    int refresh_coef_bf16();
     template <typename T>
     OpP<T> opp() const;
     // call f with the operator in use (analytic or probed), chosen at compile time in the kernels
@@ -359,6 +364,13 @@ OpA<float> windhip_solver::opa<float>() const
     double inv = 1.0 / (desc.dx * desc.dx);
     o.inv = (float)inv;
     o.neg4 = (float)(-4.0 * inv);
+    return o;
+}
+OpAB windhip_solver::opab() const
+{
+    OpAB o;
+    static_cast<OpA<float> &>(o) = opa<float>();
+    o.Bh = Bh; o.Ch = Ch; o.Hh = Hh; o.Ih = Ih; o.Lh = Lh; o.Mh = Mh;
     return o;
 }
 template <>
@@ -485,8 +497,23 @@ int windhip_solver::alloc_f32_krylov()
     return WINDHIP_OK;
 }
 
+// the bf16 coefficient rows of the multigrid inner solve, from the FP32 rows
+int windhip_solver::refresh_coef_bf16()
+{
+    if (!coef_bf16_stale) return WINDHIP_OK;
+    if (!Bh)
+        for (unsigned short **a : {&Bh, &Ch, &Hh, &Ih, &Lh, &Mh}) CHK(alloc(a, N));
+    const float *src[6] = {B32, C32, H32, I32, L32, M32};
+    unsigned short *dst[6] = {Bh, Ch, Hh, Ih, Lh, Mh};
+    for (int q = 0; q < 6; ++q) k_to_bf16<<<flat_grid(N), kBlock, 0, stream>>>(src[q], dst[q], N);
+    HIPCHK(hipGetLastError());
+    coef_bf16_stale = false;
+    return WINDHIP_OK;
+}
+
 int windhip_solver::assemble(bool full)
 {
+    coef_bf16_stale = true;
     long tot = (long)d.nx * d.ny * d.nz;
     k_assemble<<<flat_grid(tot), kBlock, 0, stream>>>(d, metrics(), full ? 1 : 0, B, C, H, I, L, M, B32, C32,
                                                        H32, I32, L32, M32);
@@ -671,7 +698,7 @@ int windhip_solver::enqueue_iterations_line(int count, bool multigrid)
     kvec<float>(&X, &R, &RH, &Pv, &V, &S, &Tv, PH, SH, &DI);
     const int fg = flat_grid(N);
     const int cb = col_blocks(g), kb = chunk_blocks(g);
-    return with_op<float>([&](auto op) -> int {
+    auto body = [&](auto op) -> int {
         using Op = decltype(op);
         // M^-1 in: y1 = L^-1 in, r = in - A y1, out = y1 + L^-1 r
         auto lsolve = [&](const float *in, const float *y, float *out, float w) {
@@ -739,7 +766,11 @@ int windhip_solver::enqueue_iterations_line(int count, bool multigrid)
         }
         HIPCHK(hipGetLastError());
         return WINDHIP_OK;
-    });
+    };
+    // the multigrid inner solve runs on the bf16-coefficient operator (the
+    // rows are refreshed by run_mixed before any batch is enqueued or captured)
+    if (multigrid && !probed) return body(opab());
+    return with_op<float>(body);
 }
 
 // One damped line-relaxation step on coarse level l, out = y + w L^-1 (b - A y)
@@ -1133,7 +1164,10 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
             return WINDHIP_OK;
         }));
         HIPCHK(hipGetLastError());
-        if (multigrid) CHK(build_mg());
+        if (multigrid) {
+            CHK(build_mg());
+            if (!probed) CHK(refresh_coef_bf16());
+        }
     }
     const double inner_tol = (o.inner_tol > 0 && o.inner_tol < 1) ? o.inner_tol : 0.5;
     const int m = std::max(1, std::min(o.max_outer, kRestart));

@@ -245,6 +245,47 @@ struct OpA {
     }
 };
 
+// The analytic FP32 operator with its six coefficient rows stored in bf16 (the
+// FP32 rows rounded to nearest even; FP32's range, 8 significant bits) and the
+// arithmetic of OpA<float>: the operator of the multigrid-preconditioned FP32
+// inner solve, whose 1e-1 inner tolerance is far above the 4e-3 relative
+// rounding, at 12 instead of 24 bytes of coefficients per cell. The FP64 outer
+// solve and its residual keep the FP64 operator. Byte offsets passed in are
+// those of the FP32 rows (4 bytes per element) and are halved here.
+struct OpAB : OpA<float> {
+    const unsigned short *Bh, *Ch, *Hh, *Ih, *Lh, *Mh;
+    static __device__ __forceinline__ float bf(unsigned short h) { return __builtin_bit_cast(float, (unsigned)h << 16); }
+    __device__ __forceinline__ Cf load(long n) const
+    {
+        return Cf{bf(Bh[n]), bf(Ch[n]), bf(Hh[n]), bf(Ih[n]), bf(Lh[n]), bf(Mh[n])};
+    }
+    struct Src { rsrc_t B, C, H, I, L, M; };
+    __device__ __forceinline__ Src src(long N) const
+    {
+        const long nb = N * 2;
+        return Src{make_rsrc(Bh, nb), make_rsrc(Ch, nb), make_rsrc(Hh, nb),
+                   make_rsrc(Ih, nb), make_rsrc(Lh, nb), make_rsrc(Mh, nb)};
+    }
+    static __device__ __forceinline__ float ld(rsrc_t r, unsigned vo, unsigned so)
+    {
+        return bf(__builtin_amdgcn_raw_buffer_load_b16(r, (int)(vo >> 1), uni(so >> 1), 0));
+    }
+    __device__ __forceinline__ Cf load_src(const Src &r, long, unsigned so, unsigned, unsigned vo) const
+    {
+        return Cf{ld(r.B, vo, so), ld(r.C, vo, so), ld(r.H, vo, so), ld(r.I, vo, so), ld(r.L, vo, so),
+                  ld(r.M, vo, so)};
+    }
+};
+
+// FP32 -> bf16, round to nearest even (finite values)
+__global__ void k_to_bf16(const float *__restrict__ src, unsigned short *__restrict__ dst, long n)
+{
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const unsigned u = __builtin_bit_cast(unsigned, src[e]);
+        dst[e] = (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+    }
+}
+
 // Probed 15-array operator (spmv probed branch, wind_iterative.F90:837-910):
 // lateral and ghost-plane identity rows.
 template <typename T>
