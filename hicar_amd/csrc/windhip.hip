@@ -1103,6 +1103,11 @@ int windhip_solver::learned_launch(double *res_plain, double **guess)
     }));
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ev_net1, net_stream));
+    // per-kernel timing (opts.profile) brackets each launch with events on the
+    // solver's stream: with the guess still running beside it those would
+    // count the wait for the other stream's kernels, so a profiled solve
+    // finishes the guess first
+    if (profiling) HIPCHK(hipStreamWaitEvent(stream, ev_net1, 0));
     net_pending = true;
     *guess = gy;
     return WINDHIP_OK;
