@@ -63,8 +63,8 @@ def kernel_bytes(kind: str, nx: int, ny: int, nz: int, coef_bytes: int = 0) -> f
     """Algorithmic HBM bytes of one launch: every array the kernel must touch,
     once. Coefficients (B, C, H, I, L, M) are read on interior cells, BC planes
     (4 per column) once, vectors over all rows. coef_bytes: bytes per
-    coefficient when they are stored narrower than the vectors (2: the bf16 rows
-    of the multigrid inner solve)."""
+    coefficient (or line factor) when they are stored narrower than the vectors
+    (2: the bf16 rows of the multigrid inner solve)."""
     cells = nx * ny * nz
     P = (nx + 2) * (ny + 2)
     N = P * (nz + 2)
@@ -82,6 +82,8 @@ def kernel_bytes(kind: str, nx: int, ny: int, nz: int, coef_bytes: int = 0) -> f
         "mv_sigma": 3 * N * t + coef,     # p_hat (stencil), r_hat, coefficients -> v
         "mv_omega": 4 * N * t + coef,     # s_hat (stencil), s, r_hat, coefficients -> t
     }
+    if kind == "line_solve_f32" and coef_bytes:  # in, out, y on every second call in FP32; 3 factor rows narrower
+        return N * (2.5 * 4 + 3 * coef_bytes)
     if kind == "ir_resid":    # lambda (stencil), b, FP64 coefficients -> v0
         return 3 * N * 8 + 6 * 8 * cells + 4 * 8 * P
     if kind == "ir_accumulate":  # lambda += sum of y_q z_q: lambda in/out + the FP32 directions (counted once)
@@ -316,7 +318,8 @@ def main():
                 kind = kind[:-4] + "_line_f32"  # without the Jacobi sweep's D^-1 terms
             avg = tot / calls
             # the multigrid inner solve reads bf16 coefficient rows
-            cb = 2 if arm.startswith("mg") and kind in ("resid_f32", "mv_sigma_f32", "mv_omega_f32") else 0
+            cb = 2 if arm.startswith("mg") and kind in ("resid_f32", "mv_sigma_f32", "mv_omega_f32",
+                                                         "line_solve_f32") else 0
             nb = kernel_bytes(kind, case.nx, case.ny, case.nz, cb)
             gbs = nb / (avg * 1e-3) / 1e9 if nb else None
             out[kind] = dict(calls=int(calls), avg_ms=round(avg, 5), total_ms=round(tot, 2), bytes=nb,

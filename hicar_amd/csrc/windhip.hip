@@ -124,6 +124,8 @@ struct windhip_solver {
     double *lr_c = nullptr, *lr_d = nullptr;   // line relaxation: Thomas coefficients
     // MIXED_LINE: FP32 Thomas factors of the column tridiagonal and scratch
     float *lf_lo = nullptr, *lf_cp = nullptr, *lf_ip = nullptr, *lf_dq = nullptr, *lf_r = nullptr;
+    // bf16 copies of the line factors for the multigrid fine level
+    unsigned short *lf_lo16 = nullptr, *lf_cp16 = nullptr, *lf_ip16 = nullptr;
     // MIXED_MG: the coarse levels of the semi-coarsening hierarchy (the fine
     // level is the solver's grid with the line factors lf_*): averaged
     // metrics, the rediscretised analytic operator (FP64 and FP32), its BC
@@ -701,8 +703,16 @@ int windhip_solver::enqueue_iterations_line(int count, bool multigrid)
     auto body = [&](auto op) -> int {
         using Op = decltype(op);
         // M^-1 in: y1 = L^-1 in, r = in - A y1, out = y1 + L^-1 r
+        // the multigrid fine level reads its factors as bf16 (a smoother of a
+        // slightly perturbed column system), the line arm as FP32
         auto lsolve = [&](const float *in, const float *y, float *out, float w) {
-            if (g.mz <= 48)
+            if (multigrid && g.mz <= 48)
+                k_line_solve_r<48, unsigned short><<<cb, kColThreads, 0, stream>>>(g, sp, in, lf_lo16, lf_cp16,
+                                                                                   lf_ip16, y, out, w);
+            else if (multigrid && g.mz <= 64)
+                k_line_solve_r<64, unsigned short><<<cb, kColThreads, 0, stream>>>(g, sp, in, lf_lo16, lf_cp16,
+                                                                                   lf_ip16, y, out, w);
+            else if (g.mz <= 48)
                 k_line_solve_r<48><<<cb, kColThreads, 0, stream>>>(g, sp, in, lf_lo, lf_cp, lf_ip, y, out, w);
             else if (g.mz <= 64)
                 k_line_solve_r<64><<<cb, kColThreads, 0, stream>>>(g, sp, in, lf_lo, lf_cp, lf_ip, y, out, w);
@@ -1172,6 +1182,12 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
         if (multigrid) {
             CHK(build_mg());
             if (!probed) CHK(refresh_coef_bf16());
+            if (!lf_lo16)
+                for (unsigned short **a : {&lf_lo16, &lf_cp16, &lf_ip16}) CHK(alloc(a, N));
+            const float *src[3] = {lf_lo, lf_cp, lf_ip};
+            unsigned short *dst[3] = {lf_lo16, lf_cp16, lf_ip16};
+            for (int q = 0; q < 3; ++q) k_to_bf16<<<flat_grid(N), kBlock, 0, stream>>>(src[q], dst[q], N);
+            HIPCHK(hipGetLastError());
         }
     }
     const double inner_tol = (o.inner_tol > 0 && o.inner_tol < 1) ? o.inner_tol : 0.5;

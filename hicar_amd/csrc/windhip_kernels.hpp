@@ -1466,11 +1466,18 @@ __global__ __launch_bounds__(kColThreads) void k_line_solve_c(Geo g, const KStat
 // forward recursion fills dq[], the back substitution reads it back, so the
 // intermediate never goes to memory (5 arrays of traffic, 6 with y, instead
 // of 7 and 8).
-template <int KM>
+// factor rows in FP32 or bf16 (FT = unsigned short: the bits of a bf16, read
+// with halved byte offsets and widened to FP32)
+template <typename FT>
+__device__ __forceinline__ float fload(rsrc_t r, unsigned vo, unsigned so)
+{
+    if constexpr (sizeof(FT) == 2) return OpAB::ld(r, vo, so);
+    else return bload<float>(r, vo, so);
+}
+template <int KM, typename FT = float>
 __global__ __launch_bounds__(kColThreads) void k_line_solve_r(Geo g, const KState *st, const float *__restrict__ in,
-                                                              const float *__restrict__ lo,
-                                                              const float *__restrict__ cp,
-                                                              const float *__restrict__ ip,
+                                                              const FT *__restrict__ lo, const FT *__restrict__ cp,
+                                                              const FT *__restrict__ ip,
                                                               const float *__restrict__ y, float *__restrict__ out,
                                                               float w)
 {
@@ -1479,9 +1486,9 @@ __global__ __launch_bounds__(kColThreads) void k_line_solve_r(Geo g, const KStat
     const int mz = __builtin_amdgcn_readfirstlane(g.mz);
     const long N = g.P * mz;
     const unsigned voc = t.oc * 4u, pb = (unsigned)(g.P * 4);
-    const long nbt = N * 4;
-    const rsrc_t rin = make_rsrc(in, nbt), rlo = make_rsrc(lo, nbt), rip = make_rsrc(ip, nbt),
-                 rcp = make_rsrc(cp, nbt), ro = make_rsrc(out, nbt);
+    const long nbt = N * 4, nbf = N * (long)sizeof(FT);
+    const rsrc_t rin = make_rsrc(in, nbt), rlo = make_rsrc(lo, nbf), rip = make_rsrc(ip, nbf),
+                 rcp = make_rsrc(cp, nbf), ro = make_rsrc(out, nbt);
     const bool hy = y != nullptr;
     const rsrc_t ry = make_rsrc(hy ? y : in, nbt);
     float dq[KM];
@@ -1493,8 +1500,8 @@ __global__ __launch_bounds__(kColThreads) void k_line_solve_r(Geo g, const KStat
         for (int e = 0; e < kLB; ++e) {
             const unsigned so = (unsigned)(k0 + e < mz ? k0 + e : mz - 1) * pb;
             a[e] = bload<float>(rin, voc, so);
-            l[e] = bload<float>(rlo, voc, so);
-            q[e] = bload<float>(rip, voc, so);
+            l[e] = fload<FT>(rlo, voc, so);
+            q[e] = fload<FT>(rip, voc, so);
         }
 #pragma unroll
         for (int e = 0; e < kLB; ++e) {
@@ -1509,7 +1516,7 @@ __global__ __launch_bounds__(kColThreads) void k_line_solve_r(Geo g, const KStat
 #pragma unroll
         for (int e = 0; e < kLB; ++e) {
             const unsigned so = (unsigned)(k1 - e < mz ? k1 - e : mz - 1) * pb;
-            c[e] = bload<float>(rcp, voc, so);
+            c[e] = fload<FT>(rcp, voc, so);
             yv[e] = hy ? bload<float>(ry, voc, so) : 0.f;
         }
 #pragma unroll
