@@ -1078,9 +1078,9 @@ int windhip_solver::line_relax(double **y, double **scratch, int steps, hipStrea
 // of the plain start (the current lambda: the previous solution, or zero).
 // learned_launch takes the plain start's residual on the solver's stream, then
 // queues the network, the relaxation and the guess's residual on a second
-// stream (its own state slot and partials), so the mixed solves can run their
-// first inner solve meanwhile; learned_join makes the solver's stream wait for
-// it and reads the guess's residual. The guess lands in t or s (FP64 vectors
+// stream (its own state slot and partials), so the mixed solves can build
+// their line factors and coarse levels meanwhile; learned_join makes the
+// solver's stream wait for it and reads the guess's residual. The guess lands in t or s (FP64 vectors
 // of the plain solve, unused by the mixed solves).
 int windhip_solver::learned_launch(double *res_plain, double **guess)
 {
@@ -1144,25 +1144,6 @@ int windhip_solver::learned_join(double *res_guess)
 int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *info, double *learned_guess,
                               double plain_res)
 {
-    // the learned guess (computed on the second stream) joins as the direction
-    // of FGMRES step 1, after the first inner solve, if its residual is below
-    // the plain start's
-    const double *first_dir = nullptr;
-    auto join_learned = [&]() -> int {
-        if (!learned_guess) return WINDHIP_OK;
-        double rg = 0.0;
-        CHK(learned_join(&rg));
-        info->guess_res = rg;
-        info->plain_res = plain_res;
-        if (rg < plain_res) {
-            k_axpby<<<flat_grid(N), kBlock, 0, stream>>>(N, 1.0, learned_guess, -1.0, x, learned_guess);
-            HIPCHK(hipGetLastError());
-            first_dir = learned_guess;
-            info->start_used = WINDHIP_START_LEARNED;
-        }
-        learned_guess = nullptr;
-        return WINDHIP_OK;
-    };
     CHK(alloc_f32_krylov());
     const bool multigrid = o.precision == WINDHIP_MIXED_MG;
     const bool line = o.precision == WINDHIP_MIXED_LINE || multigrid;
@@ -1189,6 +1170,21 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
             for (int q = 0; q < 3; ++q) k_to_bf16<<<flat_grid(N), kBlock, 0, stream>>>(src[q], dst[q], N);
             HIPCHK(hipGetLastError());
         }
+    }
+    // the learned guess, computed on the second stream while the set-up above
+    // ran, becomes the start when its residual is below the plain start's
+    if (learned_guess) {
+        double rg = 0.0;
+        CHK(learned_join(&rg));
+        info->guess_res = rg;
+        info->plain_res = plain_res;
+        if (rg < plain_res) {
+            HIPCHK(hipMemcpyAsync(x, learned_guess, N * sizeof(double), hipMemcpyDeviceToDevice, stream));
+            info->start_used = WINDHIP_START_LEARNED;
+        }
+        if (verbose)
+            fprintf(stderr, "[windhip] learned start: residual %.3e vs plain start %.3e -> %s\n", rg, plain_res,
+                    rg < plain_res ? "network" : "plain");
     }
     const double inner_tol = (o.inner_tol > 0 && o.inner_tol < 1) ? o.inner_tol : 0.5;
     const int m = std::max(1, std::min(o.max_outer, kRestart));
@@ -1218,19 +1214,6 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
         int k = 0;
         bool breakdown = false;
         for (int j = 0; j < m && steps < o.max_outer && total_inner < o.max_iters; ++j) {
-            if (j == 1 && learned_guess) CHK(join_learned());
-            if (first_dir) {
-                // the learned correction as direction j: no inner solve; the
-                // set-up still normalises v_j in place
-                const double *vsq = j == 0 ? beta2 : fg_h + (long)(j - 1) * H + j;
-                k_fg_setup<<<fb, kThreads, 0, stream>>>(N, st + 1, fg_part, vsq, 1, fg_v[j], dinv32, x32, r32,
-                                                        rh32, p32, ph32[0]);
-                k_fg_take64<<<flat_grid(N), kBlock, 0, stream>>>(N, first_dir, fg_z[j], fg_z64);
-                HIPCHK(hipGetLastError());
-                first_dir = nullptr;
-                ++steps;
-                goto have_direction;
-            }
             {
             // z_j ~ A^-1 v_j in FP32 (v_j normalised in place first). An inner
             // solve whose FP32 recurrence overflowed is run again from z = 0 with
@@ -1267,7 +1250,6 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
             ++steps;
             k_fg_take<<<flat_grid(N), kBlock, 0, stream>>>(N, x32, fg_z[j], fg_z64);
             }
-        have_direction:
             double *hc = fg_h + (long)j * H;
             // w = A z_j, orthogonalised against v_0..v_j (modified Gram-Schmidt)
             CHK(with_op<double>([&](auto op) {
@@ -1330,7 +1312,6 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
         if (!(st_host[0].outer_rnorm < before)) { status = 2; break; }
         (void)breakdown;
     }
-    CHK(join_learned());  // a solve that ended before step 1 still waits for the second stream
     info->status = st_host[0].outer_done ? 0 : (status ? status : 1);
     info->iters = total_inner;
     info->outer_iters = steps;
@@ -1689,7 +1670,7 @@ int windhip_solve(windhip_solver *s, const void *alpha, const void *div, const w
         CHK(s->learned_launch(&rp, &guess));
         inf.start_used = WINDHIP_START_WARM;
         if (mixed) {
-            mixed_guess = guess;  // joins the FGMRES after its first inner solve
+            mixed_guess = guess;  // the start, if better, once the mixed set-up is queued
         } else {
             double rg = 0.0;
             CHK(s->learned_join(&rg));

@@ -220,12 +220,12 @@ def test_mixed_line_matches_oracle():
     assert info["iters"] < mixed["iters"]
 
 
-def test_mixed_learned_direction_keeps_the_answer():
-    """In the mixed solves the learned correction joins the FGMRES as a
-    direction (computed on a second stream during the first inner solve) when
-    its residual is below the plain start's: a zero-parameter network (its
-    guess is the relaxed zero start) always qualifies, a random one may not;
-    either way the converged answer stays within tolerance."""
+def test_mixed_learned_start_keeps_the_answer():
+    """In the mixed solves the learned guess (computed on a second stream
+    during the solver's set-up) starts the FGMRES when its residual is below
+    the plain start's: a zero-parameter network (its guess is the relaxed
+    zero start) always qualifies, a random one may not; either way the
+    converged answer stays within tolerance."""
     from hicar_amd import learned
     from hicar_amd import _native as nat
     from hicar_amd.wind_iterative import WindSolver
