@@ -706,7 +706,10 @@ int windhip_solver::enqueue_iterations_line(int count, bool multigrid)
         // the multigrid fine level reads its factors as bf16 (a smoother of a
         // slightly perturbed column system), the line arm as FP32
         auto lsolve = [&](const float *in, const float *y, float *out, float w) {
-            if (multigrid && g.mz <= 48)
+            if (multigrid && g.mz <= 44)
+                k_line_solve_r<44, unsigned short, 4><<<cb, kColThreads, 0, stream>>>(g, sp, in, lf_lo16, lf_cp16,
+                                                                                      lf_ip16, y, out, w);
+            else if (multigrid && g.mz <= 48)
                 k_line_solve_r<48, unsigned short><<<cb, kColThreads, 0, stream>>>(g, sp, in, lf_lo16, lf_cp16,
                                                                                    lf_ip16, y, out, w);
             else if (multigrid && g.mz <= 64)

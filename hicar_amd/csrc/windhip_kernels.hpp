@@ -1474,13 +1474,14 @@ __device__ __forceinline__ float fload(rsrc_t r, unsigned vo, unsigned so)
     if constexpr (sizeof(FT) == 2) return OpAB::ld(r, vo, so);
     else return bload<float>(r, vo, so);
 }
-template <int KM, typename FT = float>
+template <int KM, typename FT = float, int LB = kLB>
 __global__ __launch_bounds__(kColThreads) void k_line_solve_r(Geo g, const KState *st, const float *__restrict__ in,
                                                               const FT *__restrict__ lo, const FT *__restrict__ cp,
                                                               const FT *__restrict__ ip,
                                                               const float *__restrict__ y, float *__restrict__ out,
                                                               float w)
 {
+    static_assert(KM % LB == 0, "the level batches must tile KM");
     if (st->done) return;
     const ColPos t = col_pos(g, gridDim.x);
     const int mz = __builtin_amdgcn_readfirstlane(g.mz);
@@ -1492,35 +1493,86 @@ __global__ __launch_bounds__(kColThreads) void k_line_solve_r(Geo g, const KStat
     const bool hy = y != nullptr;
     const rsrc_t ry = make_rsrc(hy ? y : in, nbt);
     float dq[KM];
-    float dprev = 0.f;
+    float dprev = 0.f, delta = 0.f;
+    if constexpr (LB < kLB) {
+        // short batches, each loaded one batch ahead of its recursion; the
+        // schedule barriers and the opaque use of the running value keep every
+        // batch in its place, so the loads of later levels do not pile up in
+        // registers (unpinned, the compiler issues all of a pass's loads first)
+        float a[2][LB], l[2][LB], q[2][LB], c[2][LB], yv[2][LB];
+        auto ldf = [&](int k0, int sl) {
 #pragma unroll
-    for (int k0 = 0; k0 < KM; k0 += kLB) {
-        float a[kLB], l[kLB], q[kLB];
+            for (int e = 0; e < LB; ++e) {
+                const unsigned so = (unsigned)(k0 + e < mz ? k0 + e : mz - 1) * pb;
+                a[sl][e] = bload<float>(rin, voc, so);
+                l[sl][e] = fload<FT>(rlo, voc, so);
+                q[sl][e] = fload<FT>(rip, voc, so);
+            }
+        };
+        auto ldb = [&](int k1, int sl) {
 #pragma unroll
-        for (int e = 0; e < kLB; ++e) {
+            for (int e = 0; e < LB; ++e) {
+                const unsigned so = (unsigned)(k1 - e < mz ? k1 - e : mz - 1) * pb;
+                c[sl][e] = fload<FT>(rcp, voc, so);
+                yv[sl][e] = hy ? bload<float>(ry, voc, so) : 0.f;
+            }
+        };
+        ldf(0, 0);
+#pragma unroll
+        for (int k0 = 0; k0 < KM; k0 += LB) {
+            const int sl = (k0 / LB) & 1;
+            __builtin_amdgcn_sched_barrier(0);
+            if (k0 + LB < KM) ldf(k0 + LB, sl ^ 1);
+            else ldb(KM - 1, 0);
+#pragma unroll
+            for (int e = 0; e < LB; ++e) {
+                dprev = (a[sl][e] - l[sl][e] * dprev) * q[sl][e];
+                dq[k0 + e] = dprev;  // levels >= mz hold values that are never used
+            }
+            asm volatile("" : "+v"(dprev));
+        }
+#pragma unroll
+        for (int k1 = KM - 1; k1 >= 0; k1 -= LB) {
+            const int sl = ((KM - 1 - k1) / LB) & 1;
+            __builtin_amdgcn_sched_barrier(0);
+            if (k1 - LB >= 0) ldb(k1 - LB, sl ^ 1);
+#pragma unroll
+            for (int e = 0; e < LB; ++e) {
+                const int k = k1 - e;
+                delta = k < mz ? dq[k] - c[sl][e] * delta : 0.f;
+                if (t.valid && k < mz) bstore(yv[sl][e] + w * delta, ro, voc, (unsigned)k * pb);
+            }
+            asm volatile("" : "+v"(delta));
+        }
+        return;
+    }
+#pragma unroll
+    for (int k0 = 0; k0 < KM; k0 += LB) {
+        float a[LB], l[LB], q[LB];
+#pragma unroll
+        for (int e = 0; e < LB; ++e) {
             const unsigned so = (unsigned)(k0 + e < mz ? k0 + e : mz - 1) * pb;
             a[e] = bload<float>(rin, voc, so);
             l[e] = fload<FT>(rlo, voc, so);
             q[e] = fload<FT>(rip, voc, so);
         }
 #pragma unroll
-        for (int e = 0; e < kLB; ++e) {
+        for (int e = 0; e < LB; ++e) {
             dprev = (a[e] - l[e] * dprev) * q[e];
             dq[k0 + e] = dprev;  // levels >= mz hold values that are never used
         }
     }
-    float delta = 0.f;
 #pragma unroll
-    for (int k1 = KM - 1; k1 >= 0; k1 -= kLB) {
-        float c[kLB], yv[kLB];
+    for (int k1 = KM - 1; k1 >= 0; k1 -= LB) {
+        float c[LB], yv[LB];
 #pragma unroll
-        for (int e = 0; e < kLB; ++e) {
+        for (int e = 0; e < LB; ++e) {
             const unsigned so = (unsigned)(k1 - e < mz ? k1 - e : mz - 1) * pb;
             c[e] = fload<FT>(rcp, voc, so);
             yv[e] = hy ? bload<float>(ry, voc, so) : 0.f;
         }
 #pragma unroll
-        for (int e = 0; e < kLB; ++e) {
+        for (int e = 0; e < LB; ++e) {
             const int k = k1 - e;
             delta = k < mz ? dq[k] - c[e] * delta : 0.f;
             if (t.valid && k < mz) bstore(yv[e] + w * delta, ro, voc, (unsigned)k * pb);
