@@ -82,7 +82,7 @@ struct windhip_solver {
     double *B, *C, *H, *I, *L, *M;
     float *B32, *C32, *H32, *I32, *L32, *M32;
     // bf16 copies of B32..M32 for the multigrid inner solve, refreshed after assembly
-    unsigned short *Bh = nullptr, *Ch = nullptr, *Hh = nullptr, *Ih = nullptr, *Lh = nullptr, *Mh = nullptr;
+    unsigned *Qh = nullptr;  // bf16 coefficient rows of the multigrid inner solve, 3 words per row (OpAB)
     bool coef_bf16_stale = true;
     double *bcx, *bcy, *bcz, *dztop;
     float *bcx32, *bcy32, *bcz32, *dztop32;
@@ -254,6 +254,9 @@ struct windhip_solver {
     OpA<T> opa() const;
     OpAB opab() const;
     int refresh_coef_bf16();
+    // the multigrid inner solve reads bf16 coefficient rows (OpAB) unless the
+    // operator was probed or its 12-byte rows exceed one buffer resource
+    bool use_bf16_rows() const { return !probed && N * 12 < (1L << 32); }
     template <typename T>
     OpP<T> opp() const;
     // call f with the operator in use (analytic or probed), chosen at compile time in the kernels
@@ -372,7 +375,7 @@ OpAB windhip_solver::opab() const
 {
     OpAB o;
     static_cast<OpA<float> &>(o) = opa<float>();
-    o.Bh = Bh; o.Ch = Ch; o.Hh = Hh; o.Ih = Ih; o.Lh = Lh; o.Mh = Mh;
+    o.Q = Qh;
     return o;
 }
 template <>
@@ -503,11 +506,8 @@ int windhip_solver::alloc_f32_krylov()
 int windhip_solver::refresh_coef_bf16()
 {
     if (!coef_bf16_stale) return WINDHIP_OK;
-    if (!Bh)
-        for (unsigned short **a : {&Bh, &Ch, &Hh, &Ih, &Lh, &Mh}) CHK(alloc(a, N));
-    const float *src[6] = {B32, C32, H32, I32, L32, M32};
-    unsigned short *dst[6] = {Bh, Ch, Hh, Ih, Lh, Mh};
-    for (int q = 0; q < 6; ++q) k_to_bf16<<<flat_grid(N), kBlock, 0, stream>>>(src[q], dst[q], N);
+    if (!Qh) CHK(alloc(&Qh, 3 * N));
+    k_to_bf16_rows<<<flat_grid(N), kBlock, 0, stream>>>(B32, C32, H32, I32, L32, M32, Qh, N);
     HIPCHK(hipGetLastError());
     coef_bf16_stale = false;
     return WINDHIP_OK;
@@ -782,7 +782,7 @@ int windhip_solver::enqueue_iterations_line(int count, bool multigrid)
     };
     // the multigrid inner solve runs on the bf16-coefficient operator (the
     // rows are refreshed by run_mixed before any batch is enqueued or captured)
-    if (multigrid && !probed) return body(opab());
+    if (multigrid && use_bf16_rows()) return body(opab());
     return with_op<float>(body);
 }
 
@@ -1165,7 +1165,7 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
         HIPCHK(hipGetLastError());
         if (multigrid) {
             CHK(build_mg());
-            if (!probed) CHK(refresh_coef_bf16());
+            if (use_bf16_rows()) CHK(refresh_coef_bf16());
             if (!lf_lo16)
                 for (unsigned short **a : {&lf_lo16, &lf_cp16, &lf_ip16}) CHK(alloc(a, N));
             const float *src[3] = {lf_lo, lf_cp, lf_ip};

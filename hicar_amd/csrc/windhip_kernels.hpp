@@ -251,38 +251,56 @@ struct OpA {
 // inner solve, whose 1e-1 inner tolerance is far above the 4e-3 relative
 // rounding, at 12 instead of 24 bytes of coefficients per cell. The FP64 outer
 // solve and its residual keep the FP64 operator. Byte offsets passed in are
-// those of the FP32 rows (4 bytes per element) and are halved here.
+// those of the FP32 rows (4 bytes per element) and are scaled here (x3 for the
+// 12-byte interleaved rows, /2 for a plain bf16 row).
 struct OpAB : OpA<float> {
-    const unsigned short *Bh, *Ch, *Hh, *Ih, *Lh, *Mh;
+    // the six coefficient rows of a level row interleaved as bf16 pairs, 12
+    // bytes per row: {B | C << 16, H | I << 16, L | M << 16}
+    const unsigned *Q;
     static __device__ __forceinline__ float bf(unsigned short h) { return __builtin_bit_cast(float, (unsigned)h << 16); }
+    static __device__ __forceinline__ float lo(unsigned w) { return __builtin_bit_cast(float, w << 16); }
+    static __device__ __forceinline__ float hi(unsigned w) { return __builtin_bit_cast(float, w & 0xFFFF0000u); }
     __device__ __forceinline__ Cf load(long n) const
     {
-        return Cf{bf(Bh[n]), bf(Ch[n]), bf(Hh[n]), bf(Ih[n]), bf(Lh[n]), bf(Mh[n])};
+        const unsigned *p = Q + 3 * n;
+        return Cf{lo(p[0]), hi(p[0]), lo(p[1]), hi(p[1]), lo(p[2]), hi(p[2])};
     }
-    struct Src { rsrc_t B, C, H, I, L, M; };
-    __device__ __forceinline__ Src src(long N) const
-    {
-        const long nb = N * 2;
-        return Src{make_rsrc(Bh, nb), make_rsrc(Ch, nb), make_rsrc(Hh, nb),
-                   make_rsrc(Ih, nb), make_rsrc(Lh, nb), make_rsrc(Mh, nb)};
-    }
+    // one bf16 element of a plain bf16 row (the line solve's factor rows)
     static __device__ __forceinline__ float ld(rsrc_t r, unsigned vo, unsigned so)
     {
         return bf(__builtin_amdgcn_raw_buffer_load_b16(r, (int)(vo >> 1), uni(so >> 1), 0));
     }
+    struct Src { rsrc_t q; };
+    __device__ __forceinline__ Src src(long N) const { return Src{make_rsrc(Q, N * 12)}; }
+    // one 12-byte load per lane: the row's offset is 3x its FP32 offset
     __device__ __forceinline__ Cf load_src(const Src &r, long, unsigned so, unsigned, unsigned vo) const
     {
-        return Cf{ld(r.B, vo, so), ld(r.C, vo, so), ld(r.H, vo, so), ld(r.I, vo, so), ld(r.L, vo, so),
-                  ld(r.M, vo, so)};
+        typedef unsigned u32x3 __attribute__((ext_vector_type(3)));
+        const u32x3 v = __builtin_amdgcn_raw_buffer_load_b96(r.q, (int)(vo * 3u), uni(so * 3u), 0);
+        return Cf{lo(v.x), hi(v.x), lo(v.y), hi(v.y), lo(v.z), hi(v.z)};
     }
 };
 
 // FP32 -> bf16, round to nearest even (finite values)
+__device__ __forceinline__ unsigned bf16_rne(float v)
+{
+    const unsigned u = __builtin_bit_cast(unsigned, v);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
 __global__ void k_to_bf16(const float *__restrict__ src, unsigned short *__restrict__ dst, long n)
 {
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+        dst[e] = (unsigned short)bf16_rne(src[e]);
+}
+// the six FP32 coefficient rows -> the interleaved bf16 rows of OpAB
+__global__ void k_to_bf16_rows(const float *__restrict__ B, const float *__restrict__ C, const float *__restrict__ H,
+                               const float *__restrict__ I, const float *__restrict__ L, const float *__restrict__ M,
+                               unsigned *__restrict__ Q, long n)
+{
     for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-        const unsigned u = __builtin_bit_cast(unsigned, src[e]);
-        dst[e] = (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+        Q[3 * e] = bf16_rne(B[e]) | (bf16_rne(C[e]) << 16);
+        Q[3 * e + 1] = bf16_rne(H[e]) | (bf16_rne(I[e]) << 16);
+        Q[3 * e + 2] = bf16_rne(L[e]) | (bf16_rne(M[e]) << 16);
     }
 }
 
