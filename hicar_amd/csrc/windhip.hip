@@ -126,6 +126,7 @@ struct windhip_solver {
     float *lf_lo = nullptr, *lf_cp = nullptr, *lf_ip = nullptr, *lf_dq = nullptr, *lf_r = nullptr;
     // bf16 copies of the line factors for the multigrid fine level
     unsigned short *lf_lo16 = nullptr, *lf_cp16 = nullptr, *lf_ip16 = nullptr;
+    unsigned *lf_lip16 = nullptr;  // {lo | ip << 16} bf16 pairs (columns of up to 44 levels)
     // MIXED_MG: the coarse levels of the semi-coarsening hierarchy (the fine
     // level is the solver's grid with the line factors lf_*): averaged
     // metrics, the rediscretised analytic operator (FP64 and FP32), its BC
@@ -707,8 +708,8 @@ int windhip_solver::enqueue_iterations_line(int count, bool multigrid)
         // slightly perturbed column system), the line arm as FP32
         auto lsolve = [&](const float *in, const float *y, float *out, float w) {
             if (multigrid && g.mz <= 44)
-                k_line_solve_r<44, unsigned short, 4><<<cb, kColThreads, 0, stream>>>(g, sp, in, lf_lo16, lf_cp16,
-                                                                                      lf_ip16, y, out, w);
+                k_line_solve_r<44, unsigned, 4><<<cb, kColThreads, 0, stream>>>(
+                    g, sp, in, lf_lip16, reinterpret_cast<const unsigned *>(lf_cp16), nullptr, y, out, w);
             else if (multigrid && g.mz <= 48)
                 k_line_solve_r<48, unsigned short><<<cb, kColThreads, 0, stream>>>(g, sp, in, lf_lo16, lf_cp16,
                                                                                    lf_ip16, y, out, w);
@@ -1166,11 +1167,19 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
         if (multigrid) {
             CHK(build_mg());
             if (use_bf16_rows()) CHK(refresh_coef_bf16());
-            if (!lf_lo16)
-                for (unsigned short **a : {&lf_lo16, &lf_cp16, &lf_ip16}) CHK(alloc(a, N));
-            const float *src[3] = {lf_lo, lf_cp, lf_ip};
-            unsigned short *dst[3] = {lf_lo16, lf_cp16, lf_ip16};
-            for (int q = 0; q < 3; ++q) k_to_bf16<<<flat_grid(N), kBlock, 0, stream>>>(src[q], dst[q], N);
+            // bf16 factors of the fine-level line solve: a {lo | ip} pair row
+            // and a cp row for columns of up to 44 levels, three rows above
+            if (!lf_cp16) CHK(alloc(&lf_cp16, N));
+            k_to_bf16<<<flat_grid(N), kBlock, 0, stream>>>(lf_cp, lf_cp16, N);
+            if (g.mz <= 44) {
+                if (!lf_lip16) CHK(alloc(&lf_lip16, N));
+                k_to_bf16_pair<<<flat_grid(N), kBlock, 0, stream>>>(lf_lo, lf_ip, lf_lip16, N);
+            } else {
+                if (!lf_lo16)
+                    for (unsigned short **a : {&lf_lo16, &lf_ip16}) CHK(alloc(a, N));
+                k_to_bf16<<<flat_grid(N), kBlock, 0, stream>>>(lf_lo, lf_lo16, N);
+                k_to_bf16<<<flat_grid(N), kBlock, 0, stream>>>(lf_ip, lf_ip16, N);
+            }
             HIPCHK(hipGetLastError());
         }
     }

@@ -292,6 +292,13 @@ __global__ void k_to_bf16(const float *__restrict__ src, unsigned short *__restr
     for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
         dst[e] = (unsigned short)bf16_rne(src[e]);
 }
+// two FP32 rows -> one row of bf16 pairs {a | b << 16}
+__global__ void k_to_bf16_pair(const float *__restrict__ a, const float *__restrict__ b, unsigned *__restrict__ dst,
+                               long n)
+{
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+        dst[e] = bf16_rne(a[e]) | (bf16_rne(b[e]) << 16);
+}
 // the six FP32 coefficient rows -> the interleaved bf16 rows of OpAB
 __global__ void k_to_bf16_rows(const float *__restrict__ B, const float *__restrict__ C, const float *__restrict__ H,
                                const float *__restrict__ I, const float *__restrict__ L, const float *__restrict__ M,
@@ -1492,6 +1499,29 @@ __device__ __forceinline__ float fload(rsrc_t r, unsigned vo, unsigned so)
     if constexpr (sizeof(FT) == 2) return OpAB::ld(r, vo, so);
     else return bload<float>(r, vo, so);
 }
+// The factor rows of k_line_solve_r: FT = float, three FP32 rows; FT =
+// unsigned short, three bf16 rows; FT = unsigned, one {lo | ip << 16} bf16 pair
+// per row (lo holds it) beside a bf16 cp row (cp), so the forward pass loads
+// both of its factors with one 32-bit load.
+template <typename FT>
+__device__ __forceinline__ void fload_lq(rsrc_t rlo, rsrc_t rip, unsigned vo, unsigned so, float &l, float &q)
+{
+    if constexpr (std::is_same<FT, unsigned>::value) {
+        const unsigned w = __builtin_amdgcn_raw_buffer_load_b32(rlo, (int)vo, uni(so), 0);
+        l = OpAB::lo(w);
+        q = OpAB::hi(w);
+    } else {
+        l = fload<FT>(rlo, vo, so);
+        q = fload<FT>(rip, vo, so);
+    }
+}
+template <typename FT>
+__device__ __forceinline__ float fload_c(rsrc_t rcp, unsigned vo, unsigned so)
+{
+    if constexpr (std::is_same<FT, unsigned>::value) return OpAB::ld(rcp, vo, so);
+    else return fload<FT>(rcp, vo, so);
+}
+
 template <int KM, typename FT = float, int LB = kLB>
 __global__ __launch_bounds__(kColThreads) void k_line_solve_r(Geo g, const KState *st, const float *__restrict__ in,
                                                               const FT *__restrict__ lo, const FT *__restrict__ cp,
@@ -1505,9 +1535,10 @@ __global__ __launch_bounds__(kColThreads) void k_line_solve_r(Geo g, const KStat
     const int mz = __builtin_amdgcn_readfirstlane(g.mz);
     const long N = g.P * mz;
     const unsigned voc = t.oc * 4u, pb = (unsigned)(g.P * 4);
+    constexpr bool kPair = std::is_same<FT, unsigned>::value;
     const long nbt = N * 4, nbf = N * (long)sizeof(FT);
-    const rsrc_t rin = make_rsrc(in, nbt), rlo = make_rsrc(lo, nbf), rip = make_rsrc(ip, nbf),
-                 rcp = make_rsrc(cp, nbf), ro = make_rsrc(out, nbt);
+    const rsrc_t rin = make_rsrc(in, nbt), rlo = make_rsrc(lo, nbf), rip = kPair ? rlo : make_rsrc(ip, nbf),
+                 rcp = make_rsrc(cp, kPair ? N * 2 : nbf), ro = make_rsrc(out, nbt);
     const bool hy = y != nullptr;
     const rsrc_t ry = make_rsrc(hy ? y : in, nbt);
     float dq[KM];
@@ -1523,15 +1554,14 @@ __global__ __launch_bounds__(kColThreads) void k_line_solve_r(Geo g, const KStat
             for (int e = 0; e < LB; ++e) {
                 const unsigned so = (unsigned)(k0 + e < mz ? k0 + e : mz - 1) * pb;
                 a[sl][e] = bload<float>(rin, voc, so);
-                l[sl][e] = fload<FT>(rlo, voc, so);
-                q[sl][e] = fload<FT>(rip, voc, so);
+                fload_lq<FT>(rlo, rip, voc, so, l[sl][e], q[sl][e]);
             }
         };
         auto ldb = [&](int k1, int sl) {
 #pragma unroll
             for (int e = 0; e < LB; ++e) {
                 const unsigned so = (unsigned)(k1 - e < mz ? k1 - e : mz - 1) * pb;
-                c[sl][e] = fload<FT>(rcp, voc, so);
+                c[sl][e] = fload_c<FT>(rcp, voc, so);
                 yv[sl][e] = hy ? bload<float>(ry, voc, so) : 0.f;
             }
         };
@@ -1571,8 +1601,7 @@ __global__ __launch_bounds__(kColThreads) void k_line_solve_r(Geo g, const KStat
         for (int e = 0; e < LB; ++e) {
             const unsigned so = (unsigned)(k0 + e < mz ? k0 + e : mz - 1) * pb;
             a[e] = bload<float>(rin, voc, so);
-            l[e] = fload<FT>(rlo, voc, so);
-            q[e] = fload<FT>(rip, voc, so);
+            fload_lq<FT>(rlo, rip, voc, so, l[e], q[e]);
         }
 #pragma unroll
         for (int e = 0; e < LB; ++e) {
@@ -1586,7 +1615,7 @@ __global__ __launch_bounds__(kColThreads) void k_line_solve_r(Geo g, const KStat
 #pragma unroll
         for (int e = 0; e < LB; ++e) {
             const unsigned so = (unsigned)(k1 - e < mz ? k1 - e : mz - 1) * pb;
-            c[e] = fload<FT>(rcp, voc, so);
+            c[e] = fload_c<FT>(rcp, voc, so);
             yv[e] = hy ? bload<float>(ry, voc, so) : 0.f;
         }
 #pragma unroll
