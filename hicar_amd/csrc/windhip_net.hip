@@ -70,10 +70,15 @@ __device__ __forceinline__ int kpad(int k) { return (k + 31) / 32 * 32; }
 // 16 outputs keeps 16 accumulators, not 64.
 constexpr int kTx = 16, kTy = 4, kTz = 4;
 constexpr int kHx = kTx + 2, kHy = kTy + 2, kHz = kTz + 2, kHv = kHx * kHy * kHz;
+// With X2 set, input channels below csplit are the nearest 2x upsampling of X2
+// (shape s.half(), row stride ldx2), read straight from it, and channel c >=
+// csplit is channel c - csplit of X: the decoder's upsample + concatenation
+// without writing the upsampled channels out (csplit a multiple of CC).
 template <int CC, int NB>
 __global__ __launch_bounds__(kT) void k_conv3_lds(Shape s, const bf16 *__restrict__ X, int ldx, int C,
                                                  const bf16 *__restrict__ Wt, int Kp, const float *__restrict__ bias,
-                                                 int N, bf16 *__restrict__ Y, int ldy, int relu)
+                                                 int N, bf16 *__restrict__ Y, int ldy, int relu,
+                                                 const bf16 *__restrict__ X2, int ldx2, int csplit)
 {
     __shared__ bf16x8 tile[(CC / 8) * kHv];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -106,7 +111,10 @@ __global__ __launch_bounds__(kT) void k_conv3_lds(Shape s, const bf16 *__restric
     // are all issued before its LDS stores, so the staging costs one memory
     // round trip per batch instead of one per entry
     constexpr int NQ = ((CC / 8) * kHv + kT - 1) / kT, kQB = NQ < 6 ? NQ : 6;
+    const Shape hs = s.half();
     for (int c0 = 0; c0 < C; c0 += CC) {
+        const bool up = X2 != nullptr && c0 < csplit;  // this chunk comes from X2, upsampled
+        const int cx = X2 != nullptr ? c0 - csplit : c0;
         __syncthreads();  // the previous chunk's reads are done
 #pragma unroll 1
         for (int i0 = 0; i0 < NQ; i0 += kQB) {
@@ -119,9 +127,14 @@ __global__ __launch_bounds__(kT) void k_conv3_lds(Shape s, const bf16 *__restric
                 const int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
                 val[e] = zero8;
                 if (i0 + e < NQ && q < (CC / 8) * kHv && x >= 0 && x < s.W && y >= 0 && y < s.H && z >= 0 &&
-                    z < s.D)
-                    val[e] = *reinterpret_cast<const bf16x8 *>(X + (vb + z * HW + (long)y * s.W + x) * ldx + c0 +
-                                                               cg * 8);
+                    z < s.D) {
+                    if (up)
+                        val[e] = *reinterpret_cast<const bf16x8 *>(
+                            X2 + ((((long)bb * hs.D + z / 2) * hs.H + y / 2) * hs.W + x / 2) * ldx2 + c0 + cg * 8);
+                    else
+                        val[e] = *reinterpret_cast<const bf16x8 *>(X + (vb + z * HW + (long)y * s.W + x) * ldx + cx +
+                                                                   cg * 8);
+                }
             }
 #pragma unroll
             for (int e = 0; e < kQB; ++e) {
@@ -180,27 +193,30 @@ __global__ __launch_bounds__(kT) void k_conv3_lds(Shape s, const bf16 *__restric
 // C and as many 16-channel output blocks per block as N needs (up to 4).
 template <int CC>
 static void conv3_cc(hipStream_t st, Shape s, const bf16 *X, int ldx, int C, const bf16 *Wt, int Kp,
-                     const float *bias, int N, bf16 *Y, int ldy, int relu)
+                     const float *bias, int N, bf16 *Y, int ldy, int relu, const bf16 *X2, int ldx2, int csplit)
 {
     const int ntx = (s.W + kTx - 1) / kTx, nty = (s.H + kTy - 1) / kTy, ntz = (s.D + kTz - 1) / kTz;
     const unsigned tiles = (unsigned)((long)ntx * nty * ntz * s.B);
     if (N <= 16)
-        k_conv3_lds<CC, 1><<<dim3(tiles, 1), kT, 0, st>>>(s, X, ldx, C, Wt, Kp, bias, N, Y, ldy, relu);
+        k_conv3_lds<CC, 1><<<dim3(tiles, 1), kT, 0, st>>>(s, X, ldx, C, Wt, Kp, bias, N, Y, ldy, relu, X2, ldx2,
+                                                          csplit);
     else if (N <= 32)
-        k_conv3_lds<CC, 2><<<dim3(tiles, 1), kT, 0, st>>>(s, X, ldx, C, Wt, Kp, bias, N, Y, ldy, relu);
+        k_conv3_lds<CC, 2><<<dim3(tiles, 1), kT, 0, st>>>(s, X, ldx, C, Wt, Kp, bias, N, Y, ldy, relu, X2, ldx2,
+                                                          csplit);
     else
         k_conv3_lds<CC, 4><<<dim3(tiles, (unsigned)((N + 63) / 64)), kT, 0, st>>>(s, X, ldx, C, Wt, Kp, bias, N, Y,
-                                                                                   ldy, relu);
+                                                                                   ldy, relu, X2, ldx2, csplit);
 }
+// X2 (optional): the coarse source of input channels [0, csplit), see k_conv3_lds
 static void conv3(hipStream_t st, Shape s, const bf16 *X, int ldx, int C, const bf16 *Wt, int Kp, const float *bias,
-                  int N, bf16 *Y, int ldy, int relu)
+                  int N, bf16 *Y, int ldy, int relu, const bf16 *X2 = nullptr, int ldx2 = 0, int csplit = 0)
 {
-    if (C % 32 == 0)
-        conv3_cc<32>(st, s, X, ldx, C, Wt, Kp, bias, N, Y, ldy, relu);
-    else if (C % 16 == 0)
-        conv3_cc<16>(st, s, X, ldx, C, Wt, Kp, bias, N, Y, ldy, relu);
+    if (C % 32 == 0 && csplit % 32 == 0)
+        conv3_cc<32>(st, s, X, ldx, C, Wt, Kp, bias, N, Y, ldy, relu, X2, ldx2, csplit);
+    else if (C % 16 == 0 && csplit % 16 == 0)
+        conv3_cc<16>(st, s, X, ldx, C, Wt, Kp, bias, N, Y, ldy, relu, X2, ldx2, csplit);
     else
-        conv3_cc<8>(st, s, X, ldx, C, Wt, Kp, bias, N, Y, ldy, relu);
+        conv3_cc<8>(st, s, X, ldx, C, Wt, Kp, bias, N, Y, ldy, relu, X2, ldx2, csplit);
 }
 
 // Weight-gradient partials of one tap over one slab of voxels:
@@ -594,10 +610,14 @@ struct windhip_net {
     }
     int ensure(Shape s);
     int prepare_weights(hipStream_t st);
-    void conv(hipStream_t st, int l, Shape s, const bf16 *X, int ldx, bf16 *Y, int ldy);
+    void conv(hipStream_t st, int l, Shape s, const bf16 *X, int ldx, bf16 *Y, int ldy, const bf16 *X2 = nullptr,
+              int ldx2 = 0, int csplit = 0);
     void conv_back(hipStream_t st, int l, Shape s, const bf16 *G, int ldg, const bf16 *Yout, int ldy,
                    const bf16 *X, int ldx, bf16 *dX, int lddx);
-    int forward(hipStream_t st, const float *in, Shape s, float *o);
+    // infer: the decoder convolutions read the upsampled channels from the
+    // coarse tensors directly (no backward pass follows, so the concatenated
+    // inputs need not exist)
+    int forward(hipStream_t st, const float *in, Shape s, float *o, bool infer = false);
     int backward(hipStream_t st, Shape s);
 };
 
@@ -650,10 +670,11 @@ int windhip_net::prepare_weights(hipStream_t st)
     return WINDHIP_OK;
 }
 
-void windhip_net::conv(hipStream_t st, int l, Shape s, const bf16 *X, int ldx, bf16 *Y, int ldy)
+void windhip_net::conv(hipStream_t st, int l, Shape s, const bf16 *X, int ldx, bf16 *Y, int ldy, const bf16 *X2,
+                       int ldx2, int csplit)
 {
     const auto &c = kConvs[l];
-    conv3(st, s, X, ldx, c.ci, wf[l], kpf[l], params + boff[l], c.co, Y, ldy, 1);
+    conv3(st, s, X, ldx, c.ci, wf[l], kpf[l], params + boff[l], c.co, Y, ldy, 1, X2, ldx2, csplit);
 }
 
 // Backward of conv l (output Yout after ReLU, gradient G w.r.t. Yout): weight
@@ -673,7 +694,7 @@ void windhip_net::conv_back(hipStream_t st, int l, Shape s, const bf16 *G, int l
     if (dX) conv3(st, s, gtmp, c.co, c.co, wb[l], kpb[l], nullptr, c.ci, dX, lddx, 0);
 }
 
-int windhip_net::forward(hipStream_t st, const float *in, Shape s, float *o)
+int windhip_net::forward(hipStream_t st, const float *in, Shape s, float *o, bool infer)
 {
     NCHK(ensure(s));
     NCHK(prepare_weights(st));
@@ -692,14 +713,28 @@ int windhip_net::forward(hipStream_t st, const float *in, Shape s, float *o)
     k_pool8<<<grid1(s3.voxels() * 8), kT, 0, st>>>(s2, 64, cat2 + 64, 128, p3, 64);
     conv(st, C3A, s3, p3, 64, a3, 64);
     conv(st, C3B, s3, a3, 64, b3, 64);
-    k_up8<<<grid1(s2.voxels() * 8), kT, 0, st>>>(s2, 64, b3, 64, cat2, 128);
-    conv(st, D2A, s2, cat2, 128, e2a, 64);
+    // decoder: upsample + concatenate [up | skip] + convolve; for inference the
+    // convolution reads the up channels from the coarse tensor itself
+    if (infer) {
+        conv(st, D2A, s2, cat2 + 64, 128, e2a, 64, b3, 64, 64);
+    } else {
+        k_up8<<<grid1(s2.voxels() * 8), kT, 0, st>>>(s2, 64, b3, 64, cat2, 128);
+        conv(st, D2A, s2, cat2, 128, e2a, 64);
+    }
     conv(st, D2B, s2, e2a, 64, e2b, 64);
-    k_up8<<<grid1(s1.voxels() * 8), kT, 0, st>>>(s1, 64, e2b, 64, cat1, 96);
-    conv(st, D1A, s1, cat1, 96, e1a, 32);
+    if (infer) {
+        conv(st, D1A, s1, cat1 + 64, 96, e1a, 32, e2b, 64, 64);
+    } else {
+        k_up8<<<grid1(s1.voxels() * 8), kT, 0, st>>>(s1, 64, e2b, 64, cat1, 96);
+        conv(st, D1A, s1, cat1, 96, e1a, 32);
+    }
     conv(st, D1B, s1, e1a, 32, e1b, 32);
-    k_up8<<<grid1(m0 * 4), kT, 0, st>>>(s, 32, e1b, 32, cat0, 48);
-    conv(st, D0A, s, cat0, 48, e0a, 16);
+    if (infer) {
+        conv(st, D0A, s, cat0 + 32, 48, e0a, 16, e1b, 32, 32);
+    } else {
+        k_up8<<<grid1(m0 * 4), kT, 0, st>>>(s, 32, e1b, 32, cat0, 48);
+        conv(st, D0A, s, cat0, 48, e0a, 16);
+    }
     conv(st, D0B, s, e0a, 16, e0b, 16);
     k_head<<<grid1(m0), kT, 0, st>>>(m0, e0b, params + hoff, o ? o : out);
     NHIP(hipGetLastError());
@@ -746,7 +781,7 @@ int windhip_net::backward(hipStream_t st, Shape s)
 
 int wnet_forward_dev(windhip_net *net, hipStream_t stream, const float *in, wnet::Shape s, float *out)
 {
-    return net->forward(stream, in, s, out);
+    return net->forward(stream, in, s, out, true);
 }
 
 // ------------------------------------------------------------------ C-ABI
@@ -832,7 +867,7 @@ int windhip_net_get_params(windhip_net *n, float *params)
 int windhip_net_forward(windhip_net *n, const float *in_dev, int B, int D, int H, int W, float *out_dev)
 {
     if (!n || !in_dev || !out_dev) return WINDHIP_ERR_ARG;
-    NCHK(n->forward(n->stream, in_dev, Shape{B, D, H, W}, out_dev));
+    NCHK(n->forward(n->stream, in_dev, Shape{B, D, H, W}, out_dev, true));
     return n->hip(hipStreamSynchronize(n->stream), "forward");
 }
 
@@ -879,9 +914,9 @@ int windhip_net_time_forward(windhip_net *n, int B, int D, int H, int W, int rep
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    int rc = n->forward(n->stream, in, s, nullptr);
+    int rc = n->forward(n->stream, in, s, nullptr, true);
     (void)hipEventRecord(e0, n->stream);
-    for (int q = 0; q < reps && !rc; ++q) rc = n->forward(n->stream, in, s, nullptr);
+    for (int q = 0; q < reps && !rc; ++q) rc = n->forward(n->stream, in, s, nullptr, true);
     (void)hipEventRecord(e1, n->stream);
     (void)hipEventSynchronize(e1);
     float t = 0.f;
