@@ -141,6 +141,11 @@ struct windhip_solver {
         double *B, *C, *H, *I, *L, *M, *bcx, *bcy, *bcz, *dztop;
         float *B32, *C32, *H32, *I32, *L32, *M32, *bcx32, *bcy32, *bcz32, *dztop32;
         float *lo, *cp, *ip, *x, *b, *r, *t;
+        // bf16 copies for the two-kernel smoothing step (levels of up to 44
+        // rows per column): interleaved coefficient rows (OpAB), {lo | ip}
+        // factor pairs and the cp factor row
+        unsigned *Q, *lip;
+        unsigned short *cp16;
     };
     std::vector<MgLevel> mg;
     int mg_pre = 2, mg_post = 2;    // fine-level line-smoothing steps before and after the coarse correction
@@ -337,7 +342,10 @@ struct windhip_solver {
     int enqueue_iterations_line(int count, bool multigrid);
     int build_mg();
     float *mg_coarse(size_t l);
-    void mg_smooth(size_t l, bool wave, const float *y, float *out);
+    // form 0: residual + line solve (FP32 rows); 1: one launch, a wavefront
+    // per column; 2: residual + line solve on the level's bf16 rows
+    void mg_smooth(size_t l, int form, const float *y, float *out);
+    OpAB level_opab(const MgLevel &L) const;
     int alloc_f32_krylov();
     int alloc_fgmres(int m);
     int correction(void *u, void *v, void *w, const void *density, int adv_den);
@@ -398,6 +406,13 @@ OpA<float> windhip_solver::level_op<float>(const MgLevel &L) const
     const double inv = 1.0 / (L.d.dx * L.d.dx);
     o.inv = (float)inv;
     o.neg4 = (float)(-4.0 * inv);
+    return o;
+}
+OpAB windhip_solver::level_opab(const MgLevel &L) const
+{
+    OpAB o;
+    static_cast<OpA<float> &>(o) = level_op<float>(L);
+    o.Q = L.Q;
     return o;
 }
 template <>
@@ -791,29 +806,36 @@ int windhip_solver::enqueue_iterations_line(int count, bool multigrid)
 // with b = mg[l].b (y null: out = w L^-1 b): in one launch, a wavefront per
 // column (k_mg_smooth_w), when `wave`; else a residual into mg[l].r and a
 // line solve.
-void windhip_solver::mg_smooth(size_t l, bool wave, const float *y, float *out)
+void windhip_solver::mg_smooth(size_t l, int form, const float *y, float *out)
 {
     MgLevel &L = mg[l];
     const KState *sp = st + 1;
     const Geo &gl = L.gg;
-    if (wave && gl.mz <= 64) {
+    if (form == 1 && gl.mz <= 64) {
         timed(PK_MG_COARSE, [&] {
             k_mg_smooth_w<OpA<float>><<<(int)((L.g.P + kWaveCols - 1) / kWaveCols), 64 * kWaveCols, 0, stream>>>(
                 level_op<float>(L), gl, sp, L.b, y, L.lo, L.cp, L.ip, out, mg_omega);
         });
         return;
     }
+    const bool half = form == 2 && L.Q;
     const float *in = L.b;
     if (y) {
         timed(PK_MG_COARSE, [&] {
-            k_resid_c<OpA<float>><<<chunk_blocks(gl), kColThreads, 0, stream>>>(level_op<float>(L), gl, sp, y, L.b,
-                                                                               L.r);
+            if (half)
+                k_resid_c<OpAB><<<chunk_blocks(gl), kColThreads, 0, stream>>>(level_opab(L), gl, sp, y, L.b, L.r);
+            else
+                k_resid_c<OpA<float>><<<chunk_blocks(gl), kColThreads, 0, stream>>>(level_op<float>(L), gl, sp, y,
+                                                                                   L.b, L.r);
         });
         in = L.r;
     }
     const int cb = col_blocks(gl);
     timed(PK_MG_COARSE, [&] {
-        if (gl.mz <= 48)
+        if (half)
+            k_line_solve_r<44, unsigned, 4><<<cb, kColThreads, 0, stream>>>(
+                gl, sp, in, L.lip, reinterpret_cast<const unsigned *>(L.cp16), nullptr, y, out, mg_omega);
+        else if (gl.mz <= 48)
             k_line_solve_r<48><<<cb, kColThreads, 0, stream>>>(gl, sp, in, L.lo, L.cp, L.ip, y, out, mg_omega);
         else if (gl.mz <= 64)
             k_line_solve_r<64><<<cb, kColThreads, 0, stream>>>(gl, sp, in, L.lo, L.cp, L.ip, y, out, mg_omega);
@@ -842,8 +864,15 @@ float *windhip_solver::mg_coarse(size_t l)
                 k_mg_line<<<mg_blocks(L.g.P), kMgThreads, 0, stream>>>(L.g, sp, L.lo, L.cp, L.ip, in, y, out, w);
         });
     };
+    // the two-kernel levels smooth on their bf16 rows when they have them
+    const int form = wave ? 1 : (L.Q ? 2 : 0);
     auto resid = [&](const float *xv) {
-        timed(PK_MG_COARSE, [&] { k_resid_c<OpA<float>><<<kb, kColThreads, 0, stream>>>(op, gl, sp, xv, L.b, L.r); });
+        timed(PK_MG_COARSE, [&] {
+            if (form == 2)
+                k_resid_c<OpAB><<<kb, kColThreads, 0, stream>>>(level_opab(L), gl, sp, xv, L.b, L.r);
+            else
+                k_resid_c<OpA<float>><<<kb, kColThreads, 0, stream>>>(op, gl, sp, xv, L.b, L.r);
+        });
     };
     size_t used = mg.size();
     if (mg_depth > 0) {
@@ -861,16 +890,16 @@ float *windhip_solver::mg_coarse(size_t l)
         return L.x;
     }
     float *cur = L.x, *alt = L.t;
-    mg_smooth(l, wave, nullptr, cur);
+    mg_smooth(l, form, nullptr, cur);
     if (l + 1 == used) {  // the coarsest level in use: smoothing only
         for (int q = 1; q < mg_cpre + mg_cpost; ++q) {
-            mg_smooth(l, wave, cur, alt);
+            mg_smooth(l, form, cur, alt);
             std::swap(cur, alt);
         }
         return cur;
     }
     for (int q = 1; q < mg_cpre; ++q) {
-        mg_smooth(l, wave, cur, alt);
+        mg_smooth(l, form, cur, alt);
         std::swap(cur, alt);
     }
     resid(cur);
@@ -881,7 +910,7 @@ float *windhip_solver::mg_coarse(size_t l)
     float *e = mg_coarse(l + 1);
     timed(PK_MG_COARSE, [&] { k_mg_prolong<<<mg_plane_grid(L.g), kMgThreads, 0, stream>>>(L.g, C.g, sp, e, cur); });
     for (int q = 0; q < mg_cpost; ++q) {
-        mg_smooth(l, wave, cur, alt);
+        mg_smooth(l, form, cur, alt);
         std::swap(cur, alt);
     }
     return cur;
@@ -953,6 +982,19 @@ int windhip_solver::build_mg()
             k_line_factor<OpA<double>><<<L.gg.nt, kThreads, 0, stream>>>(level_op<double>(L), L.gg, L.lo, L.cp,
                                                                          L.ip);
         }));
+        if (L.g.mz <= 44) {  // bf16 copies for the two-kernel smoothing step
+            const long n = L.g.N;
+            if (!L.Q) {
+                CHK(alloc(&L.Q, 3 * n));
+                CHK(alloc(&L.lip, n));
+                CHK(alloc(&L.cp16, n));
+            }
+            CHK(timed(PK_MG_SETUP, [&] {
+                k_to_bf16_rows<<<flat_grid(n), kBlock, 0, stream>>>(L.B32, L.C32, L.H32, L.I32, L.L32, L.M32, L.Q, n);
+                k_to_bf16_pair<<<flat_grid(n), kBlock, 0, stream>>>(L.lo, L.ip, L.lip, n);
+                k_to_bf16<<<flat_grid(n), kBlock, 0, stream>>>(L.cp, L.cp16, n);
+            }));
+        }
     }
     HIPCHK(hipGetLastError());
     return WINDHIP_OK;
@@ -1975,6 +2017,7 @@ int windhip_mg_smooth(windhip_solver *s, int level, int fused, const float *b, c
     if (level < 1 || level > (int)s->mg.size())
         return fail(WINDHIP_ERR_ARG, "windhip_mg_smooth: level %d outside 1..%d (run a MIXED_MG solve first)", level,
                     (int)s->mg.size());
+    if (fused < 0 || fused > 2) return fail(WINDHIP_ERR_ARG, "windhip_mg_smooth: form %d outside 0..2", fused);
     auto &L = s->mg[level - 1];
     const size_t bytes = sizeof(float) * L.g.N;
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -1982,7 +2025,7 @@ int windhip_mg_smooth(windhip_solver *s, int level, int fused, const float *b, c
     if (y) HIPCHK(hipMemcpy(L.x, y, bytes, hipMemcpyHostToDevice));
     windhip::k_state_reset<<<1, 1, 0, s->stream>>>(s->st + 1, 1, 0.0, 0.0, 0.0);  // a live state: done = 0
     HIPCHK(hipGetLastError());
-    s->mg_smooth(level - 1, fused != 0, y ? L.x : nullptr, L.t);
+    s->mg_smooth(level - 1, fused, y ? L.x : nullptr, L.t);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
     HIPCHK(hipMemcpy(out, L.t, bytes, hipMemcpyDeviceToHost));

@@ -231,10 +231,12 @@ class WindSolver:
         nat.check(self.lib.windhip_mg_level(self.h, int(level), dims, out.ctypes.data_as(C.POINTER(C.c_float))))
         return (nx, ny, mz), out
 
-    def mg_smooth(self, level: int, fused: bool, b, y=None):
+    def mg_smooth(self, level: int, fused, b, y=None):
         """One damped line-smoothing step on coarse level `level` of the last MIXED_MG
         solve: y + omega L^-1 (b - A y), FP32 [mz, ny+2, nx+2] (no counterpart in
-        the reference)."""
+        the reference). fused: True or 1, the one-launch form; False or 0, the
+        residual + line-solve kernels on FP32 rows; 2, those kernels on the
+        level's bf16 rows (levels of up to 44 rows per column)."""
         dims = (C.c_int * 3)()
         nat.check(self.lib.windhip_mg_level(self.h, int(level), dims, None))
         shape = (dims[2], dims[1] + 2, dims[0] + 2)
@@ -242,7 +244,7 @@ class WindSolver:
         bb = np.ascontiguousarray(b, dtype=np.float32).reshape(shape)
         yy = None if y is None else np.ascontiguousarray(y, dtype=np.float32).reshape(shape)
         out = np.zeros(shape, dtype=np.float32)
-        nat.check(self.lib.windhip_mg_smooth(self.h, int(level), int(bool(fused)), bb.ctypes.data_as(fpp),
+        nat.check(self.lib.windhip_mg_smooth(self.h, int(level), int(fused), bb.ctypes.data_as(fpp),
                                              None if yy is None else yy.ctypes.data_as(fpp),
                                              out.ctypes.data_as(fpp)))
         return out

@@ -247,8 +247,10 @@ int windhip_mg_level(windhip_solver *s, int level, int *dims_out, float *coef_ho
  * tridiagonal part of the level's operator; host FP32 arrays of the level's N
  * rows (layout of windhip_mg_level). fused = 1: the one-launch kernel the
  * V-cycle uses on small levels (a wavefront per column); 0: the residual and
- * line-solve kernels of the larger levels. Overwrites the level's work
- * vectors. Inspection only; no counterpart in the reference. */
+ * line-solve kernels on the level's FP32 rows and factors; 2: the same kernels
+ * on its bf16 rows and factors, the form the V-cycle uses on the larger levels
+ * of up to 44 rows per column (taller levels: as 0). Overwrites the level's
+ * work vectors. Inspection only; no counterpart in the reference. */
 int windhip_mg_smooth(windhip_solver *s, int level, int fused, const float *b, const float *y, float *out);
 /* The network input for the solver's current rhs (that of its last solve),
  * metrics and alpha, written to in_dev (1 x dims[0] x dims[1] x dims[2] x 8);
