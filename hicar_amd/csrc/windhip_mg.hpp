@@ -49,12 +49,14 @@ inline MgGeo mg_geo(int nx, int ny, int mz)
 constexpr int kMgThreads = 256;
 inline int mg_blocks(long cols) { return (int)((cols + kMgThreads - 1) / kMgThreads); }
 // the plane-by-level grid of the transfer kernels: kPlaneX x kPlaneY columns
-// per block, one grid layer per level
-constexpr int kPlaneX = 64, kPlaneY = kMgThreads / kPlaneX;
+// per block, kPlaneZ levels per thread (their loads issued together: one
+// level per thread left the fine-level transfers waiting one memory round
+// trip per 256 rows), one grid layer per kPlaneZ levels
+constexpr int kPlaneX = 64, kPlaneY = kMgThreads / kPlaneX, kPlaneZ = 4;
 inline dim3 mg_plane_grid(const MgGeo &g)
 {
     return dim3((unsigned)((g.mx + kPlaneX - 1) / kPlaneX), (unsigned)((g.my + kPlaneY - 1) / kPlaneY),
-                (unsigned)g.mz);
+                (unsigned)((g.mz + kPlaneZ - 1) / kPlaneZ));
 }
 
 // Index set of fine columns under coarse column I of a level with nf
@@ -237,24 +239,34 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_restrict(MgGeo f, MgGeo c, co
     // index divisions
     const int I = (int)(blockIdx.x * kPlaneX + (threadIdx.x & (kPlaneX - 1)));
     const int J = (int)(blockIdx.y * kPlaneY + (threadIdx.x / kPlaneX));
-    const int k = (int)blockIdx.z;
+    const int k0 = (int)blockIdx.z * kPlaneZ;
     if (I >= c.mx || J >= c.my) return;
-    const long n = (long)k * c.P + (long)J * c.mx + I;
+    const long n0 = (long)J * c.mx + I;
     if (I == 0 || I == c.mx - 1 || J == 0 || J == c.my - 1) {
-        bc[n] = 0.f;
+#pragma unroll
+        for (int q = 0; q < kPlaneZ; ++q)
+            if (k0 + q < c.mz) bc[(long)(k0 + q) * c.P + n0] = 0.f;
         return;
     }
     const int i0 = 2 * I - 1, j0 = 2 * J - 1;
-    const long m = (long)k * f.P + (long)j0 * f.mx + i0;
     const bool hi = i0 + 1 <= f.nx, hj = j0 + 1 <= f.ny;
-    float s = rf[m];
-    if (hi) s += rf[m + 1];
-    if (hj) {
-        float t = rf[m + f.mx];
-        if (hi) t += rf[m + f.mx + 1];
-        s += t;
+    float sum[kPlaneZ];
+#pragma unroll
+    for (int q = 0; q < kPlaneZ; ++q) {
+        const int k = k0 + q < c.mz ? k0 + q : c.mz - 1;  // levels past the top re-read the top one
+        const long m = (long)k * f.P + (long)j0 * f.mx + i0;
+        float s = rf[m];
+        if (hi) s += rf[m + 1];
+        if (hj) {
+            float t = rf[m + f.mx];
+            if (hi) t += rf[m + f.mx + 1];
+            s += t;
+        }
+        sum[q] = s;
     }
-    bc[n] = s / (float)((hi ? 2 : 1) * (hj ? 2 : 1));
+#pragma unroll
+    for (int q = 0; q < kPlaneZ; ++q)
+        if (k0 + q < c.mz) bc[(long)(k0 + q) * c.P + n0] = sum[q] / (float)((hi ? 2 : 1) * (hj ? 2 : 1));
 }
 
 // x_f += P e_c, one fine row per thread: cell-centred bilinear interpolation
@@ -267,14 +279,23 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_prolong(MgGeo f, MgGeo c, con
     if (st->done) return;
     const int i = (int)(blockIdx.x * kPlaneX + (threadIdx.x & (kPlaneX - 1)));
     const int j = (int)(blockIdx.y * kPlaneY + (threadIdx.x / kPlaneX));
-    const int k = (int)blockIdx.z;
+    const int k0 = (int)blockIdx.z * kPlaneZ;
     if (i <= 0 || i >= f.mx - 1 || j <= 0 || j >= f.my - 1) return;
     const int I = (i + 1) >> 1, J = (j + 1) >> 1;
     const int In = (i & 1) ? I - 1 : I + 1, Jn = (j & 1) ? J - 1 : J + 1;
-    const float *e = ec + (long)k * c.P;
-    const float v = 0.5625f * e[(long)J * c.mx + I] + 0.1875f * (e[(long)J * c.mx + In] + e[(long)Jn * c.mx + I])
-                  + 0.0625f * e[(long)Jn * c.mx + In];
-    xf[(long)k * f.P + (long)j * f.mx + i] += v;
+    const long cc = (long)J * c.mx + I, cx = (long)J * c.mx + In, cy = (long)Jn * c.mx + I, cd = (long)Jn * c.mx + In;
+    const long nf = (long)j * f.mx + i;
+    float v[kPlaneZ], xo[kPlaneZ];
+#pragma unroll
+    for (int q = 0; q < kPlaneZ; ++q) {
+        const int k = k0 + q < f.mz ? k0 + q : f.mz - 1;  // levels past the top re-read the top one
+        const float *e = ec + (long)k * c.P;
+        v[q] = 0.5625f * e[cc] + 0.1875f * (e[cx] + e[cy]) + 0.0625f * e[cd];
+        xo[q] = xf[(long)k * f.P + nf];
+    }
+#pragma unroll
+    for (int q = 0; q < kPlaneZ; ++q)
+        if (k0 + q < f.mz) xf[(long)(k0 + q) * f.P + nf] = xo[q] + v[q];
 }
 
 }  // namespace windhip
