@@ -64,7 +64,7 @@ struct windhip_solver {
     Dims d{};
     Geo g{};
     long N = 0;
-    long guard = 0;  // elements before and after every row array (mx + 1)
+    long guard = 0;  // elements before and after every row array (mx + 1 rounded up to a multiple of 4)
     hipStream_t stream = nullptr;
     hipStream_t net_stream = nullptr;  // the learned guess, concurrent with the first inner solve
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_net0 = nullptr, ev_net1 = nullptr;
@@ -173,8 +173,10 @@ struct windhip_solver {
     {
         // Arrays of N rows get `guard` zeroed elements before and after them: the
         // stencil kernels address the five columns C, W, E, S, N of every lane
-        // from one shifted base (guard = mx + 1), so lateral and out-of-grid
-        // lanes touch at most guard elements beyond either end.
+        // from one shifted base (mx + 1 elements before the row), so lateral and
+        // out-of-grid lanes touch at most mx + 1 <= guard elements beyond
+        // either end; guard is a multiple of 4, which keeps the rows 16-byte
+        // aligned for the flat kernels' 16-byte accesses.
         const long gd = guard_elems >= 0 ? guard_elems : (count == N ? guard : 0);
         size_t nb = sizeof(T) * (size_t)std::max(count + 2 * gd, 1L);
         void *q = nullptr;
@@ -456,7 +458,7 @@ int windhip_solver::init()
     g.ntj = (d.my + kTileJ - 1) / kTileJ;
     g.nt = g.nti * g.ntj;
     N = d.P * d.mz;
-    guard = d.mx + 1;
+    guard = (d.mx + 1 + 3) / 4 * 4;  // at least mx + 1, and row arrays 16-byte aligned
     if ((N + 2 * guard) * (long)sizeof(double) >= (1L << 32))
         return fail(WINDHIP_ERR_ARG, "windhip_create: %ld rows exceed the 4 GiB span of one buffer resource", N);
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));

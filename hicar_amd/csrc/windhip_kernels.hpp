@@ -879,6 +879,16 @@ __global__ __launch_bounds__(kThreads) void k_sweep_sigma(Op op, Geo g, KState *
 }
 
 // s = r - alpha v (vec_axpby_into, :690) and the first sweep s_hat = D^-1 s.
+// 16-byte vectors of the flat update kernels
+template <typename T> struct Vec16;
+template <> struct Vec16<double> { using type = double __attribute__((ext_vector_type(2))); static constexpr int n = 2; };
+template <> struct Vec16<float> { using type = float __attribute__((ext_vector_type(4))); static constexpr int n = 4; };
+template <typename... P>
+__device__ __forceinline__ bool aligned16(const P *...p)
+{
+    return ((reinterpret_cast<unsigned long long>(p) | ...) & 15ull) == 0;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_update_s(long N, const KState *st, const T *__restrict__ r,
                                                      const T *__restrict__ v, const T *__restrict__ dinv,
@@ -886,7 +896,26 @@ __global__ __launch_bounds__(kBlock) void k_update_s(long N, const KState *st, c
 {
     if (st->done) return;
     const T na = T(-st->alpha);
-    for (long n = (long)blockIdx.x * kBlock + threadIdx.x; n < N; n += (long)gridDim.x * kBlock) {
+    const long tid = (long)blockIdx.x * kBlock + threadIdx.x, stride = (long)gridDim.x * kBlock;
+    long n0 = 0;
+    if (aligned16(r, v, dinv, s, sh)) {  // 16 bytes per lane and array
+        using V = typename Vec16<T>::type;
+        constexpr int W = Vec16<T>::n;
+        n0 = N / W * W;
+        for (long q = tid; q < N / W; q += stride) {
+            const V rv = reinterpret_cast<const V *>(r)[q], vv = reinterpret_cast<const V *>(v)[q],
+                    dv = reinterpret_cast<const V *>(dinv)[q];
+            V sv, hv;
+#pragma unroll
+            for (int e = 0; e < W; ++e) {
+                sv[e] = T(1) * rv[e] + na * vv[e];
+                hv[e] = dv[e] * sv[e];
+            }
+            reinterpret_cast<V *>(s)[q] = sv;
+            reinterpret_cast<V *>(sh)[q] = hv;
+        }
+    }
+    for (long n = n0 + tid; n < N; n += stride) {
         const T sv = T(1) * r[n] + na * v[n];
         s[n] = sv;
         sh[n] = dinv[n] * sv;
@@ -948,7 +977,39 @@ __global__ __launch_bounds__(kBlock) void k_update_xrp(long N, const KState *st,
     const int mode = st->fmode;
     if (mode == 0) return;
     const T a = T(st->alpha), om = T(st->omega), be = T(st->beta), nom = T(-st->omega);
-    for (long n = (long)blockIdx.x * kBlock + threadIdx.x; n < N; n += (long)gridDim.x * kBlock) {
+    const long tid = (long)blockIdx.x * kBlock + threadIdx.x, stride = (long)gridDim.x * kBlock;
+    long n0 = 0;
+    if (mode >= 3 && aligned16(x, ph, sh, s, t, r, p, v, dinv, ph_next)) {
+        // the full update, 16 bytes per lane and array (the same per-element
+        // arithmetic as the loop below)
+        using V = typename Vec16<T>::type;
+        constexpr int W = Vec16<T>::n;
+        n0 = N / W * W;
+        for (long q = tid; q < N / W; q += stride) {
+            const V xv = reinterpret_cast<const V *>(x)[q], phv = reinterpret_cast<const V *>(ph)[q],
+                    shv = reinterpret_cast<const V *>(sh)[q], sv = reinterpret_cast<const V *>(s)[q],
+                    tv = reinterpret_cast<const V *>(t)[q], pv0 = reinterpret_cast<const V *>(p)[q],
+                    vv = reinterpret_cast<const V *>(v)[q];
+            V xo, ro, po;
+#pragma unroll
+            for (int e = 0; e < W; ++e) {
+                xo[e] = (xv[e] + a * phv[e]) + om * shv[e];
+                ro[e] = T(1) * sv[e] + nom * tv[e];
+                po[e] = ro[e] + be * (pv0[e] - om * vv[e]);
+            }
+            reinterpret_cast<V *>(x)[q] = xo;
+            reinterpret_cast<V *>(r)[q] = ro;
+            reinterpret_cast<V *>(p)[q] = po;
+            if (dinv) {
+                const V dv = reinterpret_cast<const V *>(dinv)[q];
+                V ho;
+#pragma unroll
+                for (int e = 0; e < W; ++e) ho[e] = dv[e] * po[e];
+                reinterpret_cast<V *>(ph_next)[q] = ho;
+            }
+        }
+    }
+    for (long n = n0 + tid; n < N; n += stride) {
         x[n] = (x[n] + a * ph[n]) + om * sh[n];
         if (mode >= 2) {
             const T rv = T(1) * s[n] + nom * t[n];
