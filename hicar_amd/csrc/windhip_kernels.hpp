@@ -433,9 +433,23 @@ __device__ __forceinline__ void reduce_finalize(double (&acc)[NV], double *parti
     double fold[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) fold[v] = 0.0;
-    for (int q = threadIdx.x; q < nb; q += NW * 64) {
+    // a batch of FB partials is loaded before any of them is added (each add
+    // right after its load waited one memory round trip per partial); the
+    // adds keep the order q = t, t + 64 NW, ...
+    constexpr int FB = NV == 1 ? 8 : (NV <= 2 ? 4 : 2);
+    constexpr int S = NW * 64;
+    for (int q0 = threadIdx.x; q0 < nb; q0 += FB * S) {
+        double pv[FB][NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) fold[v] = fold[v] + ld_coherent(&partials[(long)q * NV + v]);
+        for (int e = 0; e < FB; ++e)
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                pv[e][v] = q0 + e * S < nb ? ld_coherent(&partials[(long)(q0 + e * S) * NV + v]) : 0.0;
+#pragma unroll
+        for (int e = 0; e < FB; ++e)
+            if (q0 + e * S < nb)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) fold[v] = fold[v] + pv[e][v];
     }
     __syncthreads();
 #pragma unroll
