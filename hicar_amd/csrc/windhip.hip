@@ -122,6 +122,7 @@ struct windhip_solver {
     long net_vox = 0;
     int net_smooth = 4;                        // line-relaxation steps applied to the network's guess
     double *lr_c = nullptr, *lr_d = nullptr;   // line relaxation: Thomas coefficients
+    float *lr_y32 = nullptr, *lr_o32 = nullptr, *lr_b32 = nullptr;  // FP32 relaxation of the mixed solves' guess
     // MIXED_LINE: FP32 Thomas factors of the column tridiagonal and scratch
     float *lf_lo = nullptr, *lf_cp = nullptr, *lf_ip = nullptr, *lf_dq = nullptr, *lf_r = nullptr;
     // bf16 copies of the line factors for the multigrid fine level
@@ -325,7 +326,8 @@ struct windhip_solver {
     int net_dims(int *Dp, int *Hp, int *Wp) const;
     int net_input(float *in, double *scale);
     int line_relax(double **y, double **scratch, int steps, hipStream_t strm);
-    int learned_launch(double *res_plain, double **guess);
+    int line_relax32(double *y, int steps, hipStream_t strm);
+    int learned_launch(double *res_plain, double **guess, bool f32_relax);
     int learned_join(double *res_guess);
     bool net_pending = false;
     template <typename T>
@@ -1121,8 +1123,41 @@ int windhip_solver::line_relax(double **y, double **scratch, int steps, hipStrea
     return WINDHIP_OK;
 }
 
+// The same relaxation in FP32 on the FP32 operator rows, for the guess of the
+// mixed solves (whose own inner solves are FP32): y (FP64) is rounded, relaxed
+// `steps` times and widened back in place.
+int windhip_solver::line_relax32(double *y, int steps, hipStream_t strm)
+{
+    if (steps <= 0) return WINDHIP_OK;
+    if (!lr_c) {
+        CHK(alloc(&lr_c, N));
+        CHK(alloc(&lr_d, N));
+    }
+    if (!lr_y32) {
+        CHK(alloc(&lr_y32, N));  // stencil inputs: guarded and zeroed like every row array
+        CHK(alloc(&lr_o32, N));
+        CHK(alloc(&lr_b32, N));
+    }
+    // the FP64 Thomas buffers hold the FP32 ones (N doubles each)
+    float *cq = reinterpret_cast<float *>(lr_c), *dq = reinterpret_cast<float *>(lr_d);
+    k_round_f32<<<flat_grid(N), kBlock, 0, strm>>>(N, y, lr_y32);
+    k_round_f32<<<flat_grid(N), kBlock, 0, strm>>>(N, b, lr_b32);
+    float *cur = lr_y32, *nxt = lr_o32;
+    for (int q = 0; q < steps; ++q) {
+        CHK(with_op<float>([&](auto op) {
+            k_line_relax<decltype(op)><<<g.nt, kThreads, 0, strm>>>(op, g, cur, lr_b32, cq, dq, nxt);
+            return WINDHIP_OK;
+        }));
+        std::swap(cur, nxt);
+    }
+    k_widen_f64<<<flat_grid(N), kBlock, 0, strm>>>(N, cur, y);
+    HIPCHK(hipGetLastError());
+    return WINDHIP_OK;
+}
+
 // The learned guess: the network's lambda after net_smooth vertical
-// line-relaxation steps (FP64), and the FP64 true residuals of that guess and
+// line-relaxation steps (FP64 for the plain solve, FP32 for the mixed ones),
+// and the FP64 true residuals of that guess and
 // of the plain start (the current lambda: the previous solution, or zero).
 // learned_launch takes the plain start's residual on the solver's stream, then
 // queues the network, the relaxation and the guess's residual on a second
@@ -1130,7 +1165,7 @@ int windhip_solver::line_relax(double **y, double **scratch, int steps, hipStrea
 // their line factors and coarse levels meanwhile; learned_join makes the
 // solver's stream wait for it and reads the guess's residual. The guess lands in t or s (FP64 vectors
 // of the plain solve, unused by the mixed solves).
-int windhip_solver::learned_launch(double *res_plain, double **guess)
+int windhip_solver::learned_launch(double *res_plain, double **guess, bool f32_relax)
 {
     int Dp, Hp, Wp;
     net_dims(&Dp, &Hp, &Wp);
@@ -1154,7 +1189,8 @@ int windhip_solver::learned_launch(double *res_plain, double **guess)
     k_net_output<<<flat_grid(d.P), kBlock, 0, net_stream>>>(d, Dp, Hp, Wp, net_out, scale, t);
     HIPCHK(hipGetLastError());
     double *gy = t, *gs = s;
-    CHK(line_relax(&gy, &gs, net_smooth, net_stream));
+    if (f32_relax) CHK(line_relax32(gy, net_smooth, net_stream));
+    else CHK(line_relax(&gy, &gs, net_smooth, net_stream));
     CHK(with_op<double>([&](auto op) {
         k_true_resid<decltype(op)><<<g.nt, kThreads, 0, net_stream>>>(op, g, st + 2, partials2, gy, b);
         return WINDHIP_OK;
@@ -1723,7 +1759,7 @@ int windhip_solve(windhip_solver *s, const void *alpha, const void *div, const w
                        o.precision == WINDHIP_MIXED_MG;
     double *guess = nullptr, *mixed_guess = nullptr, rp = 0.0;
     if (o.start == WINDHIP_START_LEARNED) {
-        CHK(s->learned_launch(&rp, &guess));
+        CHK(s->learned_launch(&rp, &guess, mixed));
         inf.start_used = WINDHIP_START_WARM;
         if (mixed) {
             mixed_guess = guess;  // the start, if better, once the mixed set-up is queued
