@@ -1219,8 +1219,21 @@ __global__ __launch_bounds__(kBlock) void k_update_s_only(long N, const KState *
 {
     if (st->done) return;
     const T na = T(-st->alpha);
-    for (long n = (long)blockIdx.x * kBlock + threadIdx.x; n < N; n += (long)gridDim.x * kBlock)
-        s[n] = T(1) * r[n] + na * v[n];
+    const long tid = (long)blockIdx.x * kBlock + threadIdx.x, stride = (long)gridDim.x * kBlock;
+    long n0 = 0;
+    if (aligned16(r, v, s)) {  // 16 bytes per lane and array, the same per-element arithmetic
+        using V = typename Vec16<T>::type;
+        constexpr int W = Vec16<T>::n;
+        n0 = N / W * W;
+        for (long q = tid; q < N / W; q += stride) {
+            const V rv = reinterpret_cast<const V *>(r)[q], vv = reinterpret_cast<const V *>(v)[q];
+            V sv;
+#pragma unroll
+            for (int e = 0; e < W; ++e) sv[e] = T(1) * rv[e] + na * vv[e];
+            reinterpret_cast<V *>(s)[q] = sv;
+        }
+    }
+    for (long n = n0 + tid; n < N; n += stride) s[n] = T(1) * r[n] + na * v[n];
 }
 
 // t = A s_hat and the five dots <t,s>, <t,t>, <r_hat,s>, <r_hat,t>, <s,s>
