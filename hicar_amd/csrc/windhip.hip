@@ -1284,7 +1284,6 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
     const int H = m + 2;  // stride of a Hessenberg column in fg_h
     const int fb = (int)std::min<long>(kFlatBlocks, (N + kThreads - 1) / kThreads);
     k_state_reset<<<1, 1, 0, stream>>>(st, 0, o.tol_rel, o.tol_abs, 0.0);
-    k_state_reset<<<1, 1, 0, stream>>>(st + 1, o.inner_max_iters, 0.0, 0.0, inner_tol);
     double *beta2 = fg_h + (long)m * H;  // ||r||^2 of the cycle's residual
     auto resid = [&]() -> int {
         CHK(with_op<double>([&](auto op) {
@@ -1313,7 +1312,9 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
             const double *vsq = j == 0 ? beta2 : fg_h + (long)(j - 1) * H + j;
             int cap = o.inner_max_iters, spent = 0;
             for (int attempt = 0;; ++attempt) {
-                if (attempt || j > 0) k_state_reset<<<1, 1, 0, stream>>>(st + 1, cap, 0.0, 0.0, inner_tol);
+                // every inner solve starts from a fresh state with this solve's
+                // cap (o.inner_max_iters, halved on an overflow rerun)
+                k_state_reset<<<1, 1, 0, stream>>>(st + 1, cap, 0.0, 0.0, inner_tol);
                 CHK(timed(PK_FG_SETUP, [&] {
                     k_fg_setup<<<fb, kThreads, 0, stream>>>(N, st + 1, fg_part, vsq, attempt == 0, fg_v[j], dinv32,
                                                             x32, r32, rh32, p32, ph32[0]);

@@ -71,6 +71,8 @@ def _load():
         "orc_canon_reduce": (d, [_dp, i, i]),
         "orc_dot": (d, [v, _dp, _dp]),
         "orc_num_threads": (i, []),
+        "orc_plant": (None, [i]),
+        "orc_set_faithful": (None, [v, i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -102,15 +104,27 @@ def num_threads() -> int:
     return _load().orc_num_threads()
 
 
+def plant(code: int) -> None:
+    """Make the oracle commit one planted mistake (codes in wind_oracle.c,
+    0 = none). Tests only: they show that a pin fails on it."""
+    _load().orc_plant(int(code))
+
+
 class Oracle:
     """The reference path restated on the CPU, FP64 throughout."""
 
-    def __init__(self, nx: int, ny: int, nz: int, dx: float):
+    def __init__(self, nx: int, ny: int, nz: int, dx: float, faithful: bool = False):
+        """faithful: round to FP32 where the reference computes in `real`
+        (module arrays, assembly, rhs product, correction); see
+        orc_set_faithful in wind_oracle.c."""
         self.lib = _load()
         self.nx, self.ny, self.nz, self.dx = nx, ny, nz, float(dx)
         self.h = self.lib.orc_create(nx, ny, nz, float(dx))
         if not self.h:
             raise ValueError("oracle: grid too small (need nx, ny >= 3 and nz >= 2)")
+        self.faithful = bool(faithful)
+        if faithful:
+            self.lib.orc_set_faithful(self.h, 1)
         self.rows_shape = (ny + 2, nz + 2, nx + 2)
 
     def close(self):

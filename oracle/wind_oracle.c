@@ -60,6 +60,7 @@ typedef struct orc {
     int probed;             /* operator_probed */
     int structure_uploaded; /* first-call assembly done */
     int nsweeps;            /* precond_n_sweeps */
+    int faithful;           /* FP32-faithful mode (orc_set_faithful) */
     /* Krylov state over rows */
     double *x, *r, *rh, *p, *ph, *v, *s, *sh, *t, *rhs, *dinv, *pres;
 } orc_t;
@@ -76,6 +77,21 @@ typedef struct orc {
 #define V2(o, a, i, j) (a)[((i) - 1) + (long)(o)->nx * ((j) - 1)]
 
 static double *zalloc(long n) { return (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double)); }
+
+/* ------------------------------------------------------ planted mistakes
+ * Test hook: orc_plant(code) makes the oracle commit one plausible mistake,
+ * so that the tests can show that their closed-form pins catch it. 0 (the
+ * default) is the faithful restatement; nothing outside tests/ sets it.
+ *   1  bottom BC row: sign of the x-slope terms flipped
+ *   2  bottom BC row: alpha instead of alpha**2 in denom
+ *   3  bottom BC row: slope terms over dx instead of 2 dx
+ *   4  surface slopes: centred difference over dx instead of 2 dx
+ *   5  surface slopes: dzdy_surf differenced along i instead of j
+ *   6  bottom BC row: 1/dz_if taken at interface 2 instead of 1
+ */
+static int g_plant = 0;
+void orc_plant(int code) { g_plant = code; }
+static inline double f32r(double x) { return (double)(float)x; }
 
 /* ------------------------------------------------------ canonical reduction
  * The order every dot product is summed in (shared with the HIP kernels).
@@ -217,18 +233,6 @@ void orc_set_metrics(orc_t *o, const double *dzdx, const double *dzdy, const dou
     int nx = o->nx, ny = o->ny, nz = o->nz;
     long n3 = (long)nx * nz * ny;
     memcpy(o->adv_dz, adv_dz, n3 * sizeof(double));
-    /* dz_if: interface mean of advection_dz, :2350-2362 */
-    for (int j = 1; j <= ny; ++j)
-        for (int k = 2; k <= nz; ++k)
-            for (int i = 1; i <= nx; ++i)
-                DZ(o, i, k, j) = (M3(o, adv_dz, i, k, j) + M3(o, adv_dz, i, k - 1, j)) / 2;
-    for (int j = 1; j <= ny; ++j)
-        for (int i = 1; i <= nx; ++i) {
-            DZ(o, i, 1, j) = M3(o, adv_dz, i, 1, j);
-            DZ(o, i, nz + 1, j) = M3(o, adv_dz, i, nz, j);
-            S2(o, o->dzdx_surf, i, j) = M3(o, dzdx, i, 1, j);
-            S2(o, o->dzdy_surf, i, j) = M3(o, dzdy, i, 1, j);
-        }
     memcpy(o->dzdx, dzdx, n3 * sizeof(double));
     memcpy(o->dzdy, dzdy, n3 * sizeof(double));
     memcpy(o->jaco, jaco, n3 * sizeof(double));
@@ -237,26 +241,169 @@ void orc_set_metrics(orc_t *o, const double *dzdx, const double *dzdy, const dou
     memcpy(o->jaco_u, jaco_u, (long)(nx + 1) * nz * ny * sizeof(double));
     memcpy(o->dzdy_v, dzdy_v, (long)nx * nz * (ny + 1) * sizeof(double));
     memcpy(o->jaco_v, jaco_v, (long)nx * nz * (ny + 1) * sizeof(double));
-    /* centred terrain slope away from the domain edge, :2398-2409 */
+    if (o->faithful) {
+        /* the module arrays are `real` (wind_iterative.F90:89-91) */
+        double *arrs[] = {o->adv_dz, o->dzdx, o->dzdy, o->jaco, o->jaco_w};
+        for (int q = 0; q < 5; ++q)
+            for (long n = 0; n < n3; ++n) arrs[q][n] = f32r(arrs[q][n]);
+        for (long n = 0; n < (long)(nx + 1) * nz * ny; ++n) {
+            o->dzdx_u[n] = f32r(o->dzdx_u[n]); o->jaco_u[n] = f32r(o->jaco_u[n]);
+        }
+        for (long n = 0; n < (long)nx * nz * (ny + 1); ++n) {
+            o->dzdy_v[n] = f32r(o->dzdy_v[n]); o->jaco_v[n] = f32r(o->jaco_v[n]);
+        }
+    }
+    const double *az = o->adv_dz;
+    /* dz_if: interface mean of advection_dz, :2350-2362 (in FP32 arithmetic in
+     * the faithful mode: (a + b) / 2 with a, b `real`) */
     for (int j = 1; j <= ny; ++j)
-        for (int i = 2; i <= nx - 1; ++i)
-            S2(o, o->dzdx_surf, i, j) = (S2(o, terrain, i + 1, j) - S2(o, terrain, i - 1, j)) / (2 * o->dx);
+        for (int k = 2; k <= nz; ++k)
+            for (int i = 1; i <= nx; ++i) {
+                if (o->faithful)
+                    DZ(o, i, k, j) = (double)(((float)M3(o, az, i, k, j) + (float)M3(o, az, i, k - 1, j)) / 2.0f);
+                else
+                    DZ(o, i, k, j) = (M3(o, az, i, k, j) + M3(o, az, i, k - 1, j)) / 2;
+            }
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            DZ(o, i, 1, j) = M3(o, az, i, 1, j);
+            DZ(o, i, nz + 1, j) = M3(o, az, i, nz, j);
+            S2(o, o->dzdx_surf, i, j) = M3(o, o->dzdx, i, 1, j);
+            S2(o, o->dzdy_surf, i, j) = M3(o, o->dzdy, i, 1, j);
+        }
+    /* centred terrain slope away from the domain edge, :2398-2409: i in
+     * i_s_bnd..i_e_bnd = 2..nx-1 for dzdx_surf (all j), j in 2..ny-1 for
+     * dzdy_surf (all i); the edge columns keep dzdx(i,k_s,j) / dzdy(i,k_s,j)
+     * from :2363-2364 */
+    const double two_dx = g_plant == 4 ? o->dx : 2 * o->dx;
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 2; i <= nx - 1; ++i) {
+            if (o->faithful)
+                S2(o, o->dzdx_surf, i, j) = (double)(((float)S2(o, terrain, i + 1, j) - (float)S2(o, terrain, i - 1, j))
+                                                     / (float)two_dx);
+            else
+                S2(o, o->dzdx_surf, i, j) = (S2(o, terrain, i + 1, j) - S2(o, terrain, i - 1, j)) / two_dx;
+        }
     for (int j = 2; j <= ny - 1; ++j)
-        for (int i = 1; i <= nx; ++i)
-            S2(o, o->dzdy_surf, i, j) = (S2(o, terrain, i, j + 1) - S2(o, terrain, i, j - 1)) / (2 * o->dx);
-    for (int k = 1; k <= nz; ++k) o->adv_dz_col[k - 1] = M3(o, adv_dz, 1, k, 1); /* :2410-2413 */
+        for (int i = 1; i <= nx; ++i) {
+            double tp = S2(o, terrain, i, j + 1), tm = S2(o, terrain, i, j - 1);
+            if (g_plant == 5) {
+                if (i < 2 || i > nx - 1) continue;
+                tp = S2(o, terrain, i + 1, j); tm = S2(o, terrain, i - 1, j);
+            }
+            if (o->faithful)
+                S2(o, o->dzdy_surf, i, j) = (double)(((float)tp - (float)tm) / (float)two_dx);
+            else
+                S2(o, o->dzdy_surf, i, j) = (tp - tm) / two_dx;
+        }
+    for (int k = 1; k <= nz; ++k) o->adv_dz_col[k - 1] = M3(o, az, 1, k, 1); /* :2410-2413 */
 }
 
 /* calc_iter_winds :309-317 copies alpha over the solver range */
 void orc_set_alpha(orc_t *o, const double *alpha)
 {
-    memcpy(o->alpha, alpha, (long)o->nx * o->nz * o->ny * sizeof(double));
+    long n3 = (long)o->nx * o->nz * o->ny;
+    memcpy(o->alpha, alpha, n3 * sizeof(double));
+    if (o->faithful)
+        for (long n = 0; n < n3; ++n) o->alpha[n] = f32r(o->alpha[n]);
 }
 
 /* ------------------------------------------------------------- assembly */
 /* update_coefs_gpu, wind_iterative.F90:2555-2603: B, C (with X) and A per cell */
+/* FP32-faithful update_coefs_gpu: every operand and intermediate `real`
+ * (wind_iterative.F90:2556-2558), the same expression order as below. */
+static void update_coefs_f32(orc_t *o)
+{
+    int nx = o->nx, ny = o->ny, nz = o->nz;
+    const float dx = (float)o->dx;
+    double *B = o->coef[CB], *C = o->coef[CC], *A = o->coef[CA];
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j <= ny; ++j)
+        for (int k = 1; k <= nz; ++k)
+            for (int i = 1; i <= nx; ++i) {
+                float adv_dz_k = (float)o->adv_dz_col[k - 1];
+                float M_up, M_dwn;
+                float dzdx = (float)M3(o, o->dzdx, i, k, j), dzdy = (float)M3(o, o->dzdy, i, k, j);
+                float al = (float)M3(o, o->alpha, i, k, j);
+                if (k == nz) {
+                    M_up = (dzdx * dzdx + dzdy * dzdy) + al * al;
+                } else {
+                    float a1 = (float)o->adv_dz_col[k];
+                    float sum_up = adv_dz_k + a1;
+                    float jw = (float)M3(o, o->jaco_w, i, k, j);
+                    float dzdx1 = (float)M3(o, o->dzdx, i, k + 1, j), dzdy1 = (float)M3(o, o->dzdy, i, k + 1, j);
+                    float al1 = (float)M3(o, o->alpha, i, k + 1, j);
+                    M_up = ((dzdx * (dzdx * a1 + dzdx1 * adv_dz_k)) / sum_up) / jw
+                         + ((dzdy * (dzdy * a1 + dzdy1 * adv_dz_k)) / sum_up) / jw;
+                    M_up = M_up + ((al * al * a1 + al1 * al1 * adv_dz_k) / sum_up) / jw;
+                }
+                if (k == 1) {
+                    M_dwn = (dzdx * (float)S2(o, o->dzdx_surf, i, j) + dzdy * (float)S2(o, o->dzdy_surf, i, j)) + al * al;
+                } else {
+                    float am1 = (float)o->adv_dz_col[k - 2];
+                    float sum_dwn = adv_dz_k + am1;
+                    float jw = (float)M3(o, o->jaco_w, i, k - 1, j);
+                    float dzdxm = (float)M3(o, o->dzdx, i, k - 1, j), dzdym = (float)M3(o, o->dzdy, i, k - 1, j);
+                    float alm = (float)M3(o, o->alpha, i, k - 1, j);
+                    M_dwn = ((dzdx * (dzdx * am1 + dzdxm * adv_dz_k)) / sum_dwn) / jw
+                          + ((dzdy * (dzdy * am1 + dzdym * adv_dz_k)) / sum_dwn) / jw;
+                    M_dwn = M_dwn + ((al * al * am1 + alm * alm * adv_dz_k) / sum_dwn) / jw;
+                }
+                float jc = (float)M3(o, o->jaco, i, k, j);
+                float b = M_up / ((jc * adv_dz_k) * (float)DZ(o, 1, k + 1, 1));
+                float c = M_dwn / ((jc * adv_dz_k) * (float)DZ(o, 1, k, 1));
+                float md = (((float)DZ(o, i, k + 1, j) + (float)DZ(o, i, k, j)) * 2.0f) * dx;
+                float X = -((((float)U3(o, o->dzdx_u, i + 1, k, j) / (float)U3(o, o->jaco_u, i + 1, k, j)
+                              - (float)U3(o, o->dzdx_u, i, k, j) / (float)U3(o, o->jaco_u, i, k, j))
+                             + ((float)V3(o, o->dzdy_v, i, k, j + 1) / (float)V3(o, o->jaco_v, i, k, j + 1)
+                                - (float)V3(o, o->dzdy_v, i, k, j) / (float)V3(o, o->jaco_v, i, k, j))) / md);
+                b = b + X;
+                c = c - X;
+                M3(o, B, i, k, j) = b;
+                M3(o, C, i, k, j) = c;
+                M3(o, A, i, k, j) = (-4.0f / (dx * dx) - b) - c;
+            }
+}
+
+/* FP32-faithful initialize_coefs (:2424-2469): D..G and H..O in `real`. */
+static void assemble_f32(orc_t *o)
+{
+    int nx = o->nx, ny = o->ny, nz = o->nz;
+    const float dx = (float)o->dx;
+    const float inv = 1.0f / (dx * dx);
+    long n3 = (long)nx * nz * ny;
+    for (long n = 0; n < n3; ++n) {
+        o->coef[CA][n] = 1.0;
+        o->coef[CD][n] = inv; o->coef[CE][n] = inv; o->coef[CF][n] = inv; o->coef[CG][n] = inv;
+    }
+    for (int k = 1; k <= nz; ++k) {
+        float md = (2.0f * dx) * ((float)DZ(o, 1, k + 1, 1) + (float)DZ(o, 1, k, 1));
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                float jc = (float)M3(o, o->jaco, i, k, j);
+                float ax = (float)M3(o, o->dzdx, i, k, j) / jc;
+                float ay = (float)M3(o, o->dzdy, i, k, j) / jc;
+                float hx = ax + (float)U3(o, o->dzdx_u, i + 1, k, j) / (float)U3(o, o->jaco_u, i + 1, k, j);
+                float ix = ax + (float)U3(o, o->dzdx_u, i, k, j) / (float)U3(o, o->jaco_u, i, k, j);
+                float ly = ay + (float)V3(o, o->dzdy_v, i, k, j + 1) / (float)V3(o, o->jaco_v, i, k, j + 1);
+                float my = ay + (float)V3(o, o->dzdy_v, i, k, j) / (float)V3(o, o->jaco_v, i, k, j);
+                M3(o, o->coef[CH], i, k, j) = -(hx / md);
+                M3(o, o->coef[CI], i, k, j) = ix / md;
+                M3(o, o->coef[CL], i, k, j) = -(ly / md);
+                M3(o, o->coef[CM], i, k, j) = my / md;
+                M3(o, o->coef[CJ], i, k, j) = hx / md;
+                M3(o, o->coef[CK], i, k, j) = -(ix / md);
+                M3(o, o->coef[CN], i, k, j) = ly / md;
+                M3(o, o->coef[CO], i, k, j) = -(my / md);
+            }
+    }
+    update_coefs_f32(o);
+    o->probed = 0;
+}
+
 void orc_update_coefs(orc_t *o)
 {
+    if (o->faithful) { update_coefs_f32(o); return; }
     int nx = o->nx, ny = o->ny, nz = o->nz;
     double dx = o->dx;
     double *B = o->coef[CB], *C = o->coef[CC], *A = o->coef[CA];
@@ -313,6 +460,7 @@ void orc_update_coefs(orc_t *o)
  * call and every later alpha refresh produce the same operator. */
 void orc_assemble(orc_t *o)
 {
+    if (o->faithful) { assemble_f32(o); return; }
     int nx = o->nx, ny = o->ny, nz = o->nz;
     double dx = o->dx;
     double inv = 1.0 / (dx * dx);
@@ -402,11 +550,14 @@ void orc_spmv(const orc_t *o, const double *x, double *y)
         for (int i = 1; i <= mx - 2; ++i) {
             double dzdx_s = S2(o, o->dzdx_surf, i, j);
             double dzdy_s = S2(o, o->dzdy_surf, i, j);
-            double dz_kp1 = DZ(o, i, 1, j);
+            double dz_kp1 = DZ(o, i, g_plant == 6 ? 2 : 1, j);
             double al = M3(o, o->alpha, i, 1, j);
-            double denom = (2.0 * ((dzdx_s * dzdx_s + dzdy_s * dzdy_s) + al * al)) / M3(o, o->jaco, i, 1, j);
-            double cy = dzdy_s / (denom * two_dx);
-            double cx = dzdx_s / (denom * two_dx);
+            double al2 = g_plant == 2 ? al : al * al;
+            double denom = (2.0 * ((dzdx_s * dzdx_s + dzdy_s * dzdy_s) + al2)) / M3(o, o->jaco, i, 1, j);
+            double sdx = g_plant == 3 ? dx : two_dx;
+            double cy = dzdy_s / (denom * sdx);
+            double cx = dzdx_s / (denom * sdx);
+            if (g_plant == 1) cx = -cx;
             double cz = 1.0 / dz_kp1;
             R(o, y, i, 0, j) = cy * R(o, x, i, 0, j - 1) + cy * R(o, x, i, 1, j - 1)
                 + cx * R(o, x, i - 1, 0, j) + (-cz) * R(o, x, i, 0, j)
@@ -465,6 +616,8 @@ void orc_compute_rhs(orc_t *o, const double *div)
             for (int i = 0; i < mx; ++i) {
                 if (i <= 0 || j <= 0 || i >= mx - 1 || j >= my - 1 || k <= 0 || k >= mz - 1)
                     R(o, o->rhs, i, k, j) = 0.0;
+                else if (o->faithful) /* real(-2.0 * div, c_double) with div `real`, :1525 */
+                    R(o, o->rhs, i, k, j) = (double)(-2.0f * (float)M3(o, div, i, k, j));
                 else
                     R(o, o->rhs, i, k, j) = -2.0 * M3(o, div, i, k, j);
             }
@@ -474,9 +627,11 @@ void orc_compute_rhs(orc_t *o, const double *div)
 void orc_apply_precond(orc_t *o, int nsweeps, const double *in, double *out)
 {
     long N = o->N;
+#pragma omp parallel for schedule(static)
     for (long n = 0; n < N; ++n) out[n] = o->dinv[n] * in[n];
     for (int sw = 2; sw <= nsweeps; ++sw) {
         orc_spmv(o, out, o->pres);
+#pragma omp parallel for schedule(static)
         for (long n = 0; n < N; ++n) out[n] = out[n] + o->dinv[n] * (in[n] - o->pres[n]);
     }
 }
@@ -495,6 +650,7 @@ int orc_bicgstab(orc_t *o, int max_iters, double tol_rel, double tol_abs,
     double *s = o->s, *sh = o->sh, *t = o->t, *b = o->rhs;
     *status_out = 1; *iters_out = 0; *res0_out = 0.0; *resf_out = 0.0;
     orc_spmv(o, x, t);
+#pragma omp parallel for schedule(static)
     for (long n = 0; n < N; ++n) r[n] = 1.0 * b[n] + (-1.0) * t[n];
     memcpy(rh, r, N * sizeof(double));
     memcpy(p, r, N * sizeof(double));
@@ -514,6 +670,7 @@ int orc_bicgstab(orc_t *o, int max_iters, double tol_rel, double tol_abs,
         if (trace && it <= trace_cap) trace[5 * (it - 1)] = sigma;
         if (fabs(sigma) < breakdown_eps) { *status_out = 2; *iters_out = it; break; }
         alpha = rho / sigma;
+#pragma omp parallel for schedule(static)
         for (long n = 0; n < N; ++n) s[n] = 1.0 * r[n] + (-alpha) * v[n];
         orc_apply_precond(o, o->nsweeps, s, sh);
         orc_spmv(o, sh, t);
@@ -550,12 +707,15 @@ int orc_bicgstab(orc_t *o, int max_iters, double tol_rel, double tol_abs,
             trace[5 * (it - 1) + 1] = alpha; trace[5 * (it - 1) + 2] = omega;
             trace[5 * (it - 1) + 3] = rho_new; trace[5 * (it - 1) + 4] = rnorm;
         }
+#pragma omp parallel for schedule(static)
         for (long n = 0; n < N; ++n) x[n] = (x[n] + alpha * ph[n]) + omega * sh[n];
         if (rnorm <= target) { *status_out = 0; break; }
         if (fabs(omega) < breakdown_eps) { *status_out = 2; break; }
+#pragma omp parallel for schedule(static)
         for (long n = 0; n < N; ++n) r[n] = 1.0 * s[n] + (-omega) * t[n];
         if (fabs(rho) < breakdown_eps) { *status_out = 2; break; }
         double beta = (rho_new / rho) * (alpha / omega);
+#pragma omp parallel for schedule(static)
         for (long n = 0; n < N; ++n) p[n] = r[n] + beta * (p[n] - omega * v[n]);
         rho = rho_new;
     }
@@ -600,6 +760,17 @@ int orc_solve(orc_t *o, const double *alpha, const double *div, int max_iters,
 }
 
 void orc_set_nsweeps(orc_t *o, int n) { o->nsweeps = n; }
+/* FP32-faithful mode: the oracle rounds where the reference stores or computes
+ * in `real` (FP32): the module arrays, dx, dz_if and the surface slopes
+ * (init_module_vars, wind_iterative.F90:86-91, 2350-2409), the coefficient
+ * assembly (initialize_coefs / update_coefs_gpu :2424-2603), alpha, the rhs
+ * product (:1525) and the correction with lambda rounded to FP32 (:1770-2062).
+ * The Krylov solve stays FP64 as in the reference. Set it before set_metrics. */
+void orc_set_faithful(orc_t *o, int on)
+{
+    o->faithful = on ? 1 : 0;
+    if (o->faithful) o->dx = f32r(o->dx);
+}
 void orc_get_lambda(const orc_t *o, double *out) { memcpy(out, o->x, o->N * sizeof(double)); }
 void orc_set_lambda(orc_t *o, const double *in) { memcpy(o->x, in, o->N * sizeof(double)); }
 void orc_get_rows(const orc_t *o, int which, double *out)
@@ -629,9 +800,99 @@ void orc_get_planes(const orc_t *o, double *dz_if, double *dzdx_surf, double *dz
  * keeps it in FP64). rho (nx,nz,ny) may be NULL (adv_den false or rho = 1);
  * map factors may be NULL (1.0).
  */
+/* FP32-faithful calc_updated_winds: lambda rounded to `real` (lambda_3d,
+ * :1770-1778) and every array and operation of :1804-2062 in FP32, as the
+ * reference computes it; u, v, w are `real` fields, so they are rounded to
+ * FP32 on entry and hold FP32 values on exit. */
+static void correct_f32(const orc_t *o, double *u, double *v, double *w, const double *rho_in,
+                        const double *mf_mx_u, const double *mf_my_v, int adv_den)
+{
+    int nx = o->nx, ny = o->ny, nz = o->nz;
+    const float dx = (float)o->dx;
+    const int mx = o->mx, mz = o->mz, my = o->my;
+    long n3 = (long)nx * nz * ny, nu = (long)(nx + 1) * nz * ny, nv = (long)nx * nz * (ny + 1);
+    float *lam = (float *)calloc((size_t)o->N, sizeof(float));
+    float *rho = (float *)calloc((size_t)n3, sizeof(float)), *rho_w = (float *)calloc((size_t)n3, sizeof(float));
+    float *rho_u = (float *)calloc((size_t)nu, sizeof(float)), *rho_v = (float *)calloc((size_t)nv, sizeof(float));
+    for (long n = 0; n < o->N; ++n) lam[n] = (float)o->x[n];
+    for (long n = 0; n < n3; ++n) rho[n] = (adv_den && rho_in) ? (float)rho_in[n] : 1.0f;
+#define LF(i, k, j) lam[(i) + (long)mx * ((k) + (long)mz * (j))]
+#define DZF(k) ((float)DZ(o, 1, (k), 1))
+    for (int j = 1; j <= ny; ++j)
+        for (int k = 1; k <= nz; ++k) {
+            for (int i = 2; i <= nx; ++i)
+                U3(o, rho_u, i, k, j) = 0.5f * (M3(o, rho, i, k, j) + M3(o, rho, i - 1, k, j));
+            U3(o, rho_u, nx + 1, k, j) = M3(o, rho, nx, k, j);
+            U3(o, rho_u, 1, k, j) = M3(o, rho, 1, k, j);
+        }
+    for (int k = 1; k <= nz; ++k)
+        for (int i = 1; i <= nx; ++i) {
+            for (int j = 2; j <= ny; ++j)
+                V3(o, rho_v, i, k, j) = 0.5f * (M3(o, rho, i, k, j) + M3(o, rho, i, k, j - 1));
+            V3(o, rho_v, i, k, 1) = M3(o, rho, i, k, 1);
+            V3(o, rho_v, i, k, ny + 1) = M3(o, rho, i, k, ny);
+        }
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            for (int k = 1; k <= nz - 1; ++k) {
+                float dzk = (float)M3(o, o->adv_dz, i, k, j), dzk1 = (float)M3(o, o->adv_dz, i, k + 1, j);
+                M3(o, rho_w, i, k, j) = (M3(o, rho, i, k, j) * dzk1 + M3(o, rho, i, k + 1, j) * dzk) / (dzk + dzk1);
+            }
+            M3(o, rho_w, i, nz, j) = M3(o, rho, i, nz, j);
+        }
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx + 1; ++i) {
+            float ut[nz + 2];
+            for (int k = 0; k <= nz + 1; ++k) ut[k] = (LF(i, k, j) + LF(i - 1, k, j)) / 2.0f;
+            for (int k = 1; k <= nz; ++k) {
+                float dl;
+                if (k == 1) dl = (ut[2] - ut[1]) / DZF(2);
+                else if (k == nz) dl = (ut[nz] - ut[nz - 1]) / DZF(nz);
+                else dl = (ut[k + 1] - ut[k - 1]) / (DZF(k + 1) + DZF(k));
+                float mf = mf_mx_u ? (float)U2(o, mf_mx_u, i, j) : 1.0f;
+                float grad = (LF(i, k, j) - LF(i - 1, k, j)) / dx
+                           - ((float)U3(o, o->dzdx_u, i, k, j) * dl) / (float)U3(o, o->jaco_u, i, k, j);
+                float uu = (float)U3(o, u, i, k, j);
+                U3(o, u, i, k, j) = (double)(uu + ((0.5f * mf) * grad) / U3(o, rho_u, i, k, j));
+            }
+        }
+    for (int j = 1; j <= ny + 1; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            float vt[nz + 2];
+            for (int k = 0; k <= nz + 1; ++k) vt[k] = (LF(i, k, j) + LF(i, k, j - 1)) / 2.0f;
+            for (int k = 1; k <= nz; ++k) {
+                float dl;
+                if (k == 1) dl = (vt[2] - vt[1]) / DZF(2);
+                else if (k == nz) dl = (vt[nz] - vt[nz - 1]) / DZF(nz);
+                else dl = (vt[k + 1] - vt[k - 1]) / (DZF(k + 1) + DZF(k));
+                float mf = mf_my_v ? (float)V2(o, mf_my_v, i, j) : 1.0f;
+                float grad = (LF(i, k, j) - LF(i, k, j - 1)) / dx
+                           - ((float)V3(o, o->dzdy_v, i, k, j) * dl) / (float)V3(o, o->jaco_v, i, k, j);
+                float vv = (float)V3(o, v, i, k, j);
+                V3(o, v, i, k, j) = (double)(vv + ((0.5f * mf) * grad) / V3(o, rho_v, i, k, j));
+            }
+        }
+    for (int j = 1; j <= ny; ++j)
+        for (int k = 1; k <= nz; ++k)
+            for (int i = 1; i <= nx; ++i) {
+                float ww = (float)M3(o, w, i, k, j);
+                if (k < nz) {
+                    float al = (float)M3(o, o->alpha, i, k, j);
+                    ww = ww + ((((0.5f * (al * al)) * (LF(i, k + 1, j) - LF(i, k, j))) / (float)DZ(o, i, k + 1, j))
+                               / (float)M3(o, o->jaco_w, i, k, j)) / M3(o, rho_w, i, k, j);
+                }
+                M3(o, w, i, k, j) = (double)ww;
+            }
+#undef LF
+#undef DZF
+    (void)my;
+    free(lam); free(rho); free(rho_w); free(rho_u); free(rho_v);
+}
+
 void orc_correct(const orc_t *o, double *u, double *v, double *w, const double *rho_in,
                  const double *mf_mx_u, const double *mf_my_v, int adv_den)
 {
+    if (o->faithful) { correct_f32(o, u, v, w, rho_in, mf_mx_u, mf_my_v, adv_den); return; }
     int nx = o->nx, ny = o->ny, nz = o->nz;
     double dx = o->dx;
     const double *lam = o->x;
