@@ -38,6 +38,7 @@ EXPORTS = (
     "windhip_net_adam_step", "windhip_net_time_forward", "windhip_set_net", "windhip_learned_input", "windhip_set_guess_smoothing",
     "windhip_set_multigrid", "windhip_mg_level", "windhip_mg_smooth",
     "windhip_balance_uvw", "windhip_calc_w_real", "windhip_idealized_wgrid",
+    "windhip_balance_uvw_noalloc",
 )
 
 PROFILE_KINDS = {0: "sweep_f64", 1: "sweep_sigma_f64", 2: "update_s_f64", 3: "sweep_omega_f64",
@@ -148,6 +149,7 @@ def load():
         "windhip_mg_level": (i, [vp, i, C.POINTER(C.c_int), fp]),
         "windhip_mg_smooth": (i, [vp, i, i, fp, fp, fp]),
         "windhip_balance_uvw": (i, [vp, vp, vp, vp, vp, i, i]),
+        "windhip_balance_uvw_noalloc": (i, [vp, vp, vp, vp, vp, i, i]),
         "windhip_calc_w_real": (i, [vp, vp, vp, vp, vp]),
         "windhip_idealized_wgrid": (i, [vp, vp, vp, vp, vp]),
         "windhip_learned_input": (i, [vp, vp, C.POINTER(C.c_int), dp]),

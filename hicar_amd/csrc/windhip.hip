@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -57,6 +58,42 @@ int flat_grid(long n)
     return (int)std::max(1L, std::min(g, 16384L));
 }
 
+}  // namespace
+
+// Scratch of the allocation-free entry point windhip_balance_uvw_noalloc: the
+// FP64 horizontal divergence of one call, in the caller's mass-grid layout
+// (ims:ime, kms:kme, jms:jme). A device global, so every GPU the library runs
+// on holds its own copy (128 MiB) until the process ends. Its address on a
+// device is taken once, by hipGetSymbolAddress with that device current, when
+// the first handle on the device is created; host code uses only that address
+// (the array's name is a host-side stand-in, not a device address). Calls that
+// overlap on one GPU share it and race.
+constexpr long kNoAllocScratchDoubles = 16L << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per device.
+int scratch_address(double **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(WINDHIP_ERR_HIP, "hipGetDevice failed: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices) return fail(WINDHIP_ERR_HIP, "device index %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(WINDHIP_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return WINDHIP_OK;
+}
 }  // namespace
 
 struct windhip_solver {
@@ -356,10 +393,12 @@ struct windhip_solver {
     int divergence(const void *u, const void *v, const void *w, const void *density, int adv_den,
                    int horz_only, int use_dqdt, void *div_out);
     int probe(const void *alpha, const void *density, int adv_den, double *max_leak);
-    int balance_uvw(const void *u, const void *v, void *w, const void *density, int adv_den, int use_dqdt);
+    int balance_uvw(const void *u, const void *v, void *w, const void *density, int adv_den, int use_dqdt,
+                    double *scr);
     int w_real(const void *u, const void *v, const void *w_grid, void *out);
     int idealized_wgrid(const void *u, const void *v, const void *w_real_in, void *w_out);
     double *div_scr = nullptr;  // balance_uvw: the horizontal divergence, FP64, caller mass layout
+    double *noalloc_scr = nullptr;  // this device's `scratch` (windhip_balance_uvw_noalloc)
     void release();
 };
 
@@ -1491,9 +1530,8 @@ int windhip_solver::divergence(const void *u, const void *v, const void *w, cons
 // is, since balance_uvw leaves advect_density at its default) into FP64
 // scratch, then calc_w (:193-234) writes w; adv_den selects calc_w's form.
 int windhip_solver::balance_uvw(const void *u, const void *v, void *w, const void *density, int adv_den,
-                                int use_dqdt)
+                                int use_dqdt, double *scr)
 {
-    if (!div_scr) CHK(alloc(&div_scr, n_mass()));
     void *du, *dv, *dw, *dr = nullptr;
     CHK(view(u, n_u(), 2, true, &du));
     CHK(view(v, n_v(), 3, true, &dv));
@@ -1501,8 +1539,10 @@ int windhip_solver::balance_uvw(const void *u, const void *v, void *w, const voi
     const int use_rho = density ? 1 : 0;
     if (use_rho) CHK(view(density, n_mass(), 5, true, &dr));
     const long tw = (long)d.nx * d.ny;
-    const Fld<double> dd = fld<double>(div_scr, 0);
-    const Fld<const double> ddc{div_scr, dd.i0, dd.k0, dd.j0, dd.ni, dd.nk};
+    // k_divergence writes every domain cell of the scratch that k_calc_w
+    // reads, so the scratch needs no clearing between calls
+    const Fld<double> dd = fld<double>(scr, 0);
+    const Fld<const double> ddc{scr, dd.i0, dd.k0, dd.j0, dd.ni, dd.nk};
     auto run = [&](auto tag) {
         using F = decltype(tag);
         k_divergence<F, double><<<flat_grid(tw), kBlock, 0, stream>>>(d, metrics(), fld<const F>(du, 1),
@@ -1671,6 +1711,14 @@ int windhip_create(const windhip_desc *desc, windhip_solver **out)
     const char *vb = getenv("WINDHIP_VERBOSE");
     h->verbose = vb && vb[0] && vb[0] != '0';
     int rc = h->init();
+    if (rc != WINDHIP_OK) {
+        std::string keep = g_err;
+        h->release();
+        delete h;
+        g_err = keep;
+        return rc;
+    }
+    rc = scratch_address(&h->noalloc_scr);
     if (rc != WINDHIP_OK) {
         std::string keep = g_err;
         h->release();
@@ -1848,7 +1896,23 @@ int windhip_balance_uvw(windhip_solver *s, const void *u, const void *v, void *w
 {
     if (!s || !u || !v || !w) return fail(WINDHIP_ERR_ARG, "windhip_balance_uvw: null argument");
     if (!s->metrics_set) return fail(WINDHIP_ERR_ARG, "windhip_balance_uvw: metrics not set");
-    return s->balance_uvw(u, v, w, density, adv_den, use_dqdt);
+    if (!s->div_scr) CHK(s->alloc(&s->div_scr, s->n_mass()));
+    return s->balance_uvw(u, v, w, density, adv_den, use_dqdt, s->div_scr);
+}
+
+int windhip_balance_uvw_noalloc(windhip_solver *s, const void *u, const void *v, void *w, const void *density,
+                                int adv_den, int use_dqdt)
+{
+    if (!s || !u || !v || !w) return fail(WINDHIP_ERR_ARG, "windhip_balance_uvw_noalloc: null argument");
+    if (s->desc.field_mem != WINDHIP_DEVICE)
+        return fail(WINDHIP_ERR_ARG, "windhip_balance_uvw_noalloc: fields must be in device memory "
+                                     "(host fields would be staged through allocated buffers)");
+    if (s->n_mass() > kNoAllocScratchDoubles)
+        return fail(WINDHIP_ERR_ARG, "windhip_balance_uvw_noalloc: the mass grid has %ld elements, more than the "
+                                     "%ld of the static scratch", s->n_mass(), kNoAllocScratchDoubles);
+    if (!s->metrics_set) return fail(WINDHIP_ERR_ARG, "windhip_balance_uvw_noalloc: metrics not set");
+    if (!s->noalloc_scr) return fail(WINDHIP_ERR_HIP, "windhip_balance_uvw_noalloc: no scratch address");
+    return s->balance_uvw(u, v, w, density, adv_den, use_dqdt, s->noalloc_scr);
 }
 
 int windhip_calc_w_real(windhip_solver *s, const void *u, const void *v, const void *w_grid, void *w_real)

@@ -162,6 +162,18 @@ int windhip_divergence(windhip_solver *s, const void *u, const void *v, const vo
  *   w that reproduces a prescribed w_real for the given u, v. */
 int windhip_balance_uvw(windhip_solver *s, const void *u, const void *v, void *w, const void *density, int adv_den,
                         int use_dqdt);
+/* No counterpart in the reference: windhip_balance_uvw for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). Same arguments and results; it calls no
+ * hipMalloc/hipFree: its FP64 divergence scratch is a static device array of
+ * 16,777,216 doubles (128 MiB per GPU, held until the process ends), so the
+ * caller's mass grid ((ime-ims+1) (kme-kms+1) (jme-jms+1) elements) may hold
+ * at most that many elements (else WINDHIP_ERR_ARG), and the handle must
+ * have been created with field_mem = WINDHIP_DEVICE (host fields would be
+ * staged through allocated buffers; else WINDHIP_ERR_ARG). Every call on one
+ * GPU uses the same scratch: calls that overlap on a GPU race. */
+int windhip_balance_uvw_noalloc(windhip_solver *s, const void *u, const void *v, void *w, const void *density,
+                                int adv_den, int use_dqdt);
 int windhip_calc_w_real(windhip_solver *s, const void *u, const void *v, const void *w_grid, void *w_real);
 int windhip_idealized_wgrid(windhip_solver *s, const void *u, const void *v, const void *w_real, void *w);
 
