@@ -39,6 +39,7 @@ EXPORTS = (
     "windhip_set_multigrid", "windhip_mg_level", "windhip_mg_smooth",
     "windhip_balance_uvw", "windhip_calc_w_real", "windhip_idealized_wgrid",
     "windhip_balance_uvw_noalloc",
+    "windhip_probe_lambda_pattern", "windhip_probe_record", "windhip_probe_finalize",
 )
 
 PROFILE_KINDS = {0: "sweep_f64", 1: "sweep_sigma_f64", 2: "update_s_f64", 3: "sweep_omega_f64",
@@ -150,6 +151,9 @@ def load():
         "windhip_mg_smooth": (i, [vp, i, i, fp, fp, fp]),
         "windhip_balance_uvw": (i, [vp, vp, vp, vp, vp, i, i]),
         "windhip_balance_uvw_noalloc": (i, [vp, vp, vp, vp, vp, i, i]),
+        "windhip_probe_lambda_pattern": (i, [vp, i, i, i]),
+        "windhip_probe_record": (i, [vp, vp, i, i, i, dp]),
+        "windhip_probe_finalize": (i, [vp]),
         "windhip_calc_w_real": (i, [vp, vp, vp, vp, vp]),
         "windhip_idealized_wgrid": (i, [vp, vp, vp, vp, vp]),
         "windhip_learned_input": (i, [vp, vp, C.POINTER(C.c_int), dp]),

@@ -393,6 +393,9 @@ struct windhip_solver {
     int divergence(const void *u, const void *v, const void *w, const void *density, int adv_den,
                    int horz_only, int use_dqdt, void *div_out);
     int probe(const void *alpha, const void *density, int adv_den, double *max_leak);
+    int probe_alloc();
+    int probe_finish();
+    int probe_record_field(const void *T, int ca, int cb, int cc, double *max_leak);
     int balance_uvw(const void *u, const void *v, void *w, const void *density, int adv_den, int use_dqdt,
                     double *scr);
     int w_real(const void *u, const void *v, const void *w_grid, void *out);
@@ -1606,10 +1609,10 @@ int windhip_solver::idealized_wgrid(const void *u, const void *v, const void *w_
 // calibrate_projection_operator (wind.F90:1010-1099) with the probe hooks of
 // wind_iterative.F90:1096-1248, on the device and on internal FP64 scratch
 // (u, v, w, the response T and the density in the natural (i,k,j) layout).
-int windhip_solver::probe(const void *alpha, const void *density, int adv_den, double *max_leak)
+// The 15 probed coefficient arrays (FP64 and their FP32 rounding), the leak
+// counter and the natural-layout scratch of probing, made on first use.
+int windhip_solver::probe_alloc()
 {
-    if (!metrics_set) return fail(WINDHIP_ERR_ARG, "windhip_probe_operator: metrics not set");
-    CHK(gather3(alpha, 0, 1, d.nx, 1, d.nz, 1, d.ny, m_alpha, 0));
     if (!P15[0]) {
         for (int q = 0; q < 15; ++q) CHK(alloc(&P15[q], N));
         for (int q = 0; q < 15; ++q) CHK(alloc(&P15_32[q], N));
@@ -1620,6 +1623,58 @@ int windhip_solver::probe(const void *alpha, const void *density, int adv_den, d
     const long nu = (long)(d.nx + 1) * d.nz * d.ny, nv = (long)d.nx * d.nz * (d.ny + 1);
     const long nw = (long)d.nx * d.nz * d.ny;
     if (!scratch_uvw) CHK(alloc(&scratch_uvw, nu + nv + 3 * nw));
+    return WINDHIP_OK;
+}
+
+// probe_finalize (wind_iterative.F90:1232-1248): the recorded operator is the
+// solver's from now on; x_sol (the last pattern) is reset.
+int windhip_solver::probe_finish()
+{
+    for (int q = 0; q < 15; ++q) k_round_f32<<<flat_grid(N), kBlock, 0, stream>>>(N, P15[q], P15_32[q]);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemsetAsync(x, 0, N * sizeof(double), stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    probed = true;
+    structure_uploaded = true;
+    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
+    return WINDHIP_OK;
+}
+
+// probe_record (wind_iterative.F90:1171-1230): 2 T of the caller's response
+// field into the coefficients its colouring reaches; *max_leak is raised to
+// the largest |2 T| off the 15-point footprint.
+int windhip_solver::probe_record_field(const void *T, int ca, int cb, int cc, double *max_leak)
+{
+    CHK(probe_alloc());
+    const long nu = (long)(d.nx + 1) * d.nz * d.ny, nv = (long)d.nx * d.nz * (d.ny + 1);
+    const long nw = (long)d.nx * d.nz * d.ny;
+    double *sT = scratch_uvw + nu + nv + nw;
+    void *dT = nullptr;
+    CHK(view(T, n_mass(), 1, true, &dT));
+    if (desc.field_dtype == WINDHIP_F32)
+        k_to_natural<float><<<flat_grid(nw), kBlock, 0, stream>>>(d, fld<float>(dT, 0), sT);
+    else
+        k_to_natural<double><<<flat_grid(nw), kBlock, 0, stream>>>(d, fld<double>(dT, 0), sT);
+    HIPCHK(hipMemsetAsync(leak, 0, sizeof(unsigned long long), stream));
+    k_probe_record<<<flat_grid(nw), kBlock, 0, stream>>>(d, ca, cb, cc, sT, cf_dev, leak);
+    HIPCHK(hipGetLastError());
+    unsigned long long lk = 0;
+    HIPCHK(hipMemcpyAsync(&lk, leak, sizeof(lk), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    double lv;
+    memcpy(&lv, &lk, sizeof(lv));
+    if (max_leak && lv > *max_leak) *max_leak = lv;
+    return WINDHIP_OK;
+}
+
+int windhip_solver::probe(const void *alpha, const void *density, int adv_den, double *max_leak)
+{
+    if (!metrics_set) return fail(WINDHIP_ERR_ARG, "windhip_probe_operator: metrics not set");
+    CHK(gather3(alpha, 0, 1, d.nx, 1, d.nz, 1, d.ny, m_alpha, 0));
+    CHK(probe_alloc());
+    const long nu = (long)(d.nx + 1) * d.nz * d.ny, nv = (long)d.nx * d.nz * (d.ny + 1);
+    const long nw = (long)d.nx * d.nz * d.ny;
     double *su = scratch_uvw, *sv = su + nu, *sw = sv + nv, *sT = sw + nw, *srho = sT + nw;
     const int use_rho = (adv_den && density) ? 1 : 0;
     if (use_rho) {
@@ -1651,20 +1706,13 @@ int windhip_solver::probe(const void *alpha, const void *density, int adv_den, d
                 k_probe_record<<<flat_grid(nw), kBlock, 0, stream>>>(d, ca, cb, cc, sT, cf_dev, leak);
                 HIPCHK(hipGetLastError());
             }
-    for (int q = 0; q < 15; ++q) k_round_f32<<<flat_grid(N), kBlock, 0, stream>>>(N, P15[q], P15_32[q]);
-    HIPCHK(hipGetLastError());
     unsigned long long lk = 0;
     HIPCHK(hipMemcpyAsync(&lk, leak, sizeof(lk), hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipMemsetAsync(x, 0, N * sizeof(double), stream));  // probe_finalize resets x_sol (:1241)
     HIPCHK(hipStreamSynchronize(stream));
     double lv;
     memcpy(&lv, &lk, sizeof(lv));
     if (max_leak) *max_leak = lv;
-    probed = true;
-    structure_uploaded = true;
-    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
-    graphs.clear();
-    return WINDHIP_OK;
+    return probe_finish();
 }
 
 // =================================================================== C-ABI
@@ -1962,6 +2010,35 @@ int windhip_probe_operator(windhip_solver *s, const void *alpha, const void *den
 {
     if (!s || !alpha) return fail(WINDHIP_ERR_ARG, "windhip_probe_operator: null argument");
     return s->probe(alpha, density, adv_den, max_leak);
+}
+
+int windhip_probe_lambda_pattern(windhip_solver *s, int ca, int cb, int cc)
+{
+    if (!s) return fail(WINDHIP_ERR_ARG, "windhip_probe_lambda_pattern: null handle");
+    if (!s->metrics_set) return fail(WINDHIP_ERR_ARG, "windhip_probe_lambda_pattern: metrics not set");
+    if (ca < 0 || ca > 2 || cb < 0 || cb > 2 || cc < 0 || cc > 2)
+        return fail(WINDHIP_ERR_ARG, "windhip_probe_lambda_pattern: colours must be 0, 1 or 2");
+    CHK(s->probe_alloc());
+    k_probe_pattern<<<flat_grid(s->N), kBlock, 0, s->stream>>>(s->d, ca, cb, cc, s->x);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return WINDHIP_OK;
+}
+
+int windhip_probe_record(windhip_solver *s, const void *T, int ca, int cb, int cc, double *max_leak)
+{
+    if (!s || !T || !max_leak) return fail(WINDHIP_ERR_ARG, "windhip_probe_record: null argument");
+    if (!s->metrics_set) return fail(WINDHIP_ERR_ARG, "windhip_probe_record: metrics not set");
+    if (ca < 0 || ca > 2 || cb < 0 || cb > 2 || cc < 0 || cc > 2)
+        return fail(WINDHIP_ERR_ARG, "windhip_probe_record: colours must be 0, 1 or 2");
+    return s->probe_record_field(T, ca, cb, cc, max_leak);
+}
+
+int windhip_probe_finalize(windhip_solver *s)
+{
+    if (!s) return fail(WINDHIP_ERR_ARG, "windhip_probe_finalize: null handle");
+    if (!s->P15[0]) return fail(WINDHIP_ERR_ARG, "windhip_probe_finalize: nothing was recorded");
+    return s->probe_finish();
 }
 
 int windhip_time_kernel(windhip_solver *s, int which, int reps, double *avg_ms)

@@ -31,11 +31,16 @@ module wind_iterative
     use windhip_interface, only : windhip_solve_info, WINDHIP_PLAIN_F64, WINDHIP_START_WARM
     use wind_iterative_hip, only : wind_tile_t, wind_solver_options_t, tile_init => init_iter_winds, &
                                    tile_calc => calc_iter_winds, tile_finalize => finalize_iter_winds, &
-                                   tile_probe => probe_operator, last_solve_info
+                                   tile_probe => probe_operator, last_solve_info, &
+                                   tile_pattern => probe_lambda_pattern, tile_zero => probe_zero_corrections, &
+                                   tile_apply => probe_apply_corrections, tile_record => probe_record, &
+                                   tile_finalize_probe => probe_finalize
     implicit none
     private
 
     public :: init_iter_winds, calc_iter_winds, finalize_iter_winds
+    ! the probing API that calibrate_projection_operator (wind.F90:1010-1099) uses
+    public :: probe_lambda_pattern, probe_zero_corrections, probe_apply_corrections, probe_record, probe_finalize
     public :: probe_operator, set_wind_solver_choice, wind_tile_of, active_nest, last_solve_info
 
     integer, parameter :: MAX_NESTS = 4   ! wind_iterative.F90:139
@@ -83,6 +88,13 @@ contains
             tile%w            => v(x(kVARS%w)%v)%dqdt_3d
         end associate
         tile%terrain => domain%vars_2d(domain%var_indx(kVARS%neighbor_terrain)%v)%data_2d
+        ! map factors, as calc_updated_winds (wind_iterative.F90:1796-1797) and
+        ! calc_divergence (wind.F90:283-285) read them
+        if (allocated(domain%mapfac_mx_u)) tile%mapfac_mx_u => domain%mapfac_mx_u
+        if (allocated(domain%mapfac_my_u)) tile%mapfac_my_u => domain%mapfac_my_u
+        if (allocated(domain%mapfac_my_v)) tile%mapfac_my_v => domain%mapfac_my_v
+        if (allocated(domain%mapfac_mx_v)) tile%mapfac_mx_v => domain%mapfac_mx_v
+        if (allocated(domain%mapfac_mxy))  tile%mapfac_mxy  => domain%mapfac_mxy
     end subroutine wind_tile_of
 
     !> Solver choices of this build for nests set up afterwards (no counterpart
@@ -149,6 +161,57 @@ contains
         tile%nest_indx = active_nest_indx
         call tile_probe(tile, alpha_in, adv_den, max_leak)
     end subroutine probe_operator
+
+    !> probe_lambda_pattern(ca, cb, cc) (wind_iterative.F90:1096-1116): lambda of
+    !! the active nest = 1 on the physical cells of colour (ca, cb, cc), else 0.
+    subroutine probe_lambda_pattern(ca, cb, cc)
+        integer, intent(in) :: ca, cb, cc
+        if (active_nest_indx < 1) error stop "probe_lambda_pattern: init_iter_winds has not been called"
+        call tile_pattern(active_nest_indx, ca, cb, cc)
+    end subroutine probe_lambda_pattern
+
+    !> probe_zero_corrections(domain) (:1118-1153): u, v, w dqdt_3d set to zero.
+    subroutine probe_zero_corrections(domain)
+        type(domain_t), intent(inout), target :: domain
+        type(wind_tile_t) :: tile
+        call wind_tile_of(domain, tile)
+        call tile_zero(tile)
+    end subroutine probe_zero_corrections
+
+    !> probe_apply_corrections(domain, adv_den) (:1155-1161): calc_updated_winds
+    !! of the active nest's lambda into u, v, w dqdt_3d.
+    subroutine probe_apply_corrections(domain, adv_den)
+        type(domain_t), intent(inout), target :: domain
+        logical, intent(in) :: adv_den
+        type(wind_tile_t) :: tile
+        if (active_nest_indx < 1) error stop "probe_apply_corrections: init_iter_winds has not been called"
+        call wind_tile_of(domain, tile)
+        tile%nest_indx = active_nest_indx
+        call tile_apply(tile, adv_den)
+    end subroutine probe_apply_corrections
+
+    !> probe_record(domain, Tfield, ca, cb, cc, max_leak) (:1163-1230): 2 Tfield
+    !! into the active nest's probed coefficients; max_leak raised to the largest
+    !! response off the 15-point stencil.
+    subroutine probe_record(domain, Tfield, ca, cb, cc, max_leak)
+        type(domain_t), intent(in), target :: domain
+        real, intent(in) :: Tfield(domain%ims:domain%ime, domain%kms:domain%kme, domain%jms:domain%jme)
+        integer, intent(in) :: ca, cb, cc
+        real, intent(inout) :: max_leak
+        type(wind_tile_t) :: tile
+        if (active_nest_indx < 1) error stop "probe_record: init_iter_winds has not been called"
+        call wind_tile_of(domain, tile)
+        tile%nest_indx = active_nest_indx
+        call tile_record(tile, Tfield, ca, cb, cc, max_leak)
+    end subroutine probe_record
+
+    !> probe_finalize(max_leak) (:1232-1248): the active nest solves with the
+    !! probed operator from now on; its lambda is reset.
+    subroutine probe_finalize(max_leak)
+        real, intent(in) :: max_leak
+        if (active_nest_indx < 1) error stop "probe_finalize: init_iter_winds has not been called"
+        call tile_finalize_probe(active_nest_indx, max_leak)
+    end subroutine probe_finalize
 
     !> finalize_iter_winds() (wind_iterative.F90:2142-2267): every nest's state.
     subroutine finalize_iter_winds()

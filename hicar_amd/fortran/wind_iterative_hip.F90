@@ -21,6 +21,8 @@ module wind_iterative_hip
 
     public :: wind_tile_t, wind_solver_options_t
     public :: init_iter_winds, calc_iter_winds, finalize_iter_winds, probe_operator, last_solve_info
+    public :: probe_lambda_pattern, probe_zero_corrections, probe_apply_corrections, probe_record, probe_finalize
+    public :: nest_handle
     public :: calc_divergence, load_learned_guess, balance_uvw, calc_w_real, calc_idealized_wgrid
 
     integer, parameter :: MAX_NESTS = 4   ! wind_iterative.F90:139
@@ -40,6 +42,12 @@ module wind_iterative_hip
         real, pointer, contiguous :: density(:,:,:) => null()
         ! the forcing-tendency winds the solve corrects in place (u%dqdt_3d, ...)
         real, pointer, contiguous :: u(:,:,:) => null(), v(:,:,:) => null(), w(:,:,:) => null()
+        ! map factors (domain_h.F90:91-93; unassociated = 1): m_x and m_y on the
+        ! u grid (ims:ime+1, jms:jme), on the v grid (ims:ime, jms:jme+1), and
+        ! m_x m_y on the mass grid (ims:ime, jms:jme)
+        real, pointer, contiguous :: mapfac_mx_u(:,:) => null(), mapfac_my_u(:,:) => null()
+        real, pointer, contiguous :: mapfac_my_v(:,:) => null(), mapfac_mx_v(:,:) => null()
+        real, pointer, contiguous :: mapfac_mxy(:,:) => null()
     end type wind_tile_t
 
     !> options%wind%wind_solver_iterations and the solver choices of this build.
@@ -50,6 +58,10 @@ module wind_iterative_hip
         integer :: precision = WINDHIP_PLAIN_F64      ! or WINDHIP_MIXED, WINDHIP_MIXED_LINE, WINDHIP_MIXED_MG
         integer :: start = WINDHIP_START_WARM         ! WINDHIP_START_LEARNED once load_learned_guess ran
         logical :: verbose = .false.
+        ! WINDHIP_HOST: the tile's arrays are host memory (staged by the library);
+        ! WINDHIP_DEVICE: their addresses (c_loc of the tile's pointers) are
+        ! device addresses, e.g. pointers set up from device arrays
+        integer :: field_mem = WINDHIP_HOST
     end type wind_solver_options_t
 
     type :: nest_slot_t
@@ -83,13 +95,14 @@ contains
         d%ims = tile%ims; d%ime = tile%ime; d%kms = tile%kms; d%kme = tile%kme
         d%jms = tile%jms; d%jme = tile%jme
         d%field_dtype = WINDHIP_F32
-        d%field_mem = WINDHIP_HOST
+        d%field_mem = options%field_mem
         rc = windhip_create(d, slots(n)%handle)
         call check(rc, "windhip_create")
         rc = windhip_set_metrics(slots(n)%handle, c_loc(tile%dzdx), c_loc(tile%dzdy), c_loc(tile%jacobian), &
                                  c_loc(tile%jacobian_u), c_loc(tile%jacobian_v), c_loc(tile%jacobian_w), &
                                  c_loc(tile%dzdx_u), c_loc(tile%dzdy_v), c_loc(tile%advection_dz), &
-                                 c_loc(tile%terrain), c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr)
+                                 c_loc(tile%terrain), addr2(tile%mapfac_mx_u), addr2(tile%mapfac_my_u), &
+                                 addr2(tile%mapfac_my_v), addr2(tile%mapfac_mx_v), addr2(tile%mapfac_mxy))
         call check(rc, "windhip_set_metrics")
         call windhip_default_opts(slots(n)%opts)
         slots(n)%opts%max_iters = options%wind_solver_iterations
@@ -146,6 +159,61 @@ contains
                                     merge(1_c_int, 0_c_int, adv_den), max_leak)
         call check(rc, "windhip_probe_operator")
     end subroutine probe_operator
+
+    !> probe_lambda_pattern(ca, cb, cc) (wind_iterative.F90:1096-1116) on nest
+    !! nest_indx: lambda = 1 on the physical cells of colour (ca, cb, cc).
+    subroutine probe_lambda_pattern(nest_indx, ca, cb, cc)
+        integer, intent(in) :: nest_indx, ca, cb, cc
+        call check(windhip_probe_lambda_pattern(handle_of(nest_indx), int(ca, c_int), int(cb, c_int), &
+                                                int(cc, c_int)), "windhip_probe_lambda_pattern")
+    end subroutine probe_lambda_pattern
+
+    !> probe_zero_corrections(domain) (:1118-1153): the tile's u, v, w over
+    !! their memory bounds set to zero, so the correction leaves G(lambda) there.
+    subroutine probe_zero_corrections(tile)
+        type(wind_tile_t), intent(inout) :: tile
+        tile%u = 0.0
+        tile%v = 0.0
+        tile%w = 0.0
+    end subroutine probe_zero_corrections
+
+    !> probe_apply_corrections(domain, adv_den) (:1155-1161): the correction
+    !! operator G (calc_updated_winds) from the current lambda.
+    subroutine probe_apply_corrections(tile, adv_den)
+        type(wind_tile_t), intent(inout) :: tile
+        logical, intent(in) :: adv_den
+        type(c_ptr) :: rho
+        rho = c_null_ptr
+        if (adv_den .and. associated(tile%density)) rho = c_loc(tile%density)
+        call check(windhip_apply_correction(handle_of(tile%nest_indx), c_loc(tile%u), c_loc(tile%v), &
+                                            c_loc(tile%w), rho, merge(1_c_int, 0_c_int, adv_den)), &
+                   "windhip_apply_correction")
+    end subroutine probe_apply_corrections
+
+    !> probe_record(domain, Tfield, ca, cb, cc, max_leak) (:1163-1230): 2 T into
+    !! the coefficients of colour (ca, cb, cc); max_leak raised to the largest
+    !! off-stencil |2 T|.
+    subroutine probe_record(tile, Tfield, ca, cb, cc, max_leak)
+        type(wind_tile_t), intent(in) :: tile
+        real, dimension(tile%ims:tile%ime, tile%kms:tile%kme, tile%jms:tile%jme), intent(in), target :: Tfield
+        integer, intent(in) :: ca, cb, cc
+        real, intent(inout) :: max_leak
+        real(c_double) :: lk
+        lk = real(max_leak, c_double)
+        call check(windhip_probe_record(handle_of(tile%nest_indx), c_loc(Tfield), int(ca, c_int), int(cb, c_int), &
+                                        int(cc, c_int), lk), "windhip_probe_record")
+        max_leak = max(max_leak, real(lk))
+    end subroutine probe_record
+
+    !> probe_finalize(max_leak) (:1232-1248): nest nest_indx solves with the
+    !! recorded operator from now on; lambda is reset.
+    subroutine probe_finalize(nest_indx, max_leak)
+        integer, intent(in) :: nest_indx
+        real, intent(in) :: max_leak
+        call check(windhip_probe_finalize(handle_of(nest_indx)), "windhip_probe_finalize")
+        write(*,*) "Poisson projection operator calibrated by probing (A = 2*D o G) on the GPU."
+        write(*,*) "  off-stencil leakage max |T| = ", max_leak, " (should be ~0)"
+    end subroutine probe_finalize
 
     !> calc_divergence(div, domain, advect_density, horz_only, use_dqdt)
     !! (wind.F90:250-523) of the tile's u, v, w on the device.
@@ -245,6 +313,26 @@ contains
         type(windhip_solve_info) :: info
         info = info_last
     end function last_solve_info
+
+    !> The C handle of nest nest_indx (for direct C-ABI calls on it).
+    type(c_ptr) function nest_handle(nest_indx)
+        integer, intent(in) :: nest_indx
+        nest_handle = handle_of(nest_indx)
+    end function nest_handle
+
+    type(c_ptr) function handle_of(nest_indx)
+        integer, intent(in) :: nest_indx
+        if (nest_indx < 1 .or. nest_indx > MAX_NESTS) error stop "wind_iterative_hip: nest index out of range"
+        handle_of = slots(nest_indx)%handle
+        if (.not. c_associated(handle_of)) error stop "wind_iterative_hip: init_iter_winds has not been called"
+    end function handle_of
+
+    !> The address of an optional 2-D field (null when not associated).
+    type(c_ptr) function addr2(a)
+        real, pointer, contiguous, intent(in) :: a(:,:)
+        addr2 = c_null_ptr
+        if (associated(a)) addr2 = c_loc(a)
+    end function addr2
 
     subroutine check(rc, what)
         integer(c_int), intent(in) :: rc

@@ -47,7 +47,8 @@ module windhip_interface
               windhip_divergence, windhip_get_lambda, windhip_set_lambda, windhip_probe_operator, &
               windhip_net_param_count, windhip_net_create, windhip_net_destroy, windhip_net_set_params, &
               windhip_set_net, windhip_set_multigrid, windhip_balance_uvw, windhip_calc_w_real, &
-              windhip_idealized_wgrid, windhip_error_message
+              windhip_idealized_wgrid, windhip_error_message, windhip_probe_lambda_pattern, &
+              windhip_probe_record, windhip_probe_finalize, windhip_apply_operator
 
     interface
         subroutine windhip_default_opts(opts) bind(C, name="windhip_default_opts")
@@ -142,6 +143,36 @@ module windhip_interface
             type(c_ptr), value :: handle, alpha, density
             integer(c_int), value :: adv_den
             real(c_double), intent(out) :: max_leak
+        end function
+
+        ! the reference's probing hooks one by one (wind_iterative.F90:1096-1248)
+        integer(c_int) function windhip_probe_lambda_pattern(handle, ca, cb, cc) &
+                bind(C, name="windhip_probe_lambda_pattern")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle
+            integer(c_int), value :: ca, cb, cc
+        end function
+
+        integer(c_int) function windhip_probe_record(handle, t, ca, cb, cc, max_leak) &
+                bind(C, name="windhip_probe_record")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: handle, t
+            integer(c_int), value :: ca, cb, cc
+            real(c_double), intent(inout) :: max_leak
+        end function
+
+        integer(c_int) function windhip_probe_finalize(handle) bind(C, name="windhip_probe_finalize")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle
+        end function
+
+        ! y = A x with the solver's current operator, rows (0:nx+1, 0:nz+1, 0:ny+1) FP64
+        integer(c_int) function windhip_apply_operator(handle, precision, x_rows, y_rows) &
+                bind(C, name="windhip_apply_operator")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle
+            integer(c_int), value :: precision
+            type(c_ptr), value :: x_rows, y_rows
         end function
 
         ! learned initial guess: load trained parameters and attach the network

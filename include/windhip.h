@@ -187,6 +187,25 @@ int windhip_set_lambda(windhip_solver *s, const double *lambda_rows);
 int windhip_probe_operator(windhip_solver *s, const void *alpha, const void *density,
                            int adv_den, double *max_leak);
 
+/* The reference's probing hooks one by one, for a host that drives the
+ * 27-colour loop of calibrate_projection_operator (wind.F90:1010-1099)
+ * itself with its own divergence operator:
+ *   windhip_probe_lambda_pattern = probe_lambda_pattern(ca, cb, cc)
+ *     (wind_iterative.F90:1096-1116): lambda = 1 on the physical cells with
+ *     (i mod 3, k mod 3, j mod 3) = (ca, cb, cc), 0 elsewhere;
+ *   probe_zero_corrections (:1118-1153) is the caller zeroing its u, v, w;
+ *   probe_apply_corrections (:1155-1161) is windhip_apply_correction;
+ *   windhip_probe_record = probe_record(domain, Tfield, ca, cb, cc, max_leak)
+ *     (:1163-1230): 2 T of the caller's mass-grid field T (its calc_divergence
+ *     of the corrected winds) into the 15 coefficients the colouring reaches;
+ *     *max_leak (in/out) is raised to the largest |2 T| off the stencil;
+ *   windhip_probe_finalize = probe_finalize (:1232-1248): the solver uses the
+ *     recorded operator from now on and lambda is reset to 0.
+ * Colours are 0..2. */
+int windhip_probe_lambda_pattern(windhip_solver *s, int ca, int cb, int cc);
+int windhip_probe_record(windhip_solver *s, const void *T, int ca, int cb, int cc, double *max_leak);
+int windhip_probe_finalize(windhip_solver *s);
+
 /* Per-kernel timing of the solve path: average device time (ms) of `reps`
  * back-to-back launches of one kernel on the handle's stream, bracketed by
  * hipEvents. which: 0 = FP64 matvec, 1 = FP32 matvec. */

@@ -7,7 +7,7 @@
 !!   variable_t data_3d, data_2d, dqdt_3d   src/objects/variable_h.F90:12-17
 !!   index_type v                      src/objects/data_structures.F90:23-26
 !!   domain_t vars_2d, vars_3d, var_indx, dx, memory/domain bounds, grid,
-!!     nest_indx                        src/objects/domain_h.F90:40-110, flow_h.F90:20
+!!     nest_indx, mapfac_*              src/objects/domain_h.F90:40-110, flow_h.F90:20
 !!   options_t general%debug, physics%windtype, wind%wind_solver_iterations
 !!                                      src/objects/options_h.F90:27-43, opt_types.F90:16-46
 !! Only what the adapter reads is declared; the index values are arbitrary.
@@ -48,6 +48,9 @@ module domain_interface
         type(index_type) :: var_indx(kMAX_STORAGE_VARS)
         integer :: ids, ide, jds, jde, kds, kde
         integer :: ims, ime, jms, jme, kms, kme
+        real, allocatable :: mapfac_mx_u(:,:), mapfac_my_u(:,:)
+        real, allocatable :: mapfac_mx_v(:,:), mapfac_my_v(:,:)
+        real, allocatable :: mapfac_mxy(:,:)
     end type domain_t
 end module domain_interface
 
