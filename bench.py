@@ -15,19 +15,25 @@ another stop for the mixed-precision arms):
   line           FP64 FGMRES around an FP32 BiCGStab preconditioned by two
                  vertical line-relaxation steps
   mg             FP64 FGMRES around an FP32 BiCGStab preconditioned by one
-                 multigrid V-cycle (horizontal aggregation, Galerkin coarse
-                 levels, vertical line smoothing)
-  mg_learned     `mg` with the learned guess as its first FGMRES direction
-                 (the headline `value`: learned guess + FP32/FP64 refinement)
+                 multigrid V-cycle (horizontal semi-coarsening, coarse levels
+                 rediscretised from averaged metrics, vertical line
+                 smoothing; the fine level reads bf16 coefficient rows)
+  mg_learned     `mg` started from the learned guess when its residual is
+                 below the zero start's (the headline `value`: learned guess +
+                 FP32/FP64 refinement)
   line_learned, mixed_learned   `line`, `mixed` with the learned guess
 Errors (relative L2) of every arm are reported for lambda, the corrected u,
 v, w and the divergence after the correction, against the plain FP64 solve
 at the same tolerance and against a tight FP64 reference (the plain FP64
 solve to rel. residual 1e-11, untimed). Per-kernel
 device times come from hipEvents recorded around every hot launch on the
-solver's stream (one extra profiled solve per arm). The CPU baseline is the
-oracle (the reference algorithm restated in C, OpenMP) timed on this host on
-a bounded sample and scaled to a full solve.
+solver's stream (one extra profiled solve per arm). Every arm reports the
+solve alone (`ms`, the `value`) and the solve followed by the u, v, w
+correction (`ms_calc_iter_winds`, what calc_iter_winds does). The CPU baseline
+is the reference algorithm restated in C (the oracle's source, built here with
+-O3 -march=native -fopenmp -ffast-math) timed on this host on a bounded sample
+and scaled to a full solve; --cpu-study adds the build without -ffast-math,
+1-thread runs and one full solve.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --sequence S   (the 1024x1024x60 solve sequence instead)
@@ -37,8 +43,8 @@ fractal relief, 60 levels 10 m -> 12 km) whose forcing wind turns 1 deg and is
 rescaled by a seeded U[0.95, 1.05] factor each step (the mean power-law flow is
 rotated on the device; its N(0, 1 m/s) noise stays fixed), each step solved
 by the learned + FP32/FP64 multigrid solve to the mixed tolerance from three
-starts: the network guess (as the first FGMRES direction on top of the
-previous lambda), the previous-step lambda, and zero. Prints one JSON line with
+starts: the network guess (used when its residual is below the previous
+lambda's), the previous-step lambda, and zero. Prints one JSON line with
 ms and iterations per start and the peak device memory in use.
 """
 import argparse
@@ -113,28 +119,93 @@ def build_case(nx, ny, nz):
     return S.fractal_case(nx=nx, ny=ny, nz=nz, dx=50.0, relief=2500.0, speed=8.0, seed=1)
 
 
-def cpu_baseline(case, div_host, iters_full: int, sample_iters: int):
-    """The oracle timed on this host's cores: set-up plus `sample_iters`
-    BiCGStab iterations at full size, scaled to `iters_full` iterations."""
+def cpu_info() -> dict:
+    """The host's CPU model and sizes (lscpu) and the cores this process may use."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        kv = dict(line.split(":", 1) for line in out.splitlines() if ":" in line)
+        kv = {k.strip(): v.strip() for k, v in kv.items()}
+        info["model"] = kv.get("Model name")
+        sockets, per = kv.get("Socket(s)"), kv.get("Core(s) per socket")
+        if sockets and per:
+            info["host_physical_cores"] = int(sockets) * int(per)
+        info["host_logical_cpus"] = int(kv["CPU(s)"]) if "CPU(s)" in kv else None
+        info["threads_per_core"] = int(kv["Thread(s) per core"]) if "Thread(s) per core" in kv else None
+    except (OSError, ValueError, KeyError, subprocess.SubprocessError):
+        pass
+    info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    return info
+
+
+def cpu_baseline(case, div_host, iters_full: int, sample_iters: int = 50, samples: int = 5,
+                 variants=("fastmath",), thread_counts=(None,), full_solve=False, log=None):
+    """The reference algorithm (the oracle's C restatement: the same assembly,
+    preconditioner, BiCGStab and stop rule) timed on this host. It is built
+    here with -O3 -march=native -fopenmp (variant "native") or that plus
+    -ffast-math ("fastmath", the reference's release flags), every vector and
+    preconditioner loop threaded and dot products as plain parallel sums.
+    Per variant and thread count: the median over `samples` runs of the
+    first-call set-up (alpha, assembly, diagonal, rhs), of `sample_iters`
+    BiCGStab iterations and of the correction; the solve is set-up +
+    iters_full iterations (bicgstab_solve's share of calc_iter_winds), and
+    calc_iter_winds adds the correction. full_solve: also one solve to the
+    1e-8 stop with all threads, to check the extrapolation."""
+    import subprocess
+    import tempfile
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
-    o = orc.Oracle(case.nx, case.ny, case.nz, case.dx)
-    o.set_metrics(case.metrics)
-    t0 = time.perf_counter()
-    o.set_alpha(case.alpha)
-    o.assemble()
-    o.extract_diagonal()
-    o.compute_rhs(div_host)
-    t1 = time.perf_counter()
-    o.bicgstab(max_iters=sample_iters, tol_rel=1e-8)
-    t2 = time.perf_counter()
-    per_iter = (t2 - t1) / sample_iters
-    ms = 1e3 * ((t1 - t0) + per_iter * iters_full)
-    o.close()
-    return dict(value=ms, unit="ms/solve", cores=orc.num_threads(), kind="port",
-                sample=f"oracle set-up + {sample_iters} BiCGStab iterations at {case.nx}x{case.ny}x{case.nz} "
-                       f"({t2 - t0:.1f} s), scaled to the {iters_full} iterations of the plain FP64 solve",
-                ms_per_iter=1e3 * per_iter)
+    out_dir = tempfile.mkdtemp(prefix="oracle_baseline_")
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "baseline", f"BASE_OUT={out_dir}"], check=True)
+    libs = {"native": os.path.join(out_dir, "liboracle_native.so"),
+            "fastmath": os.path.join(out_dir, "liboracle_native_fastmath.so")}
+    zeros = np.zeros((case.ny + 2, case.nz + 2, case.nx + 2))
+    res = {}
+    for var in variants:
+        o = orc.Oracle(case.nx, case.ny, case.nz, case.dx, lib_path=libs[var])
+        o.set_metrics(case.metrics)
+        for nt in thread_counts:
+            o.lib.orc_set_threads(int(nt) if nt else len(os.sched_getaffinity(0)))
+            threads = o.lib.orc_num_threads()
+            su, it, co = [], [], []
+            for _ in range(samples):
+                o.set_lambda(zeros)
+                t0 = time.perf_counter()
+                o.set_alpha(case.alpha)
+                o.assemble()
+                o.extract_diagonal()
+                o.compute_rhs(div_host)
+                t1 = time.perf_counter()
+                o.bicgstab(max_iters=sample_iters, tol_rel=1e-30)
+                t2 = time.perf_counter()
+                o.correct(case.u, case.v, case.w, rho=case.rho, adv_den=True)
+                t3 = time.perf_counter()
+                su.append(t1 - t0), it.append((t2 - t1) / sample_iters), co.append(t3 - t2)
+            setup, per_iter, corr = (float(np.median(a)) for a in (su, it, co))
+            r = dict(threads=threads, setup_ms=round(1e3 * setup, 2), ms_per_iter=round(1e3 * per_iter, 3),
+                     correction_ms=round(1e3 * corr, 2), solve_ms=round(1e3 * (setup + per_iter * iters_full), 1),
+                     calc_iter_winds_ms=round(1e3 * (setup + per_iter * iters_full + corr), 1),
+                     iters_spread=[round(1e3 * min(it), 3), round(1e3 * max(it), 3)])
+            res[f"{var}_t{threads}"] = r
+            if log:
+                log(f"cpu baseline {var}, {threads} threads: {r}")
+        if full_solve:
+            o.lib.orc_set_threads(len(os.sched_getaffinity(0)))
+            o.set_lambda(zeros)
+            t0 = time.perf_counter()
+            o.set_alpha(case.alpha)
+            o.assemble()
+            o.extract_diagonal()
+            o.compute_rhs(div_host)
+            rr = o.bicgstab(max_iters=20000, tol_rel=1e-8)
+            t1 = time.perf_counter()
+            res[f"{var}_full_solve"] = dict(threads=o.lib.orc_num_threads(), ms=round(1e3 * (t1 - t0), 1),
+                                            iters=rr["iters"], status=rr["status"])
+            if log:
+                log(f"cpu baseline {var}, full solve: {res[f'{var}_full_solve']}")
+        o.close()
+    return res
 
 
 def main():
@@ -148,7 +219,10 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=0.5)
     ap.add_argument("--arms", default="plain,mixed,line,mg,mg_learned")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-iters", type=int, default=6)
+    ap.add_argument("--cpu-iters", type=int, default=50, help="BiCGStab iterations per CPU-baseline sample")
+    ap.add_argument("--cpu-samples", type=int, default=5, help="CPU-baseline samples (median)")
+    ap.add_argument("--cpu-study", action="store_true",
+                    help="CPU baseline with and without -ffast-math, at all threads and at 1, and one full solve")
     ap.add_argument("--mixed-tol", type=float, default=1e-8,
                     help="rel. residual the mixed-precision arms stop at (default the metric's 1e-8)")
     ap.add_argument("--tight", type=float, default=1e-11,
@@ -213,19 +287,31 @@ def main():
     want = [a for a in args.arms.split(",") if a in arms and (net is not None or "learned" not in a)]
     zeros = np.zeros(s.rows_shape)
 
-    def solve(arm, profile=0, tol=None, a=None):
+    def inner_tol_of(arm):
+        return arms[arm].get("inner_tol", args.inner_tol) if arms[arm]["precision"] != nat.PLAIN_F64 else None
+
+    def solve(arm, profile=0, tol=None, a=None, correct=False):
+        """One solve from a zero lambda (ms); with correct, also the solve
+        followed by the u, v, w correction of calc_iter_winds (ms)."""
         o = dict(arms[arm])
         inner = o.pop("inner_tol", args.inner_tol)
         if tol is None:
             tol = 1e-8 if o["precision"] == nat.PLAIN_F64 else args.mixed_tol
         learned.attach(s, net if o["start"] == nat.START_LEARNED else None)
         s.set_lambda(zeros)
+        if correct:
+            uu, vv, ww = u.clone(), v.clone(), w.clone()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         info = s.solve(alpha if a is None else a, div, tol_rel=tol, max_iters=50000, inner_tol=inner,
                        profile=profile, **o)
         torch.cuda.synchronize()
-        return 1e3 * (time.perf_counter() - t0), info
+        t1 = time.perf_counter()
+        if not correct:
+            return 1e3 * (t1 - t0), info
+        s.apply_correction(uu, vv, ww, density=rho, adv_den=True)
+        torch.cuda.synchronize()
+        return 1e3 * (t1 - t0), info, 1e3 * (time.perf_counter() - t0)
 
     def winds():
         uu, vv, ww = u.clone(), v.clone(), w.clone()
@@ -255,17 +341,20 @@ def main():
     for arm in want:
         for _ in range(args.warmup):
             solve(arm)
-        times, info = [], None
+        times, times_ciw, info = [], [], None
         for _ in range(args.steps):
-            ms, info = solve(arm)
+            ms, info, ms_ciw = solve(arm, correct=True)
             times.append(ms)
+            times_ciw.append(ms_ciw)
         lam = s.get_lambda()
         wnd = winds()
         s.profile(reset=True)
         solve(arm, profile=1)
         prof = s.profile(reset=True)
-        r = dict(ms=float(np.mean(times)), iters=info["iters"], outer=info["outer_iters"], status=info["status"],
+        r = dict(ms=float(np.mean(times)), ms_calc_iter_winds=round(float(np.mean(times_ciw)), 3),
+                 iters=info["iters"], outer=info["outer_iters"], status=info["status"],
                  tol_rel=1e-8 if arms[arm]["precision"] == nat.PLAIN_F64 else args.mixed_tol,
+                 inner_tol=inner_tol_of(arm),
                  true_rel_res=info["true_rel_res"], start_used=info["start_used"], prof=prof)
         if arm == "plain":
             ref = (lam, wnd)
@@ -356,13 +445,39 @@ def main():
 
     cpu = None
     if not args.no_cpu and "plain" in res:
-        cpu = cpu_baseline(case, div.cpu().numpy(), res["plain"]["iters"], args.cpu_iters)
+        iters_full = res["plain"]["iters"]
+        study = args.cpu_study
+        cres = cpu_baseline(case, div.cpu().numpy(), iters_full, sample_iters=args.cpu_iters,
+                            samples=args.cpu_samples, variants=("fastmath", "native") if study else ("fastmath",),
+                            thread_counts=(None, 1) if study else (None,), full_solve=study, log=log)
+        main_key = next(k for k in cres if k.startswith("fastmath_t") and cres[k]["threads"] > 1) \
+            if any(k.startswith("fastmath_t") and cres[k]["threads"] > 1 for k in cres) else next(iter(cres))
+        m = cres[main_key]
+        cpu = dict(value=m["solve_ms"], unit="ms/solve", cores=m["threads"], kind="port",
+                   sample=f"the reference algorithm restated in C (oracle/wind_oracle.c, -O3 -march=native "
+                          f"-fopenmp -ffast-math, threaded loops, parallel dot products) at {case.nx}x{case.ny}x"
+                          f"{case.nz}: median of {args.cpu_samples} runs of the first-call set-up and "
+                          f"{args.cpu_iters} BiCGStab iterations, scaled to the {iters_full} iterations of the "
+                          f"plain FP64 solve", calc_iter_winds_ms=m["calc_iter_winds_ms"],
+                   ms_per_iter=m["ms_per_iter"], setup_ms=m["setup_ms"], correction_ms=m["correction_ms"],
+                   cpu=cpu_info(), runs=cres)
         log(f"cpu baseline: {cpu['value']:.0f} ms/solve ({cpu['cores']} threads)")
 
     plain_ms = res["plain"]["ms"] if "plain" in res else None
+    # what the learned start costs or saves: the same multigrid solve from the
+    # zero start beside it, and the network's forward pass alone
+    learned_cost = None
+    if "mg" in res and "mg_learned" in res:
+        learned_cost = dict(mg_ms=round(res["mg"]["ms"], 3), mg_learned_ms=round(res["mg_learned"]["ms"], 3),
+                            net_cost_ms=round(res["mg_learned"]["ms"] - res["mg"]["ms"], 3),
+                            iters_mg=res["mg"]["iters"], iters_mg_learned=res["mg_learned"]["iters"],
+                            outer_mg=res["mg"]["outer"], outer_mg_learned=res["mg_learned"]["outer"],
+                            start_used=res["mg_learned"]["start_used"],
+                            network_forward_ms=netinfo["forward_ms"] if netinfo else None)
     line = {
         "metric": METRIC,
         "value": round(ms_h, 3),
+        "ms_calc_iter_winds": res[head]["ms_calc_iter_winds"],
         "unit": "ms/solve",
         "n_gpus": 1,
         "steps": args.steps,
@@ -371,17 +486,22 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f64+f32+bf16",
+        "dtype_detail": "FP64 outer FGMRES, residual and lambda; FP32 inner BiCGStab and multigrid arithmetic, "
+                        "the fine-level coefficient rows and line factors stored as bf16 (widened to FP32 on "
+                        "load); network bf16 MFMA with FP32 accumulation",
         "data": "synthetic",
         "config": {"workload": f"wind solve {case.nx}x{case.ny}x{case.nz} fractal terrain, rel. residual 1e-8, "
                                f"arm '{head}'", "grid": [case.nx, case.ny, case.nz], "dx_m": case.dx,
-                   "tol_rel": 1e-8, "tol_rel_mixed_arms": args.mixed_tol, "inner_tol": args.inner_tol,
+                   "tol_rel": 1e-8, "tol_rel_mixed_arms": args.mixed_tol,
+                   "inner_tol": {a: inner_tol_of(a) for a in res},
                    "cells": case.cells,
                    "learned_weights": os.path.relpath(WEIGHTS, ROOT) if net is not None else None},
         "roofline": roof,
         "plain_matvec_roofline": plain_mv,
         "cpu_baseline": cpu,
         "speedup_vs_plain": round(plain_ms / ms_h, 3) if plain_ms else None,
+        "learned_start_cost": learned_cost,
         "cells_per_s": case.cells / (ms_h * 1e-3),
         "tight_reference": tight_info if tight is not None else None,
         "arms": res,

@@ -21,6 +21,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib = None
+_variants = {}
 
 COEF_NAMES = "ABCDEFGHIJKLMNO"
 _dp = C.POINTER(C.c_double)
@@ -30,13 +31,23 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def _load():
+def _load(path=None):
+    """The parity build (oracle/liboracle.so), or, with `path`, another build
+    of the same source (the CPU-baseline builds of the Makefile's `baseline`)."""
     global _lib
+    if path is not None:
+        if path not in _variants:
+            _variants[path] = _bind(C.CDLL(path))
+        return _variants[path]
     if _lib is not None:
         return _lib
     if not os.path.exists(_LIB_PATH):
         build()
-    lib = C.CDLL(_LIB_PATH)
+    _lib = _bind(C.CDLL(_LIB_PATH))
+    return _lib
+
+
+def _bind(lib):
     v = C.c_void_p
     i = C.c_int
     d = C.c_double
@@ -73,12 +84,12 @@ def _load():
         "orc_num_threads": (i, []),
         "orc_plant": (None, [i]),
         "orc_set_faithful": (None, [v, i]),
+        "orc_set_threads": (None, [i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
         f.restype = res
         f.argtypes = args
-    _lib = lib
     return lib
 
 
@@ -113,11 +124,12 @@ def plant(code: int) -> None:
 class Oracle:
     """The reference path restated on the CPU, FP64 throughout."""
 
-    def __init__(self, nx: int, ny: int, nz: int, dx: float, faithful: bool = False):
+    def __init__(self, nx: int, ny: int, nz: int, dx: float, faithful: bool = False, lib_path: str | None = None):
         """faithful: round to FP32 where the reference computes in `real`
         (module arrays, assembly, rhs product, correction); see
-        orc_set_faithful in wind_oracle.c."""
-        self.lib = _load()
+        orc_set_faithful in wind_oracle.c. lib_path: another build of the
+        oracle (the CPU-baseline builds), default the parity build."""
+        self.lib = _load(lib_path)
         self.nx, self.ny, self.nz, self.dx = nx, ny, nz, float(dx)
         self.h = self.lib.orc_create(nx, ny, nz, float(dx))
         if not self.h:

@@ -172,6 +172,17 @@ static void colsums_dot(const orc_t *o, const double *a, const double *b, double
     (void)P;
 }
 
+#ifdef ORC_BASELINE
+/* CPU-baseline build: dot products as plain parallel sums (the reference's
+ * local sums and MPI allreduce fix no order either); no bitwise contract. */
+double orc_dot(const orc_t *o, const double *a, const double *b)
+{
+    double s = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : s)
+    for (long n = 0; n < o->N; ++n) s += a[n] * b[n];
+    return s;
+}
+#else
 double orc_dot(const orc_t *o, const double *a, const double *b)
 {
     double *cs = zalloc(o->P);
@@ -180,6 +191,7 @@ double orc_dot(const orc_t *o, const double *a, const double *b)
     free(cs);
     return s;
 }
+#endif
 
 /* ------------------------------------------------------------ create/free */
 orc_t *orc_create(int nx, int ny, int nz, double dx)
@@ -469,9 +481,10 @@ void orc_assemble(orc_t *o)
         o->coef[CA][n] = 1.0;
         o->coef[CD][n] = inv; o->coef[CE][n] = inv; o->coef[CF][n] = inv; o->coef[CG][n] = inv;
     }
-    for (int k = 1; k <= nz; ++k) {
-        double md = (2 * dx) * (DZ(o, 1, k + 1, 1) + DZ(o, 1, k, 1));
-        for (int j = 1; j <= ny; ++j)
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j <= ny; ++j)
+        for (int k = 1; k <= nz; ++k) {
+            double md = (2 * dx) * (DZ(o, 1, k + 1, 1) + DZ(o, 1, k, 1));
             for (int i = 1; i <= nx; ++i) {
                 double ax = M3(o, o->dzdx, i, k, j) / M3(o, o->jaco, i, k, j);
                 double ay = M3(o, o->dzdy, i, k, j) / M3(o, o->jaco, i, k, j);
@@ -594,6 +607,7 @@ void orc_spmv(const orc_t *o, const double *x, double *y)
 void orc_extract_diagonal(orc_t *o)
 {
     int mx = o->mx, my = o->my, mz = o->mz;
+#pragma omp parallel for schedule(static)
     for (int j = 0; j < my; ++j)
         for (int k = 0; k < mz; ++k)
             for (int i = 0; i < mx; ++i) {
@@ -611,6 +625,7 @@ void orc_extract_diagonal(orc_t *o)
 void orc_compute_rhs(orc_t *o, const double *div)
 {
     int mx = o->mx, my = o->my, mz = o->mz;
+#pragma omp parallel for schedule(static)
     for (int j = 0; j < my; ++j)
         for (int k = 0; k < mz; ++k)
             for (int i = 0; i < mx; ++i) {
@@ -675,6 +690,14 @@ int orc_bicgstab(orc_t *o, int max_iters, double tol_rel, double tol_abs,
         orc_apply_precond(o, o->nsweeps, s, sh);
         orc_spmv(o, sh, t);
         /* vec_dots_fused :1454-1488 */
+#ifdef ORC_BASELINE
+        double ts = 0.0, tt = 0.0, rs = 0.0, rt = 0.0, ss = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : ts, tt, rs, rt, ss)
+        for (long n = 0; n < N; ++n) {
+            double tv = t[n], sv = s[n], rv = rh[n];
+            ts += tv * sv; tt += tv * tv; rs += rv * sv; rt += rv * tv; ss += sv * sv;
+        }
+#else
         double *cts = cs, *ctt = cs + o->P, *crs = cs + 2 * o->P, *crt = cs + 3 * o->P, *css = cs + 4 * o->P;
 #pragma omp parallel for schedule(static)
         for (int j = 0; j < o->my; ++j) {
@@ -696,6 +719,7 @@ int orc_bicgstab(orc_t *o, int max_iters, double tol_rel, double tol_abs,
         double ts = orc_canon_reduce(cts, o->mx, o->my), tt = orc_canon_reduce(ctt, o->mx, o->my);
         double rs = orc_canon_reduce(crs, o->mx, o->my), rt = orc_canon_reduce(crt, o->mx, o->my);
         double ss = orc_canon_reduce(css, o->mx, o->my);
+#endif
         if (tt < breakdown_eps) { *status_out = 2; *iters_out = it; break; }
         omega = ts / tt;
         double rho_new = rs - omega * rt;
@@ -738,6 +762,7 @@ int orc_solve(orc_t *o, const double *alpha, const double *div, int max_iters,
     } else {
         long n3 = (long)o->nx * o->nz * o->ny;
         double amin = HUGE_VAL, amax = -HUGE_VAL;
+#pragma omp parallel for schedule(static) reduction(min : amin) reduction(max : amax)
         for (long n = 0; n < n3; ++n) { amin = fmin(amin, o->alpha[n]); amax = fmax(amax, o->alpha[n]); }
         if (amax > amin && !o->probed) orc_update_coefs(o);
     }
@@ -901,6 +926,7 @@ void orc_correct(const orc_t *o, double *u, double *v, double *w, const double *
     double *rho_v = zalloc((long)nx * nz * (ny + 1)), *rho_w = zalloc(n3);
     for (long n = 0; n < n3; ++n) rho[n] = (adv_den && rho_in) ? rho_in[n] : 1.0;
     /* rho_u, :1822-1869 (both domain edges on one rank) */
+#pragma omp parallel for schedule(static)
     for (int j = 1; j <= ny; ++j)
         for (int k = 1; k <= nz; ++k) {
             for (int i = 2; i <= nx; ++i)
@@ -917,6 +943,7 @@ void orc_correct(const orc_t *o, double *u, double *v, double *w, const double *
             V3(o, rho_v, i, k, ny + 1) = M3(o, rho, i, k, ny);
         }
     /* rho_w, :1940-1955 */
+#pragma omp parallel for schedule(static)
     for (int j = 1; j <= ny; ++j)
         for (int i = 1; i <= nx; ++i) {
             for (int k = 1; k <= nz - 1; ++k) {
@@ -926,6 +953,7 @@ void orc_correct(const orc_t *o, double *u, double *v, double *w, const double *
             M3(o, rho_w, i, nz, j) = M3(o, rho, i, nz, j);
         }
     /* u, :1957-2028 */
+#pragma omp parallel for schedule(static)
     for (int j = 1; j <= ny; ++j)
         for (int i = 1; i <= nx + 1; ++i) {
             double ut[nz + 2];
@@ -942,6 +970,7 @@ void orc_correct(const orc_t *o, double *u, double *v, double *w, const double *
             }
         }
     /* v, :1964-2040 */
+#pragma omp parallel for schedule(static)
     for (int j = 1; j <= ny + 1; ++j)
         for (int i = 1; i <= nx; ++i) {
             double vt[nz + 2];
@@ -958,6 +987,7 @@ void orc_correct(const orc_t *o, double *u, double *v, double *w, const double *
             }
         }
     /* w (grid-relative, interface above cell k), :2052-2062 */
+#pragma omp parallel for schedule(static)
     for (int j = 1; j <= ny; ++j)
         for (int k = 1; k <= nz - 1; ++k)
             for (int i = 1; i <= nx; ++i) {
@@ -1207,6 +1237,15 @@ void orc_idealized_wgrid(const orc_t *o, const double *u, const double *v, const
             }
             M3(o, w, i, nz, j) = 0.0;
         }
+}
+
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
 }
 
 int orc_num_threads(void)
