@@ -30,20 +30,27 @@ struct Dims {
     __device__ __forceinline__ long q(int i, int j) const { return (long)j * mx + i; }
 };
 
-// dst(rows, k*P + j*mx + i) = src(i,k,j) * scale for i in [ia,ib], k in [ka,kb], j in [ja,jb]
+// dst(rows, k*P + j*mx + i) = src(i,k,j) * scale for i in [ia,ib], k in [ka,kb], j in [ja,jb];
+// changed (may be null) is set to 1 when any gathered value differs bitwise
+// from what dst held before (the solver's test for a new alpha).
 template <typename F>
 __global__ void k_gather3(Dims d, Fld<F> src, double *__restrict__ dst, int ia, int ib, int ka, int kb,
-                          int ja, int jb, double scale)
+                          int ja, int jb, double scale, double *changed)
 {
     const int ni = ib - ia + 1, nk = kb - ka + 1, nj = jb - ja + 1;
     const long tot = (long)ni * nk * nj;
+    bool diff = false;
     for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
         const int i = ia + (int)(e % ni);
         const long t = e / ni;
         const int k = ka + (int)(t % nk);
         const int j = ja + (int)(t / nk);
-        dst[d.r(i, k, j)] = scale * (double)src.at(i, k, j);
+        const long n = d.r(i, k, j);
+        const double v = scale * (double)src.at(i, k, j);
+        if (changed) diff |= __double_as_longlong(dst[n]) != __double_as_longlong(v);
+        dst[n] = v;
     }
+    if (changed && diff) *changed = 1.0;
 }
 
 template <typename F>
