@@ -40,6 +40,7 @@ EXPORTS = (
     "windhip_balance_uvw", "windhip_calc_w_real", "windhip_idealized_wgrid",
     "windhip_balance_uvw_noalloc",
     "windhip_probe_lambda_pattern", "windhip_probe_record", "windhip_probe_finalize",
+    "windhip_mg_cycle",
 )
 
 PROFILE_KINDS = {0: "sweep_f64", 1: "sweep_sigma_f64", 2: "update_s_f64", 3: "sweep_omega_f64",
@@ -154,6 +155,7 @@ def load():
         "windhip_probe_lambda_pattern": (i, [vp, i, i, i]),
         "windhip_probe_record": (i, [vp, vp, i, i, i, dp]),
         "windhip_probe_finalize": (i, [vp]),
+        "windhip_mg_cycle": (i, [vp, i, i, fp, fp]),
         "windhip_calc_w_real": (i, [vp, vp, vp, vp, vp]),
         "windhip_idealized_wgrid": (i, [vp, vp, vp, vp, vp]),
         "windhip_learned_input": (i, [vp, vp, C.POINTER(C.c_int), dp]),

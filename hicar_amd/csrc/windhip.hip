@@ -390,7 +390,8 @@ struct windhip_solver {
     int launch_batch(int precision, int count, bool use_graph);
     int enqueue_iterations_line(int count, bool multigrid);
     int build_mg();
-    float *mg_coarse(size_t l);
+    float *mg_coarse(size_t l, bool entered = false);
+    bool mg_fused = true;  // fused descent/return kernels on the wavefront-per-column levels
     // form 0: residual + line solve (FP32 rows); 1: one launch, a wavefront
     // per column; 2: residual + line solve on the level's bf16 rows
     void mg_smooth(size_t l, int form, const float *y, float *out);
@@ -916,8 +917,10 @@ void windhip_solver::mg_smooth(size_t l, int form, const float *y, float *out)
 }
 
 // Enqueue the V-cycle below the fine level from coarse level l (rhs in
-// mg[l].b); returns the buffer that holds its correction.
-float *windhip_solver::mg_coarse(size_t l)
+// mg[l].b); returns the buffer that holds its correction. entered: the level
+// above has already written the rhs and the first smoothing step into mg[l].x
+// (k_mg_down_w).
+float *windhip_solver::mg_coarse(size_t l, bool entered)
 {
     MgLevel &L = mg[l];
     const KState *sp = st + 1;
@@ -957,11 +960,11 @@ float *windhip_solver::mg_coarse(size_t l)
         used = std::max<size_t>(q, 1);
     }
     if (l + 1 == mg.size()) {  // one interior column: its line solve is exact
-        line(L.b, nullptr, L.x, 1.0f);
+        if (!entered) line(L.b, nullptr, L.x, 1.0f);
         return L.x;
     }
     float *cur = L.x, *alt = L.t;
-    mg_smooth(l, form, nullptr, cur);
+    if (!entered) mg_smooth(l, form, nullptr, cur);
     if (l + 1 == used) {  // the coarsest level in use: smoothing only
         for (int q = 1; q < mg_cpre + mg_cpost; ++q) {
             mg_smooth(l, form, cur, alt);
@@ -973,14 +976,42 @@ float *windhip_solver::mg_coarse(size_t l)
         mg_smooth(l, form, cur, alt);
         std::swap(cur, alt);
     }
-    resid(cur);
     const MgLevel &C = mg[l + 1];
-    timed(PK_MG_COARSE, [&] {
-        k_mg_restrict<<<mg_plane_grid(C.g), kMgThreads, 0, stream>>>(L.g, C.g, sp, L.r, C.b);
-    });
-    float *e = mg_coarse(l + 1);
-    timed(PK_MG_COARSE, [&] { k_mg_prolong<<<mg_plane_grid(L.g), kMgThreads, 0, stream>>>(L.g, C.g, sp, e, cur); });
-    for (int q = 0; q < mg_cpost; ++q) {
+    // levels of one wavefront per column: the descent (residual, restriction,
+    // the coarse level's first step) and the return (prolongation, the first
+    // post-smoothing step) are one launch each
+    // (measured at 512 x 512 x 40: a wavefront per column touches one cache
+    // line per row, so the descent from a two-kernel level (128 x 128 into
+    // 64 x 64 columns: 45 us against 35 us for the three kernels) and the
+    // return onto the largest wavefront level (19 against 17 us) stay
+    // separate; below them the fused forms take 5-14 against 10-19 us)
+    const bool fuse_down = mg_fused && wave && C.g.P <= mg_wave_cols && C.gg.mz <= 64;
+    const bool fuse_up = mg_fused && L.g.P <= mg_wave_cols / 3 && gl.mz <= 64 && mg_cpost > 0;
+    if (fuse_down) {
+        const float wc = l + 2 == mg.size() ? 1.0f : mg_omega;  // the one-column level is solved exactly
+        timed(PK_MG_COARSE, [&] {
+            k_mg_down_w<OpA<float>><<<(int)((C.g.P + kWaveCols - 1) / kWaveCols), 64 * kWaveCols, 0, stream>>>(
+                op, gl, C.g, sp, cur, L.b, C.lo, C.cp, C.ip, C.b, C.x, wc);
+        });
+    } else {
+        resid(cur);
+        timed(PK_MG_COARSE, [&] {
+            k_mg_restrict<<<mg_plane_grid(C.g), kMgThreads, 0, stream>>>(L.g, C.g, sp, L.r, C.b);
+        });
+    }
+    float *e = mg_coarse(l + 1, fuse_down);
+    int post = mg_cpost;
+    if (fuse_up) {
+        timed(PK_MG_COARSE, [&] {
+            k_mg_up_w<OpA<float>><<<(int)((L.g.P + kWaveCols - 1) / kWaveCols), 64 * kWaveCols, 0, stream>>>(
+                op, gl, C.g, sp, cur, e, L.b, L.lo, L.cp, L.ip, alt, mg_omega);
+        });
+        std::swap(cur, alt);
+        --post;
+    } else {
+        timed(PK_MG_COARSE, [&] { k_mg_prolong<<<mg_plane_grid(L.g), kMgThreads, 0, stream>>>(L.g, C.g, sp, e, cur); });
+    }
+    for (int q = 0; q < post; ++q) {
         mg_smooth(l, form, cur, alt);
         std::swap(cur, alt);
     }
@@ -2246,6 +2277,29 @@ int windhip_mg_smooth(windhip_solver *s, int level, int fused, const float *b, c
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
     HIPCHK(hipMemcpy(out, L.t, bytes, hipMemcpyDeviceToHost));
+    return WINDHIP_OK;
+}
+
+int windhip_mg_cycle(windhip_solver *s, int level, int fused, const float *b, float *out)
+{
+    if (!s || !b || !out) return fail(WINDHIP_ERR_ARG, "windhip_mg_cycle: null argument");
+    if (level < 1 || level > (int)s->mg.size())
+        return fail(WINDHIP_ERR_ARG, "windhip_mg_cycle: level %d outside 1..%zu (run a MIXED_MG solve first)", level,
+                    s->mg.size());
+    windhip_solver::MgLevel &L = s->mg[(size_t)level - 1];
+    const long n = L.g.N;
+    // the V-cycle's kernels return at once when the inner state says done
+    k_state_reset<<<1, 1, 0, s->stream>>>(s->st + 1, 1, 0.0, 0.0, 0.5);
+    HIPCHK(hipMemcpyAsync(L.b, b, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    const bool keep = s->mg_fused, prof = s->profiling;
+    s->mg_fused = fused != 0;
+    s->profiling = false;
+    float *e = s->mg_coarse((size_t)level - 1);
+    s->mg_fused = keep;
+    s->profiling = prof;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, e, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
     return WINDHIP_OK;
 }
 

@@ -171,6 +171,52 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_line(MgGeo g, const KState *s
 // mx + 1 guard elements on both sides (the neighbour loads of the ring
 // columns fall on them); out must alias neither b nor y.
 constexpr int kWaveCols = 4;  // columns (wavefronts) per block
+
+// The damped line solve of one column held one row per lane: returns
+// (L^-1 r)(k) for the lane's row k, with both Thomas recurrences as inclusive
+// lane scans of affine maps on the factors lo, cp, ip of k_line_factor (lanes
+// at or above mz hold zero maps).
+__device__ __forceinline__ float wave_line_solve(int lane, bool act, float r, float lo, float cp, float ip)
+{
+    // forward: d(k) = a(k) d(k-1) + c(k), a = -ip lo, c = ip r, d(-1) = 0
+    float a = act ? -(ip * lo) : 0.f, s = act ? ip * r : 0.f;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const float pa = __shfl_up(a, off, 64), ps = __shfl_up(s, off, 64);
+        if (lane >= off) {
+            s = a * ps + s;
+            a = a * pa;
+        }
+    }
+    // backward: e(k) = -cp(k) e(k+1) + d(k), e(mz) = 0
+    float a2 = act ? -cp : 0.f, s2 = act ? s : 0.f;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const float pa = __shfl_down(a2, off, 64), ps = __shfl_down(s2, off, 64);
+        if (lane + off < 64) {
+            s2 = a2 * ps + s2;
+            a2 = a2 * pa;
+        }
+    }
+    return s2;
+}
+
+// The residual row (b - A y)(row n) of a column held one row per lane, from
+// the lane's own five columns (k-1 and k+1 taken from the neighbouring lanes).
+template <class Op>
+__device__ __forceinline__ float wave_resid(const Op &op, const float *__restrict__ y, float bv, long n, long c,
+                                            int k, int mz, bool lat, long mx)
+{
+    Win<float> x;
+    const float yc = y[n], yw = y[n - 1], ye = y[n + 1], ys = y[n - mx], yn = y[n + mx];
+    x.c1 = yc; x.w1 = yw; x.e1 = ye; x.s1 = ys; x.n1 = yn;
+    x.c0 = __shfl_up(yc, 1, 64); x.w0 = __shfl_up(yw, 1, 64); x.e0 = __shfl_up(ye, 1, 64);
+    x.s0 = __shfl_up(ys, 1, 64); x.n0 = __shfl_up(yn, 1, 64);
+    x.c2 = __shfl_down(yc, 1, 64); x.w2 = __shfl_down(yw, 1, 64); x.e2 = __shfl_down(ye, 1, 64);
+    x.s2 = __shfl_down(ys, 1, 64); x.n2 = __shfl_down(yn, 1, 64);
+    return bv - op.row(k, mz, lat, x, op.load(n), op.bc(c));
+}
+
 template <class Op>
 __global__ __launch_bounds__(64 * kWaveCols) void k_mg_smooth_w(Op op, Geo g, const KState *st,
                                                                const float *__restrict__ b,
@@ -205,28 +251,112 @@ __global__ __launch_bounds__(64 * kWaveCols) void k_mg_smooth_w(Op op, Geo g, co
         const typename Op::Bc bc = op.bc(c);
         r = r - op.row(k, mz, lat, x, f, bc);
     }
-    // forward: d(k) = a(k) d(k-1) + c(k), a = -ip lo, c = ip r, d(-1) = 0
-    const float ipv = ip[n];
-    float a = act ? -(ipv * lo[n]) : 0.f, s = act ? ipv * r : 0.f;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const float pa = __shfl_up(a, off, 64), ps = __shfl_up(s, off, 64);
-        if (lane >= off) {
-            s = a * ps + s;
-            a = a * pa;
-        }
-    }
-    // backward: e(k) = -cp(k) e(k+1) + d(k), e(mz) = 0 (lanes >= mz are zero maps)
-    float a2 = act ? -cp[n] : 0.f, s2 = act ? s : 0.f;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const float pa = __shfl_down(a2, off, 64), ps = __shfl_down(s2, off, 64);
-        if (lane + off < 64) {
-            s2 = a2 * ps + s2;
-            a2 = a2 * pa;
-        }
-    }
+    const float s2 = wave_line_solve(lane, act, r, lo[n], cp[n], ip[n]);
     if (act) out[n] = yc + w * s2;
+}
+
+// The descent into a coarse level in one launch, for coarse levels of at most
+// 64 rows per column: the residual r_f = b_f - A_f y_f of the finer level at
+// the (up to) 2 x 2 fine columns of each coarse column, their mean as the
+// coarse rhs b_c (the restriction of k_mg_restrict, in its order), and the
+// first damped line-smoothing step of the coarse level from zero,
+// x_c = w L_c^-1 b_c. One wavefront per coarse column, lane k holding row k.
+// y_f needs mx_f + 1 guard elements on both sides (the neighbour loads of
+// the finer level's ring columns fall on them). Ring columns of the coarse
+// level get b_c = x_c = 0.
+template <class Op>
+__global__ __launch_bounds__(64 * kWaveCols) void k_mg_down_w(Op opf, Geo gf, MgGeo c, const KState *st,
+                                                             const float *__restrict__ yf,
+                                                             const float *__restrict__ bf,
+                                                             const float *__restrict__ lo,
+                                                             const float *__restrict__ cp,
+                                                             const float *__restrict__ ip,
+                                                             float *__restrict__ bc, float *__restrict__ xc, float w)
+{
+    if (st->done) return;
+    const int lane = threadIdx.x & 63;
+    const long cc = (long)blockIdx.x * kWaveCols + (threadIdx.x >> 6);
+    if (cc >= c.P) return;  // uniform per wavefront
+    const int mz = c.mz;
+    const bool act = lane < mz;
+    const int k = act ? lane : mz - 1;
+    const long n = (long)k * c.P + cc;
+    const int I = (int)(cc % c.mx), J = (int)(cc / c.mx);
+    if (I == 0 || I == c.mx - 1 || J == 0 || J == c.my - 1) {
+        if (act) { bc[n] = 0.f; xc[n] = 0.f; }
+        return;
+    }
+    const long mxf = gf.mx, Pf = gf.P;
+    const int i0 = 2 * I - 1, j0 = 2 * J - 1;
+    const bool hi = i0 + 1 <= (int)mxf - 2, hj = j0 + 1 <= gf.my - 2;
+    const long m = (long)k * Pf + (long)j0 * mxf + i0, cf0 = (long)j0 * mxf + i0;
+    // the fine columns are interior columns of the finer level
+    float s = wave_resid(opf, yf, bf[m], m, cf0, k, mz, false, mxf);
+    if (hi) s += wave_resid(opf, yf, bf[m + 1], m + 1, cf0 + 1, k, mz, false, mxf);
+    if (hj) {
+        float t = wave_resid(opf, yf, bf[m + mxf], m + mxf, cf0 + mxf, k, mz, false, mxf);
+        if (hi) t += wave_resid(opf, yf, bf[m + mxf + 1], m + mxf + 1, cf0 + mxf + 1, k, mz, false, mxf);
+        s += t;
+    }
+    const float b = s / (float)((hi ? 2 : 1) * (hj ? 2 : 1));
+    const float s2 = wave_line_solve(lane, act, b, lo[n], cp[n], ip[n]);
+    if (act) {
+        bc[n] = b;
+        xc[n] = w * s2;
+    }
+}
+
+// The return to a finer level in one launch, for levels of at most 64 rows per
+// column: z = y + P e (the bilinear prolongation of k_mg_prolong of the coarse
+// correction e) at the column and its four neighbours, then one damped
+// line-smoothing step out = z + w L^-1 (b - A z). One wavefront per column of
+// the finer level, lane k holding row k. y needs mx + 1 guard elements on
+// both sides; out must alias neither y nor b.
+template <class Op>
+__global__ __launch_bounds__(64 * kWaveCols) void k_mg_up_w(Op op, Geo g, MgGeo c, const KState *st,
+                                                           const float *__restrict__ y,
+                                                           const float *__restrict__ ec,
+                                                           const float *__restrict__ b,
+                                                           const float *__restrict__ lo,
+                                                           const float *__restrict__ cp,
+                                                           const float *__restrict__ ip,
+                                                           float *__restrict__ out, float w)
+{
+    if (st->done) return;
+    const int lane = threadIdx.x & 63;
+    const long col = (long)blockIdx.x * kWaveCols + (threadIdx.x >> 6);
+    if (col >= g.P) return;  // uniform per wavefront
+    const int mz = g.mz;
+    const bool act = lane < mz;
+    const int k = act ? lane : mz - 1;
+    const long mx = g.mx;
+    const long n = (long)k * g.P + col;
+    const int i = (int)(col % mx), j = (int)(col / mx);
+    const bool lat = i == 0 || i == g.mx - 1 || j == 0 || j == g.my - 1;
+    const float *e = ec + (long)k * c.P;
+    // y + P e at fine column (ii, jj) of row k; ring columns are not prolonged
+    auto z = [&](int ii, int jj) -> float {
+        const float yv = y[(long)k * g.P + (long)jj * mx + ii];
+        if (ii <= 0 || ii >= g.mx - 1 || jj <= 0 || jj >= g.my - 1) return yv;
+        const int I = (ii + 1) >> 1, J = (jj + 1) >> 1;
+        const int In = (ii & 1) ? I - 1 : I + 1, Jn = (jj & 1) ? J - 1 : J + 1;
+        const float v = 0.5625f * e[(long)J * c.mx + I] + 0.1875f * (e[(long)J * c.mx + In] + e[(long)Jn * c.mx + I])
+                        + 0.0625f * e[(long)Jn * c.mx + In];
+        return yv + v;
+    };
+    Win<float> x;
+    // the ring's neighbours outside the level are never read by its rows
+    const float zc = z(i, j);
+    const float zw = i > 0 ? z(i - 1, j) : zc, ze = i < g.mx - 1 ? z(i + 1, j) : zc;
+    const float zs = j > 0 ? z(i, j - 1) : zc, zn = j < g.my - 1 ? z(i, j + 1) : zc;
+    x.c1 = zc; x.w1 = zw; x.e1 = ze; x.s1 = zs; x.n1 = zn;
+    x.c0 = __shfl_up(zc, 1, 64); x.w0 = __shfl_up(zw, 1, 64); x.e0 = __shfl_up(ze, 1, 64);
+    x.s0 = __shfl_up(zs, 1, 64); x.n0 = __shfl_up(zn, 1, 64);
+    x.c2 = __shfl_down(zc, 1, 64); x.w2 = __shfl_down(zw, 1, 64); x.e2 = __shfl_down(ze, 1, 64);
+    x.s2 = __shfl_down(zs, 1, 64); x.n2 = __shfl_down(zn, 1, 64);
+    const float r = b[n] - op.row(k, mz, lat, x, op.load(n), op.bc(col));
+    const float s2 = wave_line_solve(lane, act, r, lo[n], cp[n], ip[n]);
+    if (act) out[n] = zc + w * s2;
 }
 
 // b_c = R r_f, one coarse row per thread: every interior row is the mean of

@@ -249,6 +249,21 @@ class WindSolver:
                                              out.ctypes.data_as(fpp)))
         return out
 
+    def mg_cycle(self, level: int, b, fused: bool = True):
+        """One V-cycle of the last MIXED_MG solve's hierarchy from coarse level
+        `level` for the rhs b, FP32 [mz, ny+2, nx+2] (no counterpart in the
+        reference); fused selects the one-launch descent and return on the
+        levels of one wavefront per column."""
+        dims = (C.c_int * 3)()
+        nat.check(self.lib.windhip_mg_level(self.h, int(level), dims, None))
+        shape = (dims[2], dims[1] + 2, dims[0] + 2)
+        fpp = C.POINTER(C.c_float)
+        bb = np.ascontiguousarray(b, dtype=np.float32).reshape(shape)
+        out = np.zeros(shape, dtype=np.float32)
+        nat.check(self.lib.windhip_mg_cycle(self.h, int(level), int(bool(fused)), bb.ctypes.data_as(fpp),
+                                            out.ctypes.data_as(fpp)))
+        return out
+
     def probe_operator(self, alpha, density=None, adv_den=True) -> float:
         """calibrate_projection_operator (wind.F90:1010-1099): A = 2 D(G(.))."""
         a = self._arr(alpha, self.mass_shape, "alpha")

@@ -56,9 +56,9 @@ enum { WINDHIP_START_WARM = 0, WINDHIP_START_ZERO = 1, WINDHIP_START_GIVEN = 2,
  * refinement; the true residual, its norm, the rho/omega dot products and the
  * lambda accumulation stay FP64. MIXED_LINE: the same with the FP32 BiCGStab
  * preconditioned by two vertical line-relaxation steps instead; MIXED_MG:
- * preconditioned by one multigrid V-cycle (horizontal 2 x 2 aggregation down
- * to one interior column, Galerkin coarse operators, damped vertical line
- * smoothing; windhip_set_multigrid). Neither has a counterpart in the
+ * preconditioned by one multigrid V-cycle (horizontal 2 x 2 semi-coarsening,
+ * coarse operators rediscretised from the averaged metrics at each level's
+ * spacing, damped vertical line smoothing; windhip_set_multigrid). Neither has a counterpart in the
  * reference. */
 enum { WINDHIP_PLAIN_F64 = 0, WINDHIP_MIXED = 1, WINDHIP_MIXED_LINE = 2, WINDHIP_MIXED_MG = 3 };
 
@@ -283,6 +283,16 @@ int windhip_mg_level(windhip_solver *s, int level, int *dims_out, float *coef_ho
  * of up to 44 rows per column (taller levels: as 0). Overwrites the level's
  * work vectors. Inspection only; no counterpart in the reference. */
 int windhip_mg_smooth(windhip_solver *s, int level, int fused, const float *b, const float *y, float *out);
+/* One V-cycle of the MIXED_MG hierarchy from coarse level `level` (as the
+ * inner solve applies it below the fine level) for the rhs b (host FP32, the
+ * level's N rows, layout of windhip_mg_level), from a zero guess; out
+ * receives the level's correction. fused = 1: the levels of one wavefront per
+ * column descend and return in one launch each (residual, restriction and
+ * first step; prolongation and first post-step), as the solve runs them; 0:
+ * separate residual, restriction, smoothing and prolongation kernels.
+ * Overwrites the hierarchy's work vectors. Inspection only; no counterpart in
+ * the reference. */
+int windhip_mg_cycle(windhip_solver *s, int level, int fused, const float *b, float *out);
 /* The network input for the solver's current rhs (that of its last solve),
  * metrics and alpha, written to in_dev (1 x dims[0] x dims[1] x dims[2] x 8);
  * *scale turns the network output into lambda. dims_out receives the padded

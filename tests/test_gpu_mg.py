@@ -185,3 +185,44 @@ def test_mixed_mg_with_f32_fields():
     info = s.solve(a32, div, precision=nat.MIXED_MG, tol_rel=1e-8, start=1, max_iters=20000, inner_tol=0.1)
     assert info["status"] == 0 and info["true_rel_res"] <= 1e-8
     assert _rel(s.get_lambda(), o.get_lambda()) <= 1e-6
+
+
+def test_fused_coarse_cycle_matches_the_separate_kernels():
+    """The V-cycle below each coarse level with the one-launch descent
+    (residual, restriction, first step) and return (prolongation, first
+    post-step) on the levels of one wavefront per column equals the cycle of
+    separate kernels: to FP32 rounding from the first such level down, and to
+    the bf16 rounding of the separate residual (which reads the level's bf16
+    rows) from a larger level. 200 x 180 fine columns: levels of 100 x 90
+    (separate kernels), 50 x 45, 25 x 23, ... (one wavefront per column)."""
+    from hicar_amd import _native as nat
+    from hicar_amd import synthetic as S
+    from hicar_amd.wind_iterative import WindSolver
+    _dev()
+    c = S.fractal_case(nx=200, ny=180, nz=12, dx=50.0, relief=600.0, seed=8, lowest=10.0, top=6000.0)
+    s = WindSolver(c.nx, c.ny, c.nz, c.dx)
+    s.set_metrics(c.metrics)
+    div = s.divergence(c.u, c.v, c.w, density=c.rho, adv_den=True, use_dqdt=True)
+    info = s.solve(c.alpha, div, precision=nat.MIXED_MG, tol_rel=1e-8, start=1, max_iters=2000, inner_tol=0.1)
+    assert info["status"] == 0
+    rng = np.random.default_rng(5)
+    level, seen_wave = 1, False
+    while True:
+        try:
+            (nx, ny, mz), _ = s.mg_level(level)
+        except Exception:
+            break
+        b = rng.standard_normal((mz, ny + 2, nx + 2)).astype(np.float32)
+        b[:, 0, :] = b[:, -1, :] = 0.0
+        b[:, :, 0] = b[:, :, -1] = 0.0
+        fused = s.mg_cycle(level, b, fused=True)
+        sep = s.mg_cycle(level, b, fused=False)
+        wave = (nx + 2) * (ny + 2) <= 4608
+        seen_wave |= wave
+        err = _rel(fused, sep)
+        print(f"level {level} ({nx}x{ny}): fused vs separate {err:.2e}")
+        assert np.all(np.isfinite(fused)) and np.abs(fused).max() > 0
+        assert err <= (1e-5 if wave else 2e-2), (level, err)
+        level += 1
+    assert seen_wave and level > 6
+    s.close()
