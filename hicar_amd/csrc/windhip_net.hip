@@ -143,8 +143,17 @@ __global__ __launch_bounds__(kT) void k_conv3_lds(Shape s, const bf16 *__restric
             }
         }
         __syncthreads();
-#pragma unroll 2
-        for (int ks = 0; ks < KS; ++ks) {
+        // The weight fragments of the k-steps are global loads; waited for
+        // inside the MFMA loop, each k-step cost one memory round trip. They
+        // are loaded ahead: all of the chunk's at once when they fit (WA
+        // k-steps), else a ring PD k-steps ahead of their MFMAs.
+        auto wload = [&](int ks, int nb) -> bf16x8 {
+            const int t = ks * TPS + gt;
+            const int row = n0 + 16 * nb + r16;  // weight rows beyond N read as zero
+            if (t >= 27 || row >= N || nb >= nbl) return zero8;
+            return *reinterpret_cast<const bf16x8 *>(Wt + (long)row * Kp + (long)t * C + c0 + gc * 8);
+        };
+        auto kstep = [&](int ks, const bf16x8 (&bw)[NB]) {
             const int t = ks * TPS + gt;
             const bool tv = t < 27;
             const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
@@ -152,17 +161,38 @@ __global__ __launch_bounds__(kT) void k_conv3_lds(Shape s, const bf16 *__restric
             bf16x8 a[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) a[r] = tv ? tile[off + r * kHx] : zero8;
-            const long kk = (long)t * C + c0 + gc * 8;
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) {
-                if (nb < nbl) {
-                    const int row = n0 + 16 * nb + r16;  // weight rows beyond N read as zero
-                    const bf16x8 bw = tv && row < N ? *reinterpret_cast<const bf16x8 *>(Wt + (long)row * Kp + kk)
-                                                    : zero8;
+            for (int nb = 0; nb < NB; ++nb)
+                if (nb < nbl)
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
-                        acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], bw, acc[r][nb], 0, 0, 0);
-                }
+                        acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], bw[nb], acc[r][nb], 0, 0, 0);
+        };
+        constexpr int WA = KS * NB <= 16 ? KS : 0;  // all of the chunk's weights in registers
+        if constexpr (WA > 0) {
+            bf16x8 bw[KS][NB];
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) bw[ks][nb] = wload(ks, nb);
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) kstep(ks, bw[ks]);
+        } else {
+            constexpr int PD = 3;
+            bf16x8 ring[PD][NB];
+#pragma unroll
+            for (int q = 0; q < PD; ++q)
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) ring[q][nb] = q < KS ? wload(q, nb) : zero8;
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                bf16x8 cur[NB];
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) cur[nb] = ring[ks % PD][nb];
+                if (ks + PD < KS)
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb) ring[ks % PD][nb] = wload(ks + PD, nb);
+                kstep(ks, cur);
             }
         }
     }
@@ -701,36 +731,45 @@ int windhip_net::forward(hipStream_t st, const float *in, Shape s, float *o, boo
     const Shape s1 = s.half(), s2 = s1.half(), s3 = s2.half();
     const long m0 = s.voxels();
     static_assert(kInCh == 8, "k_cast_in8 converts 8 channels per voxel");
+    // The skip tensors: in training the skip channels sit after the upsampled
+    // ones in the concatenation buffers [up | skip], which the weight
+    // gradients read; an inference pass never writes the upsampled channels
+    // (the decoder convolutions read them from the coarse tensors), so it
+    // keeps each skip tensor dense in its buffer (16, 32, 64 channels per
+    // voxel instead of every third or second 16-byte group): contiguous
+    // stores, and the pooling and the decoder read no unused bytes.
+    bf16 *sk0 = infer ? cat0 : cat0 + 32, *sk1 = infer ? cat1 : cat1 + 64, *sk2 = infer ? cat2 : cat2 + 64;
+    const int ls0 = infer ? 16 : 48, ls1 = infer ? 32 : 96, ls2 = infer ? 64 : 128;
     k_cast_in8<<<grid1(m0), kT, 0, st>>>(m0, in, x0);
     conv(st, C0A, s, x0, 8, a0, 16);
-    conv(st, C0B, s, a0, 16, cat0 + 32, 48);
-    k_pool8<<<grid1(s1.voxels() * 2), kT, 0, st>>>(s, 16, cat0 + 32, 48, p1, 16);
+    conv(st, C0B, s, a0, 16, sk0, ls0);
+    k_pool8<<<grid1(s1.voxels() * 2), kT, 0, st>>>(s, 16, sk0, ls0, p1, 16);
     conv(st, C1A, s1, p1, 16, a1, 32);
-    conv(st, C1B, s1, a1, 32, cat1 + 64, 96);
-    k_pool8<<<grid1(s2.voxels() * 4), kT, 0, st>>>(s1, 32, cat1 + 64, 96, p2, 32);
+    conv(st, C1B, s1, a1, 32, sk1, ls1);
+    k_pool8<<<grid1(s2.voxels() * 4), kT, 0, st>>>(s1, 32, sk1, ls1, p2, 32);
     conv(st, C2A, s2, p2, 32, a2, 64);
-    conv(st, C2B, s2, a2, 64, cat2 + 64, 128);
-    k_pool8<<<grid1(s3.voxels() * 8), kT, 0, st>>>(s2, 64, cat2 + 64, 128, p3, 64);
+    conv(st, C2B, s2, a2, 64, sk2, ls2);
+    k_pool8<<<grid1(s3.voxels() * 8), kT, 0, st>>>(s2, 64, sk2, ls2, p3, 64);
     conv(st, C3A, s3, p3, 64, a3, 64);
     conv(st, C3B, s3, a3, 64, b3, 64);
     // decoder: upsample + concatenate [up | skip] + convolve; for inference the
     // convolution reads the up channels from the coarse tensor itself
     if (infer) {
-        conv(st, D2A, s2, cat2 + 64, 128, e2a, 64, b3, 64, 64);
+        conv(st, D2A, s2, sk2, ls2, e2a, 64, b3, 64, 64);
     } else {
         k_up8<<<grid1(s2.voxels() * 8), kT, 0, st>>>(s2, 64, b3, 64, cat2, 128);
         conv(st, D2A, s2, cat2, 128, e2a, 64);
     }
     conv(st, D2B, s2, e2a, 64, e2b, 64);
     if (infer) {
-        conv(st, D1A, s1, cat1 + 64, 96, e1a, 32, e2b, 64, 64);
+        conv(st, D1A, s1, sk1, ls1, e1a, 32, e2b, 64, 64);
     } else {
         k_up8<<<grid1(s1.voxels() * 8), kT, 0, st>>>(s1, 64, e2b, 64, cat1, 96);
         conv(st, D1A, s1, cat1, 96, e1a, 32);
     }
     conv(st, D1B, s1, e1a, 32, e1b, 32);
     if (infer) {
-        conv(st, D0A, s, cat0 + 32, 48, e0a, 16, e1b, 32, 32);
+        conv(st, D0A, s, sk0, ls0, e0a, 16, e1b, 32, 32);
     } else {
         k_up8<<<grid1(m0 * 4), kT, 0, st>>>(s, 32, e1b, 32, cat0, 48);
         conv(st, D0A, s, cat0, 48, e0a, 16);
