@@ -1808,18 +1808,6 @@ __global__ __launch_bounds__(kBlock) void k_axpby(long N, double a, const double
         out[n] = a * x[n] + b * y[n];
 }
 
-// A given FP64 direction as an FGMRES direction: z (its FP32 rounding, kept
-// for the final update) and the FP64 widening of that rounding for the matvec.
-__global__ __launch_bounds__(kBlock) void k_fg_take64(long N, const double *__restrict__ src, float *__restrict__ z,
-                                                      double *__restrict__ z64)
-{
-    for (long n = (long)blockIdx.x * kBlock + threadIdx.x; n < N; n += (long)gridDim.x * kBlock) {
-        const float v = (float)src[n];
-        z[n] = v;
-        z64[n] = (double)v;
-    }
-}
-
 // w = A z (FP64) and h[0] = <w, v0>.
 template <class Op>
 __global__ __launch_bounds__(kThreads) void k_fg_matvec(Op op, Geo g, KState *st, double *partials,

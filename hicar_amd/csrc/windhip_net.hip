@@ -145,15 +145,29 @@ __global__ __launch_bounds__(kT) void k_conv3_lds(Shape s, const bf16 *__restric
         __syncthreads();
         // The weight fragments of the k-steps are global loads; waited for
         // inside the MFMA loop, each k-step cost one memory round trip. They
-        // are loaded ahead: all of the chunk's at once when they fit (WA
-        // k-steps), else a ring PD k-steps ahead of their MFMAs.
+        // are loaded into a ring PD k-steps ahead of their MFMAs (the whole
+        // chunk's fragments at once measured slower: more registers, fewer
+        // blocks per CU).
         auto wload = [&](int ks, int nb) -> bf16x8 {
             const int t = ks * TPS + gt;
             const int row = n0 + 16 * nb + r16;  // weight rows beyond N read as zero
             if (t >= 27 || row >= N || nb >= nbl) return zero8;
             return *reinterpret_cast<const bf16x8 *>(Wt + (long)row * Kp + (long)t * C + c0 + gc * 8);
         };
-        auto kstep = [&](int ks, const bf16x8 (&bw)[NB]) {
+        constexpr int PD = 3;
+        bf16x8 ring[PD][NB];
+#pragma unroll
+        for (int q = 0; q < PD; ++q)
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) ring[q][nb] = q < KS ? wload(q, nb) : zero8;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            bf16x8 bw[NB];
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) bw[nb] = ring[ks % PD][nb];
+            if (ks + PD < KS)
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) ring[ks % PD][nb] = wload(ks + PD, nb);
             const int t = ks * TPS + gt;
             const bool tv = t < 27;
             const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
@@ -167,33 +181,6 @@ __global__ __launch_bounds__(kT) void k_conv3_lds(Shape s, const bf16 *__restric
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
                         acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], bw[nb], acc[r][nb], 0, 0, 0);
-        };
-        constexpr int WA = KS * NB <= 16 ? KS : 0;  // all of the chunk's weights in registers
-        if constexpr (WA > 0) {
-            bf16x8 bw[KS][NB];
-#pragma unroll
-            for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-                for (int nb = 0; nb < NB; ++nb) bw[ks][nb] = wload(ks, nb);
-#pragma unroll
-            for (int ks = 0; ks < KS; ++ks) kstep(ks, bw[ks]);
-        } else {
-            constexpr int PD = 3;
-            bf16x8 ring[PD][NB];
-#pragma unroll
-            for (int q = 0; q < PD; ++q)
-#pragma unroll
-                for (int nb = 0; nb < NB; ++nb) ring[q][nb] = q < KS ? wload(q, nb) : zero8;
-#pragma unroll
-            for (int ks = 0; ks < KS; ++ks) {
-                bf16x8 cur[NB];
-#pragma unroll
-                for (int nb = 0; nb < NB; ++nb) cur[nb] = ring[ks % PD][nb];
-                if (ks + PD < KS)
-#pragma unroll
-                    for (int nb = 0; nb < NB; ++nb) ring[ks % PD][nb] = wload(ks + PD, nb);
-                kstep(ks, cur);
-            }
         }
     }
     const int z = z0 + wave;
