@@ -532,7 +532,7 @@ int windhip_solver::init()
     float **pl32[] = {&bcx32, &bcy32, &bcz32, &dztop32};
     for (float **q : pl32) CHK(alloc(q, d.P));
     CHK(alloc(&partials, (long)std::max(g.nt, chunk_blocks(g)) * 8));
-    CHK(alloc(&partials2, (long)g.nt * 8));
+    CHK(alloc(&partials2, (long)std::max(g.nt, chunk_blocks(g)) * 8));
     CHK(alloc(&scan_part, 3L * 1024));
     CHK(alloc(&scan_out, 4));
     CHK(alloc(&scan_cnt, 4));
@@ -1162,7 +1162,7 @@ int windhip_solver::true_residual(double *rel, const double *xv)
     const double *xx = xv ? xv : x;
     CHK(with_op<double>([&](auto op) {
         return timed(PK_TRUE_RESID, [&] {
-            k_true_resid<decltype(op)><<<g.nt, kThreads, 0, stream>>>(op, g, st, partials, xx, b);
+            k_true_resid<decltype(op)><<<chunk_blocks(g), kColThreads, 0, stream>>>(op, g, st, partials, xx, b);
         });
     }));
     HIPCHK(hipGetLastError());
@@ -1290,7 +1290,7 @@ int windhip_solver::learned_launch(double *res_plain, double **guess, bool f32_r
     if (f32_relax) CHK(line_relax32(gy, net_smooth, net_stream));
     else CHK(line_relax(&gy, &gs, net_smooth, net_stream));
     CHK(with_op<double>([&](auto op) {
-        k_true_resid<decltype(op)><<<g.nt, kThreads, 0, net_stream>>>(op, g, st + 2, partials2, gy, b);
+        k_true_resid<decltype(op)><<<chunk_blocks(g), kColThreads, 0, net_stream>>>(op, g, st + 2, partials2, gy, b);
         return WINDHIP_OK;
     }));
     HIPCHK(hipGetLastError());
@@ -1397,7 +1397,7 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
     auto resid = [&]() -> int {
         CHK(with_op<double>([&](auto op) {
             return timed(PK_IR_RESID, [&] {
-                k_fg_resid<decltype(op)><<<g.nt, kThreads, 0, stream>>>(op, g, st, partials, x, b, fg_v[0], beta2);
+                k_fg_resid<decltype(op)><<<chunk_blocks(g), kColThreads, 0, stream>>>(op, g, st, partials, x, b, fg_v[0], beta2);
             });
         }));
         HIPCHK(hipGetLastError());
@@ -1456,7 +1456,7 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
             // w = A z_j, orthogonalised against v_0..v_j (modified Gram-Schmidt)
             CHK(with_op<double>([&](auto op) {
                 return timed(PK_FG_MATVEC, [&] {
-                    k_fg_matvec<decltype(op)><<<g.nt, kThreads, 0, stream>>>(op, g, st, partials, fg_z64, fg_v[0],
+                    k_fg_matvec<decltype(op)><<<chunk_blocks(g), kColThreads, 0, stream>>>(op, g, st, partials, fg_z64, fg_v[0],
                                                                              fg_v[j + 1], hc);
                 });
             }));

@@ -1037,29 +1037,6 @@ __global__ __launch_bounds__(kBlock) void k_update_xrp(long N, const KState *st,
     }
 }
 
-// ||b - A x||^2 in FP64 (true residual), stored as st->outer_rnorm.
-template <class Op>
-__global__ __launch_bounds__(kThreads) void k_true_resid(Op op, Geo g, KState *st, double *partials,
-                                                       const double *__restrict__ x,
-                                                       const double *__restrict__ b)
-{
-    const TilePos t = tile_pos(g);
-    double acc[2] = {0.0, 0.0};
-    if (t.inner) {
-        auto epi = [&](int, long n, double tv, double, double) {
-            const double bv = b[n];
-            const double rv = bv - tv;
-            acc[0] = acc[0] + rv * rv;
-            acc[1] = acc[1] + bv * bv;
-        };
-        march<Op, false>(op, g, x, t, epi);
-    }
-    reduce_finalize<2>(acc, partials, &st->cnt[3], t.L, g.nt, [&](const double *tot) {
-        st->outer_rnorm = sqrt(tot[0]);
-        st->outer_bnorm = sqrt(tot[1]);
-    });
-}
-
 // ------------------------------------------------------------------ line relaxation
 // One vertical line-relaxation step out = y + M^-1 (in - A y), M the
 // tridiagonal (vertical) part of A in every column: the Thomas algorithm, the
@@ -1515,6 +1492,108 @@ __global__ __launch_bounds__(kColThreads) void k_resid_c(Op op, Geo g, const KSt
     march_pipe(op, g, c.t, y, c.k0, c.k1, epi);
 }
 
+// ---------------------------------------------------------------- FP64 outer residuals
+// The FP64 stencil passes of the outer solves (FGMRES-IR and the true
+// residual) as level-chunked column marches (kChunksK chunks per column,
+// 1-D grid of chunk_blocks(g) blocks of kColThreads). Each row is the FP64
+// row of Op in the reference's term order; only the order of the reductions'
+// partial sums differs from the tiled FP64 BiCGStab kernels.
+
+// ||b - A x||^2 in FP64 (true residual): st->outer_rnorm, st->outer_bnorm.
+template <class Op>
+__global__ __launch_bounds__(kColThreads) void k_true_resid(Op op, Geo g, KState *st, double *partials,
+                                                            const double *__restrict__ x,
+                                                            const double *__restrict__ b)
+{
+    const ChunkPos c = chunk_pos(g);
+    struct Epi {
+        const double *b;
+        double acc[2];
+        bool on;
+        rsrc_t rb;
+        struct Pre { double b; };
+        __device__ void setup(long N) { rb = make_rsrc(b, N * 8); }
+        __device__ Pre load(unsigned so, unsigned vo) const { return Pre{bload<double>(rb, vo, so)}; }
+        __device__ void apply(int, unsigned, unsigned, double ax, double, const Pre &p)
+        {
+            if (!on) return;
+            const double rv = p.b - ax;
+            acc[0] = acc[0] + rv * rv;
+            acc[1] = acc[1] + p.b * p.b;
+        }
+    } epi{b, {0.0, 0.0}, c.t.valid};
+    march_pipe(op, g, c.t, x, c.k0, c.k1, epi);
+    reduce_finalize<2, kColWaves>(epi.acc, partials, &st->cnt[3], c.L, c.nb, [&](const double *tot) {
+        st->outer_rnorm = sqrt(tot[0]);
+        st->outer_bnorm = sqrt(tot[1]);
+    });
+}
+
+// r = b - A x (FP64) into v0, ||r||^2 and ||b||^2 -> st (outer_rnorm, outer_bnorm,
+// outer_target, outer_done); h[0] = ||r||^2 for the first basis vector's scale.
+template <class Op>
+__global__ __launch_bounds__(kColThreads) void k_fg_resid(Op op, Geo g, KState *st, double *partials,
+                                                          const double *__restrict__ x,
+                                                          const double *__restrict__ b, double *__restrict__ v0,
+                                                          double *h)
+{
+    const ChunkPos c = chunk_pos(g);
+    struct Epi {
+        const double *b;
+        double *v0;
+        double acc[2];
+        bool on;
+        rsrc_t rb, rv;
+        struct Pre { double b; };
+        __device__ void setup(long N) { rb = make_rsrc(b, N * 8); rv = make_rsrc(v0, N * 8); }
+        __device__ Pre load(unsigned so, unsigned vo) const { return Pre{bload<double>(rb, vo, so)}; }
+        __device__ void apply(int, unsigned so, unsigned vo, double ax, double, const Pre &p)
+        {
+            if (!on) return;
+            const double rv_ = p.b - ax;
+            bstore(rv_, rv, vo, so);
+            acc[0] = acc[0] + rv_ * rv_;
+            acc[1] = acc[1] + p.b * p.b;
+        }
+    } epi{b, v0, {0.0, 0.0}, c.t.valid};
+    march_pipe(op, g, c.t, x, c.k0, c.k1, epi);
+    reduce_finalize<2, kColWaves>(epi.acc, partials, &st->cnt[3], c.L, c.nb, [&](const double *tot) {
+        st->outer_rnorm = sqrt(tot[0]);
+        st->outer_bnorm = sqrt(tot[1]);
+        st->outer_target = fmax(st->tol_abs, st->tol_rel * st->outer_bnorm);
+        st->outer_done = st->outer_rnorm <= st->outer_target;
+        h[0] = tot[0];
+    });
+}
+
+// w = A z (FP64) and h[0] = <w, v0>.
+template <class Op>
+__global__ __launch_bounds__(kColThreads) void k_fg_matvec(Op op, Geo g, KState *st, double *partials,
+                                                           const double *__restrict__ z64,
+                                                           const double *__restrict__ v0, double *__restrict__ w,
+                                                           double *h)
+{
+    const ChunkPos c = chunk_pos(g);
+    struct Epi {
+        const double *v0;
+        double *w;
+        double acc[1];
+        bool on;
+        rsrc_t rv, rw;
+        struct Pre { double v; };
+        __device__ void setup(long N) { rv = make_rsrc(v0, N * 8); rw = make_rsrc(w, N * 8); }
+        __device__ Pre load(unsigned so, unsigned vo) const { return Pre{bload<double>(rv, vo, so)}; }
+        __device__ void apply(int, unsigned so, unsigned vo, double ax, double, const Pre &p)
+        {
+            if (!on) return;
+            bstore(ax, rw, vo, so);
+            acc[0] = acc[0] + ax * p.v;
+        }
+    } epi{v0, w, {0.0}, c.t.valid};
+    march_pipe(op, g, c.t, z64, c.k0, c.k1, epi);
+    reduce_finalize<1, kColWaves>(epi.acc, partials, &st->cnt[3], c.L, c.nb, [&](const double *tot) { h[0] = tot[0]; });
+}
+
 // out = y + w L^-1 in (y null: out = w L^-1 in), the Thomas algorithm per column
 // with the loads of kLB levels issued together on each pass (the recursion
 // itself is a few FMAs per level; its loads are what costs).
@@ -1723,35 +1802,6 @@ __global__ __launch_bounds__(kColThreads) void k_line_solve_r(Geo g, const KStat
 
 constexpr int kFlatBlocks = 1024;  // grid of the flat (row-parallel) reductions
 
-// r = b - A x (FP64) into v0, ||r||^2 and ||b||^2 -> st (outer_rnorm, outer_bnorm,
-// outer_target, outer_done); h[0] = ||r||^2 for the first basis vector's scale.
-template <class Op>
-__global__ __launch_bounds__(kThreads) void k_fg_resid(Op op, Geo g, KState *st, double *partials,
-                                                       const double *__restrict__ x,
-                                                       const double *__restrict__ b, double *__restrict__ v0,
-                                                       double *h)
-{
-    const TilePos t = tile_pos(g);
-    double acc[2] = {0.0, 0.0};
-    if (t.inner) {
-        auto epi = [&](int, long n, double tv, double, double) {
-            const double bv = b[n];
-            const double rv = bv - tv;
-            v0[n] = rv;
-            acc[0] = acc[0] + rv * rv;
-            acc[1] = acc[1] + bv * bv;
-        };
-        march<Op, false>(op, g, x, t, epi);
-    }
-    reduce_finalize<2>(acc, partials, &st->cnt[3], t.L, g.nt, [&](const double *tot) {
-        st->outer_rnorm = sqrt(tot[0]);
-        st->outer_bnorm = sqrt(tot[1]);
-        st->outer_target = fmax(st->tol_abs, st->tol_rel * st->outer_bnorm);
-        st->outer_done = st->outer_rnorm <= st->outer_target;
-        h[0] = tot[0];
-    });
-}
-
 // Start of one preconditioner application: scale the basis vector v by
 // 1/sqrt(*vsq) in place (FP64) when normalise is set, and set up the FP32
 // BiCGStab for A z = v from z = 0 (bicgstab_solve set-up,
@@ -1806,25 +1856,6 @@ __global__ __launch_bounds__(kBlock) void k_axpby(long N, double a, const double
 {
     for (long n = (long)blockIdx.x * kBlock + threadIdx.x; n < N; n += (long)gridDim.x * kBlock)
         out[n] = a * x[n] + b * y[n];
-}
-
-// w = A z (FP64) and h[0] = <w, v0>.
-template <class Op>
-__global__ __launch_bounds__(kThreads) void k_fg_matvec(Op op, Geo g, KState *st, double *partials,
-                                                        const double *__restrict__ z64,
-                                                        const double *__restrict__ v0, double *__restrict__ w,
-                                                        double *h)
-{
-    const TilePos t = tile_pos(g);
-    double acc[1] = {0.0};
-    if (t.inner) {
-        auto epi = [&](int, long n, double tv, double, double) {
-            w[n] = tv;
-            acc[0] = acc[0] + tv * v0[n];
-        };
-        march<Op, false>(op, g, z64, t, epi);
-    }
-    reduce_finalize<1>(acc, partials, &st->cnt[3], t.L, g.nt, [&](const double *tot) { h[0] = tot[0]; });
 }
 
 // One modified Gram-Schmidt step fused with the next inner product:
