@@ -102,6 +102,9 @@ def kernel_bytes(kind: str, nx: int, ny: int, nz: int, coef_bytes: int = 0) -> f
         return 3 * N * 8 + 6 * 8 * cells + 4 * 8 * P
     if kind == "fg_axpy_dot":  # w (in/out), v_i, v_{i+1}
         return 4 * N * 8
+    if kind == "smooth_f32":  # y (stencil; its centre again for the update), in, coefficients, 3 factor rows -> out
+        fb = coef_bytes or 4
+        return 4 * N * 4 + 6 * fb * cells + 4 * 4 * P + 3 * fb * N
     if kind == "update_s_line_f32":  # r, v -> s
         return 3 * N * 4
     if kind == "update_xrp_line_f32":  # x, p_hat, s_hat, s, t, p, v -> x, r, p
@@ -408,11 +411,15 @@ def main():
             avg = tot / calls
             # the multigrid inner solve reads bf16 coefficient rows
             cb = 2 if arm.startswith("mg") and kind in ("resid_f32", "mv_sigma_f32", "mv_omega_f32",
-                                                         "line_solve_f32") else 0
+                                                         "line_solve_f32", "smooth_f32") else 0
             nb = kernel_bytes(kind, case.nx, case.ny, case.nz, cb)
             gbs = nb / (avg * 1e-3) / 1e9 if nb else None
             out[kind] = dict(calls=int(calls), avg_ms=round(avg, 5), total_ms=round(tot, 2), bytes=nb,
                              GBs=round(gbs, 1) if gbs else None, frac=round(gbs / HBM_PEAK_GBS, 3) if gbs else None)
+            if kind in ("update_s_line_f32", "update_xrp_line_f32"):
+                out[kind]["note"] = ("bytes of the full update: the closing iteration of each inner solve "
+                                     "updates only x (and r), and the vectors written just before are partly "
+                                     "served from the 256 MB Infinity Cache, so frac can exceed 1")
         return out
 
     tables = {arm: kernel_table(r.pop("prof"), arm) for arm, r in res.items()}

@@ -40,7 +40,7 @@ EXPORTS = (
     "windhip_balance_uvw", "windhip_calc_w_real", "windhip_idealized_wgrid",
     "windhip_balance_uvw_noalloc",
     "windhip_probe_lambda_pattern", "windhip_probe_record", "windhip_probe_finalize",
-    "windhip_mg_cycle",
+    "windhip_mg_cycle", "windhip_set_mg_fusion",
 )
 
 PROFILE_KINDS = {0: "sweep_f64", 1: "sweep_sigma_f64", 2: "update_s_f64", 3: "sweep_omega_f64",
@@ -49,8 +49,8 @@ PROFILE_KINDS = {0: "sweep_f64", 1: "sweep_sigma_f64", 2: "update_s_f64", 3: "sw
                  10: "init_f64", 11: "ir_resid", 12: "ir_accumulate", 13: "true_resid",
                  14: "fg_setup", 15: "fg_matvec", 16: "fg_axpy_dot", 17: "line_solve_f32",
                  18: "resid_f32", 19: "mv_sigma_f32", 20: "mv_omega_f32", 21: "mg_transfer_f32",
-                 22: "mg_coarse_f32", 23: "mg_setup"}
-PROFILE_SLOTS = 24  # WINDHIP_PROFILE_KINDS
+                 22: "mg_coarse_f32", 23: "mg_setup", 24: "smooth_f32"}
+PROFILE_SLOTS = 25  # WINDHIP_PROFILE_KINDS
 
 
 class Desc(C.Structure):
@@ -156,6 +156,7 @@ def load():
         "windhip_probe_record": (i, [vp, vp, i, i, i, dp]),
         "windhip_probe_finalize": (i, [vp]),
         "windhip_mg_cycle": (i, [vp, i, i, fp, fp]),
+        "windhip_set_mg_fusion": (i, [vp, i, i]),
         "windhip_calc_w_real": (i, [vp, vp, vp, vp, vp]),
         "windhip_idealized_wgrid": (i, [vp, vp, vp, vp, vp]),
         "windhip_learned_input": (i, [vp, vp, C.POINTER(C.c_int), dp]),

@@ -392,6 +392,10 @@ struct windhip_solver {
     int build_mg();
     float *mg_coarse(size_t l, bool entered = false);
     bool mg_fused = true;  // fused descent/return kernels on the wavefront-per-column levels
+    // one-pass smoothing steps (residual and line solve in one kernel,
+    // k_smooth_r) on the fine level and the larger coarse levels of up to 44
+    // rows per column
+    bool mg_onepass = true;
     // form 0: residual + line solve (FP32 rows); 1: one launch, a wavefront
     // per column; 2: residual + line solve on the level's bf16 rows
     void mg_smooth(size_t l, int form, const float *y, float *out);
@@ -687,7 +691,7 @@ int windhip_solver::read_state(int idx)
 enum { PK_SWEEP = 0, PK_SIGMA = 1, PK_UPD_S = 2, PK_OMEGA = 3, PK_UPD_XRP = 4, PK_INIT = 10,
        PK_IR_RESID = 11, PK_IR_ACC = 12, PK_TRUE_RESID = 13, PK_FG_SETUP = 14, PK_FG_MATVEC = 15,
        PK_FG_AXPY_DOT = 16, PK_LINE_SOLVE = 17, PK_RESID = 18, PK_MV_SIGMA = 19, PK_MV_OMEGA = 20,
-       PK_MG_TRANSFER = 21, PK_MG_COARSE = 22, PK_MG_SETUP = 23, PK_COUNT = 24 };
+       PK_MG_TRANSFER = 21, PK_MG_COARSE = 22, PK_MG_SETUP = 23, PK_SMOOTH = 24, PK_COUNT = 25 };
 
 template <class Launch>
 int windhip_solver::timed(int kind, Launch &&launch)
@@ -813,6 +817,17 @@ int windhip_solver::enqueue_iterations_line(int count, bool multigrid)
         auto resid = [&](const float *y, const float *in, float *r) {
             return timed(PK_RESID, [&] { k_resid_c<Op><<<kb, kColThreads, 0, stream>>>(op, g, sp, y, in, r); });
         };
+        // one multigrid smoothing step out = y + w L^-1 (in - A y): one pass of
+        // k_smooth_r where the columns fit it, else the residual and the line solve
+        auto smooth = [&](const float *in, const float *y, float *out, float w) -> int {
+            if (mg_onepass && g.mz <= 44)
+                return timed(PK_SMOOTH, [&] {
+                    k_smooth_r<Op, 44, unsigned><<<cb, kColThreads, 0, stream>>>(
+                        op, g, sp, in, y, lf_lip16, reinterpret_cast<const unsigned *>(lf_cp16), nullptr, out, w);
+                });
+            CHK(resid(y, in, lf_r));
+            return timed(PK_LINE_SOLVE, [&] { lsolve(lf_r, y, out, w); });
+        };
         const MgGeo f0 = mg_geo(g.mx - 2, g.my - 2, g.mz);
         // out = M^-1 in, y1 and lf_dq as scratch
         auto precond = [&](const float *in, float *y1, float *out) -> int {
@@ -827,8 +842,7 @@ int windhip_solver::enqueue_iterations_line(int count, bool multigrid)
             float *cur = y1, *alt = lf_dq;
             CHK(timed(PK_LINE_SOLVE, [&] { lsolve(in, nullptr, cur, mg_omega); }));
             for (int q = 1; q < mg_pre; ++q) {
-                CHK(resid(cur, in, lf_r));
-                CHK(timed(PK_LINE_SOLVE, [&] { lsolve(lf_r, cur, alt, mg_omega); }));
+                CHK(smooth(in, cur, alt, mg_omega));
                 std::swap(cur, alt);
             }
             if (!mg.empty()) {
@@ -843,8 +857,7 @@ int windhip_solver::enqueue_iterations_line(int count, bool multigrid)
             }
             for (int q = 0; q < mg_post; ++q) {
                 float *dst = q == mg_post - 1 ? out : alt;
-                CHK(resid(cur, in, lf_r));
-                CHK(timed(PK_LINE_SOLVE, [&] { lsolve(lf_r, cur, dst, mg_omega); }));
+                CHK(smooth(in, cur, dst, mg_omega));
                 alt = cur;
                 cur = dst;
             }
@@ -875,9 +888,11 @@ int windhip_solver::enqueue_iterations_line(int count, bool multigrid)
 }
 
 // One damped line-relaxation step on coarse level l, out = y + w L^-1 (b - A y)
-// with b = mg[l].b (y null: out = w L^-1 b): in one launch, a wavefront per
-// column (k_mg_smooth_w), when `wave`; else a residual into mg[l].r and a
-// line solve.
+// with b = mg[l].b (y null: out = w L^-1 b). form 1: one launch, a wavefront
+// per column (k_mg_smooth_w); 0 and 2: a residual into mg[l].r and a line
+// solve on the level's FP32 (0) or bf16 (2) rows and factors; 3 and 4: one
+// pass of k_smooth_r on the bf16 (3) or FP32 (4) rows and factors (levels of
+// up to 44 rows per column, y given; otherwise as 2 and 0).
 void windhip_solver::mg_smooth(size_t l, int form, const float *y, float *out)
 {
     MgLevel &L = mg[l];
@@ -890,7 +905,21 @@ void windhip_solver::mg_smooth(size_t l, int form, const float *y, float *out)
         });
         return;
     }
-    const bool half = form == 2 && L.Q;
+    if (form >= 3 && y && gl.mz <= 44 && (form == 4 || L.Q)) {
+        // one pass: the residual feeds the line solve in registers
+        const int nb = (int)((L.g.P + kSmoothSmallNT - 1) / kSmoothSmallNT);
+        timed(PK_MG_COARSE, [&] {
+            if (form == 3)
+                k_smooth_r<OpAB, 44, unsigned, kSmoothSmallNT><<<nb, kSmoothSmallNT, 0, stream>>>(
+                    level_opab(L), gl, sp, L.b, y, L.lip, reinterpret_cast<const unsigned *>(L.cp16), nullptr, out,
+                    mg_omega);
+            else
+                k_smooth_r<OpA<float>, 44, float, kSmoothSmallNT><<<nb, kSmoothSmallNT, 0, stream>>>(
+                    level_op<float>(L), gl, sp, L.b, y, L.lo, L.cp, L.ip, out, mg_omega);
+        });
+        return;
+    }
+    const bool half = (form == 2 || form == 3) && L.Q;
     const float *in = L.b;
     if (y) {
         timed(PK_MG_COARSE, [&] {
@@ -938,11 +967,12 @@ float *windhip_solver::mg_coarse(size_t l, bool entered)
                 k_mg_line<<<mg_blocks(L.g.P), kMgThreads, 0, stream>>>(L.g, sp, L.lo, L.cp, L.ip, in, y, out, w);
         });
     };
-    // the two-kernel levels smooth on their bf16 rows when they have them
-    const int form = wave ? 1 : (L.Q ? 2 : 0);
+    // the two-kernel levels smooth on their bf16 rows when they have them, in
+    // one pass where their columns fit k_smooth_r
+    const int form = wave ? 1 : (L.Q ? (mg_onepass ? 3 : 2) : 0);
     auto resid = [&](const float *xv) {
         timed(PK_MG_COARSE, [&] {
-            if (form == 2)
+            if (form >= 2)
                 k_resid_c<OpAB><<<kb, kColThreads, 0, stream>>>(level_opab(L), gl, sp, xv, L.b, L.r);
             else
                 k_resid_c<OpA<float>><<<kb, kColThreads, 0, stream>>>(op, gl, sp, xv, L.b, L.r);
@@ -2243,6 +2273,16 @@ int windhip_set_multigrid(windhip_solver *s, int pre, int post, int coarse_pre, 
     return WINDHIP_OK;
 }
 
+int windhip_set_mg_fusion(windhip_solver *s, int fused_transfers, int onepass)
+{
+    if (!s) return fail(WINDHIP_ERR_ARG, "windhip_set_mg_fusion: null handle");
+    s->mg_fused = fused_transfers != 0;
+    s->mg_onepass = onepass != 0;
+    for (auto &kv : s->graphs) (void)hipGraphExecDestroy(kv.second);  // the cached V-cycles change shape
+    s->graphs.clear();
+    return WINDHIP_OK;
+}
+
 int windhip_mg_level(windhip_solver *s, int level, int *dims_out, float *coef_host)
 {
     if (!s || !dims_out) return fail(WINDHIP_ERR_ARG, "windhip_mg_level: null argument");
@@ -2265,7 +2305,7 @@ int windhip_mg_smooth(windhip_solver *s, int level, int fused, const float *b, c
     if (level < 1 || level > (int)s->mg.size())
         return fail(WINDHIP_ERR_ARG, "windhip_mg_smooth: level %d outside 1..%d (run a MIXED_MG solve first)", level,
                     (int)s->mg.size());
-    if (fused < 0 || fused > 2) return fail(WINDHIP_ERR_ARG, "windhip_mg_smooth: form %d outside 0..2", fused);
+    if (fused < 0 || fused > 4) return fail(WINDHIP_ERR_ARG, "windhip_mg_smooth: form %d outside 0..4", fused);
     auto &L = s->mg[level - 1];
     const size_t bytes = sizeof(float) * L.g.N;
     HIPCHK(hipStreamSynchronize(s->stream));

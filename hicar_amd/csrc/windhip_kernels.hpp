@@ -1794,6 +1794,111 @@ __global__ __launch_bounds__(kColThreads) void k_line_solve_r(Geo g, const KStat
     }
 }
 
+// One damped line-relaxation step in a single pass, out = y + w L^-1 (in - A y):
+// each level's residual (the row of Op on the five-column window of y, as in
+// k_resid_c) feeds the forward recursion of the column's Thomas algorithm at
+// once, the forward results stay in registers for the back substitution (as
+// in k_line_solve_r), which reloads the centre column of y. Element by
+// element the arithmetic of k_resid_c followed by k_line_solve_r, without the
+// residual's round trip through memory and in one launch. Columns of at most
+// KM levels, one column per thread, NT threads per block; the operands of a
+// level are loaded PD levels ahead (y must carry the stencil guards).
+constexpr int kSmoothSmallNT = 64;  // block of k_smooth_r on the coarse levels (fewer columns: more blocks)
+template <class Op, int KM, typename FT, int NT = kColThreads, int PD = 1, int LB = 4>
+__global__ __launch_bounds__(NT) void k_smooth_r(Op op, Geo g, const KState *st, const float *__restrict__ in,
+                                                 const float *__restrict__ y, const FT *__restrict__ lo,
+                                                 const FT *__restrict__ cp, const FT *__restrict__ ip,
+                                                 float *__restrict__ out, float w)
+{
+    static_assert(KM % LB == 0, "the level batches must tile KM");
+    using Cf = typename Op::Cf;
+    if (st->done) return;
+    const int L = logical_block(gridDim.x);
+    const long col = (long)L * NT + threadIdx.x;
+    const bool valid = col < g.P;
+    const long c = valid ? col : 0;
+    const int mx = __builtin_amdgcn_readfirstlane(g.mx);
+    const int my = __builtin_amdgcn_readfirstlane(g.my);
+    const int mz = __builtin_amdgcn_readfirstlane(g.mz);
+    const long P = (long)mx * my;
+    const long N = P * mz;
+    const int i = (int)(c % mx), j = (int)(c / mx);
+    const bool lat = i == 0 || i == mx - 1 || j == 0 || j == my - 1;
+    const unsigned oc = (unsigned)c, voc = oc * 4u, pb = (unsigned)(P * 4);
+    const long gd = mx + 1;
+    const unsigned oS = 4u, oW = (unsigned)mx * 4u, oC = (unsigned)(mx + 1) * 4u, oE = (unsigned)(mx + 2) * 4u,
+                   oN = (unsigned)(2 * mx + 1) * 4u;
+    constexpr bool kPair = std::is_same<FT, unsigned>::value;
+    const long nbf = N * (long)sizeof(FT);
+    const rsrc_t ry = make_rsrc(y - gd, (N + 2 * gd) * 4), rin = make_rsrc(in, N * 4), rlo = make_rsrc(lo, nbf),
+                 rip = kPair ? rlo : make_rsrc(ip, nbf), rcp = make_rsrc(cp, kPair ? N * 2 : nbf),
+                 ro = make_rsrc(out, N * 4);
+    const typename Op::Src rcf = op.src(N);
+    const typename Op::Bc bc = op.bc(c);
+    struct Slot {
+        float c, w, e, s, n, b, l, q;
+        Cf cf;
+    };
+    // the operands of step k: y at level k + 1, in, the row's coefficients and
+    // the forward factors at level k (levels past the top re-read the top one)
+    auto fill = [&](Slot &sl, int k) {
+        const unsigned s1 = (unsigned)(k + 1 < mz ? k + 1 : mz - 1) * pb;
+        load_x5(ry, voc, s1 + oS, s1 + oW, s1 + oC, s1 + oE, s1 + oN, sl.c, sl.w, sl.e, sl.s, sl.n);
+        const int kk = k < mz ? k : mz - 1;
+        const unsigned so = (unsigned)kk * pb;
+        sl.b = bload<float>(rin, voc, so);
+        sl.cf = op.load_src(rcf, (long)kk * P, so, oc, voc);
+        fload_lq<FT>(rlo, rip, voc, so, sl.l, sl.q);
+    };
+    Win<float> win = {};
+    load_x5(ry, voc, oS, oW, oC, oE, oN, win.c1, win.w1, win.e1, win.s1, win.n1);
+    Slot sl[PD + 1];
+#pragma unroll
+    for (int k = 0; k < PD; ++k) fill(sl[k], k);
+    float dq[KM];
+    float dprev = 0.f;
+    float cb[2][LB], yb[2][LB];
+    auto ldb = [&](int k1, int s) {
+#pragma unroll
+        for (int e = 0; e < LB; ++e) {
+            const unsigned so = (unsigned)(k1 - e < mz ? k1 - e : mz - 1) * pb;
+            cb[s][e] = fload_c<FT>(rcp, voc, so);
+            yb[s][e] = bload<float>(ry, voc, so + oC);
+        }
+    };
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+        __builtin_amdgcn_sched_barrier(0);
+        if (k + PD < KM) fill(sl[(k + PD) % (PD + 1)], k + PD);
+        else if (k + PD == KM) ldb(KM - 1, 0);
+        const Slot &cur = sl[k % (PD + 1)];
+        if (k > 0) win.shift();
+        win.c2 = cur.c; win.w2 = cur.w; win.e2 = cur.e; win.s2 = cur.s; win.n2 = cur.n;
+        float ax;
+        if (k == 0) ax = op.template row_c<kRowBottom>(lat, win, cur.cf, bc);
+        else if (k == mz - 1) ax = op.template row_c<kRowTop>(lat, win, cur.cf, bc);
+        else ax = op.template row_c<kRowInterior>(lat, win, cur.cf, bc);
+        const float r = cur.b - ax;
+        dprev = (r - cur.l * dprev) * cur.q;
+        dq[k] = dprev;  // levels >= mz hold values that are never used
+        asm volatile("" : "+v"(dprev));
+    }
+    float delta = 0.f;
+#pragma unroll
+    for (int k1 = KM - 1; k1 >= 0; k1 -= LB) {
+        const int s = ((KM - 1 - k1) / LB) & 1;
+        __builtin_amdgcn_sched_barrier(0);
+        if (k1 - LB >= 0) ldb(k1 - LB, s ^ 1);
+#pragma unroll
+        for (int e = 0; e < LB; ++e) {
+            const int k = k1 - e;
+            delta = k < mz ? dq[k] - cb[s][e] * delta : 0.f;
+            if (valid && k < mz) bstore(yb[s][e] + w * delta, ro, voc, (unsigned)k * pb);
+        }
+        asm volatile("" : "+v"(delta));
+    }
+}
+
 // ------------------------------------------------------------------ FGMRES-IR
 // Outer solve of the mixed-precision path: flexible GMRES in FP64 whose
 // preconditioner is an FP32 BiCGStab solve (GMRES-based iterative refinement).

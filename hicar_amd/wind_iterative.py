@@ -221,6 +221,12 @@ class WindSolver:
         nat.check(self.lib.windhip_set_multigrid(self.h, int(pre), int(post), int(coarse_pre), int(coarse_post),
                                                  int(depth), float(omega)))
 
+    def set_mg_fusion(self, fused_transfers: bool = True, onepass: bool = True) -> None:
+        """MIXED_MG kernel forms: the one-launch descent/return between the
+        wavefront-per-column levels and the one-pass smoothing steps (both on by
+        default; off, the separate kernels run). No counterpart in the reference."""
+        nat.check(self.lib.windhip_set_mg_fusion(self.h, int(bool(fused_transfers)), int(bool(onepass))))
+
     def mg_level(self, level: int):
         """Coarse level `level` of the last MIXED_MG solve: ((nx, ny, mz), coefficients [6, mz, ny+2, nx+2]
         = B, C, H, I, L, M of the rediscretised operator)."""
@@ -236,7 +242,9 @@ class WindSolver:
         solve: y + omega L^-1 (b - A y), FP32 [mz, ny+2, nx+2] (no counterpart in
         the reference). fused: True or 1, the one-launch form; False or 0, the
         residual + line-solve kernels on FP32 rows; 2, those kernels on the
-        level's bf16 rows (levels of up to 44 rows per column)."""
+        level's bf16 rows (levels of up to 44 rows per column); 3 and 4, the
+        one-pass step (residual and line solve in one kernel) on the bf16 (3)
+        or FP32 (4) rows, for y given."""
         dims = (C.c_int * 3)()
         nat.check(self.lib.windhip_mg_level(self.h, int(level), dims, None))
         shape = (dims[2], dims[1] + 2, dims[0] + 2)

@@ -266,6 +266,15 @@ int windhip_set_guess_smoothing(windhip_solver *s, int steps);
  * (default 0.9). No counterpart in the reference. */
 int windhip_set_multigrid(windhip_solver *s, int pre, int post, int coarse_pre, int coarse_post, int depth,
                           double omega);
+/* MIXED_MG kernel forms (both on by default): fused_transfers != 0, the
+ * one-launch descent (residual, restriction, first coarse step) and return
+ * (prolongation, first post-step) between the levels of one wavefront per
+ * column; onepass != 0, smoothing steps whose residual feeds the line solve in
+ * registers (one kernel instead of two) on the fine level and the larger
+ * coarse levels of up to 44 rows per column. Off, the separate kernels run;
+ * the one-pass step computes the same values as its two kernels. No
+ * counterpart in the reference. */
+int windhip_set_mg_fusion(windhip_solver *s, int fused_transfers, int onepass);
 /* Coarse level `level` (1 = first coarsening) of the MIXED_MG hierarchy built
  * by the last MIXED_MG solve: dims_out = (nx, ny, mz) of the level (interior
  * columns, rows per column); coef_host (may be null) receives its 6 x N FP32
@@ -279,9 +288,11 @@ int windhip_mg_level(windhip_solver *s, int level, int *dims_out, float *coef_ho
  * rows (layout of windhip_mg_level). fused = 1: the one-launch kernel the
  * V-cycle uses on small levels (a wavefront per column); 0: the residual and
  * line-solve kernels on the level's FP32 rows and factors; 2: the same kernels
- * on its bf16 rows and factors, the form the V-cycle uses on the larger levels
- * of up to 44 rows per column (taller levels: as 0). Overwrites the level's
- * work vectors. Inspection only; no counterpart in the reference. */
+ * on its bf16 rows and factors (levels of up to 44 rows per column; taller
+ * levels: as 0); 3 and 4: the one-pass step (residual and line solve in one
+ * kernel) on the bf16 (3, the form the V-cycle uses on the larger levels) or
+ * FP32 (4) rows and factors, for y given and up to 44 rows per column
+ * (otherwise as 2 and 0). Overwrites the level's work vectors. Inspection only; no counterpart in the reference. */
 int windhip_mg_smooth(windhip_solver *s, int level, int fused, const float *b, const float *y, float *out);
 /* One V-cycle of the MIXED_MG hierarchy from coarse level `level` (as the
  * inner solve applies it below the fine level) for the rhs b (host FP32, the
@@ -311,9 +322,9 @@ int windhip_learned_input(windhip_solver *s, float *in_dev, int *dims_out, doubl
  * s- and x/r/p-updates count as 7 and 9); MIXED_MG adds 21 fine-level
  * restriction and prolongation, 22 every kernel of the coarse levels, 23 the
  * coarse levels' set-up (its fine-level line solves and residuals count as 17
- * and 18). reset != 0 clears the
- * counters after reading. */
-#define WINDHIP_PROFILE_KINDS 24
+ * and 18), 24 fine-level one-pass smoothing steps (residual and line solve in
+ * one kernel). reset != 0 clears the counters after reading. */
+#define WINDHIP_PROFILE_KINDS 25
 int windhip_profile(windhip_solver *s, int reset, double *calls, double *ms);
 
 /* Device memory held by the handle (bytes) and the process high-water mark. */

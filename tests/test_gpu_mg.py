@@ -127,6 +127,11 @@ def test_fused_coarse_smoothing_matches_the_two_kernel_step(name):
             assert _rel(fused, split) <= 1e-5, (level, yy is None)
             half = s.mg_smooth(level, 2, b, yy)
             assert 0 < _rel(half, split) <= 3e-2, (level, yy is None, _rel(half, split))
+            if yy is not None:
+                # the one-pass step (residual into the line solve in registers)
+                # computes its two kernels' values exactly
+                assert np.array_equal(s.mg_smooth(level, 3, b, yy), half), level
+                assert np.array_equal(s.mg_smooth(level, 4, b, yy), split), level
             ring = np.zeros(shape, dtype=bool)
             ring[:, 0, :] = ring[:, -1, :] = ring[:, :, 0] = ring[:, :, -1] = True
             y0 = np.zeros(shape, np.float32) if yy is None else yy
@@ -226,3 +231,28 @@ def test_fused_coarse_cycle_matches_the_separate_kernels():
         level += 1
     assert seen_wave and level > 6
     s.close()
+
+
+@pytest.mark.parametrize("name", ["fractal_alpha", "hill"])
+def test_onepass_smoothing_solve_equals_the_two_kernel_solve(name):
+    """MIXED_MG with the one-pass smoothing steps (fine level and the larger
+    coarse levels) against the same solve with the residual + line-solve
+    kernels: the same iterations and a bitwise equal lambda, since the one-pass
+    kernel rounds every value as the two kernels do."""
+    from hicar_amd import _native as nat
+    c, o, div, s = _setup(name)
+    runs = []
+    for onepass in (True, False):
+        s.set_mg_fusion(True, onepass)
+        s.set_lambda(np.zeros(s.rows_shape))
+        s.profile(reset=True)
+        info = s.solve(c.alpha, div, precision=nat.MIXED_MG, tol_rel=1e-8, start=1, max_iters=20000,
+                       inner_tol=0.1, profile=1)
+        prof = s.profile(reset=True)
+        assert info["status"] == 0 and info["true_rel_res"] <= 1e-8
+        runs.append((info, s.get_lambda(), prof))
+    (a, la, pa), (b, lb, pb) = runs
+    assert pa.get("smooth_f32", (0, 0))[0] > 0 and pb.get("smooth_f32", (0, 0))[0] == 0
+    assert a["iters"] == b["iters"] and a["outer_iters"] == b["outer_iters"]
+    assert np.array_equal(la, lb)
+    s.set_mg_fusion(True, True)
