@@ -206,6 +206,163 @@ __global__ __launch_bounds__(kT) void k_conv3_lds(Shape s, const bf16 *__restric
     }
 }
 
+// Rolling-z form of the convolution for the full-resolution layers (C = CC
+// input channels, a multiple of 8, and 16 output channels): a block owns a
+// 16 x 16 (x, y) patch of output voxels over zl consecutive z levels and
+// marches up it. A ring of four zero-padded 18 x 18 halo planes of all CC
+// channels lives in LDS ([plane][channel group][voxel][8], 16 lanes of a row
+// group read 256 contiguous bytes): while the MFMAs of output plane z read
+// planes z-1, z, z+1, the global loads of plane z+2 are in flight; their LDS
+// stores follow the MFMAs, one barrier per plane. The weight fragments (one B
+// fragment per k-step) are loaded once and stay in registers for the whole
+// march where that leaves the occupancy alone (see kRegW), else they are
+// streamed through a ring PD k-steps ahead. Wave w computes rows y0 + 4w .. y0 + 4w + 3, MFMA row group r the
+// 16-voxel x-run of row 4w + r; the k order is that of k_conv3_lds (k = t C +
+// c, t = 9 dz + 3 dy + dx). With X2 set, channels below csplit are the nearest
+// 2x upsampling of X2 (as in k_conv3_lds), channel c >= csplit channel c -
+// csplit of X.
+constexpr int kRx = 16, kRy = 16, kRPx = kRx + 2, kRPy = kRy + 2, kRPv = kRPx * kRPy;
+template <int CC>
+__global__ __launch_bounds__(kT) void k_conv3_roll(Shape s, const bf16 *__restrict__ X, int ldx,
+                                                  const bf16 *__restrict__ Wt, int Kp, const float *__restrict__ bias,
+                                                  bf16 *__restrict__ Y, int ldy, int relu, int zl,
+                                                  const bf16 *__restrict__ X2, int ldx2, int csplit)
+{
+    constexpr int G = CC / 8;                  // channel groups of 8
+    constexpr int PE = G * kRPv;               // bf16x8 entries of one halo plane
+    constexpr int NE = (PE + kT - 1) / kT;     // of them per thread
+    constexpr int KS = (27 * CC + 31) / 32;    // k-steps
+    // weights held in registers: up to 16 k-steps, or all of them where the
+    // halo ring already limits the CU to one block (the 48-channel layer)
+    constexpr bool kRegW = KS <= 16 || 4 * PE * 16 > 80 * 1024;
+    constexpr int PD = 3;                      // else streamed this many k-steps ahead
+    __shared__ bf16x8 ring[4 * PE];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, g = lane >> 4;
+    const int ntx = (s.W + kRx - 1) / kRx, nty = (s.H + kRy - 1) / kRy, ntz = (s.D + zl - 1) / zl;
+    int b = blockIdx.x;
+    const int tx = b % ntx;
+    b /= ntx;
+    const int ty = b % nty;
+    b /= nty;
+    const int tz = b % ntz;
+    const int bb = b / ntz;
+    const int x0 = tx * kRx, y0 = ty * kRy, z0 = tz * zl;
+    const int z1 = min(z0 + zl, s.D);  // output planes [z0, z1)
+    const long HW = (long)s.H * s.W;
+    const long vb = (long)bb * s.D * HW;
+    const Shape hs = s.half();
+    const bf16x8 zero8 = {};
+    // k-step ks, lane group g: k = 32 ks + 8 g .. + 7, tap k / CC, channels
+    // (k % CC) .. + 7
+    auto tap_of = [&](int ks) { return (32 * ks + 8 * g) / CC; };
+    auto grp_of = [&](int ks) { return ((32 * ks + 8 * g) % CC) / 8; };
+    auto wload = [&](int ks) -> bf16x8 {
+        const int kk = 32 * ks + 8 * g;
+        return kk < 27 * CC ? *reinterpret_cast<const bf16x8 *>(Wt + (long)r16 * Kp + kk) : zero8;
+    };
+    bf16x8 wf[kRegW ? KS : 1];
+    if constexpr (kRegW) {
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) wf[ks] = wload(ks);
+    }
+    const float bv = bias ? bias[r16] : 0.f;
+    // entry q of a plane: voxel q / G (row-major 18 x 18), channel group q % G,
+    // so consecutive threads read consecutive 16-byte pieces of a voxel row
+    auto load_plane = [&](int p, bf16x8 (&v)[NE]) {
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            const int q = threadIdx.x + e * kT;
+            const int vox = q / G, cg = q - vox * G;
+            const int x = x0 + vox % kRPx - 1, y = y0 + vox / kRPx - 1;
+            v[e] = zero8;
+            if (q < PE && p >= 0 && p < s.D && x >= 0 && x < s.W && y >= 0 && y < s.H) {
+                if (X2 != nullptr && cg * 8 < csplit)
+                    v[e] = *reinterpret_cast<const bf16x8 *>(
+                        X2 + ((((long)bb * hs.D + p / 2) * hs.H + y / 2) * hs.W + x / 2) * ldx2 + cg * 8);
+                else
+                    v[e] = *reinterpret_cast<const bf16x8 *>(X + (vb + p * HW + (long)y * s.W + x) * ldx +
+                                                             (X2 != nullptr ? cg * 8 - csplit : cg * 8));
+            }
+        }
+    };
+    auto store_plane = [&](int slot, const bf16x8 (&v)[NE]) {
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            const int q = threadIdx.x + e * kT;
+            const int vox = q / G, cg = q - vox * G;
+            if (q < PE) ring[slot * PE + cg * kRPv + vox] = v[e];
+        }
+    };
+    {
+        bf16x8 v[NE];
+#pragma unroll 1
+        for (int p = 0; p < 3; ++p) {  // planes z0 - 1, z0, z0 + 1 into slots 0, 1, 2
+            load_plane(z0 - 1 + p, v);
+            store_plane(p, v);
+        }
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int z = z0; z < z1; ++z) {
+        const bool more = z + 1 < z1;  // output plane z + 1 needs plane z + 2
+        bf16x8 nxt[NE];
+        if (more) load_plane(z + 2, nxt);
+        f32x4 acc[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
+        bf16x8 wr[PD];
+        if constexpr (!kRegW) {
+#pragma unroll
+            for (int q = 0; q < PD; ++q) wr[q] = wload(q);
+        }
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            bf16x8 bw;
+            if constexpr (kRegW) {
+                bw = wf[ks];
+            } else {
+                bw = wr[ks % PD];
+                if (ks + PD < KS) wr[ks % PD] = wload(ks + PD);
+            }
+            const int t = tap_of(ks);
+            const bool tv = t < 27;
+            const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
+            const int base = ((z - z0 + dz) & 3) * PE + grp_of(ks) * kRPv + (4 * wave + dy) * kRPx + r16 + dx;
+            bf16x8 a[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) a[r] = tv ? ring[base + r * kRPx] : zero8;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], bw, acc[r], 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int y = y0 + 4 * wave + r;
+            if (y >= s.H) continue;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int x = x0 + 4 * g + e;
+                if (x < s.W) {
+                    float o = acc[r][e] + bv;
+                    if (relu) o = fmaxf(o, 0.f);
+                    Y[(vb + z * HW + (long)y * s.W + x) * ldy + r16] = (bf16)o;
+                }
+            }
+        }
+        if (more) store_plane((z - z0 + 3) & 3, nxt);
+        __syncthreads();
+    }
+}
+
+// Levels per block of k_conv3_roll: enough (x, y, z) blocks to fill the chip
+// several times over, at least 4 levels each.
+static int roll_levels(Shape s)
+{
+    const long cols = (long)((s.W + kRx - 1) / kRx) * ((s.H + kRy - 1) / kRy) * s.B;
+    const long ntz = std::max(1L, std::min((long)(s.D + 3) / 4, (2048 + cols - 1) / cols));
+    return (int)((s.D + ntz - 1) / ntz);
+}
+
 // Launch the LDS-staged convolution with the widest channel chunk that divides
 // C and as many 16-channel output blocks per block as N needs (up to 4).
 template <int CC>
@@ -228,6 +385,20 @@ static void conv3_cc(hipStream_t st, Shape s, const bf16 *X, int ldx, int C, con
 static void conv3(hipStream_t st, Shape s, const bf16 *X, int ldx, int C, const bf16 *Wt, int Kp, const float *bias,
                   int N, bf16 *Y, int ldy, int relu, const bf16 *X2 = nullptr, int ldx2 = 0, int csplit = 0)
 {
+    if (N == 16 && (C == 8 || C == 16 || C == 48) && (X2 == nullptr || csplit % 8 == 0)) {
+        // the full-resolution layers: rolling z
+        const int zl = roll_levels(s);
+        const long ntz = (s.D + zl - 1) / zl;
+        const unsigned blocks =
+            (unsigned)((long)((s.W + kRx - 1) / kRx) * ((s.H + kRy - 1) / kRy) * ntz * s.B);
+        if (C == 16)
+            k_conv3_roll<16><<<blocks, kT, 0, st>>>(s, X, ldx, Wt, Kp, bias, Y, ldy, relu, zl, X2, ldx2, csplit);
+        else if (C == 8)
+            k_conv3_roll<8><<<blocks, kT, 0, st>>>(s, X, ldx, Wt, Kp, bias, Y, ldy, relu, zl, X2, ldx2, csplit);
+        else
+            k_conv3_roll<48><<<blocks, kT, 0, st>>>(s, X, ldx, Wt, Kp, bias, Y, ldy, relu, zl, X2, ldx2, csplit);
+        return;
+    }
     if (C % 32 == 0 && csplit % 32 == 0)
         conv3_cc<32>(st, s, X, ldx, C, Wt, Kp, bias, N, Y, ldy, relu, X2, ldx2, csplit);
     else if (C % 16 == 0 && csplit % 16 == 0)
