@@ -111,6 +111,8 @@ contains
         slots(n)%opts%precision = options%precision
         slots(n)%opts%start = options%start
         slots(n)%verbose = options%verbose
+        ! the verbose report's per-region times come from the per-kernel device times
+        slots(n)%opts%profile = merge(1_c_int, 0_c_int, options%verbose)
     end subroutine init_iter_winds
 
     !> calc_iter_winds(domain, alpha_in, div_in, adv_den) (wind_iterative.F90:295-532):
@@ -135,6 +137,7 @@ contains
                        ' precond_n_sweeps=', info_last%nsweeps
             write(*,*) '   Residual at iter 0:    ', info_last%res0
             write(*,*) '   Residual at final iter:', info_last%res_final
+            call report_timing(h, info_last)
         end if
         if (rc == WINDHIP_ERR_DIVERGED) then
             ! the reference stops the model here (wind_iterative.F90:506-528)
@@ -143,6 +146,49 @@ contains
         end if
         call check(rc, "windhip_calc_iter_winds")
     end subroutine calc_iter_winds
+
+    !> The per-solve timing breakdown of the reference's verbose report
+    !! (wind_iterative.F90:447-471), from the device times of the solve's kernels
+    !! (windhip_profile, read and reset): SpMV = every kernel that applies the
+    !! operator (the last Jacobi-Richardson sweep is fused into the two matvec
+    !! kernels of an iteration and counts here); preconditioner = the lone sweeps
+    !! of the retry ladder, line solves, inner residuals and the multigrid
+    !! levels; vector ops = the updates, the FGMRES set-up, Gram-Schmidt steps and
+    !! lambda accumulation. One device, one process: no halo exchange, and the
+    !! dot products are reduced inside their kernels (no allreduce).
+    subroutine report_timing(h, info)
+        type(c_ptr), intent(in) :: h
+        type(windhip_solve_info), intent(in) :: info
+        real(c_double) :: calls(WINDHIP_PROFILE_KINDS), ms(WINDHIP_PROFILE_KINDS)
+        real(c_double) :: total, spmv, prec, vec, other
+        integer :: k
+        integer, parameter :: spmv_kinds(*) = [1, 3, 6, 8, 11, 13, 15, 19, 20]
+        integer, parameter :: prec_kinds(*) = [0, 5, 17, 18, 21, 22, 23, 24]
+        integer, parameter :: vec_kinds(*) = [2, 4, 7, 9, 12, 14, 16]
+        if (windhip_profile(h, 1_c_int, calls, ms) /= WINDHIP_OK) return
+        total = info%ms * 1.0e-3_c_double
+        spmv = 0; prec = 0; vec = 0
+        do k = 1, size(spmv_kinds)
+            spmv = spmv + ms(spmv_kinds(k) + 1) * 1.0e-3_c_double
+        end do
+        do k = 1, size(prec_kinds)
+            prec = prec + ms(prec_kinds(k) + 1) * 1.0e-3_c_double
+        end do
+        do k = 1, size(vec_kinds)
+            vec = vec + ms(vec_kinds(k) + 1) * 1.0e-3_c_double
+        end do
+        if (total <= 0) return
+        other = total - (spmv + prec + vec)
+        write(*,'(A)')                '   --- per-solve timing breakdown (device, seconds) ---'
+        write(*,'(A,F10.4,A)')        '     total           : ', total, ''
+        write(*,'(A,F10.4,A,F6.1,A)') '     SpMV            : ', spmv, '   (', 100 * spmv / total, ' %)'
+        write(*,'(A,F10.4,A,F6.1,A)') '     halo exchange   : ', 0.0_c_double, '   (', 0.0_c_double, ' %)'
+        write(*,'(A,F10.4,A,F6.1,A)') '     preconditioner  : ', prec, '   (', 100 * prec / total, ' %)'
+        write(*,'(A,F10.4,A,F6.1,A)') '     allreduce       : ', 0.0_c_double, '   (', 0.0_c_double, ' %)'
+        write(*,'(A,F10.4,A,F6.1,A)') '     vector ops      : ', vec, '   (', 100 * vec / total, ' %)'
+        write(*,'(A,F10.4,A,F6.1,A)') '     other           : ', other, '   (', 100 * other / total, ' %)'
+        if (info%iters > 0) write(*,'(A,F10.6,A)') '     per-iter total  : ', total / real(info%iters, c_double), ' s'
+    end subroutine report_timing
 
     !> calibrate_projection_operator (wind.F90:1010-1099) on the device: switch the
     !! nest to the probed operator A = 2 D(G(.)).

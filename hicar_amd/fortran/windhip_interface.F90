@@ -20,6 +20,7 @@ module windhip_interface
                                          WINDHIP_START_GIVEN = 2, WINDHIP_START_LEARNED = 3
     integer(c_int), parameter, public :: WINDHIP_PLAIN_F64 = 0, WINDHIP_MIXED = 1, WINDHIP_MIXED_LINE = 2, &
                                          WINDHIP_MIXED_MG = 3
+    integer(c_int), parameter, public :: WINDHIP_PROFILE_KINDS = 25
 
     type, bind(C), public :: windhip_desc
         integer(c_int) :: nx, ny, nz
@@ -48,7 +49,7 @@ module windhip_interface
               windhip_net_param_count, windhip_net_create, windhip_net_destroy, windhip_net_set_params, &
               windhip_set_net, windhip_set_multigrid, windhip_balance_uvw, windhip_calc_w_real, &
               windhip_idealized_wgrid, windhip_error_message, windhip_probe_lambda_pattern, &
-              windhip_probe_record, windhip_probe_finalize, windhip_apply_operator
+              windhip_probe_record, windhip_probe_finalize, windhip_apply_operator, windhip_profile
 
     interface
         subroutine windhip_default_opts(opts) bind(C, name="windhip_default_opts")
@@ -198,6 +199,13 @@ module windhip_interface
         integer(c_int) function windhip_set_net(handle, net) bind(C, name="windhip_set_net")
             import :: c_int, c_ptr
             type(c_ptr), value :: handle, net
+        end function
+
+        integer(c_int) function windhip_profile(handle, reset, calls, ms) bind(C, name="windhip_profile")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: handle
+            integer(c_int), value :: reset
+            real(c_double), intent(out) :: calls(*), ms(*)
         end function
 
         integer(c_int) function windhip_set_multigrid(handle, pre, post, coarse_pre, coarse_post, depth, &

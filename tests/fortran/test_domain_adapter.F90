@@ -43,8 +43,10 @@ program test_domain_adapter
     info = last_solve_info()
     iters1 = info%iters
 
-    ! 2. nest 2
+    ! 2. nest 2, with the verbose report (options%general%debug)
+    opts%general%debug = .true.
     call init_iter_winds(d2, opts)
+    opts%general%debug = .false.
     if (active_nest() /= 2) call fail("active nest after init(2)")
     call solve_and_check(d2, alpha2, div2, "nest 2")
 

@@ -38,6 +38,10 @@ def test_fortran_domain_adapter_two_nests():
     print(out.stdout, out.stderr)
     assert out.returncode == 0
     assert "PASS" in out.stdout and "FAIL" not in out.stdout
+    # nest 2 runs verbose: the reference's per-solve timing breakdown, once
+    assert out.stdout.count("per-solve timing breakdown") == 1
+    spmv = [ln for ln in out.stdout.splitlines() if ln.strip().startswith("SpMV")]
+    assert len(spmv) == 1 and float(spmv[0].split(":")[1].split()[0]) > 0
 
 
 def _read_dump(path):
