@@ -33,7 +33,10 @@ correction (`ms_calc_iter_winds`, what calc_iter_winds does). The CPU baseline
 is the reference algorithm restated in C (the oracle's source, built here with
 -O3 -march=native -fopenmp -ffast-math) timed on this host on a bounded sample
 and scaled to a full solve; --cpu-study adds the build without -ffast-math,
-1-thread runs and one full solve.
+1-thread runs and one full solve. `faithful_distance` carries the relative L2
+distance of the FP64 path from HICAR's own FP32 rounding (lambda, u, v, w,
+divergence; written on the host by oracle/faithful.py into
+profiles/faithful_distance.json).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --sequence S   (the 1024x1024x60 solve sequence instead)
@@ -63,6 +66,8 @@ HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 WEIGHTS = os.path.join(ROOT, "hicar_amd", "data", "lnet_params.npy")
 TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+# the FP64 path against the reference's FP32 rounding (written by oracle/faithful.py on the host)
+FAITHFUL = os.path.join(ROOT, "profiles", "faithful_distance.json")
 
 
 def kernel_bytes(kind: str, nx: int, ny: int, nz: int, coef_bytes: int = 0) -> float:
@@ -470,6 +475,13 @@ def main():
                    cpu=cpu_info(), runs=cres)
         log(f"cpu baseline: {cpu['value']:.0f} ms/solve ({cpu['cores']} threads)")
 
+    faithful = None
+    if os.path.exists(FAITHFUL):
+        with open(FAITHFUL) as f:
+            fd = json.load(f)
+        faithful = {"what": fd["what"],
+                    **{k: {q: v[q] for q in ("label", "tol_rel", "lam", "u", "v", "w", "div")}
+                       for k, v in fd["cases"].items()}}
     plain_ms = res["plain"]["ms"] if "plain" in res else None
     # what the learned start costs or saves: the same multigrid solve from the
     # zero start beside it, and the network's forward pass alone
@@ -509,6 +521,7 @@ def main():
         "cpu_baseline": cpu,
         "speedup_vs_plain": round(plain_ms / ms_h, 3) if plain_ms else None,
         "learned_start_cost": learned_cost,
+        "faithful_distance": faithful,
         "cells_per_s": case.cells / (ms_h * 1e-3),
         "tight_reference": tight_info if tight is not None else None,
         "arms": res,

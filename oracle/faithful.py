@@ -7,7 +7,7 @@ coefficient assembly, the rhs product, lambda before the correction and the
 correction itself: src/physics/wind_iterative.F90:86-91, 1525, 1770-2062).
 Both start from the same FP32 inputs, as HICAR's fields are `real`.
 
-    python oracle/faithful.py --config 0 1 2 --out oracle/faithful_distance.json
+    python oracle/faithful.py --config 0 1 2 --out profiles/faithful_distance.json
 
 solves each configuration with both oracles to the stop and to 1e-12 and
 writes the relative L2 distances of lambda, the corrected u, v, w, their
