@@ -371,8 +371,9 @@ struct windhip_solver {
     int net_dims(int *Dp, int *Hp, int *Wp) const;
     int net_input(float *in, double *scale);
     int line_relax(double **y, double **scratch, int steps, hipStream_t strm);
-    int line_relax32(double *y, int steps, hipStream_t strm);
-    int learned_launch(double *res_plain, double **guess, bool f32_relax);
+    int line_relax32(double *y, int steps, hipStream_t strm, bool mg_rows = false);
+    int learned_launch(double *res_plain, double **guess, bool f32_relax, bool mg_rows = false);
+    int prepare_fine_line(bool multigrid);
     int learned_join(double *res_guess);
     bool net_pending = false;
     template <typename T>
@@ -1254,7 +1255,7 @@ int windhip_solver::line_relax(double **y, double **scratch, int steps, hipStrea
 // The same relaxation in FP32 on the FP32 operator rows, for the guess of the
 // mixed solves (whose own inner solves are FP32): y (FP64) is rounded, relaxed
 // `steps` times and widened back in place.
-int windhip_solver::line_relax32(double *y, int steps, hipStream_t strm)
+int windhip_solver::line_relax32(double *y, int steps, hipStream_t strm, bool mg_rows)
 {
     if (steps <= 0) return WINDHIP_OK;
     if (!lr_c) {
@@ -1271,12 +1272,30 @@ int windhip_solver::line_relax32(double *y, int steps, hipStream_t strm)
     k_round_f32<<<flat_grid(N), kBlock, 0, strm>>>(N, y, lr_y32);
     k_round_f32<<<flat_grid(N), kBlock, 0, strm>>>(N, b, lr_b32);
     float *cur = lr_y32, *nxt = lr_o32;
-    for (int q = 0; q < steps; ++q) {
-        CHK(with_op<float>([&](auto op) {
-            k_line_relax<decltype(op)><<<g.nt, kThreads, 0, strm>>>(op, g, cur, lr_b32, cq, dq, nxt);
+    if (mg_rows) {
+        // the multigrid fine level's bf16 rows and factors (prepare_fine_line,
+        // queued on the solver's stream before this stream was forked): one
+        // pass of k_smooth_r per step, undamped, on the guess's own live state
+        k_state_reset<<<1, 1, 0, strm>>>(st + 2, 1, 0.0, 0.0, 0.0);
+        auto body = [&](auto op) -> int {
+            for (int q = 0; q < steps; ++q) {
+                k_smooth_r<decltype(op), 44, unsigned><<<col_blocks(g), kColThreads, 0, strm>>>(
+                    op, g, st + 2, lr_b32, cur, lf_lip16, reinterpret_cast<const unsigned *>(lf_cp16), nullptr, nxt,
+                    1.0f);
+                std::swap(cur, nxt);
+            }
             return WINDHIP_OK;
-        }));
-        std::swap(cur, nxt);
+        };
+        if (use_bf16_rows()) CHK(body(opab()));
+        else CHK(with_op<float>(body));
+    } else {
+        for (int q = 0; q < steps; ++q) {
+            CHK(with_op<float>([&](auto op) {
+                k_line_relax<decltype(op)><<<g.nt, kThreads, 0, strm>>>(op, g, cur, lr_b32, cq, dq, nxt);
+                return WINDHIP_OK;
+            }));
+            std::swap(cur, nxt);
+        }
     }
     k_widen_f64<<<flat_grid(N), kBlock, 0, strm>>>(N, cur, y);
     HIPCHK(hipGetLastError());
@@ -1293,7 +1312,7 @@ int windhip_solver::line_relax32(double *y, int steps, hipStream_t strm)
 // their line factors and coarse levels meanwhile; learned_join makes the
 // solver's stream wait for it and reads the guess's residual. The guess lands in t or s (FP64 vectors
 // of the plain solve, unused by the mixed solves).
-int windhip_solver::learned_launch(double *res_plain, double **guess, bool f32_relax)
+int windhip_solver::learned_launch(double *res_plain, double **guess, bool f32_relax, bool mg_rows)
 {
     int Dp, Hp, Wp;
     net_dims(&Dp, &Hp, &Wp);
@@ -1307,6 +1326,9 @@ int windhip_solver::learned_launch(double *res_plain, double **guess, bool f32_r
         CHK(alloc(&lr_c, N));
         CHK(alloc(&lr_d, N));
     }
+    // the multigrid solve's relaxation of the guess reads the fine level's
+    // bf16 rows and factors: queued first (nothing when they are current)
+    if (mg_rows) CHK(prepare_fine_line(true));
     double scale = 0.0;
     CHK(net_input(net_in, &scale));
     *res_plain = st_host[0].outer_rnorm;  // of the current lambda, from net_input's residual
@@ -1317,7 +1339,7 @@ int windhip_solver::learned_launch(double *res_plain, double **guess, bool f32_r
     k_net_output<<<flat_grid(d.P), kBlock, 0, net_stream>>>(d, Dp, Hp, Wp, net_out, scale, t);
     HIPCHK(hipGetLastError());
     double *gy = t, *gs = s;
-    if (f32_relax) CHK(line_relax32(gy, net_smooth, net_stream));
+    if (f32_relax) CHK(line_relax32(gy, net_smooth, net_stream, mg_rows));
     else CHK(line_relax(&gy, &gs, net_smooth, net_stream));
     CHK(with_op<double>([&](auto op) {
         k_true_resid<decltype(op)><<<chunk_blocks(g), kColThreads, 0, net_stream>>>(op, g, st + 2, partials2, gy, b);
@@ -1346,6 +1368,49 @@ int windhip_solver::learned_join(double *res_guess)
     return WINDHIP_OK;
 }
 
+// The fine level's line-relaxation factors and, for the multigrid inner
+// solve, the bf16 coefficient rows and bf16 factors, rebuilt on the solver's
+// stream only when the operator generation moved (same metrics, alpha and
+// operator: kept).
+int windhip_solver::prepare_fine_line(bool multigrid)
+{
+    if (!lf_lo) {
+        CHK(alloc(&lf_lo, N));
+        CHK(alloc(&lf_cp, N));
+        CHK(alloc(&lf_ip, N));
+        CHK(alloc(&lf_dq, N));
+        CHK(alloc(&lf_r, N));
+    }
+    if (lf_gen != op_gen) {
+        CHK(with_op<double>([&](auto op) {
+            k_line_factor<decltype(op)><<<g.nt, kThreads, 0, stream>>>(op, g, lf_lo, lf_cp, lf_ip);
+            return WINDHIP_OK;
+        }));
+        HIPCHK(hipGetLastError());
+        lf_gen = op_gen;
+    }
+    if (!multigrid) return WINDHIP_OK;
+    if (use_bf16_rows()) CHK(refresh_coef_bf16());
+    // bf16 factors of the fine-level line solve: a {lo | ip} pair row and a cp
+    // row for columns of up to 44 levels, three rows above
+    if (lf16_gen != op_gen) {
+        if (!lf_cp16) CHK(alloc(&lf_cp16, N));
+        k_to_bf16<<<flat_grid(N), kBlock, 0, stream>>>(lf_cp, lf_cp16, N);
+        if (g.mz <= 44) {
+            if (!lf_lip16) CHK(alloc(&lf_lip16, N));
+            k_to_bf16_pair<<<flat_grid(N), kBlock, 0, stream>>>(lf_lo, lf_ip, lf_lip16, N);
+        } else {
+            if (!lf_lo16)
+                for (unsigned short **a : {&lf_lo16, &lf_ip16}) CHK(alloc(a, N));
+            k_to_bf16<<<flat_grid(N), kBlock, 0, stream>>>(lf_lo, lf_lo16, N);
+            k_to_bf16<<<flat_grid(N), kBlock, 0, stream>>>(lf_ip, lf_ip16, N);
+        }
+        HIPCHK(hipGetLastError());
+        lf16_gen = op_gen;
+    }
+    return WINDHIP_OK;
+}
+
 // Mixed precision: FGMRES-IR. Flexible GMRES(m) in FP64 on A lambda = b whose
 // preconditioner is an FP32 BiCGStab solve of A z = v_j to inner_tol (the
 // reference's iteration, wind_iterative.F90:555-791, on the FP32 operator and
@@ -1359,48 +1424,10 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
     CHK(alloc_f32_krylov());
     const bool multigrid = o.precision == WINDHIP_MIXED_MG;
     const bool line = o.precision == WINDHIP_MIXED_LINE || multigrid;
-    if (line) {
-        if (!lf_lo) {
-            CHK(alloc(&lf_lo, N));
-            CHK(alloc(&lf_cp, N));
-            CHK(alloc(&lf_ip, N));
-            CHK(alloc(&lf_dq, N));
-            CHK(alloc(&lf_r, N));
-        }
-        // the line factors, their bf16 copies and the coarse levels are kept
-        // while the operator generation holds (same metrics, alpha, operator)
-        if (lf_gen != op_gen) {
-            CHK(with_op<double>([&](auto op) {
-                k_line_factor<decltype(op)><<<g.nt, kThreads, 0, stream>>>(op, g, lf_lo, lf_cp, lf_ip);
-                return WINDHIP_OK;
-            }));
-            HIPCHK(hipGetLastError());
-            lf_gen = op_gen;
-        }
-        if (multigrid) {
-            if (mg_gen != op_gen) {
-                CHK(build_mg());
-                mg_gen = op_gen;
-            }
-            if (use_bf16_rows()) CHK(refresh_coef_bf16());
-            // bf16 factors of the fine-level line solve: a {lo | ip} pair row
-            // and a cp row for columns of up to 44 levels, three rows above
-            if (lf16_gen != op_gen) {
-                if (!lf_cp16) CHK(alloc(&lf_cp16, N));
-                k_to_bf16<<<flat_grid(N), kBlock, 0, stream>>>(lf_cp, lf_cp16, N);
-                if (g.mz <= 44) {
-                    if (!lf_lip16) CHK(alloc(&lf_lip16, N));
-                    k_to_bf16_pair<<<flat_grid(N), kBlock, 0, stream>>>(lf_lo, lf_ip, lf_lip16, N);
-                } else {
-                    if (!lf_lo16)
-                        for (unsigned short **a : {&lf_lo16, &lf_ip16}) CHK(alloc(a, N));
-                    k_to_bf16<<<flat_grid(N), kBlock, 0, stream>>>(lf_lo, lf_lo16, N);
-                    k_to_bf16<<<flat_grid(N), kBlock, 0, stream>>>(lf_ip, lf_ip16, N);
-                }
-                HIPCHK(hipGetLastError());
-                lf16_gen = op_gen;
-            }
-        }
+    if (line) CHK(prepare_fine_line(multigrid));
+    if (multigrid && mg_gen != op_gen) {
+        CHK(build_mg());
+        mg_gen = op_gen;
     }
     // the learned guess, computed on the second stream while the set-up above
     // ran, becomes the start when its residual is below the plain start's
@@ -1954,7 +1981,7 @@ int windhip_solve(windhip_solver *s, const void *alpha, const void *div, const w
                        o.precision == WINDHIP_MIXED_MG;
     double *guess = nullptr, *mixed_guess = nullptr, rp = 0.0;
     if (o.start == WINDHIP_START_LEARNED) {
-        CHK(s->learned_launch(&rp, &guess, mixed));
+        CHK(s->learned_launch(&rp, &guess, mixed, o.precision == WINDHIP_MIXED_MG && s->g.mz <= 44));
         inf.start_used = WINDHIP_START_WARM;
         if (mixed) {
             mixed_guess = guess;  // the start, if better, once the mixed set-up is queued

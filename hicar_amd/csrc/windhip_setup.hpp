@@ -88,17 +88,18 @@ __global__ __launch_bounds__(kBlock) void k_scan_inputs(Dims d, const double *__
     __shared__ double smin[kBlock], smax[kBlock], snan[kBlock];
     __shared__ int is_last;
     double mn = HUGE_VAL, mxv = -HUGE_VAL, nn = 0.0;
-    const long tot = (long)d.nx * d.ny * d.nz;
-    for (long e = (long)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (long)gridDim.x * kBlock) {
-        const int i = 1 + (int)(e % d.nx);
-        const long t = e / d.nx;
-        const int j = 1 + (int)(t % d.ny);
-        const int k = 1 + (int)(t / d.ny);
-        const long n = d.r(i, k, j);
-        mn = fmin(mn, alpha[n]);
-        mxv = fmax(mxv, alpha[n]);
-        const double bv = b[n];
-        if (bv != bv) nn += 1.0;
+    // a wavefront per (j, k) row of cells, its lanes along i: contiguous
+    // loads and one 32-bit index division per row
+    constexpr int kW = kBlock / 64;
+    const int lane = threadIdx.x & 63, rows = d.ny * d.nz;
+    for (int row = blockIdx.x * kW + (threadIdx.x >> 6); row < rows; row += gridDim.x * kW) {
+        const long base = d.r(0, 1 + row / d.ny, 1 + row % d.ny);
+        for (int i = 1 + lane; i <= d.nx; i += 64) {
+            const double av = alpha[base + i], bv = b[base + i];
+            mn = fmin(mn, av);
+            mxv = fmax(mxv, av);
+            if (bv != bv) nn += 1.0;
+        }
     }
     smin[threadIdx.x] = mn; smax[threadIdx.x] = mxv; snan[threadIdx.x] = nn;
     __syncthreads();
