@@ -1476,11 +1476,18 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
             // solve whose FP32 recurrence overflowed is run again from z = 0 with
             // half the iterations it had reached, keeping a finite iterate.
             const double *vsq = j == 0 ? beta2 : fg_h + (long)(j - 1) * H + j;
+            // inexact-Krylov relaxation: once the residual estimate is within
+            // a few times the target, the inner solve of this direction only
+            // needs to reduce its rhs by about target / estimate (with a
+            // factor 2 margin; never tighter than inner_tol, never above 0.5)
+            const double est = j == 0 ? st_host[0].outer_rnorm : fabs(gv[j]);
+            const double tol_j =
+                est > 0 ? std::max(inner_tol, std::min(0.5, 0.5 * st_host[0].outer_target / est)) : inner_tol;
             int cap = o.inner_max_iters, spent = 0;
             for (int attempt = 0;; ++attempt) {
                 // every inner solve starts from a fresh state with this solve's
                 // cap (o.inner_max_iters, halved on an overflow rerun)
-                k_state_reset<<<1, 1, 0, stream>>>(st + 1, cap, 0.0, 0.0, inner_tol);
+                k_state_reset<<<1, 1, 0, stream>>>(st + 1, cap, 0.0, 0.0, tol_j);
                 CHK(timed(PK_FG_SETUP, [&] {
                     k_fg_setup<<<fb, kThreads, 0, stream>>>(N, st + 1, fg_part, vsq, attempt == 0, fg_v[j], dinv32,
                                                             x32, r32, rh32, p32, ph32[0]);

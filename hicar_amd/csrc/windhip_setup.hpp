@@ -111,21 +111,40 @@ __global__ __launch_bounds__(kBlock) void k_scan_inputs(Dims d, const double *__
         }
         __syncthreads();
     }
+    // block partials through agent-scope stores (no fence: a release here
+    // would write back the whole L2), the arrival through an agent-scope
+    // counter; the last block folds the partials with all its threads
     if (threadIdx.x == 0) {
-        part[3 * blockIdx.x] = smin[0]; part[3 * blockIdx.x + 1] = smax[0]; part[3 * blockIdx.x + 2] = snan[0];
-        __threadfence();
-        is_last = atomicAdd(counter, 1u) == (unsigned)(nblocks - 1);
+        __hip_atomic_store(&part[3 * blockIdx.x], smin[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&part[3 * blockIdx.x + 1], smax[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&part[3 * blockIdx.x + 2], snan[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);  // the partials have landed before the counter moves
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        is_last = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                  (unsigned)(nblocks - 1);
     }
     __syncthreads();
-    if (is_last && threadIdx.x == 0) {
-        __threadfence();
-        double a = HUGE_VAL, bmax = -HUGE_VAL, c = 0.0;
-        for (int q = 0; q < nblocks; ++q) {
-            a = fmin(a, ld_coherent(&part[3 * q]));
-            bmax = fmax(bmax, ld_coherent(&part[3 * q + 1]));
-            c += ld_coherent(&part[3 * q + 2]);
+    if (!is_last) return;
+    double a = HUGE_VAL, bmax = -HUGE_VAL, c = 0.0;
+    for (int q = threadIdx.x; q < nblocks; q += kBlock) {
+        a = fmin(a, ld_coherent(&part[3 * q]));
+        bmax = fmax(bmax, ld_coherent(&part[3 * q + 1]));
+        c += ld_coherent(&part[3 * q + 2]);
+    }
+    __syncthreads();
+    smin[threadIdx.x] = a; smax[threadIdx.x] = bmax; snan[threadIdx.x] = c;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + s]);
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+            snan[threadIdx.x] += snan[threadIdx.x + s];
         }
-        out[0] = a; out[1] = bmax; out[2] = c;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = smin[0]; out[1] = smax[0]; out[2] = snan[0];
         __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
