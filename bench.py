@@ -107,9 +107,9 @@ def kernel_bytes(kind: str, nx: int, ny: int, nz: int, coef_bytes: int = 0) -> f
         return 3 * N * 8 + 6 * 8 * cells + 4 * 8 * P
     if kind == "fg_axpy_dot":  # w (in/out), v_i, v_{i+1}
         return 4 * N * 8
-    if kind == "smooth_f32":  # y (stencil; its centre again for the update), in, coefficients, 3 factor rows -> out
-        fb = coef_bytes or 4
-        return 4 * N * 4 + 6 * fb * cells + 4 * 4 * P + 3 * fb * N
+    if kind == "smooth_f32":  # y (stencil), in, coefficients, 3 factor rows -> out (the kernel re-reads y's
+        fb = coef_bytes or 4  # centre for the update, mostly from cache: not counted)
+        return 3 * N * 4 + 6 * fb * cells + 4 * 4 * P + 3 * fb * N
     if kind == "update_s_line_f32":  # r, v -> s
         return 3 * N * 4
     if kind == "update_xrp_line_f32":  # x, p_hat, s_hat, s, t, p, v -> x, r, p
