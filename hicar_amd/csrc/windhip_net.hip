@@ -220,20 +220,27 @@ __global__ __launch_bounds__(kT) void k_conv3_lds(Shape s, const bf16 *__restric
 // 16-voxel x-run of row 4w + r; the k order is that of k_conv3_lds (k = t C +
 // c, t = 9 dz + 3 dy + dx). With X2 set, channels below csplit are the nearest
 // 2x upsampling of X2 (as in k_conv3_lds), channel c >= csplit channel c -
-// csplit of X.
+// csplit of X. With CS = csplit > 0 the ring holds those channels as the
+// coarse planes themselves (10 x 10 coarse voxels under the 18 x 18 halo,
+// [channel group][coarse voxel][8]) and the MFMA operand reads coarse voxel
+// (hx + 1) / 2, (hy + 1) / 2 for halo voxel (hx, hy): the 48-channel decoder
+// layer's ring shrinks from 124 to 67 KB, two blocks per CU.
 constexpr int kRx = 16, kRy = 16, kRPx = kRx + 2, kRPy = kRy + 2, kRPv = kRPx * kRPy;
-template <int CC>
+constexpr int kRCx = kRx / 2 + 2, kRCv = kRCx * kRCx;  // coarse voxels under a halo plane
+template <int CC, int CS = 0>
 __global__ __launch_bounds__(kT) void k_conv3_roll(Shape s, const bf16 *__restrict__ X, int ldx,
                                                   const bf16 *__restrict__ Wt, int Kp, const float *__restrict__ bias,
                                                   bf16 *__restrict__ Y, int ldy, int relu, int zl,
                                                   const bf16 *__restrict__ X2, int ldx2, int csplit)
 {
-    constexpr int G = CC / 8;                  // channel groups of 8
-    constexpr int PE = G * kRPv;               // bf16x8 entries of one halo plane
+    constexpr int GC = CS / 8, GF = (CC - CS) / 8;   // coarse-staged and fine channel groups of 8
+    constexpr int PC = GC * kRCv;              // coarse entries of a plane (first in the slot)
+    constexpr int PE = PC + GF * kRPv;         // bf16x8 entries of one halo plane
     constexpr int NE = (PE + kT - 1) / kT;     // of them per thread
     constexpr int KS = (27 * CC + 31) / 32;    // k-steps
     // weights held in registers: up to 16 k-steps, or all of them where the
-    // halo ring already limits the CU to one block (the 48-channel layer)
+    // halo ring already limits the CU to one block (the 48-channel layer with
+    // every channel at full resolution); streamed where two blocks fit
     constexpr bool kRegW = KS <= 16 || 4 * PE * 16 > 80 * 1024;
     constexpr int PD = 3;                      // else streamed this many k-steps ahead
     __shared__ bf16x8 ring[4 * PE];
@@ -267,31 +274,51 @@ __global__ __launch_bounds__(kT) void k_conv3_roll(Shape s, const bf16 *__restri
         for (int ks = 0; ks < KS; ++ks) wf[ks] = wload(ks);
     }
     const float bv = bias ? bias[r16] : 0.f;
-    // entry q of a plane: voxel q / G (row-major 18 x 18), channel group q % G,
-    // so consecutive threads read consecutive 16-byte pieces of a voxel row
+    // entry q of a plane: coarse entries first (CS > 0: coarse voxel q / GC of
+    // the 10 x 10 block under the halo, channel group q % GC), then the fine
+    // ones (voxel q / GF, row-major 18 x 18, channel group q % GF), so
+    // consecutive threads read consecutive 16-byte pieces of a voxel row
+    auto entry = [&](int q, int &vox, int &cg, bool &coarse) {
+        coarse = q < PC;
+        const int qq = coarse ? q : q - PC, gg = coarse ? (GC > 0 ? GC : 1) : GF;
+        vox = qq / gg;
+        cg = qq - vox * gg;
+    };
     auto load_plane = [&](int p, bf16x8 (&v)[NE]) {
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
             const int q = threadIdx.x + e * kT;
-            const int vox = q / G, cg = q - vox * G;
-            const int x = x0 + vox % kRPx - 1, y = y0 + vox / kRPx - 1;
+            int vox, cg;
+            bool coarse;
+            entry(q, vox, cg, coarse);
             v[e] = zero8;
-            if (q < PE && p >= 0 && p < s.D && x >= 0 && x < s.W && y >= 0 && y < s.H) {
-                if (X2 != nullptr && cg * 8 < csplit)
+            if (q >= PE || p < 0 || p >= s.D) continue;
+            if (coarse) {
+                const int cx = x0 / 2 - 1 + vox % kRCx, cy = y0 / 2 - 1 + vox / kRCx;
+                if (cx >= 0 && cx < hs.W && cy >= 0 && cy < hs.H)
                     v[e] = *reinterpret_cast<const bf16x8 *>(
-                        X2 + ((((long)bb * hs.D + p / 2) * hs.H + y / 2) * hs.W + x / 2) * ldx2 + cg * 8);
-                else
-                    v[e] = *reinterpret_cast<const bf16x8 *>(X + (vb + p * HW + (long)y * s.W + x) * ldx +
-                                                             (X2 != nullptr ? cg * 8 - csplit : cg * 8));
+                        X2 + ((((long)bb * hs.D + p / 2) * hs.H + cy) * hs.W + cx) * ldx2 + cg * 8);
+                continue;
             }
+            const int x = x0 + vox % kRPx - 1, y = y0 + vox / kRPx - 1;
+            if (x < 0 || x >= s.W || y < 0 || y >= s.H) continue;
+            if (CS == 0 && X2 != nullptr && cg * 8 < csplit)
+                v[e] = *reinterpret_cast<const bf16x8 *>(
+                    X2 + ((((long)bb * hs.D + p / 2) * hs.H + y / 2) * hs.W + x / 2) * ldx2 + cg * 8);
+            else
+                v[e] = *reinterpret_cast<const bf16x8 *>(
+                    X + (vb + p * HW + (long)y * s.W + x) * ldx +
+                    (CS > 0 ? cg * 8 : (X2 != nullptr ? cg * 8 - csplit : cg * 8)));
         }
     };
     auto store_plane = [&](int slot, const bf16x8 (&v)[NE]) {
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
             const int q = threadIdx.x + e * kT;
-            const int vox = q / G, cg = q - vox * G;
-            if (q < PE) ring[slot * PE + cg * kRPv + vox] = v[e];
+            int vox, cg;
+            bool coarse;
+            entry(q, vox, cg, coarse);
+            if (q < PE) ring[slot * PE + (coarse ? cg * kRCv : PC + cg * kRPv) + vox] = v[e];
         }
     };
     {
@@ -328,10 +355,165 @@ __global__ __launch_bounds__(kT) void k_conv3_roll(Shape s, const bf16 *__restri
             const int t = tap_of(ks);
             const bool tv = t < 27;
             const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
-            const int base = ((z - z0 + dz) & 3) * PE + grp_of(ks) * kRPv + (4 * wave + dy) * kRPx + r16 + dx;
+            static_assert(CS == 0, "coarse-staged sources go through k_conv3_roll_up");
+            const int base = ((z - z0 + dz) & 3) * PE + PC + (grp_of(ks) - GC) * kRPv + (4 * wave + dy) * kRPx + r16 + dx;
             bf16x8 a[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) a[r] = tv ? ring[base + r * kRPx] : zero8;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], bw, acc[r], 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int y = y0 + 4 * wave + r;
+            if (y >= s.H) continue;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int x = x0 + 4 * g + e;
+                if (x < s.W) {
+                    float o = acc[r][e] + bv;
+                    if (relu) o = fmaxf(o, 0.f);
+                    Y[(vb + z * HW + (long)y * s.W + x) * ldy + r16] = (bf16)o;
+                }
+            }
+        }
+        if (more) store_plane((z - z0 + 3) & 3, nxt);
+        __syncthreads();
+    }
+}
+
+// The decoder's full-resolution layer (CS channels of the nearest 2x upsampling
+// of the coarse tensor X2, then CF skip channels of X; 16 outputs) in the
+// rolling-z form of k_conv3_roll, with the upsampled channels staged as the
+// coarse planes themselves: 10 x 10 coarse voxels under the 18 x 18 halo, in
+// front of the fine skip planes in each ring slot ([channel group][voxel][8]),
+// so the ring takes 67 instead of 124 KB and a CU holds two blocks. The k
+// dimension (the weights' k = t C + c, C = CS + CF) is walked in two phases so
+// that every address stays cheap: 27 k-steps over the coarse channels, one tap
+// per k-step (lane group g: channels 8g .. 8g + 7; the operand of halo voxel
+// (hx, hy) is coarse voxel ((hx + 1) / 2, (hy + 1) / 2)), then the skip
+// channels two taps per k-step as in k_conv3_roll. Weight fragments are
+// streamed PD k-steps ahead.
+template <int CS, int CF>
+__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(2))) void k_conv3_roll_up(Shape s, const bf16 *__restrict__ X, int ldx,
+                                                     const bf16 *__restrict__ Wt, int Kp,
+                                                     const float *__restrict__ bias, bf16 *__restrict__ Y, int ldy,
+                                                     int relu, int zl, const bf16 *__restrict__ X2, int ldx2)
+{
+    static_assert(CS == 32 && CF == 16, "one k-step covers the four coarse channel groups, two fine taps");
+    constexpr int CC = CS + CF, GC = CS / 8, GF = CF / 8;
+    constexpr int PC = GC * kRCv, PE = PC + GF * kRPv, NE = (PE + kT - 1) / kT;
+    constexpr int KA = 27, KB = (27 + 1) / 2, KS = KA + KB, PD = 3;
+    __shared__ bf16x8 ring[4 * PE];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, g = lane >> 4;
+    const int ntx = (s.W + kRx - 1) / kRx, nty = (s.H + kRy - 1) / kRy, ntz = (s.D + zl - 1) / zl;
+    int b = blockIdx.x;
+    const int tx = b % ntx;
+    b /= ntx;
+    const int ty = b % nty;
+    b /= nty;
+    const int tz = b % ntz;
+    const int bb = b / ntz;
+    const int x0 = tx * kRx, y0 = ty * kRy, z0 = tz * zl;
+    const int z1 = min(z0 + zl, s.D);
+    const long HW = (long)s.H * s.W;
+    const long vb = (long)bb * s.D * HW;
+    const Shape hs = s.half();
+    const bf16x8 zero8 = {};
+    // weight k index of k-step ks for this lane group (phase A: tap ks, coarse
+    // channels 8g..; phase B: tap 2 (ks - KA) + g / 2, skip channels 8 (g % 2)..)
+    auto kk_of = [&](int ks) {
+        if (ks < KA) return ks * CC + 8 * g;
+        const int t = 2 * (ks - KA) + (g >> 1);
+        return t < 27 ? t * CC + CS + 8 * (g & 1) : -1;
+    };
+    // the weights are re-read (from L2) for every plane: an opaque copy of the
+    // base per plane keeps the compiler from hoisting all 41 fragments out of
+    // the plane loop (which would cost the second block per CU)
+    const bf16 *wbase = Wt;
+    auto wload = [&](int ks) -> bf16x8 {
+        const int kk = kk_of(ks);
+        return kk >= 0 ? *reinterpret_cast<const bf16x8 *>(wbase + (long)r16 * Kp + kk) : zero8;
+    };
+    const float bv = bias ? bias[r16] : 0.f;
+    auto load_plane = [&](int p, bf16x8 (&v)[NE]) {
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            const int q = threadIdx.x + e * kT;
+            v[e] = zero8;
+            if (q >= PE || p < 0 || p >= s.D) continue;
+            if (q < PC) {
+                const int vox = q / GC, cg = q - vox * GC;
+                const int cx = x0 / 2 - 1 + vox % kRCx, cy = y0 / 2 - 1 + vox / kRCx;
+                if (cx >= 0 && cx < hs.W && cy >= 0 && cy < hs.H)
+                    v[e] = *reinterpret_cast<const bf16x8 *>(
+                        X2 + ((((long)bb * hs.D + p / 2) * hs.H + cy) * hs.W + cx) * ldx2 + cg * 8);
+            } else {
+                const int qq = q - PC, vox = qq / GF, cg = qq - vox * GF;
+                const int x = x0 + vox % kRPx - 1, y = y0 + vox / kRPx - 1;
+                if (x >= 0 && x < s.W && y >= 0 && y < s.H)
+                    v[e] = *reinterpret_cast<const bf16x8 *>(X + (vb + p * HW + (long)y * s.W + x) * ldx + cg * 8);
+            }
+        }
+    };
+    auto store_plane = [&](int slot, const bf16x8 (&v)[NE]) {
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            const int q = threadIdx.x + e * kT;
+            if (q >= PE) continue;
+            int idx;
+            if (q < PC) {
+                const int vox = q / GC, cg = q - vox * GC;
+                idx = cg * kRCv + vox;
+            } else {
+                const int qq = q - PC, vox = qq / GF, cg = qq - vox * GF;
+                idx = PC + cg * kRPv + vox;
+            }
+            ring[slot * PE + idx] = v[e];
+        }
+    };
+    {
+        bf16x8 v[NE];
+#pragma unroll 1
+        for (int p = 0; p < 3; ++p) {
+            load_plane(z0 - 1 + p, v);
+            store_plane(p, v);
+        }
+    }
+    __syncthreads();
+    // coarse column of halo voxel hx = r16 + dx, for dx = 0, 1, 2
+    const int cx0 = (r16 + 1) >> 1, cx1 = (r16 + 2) >> 1, cx2 = (r16 + 3) >> 1;
+#pragma unroll 1
+    for (int z = z0; z < z1; ++z) {
+        const bool more = z + 1 < z1;
+        bf16x8 nxt[NE];
+        if (more) load_plane(z + 2, nxt);
+        f32x4 acc[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
+        asm volatile("" : "+s"(wbase));
+        bf16x8 wr[PD];
+#pragma unroll
+        for (int q = 0; q < PD; ++q) wr[q] = wload(q);
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const bf16x8 bw = wr[ks % PD];
+            if (ks + PD < KS) wr[ks % PD] = wload(ks + PD);
+            bf16x8 a[4];
+            if (ks < KA) {  // one tap for all lanes: uniform dz, dy, dx
+                const int dz = ks / 9, dy = (ks / 3) % 3, dx = ks % 3;
+                const int cb = ((z - z0 + dz) & 3) * PE + g * kRCv + (dx == 0 ? cx0 : (dx == 1 ? cx1 : cx2));
+#pragma unroll
+                for (int r = 0; r < 4; ++r) a[r] = ring[cb + ((4 * wave + r + dy + 1) >> 1) * kRCx];
+            } else {
+                const int t = 2 * (ks - KA) + (g >> 1);
+                const bool tv = t < 27;
+                const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
+                const int base = ((z - z0 + dz) & 3) * PE + PC + (g & 1) * kRPv + (4 * wave + dy) * kRPx + r16 + dx;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) a[r] = tv ? ring[base + r * kRPx] : zero8;
+            }
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], bw, acc[r], 0, 0, 0);
         }
@@ -395,6 +577,8 @@ static void conv3(hipStream_t st, Shape s, const bf16 *X, int ldx, int C, const 
             k_conv3_roll<16><<<blocks, kT, 0, st>>>(s, X, ldx, Wt, Kp, bias, Y, ldy, relu, zl, X2, ldx2, csplit);
         else if (C == 8)
             k_conv3_roll<8><<<blocks, kT, 0, st>>>(s, X, ldx, Wt, Kp, bias, Y, ldy, relu, zl, X2, ldx2, csplit);
+        else if (X2 != nullptr && csplit == 32)
+            k_conv3_roll_up<32, 16><<<blocks, kT, 0, st>>>(s, X, ldx, Wt, Kp, bias, Y, ldy, relu, zl, X2, ldx2);
         else
             k_conv3_roll<48><<<blocks, kT, 0, st>>>(s, X, ldx, Wt, Kp, bias, Y, ldy, relu, zl, X2, ldx2, csplit);
         return;
