@@ -184,6 +184,12 @@ struct windhip_solver {
         // factor pairs and the cp factor row
         unsigned *Q, *lip;
         unsigned short *cp16;
+        // levels smoothed a wavefront per column (at most mg_wave_cols columns
+        // of up to 64 rows) keep their vectors b, r, x, t column-major (row
+        // c mz + k) and column-major copies of the FP32 rows and line factors
+        bool cm;
+        float *cB, *cC, *cH, *cI, *cL, *cM, *clo, *ccp, *cip;
+        long S() const { return cm ? g.mz : 0; }  // MgIdx stride of the level's vectors
     };
     std::vector<MgLevel> mg;
     int mg_pre = 2, mg_post = 2;    // fine-level line-smoothing steps before and after the coarse correction
@@ -200,6 +206,7 @@ struct windhip_solver {
     long mg_wave_cols = 4608;
     template <typename T>
     OpA<T> level_op(const MgLevel &L) const;
+    OpA<float> level_op_cm(const MgLevel &L) const;  // the column-major FP32 rows of a wavefront level
     bool metrics_set = false, structure_uploaded = false, probed = false;
     // Operator generation: moves whenever the operator or anything built from
     // it may change (first assembly after set_metrics, a bitwise change of
@@ -466,6 +473,12 @@ OpA<float> windhip_solver::level_op<float>(const MgLevel &L) const
     const double inv = 1.0 / (L.d.dx * L.d.dx);
     o.inv = (float)inv;
     o.neg4 = (float)(-4.0 * inv);
+    return o;
+}
+OpA<float> windhip_solver::level_op_cm(const MgLevel &L) const
+{
+    OpA<float> o = level_op<float>(L);
+    o.B = L.cB; o.C = L.cC; o.H = L.cH; o.I = L.cI; o.L = L.cL; o.M = L.cM;
     return o;
 }
 OpAB windhip_solver::level_opab(const MgLevel &L) const
@@ -849,11 +862,12 @@ int windhip_solver::enqueue_iterations_line(int count, bool multigrid)
             if (!mg.empty()) {
                 CHK(resid(cur, in, lf_r));
                 CHK(timed(PK_MG_TRANSFER, [&] {
-                    k_mg_restrict<<<mg_plane_grid(mg[0].g), kMgThreads, 0, stream>>>(f0, mg[0].g, sp, lf_r, mg[0].b);
+                    k_mg_restrict<<<mg_plane_grid(mg[0].g), kMgThreads, 0, stream>>>(f0, mg[0].g, sp, lf_r, mg[0].b,
+                                                                                      0, mg[0].S());
                 }));
                 float *e = mg_coarse(0);
                 CHK(timed(PK_MG_TRANSFER, [&] {
-                    k_mg_prolong<<<mg_plane_grid(f0), kMgThreads, 0, stream>>>(f0, mg[0].g, sp, e, cur);
+                    k_mg_prolong<<<mg_plane_grid(f0), kMgThreads, 0, stream>>>(f0, mg[0].g, sp, e, cur, 0, mg[0].S());
                 }));
             }
             for (int q = 0; q < mg_post; ++q) {
@@ -890,7 +904,8 @@ int windhip_solver::enqueue_iterations_line(int count, bool multigrid)
 
 // One damped line-relaxation step on coarse level l, out = y + w L^-1 (b - A y)
 // with b = mg[l].b (y null: out = w L^-1 b). form 1: one launch, a wavefront
-// per column (k_mg_smooth_w); 0 and 2: a residual into mg[l].r and a line
+// per column (k_mg_smooth_w; on a column-major level, L.cm, its vectors are
+// column-major); 0 and 2: a residual into mg[l].r and a line
 // solve on the level's FP32 (0) or bf16 (2) rows and factors; 3 and 4: one
 // pass of k_smooth_r on the bf16 (3) or FP32 (4) rows and factors (levels of
 // up to 44 rows per column, y given; otherwise as 2 and 0).
@@ -900,9 +915,14 @@ void windhip_solver::mg_smooth(size_t l, int form, const float *y, float *out)
     const KState *sp = st + 1;
     const Geo &gl = L.gg;
     if (form == 1 && gl.mz <= 64) {
+        const int nb = (int)((L.g.P + kWaveCols - 1) / kWaveCols);
         timed(PK_MG_COARSE, [&] {
-            k_mg_smooth_w<OpA<float>><<<(int)((L.g.P + kWaveCols - 1) / kWaveCols), 64 * kWaveCols, 0, stream>>>(
-                level_op<float>(L), gl, sp, L.b, y, L.lo, L.cp, L.ip, out, mg_omega);
+            if (L.cm)  // the level's vectors are column-major
+                k_mg_smooth_w<OpA<float>, true><<<nb, 64 * kWaveCols, 0, stream>>>(
+                    level_op_cm(L), gl, sp, L.b, y, L.clo, L.ccp, L.cip, out, mg_omega);
+            else
+                k_mg_smooth_w<OpA<float>><<<nb, 64 * kWaveCols, 0, stream>>>(level_op<float>(L), gl, sp, L.b, y,
+                                                                           L.lo, L.cp, L.ip, out, mg_omega);
         });
         return;
     }
@@ -973,7 +993,10 @@ float *windhip_solver::mg_coarse(size_t l, bool entered)
     const int form = wave ? 1 : (L.Q ? (mg_onepass ? 3 : 2) : 0);
     auto resid = [&](const float *xv) {
         timed(PK_MG_COARSE, [&] {
-            if (form >= 2)
+            if (L.cm)
+                k_mg_resid_w<OpA<float>, true><<<(int)((L.g.P + kWaveCols - 1) / kWaveCols), 64 * kWaveCols, 0,
+                                                  stream>>>(level_op_cm(L), gl, sp, xv, L.b, L.r);
+            else if (form >= 2)
                 k_resid_c<OpAB><<<kb, kColThreads, 0, stream>>>(level_opab(L), gl, sp, xv, L.b, L.r);
             else
                 k_resid_c<OpA<float>><<<kb, kColThreads, 0, stream>>>(op, gl, sp, xv, L.b, L.r);
@@ -991,7 +1014,16 @@ float *windhip_solver::mg_coarse(size_t l, bool entered)
         used = std::max<size_t>(q, 1);
     }
     if (l + 1 == mg.size()) {  // one interior column: its line solve is exact
-        if (!entered) line(L.b, nullptr, L.x, 1.0f);
+        if (!entered) {
+            if (L.cm)
+                timed(PK_MG_COARSE, [&] {
+                    k_mg_smooth_w<OpA<float>, true><<<(int)((L.g.P + kWaveCols - 1) / kWaveCols), 64 * kWaveCols, 0,
+                                                       stream>>>(level_op_cm(L), gl, sp, L.b, nullptr, L.clo, L.ccp,
+                                                                 L.cip, L.x, 1.0f);
+                });
+            else
+                line(L.b, nullptr, L.x, 1.0f);
+        }
         return L.x;
     }
     float *cur = L.x, *alt = L.t;
@@ -1008,39 +1040,40 @@ float *windhip_solver::mg_coarse(size_t l, bool entered)
         std::swap(cur, alt);
     }
     const MgLevel &C = mg[l + 1];
-    // levels of one wavefront per column: the descent (residual, restriction,
-    // the coarse level's first step) and the return (prolongation, the first
-    // post-smoothing step) are one launch each
-    // (measured at 512 x 512 x 40: a wavefront per column touches one cache
-    // line per row, so the descent from a two-kernel level (128 x 128 into
-    // 64 x 64 columns: 45 us against 35 us for the three kernels) and the
-    // return onto the largest wavefront level (19 against 17 us) stay
-    // separate; below them the fused forms take 5-14 against 10-19 us)
-    const bool fuse_down = mg_fused && wave && C.g.P <= mg_wave_cols && C.gg.mz <= 64;
-    const bool fuse_up = mg_fused && L.g.P <= mg_wave_cols / 3 && gl.mz <= 64 && mg_cpost > 0;
+    // between two levels of one wavefront per column (both column-major) the
+    // descent (residual, restriction, the coarse level's first step) and the
+    // return (prolongation, the first post-smoothing step) are one launch
+    // each; the descent from a two-kernel level keeps its separate kernels
+    // (measured at 512 x 512 x 40 with plane-major wavefront levels: 45 us
+    // against 35 us for the three kernels)
+    const bool fuse_down = mg_fused && L.cm && C.cm;
+    const bool fuse_up = mg_fused && L.cm && C.cm && mg_cpost > 0;
     if (fuse_down) {
         const float wc = l + 2 == mg.size() ? 1.0f : mg_omega;  // the one-column level is solved exactly
+        // both levels are column-major (L.cm, C.cm)
         timed(PK_MG_COARSE, [&] {
-            k_mg_down_w<OpA<float>><<<(int)((C.g.P + kWaveCols - 1) / kWaveCols), 64 * kWaveCols, 0, stream>>>(
-                op, gl, C.g, sp, cur, L.b, C.lo, C.cp, C.ip, C.b, C.x, wc);
+            k_mg_down_w<OpA<float>, true><<<(int)((C.g.P + kWaveCols - 1) / kWaveCols), 64 * kWaveCols, 0, stream>>>(
+                level_op_cm(L), gl, C.g, sp, cur, L.b, C.clo, C.ccp, C.cip, C.b, C.x, wc);
         });
     } else {
         resid(cur);
         timed(PK_MG_COARSE, [&] {
-            k_mg_restrict<<<mg_plane_grid(C.g), kMgThreads, 0, stream>>>(L.g, C.g, sp, L.r, C.b);
+            k_mg_restrict<<<mg_plane_grid(C.g), kMgThreads, 0, stream>>>(L.g, C.g, sp, L.r, C.b, L.S(), C.S());
         });
     }
     float *e = mg_coarse(l + 1, fuse_down);
     int post = mg_cpost;
     if (fuse_up) {
-        timed(PK_MG_COARSE, [&] {
-            k_mg_up_w<OpA<float>><<<(int)((L.g.P + kWaveCols - 1) / kWaveCols), 64 * kWaveCols, 0, stream>>>(
-                op, gl, C.g, sp, cur, e, L.b, L.lo, L.cp, L.ip, alt, mg_omega);
+        timed(PK_MG_COARSE, [&] {  // both levels are column-major
+            k_mg_up_w<OpA<float>, true><<<(int)((L.g.P + kWaveCols - 1) / kWaveCols), 64 * kWaveCols, 0, stream>>>(
+                level_op_cm(L), gl, C.g, sp, cur, e, L.b, L.clo, L.ccp, L.cip, alt, mg_omega);
         });
         std::swap(cur, alt);
         --post;
     } else {
-        timed(PK_MG_COARSE, [&] { k_mg_prolong<<<mg_plane_grid(L.g), kMgThreads, 0, stream>>>(L.g, C.g, sp, e, cur); });
+        timed(PK_MG_COARSE, [&] {
+            k_mg_prolong<<<mg_plane_grid(L.g), kMgThreads, 0, stream>>>(L.g, C.g, sp, e, cur, L.S(), C.S());
+        });
     }
     for (int q = 0; q < post; ++q) {
         mg_smooth(l, form, cur, alt);
@@ -1070,7 +1103,10 @@ int windhip_solver::build_mg()
             L.gg.nt = L.gg.nti * L.gg.ntj;
             L.d.nx = L.g.nx; L.d.ny = L.g.ny; L.d.nz = L.g.mz - 2;
             L.d.mx = L.g.mx; L.d.my = L.g.my; L.d.mz = L.g.mz; L.d.P = L.g.P; L.d.dx = dx;
-            const long n = L.g.N, pl = L.g.P, gd = L.g.mx + 1;
+            const long n = L.g.N, pl = L.g.P;
+            L.cm = L.g.P <= mg_wave_cols && L.g.mz <= 64;
+            // stencil guards of the level's iterates: mx + 1 columns
+            const long gd = (L.g.mx + 1) * (L.cm ? (long)L.g.mz : 1L);
             for (double **a : {&L.dzdx, &L.dzdy, &L.jaco, &L.jaco_w, &L.alpha, &L.adv_dz, &L.dz_if, &L.dzdx_u,
                                &L.jaco_u, &L.dzdy_v, &L.jaco_v, &L.B, &L.C, &L.H, &L.I, &L.L, &L.M})
                 CHK(alloc(a, n));
@@ -1080,6 +1116,8 @@ int windhip_solver::build_mg()
             for (float **a : {&L.bcx32, &L.bcy32, &L.bcz32, &L.dztop32}) CHK(alloc(a, pl));
             CHK(alloc(&L.x, n, true, gd));  // stencil inputs of the level's residual
             CHK(alloc(&L.t, n, true, gd));
+            if (L.cm)
+                for (float **a : {&L.cB, &L.cC, &L.cH, &L.cI, &L.cL, &L.cM, &L.clo, &L.ccp, &L.cip}) CHK(alloc(a, n));
             mg.push_back(L);
             f = L.g;
         }
@@ -1115,6 +1153,15 @@ int windhip_solver::build_mg()
             k_line_factor<OpA<double>><<<L.gg.nt, kThreads, 0, stream>>>(level_op<double>(L), L.gg, L.lo, L.cp,
                                                                          L.ip);
         }));
+        if (L.cm) {  // column-major copies for the wavefront kernels
+            const long n = L.g.N;
+            CHK(timed(PK_MG_SETUP, [&] {
+                const float *src[] = {L.B32, L.C32, L.H32, L.I32, L.L32, L.M32, L.lo, L.cp, L.ip};
+                float *dst[] = {L.cB, L.cC, L.cH, L.cI, L.cL, L.cM, L.clo, L.ccp, L.cip};
+                for (int q = 0; q < 9; ++q)
+                    k_mg_to_cm<<<flat_grid(n), kBlock, 0, stream>>>(L.g.P, L.g.mz, src[q], dst[q]);
+            }));
+        }
         if (L.g.mz <= 44) {  // bf16 copies for the two-kernel smoothing step
             const long n = L.g.N;
             if (!L.Q) {
@@ -2333,6 +2380,21 @@ int windhip_mg_level(windhip_solver *s, int level, int *dims_out, float *coef_ho
     return WINDHIP_OK;
 }
 
+// Host rows of a level between the plane-major layout of the C-ABI (rows
+// [k][j][i]) and the column-major layout of the wavefront levels (row c mz + k).
+static std::vector<float> mg_rows_to_cm(const MgGeo &g, const float *src)
+{
+    std::vector<float> d((size_t)g.N);
+    for (long c = 0; c < g.P; ++c)
+        for (int k = 0; k < g.mz; ++k) d[(size_t)(c * g.mz + k)] = src[(long)k * g.P + c];
+    return d;
+}
+static void mg_rows_from_cm(const MgGeo &g, const float *src, float *dst)
+{
+    for (long c = 0; c < g.P; ++c)
+        for (int k = 0; k < g.mz; ++k) dst[(long)k * g.P + c] = src[c * g.mz + k];
+}
+
 int windhip_mg_smooth(windhip_solver *s, int level, int fused, const float *b, const float *y, float *out)
 {
     if (!s || !b || !out) return fail(WINDHIP_ERR_ARG, "windhip_mg_smooth: null argument");
@@ -2342,15 +2404,25 @@ int windhip_mg_smooth(windhip_solver *s, int level, int fused, const float *b, c
     if (fused < 0 || fused > 4) return fail(WINDHIP_ERR_ARG, "windhip_mg_smooth: form %d outside 0..4", fused);
     auto &L = s->mg[level - 1];
     const size_t bytes = sizeof(float) * L.g.N;
+    // the one-launch form works on the level's own layout (column-major on
+    // the wavefront levels), the others on the plane-major rows
+    const bool cm = fused == 1 && L.cm;
+    std::vector<float> tb, ty, to;
+    if (cm) {
+        tb = mg_rows_to_cm(L.g, b);
+        if (y) ty = mg_rows_to_cm(L.g, y);
+        to.resize(L.g.N);
+    }
     HIPCHK(hipStreamSynchronize(s->stream));
-    HIPCHK(hipMemcpy(L.b, b, bytes, hipMemcpyHostToDevice));
-    if (y) HIPCHK(hipMemcpy(L.x, y, bytes, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(L.b, cm ? tb.data() : b, bytes, hipMemcpyHostToDevice));
+    if (y) HIPCHK(hipMemcpy(L.x, cm ? ty.data() : y, bytes, hipMemcpyHostToDevice));
     windhip::k_state_reset<<<1, 1, 0, s->stream>>>(s->st + 1, 1, 0.0, 0.0, 0.0);  // a live state: done = 0
     HIPCHK(hipGetLastError());
     s->mg_smooth(level - 1, fused, y ? L.x : nullptr, L.t);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
-    HIPCHK(hipMemcpy(out, L.t, bytes, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cm ? to.data() : out, L.t, bytes, hipMemcpyDeviceToHost));
+    if (cm) mg_rows_from_cm(L.g, to.data(), out);
     return WINDHIP_OK;
 }
 
@@ -2362,9 +2434,15 @@ int windhip_mg_cycle(windhip_solver *s, int level, int fused, const float *b, fl
                     s->mg.size());
     windhip_solver::MgLevel &L = s->mg[(size_t)level - 1];
     const long n = L.g.N;
+    // the level's vectors are column-major on the wavefront levels
+    std::vector<float> tb, to;
+    if (L.cm) {
+        tb = mg_rows_to_cm(L.g, b);
+        to.resize(n);
+    }
     // the V-cycle's kernels return at once when the inner state says done
     k_state_reset<<<1, 1, 0, s->stream>>>(s->st + 1, 1, 0.0, 0.0, 0.5);
-    HIPCHK(hipMemcpyAsync(L.b, b, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(L.b, L.cm ? tb.data() : b, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
     const bool keep = s->mg_fused, prof = s->profiling;
     s->mg_fused = fused != 0;
     s->profiling = false;
@@ -2372,8 +2450,9 @@ int windhip_mg_cycle(windhip_solver *s, int level, int fused, const float *b, fl
     s->mg_fused = keep;
     s->profiling = prof;
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(out, e, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(L.cm ? to.data() : out, e, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
+    if (L.cm) mg_rows_from_cm(L.g, to.data(), out);
     return WINDHIP_OK;
 }
 

@@ -12,8 +12,13 @@
 // horizontal resolution level by level (down to one interior column, all
 // levels kept) removes the smooth horizontal error at its own scale.
 //
-// Every level keeps the layout of the fine grid: rows [k][j][i], n = k P + c,
-// a ring of lateral identity columns around nx x ny interior columns. Coarse
+// Every level keeps the columns of the fine grid (a ring of lateral identity
+// columns around nx x ny interior columns) and all its rows per column. The
+// levels smoothed a wavefront per column (at most mg_wave_cols columns of up
+// to 64 rows) store their vectors and their copies of the operator rows and
+// line factors column-major, n = c S + k with S = mz, so the 64 lanes of a
+// wavefront read one column's rows from consecutive addresses; the other
+// levels keep the fine layout, rows [k][j][i], n = k P + c (MgIdx). Coarse
 // column (I, J) covers the fine columns 2I-1, 2I and 2J-1, 2J (one of each at
 // an odd edge). A coarse operator is the analytic operator rediscretised at
 // 2 dx from the fine level's metrics averaged over these 2 x 2 columns (cell
@@ -44,6 +49,26 @@ inline MgGeo mg_geo(int nx, int ny, int mz)
     m.P = (long)m.mx * m.my;
     m.N = m.P * mz;
     return m;
+}
+
+// Row n of (level k, column c): plane-major (S = 0, n = k P + c) or
+// column-major (S = mz, n = c S + k); col(d) is the offset of the column d
+// columns away at the same level.
+struct MgIdx {
+    long P, S;
+    __device__ __forceinline__ long n(int k, long c) const { return S ? c * S + k : (long)k * P + c; }
+    __device__ __forceinline__ long col(long d) const { return S ? d * S : d; }
+};
+
+// dst (column-major) = src (plane-major) over all N rows of a level
+__global__ void k_mg_to_cm(long P, int mz, const float *__restrict__ src, float *__restrict__ dst)
+{
+    const long N = P * mz;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x) {
+        const long c = e / mz;
+        const int k = (int)(e - c * mz);
+        dst[e] = src[(long)k * P + c];
+    }
 }
 
 constexpr int kMgThreads = 256;
@@ -205,10 +230,10 @@ __device__ __forceinline__ float wave_line_solve(int lane, bool act, float r, fl
 // the lane's own five columns (k-1 and k+1 taken from the neighbouring lanes).
 template <class Op>
 __device__ __forceinline__ float wave_resid(const Op &op, const float *__restrict__ y, float bv, long n, long c,
-                                            int k, int mz, bool lat, long mx)
+                                            int k, int mz, bool lat, long dx, long dy)
 {
     Win<float> x;
-    const float yc = y[n], yw = y[n - 1], ye = y[n + 1], ys = y[n - mx], yn = y[n + mx];
+    const float yc = y[n], yw = y[n - dx], ye = y[n + dx], ys = y[n - dy], yn = y[n + dy];
     x.c1 = yc; x.w1 = yw; x.e1 = ye; x.s1 = ys; x.n1 = yn;
     x.c0 = __shfl_up(yc, 1, 64); x.w0 = __shfl_up(yw, 1, 64); x.e0 = __shfl_up(ye, 1, 64);
     x.s0 = __shfl_up(ys, 1, 64); x.n0 = __shfl_up(yn, 1, 64);
@@ -217,7 +242,7 @@ __device__ __forceinline__ float wave_resid(const Op &op, const float *__restric
     return bv - op.row(k, mz, lat, x, op.load(n), op.bc(c));
 }
 
-template <class Op>
+template <class Op, bool CM = false>
 __global__ __launch_bounds__(64 * kWaveCols) void k_mg_smooth_w(Op op, Geo g, const KState *st,
                                                                const float *__restrict__ b,
                                                                const float *__restrict__ y,
@@ -233,7 +258,8 @@ __global__ __launch_bounds__(64 * kWaveCols) void k_mg_smooth_w(Op op, Geo g, co
     const int mz = g.mz;
     const bool act = lane < mz;
     const int k = act ? lane : mz - 1;
-    const long n = (long)k * g.P + c;
+    const MgIdx ix{g.P, CM ? mz : 0};
+    const long n = ix.n(k, c), dx = ix.col(1), dy = ix.col(g.mx);
     const int i = (int)(c % g.mx), j = (int)(c / g.mx);
     const bool lat = i == 0 || i == g.mx - 1 || j == 0 || j == g.my - 1;
     float r = b[n];
@@ -241,7 +267,7 @@ __global__ __launch_bounds__(64 * kWaveCols) void k_mg_smooth_w(Op op, Geo g, co
     if (y) {
         Win<float> x;
         yc = y[n];
-        const float yw = y[n - 1], ye = y[n + 1], ys = y[n - g.mx], yn = y[n + g.mx];
+        const float yw = y[n - dx], ye = y[n + dx], ys = y[n - dy], yn = y[n + dy];
         x.c1 = yc; x.w1 = yw; x.e1 = ye; x.s1 = ys; x.n1 = yn;
         x.c0 = __shfl_up(yc, 1, 64); x.w0 = __shfl_up(yw, 1, 64); x.e0 = __shfl_up(ye, 1, 64);
         x.s0 = __shfl_up(ys, 1, 64); x.n0 = __shfl_up(yn, 1, 64);
@@ -264,7 +290,7 @@ __global__ __launch_bounds__(64 * kWaveCols) void k_mg_smooth_w(Op op, Geo g, co
 // y_f needs mx_f + 1 guard elements on both sides (the neighbour loads of
 // the finer level's ring columns fall on them). Ring columns of the coarse
 // level get b_c = x_c = 0.
-template <class Op>
+template <class Op, bool CM = false>
 __global__ __launch_bounds__(64 * kWaveCols) void k_mg_down_w(Op opf, Geo gf, MgGeo c, const KState *st,
                                                              const float *__restrict__ yf,
                                                              const float *__restrict__ bf,
@@ -280,22 +306,23 @@ __global__ __launch_bounds__(64 * kWaveCols) void k_mg_down_w(Op opf, Geo gf, Mg
     const int mz = c.mz;
     const bool act = lane < mz;
     const int k = act ? lane : mz - 1;
-    const long n = (long)k * c.P + cc;
+    const MgIdx ic{c.P, CM ? mz : 0}, iff{gf.P, CM ? mz : 0};
+    const long n = ic.n(k, cc);
     const int I = (int)(cc % c.mx), J = (int)(cc / c.mx);
     if (I == 0 || I == c.mx - 1 || J == 0 || J == c.my - 1) {
         if (act) { bc[n] = 0.f; xc[n] = 0.f; }
         return;
     }
-    const long mxf = gf.mx, Pf = gf.P;
+    const long mxf = gf.mx;
     const int i0 = 2 * I - 1, j0 = 2 * J - 1;
     const bool hi = i0 + 1 <= (int)mxf - 2, hj = j0 + 1 <= gf.my - 2;
-    const long m = (long)k * Pf + (long)j0 * mxf + i0, cf0 = (long)j0 * mxf + i0;
+    const long cf0 = (long)j0 * mxf + i0, m = iff.n(k, cf0), dx = iff.col(1), dy = iff.col(mxf);
     // the fine columns are interior columns of the finer level
-    float s = wave_resid(opf, yf, bf[m], m, cf0, k, mz, false, mxf);
-    if (hi) s += wave_resid(opf, yf, bf[m + 1], m + 1, cf0 + 1, k, mz, false, mxf);
+    float s = wave_resid(opf, yf, bf[m], m, cf0, k, mz, false, dx, dy);
+    if (hi) s += wave_resid(opf, yf, bf[m + dx], m + dx, cf0 + 1, k, mz, false, dx, dy);
     if (hj) {
-        float t = wave_resid(opf, yf, bf[m + mxf], m + mxf, cf0 + mxf, k, mz, false, mxf);
-        if (hi) t += wave_resid(opf, yf, bf[m + mxf + 1], m + mxf + 1, cf0 + mxf + 1, k, mz, false, mxf);
+        float t = wave_resid(opf, yf, bf[m + dy], m + dy, cf0 + mxf, k, mz, false, dx, dy);
+        if (hi) t += wave_resid(opf, yf, bf[m + dy + dx], m + dy + dx, cf0 + mxf + 1, k, mz, false, dx, dy);
         s += t;
     }
     const float b = s / (float)((hi ? 2 : 1) * (hj ? 2 : 1));
@@ -312,7 +339,7 @@ __global__ __launch_bounds__(64 * kWaveCols) void k_mg_down_w(Op opf, Geo gf, Mg
 // line-smoothing step out = z + w L^-1 (b - A z). One wavefront per column of
 // the finer level, lane k holding row k. y needs mx + 1 guard elements on
 // both sides; out must alias neither y nor b.
-template <class Op>
+template <class Op, bool CM = false>
 __global__ __launch_bounds__(64 * kWaveCols) void k_mg_up_w(Op op, Geo g, MgGeo c, const KState *st,
                                                            const float *__restrict__ y,
                                                            const float *__restrict__ ec,
@@ -330,18 +357,19 @@ __global__ __launch_bounds__(64 * kWaveCols) void k_mg_up_w(Op op, Geo g, MgGeo 
     const bool act = lane < mz;
     const int k = act ? lane : mz - 1;
     const long mx = g.mx;
-    const long n = (long)k * g.P + col;
+    const MgIdx ix{g.P, CM ? mz : 0}, ic{c.P, CM ? mz : 0};
+    const long n = ix.n(k, col);
     const int i = (int)(col % mx), j = (int)(col / mx);
     const bool lat = i == 0 || i == g.mx - 1 || j == 0 || j == g.my - 1;
-    const float *e = ec + (long)k * c.P;
     // y + P e at fine column (ii, jj) of row k; ring columns are not prolonged
     auto z = [&](int ii, int jj) -> float {
-        const float yv = y[(long)k * g.P + (long)jj * mx + ii];
+        const float yv = y[ix.n(k, (long)jj * mx + ii)];
         if (ii <= 0 || ii >= g.mx - 1 || jj <= 0 || jj >= g.my - 1) return yv;
         const int I = (ii + 1) >> 1, J = (jj + 1) >> 1;
         const int In = (ii & 1) ? I - 1 : I + 1, Jn = (jj & 1) ? J - 1 : J + 1;
-        const float v = 0.5625f * e[(long)J * c.mx + I] + 0.1875f * (e[(long)J * c.mx + In] + e[(long)Jn * c.mx + I])
-                        + 0.0625f * e[(long)Jn * c.mx + In];
+        const float v = 0.5625f * ec[ic.n(k, (long)J * c.mx + I)] +
+                        0.1875f * (ec[ic.n(k, (long)J * c.mx + In)] + ec[ic.n(k, (long)Jn * c.mx + I)]) +
+                        0.0625f * ec[ic.n(k, (long)Jn * c.mx + In)];
         return yv + v;
     };
     Win<float> x;
@@ -362,8 +390,10 @@ __global__ __launch_bounds__(64 * kWaveCols) void k_mg_up_w(Op op, Geo g, MgGeo 
 // b_c = R r_f, one coarse row per thread: every interior row is the mean of
 // the rows of its aggregate; lateral rows get zero.
 __global__ __launch_bounds__(kMgThreads) void k_mg_restrict(MgGeo f, MgGeo c, const KState *st,
-                                                            const float *__restrict__ rf, float *__restrict__ bc)
+                                                            const float *__restrict__ rf, float *__restrict__ bc,
+                                                            long fS, long cS)
 {
+    const MgIdx fi{f.P, fS}, ci{c.P, cS};
     if (st->done) return;
     // a 64 x 4 block of (I, J) per grid cell (x, y), the level as grid z: no
     // index divisions
@@ -375,28 +405,29 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_restrict(MgGeo f, MgGeo c, co
     if (I == 0 || I == c.mx - 1 || J == 0 || J == c.my - 1) {
 #pragma unroll
         for (int q = 0; q < kPlaneZ; ++q)
-            if (k0 + q < c.mz) bc[(long)(k0 + q) * c.P + n0] = 0.f;
+            if (k0 + q < c.mz) bc[ci.n(k0 + q, n0)] = 0.f;
         return;
     }
     const int i0 = 2 * I - 1, j0 = 2 * J - 1;
     const bool hi = i0 + 1 <= f.nx, hj = j0 + 1 <= f.ny;
+    const long cf0 = (long)j0 * f.mx + i0, dx = fi.col(1), dy = fi.col(f.mx);
     float sum[kPlaneZ];
 #pragma unroll
     for (int q = 0; q < kPlaneZ; ++q) {
         const int k = k0 + q < c.mz ? k0 + q : c.mz - 1;  // levels past the top re-read the top one
-        const long m = (long)k * f.P + (long)j0 * f.mx + i0;
+        const long m = fi.n(k, cf0);
         float s = rf[m];
-        if (hi) s += rf[m + 1];
+        if (hi) s += rf[m + dx];
         if (hj) {
-            float t = rf[m + f.mx];
-            if (hi) t += rf[m + f.mx + 1];
+            float t = rf[m + dy];
+            if (hi) t += rf[m + dy + dx];
             s += t;
         }
         sum[q] = s;
     }
 #pragma unroll
     for (int q = 0; q < kPlaneZ; ++q)
-        if (k0 + q < c.mz) bc[(long)(k0 + q) * c.P + n0] = sum[q] / (float)((hi ? 2 : 1) * (hj ? 2 : 1));
+        if (k0 + q < c.mz) bc[ci.n(k0 + q, n0)] = sum[q] / (float)((hi ? 2 : 1) * (hj ? 2 : 1));
 }
 
 // x_f += P e_c, one fine row per thread: cell-centred bilinear interpolation
@@ -404,8 +435,10 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_restrict(MgGeo f, MgGeo c, co
 // coarse cell: weights 9/16 on its own coarse cell, 3/16 on the two nearer
 // neighbours, 1/16 on the diagonal one; ring values are zero), the level kept.
 __global__ __launch_bounds__(kMgThreads) void k_mg_prolong(MgGeo f, MgGeo c, const KState *st,
-                                                           const float *__restrict__ ec, float *__restrict__ xf)
+                                                           const float *__restrict__ ec, float *__restrict__ xf,
+                                                           long fS, long cS)
 {
+    const MgIdx fi{f.P, fS}, ci{c.P, cS};
     if (st->done) return;
     const int i = (int)(blockIdx.x * kPlaneX + (threadIdx.x & (kPlaneX - 1)));
     const int j = (int)(blockIdx.y * kPlaneY + (threadIdx.x / kPlaneX));
@@ -419,13 +452,34 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_prolong(MgGeo f, MgGeo c, con
 #pragma unroll
     for (int q = 0; q < kPlaneZ; ++q) {
         const int k = k0 + q < f.mz ? k0 + q : f.mz - 1;  // levels past the top re-read the top one
-        const float *e = ec + (long)k * c.P;
-        v[q] = 0.5625f * e[cc] + 0.1875f * (e[cx] + e[cy]) + 0.0625f * e[cd];
-        xo[q] = xf[(long)k * f.P + nf];
+        v[q] = 0.5625f * ec[ci.n(k, cc)] + 0.1875f * (ec[ci.n(k, cx)] + ec[ci.n(k, cy)]) + 0.0625f * ec[ci.n(k, cd)];
+        xo[q] = xf[fi.n(k, nf)];
     }
 #pragma unroll
     for (int q = 0; q < kPlaneZ; ++q)
-        if (k0 + q < f.mz) xf[(long)(k0 + q) * f.P + nf] = xo[q] + v[q];
+        if (k0 + q < f.mz) xf[fi.n(k0 + q, nf)] = xo[q] + v[q];
+}
+
+// The residual r = b - A y of a level of one wavefront per column (the
+// separate-kernel form of the descent on those levels), lane k holding row k.
+template <class Op, bool CM = false>
+__global__ __launch_bounds__(64 * kWaveCols) void k_mg_resid_w(Op op, Geo g, const KState *st,
+                                                              const float *__restrict__ y,
+                                                              const float *__restrict__ b, float *__restrict__ r)
+{
+    if (st->done) return;
+    const int lane = threadIdx.x & 63;
+    const long c = (long)blockIdx.x * kWaveCols + (threadIdx.x >> 6);
+    if (c >= g.P) return;  // uniform per wavefront
+    const int mz = g.mz;
+    const bool act = lane < mz;
+    const int k = act ? lane : mz - 1;
+    const MgIdx ix{g.P, CM ? mz : 0};
+    const long n = ix.n(k, c);
+    const int i = (int)(c % g.mx), j = (int)(c / g.mx);
+    const bool lat = i == 0 || i == g.mx - 1 || j == 0 || j == g.my - 1;
+    const float rv = wave_resid(op, y, b[n], n, c, k, mz, lat, ix.col(1), ix.col(g.mx));
+    if (act) r[n] = rv;
 }
 
 }  // namespace windhip
