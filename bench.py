@@ -110,6 +110,9 @@ def kernel_bytes(kind: str, nx: int, ny: int, nz: int, coef_bytes: int = 0) -> f
     if kind == "smooth_f32":  # y (stencil), in, coefficients, 3 factor rows -> out (the kernel re-reads y's
         fb = coef_bytes or 4  # centre for the update, mostly from cache: not counted)
         return 3 * N * 4 + 6 * fb * cells + 4 * 4 * P + 3 * fb * N
+    if kind == "resid_restrict_f32":  # y (stencil), in, coefficients -> the coarse rhs (a quarter of the rows)
+        fb = coef_bytes or 4
+        return 2 * N * 4 + 6 * fb * cells + 4 * 4 * P + N
     if kind == "update_s_line_f32":  # r, v -> s
         return 3 * N * 4
     if kind == "update_xrp_line_f32":  # x, p_hat, s_hat, s, t, p, v -> x, r, p
@@ -241,7 +244,8 @@ def main():
     ap.add_argument("--sequence", type=int, default=0,
                     help="run S consecutive solves of the 1024x1024x60 sequence instead of the benchmark")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="first copy a 1 GiB device buffer three times (known traffic for counter calibration)")
+                    help="no-op kept for the profiling commands: every run first copies a 1 GiB device buffer "
+                         "three times (the copy ceiling in `roofline`, and known traffic for counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -273,14 +277,23 @@ def main():
     div = torch.empty(s.mass_shape, dtype=torch.float64, device=dev)
     s.divergence(u, v, w, density=rho, adv_den=True, use_dqdt=True, out=div)
     torch.cuda.synchronize()
-    if args.copy_calibration:
-        src = torch.ones(2 ** 27, dtype=torch.float64, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
+    # the HBM rate a plain device copy reaches on this GPU (1 GiB read + 1 GiB
+    # written, best of 3, events on torch's stream): the practical ceiling of
+    # the streaming kernels, reported beside the published peak
+    src = torch.ones(2 ** 27, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    copy_ms = []
+    for _ in range(3):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
         torch.cuda.synchronize()
-        del src, dst
-        log("copy calibration: 3 x 1 GiB device copies")
+        copy_ms.append(e0.elapsed_time(e1))
+    del src, dst
+    copy_gbs = 2 * 2 ** 30 / (min(copy_ms) * 1e-3) / 1e9
+    log(f"device copy of 1 GiB: {min(copy_ms):.3f} ms, {copy_gbs:.0f} GB/s"
+        + (" (also the calibration of the PMC traffic passes)" if args.copy_calibration else ""))
 
     net = learned.Net.load(WEIGHTS) if os.path.exists(WEIGHTS) else None
     arms = {
@@ -416,7 +429,8 @@ def main():
             avg = tot / calls
             # the multigrid inner solve reads bf16 coefficient rows
             cb = 2 if arm.startswith("mg") and kind in ("resid_f32", "mv_sigma_f32", "mv_omega_f32",
-                                                         "line_solve_f32", "smooth_f32") else 0
+                                                         "line_solve_f32", "smooth_f32",
+                                                         "resid_restrict_f32") else 0
             nb = kernel_bytes(kind, case.nx, case.ny, case.nz, cb)
             gbs = nb / (avg * 1e-3) / 1e9 if nb else None
             out[kind] = dict(calls=int(calls), avg_ms=round(avg, 5), total_ms=round(tot, 2), bytes=nb,
@@ -439,13 +453,14 @@ def main():
         if tr:
             traffic = tr["bytes_per_launch"]
     roof = dict(bound="hbm", achieved=d["GBs"], peak=HBM_PEAK_GBS, unit="GB/s", frac=d["frac"], traffic=traffic,
-                kernel=dom, bytes_per_launch=d["bytes"], avg_ms=d["avg_ms"])
+                kernel=dom, bytes_per_launch=d["bytes"], avg_ms=d["avg_ms"],
+                copy_ceiling_GBs=round(copy_gbs, 1), frac_of_copy=round(d["GBs"] / copy_gbs, 3))
     # the plain FP64 solve's matvec (fused with the last Jacobi sweep: v = A p_hat and sigma)
     plain_mv = None
     if "plain" in tables and "sweep_sigma_f64" in tables["plain"]:
         pm = tables["plain"]["sweep_sigma_f64"]
         plain_mv = dict(kernel="sweep_sigma_f64", achieved=pm["GBs"], peak=HBM_PEAK_GBS, unit="GB/s",
-                        frac=pm["frac"], avg_ms=pm["avg_ms"],
+                        frac=pm["frac"], frac_of_copy=round(pm["GBs"] / copy_gbs, 3), avg_ms=pm["avg_ms"],
                         bytes_per_launch=kernel_bytes("sweep_sigma_f64", case.nx, case.ny, case.nz))
 
     netinfo = None

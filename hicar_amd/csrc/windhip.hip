@@ -400,9 +400,10 @@ struct windhip_solver {
     int build_mg();
     float *mg_coarse(size_t l, bool entered = false);
     bool mg_fused = true;  // fused descent/return kernels on the wavefront-per-column levels
-    // one-pass smoothing steps (residual and line solve in one kernel,
-    // k_smooth_r) on the fine level and the larger coarse levels of up to 44
-    // rows per column
+    // one-pass kernels: smoothing steps with the residual and the line solve in
+    // one kernel (k_smooth_r) on the fine level and the larger coarse levels
+    // of up to 44 rows per column, and the fine level's residual with the
+    // restriction (k_resid_restrict_c)
     bool mg_onepass = true;
     // form 0: residual + line solve (FP32 rows); 1: one launch, a wavefront
     // per column; 2: residual + line solve on the level's bf16 rows
@@ -705,7 +706,7 @@ int windhip_solver::read_state(int idx)
 enum { PK_SWEEP = 0, PK_SIGMA = 1, PK_UPD_S = 2, PK_OMEGA = 3, PK_UPD_XRP = 4, PK_INIT = 10,
        PK_IR_RESID = 11, PK_IR_ACC = 12, PK_TRUE_RESID = 13, PK_FG_SETUP = 14, PK_FG_MATVEC = 15,
        PK_FG_AXPY_DOT = 16, PK_LINE_SOLVE = 17, PK_RESID = 18, PK_MV_SIGMA = 19, PK_MV_OMEGA = 20,
-       PK_MG_TRANSFER = 21, PK_MG_COARSE = 22, PK_MG_SETUP = 23, PK_SMOOTH = 24, PK_COUNT = 25 };
+       PK_MG_TRANSFER = 21, PK_MG_COARSE = 22, PK_MG_SETUP = 23, PK_SMOOTH = 24, PK_RESID_RESTRICT = 25, PK_COUNT = 26 };
 
 template <class Launch>
 int windhip_solver::timed(int kind, Launch &&launch)
@@ -860,11 +861,19 @@ int windhip_solver::enqueue_iterations_line(int count, bool multigrid)
                 std::swap(cur, alt);
             }
             if (!mg.empty()) {
-                CHK(resid(cur, in, lf_r));
-                CHK(timed(PK_MG_TRANSFER, [&] {
-                    k_mg_restrict<<<mg_plane_grid(mg[0].g), kMgThreads, 0, stream>>>(f0, mg[0].g, sp, lf_r, mg[0].b,
-                                                                                      0, mg[0].S());
-                }));
+                if (mg_onepass) {  // residual and restriction in one pass
+                    CHK(timed(PK_RESID_RESTRICT, [&] {
+                        k_resid_restrict_c<Op><<<(int)((mg[0].g.P + kColThreads - 1) / kColThreads) * kRRChunks,
+                                                 kColThreads, 0, stream>>>(op, g, mg[0].g, sp, cur, in, mg[0].b,
+                                                                           mg[0].S());
+                    }));
+                } else {
+                    CHK(resid(cur, in, lf_r));
+                    CHK(timed(PK_MG_TRANSFER, [&] {
+                        k_mg_restrict<<<mg_plane_grid(mg[0].g), kMgThreads, 0, stream>>>(f0, mg[0].g, sp, lf_r,
+                                                                                          mg[0].b, 0, mg[0].S());
+                    }));
+                }
                 float *e = mg_coarse(0);
                 CHK(timed(PK_MG_TRANSFER, [&] {
                     k_mg_prolong<<<mg_plane_grid(f0), kMgThreads, 0, stream>>>(f0, mg[0].g, sp, e, cur, 0, mg[0].S());

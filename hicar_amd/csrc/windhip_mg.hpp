@@ -482,4 +482,138 @@ __global__ __launch_bounds__(64 * kWaveCols) void k_mg_resid_w(Op op, Geo g, con
     if (act) r[n] = rv;
 }
 
+
+// The fine level's descent in one pass: the residual r = b - A y of the 2 x 2
+// fine columns under each coarse column (I, J) and their mean as the coarse
+// rhs (k_resid_c followed by k_mg_restrict, the same arithmetic element by
+// element, without the residual's round trip through memory). One thread per
+// coarse column and level chunk (kRRChunks chunks per column, 256 coarse
+// columns per block), marching up with a three-level window of the twelve
+// fine columns the four rows read:
+//   row j0 - 1:        S0 (i0), S1 (i0 + 1)
+//   row j0:      W0 (i0 - 1), C00, C10, E0 (i0 + 2)
+//   row j0 + 1:  W1 (i0 - 1), C01, C11, E1 (i0 + 2)
+//   row j0 + 2:        N0 (i0), N1 (i0 + 1)
+// with i0 = 2I - 1, j0 = 2J - 1. y needs mx + 1 guard elements on both sides
+// (the neighbour loads of the ring's fine columns fall on them); the fine
+// level is plane-major, the coarse one of stride cS (MgIdx). Ring columns of
+// the coarse level get zero.
+constexpr int kRRChunks = 6;  // level chunks per coarse column of k_resid_restrict_c
+template <class Op>
+__global__ __launch_bounds__(kColThreads) void k_resid_restrict_c(Op op, Geo g, MgGeo c, const KState *st,
+                                                                  const float *__restrict__ y,
+                                                                  const float *__restrict__ in,
+                                                                  float *__restrict__ bc, long cS)
+{
+    if (st->done) return;
+    const int nb = gridDim.x, L = logical_block(nb);
+    const int cbk = L / kRRChunks, q = L - cbk * kRRChunks;
+    const long cc = (long)cbk * kColThreads + threadIdx.x;
+    if (cc >= c.P) return;
+    const int mz = __builtin_amdgcn_readfirstlane(g.mz);
+    const int k0 = (int)((long)q * mz / kRRChunks), k1 = (int)((long)(q + 1) * mz / kRRChunks);
+    const MgIdx ci{c.P, cS};
+    const int I = (int)(cc % c.mx), J = (int)(cc / c.mx);
+    if (I == 0 || I == c.mx - 1 || J == 0 || J == c.my - 1) {
+        for (int k = k0; k < k1; ++k) bc[ci.n(k, cc)] = 0.f;
+        return;
+    }
+    const int mx = __builtin_amdgcn_readfirstlane(g.mx);
+    const long P = (long)mx * __builtin_amdgcn_readfirstlane(g.my);
+    const long N = P * mz;
+    const int i0 = 2 * I - 1, j0 = 2 * J - 1;
+    const bool hi = i0 + 1 <= mx - 2, hj = j0 + 1 <= g.my - 2;
+    const long c00 = (long)j0 * mx + i0;
+    const unsigned oc = (unsigned)c00, voc = oc * 4u, pb = (unsigned)(P * 4);
+    const long gd = mx + 1;
+    // byte offsets (uniform) of the twelve columns from (y - gd) + c00, and of
+    // the four fine columns from c00
+    auto yo = [&](int di, int dj) { return (unsigned)((gd + (long)dj * mx + di) * 4); };
+    const unsigned oW0 = yo(-1, 0), oC00 = yo(0, 0), oC10 = yo(1, 0), oE0 = yo(2, 0), oS0 = yo(0, -1),
+                   oS1 = yo(1, -1), oW1 = yo(-1, 1), oC01 = yo(0, 1), oC11 = yo(1, 1), oE1 = yo(2, 1),
+                   oN0 = yo(0, 2), oN1 = yo(1, 2);
+    const long r10 = 1, r01 = mx, r11 = mx + 1;  // fine column offsets of the other three from c00
+    const rsrc_t ry = make_rsrc(y - gd, (N + 2 * gd) * 4), rb = make_rsrc(in, N * 4);
+    const typename Op::Src rcf = op.src(N);
+    using Cf = typename Op::Cf;
+    const typename Op::Bc b00 = op.bc(c00), b10 = op.bc(c00 + r10), b01 = op.bc(c00 + r01),
+                          b11 = op.bc(c00 + r11);
+    // the twelve columns at one level
+    struct Lv { float W0, C00, C10, E0, S0, S1, W1, C01, C11, E1, N0, N1; };
+    auto ld = [&](Lv &v, int k) {
+        const unsigned so = (unsigned)k * pb;
+        v.W0 = bload<float>(ry, voc, so + oW0); v.C00 = bload<float>(ry, voc, so + oC00);
+        v.C10 = bload<float>(ry, voc, so + oC10); v.E0 = bload<float>(ry, voc, so + oE0);
+        v.S0 = bload<float>(ry, voc, so + oS0); v.S1 = bload<float>(ry, voc, so + oS1);
+        v.W1 = bload<float>(ry, voc, so + oW1); v.C01 = bload<float>(ry, voc, so + oC01);
+        v.C11 = bload<float>(ry, voc, so + oC11); v.E1 = bload<float>(ry, voc, so + oE1);
+        v.N0 = bload<float>(ry, voc, so + oN0); v.N1 = bload<float>(ry, voc, so + oN1);
+    };
+    // per level: b and the coefficient row of the four fine columns
+    struct Rw { float b00, b10, b01, b11; Cf f00, f10, f01, f11; };
+    auto ldr = [&](Rw &w, int k) {
+        const unsigned so = (unsigned)k * pb;
+        const long lo = (long)k * P;
+        w.b00 = bload<float>(rb, voc, so); w.b10 = bload<float>(rb, voc, so + (unsigned)(r10 * 4));
+        w.b01 = bload<float>(rb, voc, so + (unsigned)(r01 * 4));
+        w.b11 = bload<float>(rb, voc, so + (unsigned)(r11 * 4));
+        w.f00 = op.load_src(rcf, lo, so, oc, voc);
+        w.f10 = op.load_src(rcf, lo + r10, so + (unsigned)(r10 * 4), oc, voc);
+        w.f01 = op.load_src(rcf, lo + r01, so + (unsigned)(r01 * 4), oc, voc);
+        w.f11 = op.load_src(rcf, lo + r11, so + (unsigned)(r11 * 4), oc, voc);
+    };
+    // the 5-column window of one fine column from the level windows a (k-1), b (k), c (k+1)
+    auto win = [](float c0, float c1, float c2, float w0, float w1, float w2, float e0, float e1, float e2, float s0,
+                  float s1, float s2, float n0, float n1, float n2) {
+        Win<float> x;
+        x.c0 = c0; x.c1 = c1; x.c2 = c2; x.w0 = w0; x.w1 = w1; x.w2 = w2; x.e0 = e0; x.e1 = e1; x.e2 = e2;
+        x.s0 = s0; x.s1 = s1; x.s2 = s2; x.n0 = n0; x.n1 = n1; x.n2 = n2;
+        return x;
+    };
+    Lv va, vb, vc;  // levels k-1, k, k+1
+    Rw rw, rn;
+    ld(vb, k0);
+    if (k0 > 0) ld(va, k0 - 1);
+    else va = vb;  // unused by the bottom row
+    ld(vc, k0 + 1 < mz ? k0 + 1 : mz - 1);
+    ldr(rw, k0);
+    for (int k = k0; k < k1; ++k) {
+        // operands of the next level, one step ahead
+        Lv vn;
+        const bool more = k + 1 < k1;
+        if (more) {
+            ld(vn, k + 2 < mz ? k + 2 : mz - 1);
+            ldr(rn, k + 1);
+        }
+        auto row = [&](const Win<float> &x, const Cf &f, const typename Op::Bc &bcv) {
+            if (k == 0) return op.template row_c<kRowBottom>(false, x, f, bcv);
+            if (k == mz - 1) return op.template row_c<kRowTop>(false, x, f, bcv);
+            return op.template row_c<kRowInterior>(false, x, f, bcv);
+        };
+        const float r00 = rw.b00 - row(win(va.C00, vb.C00, vc.C00, va.W0, vb.W0, vc.W0, va.C10, vb.C10, vc.C10,
+                                           va.S0, vb.S0, vc.S0, va.C01, vb.C01, vc.C01), rw.f00, b00);
+        float s = r00;
+        if (hi) {
+            const float r = rw.b10 - row(win(va.C10, vb.C10, vc.C10, va.C00, vb.C00, vc.C00, va.E0, vb.E0, vc.E0,
+                                             va.S1, vb.S1, vc.S1, va.C11, vb.C11, vc.C11), rw.f10, b10);
+            s += r;
+        }
+        if (hj) {
+            float t = rw.b01 - row(win(va.C01, vb.C01, vc.C01, va.W1, vb.W1, vc.W1, va.C11, vb.C11, vc.C11,
+                                       va.C00, vb.C00, vc.C00, va.N0, vb.N0, vc.N0), rw.f01, b01);
+            if (hi)
+                t += rw.b11 - row(win(va.C11, vb.C11, vc.C11, va.C01, vb.C01, vc.C01, va.E1, vb.E1, vc.E1,
+                                      va.C10, vb.C10, vc.C10, va.N1, vb.N1, vc.N1), rw.f11, b11);
+            s += t;
+        }
+        bc[ci.n(k, cc)] = s / (float)((hi ? 2 : 1) * (hj ? 2 : 1));
+        if (more) {
+            va = vb;
+            vb = vc;
+            vc = vn;
+            rw = rn;
+        }
+    }
+}
+
 }  // namespace windhip

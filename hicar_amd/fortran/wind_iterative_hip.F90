@@ -163,7 +163,7 @@ contains
         real(c_double) :: total, spmv, prec, vec, other
         integer :: k
         integer, parameter :: spmv_kinds(*) = [1, 3, 6, 8, 11, 13, 15, 19, 20]
-        integer, parameter :: prec_kinds(*) = [0, 5, 17, 18, 21, 22, 23, 24]
+        integer, parameter :: prec_kinds(*) = [0, 5, 17, 18, 21, 22, 23, 24, 25]
         integer, parameter :: vec_kinds(*) = [2, 4, 7, 9, 12, 14, 16]
         if (windhip_profile(h, 1_c_int, calls, ms) /= WINDHIP_OK) return
         total = info%ms * 1.0e-3_c_double

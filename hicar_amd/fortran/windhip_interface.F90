@@ -20,7 +20,7 @@ module windhip_interface
                                          WINDHIP_START_GIVEN = 2, WINDHIP_START_LEARNED = 3
     integer(c_int), parameter, public :: WINDHIP_PLAIN_F64 = 0, WINDHIP_MIXED = 1, WINDHIP_MIXED_LINE = 2, &
                                          WINDHIP_MIXED_MG = 3
-    integer(c_int), parameter, public :: WINDHIP_PROFILE_KINDS = 25
+    integer(c_int), parameter, public :: WINDHIP_PROFILE_KINDS = 26
 
     type, bind(C), public :: windhip_desc
         integer(c_int) :: nx, ny, nz

@@ -271,8 +271,9 @@ int windhip_set_multigrid(windhip_solver *s, int pre, int post, int coarse_pre, 
  * (prolongation, first post-step) between the levels of one wavefront per
  * column; onepass != 0, smoothing steps whose residual feeds the line solve in
  * registers (one kernel instead of two) on the fine level and the larger
- * coarse levels of up to 44 rows per column. Off, the separate kernels run;
- * the one-pass step computes the same values as its two kernels. No
+ * coarse levels of up to 44 rows per column, and the fine level's residual
+ * and restriction in one kernel. Off, the separate kernels run; the one-pass
+ * kernels compute the same values as their pairs. No
  * counterpart in the reference. */
 int windhip_set_mg_fusion(windhip_solver *s, int fused_transfers, int onepass);
 /* Coarse level `level` (1 = first coarsening) of the MIXED_MG hierarchy built
@@ -323,8 +324,9 @@ int windhip_learned_input(windhip_solver *s, float *in_dev, int *dims_out, doubl
  * restriction and prolongation, 22 every kernel of the coarse levels, 23 the
  * coarse levels' set-up (its fine-level line solves and residuals count as 17
  * and 18), 24 fine-level one-pass smoothing steps (residual and line solve in
- * one kernel). reset != 0 clears the counters after reading. */
-#define WINDHIP_PROFILE_KINDS 25
+ * one kernel), 25 the fine level's residual and restriction in one kernel.
+ * reset != 0 clears the counters after reading. */
+#define WINDHIP_PROFILE_KINDS 26
 int windhip_profile(windhip_solver *s, int reset, double *calls, double *ms);
 
 /* Device memory held by the handle (bytes) and the process high-water mark. */
