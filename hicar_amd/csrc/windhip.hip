@@ -137,7 +137,7 @@ struct windhip_solver {
     int fg_m = 0;
     double *fg_v[kMaxRestart + 1] = {};
     float *fg_z[kMaxRestart] = {};
-    double *fg_z64 = nullptr, *fg_part = nullptr, *fg_h = nullptr, *fg_y = nullptr;
+    double *fg_part = nullptr, *fg_h = nullptr, *fg_y = nullptr;
     float **fg_zs = nullptr;
     // reductions and state
     double *partials;
@@ -373,7 +373,7 @@ struct windhip_solver {
     int run_plain(int max_iters, double tol_rel, double tol_abs, int batch, bool use_graph, int *status,
                   int *iters, double *res0, double *resf);
     int run_mixed(const windhip_solve_opts &o, windhip_solve_info *info, double *learned_guess = nullptr,
-                  double plain_res = 0.0);
+                  double plain_res = 0.0, bool x_zero = false);
     int true_residual(double *rel, const double *xv = nullptr);
     int net_dims(int *Dp, int *Hp, int *Wp) const;
     int net_input(float *in, double *scale);
@@ -1475,7 +1475,7 @@ int windhip_solver::prepare_fine_line(bool multigrid)
 // directions of a cycle, so an inaccurate inner solve slows it down but cannot
 // make the residual grow.
 int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *info, double *learned_guess,
-                              double plain_res)
+                              double plain_res, bool x_zero)
 {
     CHK(alloc_f32_krylov());
     const bool multigrid = o.precision == WINDHIP_MIXED_MG;
@@ -1516,7 +1516,14 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
         HIPCHK(hipGetLastError());
         return read_state(0);
     };
-    CHK(resid());
+    if (x_zero && info->start_used != WINDHIP_START_LEARNED) {
+        // from a zero lambda the first residual is b itself: no FP64 matvec
+        CHK(timed(PK_IR_RESID, [&] { k_fg_resid_zero<<<fb, kThreads, 0, stream>>>(N, st, fg_part, b, fg_v[0], beta2); }));
+        HIPCHK(hipGetLastError());
+        CHK(read_state(0));
+    } else {
+        CHK(resid());
+    }
     info->res0 = st_host[0].outer_rnorm;
     info->bnorm = st_host[0].outer_bnorm;
     int total_inner = 0, steps = 0, status = 0;
@@ -1545,7 +1552,8 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
                 // cap (o.inner_max_iters, halved on an overflow rerun)
                 k_state_reset<<<1, 1, 0, stream>>>(st + 1, cap, 0.0, 0.0, tol_j);
                 CHK(timed(PK_FG_SETUP, [&] {
-                    k_fg_setup<<<fb, kThreads, 0, stream>>>(N, st + 1, fg_part, vsq, attempt == 0, fg_v[j], dinv32,
+                    k_fg_setup<<<fb, kThreads, 0, stream>>>(N, st + 1, fg_part, vsq, attempt == 0, fg_v[j],
+                                                            line ? nullptr : dinv32,
                                                             x32, r32, rh32, p32, ph32[0]);
                 }));
                 HIPCHK(hipGetLastError());
@@ -1570,14 +1578,13 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
             if (breakdown) { total_inner += spent; ++steps; break; }
             total_inner += spent;
             ++steps;
-            k_fg_take<<<flat_grid(N), kBlock, 0, stream>>>(N, x32, fg_z[j], fg_z64);
             }
             double *hc = fg_h + (long)j * H;
             // w = A z_j, orthogonalised against v_0..v_j (modified Gram-Schmidt)
             CHK(with_op<double>([&](auto op) {
                 return timed(PK_FG_MATVEC, [&] {
-                    k_fg_matvec<decltype(op)><<<chunk_blocks(g), kColThreads, 0, stream>>>(op, g, st, partials, fg_z64, fg_v[0],
-                                                                             fg_v[j + 1], hc);
+                    k_fg_matvec<decltype(op)><<<chunk_blocks(g), kColThreads, 0, stream>>>(
+                        op, g, st, partials, x32, fg_z[j], fg_v[0], fg_v[j + 1], hc);
                 });
             }));
             for (int i = 0; i <= j; ++i)
@@ -1650,7 +1657,6 @@ int windhip_solver::alloc_fgmres(int m)
     for (int q = fg_m + 1; q <= m; ++q) CHK(alloc(&fg_v[q], N));
     if (fg_m == 0) CHK(alloc(&fg_v[0], N));
     for (int q = fg_m; q < m; ++q) CHK(alloc(&fg_z[q], N));
-    if (!fg_z64) CHK(alloc(&fg_z64, N));
     if (!fg_part) {
         CHK(alloc(&fg_part, (long)kFlatBlocks * 2));
         CHK(alloc(&fg_h, (long)(kMaxRestart + 1) * (kMaxRestart + 2)));
@@ -2062,7 +2068,7 @@ int windhip_solve(windhip_solver *s, const void *alpha, const void *div, const w
         }
     }
     if (mixed) {
-        CHK(s->run_mixed(o, &inf, mixed_guess, rp));
+        CHK(s->run_mixed(o, &inf, mixed_guess, rp, o.start == WINDHIP_START_ZERO));
         inf.nsweeps = s->nsweeps;
     } else {
         int status = 1, iters = 0;

@@ -1274,9 +1274,11 @@ __device__ __forceinline__ ColPos col_pos(const Geo &g, int nb)
 // pre-loads) are loaded one level ahead into alternating register sets, all
 // through buffer resources; the ghost planes are peeled where they fall in the
 // range. epi.apply(k, so, vo, ax, xc, pre) per level.
-template <class Op, class Epi>
-__device__ __forceinline__ void march_pipe(const Op &op, const Geo &g, const ColPos &t,
-                                           const typename Op::type *__restrict__ x, int k0, int k1, Epi &epi)
+// XT: the element type of x when it differs from the operator's (an FP32
+// vector under the FP64 operator: widened on load).
+template <class Op, class Epi, typename XT = typename Op::type>
+__device__ __forceinline__ void march_pipe(const Op &op, const Geo &g, const ColPos &t, const XT *__restrict__ x,
+                                           int k0, int k1, Epi &epi)
 {
     using T = typename Op::type;
     using Cf = typename Op::Cf;
@@ -1290,20 +1292,31 @@ __device__ __forceinline__ void march_pipe(const Op &op, const Geo &g, const Col
     const int mz = __builtin_amdgcn_readfirstlane(g.mz);
     const long P = (long)mx * __builtin_amdgcn_readfirstlane(g.my);
     const long N = P * mz;
-    constexpr unsigned sz = sizeof(T);
+    constexpr unsigned sz = sizeof(T), xs = sizeof(XT);
     const unsigned pb = (unsigned)(P * (long)sz);
     const unsigned voc = t.oc * sz;
+    // offsets of x (element size xs)
+    const unsigned xpb = (unsigned)(P * (long)xs), xvoc = t.oc * xs;
     const long gd = mx + 1;
-    const unsigned oS = sz, oW = (unsigned)mx * sz, oC = (unsigned)(mx + 1) * sz, oE = (unsigned)(mx + 2) * sz,
-                   oN = (unsigned)(2 * mx + 1) * sz;
-    const rsrc_t rx = make_rsrc(x - gd, (N + 2 * gd) * (long)sz);
+    const unsigned oS = xs, oW = (unsigned)mx * xs, oC = (unsigned)(mx + 1) * xs, oE = (unsigned)(mx + 2) * xs,
+                   oN = (unsigned)(2 * mx + 1) * xs;
+    const rsrc_t rx = make_rsrc(x - gd, (N + 2 * gd) * (long)xs);
     const typename Op::Src rcf = op.src(N);
     epi.setup(N);
+    auto lx5 = [&](unsigned so, T &c, T &w, T &e, T &s, T &n) {
+        if constexpr (sizeof(XT) == sizeof(T)) {
+            load_x5(rx, xvoc, so + oS, so + oW, so + oC, so + oE, so + oN, c, w, e, s, n);
+        } else {
+            XT c_, w_, e_, s_, n_;
+            load_x5(rx, xvoc, so + oS, so + oW, so + oC, so + oE, so + oN, c_, w_, e_, s_, n_);
+            c = (T)c_; w = (T)w_; e = (T)e_; s = (T)s_; n = (T)n_;
+        }
+    };
     auto fill = [&](Slot &sl, int m, auto pos) {
         constexpr int ps = decltype(pos)::value;
         if (ps != kTop) {
-            const unsigned so = (unsigned)(m + 1 < mz - 1 ? m + 1 : mz - 1) * pb;
-            load_x5(rx, voc, so + oS, so + oW, so + oC, so + oE, so + oN, sl.c, sl.w, sl.e, sl.s, sl.n);
+            const unsigned so = (unsigned)(m + 1 < mz - 1 ? m + 1 : mz - 1) * xpb;
+            lx5(so, sl.c, sl.w, sl.e, sl.s, sl.n);
         }
         if (ps == kMid) {
             const int lv = m < mz - 2 ? m : mz - 2;
@@ -1328,16 +1341,16 @@ __device__ __forceinline__ void march_pipe(const Op &op, const Geo &g, const Col
     Slot sa, sb;
     int m = k0;
     if (k0 == 0) {
-        load_x5(rx, voc, oS, oW, oC, oE, oN, win.c1, win.w1, win.e1, win.s1, win.n1);
+        lx5(0u, win.c1, win.w1, win.e1, win.s1, win.n1);
         fill(sa, 0, PF{});
         fill(sb, 1, PM{});
         step(sa, 0, PF{});
         m = 1;
     } else {
         // window before the first step: level k0-1 as c1, level k0 as c2
-        const unsigned s1 = (unsigned)(k0 - 1) * pb, s2 = (unsigned)k0 * pb;
-        load_x5(rx, voc, s1 + oS, s1 + oW, s1 + oC, s1 + oE, s1 + oN, win.c1, win.w1, win.e1, win.s1, win.n1);
-        load_x5(rx, voc, s2 + oS, s2 + oW, s2 + oC, s2 + oE, s2 + oN, win.c2, win.w2, win.e2, win.s2, win.n2);
+        const unsigned s1 = (unsigned)(k0 - 1) * xpb, s2 = (unsigned)k0 * xpb;
+        lx5(s1, win.c1, win.w1, win.e1, win.s1, win.n1);
+        lx5(s2, win.c2, win.w2, win.e2, win.s2, win.n2);
         fill(sb, k0, PM{});
     }
     // sb holds the operands of step m; interior steps m .. mend-1
@@ -1566,10 +1579,11 @@ __global__ __launch_bounds__(kColThreads) void k_fg_resid(Op op, Geo g, KState *
     });
 }
 
-// w = A z (FP64) and h[0] = <w, v0>.
+// w = A z (FP64, z the FP32 inner solution widened on load) and h[0] =
+// <w, v0>; z is kept in zk (FP32) for the final update.
 template <class Op>
 __global__ __launch_bounds__(kColThreads) void k_fg_matvec(Op op, Geo g, KState *st, double *partials,
-                                                           const double *__restrict__ z64,
+                                                           const float *__restrict__ z, float *__restrict__ zk,
                                                            const double *__restrict__ v0, double *__restrict__ w,
                                                            double *h)
 {
@@ -1577,20 +1591,22 @@ __global__ __launch_bounds__(kColThreads) void k_fg_matvec(Op op, Geo g, KState 
     struct Epi {
         const double *v0;
         double *w;
+        float *zk;
         double acc[1];
         bool on;
-        rsrc_t rv, rw;
+        rsrc_t rv, rw, rz;
         struct Pre { double v; };
-        __device__ void setup(long N) { rv = make_rsrc(v0, N * 8); rw = make_rsrc(w, N * 8); }
+        __device__ void setup(long N) { rv = make_rsrc(v0, N * 8); rw = make_rsrc(w, N * 8); rz = make_rsrc(zk, N * 4); }
         __device__ Pre load(unsigned so, unsigned vo) const { return Pre{bload<double>(rv, vo, so)}; }
-        __device__ void apply(int, unsigned so, unsigned vo, double ax, double, const Pre &p)
+        __device__ void apply(int, unsigned so, unsigned vo, double ax, double xc, const Pre &p)
         {
             if (!on) return;
             bstore(ax, rw, vo, so);
+            bstore((float)xc, rz, vo >> 1, so >> 1);  // exact: xc is the FP32 value widened
             acc[0] = acc[0] + ax * p.v;
         }
-    } epi{v0, w, {0.0}, c.t.valid};
-    march_pipe(op, g, c.t, z64, c.k0, c.k1, epi);
+    } epi{v0, w, zk, {0.0}, c.t.valid};
+    march_pipe(op, g, c.t, z, c.k0, c.k1, epi);
     reduce_finalize<1, kColWaves>(epi.acc, partials, &st->cnt[3], c.L, c.nb, [&](const double *tot) { h[0] = tot[0]; });
 }
 
@@ -1907,6 +1923,28 @@ __global__ __launch_bounds__(NT) void k_smooth_r(Op op, Geo g, const KState *st,
 
 constexpr int kFlatBlocks = 1024;  // grid of the flat (row-parallel) reductions
 
+// The first FGMRES residual from a zero start: v0 = b (exactly b - A 0), its
+// norm and ||b|| as k_fg_resid states them.
+__global__ __launch_bounds__(kThreads) void k_fg_resid_zero(long N, KState *st, double *partials,
+                                                            const double *__restrict__ b, double *__restrict__ v0,
+                                                            double *h)
+{
+    double acc[2] = {0.0, 0.0};
+    for (long n = (long)blockIdx.x * kThreads + threadIdx.x; n < N; n += (long)gridDim.x * kThreads) {
+        const double bv = b[n];
+        v0[n] = bv;
+        acc[0] = acc[0] + bv * bv;
+    }
+    acc[1] = acc[0];
+    reduce_finalize<2>(acc, partials, &st->cnt[3], blockIdx.x, gridDim.x, [&](const double *tot) {
+        st->outer_rnorm = sqrt(tot[0]);
+        st->outer_bnorm = sqrt(tot[1]);
+        st->outer_target = fmax(st->tol_abs, st->tol_rel * st->outer_bnorm);
+        st->outer_done = st->outer_rnorm <= st->outer_target;
+        h[0] = tot[0];
+    });
+}
+
 // Start of one preconditioner application: scale the basis vector v by
 // 1/sqrt(*vsq) in place (FP64) when normalise is set, and set up the FP32
 // BiCGStab for A z = v from z = 0 (bicgstab_solve set-up,
@@ -1925,7 +1963,7 @@ __global__ __launch_bounds__(kThreads) void k_fg_setup(long N, KState *st_in, do
         const float rf = (float)vv;
         x32[n] = 0.0f;
         r32[n] = rf; rh32[n] = rf; p32[n] = rf;
-        ph32[n] = dinv32[n] * rf;
+        if (dinv32) ph32[n] = dinv32[n] * rf;  // the first Jacobi sweep of p (no dinv32: unused)
         acc[0] = acc[0] + (double)rf * (double)rf;
     }
     reduce_finalize<1>(acc, partials, &st_in->cnt[3], blockIdx.x, gridDim.x, [&](const double *tot) {
@@ -1941,18 +1979,6 @@ __global__ __launch_bounds__(kThreads) void k_fg_setup(long N, KState *st_in, do
         if (st_in->rnorm <= st_in->target) { st_in->status = 0; st_in->done = 1; }
         else { st_in->status = 1; st_in->done = 0; }
     });
-}
-
-// Keep the inner solution: z (FP32 copy for the final update) and its FP64
-// widening for the FP64 matvec.
-__global__ __launch_bounds__(kBlock) void k_fg_take(long N, const float *__restrict__ x32, float *__restrict__ z,
-                                                    double *__restrict__ z64)
-{
-    for (long n = (long)blockIdx.x * kBlock + threadIdx.x; n < N; n += (long)gridDim.x * kBlock) {
-        const float v = x32[n];
-        z[n] = v;
-        z64[n] = (double)v;
-    }
 }
 
 // out = a x + b y (out may alias x or y)
