@@ -155,7 +155,8 @@ struct windhip_solver {
 
     // learned initial guess: attached network, its padded input/output volumes
     windhip_net *net = nullptr;
-    float *net_in = nullptr, *net_out = nullptr;
+    float *net_out = nullptr;
+    __bf16 *net_in = nullptr;  // the learned start's network input, bf16
     long net_vox = 0;
     int net_smooth = 4;                        // line-relaxation steps applied to the network's guess
     double *lr_c = nullptr, *lr_d = nullptr;   // line relaxation: Thomas coefficients
@@ -376,7 +377,8 @@ struct windhip_solver {
                   double plain_res = 0.0, bool x_zero = false);
     int true_residual(double *rel, const double *xv = nullptr);
     int net_dims(int *Dp, int *Hp, int *Wp) const;
-    int net_input(float *in, double *scale);
+    template <typename OT>
+    int net_input(OT *in, double *scale);
     int line_relax(double **y, double **scratch, int steps, hipStream_t strm);
     int line_relax32(double *y, int steps, hipStream_t strm, bool mg_rows = false);
     int learned_launch(double *res_plain, double **guess, bool f32_relax, bool mg_rows = false);
@@ -1272,7 +1274,8 @@ int windhip_solver::net_dims(int *Dp, int *Hp, int *Wp) const
 // Network input from the current rhs b, metrics and alpha. The rhs is scaled
 // by 1 / rms(b) over the cells (the solution is linear in b); *scale returns
 // the factor that turns the network output back into lambda, rms(b) dx^2.
-int windhip_solver::net_input(float *in, double *scale)
+template <typename OT>
+int windhip_solver::net_input(OT *in, double *scale)
 {
     int Dp, Hp, Wp;
     net_dims(&Dp, &Hp, &Wp);
@@ -1281,7 +1284,7 @@ int windhip_solver::net_input(float *in, double *scale)
     const double bn = st_host[0].outer_bnorm;
     const double rms = bn / sqrt((double)d.nx * d.ny * d.nz);
     const double bs = rms > 0 ? 1.0 / rms : 0.0;
-    k_net_input<<<flat_grid((long)Hp * Wp), kBlock, 0, stream>>>(d, Dp, Hp, Wp, b, bs, m_dzdx, m_dzdy, m_jaco,
+    k_net_input<OT><<<flat_grid((long)Hp * Wp), kBlock, 0, stream>>>(d, Dp, Hp, Wp, b, bs, m_dzdx, m_dzdy, m_jaco,
                                                                  m_alpha, m_adv_dz, in);
     HIPCHK(hipGetLastError());
     *scale = rms * d.dx * d.dx;
@@ -1390,7 +1393,7 @@ int windhip_solver::learned_launch(double *res_plain, double **guess, bool f32_r
     *res_plain = st_host[0].outer_rnorm;  // of the current lambda, from net_input's residual
     HIPCHK(hipEventRecord(ev_net0, stream));
     HIPCHK(hipStreamWaitEvent(net_stream, ev_net0, 0));
-    const int rc = wnet_forward_dev(net, net_stream, net_in, wnet::Shape{1, Dp, Hp, Wp}, net_out);
+    const int rc = wnet_forward_dev_bf16(net, net_stream, net_in, wnet::Shape{1, Dp, Hp, Wp}, net_out);
     if (rc) return fail(rc, "learned start: network forward failed");
     k_net_output<<<flat_grid(d.P), kBlock, 0, net_stream>>>(d, Dp, Hp, Wp, net_out, scale, t);
     HIPCHK(hipGetLastError());

@@ -989,7 +989,7 @@ struct windhip_net {
     // infer: the decoder convolutions read the upsampled channels from the
     // coarse tensors directly (no backward pass follows, so the concatenated
     // inputs need not exist)
-    int forward(hipStream_t st, const float *in, Shape s, float *o, bool infer = false);
+    int forward(hipStream_t st, const float *in, Shape s, float *o, bool infer = false, const bf16 *in16 = nullptr);
     int backward(hipStream_t st, Shape s);
 };
 
@@ -1066,7 +1066,7 @@ void windhip_net::conv_back(hipStream_t st, int l, Shape s, const bf16 *G, int l
     if (dX) conv3(st, s, gtmp, c.co, c.co, wb[l], kpb[l], nullptr, c.ci, dX, lddx, 0);
 }
 
-int windhip_net::forward(hipStream_t st, const float *in, Shape s, float *o, bool infer)
+int windhip_net::forward(hipStream_t st, const float *in, Shape s, float *o, bool infer, const bf16 *in16)
 {
     NCHK(ensure(s));
     NCHK(prepare_weights(st));
@@ -1082,8 +1082,8 @@ int windhip_net::forward(hipStream_t st, const float *in, Shape s, float *o, boo
     // stores, and the pooling and the decoder read no unused bytes.
     bf16 *sk0 = infer ? cat0 : cat0 + 32, *sk1 = infer ? cat1 : cat1 + 64, *sk2 = infer ? cat2 : cat2 + 64;
     const int ls0 = infer ? 16 : 48, ls1 = infer ? 32 : 96, ls2 = infer ? 64 : 128;
-    k_cast_in8<<<grid1(m0), kT, 0, st>>>(m0, in, x0);
-    conv(st, C0A, s, x0, 8, a0, 16);
+    if (!in16) k_cast_in8<<<grid1(m0), kT, 0, st>>>(m0, in, x0);
+    conv(st, C0A, s, in16 ? in16 : x0, 8, a0, 16);
     conv(st, C0B, s, a0, 16, sk0, ls0);
     k_pool8<<<grid1(s1.voxels() * 2), kT, 0, st>>>(s, 16, sk0, ls0, p1, 16);
     conv(st, C1A, s1, p1, 16, a1, 32);
@@ -1163,6 +1163,11 @@ int windhip_net::backward(hipStream_t st, Shape s)
 int wnet_forward_dev(windhip_net *net, hipStream_t stream, const float *in, wnet::Shape s, float *out)
 {
     return net->forward(stream, in, s, out, true);
+}
+
+int wnet_forward_dev_bf16(windhip_net *net, hipStream_t stream, const bf16 *in, wnet::Shape s, float *out)
+{
+    return net->forward(stream, nullptr, s, out, true, in);
 }
 
 // ------------------------------------------------------------------ C-ABI

@@ -40,3 +40,6 @@ struct windhip_net;
 // Forward pass of a batch already assembled on the device: in [B][D][H][W][8]
 // FP32 (channels-last, padded dims multiples of 8), out [B][D][H][W] FP32.
 int wnet_forward_dev(windhip_net *net, hipStream_t stream, const float *in, wnet::Shape s, float *out);
+// The same with the input already in bf16 ([B][D][H][W][8], the rounding the
+// forward pass applies to an FP32 input): no conversion pass.
+int wnet_forward_dev_bf16(windhip_net *net, hipStream_t stream, const __bf16 *in, wnet::Shape s, float *out);

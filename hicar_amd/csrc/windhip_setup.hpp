@@ -654,10 +654,13 @@ __global__ void k_widen_f64(long n, const float *__restrict__ a, double *__restr
 // 0 rhs b * bscale, 1 dz/dx, 2 dz/dy, 3 Jacobian, 4 alpha, 5 height of the
 // cell centre above the surface / column height, 6..7 zero; zero outside the
 // grid.
+// OT: float, or __bf16 (the network's own input type: the rounding of its
+// FP32 input conversion, one pass less)
+template <typename OT>
 __global__ void k_net_input(Dims d, int Dp, int Hp, int Wp, const double *__restrict__ b, double bscale,
                             const double *__restrict__ dzdx, const double *__restrict__ dzdy,
                             const double *__restrict__ jaco, const double *__restrict__ alpha,
-                            const double *__restrict__ adv_dz, float *__restrict__ in)
+                            const double *__restrict__ adv_dz, OT *__restrict__ in)
 {
     const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (long)Hp * Wp) return;
@@ -668,7 +671,7 @@ __global__ void k_net_input(Dims d, int Dp, int Hp, int Wp, const double *__rest
         for (int k = 1; k <= d.nz; ++k) top += adv_dz[d.r(x + 1, k, y + 1)];
     double h = 0.0;
     for (int z = 0; z < Dp; ++z) {
-        float *o = in + (((long)z * Hp + y) * Wp + x) * 8;
+        OT *o = in + (((long)z * Hp + y) * Wp + x) * 8;
         float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         if (col && z < d.nz) {
             const long n = d.r(x + 1, z + 1, y + 1);
@@ -681,7 +684,7 @@ __global__ void k_net_input(Dims d, int Dp, int Hp, int Wp, const double *__rest
             v[5] = (float)((h + 0.5 * dz) / top);
             h += dz;
         }
-        for (int c = 0; c < 8; ++c) o[c] = v[c];
+        for (int c = 0; c < 8; ++c) o[c] = (OT)v[c];
     }
 }
 
