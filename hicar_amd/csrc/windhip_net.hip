@@ -207,46 +207,43 @@ __global__ __launch_bounds__(kT) void k_conv3_lds(Shape s, const bf16 *__restric
 }
 
 // Rolling-z form of the convolution for the full-resolution layers (C = CC
-// input channels, a multiple of 8, and 16 output channels): a block owns a
-// 16 x 16 (x, y) patch of output voxels over zl consecutive z levels and
-// marches up it. A ring of four zero-padded 18 x 18 halo planes of all CC
-// channels lives in LDS ([plane][channel group][voxel][8], 16 lanes of a row
-// group read 256 contiguous bytes): while the MFMAs of output plane z read
-// planes z-1, z, z+1, the global loads of plane z+2 are in flight; their LDS
-// stores follow the MFMAs, one barrier per plane. The weight fragments (one B
-// fragment per k-step) are loaded once and stay in registers for the whole
-// march where that leaves the occupancy alone (see kRegW), else they are
-// streamed through a ring PD k-steps ahead. Wave w computes rows y0 + 4w .. y0 + 4w + 3, MFMA row group r the
-// 16-voxel x-run of row 4w + r; the k order is that of k_conv3_lds (k = t C +
-// c, t = 9 dz + 3 dy + dx). With X2 set, channels below csplit are the nearest
-// 2x upsampling of X2 (as in k_conv3_lds), channel c >= csplit channel c -
-// csplit of X. With CS = csplit > 0 the ring holds those channels as the
-// coarse planes themselves (10 x 10 coarse voxels under the 18 x 18 halo,
-// [channel group][coarse voxel][8]) and the MFMA operand reads coarse voxel
-// (hx + 1) / 2, (hy + 1) / 2 for halo voxel (hx, hy): the 48-channel decoder
-// layer's ring shrinks from 124 to 67 KB, two blocks per CU.
-constexpr int kRx = 16, kRy = 16, kRPx = kRx + 2, kRPy = kRy + 2, kRPv = kRPx * kRPy;
-constexpr int kRCx = kRx / 2 + 2, kRCv = kRCx * kRCx;  // coarse voxels under a halo plane
-template <int CC, int CS = 0>
+// input channels, a multiple of 8, and N <= 16 NB output channels): a block
+// owns a 16 x 4 RW (x, y) patch of output voxels over zl consecutive z levels
+// and marches up it. A ring of four zero-padded halo planes (18 x (4 RW + 2)
+// voxels) of all CC channels lives in LDS ([plane][channel group][voxel][8],
+// 16 lanes of a row group read 256 contiguous bytes): while the MFMAs of
+// output plane z read planes z-1, z, z+1, the global loads of plane z+2 are in
+// flight; their LDS stores follow the MFMAs, one barrier per plane. The
+// weight fragments (NB B fragments per k-step) stay in registers for the
+// whole march where that leaves the occupancy alone (see kRegW), else they
+// are streamed through a ring PD k-steps ahead from a base the compiler may
+// not hoist. Wave w computes rows y0 + RW w .. y0 + RW w + RW - 1, MFMA row
+// group r the 16-voxel x-run of row RW w + r, and each A fragment feeds NB
+// MFMAs; the k order is that of k_conv3_lds (k = t C + c, t = 9 dz + 3 dy +
+// dx). With X2 set, channels below csplit are the nearest 2x upsampling of X2
+// (as in k_conv3_lds), channel c >= csplit channel c - csplit of X.
+constexpr int kRx = 16, kRPx = kRx + 2;
+constexpr int kRCx = kRx / 2 + 2, kRCv = kRCx * kRCx;  // coarse voxels under a 16 x 16 patch's halo
+template <int CC, int NB = 1, int RW = 4>
 __global__ __launch_bounds__(kT) void k_conv3_roll(Shape s, const bf16 *__restrict__ X, int ldx,
                                                   const bf16 *__restrict__ Wt, int Kp, const float *__restrict__ bias,
-                                                  bf16 *__restrict__ Y, int ldy, int relu, int zl,
+                                                  int N, bf16 *__restrict__ Y, int ldy, int relu, int zl,
                                                   const bf16 *__restrict__ X2, int ldx2, int csplit)
 {
-    constexpr int GC = CS / 8, GF = (CC - CS) / 8;   // coarse-staged and fine channel groups of 8
-    constexpr int PC = GC * kRCv;              // coarse entries of a plane (first in the slot)
-    constexpr int PE = PC + GF * kRPv;         // bf16x8 entries of one halo plane
+    constexpr int G = CC / 8;                  // channel groups of 8
+    constexpr int TY = 4 * RW, PY = TY + 2, PV = kRPx * PY;
+    constexpr int PE = G * PV;                 // bf16x8 entries of one halo plane
     constexpr int NE = (PE + kT - 1) / kT;     // of them per thread
     constexpr int KS = (27 * CC + 31) / 32;    // k-steps
-    // weights held in registers: up to 16 k-steps, or all of them where the
+    // weights held in registers: up to 16 fragments, or all of them where the
     // halo ring already limits the CU to one block (the 48-channel layer with
-    // every channel at full resolution); streamed where two blocks fit
-    constexpr bool kRegW = KS <= 16 || 4 * PE * 16 > 80 * 1024;
+    // every channel at full resolution)
+    constexpr bool kRegW = KS * NB <= 16 || 4 * PE * 16 > 80 * 1024;
     constexpr int PD = 3;                      // else streamed this many k-steps ahead
     __shared__ bf16x8 ring[4 * PE];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r16 = lane & 15, g = lane >> 4;
-    const int ntx = (s.W + kRx - 1) / kRx, nty = (s.H + kRy - 1) / kRy, ntz = (s.D + zl - 1) / zl;
+    const int ntx = (s.W + kRx - 1) / kRx, nty = (s.H + TY - 1) / TY, ntz = (s.D + zl - 1) / zl;
     int b = blockIdx.x;
     const int tx = b % ntx;
     b /= ntx;
@@ -254,7 +251,7 @@ __global__ __launch_bounds__(kT) void k_conv3_roll(Shape s, const bf16 *__restri
     b /= nty;
     const int tz = b % ntz;
     const int bb = b / ntz;
-    const int x0 = tx * kRx, y0 = ty * kRy, z0 = tz * zl;
+    const int x0 = tx * kRx, y0 = ty * TY, z0 = tz * zl;
     const int z1 = min(z0 + zl, s.D);  // output planes [z0, z1)
     const long HW = (long)s.H * s.W;
     const long vb = (long)bb * s.D * HW;
@@ -264,61 +261,46 @@ __global__ __launch_bounds__(kT) void k_conv3_roll(Shape s, const bf16 *__restri
     // (k % CC) .. + 7
     auto tap_of = [&](int ks) { return (32 * ks + 8 * g) / CC; };
     auto grp_of = [&](int ks) { return ((32 * ks + 8 * g) % CC) / 8; };
-    auto wload = [&](int ks) -> bf16x8 {
-        const int kk = 32 * ks + 8 * g;
-        return kk < 27 * CC ? *reinterpret_cast<const bf16x8 *>(Wt + (long)r16 * Kp + kk) : zero8;
+    const bf16 *wbase = Wt;
+    auto wload = [&](int ks, int nb) -> bf16x8 {
+        const int kk = 32 * ks + 8 * g, row = 16 * nb + r16;  // weight rows beyond N read as zero
+        return kk < 27 * CC && row < N ? *reinterpret_cast<const bf16x8 *>(wbase + (long)row * Kp + kk) : zero8;
     };
-    bf16x8 wf[kRegW ? KS : 1];
+    bf16x8 wf[kRegW ? KS : 1][NB];
     if constexpr (kRegW) {
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) wf[ks] = wload(ks);
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) wf[ks][nb] = wload(ks, nb);
     }
-    const float bv = bias ? bias[r16] : 0.f;
-    // entry q of a plane: coarse entries first (CS > 0: coarse voxel q / GC of
-    // the 10 x 10 block under the halo, channel group q % GC), then the fine
-    // ones (voxel q / GF, row-major 18 x 18, channel group q % GF), so
-    // consecutive threads read consecutive 16-byte pieces of a voxel row
-    auto entry = [&](int q, int &vox, int &cg, bool &coarse) {
-        coarse = q < PC;
-        const int qq = coarse ? q : q - PC, gg = coarse ? (GC > 0 ? GC : 1) : GF;
-        vox = qq / gg;
-        cg = qq - vox * gg;
-    };
+    float bv[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) bv[nb] = bias && 16 * nb + r16 < N ? bias[16 * nb + r16] : 0.f;
+    // entry q of a plane: voxel q / G (row-major 18 x PY), channel group q % G,
+    // so consecutive threads read consecutive 16-byte pieces of a voxel row
     auto load_plane = [&](int p, bf16x8 (&v)[NE]) {
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
             const int q = threadIdx.x + e * kT;
-            int vox, cg;
-            bool coarse;
-            entry(q, vox, cg, coarse);
-            v[e] = zero8;
-            if (q >= PE || p < 0 || p >= s.D) continue;
-            if (coarse) {
-                const int cx = x0 / 2 - 1 + vox % kRCx, cy = y0 / 2 - 1 + vox / kRCx;
-                if (cx >= 0 && cx < hs.W && cy >= 0 && cy < hs.H)
-                    v[e] = *reinterpret_cast<const bf16x8 *>(
-                        X2 + ((((long)bb * hs.D + p / 2) * hs.H + cy) * hs.W + cx) * ldx2 + cg * 8);
-                continue;
-            }
+            const int vox = q / G, cg = q - vox * G;
             const int x = x0 + vox % kRPx - 1, y = y0 + vox / kRPx - 1;
-            if (x < 0 || x >= s.W || y < 0 || y >= s.H) continue;
-            if (CS == 0 && X2 != nullptr && cg * 8 < csplit)
-                v[e] = *reinterpret_cast<const bf16x8 *>(
-                    X2 + ((((long)bb * hs.D + p / 2) * hs.H + y / 2) * hs.W + x / 2) * ldx2 + cg * 8);
-            else
-                v[e] = *reinterpret_cast<const bf16x8 *>(
-                    X + (vb + p * HW + (long)y * s.W + x) * ldx +
-                    (CS > 0 ? cg * 8 : (X2 != nullptr ? cg * 8 - csplit : cg * 8)));
+            v[e] = zero8;
+            if (q < PE && p >= 0 && p < s.D && x >= 0 && x < s.W && y >= 0 && y < s.H) {
+                if (X2 != nullptr && cg * 8 < csplit)
+                    v[e] = *reinterpret_cast<const bf16x8 *>(
+                        X2 + ((((long)bb * hs.D + p / 2) * hs.H + y / 2) * hs.W + x / 2) * ldx2 + cg * 8);
+                else
+                    v[e] = *reinterpret_cast<const bf16x8 *>(X + (vb + p * HW + (long)y * s.W + x) * ldx +
+                                                             (X2 != nullptr ? cg * 8 - csplit : cg * 8));
+            }
         }
     };
     auto store_plane = [&](int slot, const bf16x8 (&v)[NE]) {
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
             const int q = threadIdx.x + e * kT;
-            int vox, cg;
-            bool coarse;
-            entry(q, vox, cg, coarse);
-            if (q < PE) ring[slot * PE + (coarse ? cg * kRCv : PC + cg * kRPv) + vox] = v[e];
+            const int vox = q / G, cg = q - vox * G;
+            if (q < PE) ring[slot * PE + cg * PV + vox] = v[e];
         }
     };
     {
@@ -335,45 +317,60 @@ __global__ __launch_bounds__(kT) void k_conv3_roll(Shape s, const bf16 *__restri
         const bool more = z + 1 < z1;  // output plane z + 1 needs plane z + 2
         bf16x8 nxt[NE];
         if (more) load_plane(z + 2, nxt);
-        f32x4 acc[4];
+        f32x4 acc[RW][NB];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
-        bf16x8 wr[PD];
+        for (int r = 0; r < RW; ++r)
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) acc[r][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        bf16x8 wr[PD][NB];
         if constexpr (!kRegW) {
+            asm volatile("" : "+s"(wbase));
 #pragma unroll
-            for (int q = 0; q < PD; ++q) wr[q] = wload(q);
+            for (int q = 0; q < PD; ++q)
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) wr[q][nb] = wload(q, nb);
         }
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
-            bf16x8 bw;
-            if constexpr (kRegW) {
-                bw = wf[ks];
-            } else {
-                bw = wr[ks % PD];
-                if (ks + PD < KS) wr[ks % PD] = wload(ks + PD);
+            bf16x8 bw[NB];
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                if constexpr (kRegW) {
+                    bw[nb] = wf[ks][nb];
+                } else {
+                    bw[nb] = wr[ks % PD][nb];
+                    if (ks + PD < KS) wr[ks % PD][nb] = wload(ks + PD, nb);
+                }
             }
             const int t = tap_of(ks);
             const bool tv = t < 27;
             const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
-            static_assert(CS == 0, "coarse-staged sources go through k_conv3_roll_up");
-            const int base = ((z - z0 + dz) & 3) * PE + PC + (grp_of(ks) - GC) * kRPv + (4 * wave + dy) * kRPx + r16 + dx;
-            bf16x8 a[4];
+            const int base = ((z - z0 + dz) & 3) * PE + grp_of(ks) * PV + (RW * wave + dy) * kRPx + r16 + dx;
+            bf16x8 a[RW];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) a[r] = tv ? ring[base + r * kRPx] : zero8;
+            for (int r = 0; r < RW; ++r) a[r] = tv ? ring[base + r * kRPx] : zero8;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], bw, acc[r], 0, 0, 0);
+            for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+                for (int r = 0; r < RW; ++r)
+                    acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], bw[nb], acc[r][nb], 0, 0, 0);
         }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int y = y0 + 4 * wave + r;
+        for (int r = 0; r < RW; ++r) {
+            const int y = y0 + RW * wave + r;
             if (y >= s.H) continue;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int x = x0 + 4 * g + e;
-                if (x < s.W) {
-                    float o = acc[r][e] + bv;
-                    if (relu) o = fmaxf(o, 0.f);
-                    Y[(vb + z * HW + (long)y * s.W + x) * ldy + r16] = (bf16)o;
+            for (int nb = 0; nb < NB; ++nb) {
+                const int n = 16 * nb + r16;
+                if (n >= N) continue;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int x = x0 + 4 * g + e;
+                    if (x < s.W) {
+                        float o = acc[r][nb][e] + bv[nb];
+                        if (relu) o = fmaxf(o, 0.f);
+                        Y[(vb + z * HW + (long)y * s.W + x) * ldy + n] = (bf16)o;
+                    }
                 }
             }
         }
@@ -394,6 +391,7 @@ __global__ __launch_bounds__(kT) void k_conv3_roll(Shape s, const bf16 *__restri
 // (hx, hy) is coarse voxel ((hx + 1) / 2, (hy + 1) / 2)), then the skip
 // channels two taps per k-step as in k_conv3_roll. Weight fragments are
 // streamed PD k-steps ahead.
+constexpr int kRy = 16, kRPy = kRy + 2, kRPv = kRPx * kRPy;  // the 16 x 16 patch of k_conv3_roll_up
 template <int CS, int CF>
 __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(2))) void k_conv3_roll_up(Shape s, const bf16 *__restrict__ X, int ldx,
                                                      const bf16 *__restrict__ Wt, int Kp,
@@ -536,11 +534,11 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(2))) void k_
     }
 }
 
-// Levels per block of k_conv3_roll: enough (x, y, z) blocks to fill the chip
-// several times over, at least 4 levels each.
-static int roll_levels(Shape s)
+// Levels per block of the rolling kernels (tiles of 16 x ty columns): enough
+// (x, y, z) blocks to fill the chip several times over, at least 4 levels each.
+static int roll_levels(Shape s, int ty = 16)
 {
-    const long cols = (long)((s.W + kRx - 1) / kRx) * ((s.H + kRy - 1) / kRy) * s.B;
+    const long cols = (long)((s.W + kRx - 1) / kRx) * ((s.H + ty - 1) / ty) * s.B;
     const long ntz = std::max(1L, std::min((long)(s.D + 3) / 4, (2048 + cols - 1) / cols));
     return (int)((s.D + ntz - 1) / ntz);
 }
@@ -567,20 +565,37 @@ static void conv3_cc(hipStream_t st, Shape s, const bf16 *X, int ldx, int C, con
 static void conv3(hipStream_t st, Shape s, const bf16 *X, int ldx, int C, const bf16 *Wt, int Kp, const float *bias,
                   int N, bf16 *Y, int ldy, int relu, const bf16 *X2 = nullptr, int ldx2 = 0, int csplit = 0)
 {
-    if (N == 16 && (C == 8 || C == 16 || C == 48) && (X2 == nullptr || csplit % 8 == 0)) {
-        // the full-resolution layers: rolling z
-        const int zl = roll_levels(s);
+    auto roll_grid = [&](int ty, int &zl) {
+        zl = roll_levels(s, ty);
         const long ntz = (s.D + zl - 1) / zl;
-        const unsigned blocks =
-            (unsigned)((long)((s.W + kRx - 1) / kRx) * ((s.H + kRy - 1) / kRy) * ntz * s.B);
+        return (unsigned)((long)((s.W + kRx - 1) / kRx) * ((s.H + ty - 1) / ty) * ntz * s.B);
+    };
+    if (N == 16 && (C == 8 || C == 16 || C == 48) && (X2 == nullptr || csplit % 8 == 0)) {
+        // the full-resolution layers: rolling z, 16 x 16 patches
+        int zl;
+        const unsigned blocks = roll_grid(16, zl);
         if (C == 16)
-            k_conv3_roll<16><<<blocks, kT, 0, st>>>(s, X, ldx, Wt, Kp, bias, Y, ldy, relu, zl, X2, ldx2, csplit);
+            k_conv3_roll<16><<<blocks, kT, 0, st>>>(s, X, ldx, Wt, Kp, bias, N, Y, ldy, relu, zl, X2, ldx2, csplit);
         else if (C == 8)
-            k_conv3_roll<8><<<blocks, kT, 0, st>>>(s, X, ldx, Wt, Kp, bias, Y, ldy, relu, zl, X2, ldx2, csplit);
+            k_conv3_roll<8><<<blocks, kT, 0, st>>>(s, X, ldx, Wt, Kp, bias, N, Y, ldy, relu, zl, X2, ldx2, csplit);
         else if (X2 != nullptr && csplit == 32)
             k_conv3_roll_up<32, 16><<<blocks, kT, 0, st>>>(s, X, ldx, Wt, Kp, bias, Y, ldy, relu, zl, X2, ldx2);
         else
-            k_conv3_roll<48><<<blocks, kT, 0, st>>>(s, X, ldx, Wt, Kp, bias, Y, ldy, relu, zl, X2, ldx2, csplit);
+            k_conv3_roll<48><<<blocks, kT, 0, st>>>(s, X, ldx, Wt, Kp, bias, N, Y, ldy, relu, zl, X2, ldx2, csplit);
+        return;
+    }
+    if (N == 32 && (C == 16 || C == 32) && X2 == nullptr) {
+        // the 32-output layers: rolling z, 16 x 8 patches, two output blocks per A fragment
+        int zl;
+        unsigned blocks = roll_grid(8, zl);
+        if (C == 32) {
+            blocks = roll_grid(16, zl);
+            k_conv3_roll<32, 2, 4><<<blocks, kT, 0, st>>>(s, X, ldx, Wt, Kp, bias, N, Y, ldy, relu, zl, X2, ldx2,
+                                                         csplit);
+        }
+        else
+            k_conv3_roll<16, 2, 2><<<blocks, kT, 0, st>>>(s, X, ldx, Wt, Kp, bias, N, Y, ldy, relu, zl, X2, ldx2,
+                                                         csplit);
         return;
     }
     if (C % 32 == 0 && csplit % 32 == 0)
