@@ -1066,6 +1066,12 @@ float *windhip_solver::mg_coarse(size_t l, bool entered)
             k_mg_down_w<OpA<float>, true><<<(int)((C.g.P + kWaveCols - 1) / kWaveCols), 64 * kWaveCols, 0, stream>>>(
                 level_op_cm(L), gl, C.g, sp, cur, L.b, C.clo, C.ccp, C.cip, C.b, C.x, wc);
         });
+    } else if (mg_onepass && form == 3) {
+        // a plane-major level with bf16 rows: residual and restriction in one pass
+        timed(PK_MG_COARSE, [&] {
+            k_resid_restrict_c<OpAB><<<(int)((C.g.P + kColThreads - 1) / kColThreads) * kRRChunks, kColThreads, 0,
+                                       stream>>>(level_opab(L), gl, C.g, sp, cur, L.b, C.b, C.S());
+        });
     } else {
         resid(cur);
         timed(PK_MG_COARSE, [&] {
