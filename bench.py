@@ -125,6 +125,71 @@ def kernel_bytes(kind: str, nx: int, ny: int, nz: int, coef_bytes: int = 0) -> f
     return float(table[base])
 
 
+def coarse_cycle_model(levels, cpre=2, cpost=2, wave_cols=4608):
+    """Algorithmic bytes and launches of the coarse part of one V-cycle (the
+    default MIXED_MG cycle: V(2,2), automatic depth, fused transfers between
+    the wavefront-per-column levels, one-pass steps with bf16 rows and factors
+    on the larger levels, FP32 on the wavefront levels). levels: (nx, ny, mz)
+    of the coarse levels, coarsest last. Each step reads what it must once:
+    from zero, the rhs and the factors; with an iterate, the iterate, the rhs,
+    the six coefficient rows, the four BC planes and the three factor rows;
+    every step writes its iterate; a descent reads the finer iterate, rhs and
+    rows and writes the coarser rhs (and, fused, the coarser first step); a
+    return reads the coarser correction."""
+    geo = []
+    for nx, ny, mz in levels:
+        P = (nx + 2) * (ny + 2)
+        wave = P <= wave_cols and mz <= 64
+        geo.append(dict(P=P, N=P * mz, fb=4 if wave else 2, wave=wave, small=min(nx, ny)))
+    used = 0
+    while used < len(geo) and geo[used]["small"] >= 4:
+        used += 1
+    used = max(used, 1)
+    tot = dict(bytes=0.0, launches=0)
+
+    def add(b, n=1):
+        tot["bytes"] += b
+        tot["launches"] += n
+
+    def zero_step(g):
+        add((8 + 3 * g["fb"]) * g["N"])
+
+    def step(g):
+        add((12 + 9 * g["fb"]) * g["N"] + 16 * g["P"])
+
+    def cycle(l, entered):
+        g = geo[l]
+        if l + 1 == len(geo):
+            if not entered:
+                zero_step(g)
+            return
+        if not entered:
+            zero_step(g)
+        if l + 1 == used:
+            for _ in range(cpre + cpost - 1):
+                step(g)
+            return
+        for _ in range(cpre - 1):
+            step(g)
+        c = geo[l + 1]
+        fused = g["wave"] and c["wave"]
+        desc = (8 + 6 * g["fb"]) * g["N"] + 16 * g["P"] + 4 * c["N"]
+        add(desc + ((4 + 3 * c["fb"]) * c["N"] if fused else 0))
+        cycle(l + 1, fused)
+        if fused:
+            step(g)
+            add(4 * c["N"], 0)
+            post = cpost - 1
+        else:
+            add(4 * c["N"] + 8 * g["N"])
+            post = cpost
+        for _ in range(post):
+            step(g)
+
+    cycle(0, False)
+    return tot
+
+
 def build_case(nx, ny, nz):
     from hicar_amd import synthetic as S
     return S.fractal_case(nx=nx, ny=ny, nz=nz, dx=50.0, relief=2500.0, speed=8.0, seed=1)
@@ -432,15 +497,33 @@ def main():
                                                          "line_solve_f32", "smooth_f32",
                                                          "resid_restrict_f32") else 0
             nb = kernel_bytes(kind, case.nx, case.ny, case.nz, cb)
+            if kind == "mg_coarse_f32" and coarse is not None:
+                nb = coarse["bytes"]
             gbs = nb / (avg * 1e-3) / 1e9 if nb else None
             out[kind] = dict(calls=int(calls), avg_ms=round(avg, 5), total_ms=round(tot, 2), bytes=nb,
                              GBs=round(gbs, 1) if gbs else None, frac=round(gbs / HBM_PEAK_GBS, 3) if gbs else None)
+            if kind == "mg_coarse_f32" and coarse is not None:
+                out[kind].update(
+                    launches_per_call=coarse["launches"], us_per_launch=round(1e3 * avg / coarse["launches"], 2),
+                    note=("one call = the coarse part of one V-cycle (every launch from the fine restriction to "
+                          "the fine prolongation, timed as one region); bytes = the cycle's algorithmic bytes "
+                          "(coarse_cycle_model). Latency-bound: the levels below 130 x 130 columns hold about a "
+                          "microsecond of work per launch"))
             if kind in ("update_s_line_f32", "update_xrp_line_f32"):
                 out[kind]["note"] = ("bytes of the full update: the closing iteration of each inner solve "
                                      "updates only x (and r), and the vectors written just before are partly "
                                      "served from the 256 MB Infinity Cache, so frac can exceed 1")
         return out
 
+    coarse = None
+    if any(a.startswith("mg") for a in res):
+        import ctypes
+        lv, q = [], 1
+        dims = (ctypes.c_int * 3)()
+        while s.lib.windhip_mg_level(s.h, q, dims, None) == 0:  # the hierarchy of the last MIXED_MG solve
+            lv.append((dims[0], dims[1], dims[2]))
+            q += 1
+        coarse = coarse_cycle_model(lv) if lv else None
     tables = {arm: kernel_table(r.pop("prof"), arm) for arm, r in res.items()}
     kh = tables[head]
     # the dominant single kernel (the coarse-level kind mixes every kernel of every coarse level)

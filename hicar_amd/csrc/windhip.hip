@@ -389,6 +389,7 @@ struct windhip_solver {
     int enqueue_iterations(int count);
     template <class Launch>
     int timed(int kind, Launch &&launch);
+    bool prof_nested = false;  // a timed region is open (its inner launches get no events of their own)
     int harvest();
     template <typename T>
     void kvec(T **X, T **R, T **RH, T **P, T **V, T **S, T **Tv, T **PH, T **SH, const T **DI);
@@ -407,6 +408,24 @@ struct windhip_solver {
     // of up to 44 rows per column, and the fine level's residual with the
     // restriction (k_resid_restrict_c)
     bool mg_onepass = true;
+    // The cycle from the first column-major level down as one persistent
+    // launch (k_mg_tail) instead of a launch per step (off by default: slower
+    // at 512 x 512 x 40): its plan (uploaded when the cycle's shape changes),
+    // grid-barrier state, grid, the level it starts at (SIZE_MAX: off) and the
+    // buffer that holds its result.
+    bool mg_tail = false;
+    MgTailPlan tail_host{};
+    MgTailPlan *tail_plan = nullptr;
+    MgGridBar *tail_bar = nullptr;
+    int *tail_err_host = nullptr;
+    int tail_blocks = 0;
+    long tail_resident = 0;
+    size_t tail_l0 = SIZE_MAX;
+    float *tail_out = nullptr;
+    size_t mg_used() const;
+    float *plan_coarse(size_t l, bool entered, size_t l0, std::vector<MgTailStep> &ps);
+    int prepare_tail(size_t l0);
+    int check_tail();
     // form 0: residual + line solve (FP32 rows); 1: one launch, a wavefront
     // per column; 2: residual + line solve on the level's bf16 rows
     void mg_smooth(size_t l, int form, const float *y, float *out);
@@ -577,6 +596,7 @@ void windhip_solver::release()
         if (stage[q]) (void)hipFree(stage[q]);
     if (st_host) (void)hipHostFree(st_host);
     if (scan_host) (void)hipHostFree(scan_host);
+    if (tail_err_host) (void)hipHostFree(tail_err_host);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (ev_net0) (void)hipEventDestroy(ev_net0);
@@ -713,7 +733,7 @@ enum { PK_SWEEP = 0, PK_SIGMA = 1, PK_UPD_S = 2, PK_OMEGA = 3, PK_UPD_XRP = 4, P
 template <class Launch>
 int windhip_solver::timed(int kind, Launch &&launch)
 {
-    if (!profiling) {
+    if (!profiling || prof_nested) {  // inside a timed region: its launches are timed as a whole
         launch();
         return WINDHIP_OK;
     }
@@ -725,7 +745,9 @@ int windhip_solver::timed(int kind, Launch &&launch)
     }
     auto &pr = ev_pool[ev_next];
     HIPCHK(hipEventRecord(pr.first, stream));
+    prof_nested = true;
     launch();
+    prof_nested = false;
     HIPCHK(hipEventRecord(pr.second, stream));
     ev_kind.push_back(kind);
     ++ev_next;
@@ -876,7 +898,10 @@ int windhip_solver::enqueue_iterations_line(int count, bool multigrid)
                                                                                           mg[0].b, 0, mg[0].S());
                     }));
                 }
-                float *e = mg_coarse(0);
+                // the coarse levels as one timed region (per-launch events would
+                // add their own gaps to the microsecond launches there)
+                float *e = nullptr;
+                CHK(timed(PK_MG_COARSE, [&] { e = mg_coarse(0); }));
                 CHK(timed(PK_MG_TRANSFER, [&] {
                     k_mg_prolong<<<mg_plane_grid(f0), kMgThreads, 0, stream>>>(f0, mg[0].g, sp, e, cur, 0, mg[0].S());
                 }));
@@ -977,6 +1002,130 @@ void windhip_solver::mg_smooth(size_t l, int form, const float *y, float *out)
     });
 }
 
+// Coarse levels the V-cycle uses.
+size_t windhip_solver::mg_used() const
+{
+    if (mg_depth > 0) return std::min(mg.size(), (size_t)mg_depth);
+    // the deepest level that still has 4 interior columns each way: below it
+    // the cell-centred levels misplace the lateral boundary by half a coarse
+    // cell and their correction of the smoothest error hurts
+    size_t q = 0;
+    while (q < mg.size() && std::min(mg[q].g.nx, mg[q].g.ny) >= 4) ++q;
+    return std::max<size_t>(q, 1);
+}
+
+// The steps mg_coarse enqueues from column-major level l down (every level
+// from l0 on column-major, fused transfers, mg_cpost > 0), as the steps of
+// k_mg_tail, in the same order with the same buffers; returns the buffer that
+// holds the correction of level l.
+float *windhip_solver::plan_coarse(size_t l, bool entered, size_t l0, std::vector<MgTailStep> &ps)
+{
+    MgLevel &L = mg[l];
+    const int lv = (int)(l - l0);
+    auto step = [&](int kind, float w, const float *y, const float *e, float *out) {
+        MgTailStep t{};
+        t.kind = kind; t.level = lv; t.w = w; t.y = y; t.e = e; t.out = out;
+        ps.push_back(t);
+    };
+    if (l + 1 == mg.size()) {  // one interior column: its line solve is exact
+        if (!entered) step(kTailSmooth, 1.0f, nullptr, nullptr, L.x);
+        return L.x;
+    }
+    float *cur = L.x, *alt = L.t;
+    if (!entered) step(kTailSmooth, mg_omega, nullptr, nullptr, cur);
+    if (l + 1 == mg_used()) {
+        for (int q = 1; q < mg_cpre + mg_cpost; ++q) {
+            step(kTailSmooth, mg_omega, cur, nullptr, alt);
+            std::swap(cur, alt);
+        }
+        return cur;
+    }
+    for (int q = 1; q < mg_cpre; ++q) {
+        step(kTailSmooth, mg_omega, cur, nullptr, alt);
+        std::swap(cur, alt);
+    }
+    const MgLevel &C = mg[l + 1];
+    step(kTailDown, l + 2 == mg.size() ? 1.0f : mg_omega, cur, nullptr, C.x);
+    float *e = plan_coarse(l + 1, true, l0, ps);
+    step(kTailUp, mg_omega, cur, e, alt);
+    std::swap(cur, alt);
+    for (int q = 1; q < mg_cpost; ++q) {
+        step(kTailSmooth, mg_omega, cur, nullptr, alt);
+        std::swap(cur, alt);
+    }
+    return cur;
+}
+
+// Make mg_coarse(l0) one k_mg_tail launch when the cycle from l0 down fits
+// it, else leave the per-step launches (tail_l0 = SIZE_MAX). Uploads the plan
+// when it changed; never called while the stream is captured.
+int windhip_solver::prepare_tail(size_t l0)
+{
+    tail_l0 = SIZE_MAX;
+    if (!mg_tail || !mg_fused || mg_cpost <= 0 || l0 >= mg.size() || mg.size() - l0 > (size_t)kTailMaxLevels)
+        return WINDHIP_OK;
+    for (size_t l = l0; l < mg.size(); ++l)
+        if (!mg[l].cm) return WINDHIP_OK;
+    std::vector<MgTailStep> ps;
+    float *res = plan_coarse(l0, false, l0, ps);
+    if (ps.size() > (size_t)kTailMaxSteps) return WINDHIP_OK;
+    if (!tail_plan) {
+        int dev = 0, cus = 0, per_cu = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mg_tail, 64 * kTailWaves, 0));
+        // blocks that can all be resident at once (the grid barrier waits for every block)
+        tail_resident = (long)per_cu * cus;
+        if (tail_resident < 1) return WINDHIP_OK;
+        CHK(alloc(&tail_plan, 1));
+        CHK(alloc(&tail_bar, 1));
+        HIPCHK(hipHostMalloc((void **)&tail_err_host, sizeof(int), hipHostMallocDefault));
+        *tail_err_host = 0;
+        std::memset(&tail_host, 0, sizeof tail_host);
+    }
+    MgTailPlan h;
+    std::memset(&h, 0, sizeof h);  // padding too: the plans are compared bytewise
+    h.nsteps = (int)ps.size();
+    h.nlevels = (int)(mg.size() - l0);
+    for (size_t q = 0; q < ps.size(); ++q) h.step[q] = ps[q];
+    for (int q = 0; q < h.nlevels; ++q) {
+        const MgLevel &L = mg[l0 + (size_t)q];
+        MgTailLevel &v = h.lv[q];
+        v.op = level_op_cm(L);
+        v.g = L.gg;
+        v.m = L.g;
+        v.lo = L.clo; v.cp = L.ccp; v.ip = L.cip;
+        v.b = L.b;
+    }
+    if (std::memcmp(&h, &tail_host, sizeof h) != 0) {
+        tail_host = h;
+        HIPCHK(hipMemcpyAsync(tail_plan, &tail_host, sizeof tail_host, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+    // a wavefront per column of the largest level, at most the resident grid
+    const long need = (mg[l0].g.P + kTailWaves - 1) / kTailWaves;
+    tail_blocks = (int)std::min<long>(std::min<long>(need, tail_resident), kTailMaxBlocks);
+    if (tail_blocks < 1) return WINDHIP_OK;
+    tail_out = res;
+    tail_l0 = l0;
+    return WINDHIP_OK;
+}
+
+// A grid barrier of k_mg_tail that ran out of time (sticky) is an error.
+int windhip_solver::check_tail()
+{
+    if (!tail_bar) return WINDHIP_OK;
+    HIPCHK(hipMemcpyAsync(tail_err_host, &tail_bar->err, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    if (*tail_err_host) {
+        *tail_err_host = 0;
+        HIPCHK(hipMemsetAsync(tail_bar, 0, sizeof(MgGridBar), stream));  // a fresh barrier for the next solve
+        HIPCHK(hipStreamSynchronize(stream));
+        return fail(WINDHIP_ERR_HIP, "multigrid: a grid barrier of the persistent coarse-level launch timed out");
+    }
+    return WINDHIP_OK;
+}
+
 // Enqueue the V-cycle below the fine level from coarse level l (rhs in
 // mg[l].b); returns the buffer that holds its correction. entered: the level
 // above has already written the rhs and the first smoothing step into mg[l].x
@@ -1013,17 +1162,13 @@ float *windhip_solver::mg_coarse(size_t l, bool entered)
                 k_resid_c<OpA<float>><<<kb, kColThreads, 0, stream>>>(op, gl, sp, xv, L.b, L.r);
         });
     };
-    size_t used = mg.size();
-    if (mg_depth > 0) {
-        used = std::min(mg.size(), (size_t)mg_depth);
-    } else {
-        // the deepest level that still has 4 interior columns each way: below
-        // it the cell-centred levels misplace the lateral boundary by half a
-        // coarse cell and their correction of the smoothest error hurts
-        size_t q = 0;
-        while (q < mg.size() && std::min(mg[q].g.nx, mg[q].g.ny) >= 4) ++q;
-        used = std::max<size_t>(q, 1);
+    if (!entered && l == tail_l0) {  // this level and all below in one launch
+        timed(PK_MG_COARSE, [&] {
+            k_mg_tail<<<tail_blocks, 64 * kTailWaves, 0, stream>>>(tail_plan, sp, tail_bar);
+        });
+        return tail_out;
     }
+    const size_t used = mg_used();
     if (l + 1 == mg.size()) {  // one interior column: its line solve is exact
         if (!entered) {
             if (L.cm)
@@ -1494,6 +1639,11 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
         CHK(build_mg());
         mg_gen = op_gen;
     }
+    if (multigrid) {  // the column-major levels as one launch where they fit it
+        size_t l0 = 0;
+        while (l0 < mg.size() && !mg[l0].cm) ++l0;
+        CHK(prepare_tail(l0));
+    }
     // the learned guess, computed on the second stream while the set-up above
     // ran, becomes the start when its residual is below the plain start's
     if (learned_guess) {
@@ -1650,6 +1800,7 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
         if (!(st_host[0].outer_rnorm < before)) { status = 2; break; }
         (void)breakdown;
     }
+    if (multigrid && tail_l0 != SIZE_MAX) CHK(check_tail());
     info->status = st_host[0].outer_done ? 0 : (status ? status : 1);
     info->iters = total_inner;
     info->outer_iters = steps;
@@ -2381,7 +2532,10 @@ int windhip_set_multigrid(windhip_solver *s, int pre, int post, int coarse_pre, 
 int windhip_set_mg_fusion(windhip_solver *s, int fused_transfers, int onepass)
 {
     if (!s) return fail(WINDHIP_ERR_ARG, "windhip_set_mg_fusion: null handle");
+    if (fused_transfers < 0 || fused_transfers > 2)
+        return fail(WINDHIP_ERR_ARG, "windhip_set_mg_fusion: fused_transfers %d outside 0..2", fused_transfers);
     s->mg_fused = fused_transfers != 0;
+    s->mg_tail = fused_transfers == 2;
     s->mg_onepass = onepass != 0;
     for (auto &kv : s->graphs) (void)hipGraphExecDestroy(kv.second);  // the cached V-cycles change shape
     s->graphs.clear();
@@ -2453,6 +2607,7 @@ int windhip_mg_smooth(windhip_solver *s, int level, int fused, const float *b, c
 int windhip_mg_cycle(windhip_solver *s, int level, int fused, const float *b, float *out)
 {
     if (!s || !b || !out) return fail(WINDHIP_ERR_ARG, "windhip_mg_cycle: null argument");
+    if (fused < 0 || fused > 2) return fail(WINDHIP_ERR_ARG, "windhip_mg_cycle: fused %d outside 0..2", fused);
     if (level < 1 || level > (int)s->mg.size())
         return fail(WINDHIP_ERR_ARG, "windhip_mg_cycle: level %d outside 1..%zu (run a MIXED_MG solve first)", level,
                     s->mg.size());
@@ -2467,15 +2622,24 @@ int windhip_mg_cycle(windhip_solver *s, int level, int fused, const float *b, fl
     // the V-cycle's kernels return at once when the inner state says done
     k_state_reset<<<1, 1, 0, s->stream>>>(s->st + 1, 1, 0.0, 0.0, 0.5);
     HIPCHK(hipMemcpyAsync(L.b, L.cm ? tb.data() : b, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
-    const bool keep = s->mg_fused, prof = s->profiling;
+    const bool keep = s->mg_fused, keep_tail = s->mg_tail, prof = s->profiling;
     s->mg_fused = fused != 0;
+    s->mg_tail = fused == 2;
     s->profiling = false;
-    float *e = s->mg_coarse((size_t)level - 1);
+    size_t l0 = (size_t)level - 1;  // the persistent launch starts at the first column-major level
+    while (l0 < s->mg.size() && !s->mg[l0].cm) ++l0;
+    int rc = s->prepare_tail(fused == 2 ? l0 : SIZE_MAX);
+    float *e = rc == WINDHIP_OK ? s->mg_coarse((size_t)level - 1) : nullptr;
+    const bool tail = s->tail_l0 != SIZE_MAX;
+    s->tail_l0 = SIZE_MAX;  // the next solve plans its own
     s->mg_fused = keep;
+    s->mg_tail = keep_tail;
     s->profiling = prof;
+    CHK(rc);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(L.cm ? to.data() : out, e, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
+    if (tail) CHK(s->check_tail());
     if (L.cm) mg_rows_from_cm(L.g, to.data(), out);
     return WINDHIP_OK;
 }

@@ -226,20 +226,77 @@ __device__ __forceinline__ float wave_line_solve(int lane, bool act, float r, fl
     return s2;
 }
 
+// Vector element access of the wavefront steps. COH: at the device-coherent
+// level (agent-scope relaxed atomics), for vectors that one launch both
+// writes and reads across blocks (k_mg_tail): no L2 write-back or invalidation
+// is then needed between its steps; plain accesses otherwise.
+template <bool COH>
+__device__ __forceinline__ float vld(const float *p, long i)
+{
+    if constexpr (COH)
+        return __hip_atomic_load(const_cast<float *>(p) + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        return p[i];
+}
+template <bool COH>
+__device__ __forceinline__ void vst(float *p, long i, float v)
+{
+    if constexpr (COH)
+        __hip_atomic_store(p + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        p[i] = v;
+}
+
 // The residual row (b - A y)(row n) of a column held one row per lane, from
 // the lane's own five columns (k-1 and k+1 taken from the neighbouring lanes).
-template <class Op>
+template <class Op, bool COH = false>
 __device__ __forceinline__ float wave_resid(const Op &op, const float *__restrict__ y, float bv, long n, long c,
                                             int k, int mz, bool lat, long dx, long dy)
 {
     Win<float> x;
-    const float yc = y[n], yw = y[n - dx], ye = y[n + dx], ys = y[n - dy], yn = y[n + dy];
+    const float yc = vld<COH>(y, n), yw = vld<COH>(y, n - dx), ye = vld<COH>(y, n + dx), ys = vld<COH>(y, n - dy),
+                yn = vld<COH>(y, n + dy);
     x.c1 = yc; x.w1 = yw; x.e1 = ye; x.s1 = ys; x.n1 = yn;
     x.c0 = __shfl_up(yc, 1, 64); x.w0 = __shfl_up(yw, 1, 64); x.e0 = __shfl_up(ye, 1, 64);
     x.s0 = __shfl_up(ys, 1, 64); x.n0 = __shfl_up(yn, 1, 64);
     x.c2 = __shfl_down(yc, 1, 64); x.w2 = __shfl_down(yw, 1, 64); x.e2 = __shfl_down(ye, 1, 64);
     x.s2 = __shfl_down(ys, 1, 64); x.n2 = __shfl_down(yn, 1, 64);
     return bv - op.row(k, mz, lat, x, op.load(n), op.bc(c));
+}
+
+// One damped line-smoothing step of column c of a level, lane k holding row
+// k: out = y + w L^-1 (b - A y) (y null: out = w L^-1 b).
+template <class Op, bool CM, bool COH = false>
+__device__ __forceinline__ void wave_smooth_col(const Op &op, const Geo &g, const float *__restrict__ b,
+                                                const float *__restrict__ y, const float *__restrict__ lo,
+                                                const float *__restrict__ cp, const float *__restrict__ ip,
+                                                float *__restrict__ out, float w, long c, int lane)
+{
+    const int mz = g.mz;
+    const bool act = lane < mz;
+    const int k = act ? lane : mz - 1;
+    const MgIdx ix{g.P, CM ? mz : 0};
+    const long n = ix.n(k, c), dx = ix.col(1), dy = ix.col(g.mx);
+    const int i = (int)(c % g.mx), j = (int)(c / g.mx);
+    const bool lat = i == 0 || i == g.mx - 1 || j == 0 || j == g.my - 1;
+    float r = vld<COH>(b, n);
+    float yc = 0.f;
+    if (y) {
+        Win<float> x;
+        yc = vld<COH>(y, n);
+        const float yw = vld<COH>(y, n - dx), ye = vld<COH>(y, n + dx), ys = vld<COH>(y, n - dy),
+                    yn = vld<COH>(y, n + dy);
+        x.c1 = yc; x.w1 = yw; x.e1 = ye; x.s1 = ys; x.n1 = yn;
+        x.c0 = __shfl_up(yc, 1, 64); x.w0 = __shfl_up(yw, 1, 64); x.e0 = __shfl_up(ye, 1, 64);
+        x.s0 = __shfl_up(ys, 1, 64); x.n0 = __shfl_up(yn, 1, 64);
+        x.c2 = __shfl_down(yc, 1, 64); x.w2 = __shfl_down(yw, 1, 64); x.e2 = __shfl_down(ye, 1, 64);
+        x.s2 = __shfl_down(ys, 1, 64); x.n2 = __shfl_down(yn, 1, 64);
+        const typename Op::Cf f = op.load(n);
+        const typename Op::Bc bc = op.bc(c);
+        r = r - op.row(k, mz, lat, x, f, bc);
+    }
+    const float s2 = wave_line_solve(lane, act, r, lo[n], cp[n], ip[n]);
+    if (act) vst<COH>(out, n, yc + w * s2);
 }
 
 template <class Op, bool CM = false>
@@ -252,33 +309,9 @@ __global__ __launch_bounds__(64 * kWaveCols) void k_mg_smooth_w(Op op, Geo g, co
                                                                float *__restrict__ out, float w)
 {
     if (st->done) return;
-    const int lane = threadIdx.x & 63;
     const long c = (long)blockIdx.x * kWaveCols + (threadIdx.x >> 6);
     if (c >= g.P) return;  // uniform per wavefront
-    const int mz = g.mz;
-    const bool act = lane < mz;
-    const int k = act ? lane : mz - 1;
-    const MgIdx ix{g.P, CM ? mz : 0};
-    const long n = ix.n(k, c), dx = ix.col(1), dy = ix.col(g.mx);
-    const int i = (int)(c % g.mx), j = (int)(c / g.mx);
-    const bool lat = i == 0 || i == g.mx - 1 || j == 0 || j == g.my - 1;
-    float r = b[n];
-    float yc = 0.f;
-    if (y) {
-        Win<float> x;
-        yc = y[n];
-        const float yw = y[n - dx], ye = y[n + dx], ys = y[n - dy], yn = y[n + dy];
-        x.c1 = yc; x.w1 = yw; x.e1 = ye; x.s1 = ys; x.n1 = yn;
-        x.c0 = __shfl_up(yc, 1, 64); x.w0 = __shfl_up(yw, 1, 64); x.e0 = __shfl_up(ye, 1, 64);
-        x.s0 = __shfl_up(ys, 1, 64); x.n0 = __shfl_up(yn, 1, 64);
-        x.c2 = __shfl_down(yc, 1, 64); x.w2 = __shfl_down(yw, 1, 64); x.e2 = __shfl_down(ye, 1, 64);
-        x.s2 = __shfl_down(ys, 1, 64); x.n2 = __shfl_down(yn, 1, 64);
-        const typename Op::Cf f = op.load(n);
-        const typename Op::Bc bc = op.bc(c);
-        r = r - op.row(k, mz, lat, x, f, bc);
-    }
-    const float s2 = wave_line_solve(lane, act, r, lo[n], cp[n], ip[n]);
-    if (act) out[n] = yc + w * s2;
+    wave_smooth_col<Op, CM>(op, g, b, y, lo, cp, ip, out, w, c, threadIdx.x & 63);
 }
 
 // The descent into a coarse level in one launch, for coarse levels of at most
@@ -289,7 +322,46 @@ __global__ __launch_bounds__(64 * kWaveCols) void k_mg_smooth_w(Op op, Geo g, co
 // x_c = w L_c^-1 b_c. One wavefront per coarse column, lane k holding row k.
 // y_f needs mx_f + 1 guard elements on both sides (the neighbour loads of
 // the finer level's ring columns fall on them). Ring columns of the coarse
-// level get b_c = x_c = 0.
+// level get b_c = x_c = 0. wave_down_col: coarse column cc.
+template <class Op, bool CM, bool COH = false>
+__device__ __forceinline__ void wave_down_col(const Op &opf, const Geo &gf, const MgGeo &c,
+                                              const float *__restrict__ yf, const float *__restrict__ bf,
+                                              const float *__restrict__ lo, const float *__restrict__ cp,
+                                              const float *__restrict__ ip, float *__restrict__ bc,
+                                              float *__restrict__ xc, float w, long cc, int lane)
+{
+    const int mz = c.mz;
+    const bool act = lane < mz;
+    const int k = act ? lane : mz - 1;
+    const MgIdx ic{c.P, CM ? mz : 0}, iff{gf.P, CM ? mz : 0};
+    const long n = ic.n(k, cc);
+    const int I = (int)(cc % c.mx), J = (int)(cc / c.mx);
+    if (I == 0 || I == c.mx - 1 || J == 0 || J == c.my - 1) {
+        if (act) { vst<COH>(bc, n, 0.f); vst<COH>(xc, n, 0.f); }
+        return;
+    }
+    const long mxf = gf.mx;
+    const int i0 = 2 * I - 1, j0 = 2 * J - 1;
+    const bool hi = i0 + 1 <= (int)mxf - 2, hj = j0 + 1 <= gf.my - 2;
+    const long cf0 = (long)j0 * mxf + i0, m = iff.n(k, cf0), dx = iff.col(1), dy = iff.col(mxf);
+    // the fine columns are interior columns of the finer level
+    float s = wave_resid<Op, COH>(opf, yf, vld<COH>(bf, m), m, cf0, k, mz, false, dx, dy);
+    if (hi) s += wave_resid<Op, COH>(opf, yf, vld<COH>(bf, m + dx), m + dx, cf0 + 1, k, mz, false, dx, dy);
+    if (hj) {
+        float t = wave_resid<Op, COH>(opf, yf, vld<COH>(bf, m + dy), m + dy, cf0 + mxf, k, mz, false, dx, dy);
+        if (hi)
+            t += wave_resid<Op, COH>(opf, yf, vld<COH>(bf, m + dy + dx), m + dy + dx, cf0 + mxf + 1, k, mz, false,
+                                     dx, dy);
+        s += t;
+    }
+    const float b = s / (float)((hi ? 2 : 1) * (hj ? 2 : 1));
+    const float s2 = wave_line_solve(lane, act, b, lo[n], cp[n], ip[n]);
+    if (act) {
+        vst<COH>(bc, n, b);
+        vst<COH>(xc, n, w * s2);
+    }
+}
+
 template <class Op, bool CM = false>
 __global__ __launch_bounds__(64 * kWaveCols) void k_mg_down_w(Op opf, Geo gf, MgGeo c, const KState *st,
                                                              const float *__restrict__ yf,
@@ -300,37 +372,9 @@ __global__ __launch_bounds__(64 * kWaveCols) void k_mg_down_w(Op opf, Geo gf, Mg
                                                              float *__restrict__ bc, float *__restrict__ xc, float w)
 {
     if (st->done) return;
-    const int lane = threadIdx.x & 63;
     const long cc = (long)blockIdx.x * kWaveCols + (threadIdx.x >> 6);
     if (cc >= c.P) return;  // uniform per wavefront
-    const int mz = c.mz;
-    const bool act = lane < mz;
-    const int k = act ? lane : mz - 1;
-    const MgIdx ic{c.P, CM ? mz : 0}, iff{gf.P, CM ? mz : 0};
-    const long n = ic.n(k, cc);
-    const int I = (int)(cc % c.mx), J = (int)(cc / c.mx);
-    if (I == 0 || I == c.mx - 1 || J == 0 || J == c.my - 1) {
-        if (act) { bc[n] = 0.f; xc[n] = 0.f; }
-        return;
-    }
-    const long mxf = gf.mx;
-    const int i0 = 2 * I - 1, j0 = 2 * J - 1;
-    const bool hi = i0 + 1 <= (int)mxf - 2, hj = j0 + 1 <= gf.my - 2;
-    const long cf0 = (long)j0 * mxf + i0, m = iff.n(k, cf0), dx = iff.col(1), dy = iff.col(mxf);
-    // the fine columns are interior columns of the finer level
-    float s = wave_resid(opf, yf, bf[m], m, cf0, k, mz, false, dx, dy);
-    if (hi) s += wave_resid(opf, yf, bf[m + dx], m + dx, cf0 + 1, k, mz, false, dx, dy);
-    if (hj) {
-        float t = wave_resid(opf, yf, bf[m + dy], m + dy, cf0 + mxf, k, mz, false, dx, dy);
-        if (hi) t += wave_resid(opf, yf, bf[m + dy + dx], m + dy + dx, cf0 + mxf + 1, k, mz, false, dx, dy);
-        s += t;
-    }
-    const float b = s / (float)((hi ? 2 : 1) * (hj ? 2 : 1));
-    const float s2 = wave_line_solve(lane, act, b, lo[n], cp[n], ip[n]);
-    if (act) {
-        bc[n] = b;
-        xc[n] = w * s2;
-    }
+    wave_down_col<Op, CM>(opf, gf, c, yf, bf, lo, cp, ip, bc, xc, w, cc, threadIdx.x & 63);
 }
 
 // The return to a finer level in one launch, for levels of at most 64 rows per
@@ -338,21 +382,14 @@ __global__ __launch_bounds__(64 * kWaveCols) void k_mg_down_w(Op opf, Geo gf, Mg
 // correction e) at the column and its four neighbours, then one damped
 // line-smoothing step out = z + w L^-1 (b - A z). One wavefront per column of
 // the finer level, lane k holding row k. y needs mx + 1 guard elements on
-// both sides; out must alias neither y nor b.
-template <class Op, bool CM = false>
-__global__ __launch_bounds__(64 * kWaveCols) void k_mg_up_w(Op op, Geo g, MgGeo c, const KState *st,
-                                                           const float *__restrict__ y,
-                                                           const float *__restrict__ ec,
-                                                           const float *__restrict__ b,
-                                                           const float *__restrict__ lo,
-                                                           const float *__restrict__ cp,
-                                                           const float *__restrict__ ip,
-                                                           float *__restrict__ out, float w)
+// both sides; out must alias neither y nor b. wave_up_col: column col.
+template <class Op, bool CM, bool COH = false>
+__device__ __forceinline__ void wave_up_col(const Op &op, const Geo &g, const MgGeo &c, const float *__restrict__ y,
+                                            const float *__restrict__ ec, const float *__restrict__ b,
+                                            const float *__restrict__ lo, const float *__restrict__ cp,
+                                            const float *__restrict__ ip, float *__restrict__ out, float w, long col,
+                                            int lane)
 {
-    if (st->done) return;
-    const int lane = threadIdx.x & 63;
-    const long col = (long)blockIdx.x * kWaveCols + (threadIdx.x >> 6);
-    if (col >= g.P) return;  // uniform per wavefront
     const int mz = g.mz;
     const bool act = lane < mz;
     const int k = act ? lane : mz - 1;
@@ -363,13 +400,14 @@ __global__ __launch_bounds__(64 * kWaveCols) void k_mg_up_w(Op op, Geo g, MgGeo 
     const bool lat = i == 0 || i == g.mx - 1 || j == 0 || j == g.my - 1;
     // y + P e at fine column (ii, jj) of row k; ring columns are not prolonged
     auto z = [&](int ii, int jj) -> float {
-        const float yv = y[ix.n(k, (long)jj * mx + ii)];
+        const float yv = vld<COH>(y, ix.n(k, (long)jj * mx + ii));
         if (ii <= 0 || ii >= g.mx - 1 || jj <= 0 || jj >= g.my - 1) return yv;
         const int I = (ii + 1) >> 1, J = (jj + 1) >> 1;
         const int In = (ii & 1) ? I - 1 : I + 1, Jn = (jj & 1) ? J - 1 : J + 1;
-        const float v = 0.5625f * ec[ic.n(k, (long)J * c.mx + I)] +
-                        0.1875f * (ec[ic.n(k, (long)J * c.mx + In)] + ec[ic.n(k, (long)Jn * c.mx + I)]) +
-                        0.0625f * ec[ic.n(k, (long)Jn * c.mx + In)];
+        const float v = 0.5625f * vld<COH>(ec, ic.n(k, (long)J * c.mx + I)) +
+                        0.1875f * (vld<COH>(ec, ic.n(k, (long)J * c.mx + In)) +
+                                   vld<COH>(ec, ic.n(k, (long)Jn * c.mx + I))) +
+                        0.0625f * vld<COH>(ec, ic.n(k, (long)Jn * c.mx + In));
         return yv + v;
     };
     Win<float> x;
@@ -382,9 +420,166 @@ __global__ __launch_bounds__(64 * kWaveCols) void k_mg_up_w(Op op, Geo g, MgGeo 
     x.s0 = __shfl_up(zs, 1, 64); x.n0 = __shfl_up(zn, 1, 64);
     x.c2 = __shfl_down(zc, 1, 64); x.w2 = __shfl_down(zw, 1, 64); x.e2 = __shfl_down(ze, 1, 64);
     x.s2 = __shfl_down(zs, 1, 64); x.n2 = __shfl_down(zn, 1, 64);
-    const float r = b[n] - op.row(k, mz, lat, x, op.load(n), op.bc(col));
+    const float r = vld<COH>(b, n) - op.row(k, mz, lat, x, op.load(n), op.bc(col));
     const float s2 = wave_line_solve(lane, act, r, lo[n], cp[n], ip[n]);
-    if (act) out[n] = zc + w * s2;
+    if (act) vst<COH>(out, n, zc + w * s2);
+}
+
+template <class Op, bool CM = false>
+__global__ __launch_bounds__(64 * kWaveCols) void k_mg_up_w(Op op, Geo g, MgGeo c, const KState *st,
+                                                           const float *__restrict__ y,
+                                                           const float *__restrict__ ec,
+                                                           const float *__restrict__ b,
+                                                           const float *__restrict__ lo,
+                                                           const float *__restrict__ cp,
+                                                           const float *__restrict__ ip,
+                                                           float *__restrict__ out, float w)
+{
+    if (st->done) return;
+    const long col = (long)blockIdx.x * kWaveCols + (threadIdx.x >> 6);
+    if (col >= g.P) return;  // uniform per wavefront
+    wave_up_col<Op, CM>(op, g, c, y, ec, b, lo, cp, ip, out, w, col, threadIdx.x & 63);
+}
+
+// The cycle below a column-major level as one persistent launch: the same
+// steps as the k_mg_smooth_w / k_mg_down_w / k_mg_up_w launches of that part
+// of the V-cycle, each a loop of the grid's wavefronts over the level's
+// columns, separated by grid-wide barriers instead of kernel boundaries (the
+// levels of at most 66 x 66 columns hold about a microsecond of work per
+// step). Per column the arithmetic is that of the separate kernels, so the
+// result is bitwise the same. The vectors the steps pass to each other are
+// read and written at the device-coherent level (vld/vst<true>), so a barrier
+// needs no L2 write-back or invalidation; the rows and factors, constant
+// during the launch, stay cached. Measured at 512 x 512 x 40 (20 steps, 545
+// blocks): 130-145 us per launch against 115 us for the 20 separate launches
+// (a barrier costs what a kernel boundary does, 5-7 us), so the solve runs
+// the separate launches unless asked for this form.
+struct MgTailLevel {
+    OpA<float> op;  // column-major FP32 rows
+    Geo g;
+    MgGeo m;
+    const float *lo, *cp, *ip;  // column-major line factors
+    float *b;                   // rhs (column-major)
+};
+enum { kTailSmooth = 0, kTailDown = 1, kTailUp = 2 };
+struct MgTailStep {
+    int kind;      // kTailSmooth: step on `level`; kTailDown / kTailUp: transfer between level and level + 1
+    int level;
+    float w;       // damping of the step's line solve
+    const float *y;  // iterate on `level` (kTailSmooth: null = from zero)
+    const float *e;  // kTailUp: the correction on level + 1
+    float *out;      // kTailSmooth / kTailUp: new iterate on `level`; kTailDown: first iterate on level + 1
+};
+constexpr int kTailMaxSteps = 64, kTailMaxLevels = 12, kTailMaxBlocks = 4096;
+struct MgTailPlan {
+    int nsteps, nlevels;
+    MgTailStep step[kTailMaxSteps];
+    MgTailLevel lv[kTailMaxLevels];
+};
+// Grid barrier state (zeroed once). Barrier q of a launch uses the epoch
+// E = G + q, G the value of gen at the launch's start: every block but block
+// 0 writes E to its flag and waits for gen to reach E; block 0 waits for all
+// flags to read E and then writes E to the eight copies of gen (one cache
+// line each, polled by an eighth of the blocks). err is sticky: a wait that
+// ran out sets it, every later launch returns at once, and the host reports
+// it (and clears the state).
+struct MgGridBar {
+    unsigned gen[8][32];
+    int err;
+    unsigned pad[31];
+    unsigned flag[kTailMaxBlocks];
+};
+constexpr int kTailWaves = 8;                   // wavefronts (columns in flight) per block
+constexpr unsigned kTailSpinLimit = 1u << 20;   // polls of one wait (about a second) before it gives up
+
+__device__ __forceinline__ unsigned ld_coh_u(const unsigned *p)
+{
+    return __hip_atomic_load(const_cast<unsigned *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coh_u(unsigned *p, unsigned v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// All blocks of the grid (resident together: the host sizes the grid within
+// the device's occupancy) meet here; the coherent stores before it are seen
+// by the coherent loads after it in every block. Returns false when a wait
+// ran out (here or in another block).
+__device__ __forceinline__ bool grid_barrier(MgGridBar *bar, unsigned nblocks, unsigned epoch)
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);  // this wavefront's stores have landed
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (blockIdx.x == 0) {
+        __syncthreads();
+        for (unsigned spins = 0;; ++spins) {
+            bool mine = true;
+            for (unsigned q = threadIdx.x + 1; q < nblocks; q += blockDim.x) mine &= ld_coh_u(&bar->flag[q]) == epoch;
+            if (__syncthreads_and(mine)) break;
+            if ((spins & 255u) == 255u) {
+                const bool bad = spins > kTailSpinLimit || __hip_atomic_load(&bar->err, __ATOMIC_RELAXED,
+                                                                             __HIP_MEMORY_SCOPE_AGENT) != 0;
+                if (__syncthreads_or(bad)) {
+                    if (threadIdx.x == 0) __hip_atomic_store(&bar->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    return false;
+                }
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (threadIdx.x < 8) st_coh_u(&bar->gen[threadIdx.x][0], epoch);
+        return true;
+    }
+    __shared__ int ok_s;
+    __syncthreads();  // every wavefront's stores have landed
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        st_coh_u(&bar->flag[blockIdx.x], epoch);
+        const unsigned *gp = &bar->gen[blockIdx.x & 7][0];
+        unsigned spins = 0;
+        while (ld_coh_u(gp) != epoch) {
+            if (++spins > kTailSpinLimit ||
+                __hip_atomic_load(&bar->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                __hip_atomic_store(&bar->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        ok_s = ok;
+    }
+    __syncthreads();
+    return ok_s != 0;
+}
+
+__global__ __launch_bounds__(64 * kTailWaves) void k_mg_tail(const MgTailPlan *__restrict__ plan, const KState *st,
+                                                             MgGridBar *bar)
+{
+    if (st->done) return;  // set before this launch: every block returns
+    if (__hip_atomic_load(&bar->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    const unsigned base = ld_coh_u(&bar->gen[blockIdx.x & 7][0]);  // the same in every copy between launches
+    const int lane = threadIdx.x & 63;
+    const long wave = (long)blockIdx.x * kTailWaves + (threadIdx.x >> 6);
+    const long nwaves = (long)gridDim.x * kTailWaves;
+    const int ns = plan->nsteps;
+    for (int q = 0; q < ns; ++q) {
+        if (q > 0 && !grid_barrier(bar, gridDim.x, base + (unsigned)q)) return;
+        const MgTailStep &t = plan->step[q];
+        const MgTailLevel &L = plan->lv[t.level];
+        if (t.kind == kTailSmooth) {
+            for (long c = wave; c < L.g.P; c += nwaves)
+                wave_smooth_col<OpA<float>, true, true>(L.op, L.g, L.b, t.y, L.lo, L.cp, L.ip, t.out, t.w, c, lane);
+        } else if (t.kind == kTailDown) {
+            const MgTailLevel &C = plan->lv[t.level + 1];
+            for (long c = wave; c < C.m.P; c += nwaves)
+                wave_down_col<OpA<float>, true, true>(L.op, L.g, C.m, t.y, L.b, C.lo, C.cp, C.ip, C.b, t.out, t.w, c,
+                                                      lane);
+        } else {
+            const MgTailLevel &C = plan->lv[t.level + 1];
+            for (long c = wave; c < L.g.P; c += nwaves)
+                wave_up_col<OpA<float>, true, true>(L.op, L.g, C.m, t.y, t.e, L.b, L.lo, L.cp, L.ip, t.out, t.w, c,
+                                                    lane);
+        }
+    }
 }
 
 // b_c = R r_f, one coarse row per thread: every interior row is the mean of

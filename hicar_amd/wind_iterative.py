@@ -221,11 +221,14 @@ class WindSolver:
         nat.check(self.lib.windhip_set_multigrid(self.h, int(pre), int(post), int(coarse_pre), int(coarse_post),
                                                  int(depth), float(omega)))
 
-    def set_mg_fusion(self, fused_transfers: bool = True, onepass: bool = True) -> None:
-        """MIXED_MG kernel forms: the one-launch descent/return between the
-        wavefront-per-column levels and the one-pass smoothing steps (both on by
-        default; off, the separate kernels run). No counterpart in the reference."""
-        nat.check(self.lib.windhip_set_mg_fusion(self.h, int(bool(fused_transfers)), int(bool(onepass))))
+    def set_mg_fusion(self, fused_transfers: int = 1, onepass: bool = True) -> None:
+        """MIXED_MG kernel forms. fused_transfers: 1 (or True, default), the
+        one-launch descent/return between the wavefront-per-column levels; 2, as
+        1 with the cycle from the first such level down as one persistent launch
+        (grid-wide barriers between its steps); 0 (or False), separate kernels.
+        onepass: the one-pass smoothing steps (on by default). No counterpart in
+        the reference."""
+        nat.check(self.lib.windhip_set_mg_fusion(self.h, int(fused_transfers), int(bool(onepass))))
 
     def mg_level(self, level: int):
         """Coarse level `level` of the last MIXED_MG solve: ((nx, ny, mz), coefficients [6, mz, ny+2, nx+2]
@@ -257,18 +260,19 @@ class WindSolver:
                                              out.ctypes.data_as(fpp)))
         return out
 
-    def mg_cycle(self, level: int, b, fused: bool = True):
+    def mg_cycle(self, level: int, b, fused: int = 1):
         """One V-cycle of the last MIXED_MG solve's hierarchy from coarse level
         `level` for the rhs b, FP32 [mz, ny+2, nx+2] (no counterpart in the
-        reference); fused selects the one-launch descent and return on the
-        levels of one wavefront per column."""
+        reference); fused 1 (or True) selects the one-launch descent and return
+        on the levels of one wavefront per column, 2 in addition the persistent
+        launch from the first such level down, 0 (or False) separate kernels."""
         dims = (C.c_int * 3)()
         nat.check(self.lib.windhip_mg_level(self.h, int(level), dims, None))
         shape = (dims[2], dims[1] + 2, dims[0] + 2)
         fpp = C.POINTER(C.c_float)
         bb = np.ascontiguousarray(b, dtype=np.float32).reshape(shape)
         out = np.zeros(shape, dtype=np.float32)
-        nat.check(self.lib.windhip_mg_cycle(self.h, int(level), int(bool(fused)), bb.ctypes.data_as(fpp),
+        nat.check(self.lib.windhip_mg_cycle(self.h, int(level), int(fused), bb.ctypes.data_as(fpp),
                                             out.ctypes.data_as(fpp)))
         return out
 

@@ -266,15 +266,19 @@ int windhip_set_guess_smoothing(windhip_solver *s, int steps);
  * (default 0.9). No counterpart in the reference. */
 int windhip_set_multigrid(windhip_solver *s, int pre, int post, int coarse_pre, int coarse_post, int depth,
                           double omega);
-/* MIXED_MG kernel forms (both on by default): fused_transfers != 0, the
- * one-launch descent (residual, restriction, first coarse step) and return
- * (prolongation, first post-step) between the levels of one wavefront per
- * column; onepass != 0, smoothing steps whose residual feeds the line solve in
+/* MIXED_MG kernel forms (default fused_transfers = 1, onepass = 1):
+ * fused_transfers 1, the one-launch descent (residual, restriction, first
+ * coarse step) and return (prolongation, first post-step) between the levels
+ * of one wavefront per column; 2, as 1, and the whole cycle from the first
+ * such level down is one persistent launch whose steps are separated by
+ * grid-wide barriers (bitwise the same values; slower at 512 x 512 x 40);
+ * 0, separate kernels.
+ * onepass != 0, smoothing steps whose residual feeds the line solve in
  * registers (one kernel instead of two) on the fine level and the larger
  * coarse levels of up to 44 rows per column, and the fine level's residual
  * and restriction in one kernel. Off, the separate kernels run; the one-pass
- * kernels compute the same values as their pairs. No
- * counterpart in the reference. */
+ * kernels compute the same values as their pairs. fused_transfers outside
+ * 0..2: WINDHIP_ERR_ARG. No counterpart in the reference. */
 int windhip_set_mg_fusion(windhip_solver *s, int fused_transfers, int onepass);
 /* Coarse level `level` (1 = first coarsening) of the MIXED_MG hierarchy built
  * by the last MIXED_MG solve: dims_out = (nx, ny, mz) of the level (interior
@@ -300,10 +304,11 @@ int windhip_mg_smooth(windhip_solver *s, int level, int fused, const float *b, c
  * level's N rows, layout of windhip_mg_level), from a zero guess; out
  * receives the level's correction. fused = 1: the levels of one wavefront per
  * column descend and return in one launch each (residual, restriction and
- * first step; prolongation and first post-step), as the solve runs them; 0:
- * separate residual, restriction, smoothing and prolongation kernels.
- * Overwrites the hierarchy's work vectors. Inspection only; no counterpart in
- * the reference. */
+ * first step; prolongation and first post-step), as the solve runs them by
+ * default; 2: as 1, with the cycle from the first such level down as one
+ * persistent launch; 0: separate residual, restriction, smoothing and
+ * prolongation kernels. Overwrites the hierarchy's work vectors. Inspection only; no
+ * counterpart in the reference. */
 int windhip_mg_cycle(windhip_solver *s, int level, int fused, const float *b, float *out);
 /* The network input for the solver's current rhs (that of its last solve),
  * metrics and alpha, written to in_dev (1 x dims[0] x dims[1] x dims[2] x 8);
@@ -321,7 +326,9 @@ int windhip_learned_input(windhip_solver *s, float *in_dev, int *dims_out, doubl
  * 16 FGMRES Gram-Schmidt step; MIXED_LINE inner iteration: 17 line solve,
  * 18 residual in - A y, 19 v = A p_hat + sigma, 20 t = A s_hat + 5 dots (its
  * s- and x/r/p-updates count as 7 and 9); MIXED_MG adds 21 fine-level
- * restriction and prolongation, 22 every kernel of the coarse levels, 23 the
+ * restriction and prolongation, 22 the coarse levels (one region per V-cycle
+ * from the fine level's restriction to its prolongation, every launch
+ * between them included; `calls` counts those regions), 23 the
  * coarse levels' set-up (its fine-level line solves and residuals count as 17
  * and 18), 24 fine-level one-pass smoothing steps (residual and line solve in
  * one kernel), 25 the fine level's residual and restriction in one kernel.

@@ -233,6 +233,56 @@ def test_fused_coarse_cycle_matches_the_separate_kernels():
     s.close()
 
 
+def test_persistent_coarse_cycle_equals_the_step_launches():
+    """The cycle from each coarse level down with the column-major levels as
+    one persistent launch (grid-wide barriers between its steps) against the
+    same cycle launched step by step: bitwise equal, since every column
+    computes the same values in the same order; twice in a row (the barrier
+    state returns to its start after each launch). Then MIXED_MG solves with
+    and without it: the same iterations and a bitwise equal lambda."""
+    from hicar_amd import _native as nat
+    from hicar_amd import synthetic as S
+    from hicar_amd.wind_iterative import WindSolver
+    _dev()
+    c = S.fractal_case(nx=200, ny=180, nz=12, dx=50.0, relief=600.0, seed=8, lowest=10.0, top=6000.0)
+    s = WindSolver(c.nx, c.ny, c.nz, c.dx)
+    s.set_metrics(c.metrics)
+    div = s.divergence(c.u, c.v, c.w, density=c.rho, adv_den=True, use_dqdt=True)
+    runs = []
+    for mode in (2, 1):
+        s.set_mg_fusion(mode, True)
+        s.set_lambda(np.zeros(s.rows_shape))
+        info = s.solve(c.alpha, div, precision=nat.MIXED_MG, tol_rel=1e-8, start=1, max_iters=2000, inner_tol=0.1)
+        assert info["status"] == 0 and info["true_rel_res"] <= 1e-8
+        runs.append((info, s.get_lambda()))
+    (a, la), (b, lb) = runs
+    print(f"persistent: {a['iters']} iterations, step launches: {b['iters']}")
+    assert a["iters"] == b["iters"] and a["outer_iters"] == b["outer_iters"]
+    assert np.array_equal(la, lb)
+    s.set_mg_fusion(1, True)
+    rng = np.random.default_rng(11)
+    level, seen = 1, 0
+    while True:
+        try:
+            (nx, ny, mz), _ = s.mg_level(level)
+        except Exception:
+            break
+        b = rng.standard_normal((mz, ny + 2, nx + 2)).astype(np.float32)
+        b[:, 0, :] = b[:, -1, :] = 0.0
+        b[:, :, 0] = b[:, :, -1] = 0.0
+        p1 = s.mg_cycle(level, b, fused=2)
+        p2 = s.mg_cycle(level, b, fused=2)
+        f = s.mg_cycle(level, b, fused=1)
+        print(f"level {level} ({nx}x{ny}): persistent vs step launches max |diff| {np.abs(p1 - f).max():.3e}")
+        assert np.all(np.isfinite(p1)) and np.abs(p1).max() > 0
+        assert np.array_equal(p1, f), level
+        assert np.array_equal(p1, p2), level
+        seen += (nx + 2) * (ny + 2) <= 4608
+        level += 1
+    assert seen >= 3 and level > 6
+    s.close()
+
+
 @pytest.mark.parametrize("name", ["fractal_alpha", "hill"])
 def test_onepass_smoothing_solve_equals_the_two_kernel_solve(name):
     """MIXED_MG with the one-pass smoothing steps (fine level and the larger
@@ -243,7 +293,7 @@ def test_onepass_smoothing_solve_equals_the_two_kernel_solve(name):
     c, o, div, s = _setup(name)
     runs = []
     for onepass in (True, False):
-        s.set_mg_fusion(True, onepass)
+        s.set_mg_fusion(1, onepass)
         s.set_lambda(np.zeros(s.rows_shape))
         s.profile(reset=True)
         info = s.solve(c.alpha, div, precision=nat.MIXED_MG, tol_rel=1e-8, start=1, max_iters=20000,
@@ -255,4 +305,4 @@ def test_onepass_smoothing_solve_equals_the_two_kernel_solve(name):
     assert pa.get("smooth_f32", (0, 0))[0] > 0 and pb.get("smooth_f32", (0, 0))[0] == 0
     assert a["iters"] == b["iters"] and a["outer_iters"] == b["outer_iters"]
     assert np.array_equal(la, lb)
-    s.set_mg_fusion(True, True)
+    s.set_mg_fusion(1, True)
