@@ -160,6 +160,14 @@ struct windhip_solver {
     long net_vox = 0;
     int net_smooth = 4;                        // line-relaxation steps applied to the network's guess
     double *lr_c = nullptr, *lr_d = nullptr;   // line relaxation: Thomas coefficients
+    // the learned guess's FP64 residual b - A guess and its squared norm, taken
+    // on the guess's stream for the mixed solves: the first FGMRES residual
+    // when the guess becomes the start
+    double *lg_res = nullptr, *lg_beta = nullptr;
+    double *zn_part = nullptr;                 // block partials of the norm of b (zero start)
+    // x holds +0.0 in every row (set_lambda of zeros, probe_finalize): a start
+    // from it needs no FP64 matvec
+    bool x_zero_known = false;
     float *lr_y32 = nullptr, *lr_o32 = nullptr, *lr_b32 = nullptr;  // FP32 relaxation of the mixed solves' guess
     // MIXED_LINE: FP32 Thomas factors of the column tridiagonal and scratch
     float *lf_lo = nullptr, *lf_cp = nullptr, *lf_ip = nullptr, *lf_dq = nullptr, *lf_r = nullptr;
@@ -378,10 +386,11 @@ struct windhip_solver {
     int true_residual(double *rel, const double *xv = nullptr);
     int net_dims(int *Dp, int *Hp, int *Wp) const;
     template <typename OT>
-    int net_input(OT *in, double *scale);
+    int net_input(OT *in, double *scale, bool x_zero = false);
     int line_relax(double **y, double **scratch, int steps, hipStream_t strm);
     int line_relax32(double *y, int steps, hipStream_t strm, bool mg_rows = false);
-    int learned_launch(double *res_plain, double **guess, bool f32_relax, bool mg_rows = false);
+    int learned_launch(double *res_plain, double **guess, bool f32_relax, bool mg_rows, bool x_zero,
+                       const windhip_solve_opts &o);
     int prepare_fine_line(bool multigrid);
     int learned_join(double *res_guess);
     bool net_pending = false;
@@ -1426,12 +1435,22 @@ int windhip_solver::net_dims(int *Dp, int *Hp, int *Wp) const
 // by 1 / rms(b) over the cells (the solution is linear in b); *scale returns
 // the factor that turns the network output back into lambda, rms(b) dx^2.
 template <typename OT>
-int windhip_solver::net_input(OT *in, double *scale)
+int windhip_solver::net_input(OT *in, double *scale, bool x_zero)
 {
     int Dp, Hp, Wp;
     net_dims(&Dp, &Hp, &Wp);
-    double rel;
-    CHK(true_residual(&rel, nullptr));  // also ||b||
+    if (x_zero) {  // the residual of a zero lambda is b: its norm only
+        if (!zn_part) CHK(alloc(&zn_part, 2L * kFlatBlocks + 1));  // 2 partials per block, then ||b||^2
+        const int fb = (int)std::min<long>(kFlatBlocks, (N + kThreads - 1) / kThreads);
+        CHK(timed(PK_TRUE_RESID, [&] {
+            k_fg_resid_zero<<<fb, kThreads, 0, stream>>>(N, st, zn_part, b, nullptr, zn_part + 2L * kFlatBlocks);
+        }));
+        HIPCHK(hipGetLastError());
+        CHK(read_state(0));
+    } else {
+        double rel;
+        CHK(true_residual(&rel, nullptr));  // also ||b||
+    }
     const double bn = st_host[0].outer_bnorm;
     const double rms = bn / sqrt((double)d.nx * d.ny * d.nz);
     const double bs = rms > 0 ? 1.0 / rms : 0.0;
@@ -1522,7 +1541,8 @@ int windhip_solver::line_relax32(double *y, int steps, hipStream_t strm, bool mg
 // their line factors and coarse levels meanwhile; learned_join makes the
 // solver's stream wait for it and reads the guess's residual. The guess lands in t or s (FP64 vectors
 // of the plain solve, unused by the mixed solves).
-int windhip_solver::learned_launch(double *res_plain, double **guess, bool f32_relax, bool mg_rows)
+int windhip_solver::learned_launch(double *res_plain, double **guess, bool f32_relax, bool mg_rows, bool x_zero,
+                                   const windhip_solve_opts &o)
 {
     int Dp, Hp, Wp;
     net_dims(&Dp, &Hp, &Wp);
@@ -1540,7 +1560,7 @@ int windhip_solver::learned_launch(double *res_plain, double **guess, bool f32_r
     // bf16 rows and factors: queued first (nothing when they are current)
     if (mg_rows) CHK(prepare_fine_line(true));
     double scale = 0.0;
-    CHK(net_input(net_in, &scale));
+    CHK(net_input(net_in, &scale, x_zero));
     *res_plain = st_host[0].outer_rnorm;  // of the current lambda, from net_input's residual
     HIPCHK(hipEventRecord(ev_net0, stream));
     HIPCHK(hipStreamWaitEvent(net_stream, ev_net0, 0));
@@ -1551,10 +1571,25 @@ int windhip_solver::learned_launch(double *res_plain, double **guess, bool f32_r
     double *gy = t, *gs = s;
     if (f32_relax) CHK(line_relax32(gy, net_smooth, net_stream, mg_rows));
     else CHK(line_relax(&gy, &gs, net_smooth, net_stream));
-    CHK(with_op<double>([&](auto op) {
-        k_true_resid<decltype(op)><<<chunk_blocks(g), kColThreads, 0, net_stream>>>(op, g, st + 2, partials2, gy, b);
-        return WINDHIP_OK;
-    }));
+    if (f32_relax) {
+        // the mixed solves: the guess's residual vector too, the first FGMRES
+        // residual if the guess becomes the start (k_fg_resid, as there)
+        if (!lg_res) {
+            CHK(alloc(&lg_res, N));
+            CHK(alloc(&lg_beta, 1));
+        }
+        k_state_reset<<<1, 1, 0, net_stream>>>(st + 2, 0, o.tol_rel, o.tol_abs, 0.0);
+        CHK(with_op<double>([&](auto op) {
+            k_fg_resid<decltype(op)><<<chunk_blocks(g), kColThreads, 0, net_stream>>>(op, g, st + 2, partials2, gy, b,
+                                                                                      lg_res, lg_beta);
+            return WINDHIP_OK;
+        }));
+    } else {
+        CHK(with_op<double>([&](auto op) {
+            k_true_resid<decltype(op)><<<chunk_blocks(g), kColThreads, 0, net_stream>>>(op, g, st + 2, partials2, gy, b);
+            return WINDHIP_OK;
+        }));
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ev_net1, net_stream));
     // per-kernel timing (opts.profile) brackets each launch with events on the
@@ -1675,7 +1710,14 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
         HIPCHK(hipGetLastError());
         return read_state(0);
     };
-    if (x_zero && info->start_used != WINDHIP_START_LEARNED) {
+    if (info->start_used == WINDHIP_START_LEARNED) {
+        // the guess's residual, taken beside the network on its stream by the
+        // same kernel and state set-up as resid() would use here
+        std::swap(fg_v[0], lg_res);
+        HIPCHK(hipMemcpyAsync(beta2, lg_beta, sizeof(double), hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpyAsync(st, st + 2, sizeof(KState), hipMemcpyDeviceToDevice, stream));
+        st_host[0] = st_host[2];
+    } else if (x_zero) {
         // from a zero lambda the first residual is b itself: no FP64 matvec
         CHK(timed(PK_IR_RESID, [&] { k_fg_resid_zero<<<fb, kThreads, 0, stream>>>(N, st, fg_part, b, fg_v[0], beta2); }));
         HIPCHK(hipGetLastError());
@@ -2024,6 +2066,7 @@ int windhip_solver::probe_record_field(const void *T, int ca, int cb, int cc, do
 
 int windhip_solver::probe(const void *alpha, const void *density, int adv_den, double *max_leak)
 {
+    x_zero_known = false;  // the colourings are written into lambda
     if (!metrics_set) return fail(WINDHIP_ERR_ARG, "windhip_probe_operator: metrics not set");
     CHK(gather3(alpha, 0, 1, d.nx, 1, d.nz, 1, d.ny, m_alpha, 0));
     CHK(probe_alloc());
@@ -2206,11 +2249,15 @@ int windhip_solve(windhip_solver *s, const void *alpha, const void *div, const w
     CHK(s->prepare_call(alpha, div));
     if (o.start == WINDHIP_START_ZERO) HIPCHK(hipMemsetAsync(s->x, 0, s->N * sizeof(double), s->stream));
     inf.start_used = o.start;
+    // a start from a lambda known to be zero (a zero start, or the current
+    // lambda when it was set to zeros) takes its first residual without a matvec
+    const bool x_zero = o.start == WINDHIP_START_ZERO || (o.start == WINDHIP_START_LEARNED && s->x_zero_known);
+    s->x_zero_known = false;  // the solve writes lambda
     const bool mixed = o.precision == WINDHIP_MIXED || o.precision == WINDHIP_MIXED_LINE ||
                        o.precision == WINDHIP_MIXED_MG;
     double *guess = nullptr, *mixed_guess = nullptr, rp = 0.0;
     if (o.start == WINDHIP_START_LEARNED) {
-        CHK(s->learned_launch(&rp, &guess, mixed, o.precision == WINDHIP_MIXED_MG && s->g.mz <= 44));
+        CHK(s->learned_launch(&rp, &guess, mixed, o.precision == WINDHIP_MIXED_MG && s->g.mz <= 44, x_zero, o));
         inf.start_used = WINDHIP_START_WARM;
         if (mixed) {
             mixed_guess = guess;  // the start, if better, once the mixed set-up is queued
@@ -2228,7 +2275,7 @@ int windhip_solve(windhip_solver *s, const void *alpha, const void *div, const w
         }
     }
     if (mixed) {
-        CHK(s->run_mixed(o, &inf, mixed_guess, rp, o.start == WINDHIP_START_ZERO));
+        CHK(s->run_mixed(o, &inf, mixed_guess, rp, x_zero));
         inf.nsweeps = s->nsweeps;
     } else {
         int status = 1, iters = 0;
@@ -2351,12 +2398,18 @@ int windhip_set_lambda(windhip_solver *s, const double *lam)
     if (!s || !lam) return fail(WINDHIP_ERR_ARG, "windhip_set_lambda: null argument");
     std::vector<double> tmp(s->N);
     const int mx = s->d.mx, my = s->d.my, mz = s->d.mz;
+    unsigned long long bits = 0;  // +0.0 everywhere iff no bit is set
     for (int j = 0; j < my; ++j)
         for (int k = 0; k < mz; ++k)
-            for (int i = 0; i < mx; ++i)
-                tmp[(long)k * s->d.P + (long)j * mx + i] = lam[i + (long)mx * (k + (long)mz * j)];
+            for (int i = 0; i < mx; ++i) {
+                const double v = lam[i + (long)mx * (k + (long)mz * j)];
+                tmp[(long)k * s->d.P + (long)j * mx + i] = v;
+                bits |= __builtin_bit_cast(unsigned long long, v);
+            }
+    s->x_zero_known = false;
     HIPCHK(hipMemcpyAsync(s->x, tmp.data(), s->N * sizeof(double), hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
+    s->x_zero_known = bits == 0;
     return WINDHIP_OK;
 }
 
@@ -2373,6 +2426,7 @@ int windhip_probe_lambda_pattern(windhip_solver *s, int ca, int cb, int cc)
     if (ca < 0 || ca > 2 || cb < 0 || cb > 2 || cc < 0 || cc > 2)
         return fail(WINDHIP_ERR_ARG, "windhip_probe_lambda_pattern: colours must be 0, 1 or 2");
     CHK(s->probe_alloc());
+    s->x_zero_known = false;
     k_probe_pattern<<<flat_grid(s->N), kBlock, 0, s->stream>>>(s->d, ca, cb, cc, s->x);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));

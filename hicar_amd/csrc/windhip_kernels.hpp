@@ -1924,7 +1924,7 @@ __global__ __launch_bounds__(NT) void k_smooth_r(Op op, Geo g, const KState *st,
 constexpr int kFlatBlocks = 1024;  // grid of the flat (row-parallel) reductions
 
 // The first FGMRES residual from a zero start: v0 = b (exactly b - A 0), its
-// norm and ||b|| as k_fg_resid states them.
+// norm and ||b|| as k_fg_resid states them (v0 null: the norms only).
 __global__ __launch_bounds__(kThreads) void k_fg_resid_zero(long N, KState *st, double *partials,
                                                             const double *__restrict__ b, double *__restrict__ v0,
                                                             double *h)
@@ -1932,7 +1932,7 @@ __global__ __launch_bounds__(kThreads) void k_fg_resid_zero(long N, KState *st, 
     double acc[2] = {0.0, 0.0};
     for (long n = (long)blockIdx.x * kThreads + threadIdx.x; n < N; n += (long)gridDim.x * kThreads) {
         const double bv = b[n];
-        v0[n] = bv;
+        if (v0) v0[n] = bv;  // null: the norms only
         acc[0] = acc[0] + bv * bv;
     }
     acc[1] = acc[0];
