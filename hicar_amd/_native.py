@@ -40,7 +40,7 @@ EXPORTS = (
     "windhip_balance_uvw", "windhip_calc_w_real", "windhip_idealized_wgrid",
     "windhip_balance_uvw_noalloc",
     "windhip_probe_lambda_pattern", "windhip_probe_record", "windhip_probe_finalize",
-    "windhip_mg_cycle", "windhip_set_mg_fusion",
+    "windhip_mg_cycle", "windhip_set_mg_fusion", "windhip_set_mg_galerkin", "windhip_mg_stencil",
 )
 
 PROFILE_KINDS = {0: "sweep_f64", 1: "sweep_sigma_f64", 2: "update_s_f64", 3: "sweep_omega_f64",
@@ -158,6 +158,8 @@ def load():
         "windhip_probe_finalize": (i, [vp]),
         "windhip_mg_cycle": (i, [vp, i, i, fp, fp]),
         "windhip_set_mg_fusion": (i, [vp, i, i]),
+        "windhip_set_mg_galerkin": (i, [vp, i]),
+        "windhip_mg_stencil": (i, [vp, i, fp]),
         "windhip_calc_w_real": (i, [vp, vp, vp, vp, vp]),
         "windhip_idealized_wgrid": (i, [vp, vp, vp, vp, vp]),
         "windhip_learned_input": (i, [vp, vp, C.POINTER(C.c_int), dp]),

@@ -198,6 +198,11 @@ struct windhip_solver {
         // c mz + k) and column-major copies of the FP32 rows and line factors
         bool cm;
         float *cB, *cC, *cH, *cI, *cL, *cM, *clo, *ccp, *cip;
+        // the level's operator is the Galerkin product of the finer level's
+        // (OpG: 27 column-major planar coefficient rows in G; column-major
+        // vectors, line factors clo, ccp, cip from its vertical part)
+        bool gal;
+        float *G;
         long S() const { return cm ? g.mz : 0; }  // MgIdx stride of the level's vectors
     };
     std::vector<MgLevel> mg;
@@ -213,6 +218,14 @@ struct windhip_solver {
     // their latency (512 x 512 x 40: 66 x 66 columns 4 us faster per step,
     // 130 x 130 columns 17 us slower).
     long mg_wave_cols = 4608;
+    // Coarse levels from mg[mg_gal_from] on take the Galerkin product of the
+    // finer level's operator instead of the rediscretised one (0: none). The
+    // first coarse level stays rediscretised: its 15-point bf16 rows keep the
+    // largest coarse level's smoothing cheap, while the rediscretised
+    // operators below it (averaged, smoothed terrain) make the cycle lose the
+    // smooth error modes (a V-cycle alone diverges on steep terrain).
+    int mg_gal_from = 1;
+    OpG level_opg(const MgLevel &L) const { return OpG{L.G, L.g.N}; }
     template <typename T>
     OpA<T> level_op(const MgLevel &L) const;
     OpA<float> level_op_cm(const MgLevel &L) const;  // the column-major FP32 rows of a wavefront level
@@ -962,7 +975,10 @@ void windhip_solver::mg_smooth(size_t l, int form, const float *y, float *out)
     if (form == 1 && gl.mz <= 64) {
         const int nb = (int)((L.g.P + kWaveCols - 1) / kWaveCols);
         timed(PK_MG_COARSE, [&] {
-            if (L.cm)  // the level's vectors are column-major
+            if (L.gal)  // the Galerkin operator (column-major)
+                k_mg_smooth_w<OpG, true><<<nb, 64 * kWaveCols, 0, stream>>>(
+                    level_opg(L), gl, sp, L.b, y, L.clo, L.ccp, L.cip, out, mg_omega);
+            else if (L.cm)  // the level's vectors are column-major
                 k_mg_smooth_w<OpA<float>, true><<<nb, 64 * kWaveCols, 0, stream>>>(
                     level_op_cm(L), gl, sp, L.b, y, L.clo, L.ccp, L.cip, out, mg_omega);
             else
@@ -1074,7 +1090,7 @@ int windhip_solver::prepare_tail(size_t l0)
     if (!mg_tail || !mg_fused || mg_cpost <= 0 || l0 >= mg.size() || mg.size() - l0 > (size_t)kTailMaxLevels)
         return WINDHIP_OK;
     for (size_t l = l0; l < mg.size(); ++l)
-        if (!mg[l].cm) return WINDHIP_OK;
+        if (!mg[l].cm || mg[l].gal) return WINDHIP_OK;  // the persistent steps read the analytic rows
     std::vector<MgTailStep> ps;
     float *res = plan_coarse(l0, false, l0, ps);
     if (ps.size() > (size_t)kTailMaxSteps) return WINDHIP_OK;
@@ -1146,7 +1162,7 @@ float *windhip_solver::mg_coarse(size_t l, bool entered)
     const Geo &gl = L.gg;
     const int cb = col_blocks(gl), kb = chunk_blocks(gl);
     const OpA<float> op = level_op<float>(L);
-    const bool wave = L.g.P <= mg_wave_cols;
+    const bool wave = L.g.P <= mg_wave_cols || L.gal;
     auto line = [&](const float *in, const float *y, float *out, float w) {
         timed(PK_MG_COARSE, [&] {
             if (gl.mz <= 48)
@@ -1162,7 +1178,10 @@ float *windhip_solver::mg_coarse(size_t l, bool entered)
     const int form = wave ? 1 : (L.Q ? (mg_onepass ? 3 : 2) : 0);
     auto resid = [&](const float *xv) {
         timed(PK_MG_COARSE, [&] {
-            if (L.cm)
+            if (L.gal)
+                k_mg_resid_w<OpG, true><<<(int)((L.g.P + kWaveCols - 1) / kWaveCols), 64 * kWaveCols, 0, stream>>>(
+                    level_opg(L), gl, sp, xv, L.b, L.r);
+            else if (L.cm)
                 k_mg_resid_w<OpA<float>, true><<<(int)((L.g.P + kWaveCols - 1) / kWaveCols), 64 * kWaveCols, 0,
                                                   stream>>>(level_op_cm(L), gl, sp, xv, L.b, L.r);
             else if (form >= 2)
@@ -1217,8 +1236,13 @@ float *windhip_solver::mg_coarse(size_t l, bool entered)
         const float wc = l + 2 == mg.size() ? 1.0f : mg_omega;  // the one-column level is solved exactly
         // both levels are column-major (L.cm, C.cm)
         timed(PK_MG_COARSE, [&] {
-            k_mg_down_w<OpA<float>, true><<<(int)((C.g.P + kWaveCols - 1) / kWaveCols), 64 * kWaveCols, 0, stream>>>(
-                level_op_cm(L), gl, C.g, sp, cur, L.b, C.clo, C.ccp, C.cip, C.b, C.x, wc);
+            const int nb = (int)((C.g.P + kWaveCols - 1) / kWaveCols);
+            if (L.gal)
+                k_mg_down_w<OpG, true><<<nb, 64 * kWaveCols, 0, stream>>>(level_opg(L), gl, C.g, sp, cur, L.b, C.clo,
+                                                                         C.ccp, C.cip, C.b, C.x, wc);
+            else
+                k_mg_down_w<OpA<float>, true><<<nb, 64 * kWaveCols, 0, stream>>>(
+                    level_op_cm(L), gl, C.g, sp, cur, L.b, C.clo, C.ccp, C.cip, C.b, C.x, wc);
         });
     } else if (mg_onepass && form == 3) {
         // a plane-major level with bf16 rows: residual and restriction in one pass
@@ -1236,8 +1260,13 @@ float *windhip_solver::mg_coarse(size_t l, bool entered)
     int post = mg_cpost;
     if (fuse_up) {
         timed(PK_MG_COARSE, [&] {  // both levels are column-major
-            k_mg_up_w<OpA<float>, true><<<(int)((L.g.P + kWaveCols - 1) / kWaveCols), 64 * kWaveCols, 0, stream>>>(
-                level_op_cm(L), gl, C.g, sp, cur, e, L.b, L.clo, L.ccp, L.cip, alt, mg_omega);
+            const int nb = (int)((L.g.P + kWaveCols - 1) / kWaveCols);
+            if (L.gal)
+                k_mg_up_w<OpG, true><<<nb, 64 * kWaveCols, 0, stream>>>(level_opg(L), gl, C.g, sp, cur, e, L.b, L.clo,
+                                                                       L.ccp, L.cip, alt, mg_omega);
+            else
+                k_mg_up_w<OpA<float>, true><<<nb, 64 * kWaveCols, 0, stream>>>(
+                    level_op_cm(L), gl, C.g, sp, cur, e, L.b, L.clo, L.ccp, L.cip, alt, mg_omega);
         });
         std::swap(cur, alt);
         --post;
@@ -1275,7 +1304,8 @@ int windhip_solver::build_mg()
             L.d.nx = L.g.nx; L.d.ny = L.g.ny; L.d.nz = L.g.mz - 2;
             L.d.mx = L.g.mx; L.d.my = L.g.my; L.d.mz = L.g.mz; L.d.P = L.g.P; L.d.dx = dx;
             const long n = L.g.N, pl = L.g.P;
-            L.cm = L.g.P <= mg_wave_cols && L.g.mz <= 64;
+            L.gal = mg_gal_from > 0 && (long)mg.size() >= mg_gal_from && L.g.mz <= 64;
+            L.cm = (L.g.P <= mg_wave_cols || L.gal) && L.g.mz <= 64;
             // stencil guards of the level's iterates: mx + 1 columns
             const long gd = (L.g.mx + 1) * (L.cm ? (long)L.g.mz : 1L);
             for (double **a : {&L.dzdx, &L.dzdy, &L.jaco, &L.jaco_w, &L.alpha, &L.adv_dz, &L.dz_if, &L.dzdx_u,
@@ -1289,6 +1319,7 @@ int windhip_solver::build_mg()
             CHK(alloc(&L.t, n, true, gd));
             if (L.cm)
                 for (float **a : {&L.cB, &L.cC, &L.cH, &L.cI, &L.cL, &L.cM, &L.clo, &L.ccp, &L.cip}) CHK(alloc(a, n));
+            if (L.gal) CHK(alloc(&L.G, 27 * n));
             mg.push_back(L);
             f = L.g;
         }
@@ -1296,6 +1327,19 @@ int windhip_solver::build_mg()
     if (mg.empty()) return WINDHIP_OK;
     for (size_t l = 0; l < mg.size(); ++l) {
         MgLevel &L = mg[l];
+        if (L.gal) {  // the Galerkin product of the finer level's operator, and its line factors
+            const MgLevel &F = mg[l - 1];
+            CHK(timed(PK_MG_SETUP, [&] {
+                if (F.gal)
+                    k_mg_rap<FineGal><<<mg_blocks(L.g.N), kMgThreads, 0, stream>>>(
+                        FineGal{level_opg(F), F.g.mx, F.g.my, F.g.mz}, F.g, L.g, L.G);
+                else
+                    k_mg_rap<FineRedisc><<<mg_blocks(L.g.N), kMgThreads, 0, stream>>>(
+                        FineRedisc{level_op<float>(F), F.g.P, F.g.mx, F.g.my, F.g.mz}, F.g, L.g, L.G);
+                k_mg_factor_g<<<mg_blocks(L.g.P), kMgThreads, 0, stream>>>(L.g, level_opg(L), L.clo, L.ccp, L.cip);
+            }));
+            continue;
+        }
         MgMetricsIn in{};
         if (l == 0) {
             in = MgMetricsIn{m_dzdx, m_dzdy, m_jaco, m_jaco_w, m_alpha, m_adv_dz, m_dzdx_u, m_jaco_u,
@@ -2596,6 +2640,40 @@ int windhip_set_mg_fusion(windhip_solver *s, int fused_transfers, int onepass)
     return WINDHIP_OK;
 }
 
+static void mg_rows_from_cm(const MgGeo &g, const float *src, float *dst);
+
+int windhip_set_mg_galerkin(windhip_solver *s, int first_level)
+{
+    if (!s) return fail(WINDHIP_ERR_ARG, "windhip_set_mg_galerkin: null handle");
+    if (first_level != 0 && first_level < 2)
+        return fail(WINDHIP_ERR_ARG, "windhip_set_mg_galerkin: first level %d (0: none, >= 2; level 1 stays "
+                                     "rediscretised)", first_level);
+    const int from = first_level == 0 ? 0 : first_level - 1;
+    if (!s->mg.empty() && from != s->mg_gal_from)
+        return fail(WINDHIP_ERR_ARG, "windhip_set_mg_galerkin: the hierarchy is built; set it before the first "
+                                     "MIXED_MG solve");
+    s->mg_gal_from = from;
+    return WINDHIP_OK;
+}
+
+int windhip_mg_stencil(windhip_solver *s, int level, float *q27_host)
+{
+    if (!s) return fail(WINDHIP_ERR_ARG, "windhip_mg_stencil: null handle");
+    if (level < 1 || level > (int)s->mg.size())
+        return fail(WINDHIP_ERR_ARG, "windhip_mg_stencil: level %d outside 1..%d (run a MIXED_MG solve first)", level,
+                    (int)s->mg.size());
+    const auto &L = s->mg[level - 1];
+    if (!L.gal) return fail(WINDHIP_ERR_ARG, "windhip_mg_stencil: level %d is rediscretised", level);
+    if (!q27_host) return WINDHIP_OK;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    std::vector<float> t((size_t)L.g.N);
+    for (int o = 0; o < 27; ++o) {
+        HIPCHK(hipMemcpy(t.data(), L.G + (long)o * L.g.N, sizeof(float) * L.g.N, hipMemcpyDeviceToHost));
+        mg_rows_from_cm(L.g, t.data(), q27_host + (long)o * L.g.N);
+    }
+    return WINDHIP_OK;
+}
+
 int windhip_mg_level(windhip_solver *s, int level, int *dims_out, float *coef_host)
 {
     if (!s || !dims_out) return fail(WINDHIP_ERR_ARG, "windhip_mg_level: null argument");
@@ -2605,6 +2683,9 @@ int windhip_mg_level(windhip_solver *s, int level, int *dims_out, float *coef_ho
     const auto &L = s->mg[level - 1];
     dims_out[0] = L.g.nx; dims_out[1] = L.g.ny; dims_out[2] = L.g.mz;
     if (!coef_host) return WINDHIP_OK;
+    if (L.gal)
+        return fail(WINDHIP_ERR_ARG, "windhip_mg_level: level %d holds a Galerkin operator (windhip_mg_stencil)",
+                    level);
     HIPCHK(hipStreamSynchronize(s->stream));
     const float *arr[6] = {L.B32, L.C32, L.H32, L.I32, L.L32, L.M32};
     for (int q = 0; q < 6; ++q)
@@ -2635,6 +2716,8 @@ int windhip_mg_smooth(windhip_solver *s, int level, int fused, const float *b, c
                     (int)s->mg.size());
     if (fused < 0 || fused > 4) return fail(WINDHIP_ERR_ARG, "windhip_mg_smooth: form %d outside 0..4", fused);
     auto &L = s->mg[level - 1];
+    if (L.gal && fused != 1)
+        return fail(WINDHIP_ERR_ARG, "windhip_mg_smooth: level %d holds a Galerkin operator: form 1 only", level);
     const size_t bytes = sizeof(float) * L.g.N;
     // the one-launch form works on the level's own layout (column-major on
     // the wavefront levels), the others on the plane-major rows

@@ -247,21 +247,198 @@ __device__ __forceinline__ void vst(float *p, long i, float v)
         p[i] = v;
 }
 
-// The residual row (b - A y)(row n) of a column held one row per lane, from
-// the lane's own five columns (k-1 and k+1 taken from the neighbouring lanes).
-template <class Op, bool COH = false>
-__device__ __forceinline__ float wave_resid(const Op &op, const float *__restrict__ y, float bv, long n, long c,
-                                            int k, int mz, bool lat, long dx, long dy)
+// A coarse level whose operator is the Galerkin product R A P of the next
+// finer level's operator with the cycle's transfers (R the mean over an
+// aggregate's columns, P the bilinear prolongation of k_mg_prolong): 27
+// coefficients per row, for the 3 x 3 columns around the row's column at
+// levels k-1, k, k+1. Stored column-major (row n = c mz + k, as the level's
+// vectors) and planar: coefficient o = 9 (dk + 1) + 3 (dj + 1) + (di + 1) of
+// row n at q[o N + n]. Lateral ring rows are identity rows; the ghost-plane
+// rows hold only the offsets they have (zero elsewhere).
+struct OpG {
+    const float *q;
+    long N;
+};
+__host__ __device__ constexpr int g_off(int di, int dk, int dj) { return 9 * (dk + 1) + 3 * (dj + 1) + (di + 1); }
+
+// (A v)(row n) of a column held one row per lane: val(di, dj) is the lane's
+// value at the column di, dj away (level k); the values at k-1 and k+1 come
+// from the neighbouring lanes. The analytic operator reads the five columns
+// C, W, E, S, N.
+template <class Op, class Val>
+__device__ __forceinline__ float wave_apply(const Op &op, int k, int mz, bool lat, long n, long c, Val val)
 {
     Win<float> x;
-    const float yc = vld<COH>(y, n), yw = vld<COH>(y, n - dx), ye = vld<COH>(y, n + dx), ys = vld<COH>(y, n - dy),
-                yn = vld<COH>(y, n + dy);
+    const float yc = val(0, 0), yw = val(-1, 0), ye = val(1, 0), ys = val(0, -1), yn = val(0, 1);
     x.c1 = yc; x.w1 = yw; x.e1 = ye; x.s1 = ys; x.n1 = yn;
     x.c0 = __shfl_up(yc, 1, 64); x.w0 = __shfl_up(yw, 1, 64); x.e0 = __shfl_up(ye, 1, 64);
     x.s0 = __shfl_up(ys, 1, 64); x.n0 = __shfl_up(yn, 1, 64);
     x.c2 = __shfl_down(yc, 1, 64); x.w2 = __shfl_down(yw, 1, 64); x.e2 = __shfl_down(ye, 1, 64);
     x.s2 = __shfl_down(ys, 1, 64); x.n2 = __shfl_down(yn, 1, 64);
-    return bv - op.row(k, mz, lat, x, op.load(n), op.bc(c));
+    return op.row(k, mz, lat, x, op.load(n), op.bc(c));
+}
+// The Galerkin operator reads the nine columns around the row's column; its
+// terms are summed in coefficient order.
+template <class Val>
+__device__ __forceinline__ float wave_apply(const OpG &op, int, int, bool lat, long n, long, Val val)
+{
+    float v[9], u[9], d[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) v[q] = val(q % 3 - 1, q / 3 - 1);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        u[q] = __shfl_up(v[q], 1, 64);
+        d[q] = __shfl_down(v[q], 1, 64);
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int o = 0; o < 27; ++o) {
+        const float xv = o < 9 ? u[o % 9] : (o < 18 ? v[o % 9] : d[o % 9]);
+        s = s + op.q[o * op.N + n] * xv;
+    }
+    return lat ? v[4] : s;
+}
+
+// The residual row (b - A y)(row n) of a column held one row per lane.
+template <class Op, bool COH = false>
+__device__ __forceinline__ float wave_resid(const Op &op, const float *__restrict__ y, float bv, long n, long c,
+                                            int k, int mz, bool lat, long dx, long dy)
+{
+    return bv - wave_apply(op, k, mz, lat, n, c, [&](int di, int dj) { return vld<COH>(y, n + di * dx + dj * dy); });
+}
+
+// Rows of a finer level as 27-point rows for the Galerkin product: a[o]
+// (g_off numbering) of row (i, k, j), zero where the row has no such entry.
+// A rediscretised level: its FP32 analytic operator (interior 15-point rows
+// J = -H, K = -I, N = -L, O = -M, the bottom BC row, the top row; the
+// wind_iterative.F90:912-1025 stencil), plane-major.
+struct FineRedisc {
+    OpA<float> op;
+    long P;
+    int mx, my, mz;
+    __device__ void row(int i, int k, int j, double (&a)[27]) const
+    {
+#pragma unroll
+        for (int o = 0; o < 27; ++o) a[o] = 0.0;
+        const long c = (long)j * mx + i;
+        if (i == 0 || i == mx - 1 || j == 0 || j == my - 1) { a[g_off(0, 0, 0)] = 1.0; return; }
+        const typename OpA<float>::Bc b = op.bc(c);
+        if (k == 0) {
+            a[g_off(0, 0, -1)] = b.by; a[g_off(0, 1, -1)] = b.by;
+            a[g_off(-1, 0, 0)] = b.bx; a[g_off(-1, 1, 0)] = b.bx;
+            a[g_off(0, 0, 0)] = -b.bz; a[g_off(0, 1, 0)] = b.bz;
+            a[g_off(1, 0, 0)] = -b.bx; a[g_off(1, 1, 0)] = -b.bx;
+            a[g_off(0, 0, 1)] = -b.by; a[g_off(0, 1, 1)] = -b.by;
+            return;
+        }
+        if (k == mz - 1) {
+            a[g_off(0, 0, 0)] = 1.0 / (double)b.dzt;
+            a[g_off(0, -1, 0)] = -1.0 / (double)b.dzt;
+            return;
+        }
+        const typename OpA<float>::Cf f = op.load((long)k * P + c);
+        const double inv = op.inv;
+        a[g_off(0, 0, 0)] = ((double)op.neg4 - f.B) - f.C;
+        a[g_off(0, 1, 0)] = f.B; a[g_off(0, -1, 0)] = f.C;
+        a[g_off(1, 0, 0)] = inv; a[g_off(-1, 0, 0)] = inv; a[g_off(0, 0, 1)] = inv; a[g_off(0, 0, -1)] = inv;
+        a[g_off(1, 1, 0)] = f.H; a[g_off(-1, 1, 0)] = f.I; a[g_off(1, -1, 0)] = -f.H; a[g_off(-1, -1, 0)] = -f.I;
+        a[g_off(0, 1, 1)] = f.L; a[g_off(0, 1, -1)] = f.M; a[g_off(0, -1, 1)] = -f.L; a[g_off(0, -1, -1)] = -f.M;
+    }
+};
+// A Galerkin level (column-major 27-point rows).
+struct FineGal {
+    OpG op;
+    int mx, my, mz;
+    __device__ void row(int i, int k, int j, double (&a)[27]) const
+    {
+        const long n = ((long)j * mx + i) * mz + k;
+#pragma unroll
+        for (int o = 0; o < 27; ++o) a[o] = op.q[o * op.N + n];
+    }
+};
+
+// The Galerkin product A_c = R A_f P, one coarse row per thread, column-major
+// output (OpG layout). R: the mean over the (up to) 2 x 2 fine columns 2I-1,
+// 2I x 2J-1, 2J of coarse column (I, J) at the same level; P: the bilinear
+// prolongation of k_mg_prolong (a fine column takes 9/16 of its own coarse
+// cell, 3/16 of the two nearer neighbours, 1/16 of the diagonal one; ring
+// columns, fine or coarse, carry nothing). Every index offset is a compile-time
+// constant after unrolling: fine column 2I-1+t (t = aggregate position +
+// fine offset, -1..2) has its own coarse cell at I + (t == -1 ? -1 : t == 2)
+// and its neighbouring cell at I + (t == -1 || t == 2 ? 0 : t == 0 ? -1 : 1).
+// The sums run in FP64; the coefficients are stored in FP32. Ring rows of the
+// coarse level are identity rows.
+template <class Fine>
+__global__ __launch_bounds__(kMgThreads) void k_mg_rap(Fine fine, MgGeo f, MgGeo c, float *__restrict__ q)
+{
+    const long r = (long)blockIdx.x * kMgThreads + threadIdx.x;
+    if (r >= c.N) return;
+    const int mz = c.mz;
+    const long cc = r / mz;
+    const int k = (int)(r - cc * mz);
+    const int I = (int)(cc % c.mx), J = (int)(cc / c.mx);
+    double acc[27];
+#pragma unroll
+    for (int o = 0; o < 27; ++o) acc[o] = 0.0;
+    if (I == 0 || I == c.mx - 1 || J == 0 || J == c.my - 1) {
+        acc[g_off(0, 0, 0)] = 1.0;
+    } else {
+        const int ni = 2 * I <= f.nx ? 2 : 1, nj = 2 * J <= f.ny ? 2 : 1;
+        const double rw = 1.0 / (double)(ni * nj);
+#pragma unroll
+        for (int aj = 0; aj < 2; ++aj)
+#pragma unroll
+            for (int ai = 0; ai < 2; ++ai) {
+                if (ai >= ni || aj >= nj) continue;
+                const int i = 2 * I - 1 + ai, j = 2 * J - 1 + aj;
+                double a[27];
+                fine.row(i, k, j, a);
+#pragma unroll
+                for (int o = 0; o < 27; ++o) {
+                    const int di = o % 3 - 1, dj = (o / 3) % 3 - 1, dk = o / 9 - 1;
+                    const double v = a[o];
+                    if (v == 0.0) continue;
+                    const int it = i + di, jt = j + dj;
+                    if (it <= 0 || it >= f.mx - 1 || jt <= 0 || jt >= f.my - 1) continue;  // fine ring: P row zero
+                    const int tx = ai + di, ty = aj + dj;  // -1..2
+                    const int ox = tx == -1 ? -1 : (tx == 2 ? 1 : 0), nxo = tx == -1 || tx == 2 ? 0 : (tx == 0 ? -1 : 1);
+                    const int oy = ty == -1 ? -1 : (ty == 2 ? 1 : 0), nyo = ty == -1 || ty == 2 ? 0 : (ty == 0 ? -1 : 1);
+                    const double rv = rw * v;
+#pragma unroll
+                    for (int pq = 0; pq < 4; ++pq) {
+                        const int cx = (pq & 1) ? nxo : ox, cy = (pq & 2) ? nyo : oy;
+                        const int It = I + cx, Jt = J + cy;
+                        if (It <= 0 || It >= c.mx - 1 || Jt <= 0 || Jt >= c.my - 1) continue;  // coarse ring: zero
+                        const double w = ((pq & 1) ? 0.25 : 0.75) * ((pq & 2) ? 0.25 : 0.75);
+                        acc[g_off(cx, dk, cy)] += rv * w;
+                    }
+                }
+            }
+    }
+#pragma unroll
+    for (int o = 0; o < 27; ++o) q[o * c.N + r] = (float)acc[o];
+}
+
+// Line factors of a Galerkin level (column-major): the Thomas factors of each
+// column's tridiagonal part (coefficients of k-1, k, k+1 in the own column),
+// in FP64 as k_line_factor forms them, stored in FP32: lo = lower coefficient,
+// ip = 1 / pivot, cp = upper / pivot.
+__global__ __launch_bounds__(kMgThreads) void k_mg_factor_g(MgGeo g, OpG op, float *__restrict__ lo,
+                                                            float *__restrict__ cp, float *__restrict__ ip)
+{
+    const long c = (long)blockIdx.x * kMgThreads + threadIdx.x;
+    if (c >= g.P) return;
+    double cprev = 0.0;
+    for (int k = 0; k < g.mz; ++k) {
+        const long n = c * g.mz + k;
+        const double l = op.q[g_off(0, -1, 0) * op.N + n], dd = op.q[g_off(0, 0, 0) * op.N + n],
+                     u = op.q[g_off(0, 1, 0) * op.N + n];
+        const double m = dd - l * cprev;
+        cprev = u / m;
+        lo[n] = (float)l;
+        ip[n] = (float)(1.0 / m);
+        cp[n] = (float)cprev;
+    }
 }
 
 // One damped line-smoothing step of column c of a level, lane k holding row
@@ -282,18 +459,10 @@ __device__ __forceinline__ void wave_smooth_col(const Op &op, const Geo &g, cons
     float r = vld<COH>(b, n);
     float yc = 0.f;
     if (y) {
-        Win<float> x;
         yc = vld<COH>(y, n);
-        const float yw = vld<COH>(y, n - dx), ye = vld<COH>(y, n + dx), ys = vld<COH>(y, n - dy),
-                    yn = vld<COH>(y, n + dy);
-        x.c1 = yc; x.w1 = yw; x.e1 = ye; x.s1 = ys; x.n1 = yn;
-        x.c0 = __shfl_up(yc, 1, 64); x.w0 = __shfl_up(yw, 1, 64); x.e0 = __shfl_up(ye, 1, 64);
-        x.s0 = __shfl_up(ys, 1, 64); x.n0 = __shfl_up(yn, 1, 64);
-        x.c2 = __shfl_down(yc, 1, 64); x.w2 = __shfl_down(yw, 1, 64); x.e2 = __shfl_down(ye, 1, 64);
-        x.s2 = __shfl_down(ys, 1, 64); x.n2 = __shfl_down(yn, 1, 64);
-        const typename Op::Cf f = op.load(n);
-        const typename Op::Bc bc = op.bc(c);
-        r = r - op.row(k, mz, lat, x, f, bc);
+        r = r - wave_apply(op, k, mz, lat, n, c, [&](int di, int dj) {
+                return di == 0 && dj == 0 ? yc : vld<COH>(y, n + di * dx + dj * dy);
+            });
     }
     const float s2 = wave_line_solve(lane, act, r, lo[n], cp[n], ip[n]);
     if (act) vst<COH>(out, n, yc + w * s2);
@@ -410,17 +579,13 @@ __device__ __forceinline__ void wave_up_col(const Op &op, const Geo &g, const Mg
                         0.0625f * vld<COH>(ec, ic.n(k, (long)Jn * c.mx + In));
         return yv + v;
     };
-    Win<float> x;
     // the ring's neighbours outside the level are never read by its rows
     const float zc = z(i, j);
-    const float zw = i > 0 ? z(i - 1, j) : zc, ze = i < g.mx - 1 ? z(i + 1, j) : zc;
-    const float zs = j > 0 ? z(i, j - 1) : zc, zn = j < g.my - 1 ? z(i, j + 1) : zc;
-    x.c1 = zc; x.w1 = zw; x.e1 = ze; x.s1 = zs; x.n1 = zn;
-    x.c0 = __shfl_up(zc, 1, 64); x.w0 = __shfl_up(zw, 1, 64); x.e0 = __shfl_up(ze, 1, 64);
-    x.s0 = __shfl_up(zs, 1, 64); x.n0 = __shfl_up(zn, 1, 64);
-    x.c2 = __shfl_down(zc, 1, 64); x.w2 = __shfl_down(zw, 1, 64); x.e2 = __shfl_down(ze, 1, 64);
-    x.s2 = __shfl_down(zs, 1, 64); x.n2 = __shfl_down(zn, 1, 64);
-    const float r = vld<COH>(b, n) - op.row(k, mz, lat, x, op.load(n), op.bc(col));
+    const float r = vld<COH>(b, n) - wave_apply(op, k, mz, lat, n, col, [&](int di, int dj) {
+                        const int ii = i + di, jj = j + dj;
+                        if (di == 0 && dj == 0) return zc;
+                        return ii >= 0 && ii < g.mx && jj >= 0 && jj < g.my ? z(ii, jj) : zc;
+                    });
     const float s2 = wave_line_solve(lane, act, r, lo[n], cp[n], ip[n]);
     if (act) vst<COH>(out, n, zc + w * s2);
 }

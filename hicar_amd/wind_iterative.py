@@ -230,15 +230,35 @@ class WindSolver:
         the reference."""
         nat.check(self.lib.windhip_set_mg_fusion(self.h, int(fused_transfers), int(bool(onepass))))
 
+    def set_mg_galerkin(self, first_level: int = 2) -> None:
+        """Coarse levels from `first_level` on take the Galerkin product R A P of the
+        finer level's operator (0: every level rediscretised); before the first
+        MIXED_MG solve. No counterpart in the reference."""
+        nat.check(self.lib.windhip_set_mg_galerkin(self.h, int(first_level)))
+
     def mg_level(self, level: int):
         """Coarse level `level` of the last MIXED_MG solve: ((nx, ny, mz), coefficients [6, mz, ny+2, nx+2]
-        = B, C, H, I, L, M of the rediscretised operator)."""
+        = B, C, H, I, L, M of the rediscretised operator; None on a Galerkin level)."""
         dims = (C.c_int * 3)()
         nat.check(self.lib.windhip_mg_level(self.h, int(level), dims, None))
         nx, ny, mz = dims[0], dims[1], dims[2]
+        if self.lib.windhip_mg_stencil(self.h, int(level), None) == 0:
+            return (nx, ny, mz), None
         out = np.zeros((6, mz, ny + 2, nx + 2), dtype=np.float32)
         nat.check(self.lib.windhip_mg_level(self.h, int(level), dims, out.ctypes.data_as(C.POINTER(C.c_float))))
         return (nx, ny, mz), out
+
+    def mg_stencil(self, level: int):
+        """The 27 coefficients [27, mz, ny+2, nx+2] of Galerkin level `level`
+        (coefficient 9 (dk+1) + 3 (dj+1) + (di+1)); None on a rediscretised level."""
+        dims = (C.c_int * 3)()
+        nat.check(self.lib.windhip_mg_level(self.h, int(level), dims, None))
+        if self.lib.windhip_mg_stencil(self.h, int(level), None) != 0:
+            return None
+        nx, ny, mz = dims[0], dims[1], dims[2]
+        out = np.zeros((27, mz, ny + 2, nx + 2), dtype=np.float32)
+        nat.check(self.lib.windhip_mg_stencil(self.h, int(level), out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
 
     def mg_smooth(self, level: int, fused, b, y=None):
         """One damped line-smoothing step on coarse level `level` of the last MIXED_MG

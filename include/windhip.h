@@ -280,13 +280,32 @@ int windhip_set_multigrid(windhip_solver *s, int pre, int post, int coarse_pre, 
  * kernels compute the same values as their pairs. fused_transfers outside
  * 0..2: WINDHIP_ERR_ARG. No counterpart in the reference. */
 int windhip_set_mg_fusion(windhip_solver *s, int fused_transfers, int onepass);
+/* Coarse operators of the MIXED_MG hierarchy: levels from `first_level` on
+ * (numbered as in windhip_mg_level) take the Galerkin product R A P of the
+ * next finer level's operator with the cycle's transfers (R the mean over a
+ * coarse column's 2 x 2 fine columns, P the bilinear prolongation): 27
+ * coefficients per row, smoothed a wavefront per column. Levels before it
+ * rediscretise the analytic operator at their dx from averaged metrics.
+ * first_level 0: every level rediscretised; otherwise >= 2 (default 2: the
+ * first coarse level keeps its cheap 15-point rows). Set before the first
+ * MIXED_MG solve (a different value afterwards: WINDHIP_ERR_ARG). No
+ * counterpart in the reference. */
+int windhip_set_mg_galerkin(windhip_solver *s, int first_level);
 /* Coarse level `level` (1 = first coarsening) of the MIXED_MG hierarchy built
  * by the last MIXED_MG solve: dims_out = (nx, ny, mz) of the level (interior
  * columns, rows per column); coef_host (may be null) receives its 6 x N FP32
  * coefficients B, C, H, I, L, M (the analytic operator rediscretised at the
  * level's dx from its averaged metrics), rows [k][j][i] of (nx+2) x (ny+2)
- * columns. Inspection only; no counterpart in the reference. */
+ * columns; on a Galerkin level coef_host must be null (WINDHIP_ERR_ARG
+ * otherwise). Inspection only; no counterpart in the reference. */
 int windhip_mg_level(windhip_solver *s, int level, int *dims_out, float *coef_host);
+/* The 27 x N FP32 coefficients of Galerkin level `level` (numbered as in
+ * windhip_mg_level): coefficient o = 9 (dk + 1) + 3 (dj + 1) + (di + 1), the
+ * entry of row (i, k, j) for column (i + di, j + dj) at level k + dk, at
+ * q27_host[o N + n], rows n in the layout of windhip_mg_level. q27_host null:
+ * returns WINDHIP_OK if the level is a Galerkin level. A rediscretised level:
+ * WINDHIP_ERR_ARG. Inspection only; no counterpart in the reference. */
+int windhip_mg_stencil(windhip_solver *s, int level, float *q27_host);
 /* One damped line-smoothing step of the V-cycle on coarse level `level`, out =
  * y + omega L^-1 (b - A y) (y null: out = omega L^-1 b), L the vertical
  * tridiagonal part of the level's operator; host FP32 arrays of the level's N
@@ -297,7 +316,8 @@ int windhip_mg_level(windhip_solver *s, int level, int *dims_out, float *coef_ho
  * levels: as 0); 3 and 4: the one-pass step (residual and line solve in one
  * kernel) on the bf16 (3, the form the V-cycle uses on the larger levels) or
  * FP32 (4) rows and factors, for y given and up to 44 rows per column
- * (otherwise as 2 and 0). Overwrites the level's work vectors. Inspection only; no counterpart in the reference. */
+ * (otherwise as 2 and 0). A Galerkin level takes form 1 only (WINDHIP_ERR_ARG
+ * otherwise). Overwrites the level's work vectors. Inspection only; no counterpart in the reference. */
 int windhip_mg_smooth(windhip_solver *s, int level, int fused, const float *b, const float *y, float *out);
 /* One V-cycle of the MIXED_MG hierarchy from coarse level `level` (as the
  * inner solve applies it below the fine level) for the rhs b (host FP32, the

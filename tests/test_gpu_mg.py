@@ -59,6 +59,7 @@ def test_coarse_levels_rediscretise_the_operator():
     the metrics and alpha averaged over 2 x 2 columns (odd edge: ny = 23)."""
     from hicar_amd import _native as nat
     c, o, div, s = _setup()
+    s.set_mg_galerkin(0)  # every coarse level rediscretised
     info = s.solve(c.alpha, div, precision=nat.MIXED_MG, tol_rel=1e-6, start=1, max_iters=2000)
     assert info["status"] == 0
     keys = mo.CELL + mo.U_FACE + mo.V_FACE + ("terrain",)
@@ -109,6 +110,7 @@ def test_fused_coarse_smoothing_matches_the_two_kernel_step(name):
     within bf16 rounding (2^-9 per coefficient) of the FP32 step."""
     from hicar_amd import _native as nat
     c, o, div, s = _setup(name)
+    s.set_mg_galerkin(0)  # the kernel forms of the rediscretised levels
     s.solve(c.alpha, div, precision=nat.MIXED_MG, tol_rel=1e-6, start=1, max_iters=2000)
     rng = np.random.default_rng(21)
     level = 1
@@ -246,6 +248,7 @@ def test_persistent_coarse_cycle_equals_the_step_launches():
     _dev()
     c = S.fractal_case(nx=200, ny=180, nz=12, dx=50.0, relief=600.0, seed=8, lowest=10.0, top=6000.0)
     s = WindSolver(c.nx, c.ny, c.nz, c.dx)
+    s.set_mg_galerkin(0)  # the persistent launch runs the rediscretised levels' steps
     s.set_metrics(c.metrics)
     div = s.divergence(c.u, c.v, c.w, density=c.rho, adv_den=True, use_dqdt=True)
     runs = []
@@ -306,3 +309,178 @@ def test_onepass_smoothing_solve_equals_the_two_kernel_solve(name):
     assert a["iters"] == b["iters"] and a["outer_iters"] == b["outer_iters"]
     assert np.array_equal(la, lb)
     s.set_mg_fusion(1, True)
+
+
+# ----------------------------------------------------------- Galerkin levels
+def _transfers(nxf, nyf, nxc, nyc, mz):
+    """R (mean over a coarse column's 2 x 2 fine columns) and P (cell-centred
+    bilinear, 9/16 3/16 3/16 1/16; ring columns carry nothing) as sparse
+    matrices on rows [k][j][i], written from the definitions."""
+    import scipy.sparse as sp
+    mxf, myf, mxc, myc = nxf + 2, nyf + 2, nxc + 2, nyc + 2
+    fidx = lambda i, k, j: (k * myf + j) * mxf + i  # noqa: E731
+    cidx = lambda i, k, j: (k * myc + j) * mxc + i  # noqa: E731
+    rr, rc, rv, pr, pc, pv = [], [], [], [], [], []
+    for J in range(1, nyc + 1):
+        for I in range(1, nxc + 1):
+            fi = [i for i in (2 * I - 1, 2 * I) if i <= nxf]
+            fj = [j for j in (2 * J - 1, 2 * J) if j <= nyf]
+            for k in range(mz):
+                for j in fj:
+                    for i in fi:
+                        rr.append(cidx(I, k, J)); rc.append(fidx(i, k, j)); rv.append(1.0 / (len(fi) * len(fj)))
+    for j in range(1, nyf + 1):
+        for i in range(1, nxf + 1):
+            I, J = (i + 1) // 2, (j + 1) // 2
+            In, Jn = (I - 1 if i % 2 else I + 1), (J - 1 if j % 2 else J + 1)
+            for k in range(mz):
+                for (a, b), w in (((I, J), 0.5625), ((In, J), 0.1875), ((I, Jn), 0.1875), ((In, Jn), 0.0625)):
+                    if 1 <= a <= nxc and 1 <= b <= nyc:
+                        pr.append(fidx(i, k, j)); pc.append(cidx(a, k, b)); pv.append(w)
+    R = sp.csr_matrix((rv, (rr, rc)), shape=(mxc * myc * mz, mxf * myf * mz))
+    P = sp.csr_matrix((pv, (pr, pc)), shape=(mxf * myf * mz, mxc * myc * mz))
+    return R, P
+
+
+def _stencil_to_matrix(q, nx, ny, mz):
+    """The sparse matrix of 27 coefficient arrays [27, mz, ny+2, nx+2] (rows [k][j][i])."""
+    import scipy.sparse as sp
+    mx, my = nx + 2, ny + 2
+    k, j, i = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
+    rows, cols, vals = [], [], []
+    for o in range(27):
+        di, dj, dk = o % 3 - 1, (o // 3) % 3 - 1, o // 9 - 1
+        v = q[o]
+        ok = (v != 0) & (i + di >= 0) & (i + di < mx) & (j + dj >= 0) & (j + dj < my) & (k + dk >= 0) & (k + dk < mz)
+        rows.append(((k * my + j) * mx + i)[ok])
+        cols.append((((k + dk) * my + (j + dj)) * mx + (i + di))[ok])
+        vals.append(v[ok].astype(np.float64))
+    n = mx * my * mz
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+
+
+def _galerkin(Af, nxf, nyf, nxc, nyc, mz):
+    """R A P with the coarse ring rows identity and the ring columns dropped."""
+    import scipy.sparse as sp
+    R, P = _transfers(nxf, nyf, nxc, nyc, mz)
+    A = (R @ Af @ P).tocsr()
+    mx, my = nxc + 2, nyc + 2
+    k, j, i = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
+    ring = ((i == 0) | (i == mx - 1) | (j == 0) | (j == my - 1)).ravel()
+    D = sp.diags((~ring).astype(float))
+    return (D @ A @ D + sp.diags(ring.astype(float))).tocsr()
+
+
+def _to_kji(A_oracle, nx, ny, mz):
+    """A matrix on the oracle's rows i + mx (k + mz j) moved to rows [k][j][i]."""
+    mx, my = nx + 2, ny + 2
+    k, j, i = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
+    perm = (i + mx * (k + mz * j)).ravel()  # [k][j][i] row -> oracle row
+    return A_oracle[perm][:, perm].tocsr()
+
+
+def _rowwise_close(got, want, rtol):
+    """|got - want| <= rtol * max |want| of the row, entry by entry."""
+    d = (got - want).tocsr()
+    scale = abs(want).max(axis=1).toarray().ravel()
+    err = abs(d).max(axis=1).toarray().ravel()
+    return float(np.max(err / np.maximum(scale, 1e-300)))
+
+
+def test_galerkin_levels_are_the_product_R_A_P():
+    """From the second coarse level on, each level's operator is R A P of the
+    finer level's: level 2 against the product of the oracle's level-1
+    operator (assembled at 2 dx from the averaged metrics: the rediscretised
+    first level), level 3 against the product of level 2's own coefficients,
+    to the FP32 rounding of the stored coefficients; 27 offsets per row,
+    identity rows on the ring. The transfers are written here from their
+    definitions (sparse matrices), independently of the kernels."""
+    from hicar_amd import _native as nat
+    from oracle import Oracle
+    from test_oracle import stencil_matrix
+    c, o, div, s = _setup()
+    info = s.solve(c.alpha, div, precision=nat.MIXED_MG, tol_rel=1e-6, start=1, max_iters=2000)
+    assert info["status"] == 0
+    assert s.mg_stencil(1) is None and s.mg_stencil(2) is not None
+    keys = mo.CELL + mo.U_FACE + mo.V_FACE + ("terrain",)
+    m1, a1 = mo.coarsen_metrics({k: c.metrics[k] for k in keys}, c.alpha)
+    (nx1, ny1, mz), _ = s.mg_level(1)
+    o1 = Oracle(nx1, ny1, c.nz, 2 * c.dx)
+    o1.set_metrics(dict(m1, terrain=m1["terrain"]))
+    o1.set_alpha(a1)
+    o1.assemble()
+
+    class _Case:
+        pass
+    cc = _Case(); cc.alpha = a1; cc.metrics = m1
+    A1 = _to_kji(stencil_matrix(o1, cc), nx1, ny1, mz)
+    o1.close()
+    (nx2, ny2, _), _ = s.mg_level(2)
+    q2 = s.mg_stencil(2)
+    assert q2.shape == (27, mz, ny2 + 2, nx2 + 2)
+    A2 = _stencil_to_matrix(q2, nx2, ny2, mz)
+    err2 = _rowwise_close(A2, _galerkin(A1, nx1, ny1, nx2, ny2, mz), 0)
+    print(f"level 2 vs R A1 P: {err2:.2e}")
+    assert err2 <= 2e-5
+    (nx3, ny3, _), _ = s.mg_level(3)
+    A3 = _stencil_to_matrix(s.mg_stencil(3), nx3, ny3, mz)
+    err3 = _rowwise_close(A3, _galerkin(A2, nx2, ny2, nx3, ny3, mz), 0)
+    print(f"level 3 vs R A2 P: {err3:.2e}")
+    assert err3 <= 2e-5
+    # interior rows below the ghost planes carry all 27 offsets
+    assert np.count_nonzero(q2[:, mz // 2, 2, 2]) == 27
+
+
+def test_galerkin_smoothing_step():
+    """The wavefront smoothing step on a Galerkin level, out = y + 0.9 M^-1
+    (b - A y) with A the level's 27-point operator and M its vertical
+    tridiagonal part, against the same step by sparse solves in FP64."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    from hicar_amd import _native as nat
+    c, o, div, s = _setup("fractal")
+    s.solve(c.alpha, div, precision=nat.MIXED_MG, tol_rel=1e-6, start=1, max_iters=2000)
+    rng = np.random.default_rng(4)
+    for level in (2, 3):
+        (nx, ny, mz), _ = s.mg_level(level)
+        q = s.mg_stencil(level)
+        A = _stencil_to_matrix(q, nx, ny, mz)
+        Ac = A.tocoo()
+        vert = np.abs(Ac.row - Ac.col) % ((nx + 2) * (ny + 2)) == 0
+        M = sp.csr_matrix((Ac.data[vert], (Ac.row[vert], Ac.col[vert])), shape=A.shape)
+        shape = (mz, ny + 2, nx + 2)
+        b = rng.standard_normal(shape).astype(np.float32)
+        y = rng.standard_normal(shape).astype(np.float32)
+        got = s.mg_smooth(level, True, b, y)
+        r = b.ravel().astype(np.float64) - A @ y.ravel().astype(np.float64)
+        want = y.ravel() + 0.9 * spla.spsolve(M.tocsc(), r)
+        err = _rel(got.ravel(), want)
+        print(f"level {level} ({nx}x{ny}): smoothing step vs FP64 sparse {err:.2e}")
+        assert err <= 1e-4, (level, err)
+        with pytest.raises(RuntimeError):
+            s.mg_smooth(level, False, b, y)  # the 15-point kernel forms do not apply
+
+
+def test_galerkin_levels_take_fewer_iterations():
+    """Steep fractal terrain (1500 m over 100 x 90 columns at 50 m): the
+    hierarchy with Galerkin operators from its second coarse level reaches the
+    same tolerance in fewer inner iterations than the rediscretised one, and
+    both reach the answer (the same lambda to the solve's accuracy)."""
+    from hicar_amd import _native as nat
+    from hicar_amd import synthetic as S
+    from hicar_amd.wind_iterative import WindSolver
+    _dev()
+    c = S.fractal_case(nx=100, ny=90, nz=16, dx=50.0, relief=1500.0, seed=5, lowest=10.0, top=8000.0)
+    res = {}
+    for gal in (2, 0):
+        s = WindSolver(c.nx, c.ny, c.nz, c.dx)
+        s.set_mg_galerkin(gal)
+        s.set_metrics(c.metrics)
+        div = s.divergence(c.u, c.v, c.w, density=c.rho, adv_den=True, use_dqdt=True)
+        info = s.solve(c.alpha, div, precision=nat.MIXED_MG, tol_rel=1e-8, start=1, max_iters=5000, inner_tol=0.1)
+        assert info["status"] == 0 and info["true_rel_res"] <= 1e-8
+        res[gal] = (info["iters"], s.get_lambda())
+        s.close()
+    print(f"inner iterations: Galerkin {res[2][0]}, rediscretised {res[0][0]}")
+    assert res[2][0] < res[0][0]
+    assert _rel(res[2][1], res[0][1]) <= 1e-6
