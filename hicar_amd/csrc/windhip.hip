@@ -207,7 +207,7 @@ struct windhip_solver {
     };
     std::vector<MgLevel> mg;
     int mg_pre = 2, mg_post = 2;    // fine-level line-smoothing steps before and after the coarse correction
-    int mg_cpre = 2, mg_cpost = 2;  // the same on the coarse levels
+    int mg_cpre = 1, mg_cpost = 1;  // the same on the coarse levels
     int mg_depth = 0;               // coarse levels used (0: down to the last level with >= 4 columns each way)
     float mg_omega = 0.9f;          // damping of the line smoother
     // Coarse levels of at most this many columns (64 x 64 interior columns and
@@ -419,6 +419,19 @@ struct windhip_solver {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
     std::vector<int> ev_kind;
     int ev_next = 0;
+    // Krylov launches after the solve has finished return at once: while
+    // prof_gate is set, each timed launch first records the state's
+    // done/fmode (k_prof_gate into prof_flag[event]) and the harvest leaves
+    // such no-op launches out of the per-kind times and counts
+    const KState *prof_gate = nullptr;
+    int *prof_flag = nullptr;
+    static constexpr int kProfFlags = 1 << 16;
+    std::vector<char> ev_gated;
+    struct GateScope {
+        const KState *&g;
+        GateScope(const KState *&g_, const KState *v) : g(g_) { g = v; }
+        ~GateScope() { g = nullptr; }
+    };
     double prof_ms[WINDHIP_PROFILE_KINDS] = {}, prof_calls[WINDHIP_PROFILE_KINDS] = {};
     int launch_batch(int precision, int count, bool use_graph);
     int enqueue_iterations_line(int count, bool multigrid);
@@ -766,6 +779,9 @@ int windhip_solver::timed(int kind, Launch &&launch)
         ev_pool.push_back({e0, e1});
     }
     auto &pr = ev_pool[ev_next];
+    const bool gated = prof_gate && prof_flag && ev_next < kProfFlags;
+    if (gated) k_prof_gate<<<1, 64, 0, stream>>>(prof_gate, prof_flag + ev_next);
+    ev_gated.push_back(gated);
     HIPCHK(hipEventRecord(pr.first, stream));
     prof_nested = true;
     launch();
@@ -779,14 +795,28 @@ int windhip_solver::timed(int kind, Launch &&launch)
 // Accumulate the timed launches after the stream has been synchronized.
 int windhip_solver::harvest()
 {
+    std::vector<int> flag;
+    if (prof_flag && ev_next > 0) {
+        flag.resize(std::min(ev_next, kProfFlags));
+        HIPCHK(hipMemcpy(flag.data(), prof_flag, flag.size() * sizeof(int), hipMemcpyDeviceToHost));
+    }
     for (int q = 0; q < ev_next; ++q) {
+        const int kind = ev_kind[q];
+        if (ev_gated[q]) {
+            // a no-op launch: the solve was done before it (bit 0), or, for the
+            // x/r/p update, no update mode was set (bit 1 clear; a finished
+            // solve's closing half-step still updates x)
+            const bool xrp = kind == PK_UPD_XRP || kind == 5 + PK_UPD_XRP;
+            if (xrp ? !(flag[q] & 2) : (flag[q] & 1)) continue;
+        }
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, ev_pool[q].first, ev_pool[q].second));
-        prof_ms[ev_kind[q]] += ms;
-        prof_calls[ev_kind[q]] += 1;
+        prof_ms[kind] += ms;
+        prof_calls[kind] += 1;
     }
     ev_next = 0;
     ev_kind.clear();
+    ev_gated.clear();
     return WINDHIP_OK;
 }
 
@@ -803,6 +833,7 @@ int windhip_solver::enqueue_iterations(int count)
     kvec<T>(&X, &R, &RH, &Pv, &V, &S, &Tv, PH, SH, &DI);
     const int fg = flat_grid(N);
     const int base = 5 * prec;
+    GateScope gate(prof_gate, sp);
     return with_op<T>([&](auto op) -> int {
         using Op = decltype(op);
         for (int it = 0; it < count; ++it) {
@@ -853,6 +884,7 @@ int windhip_solver::enqueue_iterations_line(int count, bool multigrid)
     kvec<float>(&X, &R, &RH, &Pv, &V, &S, &Tv, PH, SH, &DI);
     const int fg = flat_grid(N);
     const int cb = col_blocks(g), kb = chunk_blocks(g);
+    GateScope gate(prof_gate, sp);
     auto body = [&](auto op) -> int {
         using Op = decltype(op);
         // M^-1 in: y1 = L^-1 in, r = in - A y1, out = y1 + L^-1 r
@@ -1773,12 +1805,42 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
     info->bnorm = st_host[0].outer_bnorm;
     int total_inner = 0, steps = 0, status = 0;
     std::vector<double> Hm((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), y(m), col(H);
+    // With the multigrid preconditioner an inner solve usually ends within its
+    // first batch of iterations. Its direction's FGMRES step (below) is then
+    // enqueued right behind that batch, and one synchronisation reads both the
+    // inner state and the Hessenberg column. When the inner solve had not
+    // ended, the step's results are discarded and the step runs again once it
+    // has (the same kernels on the same z_j, so the same values). Ahead only
+    // while the previous direction's inner solve ended within one batch.
+    bool ahead = multigrid;
     while (!st_host[0].outer_done && steps < o.max_outer && total_inner < o.max_iters) {
         std::fill(gv.begin(), gv.end(), 0.0);
         gv[0] = st_host[0].outer_rnorm;
         int k = 0;
         bool breakdown = false;
         for (int j = 0; j < m && steps < o.max_outer && total_inner < o.max_iters; ++j) {
+            double *hc = fg_h + (long)j * H;
+            // w = A z_j, orthogonalised against v_0..v_j (modified Gram-Schmidt);
+            // the Hessenberg column j and both states to the host
+            auto enqueue_step = [&]() -> int {
+                CHK(with_op<double>([&](auto op) {
+                    return timed(PK_FG_MATVEC, [&] {
+                        k_fg_matvec<decltype(op)><<<chunk_blocks(g), kColThreads, 0, stream>>>(
+                            op, g, st, partials, x32, fg_z[j], fg_v[0], fg_v[j + 1], hc);
+                    });
+                }));
+                for (int i = 0; i <= j; ++i)
+                    CHK(timed(PK_FG_AXPY_DOT, [&] {
+                        k_fg_axpy_dot<<<fb, kThreads, 0, stream>>>(N, st, fg_part, fg_v[j + 1], fg_v[i],
+                                                                   i < j ? fg_v[i + 1] : nullptr, hc, i);
+                    }));
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipMemcpyAsync(col.data(), hc, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, stream));
+                HIPCHK(hipMemcpyAsync(st_host, st, 2 * sizeof(KState), hipMemcpyDeviceToHost, stream));
+                HIPCHK(hipStreamSynchronize(stream));
+                return harvest();
+            };
+            bool stepped = false;  // col holds this direction's step
             {
             // z_j ~ A^-1 v_j in FP32 (v_j normalised in place first). An inner
             // solve whose FP32 recurrence overflowed is run again from z = 0 with
@@ -1802,8 +1864,15 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
                                                             x32, r32, rh32, p32, ph32[0]);
                 }));
                 HIPCHK(hipGetLastError());
-                CHK(read_state(1));
                 int done_iters = 0;
+                if (ahead && attempt == 0) {
+                    done_iters = std::min(o.iters_per_sync, std::max(1, cap));
+                    CHK(launch_batch(3, done_iters, o.use_graph != 0));
+                    CHK(enqueue_step());
+                    stepped = st_host[1].done != 0;
+                } else {
+                    CHK(read_state(1));
+                }
                 while (!st_host[1].done) {
                     const int n = std::min(o.iters_per_sync, std::max(1, cap - done_iters));
                     CHK(launch_batch(multigrid ? 3 : (line ? 2 : 1), n, o.use_graph != 0));
@@ -1811,9 +1880,11 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
                     done_iters += n;
                     if (done_iters > cap + o.iters_per_sync) break;
                 }
+                if (multigrid) ahead = done_iters <= o.iters_per_sync;
                 spent += st_host[1].iter;
                 const double rn = st_host[1].rnorm;
                 if (rn == rn && rn < 1e300) break;
+                stepped = false;
                 if (verbose)
                     fprintf(stderr, "[windhip] FP32 inner solve overflowed after %d iterations; rerun with %d\n",
                             st_host[1].iter, st_host[1].iter / 2);
@@ -1824,23 +1895,7 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
             total_inner += spent;
             ++steps;
             }
-            double *hc = fg_h + (long)j * H;
-            // w = A z_j, orthogonalised against v_0..v_j (modified Gram-Schmidt)
-            CHK(with_op<double>([&](auto op) {
-                return timed(PK_FG_MATVEC, [&] {
-                    k_fg_matvec<decltype(op)><<<chunk_blocks(g), kColThreads, 0, stream>>>(
-                        op, g, st, partials, x32, fg_z[j], fg_v[0], fg_v[j + 1], hc);
-                });
-            }));
-            for (int i = 0; i <= j; ++i)
-                CHK(timed(PK_FG_AXPY_DOT, [&] {
-                    k_fg_axpy_dot<<<fb, kThreads, 0, stream>>>(N, st, fg_part, fg_v[j + 1], fg_v[i],
-                                                               i < j ? fg_v[i + 1] : nullptr, hc, i);
-                }));
-            HIPCHK(hipGetLastError());
-            HIPCHK(hipMemcpyAsync(col.data(), hc, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, stream));
-            HIPCHK(hipStreamSynchronize(stream));
-            CHK(harvest());
+            if (!stepped) CHK(enqueue_step());
             // Hessenberg column j, previous Givens rotations, new rotation
             for (int i = 0; i <= j; ++i) Hm[(size_t)i * m + j] = col[i];
             const double hn = sqrt(std::max(col[j + 1], 0.0));
@@ -2286,9 +2341,12 @@ int windhip_solve(windhip_solver *s, const void *alpha, const void *div, const w
     if (o.iters_per_sync <= 0) {
         long per = 64000000L / std::max(1L, s->N);
         o.iters_per_sync = (int)std::max(4L, std::min(64L, per));
-        if (o.precision == WINDHIP_MIXED_MG) o.iters_per_sync = 2;  // a few V-cycle iterations per inner solve
+        // a V-cycle preconditioned inner solve mostly ends after one iteration:
+        // a longer batch would end in launches that return at once
+        if (o.precision == WINDHIP_MIXED_MG) o.iters_per_sync = 1;
     }
     s->profiling = o.profile != 0;
+    if (s->profiling && !s->prof_flag) CHK(s->alloc(&s->prof_flag, windhip_solver::kProfFlags, false, 0));
     HIPCHK(hipEventRecord(s->ev0, s->stream));
     CHK(s->prepare_call(alpha, div));
     if (o.start == WINDHIP_START_ZERO) HIPCHK(hipMemsetAsync(s->x, 0, s->N * sizeof(double), s->stream));

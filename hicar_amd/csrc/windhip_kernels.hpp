@@ -66,6 +66,13 @@ struct KState {
     unsigned cnt[4];
 };
 
+// Profiling: whether the next launch of a solve does work, as it sees the
+// state: bit 0 the solve is done, bit 1 an x/r/p update mode is set.
+__global__ void k_prof_gate(const KState *st, int *out)
+{
+    if (threadIdx.x == 0) *out = (st->done ? 1 : 0) | (st->fmode ? 2 : 0);
+}
+
 // Three-level window of the five columns C, W, E, S, N: levels k-1, k, k+1.
 template <typename T>
 struct Win {

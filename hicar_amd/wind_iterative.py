@@ -215,7 +215,7 @@ class WindSolver:
         lam = np.ascontiguousarray(lam, dtype=np.float64).reshape(self.rows_shape)
         nat.check(self.lib.windhip_set_lambda(self.h, lam.ctypes.data_as(C.POINTER(C.c_double))))
 
-    def set_multigrid(self, pre: int = 2, post: int = 2, coarse_pre: int = 2, coarse_post: int = 2, depth: int = 0,
+    def set_multigrid(self, pre: int = 2, post: int = 2, coarse_pre: int = 1, coarse_post: int = 1, depth: int = 0,
                       omega: float = 0.9) -> None:
         """V-cycle of the MIXED_MG inner preconditioner (no counterpart in the reference)."""
         nat.check(self.lib.windhip_set_multigrid(self.h, int(pre), int(post), int(coarse_pre), int(coarse_post),

@@ -259,7 +259,7 @@ int windhip_set_net(windhip_solver *s, windhip_net *net);
 int windhip_set_guess_smoothing(windhip_solver *s, int steps);
 /* MIXED_MG V-cycle: `pre` and `post` damped line-smoothing steps on the fine
  * level and `coarse_pre` (1..8), `coarse_post` (0..8) on the coarse levels
- * (defaults 2 each) around each coarse correction; `depth` coarse levels used
+ * (defaults 2 on the fine level, 1 on the coarse ones) around each coarse correction; `depth` coarse levels used
  * (0, the default: down to the deepest level with at least 4 interior columns
  * each way; a level with one interior column is solved exactly, any other
  * last level is smoothed coarse_pre + coarse_post times); damping omega

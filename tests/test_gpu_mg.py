@@ -311,6 +311,63 @@ def test_onepass_smoothing_solve_equals_the_two_kernel_solve(name):
     s.set_mg_fusion(1, True)
 
 
+@pytest.mark.parametrize("inner_tol", [0.3, 0.01])
+def test_fgmres_step_enqueued_ahead_keeps_the_solve(inner_tol):
+    """MIXED_MG enqueues each direction's FGMRES step (FP64 matvec, Gram-Schmidt,
+    Hessenberg column) right behind the first batch of its inner solve and
+    repeats it when the inner solve had not ended there. Batches of 1 iteration
+    (the default: the step is discarded whenever an inner solve needs more, as
+    at inner_tol 0.01), of 8 and without graphs run the same iterations on the
+    same values: equal iteration counts and a bitwise equal lambda."""
+    from hicar_amd import _native as nat
+    c, o, div, s = _setup("fractal_alpha")
+    runs = []
+    for ips, graph in ((1, 1), (8, 1), (1, 0)):
+        s.set_lambda(np.zeros(s.rows_shape))
+        info = s.solve(c.alpha, div, precision=nat.MIXED_MG, tol_rel=1e-8, start=1, max_iters=5000,
+                       inner_tol=inner_tol, iters_per_sync=ips, use_graph=graph)
+        assert info["status"] == 0 and info["true_rel_res"] <= 1e-8
+        runs.append((info, s.get_lambda()))
+    (a, la), (b, lb), (d, ld) = runs
+    if inner_tol < 0.1:
+        assert a["iters"] > a["outer_iters"]  # inner solves of several iterations
+    for info, lam in ((b, lb), (d, ld)):
+        assert info["iters"] == a["iters"] and info["outer_iters"] == a["outer_iters"]
+        assert np.array_equal(lam, la)
+    s.close()
+
+
+def test_profile_counts_only_the_launches_that_ran():
+    """The profiling mode times every Krylov launch, but a launch enqueued
+    after its solve has finished (the rest of a batch of iterations) returns at
+    once: such launches are left out of the per-kind counts. Each BiCGStab
+    iteration then counts one sigma and one omega matvec, and each V-cycle (two
+    per iteration, zero start, fine V(2,2)) one residual + restriction, one
+    coarse region and three one-pass smoothing steps; the plain FP64 solve one
+    sigma sweep, one omega sweep and one x/r/p update per iteration."""
+    from hicar_amd import _native as nat
+    c, o, div, s = _setup("hill")
+    s.set_lambda(np.zeros(s.rows_shape))
+    s.profile(reset=True)
+    info = s.solve(c.alpha, div, precision=nat.MIXED_MG, tol_rel=1e-8, start=1, max_iters=2000, inner_tol=0.3,
+                   profile=1)
+    prof = s.profile(reset=True)
+    assert info["status"] == 0
+    it = info["iters"]
+    assert prof["mv_sigma_f32"][0] == it and prof["mv_omega_f32"][0] == it
+    assert prof["resid_restrict_f32"][0] == 2 * it and prof["mg_coarse_f32"][0] == 2 * it
+    assert prof["smooth_f32"][0] == 3 * 2 * it
+    s.set_lambda(np.zeros(s.rows_shape))
+    info = s.solve(c.alpha, div, precision=nat.PLAIN_F64, tol_rel=1e-8, start=1, max_iters=5000, profile=1,
+                   iters_per_sync=64)
+    prof = s.profile(reset=True)
+    assert info["status"] == 0
+    it = info["iters"]
+    for kind in ("sweep_sigma_f64", "sweep_omega_f64", "update_xrp_f64", "update_s_f64"):
+        assert prof[kind][0] == it, kind
+    s.close()
+
+
 # ----------------------------------------------------------- Galerkin levels
 def _transfers(nxf, nyf, nxc, nyc, mz):
     """R (mean over a coarse column's 2 x 2 fine columns) and P (cell-centred
