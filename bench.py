@@ -79,7 +79,7 @@ def kernel_bytes(kind: str, nx: int, ny: int, nz: int, coef_bytes: int = 0) -> f
     cells = nx * ny * nz
     P = (nx + 2) * (ny + 2)
     N = P * (nz + 2)
-    t = 8 if kind.endswith("f64") or kind in ("ir_resid", "true_resid", "fg_matvec", "fg_axpy_dot") else 4
+    t = 8 if kind.endswith("f64") or kind in ("ir_resid", "true_resid", "fg_matvec") else 4
     coef = 6 * (coef_bytes or t) * cells + 4 * t * P
     table = {
         "sweep": 3 * N * t + coef,        # y (stencil), in, coefficients -> out
@@ -105,8 +105,6 @@ def kernel_bytes(kind: str, nx: int, ny: int, nz: int, coef_bytes: int = 0) -> f
         return N * (16 + 4 * 6)
     if kind == "fg_matvec":   # z (FP64 widened), v0, coefficients -> w
         return 3 * N * 8 + 6 * 8 * cells + 4 * 8 * P
-    if kind == "fg_axpy_dot":  # w (in/out), v_i, v_{i+1}
-        return 4 * N * 8
     if kind == "smooth_f32":  # y (stencil), in, coefficients, 3 factor rows -> out (the kernel re-reads y's
         fb = coef_bytes or 4  # centre for the update, mostly from cache: not counted)
         return 3 * N * 4 + 6 * fb * cells + 4 * 4 * P + 3 * fb * N
@@ -119,10 +117,23 @@ def kernel_bytes(kind: str, nx: int, ny: int, nz: int, coef_bytes: int = 0) -> f
         return 10 * N * 4
     if kind == "mg_transfer_f32":  # restriction (r -> b_1) and prolongation (y += P e_1) alternate
         return 0.5 * (1.25 + 2.25) * N * 4
-    if kind in ("mg_coarse_f32", "mg_setup"):  # every kernel of the coarse levels / the set-up: no single size
+    if kind in ("mg_coarse_f32", "mg_setup", "fg_gram_schmidt"):  # no single size per call
         return None
     base = kind.rsplit("_", 1)[0]
     return float(table[base])
+
+
+def gram_schmidt_bytes(nx: int, ny: int, nz: int, steps: int, restart: int = 32) -> float:
+    """Average algorithmic bytes of the Gram-Schmidt region of one FGMRES step
+    over `steps` consecutive steps (cycles of `restart` steps): step j reads w
+    and v_1..v_j for the inner products (none at j = 0; h_0 comes with the
+    matvec), then w and v_0..v_j, writing w, for the update and ||w||^2."""
+    N = (nx + 2) * (ny + 2) * (nz + 2)
+    tot = 0
+    for q in range(steps):
+        j = q % restart
+        tot += ((j + 1) if j else 0) + (j + 3)
+    return 8.0 * N * tot / steps
 
 
 def coarse_cycle_model(levels, cpre=1, cpost=1, wave_cols=4608, gal_from=2):
@@ -504,6 +515,8 @@ def main():
                                                          "line_solve_f32", "smooth_f32",
                                                          "resid_restrict_f32") else 0
             nb = kernel_bytes(kind, case.nx, case.ny, case.nz, cb)
+            if kind == "fg_gram_schmidt":
+                nb = gram_schmidt_bytes(case.nx, case.ny, case.nz, int(calls))
             if kind == "mg_coarse_f32" and coarse is not None:
                 nb = coarse["bytes"]
             gbs = nb / (avg * 1e-3) / 1e9 if nb else None

@@ -47,7 +47,7 @@ PROFILE_KINDS = {0: "sweep_f64", 1: "sweep_sigma_f64", 2: "update_s_f64", 3: "sw
                  4: "update_xrp_f64", 5: "sweep_f32", 6: "sweep_sigma_f32", 7: "update_s_f32",
                  8: "sweep_omega_f32", 9: "update_xrp_f32",
                  10: "init_f64", 11: "ir_resid", 12: "ir_accumulate", 13: "true_resid",
-                 14: "fg_setup", 15: "fg_matvec", 16: "fg_axpy_dot", 17: "line_solve_f32",
+                 14: "fg_setup", 15: "fg_matvec", 16: "fg_gram_schmidt", 17: "line_solve_f32",
                  18: "resid_f32", 19: "mv_sigma_f32", 20: "mv_omega_f32", 21: "mg_transfer_f32",
                  22: "mg_coarse_f32", 23: "mg_setup", 24: "smooth_f32",
                  25: "resid_restrict_f32"}

@@ -134,6 +134,7 @@ struct windhip_solver {
     // FGMRES-IR (mixed precision): basis, FP32 directions, Hessenberg columns
     static constexpr int kMaxRestart = 40;
     static constexpr int kRestart = 32;  // FGMRES cycle length (directions kept per cycle)
+    static_assert(kRestart <= 32 && kRestart < kFgMaxBasis, "the Gram-Schmidt pass holds 32 basis vectors");
     int fg_m = 0;
     double *fg_v[kMaxRestart + 1] = {};
     float *fg_z[kMaxRestart] = {};
@@ -467,6 +468,32 @@ struct windhip_solver {
     OpAB level_opab(const MgLevel &L) const;
     int alloc_f32_krylov();
     int alloc_fgmres(int m);
+    // the two classical Gram-Schmidt passes of FGMRES step j, instantiated
+    // for the smallest of 2, 4, 8 (16, 32) products that holds the pass
+    template <class F>
+    static void nv_bucket(int cnt, F &&f)
+    {
+        if (cnt <= 2) f(std::integral_constant<int, 2>{});
+        else if (cnt <= 4) f(std::integral_constant<int, 4>{});
+        else if (cnt <= 8) f(std::integral_constant<int, 8>{});
+        else if (cnt <= 16) f(std::integral_constant<int, 16>{});
+        else f(std::integral_constant<int, 32>{});
+    }
+    void fg_dots(const FgBasis &vb, int j, const double *w, double *h, int fb)
+    {
+        for (int i0 = 1; i0 <= j; i0 += 8) {  // at most 8 products per pass (registers)
+            const int cnt = std::min(8, j + 1 - i0);
+            if (cnt <= 2) k_fg_dots<2><<<fb, kThreads, 0, stream>>>(N, st, fg_part, w, vb, i0, cnt, h);
+            else if (cnt <= 4) k_fg_dots<4><<<fb, kThreads, 0, stream>>>(N, st, fg_part, w, vb, i0, cnt, h);
+            else k_fg_dots<8><<<fb, kThreads, 0, stream>>>(N, st, fg_part, w, vb, i0, cnt, h);
+        }
+    }
+    void fg_orth(const FgBasis &vb, int j, double *w, double *h, int fb)
+    {
+        nv_bucket(j + 1, [&](auto nv) {
+            k_fg_orth<decltype(nv)::value><<<fb, kThreads, 0, stream>>>(N, st, fg_part, w, vb, j, h);
+        });
+    }
     int correction(void *u, void *v, void *w, const void *density, int adv_den);
     int divergence(const void *u, const void *v, const void *w, const void *density, int adv_den,
                    int horz_only, int use_dqdt, void *div_out);
@@ -762,7 +789,7 @@ int windhip_solver::read_state(int idx)
 // the handle's stream): base + 5 * precision (0 FP64, 1 FP32).
 enum { PK_SWEEP = 0, PK_SIGMA = 1, PK_UPD_S = 2, PK_OMEGA = 3, PK_UPD_XRP = 4, PK_INIT = 10,
        PK_IR_RESID = 11, PK_IR_ACC = 12, PK_TRUE_RESID = 13, PK_FG_SETUP = 14, PK_FG_MATVEC = 15,
-       PK_FG_AXPY_DOT = 16, PK_LINE_SOLVE = 17, PK_RESID = 18, PK_MV_SIGMA = 19, PK_MV_OMEGA = 20,
+       PK_FG_GS = 16, PK_LINE_SOLVE = 17, PK_RESID = 18, PK_MV_SIGMA = 19, PK_MV_OMEGA = 20,
        PK_MG_TRANSFER = 21, PK_MG_COARSE = 22, PK_MG_SETUP = 23, PK_SMOOTH = 24, PK_RESID_RESTRICT = 25, PK_COUNT = 26 };
 
 template <class Launch>
@@ -1820,8 +1847,12 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
         bool breakdown = false;
         for (int j = 0; j < m && steps < o.max_outer && total_inner < o.max_iters; ++j) {
             double *hc = fg_h + (long)j * H;
-            // w = A z_j, orthogonalised against v_0..v_j (modified Gram-Schmidt);
-            // the Hessenberg column j and both states to the host
+            // w = A z_j (with h_0 = <w, v_0>), orthogonalised against v_0..v_j by
+            // classical Gram-Schmidt: the other inner products from one read of
+            // w, then w -= sum h_i v_i with ||w||^2 in a second pass (2j + 4
+            // vector passes against 4j + 3 for modified Gram-Schmidt; FP64, and
+            // the true residual closes every cycle). The Hessenberg column j and
+            // both states to the host.
             auto enqueue_step = [&]() -> int {
                 CHK(with_op<double>([&](auto op) {
                     return timed(PK_FG_MATVEC, [&] {
@@ -1829,11 +1860,12 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
                             op, g, st, partials, x32, fg_z[j], fg_v[0], fg_v[j + 1], hc);
                     });
                 }));
-                for (int i = 0; i <= j; ++i)
-                    CHK(timed(PK_FG_AXPY_DOT, [&] {
-                        k_fg_axpy_dot<<<fb, kThreads, 0, stream>>>(N, st, fg_part, fg_v[j + 1], fg_v[i],
-                                                                   i < j ? fg_v[i + 1] : nullptr, hc, i);
-                    }));
+                FgBasis vb{};
+                for (int i = 0; i <= j; ++i) vb.v[i] = fg_v[i];
+                CHK(timed(PK_FG_GS, [&] {
+                    if (j > 0) fg_dots(vb, j, fg_v[j + 1], hc, fb);
+                    fg_orth(vb, j, fg_v[j + 1], hc, fb);
+                }));
                 HIPCHK(hipGetLastError());
                 HIPCHK(hipMemcpyAsync(col.data(), hc, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, stream));
                 HIPCHK(hipMemcpyAsync(st_host, st, 2 * sizeof(KState), hipMemcpyDeviceToHost, stream));
@@ -1959,7 +1991,7 @@ int windhip_solver::alloc_fgmres(int m)
     if (fg_m == 0) CHK(alloc(&fg_v[0], N));
     for (int q = fg_m; q < m; ++q) CHK(alloc(&fg_z[q], N));
     if (!fg_part) {
-        CHK(alloc(&fg_part, (long)kFlatBlocks * 2));
+        CHK(alloc(&fg_part, (long)kFlatBlocks * 8));  // up to 8 products per block (Gram-Schmidt)
         CHK(alloc(&fg_h, (long)(kMaxRestart + 1) * (kMaxRestart + 2)));
         CHK(alloc(&fg_y, kMaxRestart));
         CHK(alloc(&fg_zs, kMaxRestart));

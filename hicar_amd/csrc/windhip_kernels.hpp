@@ -443,7 +443,7 @@ __device__ __forceinline__ void reduce_finalize(double (&acc)[NV], double *parti
     // a batch of FB partials is loaded before any of them is added (each add
     // right after its load waited one memory round trip per partial); the
     // adds keep the order q = t, t + 64 NW, ...
-    constexpr int FB = NV == 1 ? 8 : (NV <= 2 ? 4 : 2);
+    constexpr int FB = NV == 1 ? 8 : (NV <= 2 ? 4 : (NV < 8 ? 2 : 1));
     constexpr int S = NW * 64;
     for (int q0 = threadIdx.x; q0 < nb; q0 += FB * S) {
         double pv[FB][NV];
@@ -2012,6 +2012,55 @@ __global__ __launch_bounds__(kThreads) void k_fg_axpy_dot(long N, KState *st, do
     }
     reduce_finalize<1>(acc, partials, &st->cnt[3], blockIdx.x, gridDim.x,
                        [&](const double *tot) { h[i + 1] = tot[0]; });
+}
+
+// The Arnoldi basis vectors one classical Gram-Schmidt pass reads.
+constexpr int kFgMaxBasis = 33;
+struct FgBasis {
+    const double *v[kFgMaxBasis];
+};
+
+// Classical Gram-Schmidt, first pass: h[i] = <w, v_i> for i = i0 .. i0+cnt-1
+// (cnt <= NV), all from one read of w.
+template <int NV>
+__global__ __launch_bounds__(kThreads) void k_fg_dots(long N, KState *st, double *partials,
+                                                      const double *__restrict__ w, FgBasis vb, int i0, int cnt,
+                                                      double *h)
+{
+    double acc[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+    for (long n = (long)blockIdx.x * kThreads + threadIdx.x; n < N; n += (long)gridDim.x * kThreads) {
+        const double wv = w[n];
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+            if (q < cnt) acc[q] = acc[q] + wv * vb.v[i0 + q][n];
+    }
+    reduce_finalize<NV>(acc, partials, &st->cnt[3], blockIdx.x, gridDim.x, [&](const double *tot) {
+        for (int q = 0; q < cnt; ++q) h[i0 + q] = tot[q];
+    });
+}
+
+// Classical Gram-Schmidt, second pass: w -= sum_{i<=j} h[i] v_i, then
+// h[j+1] = ||w||^2. j + 1 <= NV.
+template <int NV>
+__global__ __launch_bounds__(kThreads) void k_fg_orth(long N, KState *st, double *partials, double *__restrict__ w,
+                                                      FgBasis vb, int j, double *h)
+{
+    double c[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) c[q] = q <= j ? h[q] : 0.0;
+    double acc[1] = {0.0};
+    for (long n = (long)blockIdx.x * kThreads + threadIdx.x; n < N; n += (long)gridDim.x * kThreads) {
+        double wv = w[n];
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+            if (q <= j) wv = wv - c[q] * vb.v[q][n];
+        w[n] = wv;
+        acc[0] = acc[0] + wv * wv;
+    }
+    reduce_finalize<1>(acc, partials, &st->cnt[3], blockIdx.x, gridDim.x,
+                       [&](const double *tot) { h[j + 1] = tot[0]; });
 }
 
 // x += sum_q y[q] z_q (FP64 accumulation of the FP32 directions).

@@ -343,7 +343,8 @@ int windhip_learned_input(windhip_solver *s, float *in_dev, int *dims_out, doubl
  * s-update, last sweep + t = A s + 5 dots, x/r/p-update; 5..9 the same in FP32;
  * 10 set-up, 11 FP64 residual of the mixed solve, 12 mixed-solve lambda update,
  * 13 true residual, 14 FGMRES inner set-up, 15 FGMRES FP64 matvec,
- * 16 FGMRES Gram-Schmidt step; MIXED_LINE inner iteration: 17 line solve,
+ * 16 the Gram-Schmidt orthogonalisation of one FGMRES step (its two passes
+ * as one region); MIXED_LINE inner iteration: 17 line solve,
  * 18 residual in - A y, 19 v = A p_hat + sigma, 20 t = A s_hat + 5 dots (its
  * s- and x/r/p-updates count as 7 and 9); MIXED_MG adds 21 fine-level
  * restriction and prolongation, 22 the coarse levels (one region per V-cycle
@@ -352,7 +353,9 @@ int windhip_learned_input(windhip_solver *s, float *in_dev, int *dims_out, doubl
  * coarse levels' set-up (its fine-level line solves and residuals count as 17
  * and 18), 24 fine-level one-pass smoothing steps (residual and line solve in
  * one kernel), 25 the fine level's residual and restriction in one kernel.
- * reset != 0 clears the counters after reading. */
+ * Launches that a finished solve left behind (the rest of a batch of
+ * iterations; they return at once) are not counted. reset != 0 clears the
+ * counters after reading. */
 #define WINDHIP_PROFILE_KINDS 26
 int windhip_profile(windhip_solver *s, int reset, double *calls, double *ms);
 
