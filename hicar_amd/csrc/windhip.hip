@@ -134,7 +134,7 @@ struct windhip_solver {
     // FGMRES-IR (mixed precision): basis, FP32 directions, Hessenberg columns
     static constexpr int kMaxRestart = 40;
     static constexpr int kRestart = 32;  // FGMRES cycle length (directions kept per cycle)
-    static_assert(kRestart <= 32 && kRestart < kFgMaxBasis, "the Gram-Schmidt pass holds 32 basis vectors");
+    static_assert(kRestart < kFgMaxBasis, "the Gram-Schmidt passes address the whole basis");
     int fg_m = 0;
     double *fg_v[kMaxRestart + 1] = {};
     float *fg_z[kMaxRestart] = {};
@@ -468,31 +468,25 @@ struct windhip_solver {
     OpAB level_opab(const MgLevel &L) const;
     int alloc_f32_krylov();
     int alloc_fgmres(int m);
-    // the two classical Gram-Schmidt passes of FGMRES step j, instantiated
-    // for the smallest of 2, 4, 8 (16, 32) products that holds the pass
-    template <class F>
-    static void nv_bucket(int cnt, F &&f)
-    {
-        if (cnt <= 2) f(std::integral_constant<int, 2>{});
-        else if (cnt <= 4) f(std::integral_constant<int, 4>{});
-        else if (cnt <= 8) f(std::integral_constant<int, 8>{});
-        else if (cnt <= 16) f(std::integral_constant<int, 16>{});
-        else f(std::integral_constant<int, 32>{});
-    }
+    // the two classical Gram-Schmidt passes of FGMRES step j, over at most 8
+    // basis vectors per launch (registers): the products with v_1..v_j (h_0
+    // comes with the matvec), then the update with ||w||^2
     void fg_dots(const FgBasis &vb, int j, const double *w, double *h, int fb)
     {
-        for (int i0 = 1; i0 <= j; i0 += 8) {  // at most 8 products per pass (registers)
-            const int cnt = std::min(8, j + 1 - i0);
-            if (cnt <= 2) k_fg_dots<2><<<fb, kThreads, 0, stream>>>(N, st, fg_part, w, vb, i0, cnt, h);
-            else if (cnt <= 4) k_fg_dots<4><<<fb, kThreads, 0, stream>>>(N, st, fg_part, w, vb, i0, cnt, h);
-            else k_fg_dots<8><<<fb, kThreads, 0, stream>>>(N, st, fg_part, w, vb, i0, cnt, h);
-        }
+        for (int i0 = 1; i0 <= j; i0 += 8)
+            with_count8(std::min(8, j + 1 - i0), [&](auto c) {
+                k_fg_dots<decltype(c)::value><<<fb, kThreads, 0, stream>>>(N, st, fg_part, w, vb, i0, h);
+            });
     }
     void fg_orth(const FgBasis &vb, int j, double *w, double *h, int fb)
     {
-        nv_bucket(j + 1, [&](auto nv) {
-            k_fg_orth<decltype(nv)::value><<<fb, kThreads, 0, stream>>>(N, st, fg_part, w, vb, j, h);
-        });
+        for (int i0 = 0; i0 <= j; i0 += 8) {
+            const int cnt = std::min(8, j + 1 - i0);
+            with_count8(cnt, [&](auto c) {
+                k_fg_orth<decltype(c)::value><<<fb, kThreads, 0, stream>>>(N, st, fg_part, w, vb, i0, j,
+                                                                           i0 + cnt > j, h);
+            });
+        }
     }
     int correction(void *u, void *v, void *w, const void *density, int adv_den);
     int divergence(const void *u, const void *v, const void *w, const void *density, int adv_den,

@@ -2020,47 +2020,71 @@ struct FgBasis {
     const double *v[kFgMaxBasis];
 };
 
-// Classical Gram-Schmidt, first pass: h[i] = <w, v_i> for i = i0 .. i0+cnt-1
-// (cnt <= NV), all from one read of w.
-template <int NV>
+// Classical Gram-Schmidt, first pass: h[i0 + q] = <w, v_{i0+q}> for the C
+// basis vectors q < C, all from one read of w (C a compile-time count, so
+// every load is unconditional and a row's loads are in flight together).
+template <int C>
 __global__ __launch_bounds__(kThreads) void k_fg_dots(long N, KState *st, double *partials,
-                                                      const double *__restrict__ w, FgBasis vb, int i0, int cnt,
-                                                      double *h)
+                                                      const double *__restrict__ w, FgBasis vb, int i0, double *h)
 {
-    double acc[NV];
+    const double *v[C];
 #pragma unroll
-    for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+    for (int q = 0; q < C; ++q) v[q] = vb.v[i0 + q];
+    double acc[C];
+#pragma unroll
+    for (int q = 0; q < C; ++q) acc[q] = 0.0;
     for (long n = (long)blockIdx.x * kThreads + threadIdx.x; n < N; n += (long)gridDim.x * kThreads) {
         const double wv = w[n];
 #pragma unroll
-        for (int q = 0; q < NV; ++q)
-            if (q < cnt) acc[q] = acc[q] + wv * vb.v[i0 + q][n];
+        for (int q = 0; q < C; ++q) acc[q] = acc[q] + wv * v[q][n];
     }
-    reduce_finalize<NV>(acc, partials, &st->cnt[3], blockIdx.x, gridDim.x, [&](const double *tot) {
-        for (int q = 0; q < cnt; ++q) h[i0 + q] = tot[q];
+    reduce_finalize<C>(acc, partials, &st->cnt[3], blockIdx.x, gridDim.x, [&](const double *tot) {
+#pragma unroll
+        for (int q = 0; q < C; ++q) h[i0 + q] = tot[q];
     });
 }
 
-// Classical Gram-Schmidt, second pass: w -= sum_{i<=j} h[i] v_i, then
-// h[j+1] = ||w||^2. j + 1 <= NV.
-template <int NV>
+// Classical Gram-Schmidt, second pass, over the C basis vectors from i0:
+// w -= sum_q h[i0 + q] v_{i0+q} (in that order); the pass that ends at v_j
+// (last) also forms h[j+1] = ||w||^2.
+template <int C>
 __global__ __launch_bounds__(kThreads) void k_fg_orth(long N, KState *st, double *partials, double *__restrict__ w,
-                                                      FgBasis vb, int j, double *h)
+                                                      FgBasis vb, int i0, int j, int last, double *h)
 {
-    double c[NV];
+    const double *v[C];
+    double c[C];
 #pragma unroll
-    for (int q = 0; q < NV; ++q) c[q] = q <= j ? h[q] : 0.0;
+    for (int q = 0; q < C; ++q) {
+        v[q] = vb.v[i0 + q];
+        c[q] = h[i0 + q];
+    }
     double acc[1] = {0.0};
     for (long n = (long)blockIdx.x * kThreads + threadIdx.x; n < N; n += (long)gridDim.x * kThreads) {
         double wv = w[n];
 #pragma unroll
-        for (int q = 0; q < NV; ++q)
-            if (q <= j) wv = wv - c[q] * vb.v[q][n];
+        for (int q = 0; q < C; ++q) wv = wv - c[q] * v[q][n];
         w[n] = wv;
         acc[0] = acc[0] + wv * wv;
     }
-    reduce_finalize<1>(acc, partials, &st->cnt[3], blockIdx.x, gridDim.x,
-                       [&](const double *tot) { h[j + 1] = tot[0]; });
+    if (last)
+        reduce_finalize<1>(acc, partials, &st->cnt[3], blockIdx.x, gridDim.x,
+                           [&](const double *tot) { h[j + 1] = tot[0]; });
+}
+
+// f(std::integral_constant<int, c>) for a count 1 <= c <= 8.
+template <class F>
+__host__ void with_count8(int c, F &&f)
+{
+    switch (c) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 5: f(std::integral_constant<int, 5>{}); break;
+    case 6: f(std::integral_constant<int, 6>{}); break;
+    case 7: f(std::integral_constant<int, 7>{}); break;
+    default: f(std::integral_constant<int, 8>{}); break;
+    }
 }
 
 // x += sum_q y[q] z_q (FP64 accumulation of the FP32 directions).
