@@ -1791,7 +1791,9 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
             fprintf(stderr, "[windhip] learned start: residual %.3e vs plain start %.3e -> %s\n", rg, plain_res,
                     rg < plain_res ? "network" : "plain");
     }
-    const double inner_tol = (o.inner_tol > 0 && o.inner_tol < 1) ? o.inner_tol : 0.5;
+    // 0 (or out of range): one V-cycle preconditioned iteration mostly reaches
+    // 0.3; the other inner solves run to 0.5
+    const double inner_tol = (o.inner_tol > 0 && o.inner_tol < 1) ? o.inner_tol : (multigrid ? 0.3 : 0.5);
     const int m = std::max(1, std::min(o.max_outer, kRestart));
     CHK(alloc_fgmres(m));
     const int H = m + 2;  // stride of a Hessenberg column in fg_h
@@ -2250,7 +2252,7 @@ void windhip_default_opts(windhip_solve_opts *o)
     o->max_iters = 1500;   // wind_solver_max_iters default (:57)
     o->start = WINDHIP_START_WARM;
     o->precision = WINDHIP_PLAIN_F64;
-    o->inner_tol = 0.5;
+    o->inner_tol = 0.0;  // automatic: 0.3 with MIXED_MG, else 0.5
     o->inner_max_iters = 2000;
     o->max_outer = 60;
     o->iters_per_sync = 0;

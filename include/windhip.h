@@ -57,9 +57,10 @@ enum { WINDHIP_START_WARM = 0, WINDHIP_START_ZERO = 1, WINDHIP_START_GIVEN = 2,
  * lambda accumulation stay FP64. MIXED_LINE: the same with the FP32 BiCGStab
  * preconditioned by two vertical line-relaxation steps instead; MIXED_MG:
  * preconditioned by one multigrid V-cycle (horizontal 2 x 2 semi-coarsening,
- * coarse operators rediscretised from the averaged metrics at each level's
- * spacing, damped vertical line smoothing; windhip_set_multigrid). Neither has a counterpart in the
- * reference. */
+ * the first coarse operator rediscretised from the averaged metrics at its
+ * spacing, the deeper ones Galerkin products R A P (windhip_set_mg_galerkin),
+ * damped vertical line smoothing; windhip_set_multigrid). Neither has a
+ * counterpart in the reference. */
 enum { WINDHIP_PLAIN_F64 = 0, WINDHIP_MIXED = 1, WINDHIP_MIXED_LINE = 2, WINDHIP_MIXED_MG = 3 };
 
 typedef struct windhip_desc {
@@ -76,7 +77,8 @@ typedef struct windhip_solve_opts {
     int max_iters;         /* wind_solver_iterations (1500, :57); MIXED: cap on all FP32 iterations */
     int start;             /* WINDHIP_START_* */
     int precision;         /* WINDHIP_PLAIN_F64, WINDHIP_MIXED, WINDHIP_MIXED_LINE or WINDHIP_MIXED_MG */
-    double inner_tol;      /* MIXED: relative reduction asked of each FP32 inner solve (0.5) */
+    double inner_tol;      /* mixed modes: relative reduction asked of each FP32 inner solve
+                              (0 = automatic: 0.3 with MIXED_MG, else 0.5) */
     int inner_max_iters;   /* MIXED: cap per inner solve */
     int max_outer;         /* MIXED: cap on FGMRES steps (= inner solves); cycles restart every 32 */
     int iters_per_sync;    /* iterations launched between host convergence checks; 0 = auto */
