@@ -159,7 +159,8 @@ struct windhip_solver {
     float *net_out = nullptr;
     __bf16 *net_in = nullptr;  // the learned start's network input, bf16
     long net_vox = 0;
-    int net_smooth = 4;                        // line-relaxation steps applied to the network's guess
+    int net_smooth = -1;  // line-relaxation steps applied to the network's guess (-1: 10 with the
+                          // multigrid's one-pass steps, else 4)
     double *lr_c = nullptr, *lr_d = nullptr;   // line relaxation: Thomas coefficients
     // the learned guess's FP64 residual b - A guess and its squared norm, taken
     // on the guess's stream for the mixed solves: the first FGMRES residual
@@ -1666,8 +1667,9 @@ int windhip_solver::learned_launch(double *res_plain, double **guess, bool f32_r
     k_net_output<<<flat_grid(d.P), kBlock, 0, net_stream>>>(d, Dp, Hp, Wp, net_out, scale, t);
     HIPCHK(hipGetLastError());
     double *gy = t, *gs = s;
-    if (f32_relax) CHK(line_relax32(gy, net_smooth, net_stream, mg_rows));
-    else CHK(line_relax(&gy, &gs, net_smooth, net_stream));
+    const int steps = net_smooth >= 0 ? net_smooth : (mg_rows ? 10 : 4);
+    if (f32_relax) CHK(line_relax32(gy, steps, net_stream, mg_rows));
+    else CHK(line_relax(&gy, &gs, steps, net_stream));
     if (f32_relax) {
         // the mixed solves: the guess's residual vector too, the first FGMRES
         // residual if the guess becomes the start (k_fg_resid, as there)

@@ -252,7 +252,8 @@ int windhip_net_adam_step(windhip_net *net, float lr, float beta1, float beta2, 
 /* Mean device time of the forward pass over a B x D x H x W volume */
 int windhip_net_time_forward(windhip_net *net, int B, int D, int H, int W, int reps, double *ms);
 /* Attach a network to a solver: opts.start = WINDHIP_START_LEARNED then
- * starts from the network's guess, after `steps` (default 4) vertical
+ * starts from the network's guess, after `steps` (default: 10 in MIXED_MG,
+ * whose steps are one-pass kernels on bf16 rows, else 4) vertical
  * line-relaxation steps lambda += M^-1 (b - A lambda) with M the tridiagonal
  * column part of A, when its FP64 residual is below the plain start's (the
  * current lambda), and from the plain start otherwise. A network whose
