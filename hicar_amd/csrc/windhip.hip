@@ -435,8 +435,8 @@ struct windhip_solver {
         ~GateScope() { g = nullptr; }
     };
     double prof_ms[WINDHIP_PROFILE_KINDS] = {}, prof_calls[WINDHIP_PROFILE_KINDS] = {};
-    int launch_batch(int precision, int count, bool use_graph);
-    int enqueue_iterations_line(int count, bool multigrid);
+    int launch_batch(int precision, int count, bool use_graph, bool first = false);
+    int enqueue_iterations_line(int count, bool multigrid, bool first);
     int build_mg();
     float *mg_coarse(size_t l, bool entered = false);
     bool mg_fused = true;  // fused descent/return kernels on the wavefront-per-column levels
@@ -898,7 +898,7 @@ int windhip_solver::enqueue_iterations(int count)
 // v = A p_hat with sigma, s = r - alpha v, s_hat = M^-1 s, t = A s_hat with
 // the five dots, the x/r/p update. M^-1 is two vertical line-relaxation steps
 // (line solve, residual, line solve) or, with multigrid, one V-cycle.
-int windhip_solver::enqueue_iterations_line(int count, bool multigrid)
+int windhip_solver::enqueue_iterations_line(int count, bool multigrid, bool first)
 {
     KState *sp = st + 1;
     float *X, *R, *RH, *Pv, *V, *S, *Tv, *PH[3], *SH[3];
@@ -991,18 +991,26 @@ int windhip_solver::enqueue_iterations_line(int count, bool multigrid)
             return WINDHIP_OK;
         };
         for (int it = 0; it < count; ++it) {
-            CHK(precond(Pv, PH[1], PH[2]));
+            // the first iteration from the set-up: r = p = r_hat, x = 0
+            const bool f = first && it == 0;
+            CHK(precond(f ? RH : Pv, PH[1], PH[2]));
             CHK(timed(PK_MV_SIGMA, [&] {
                 k_mv_sigma_c<Op><<<kb, kColThreads, 0, stream>>>(op, g, sp, partials, PH[2], RH, V);
             }));
-            CHK(timed(5 + PK_UPD_S, [&] { k_update_s_only<float><<<fg, kBlock, 0, stream>>>(N, sp, R, V, S); }));
+            CHK(timed(5 + PK_UPD_S, [&] {
+                k_update_s_only<float><<<fg, kBlock, 0, stream>>>(N, sp, f ? RH : R, V, S);
+            }));
             CHK(precond(S, SH[1], SH[2]));
             CHK(timed(PK_MV_OMEGA, [&] {
                 k_mv_omega_c<Op><<<kb, kColThreads, 0, stream>>>(op, g, sp, partials, SH[2], S, RH, Tv);
             }));
             CHK(timed(5 + PK_UPD_XRP, [&] {
-                k_update_xrp<float><<<fg, kBlock, 0, stream>>>(N, sp, X, PH[2], SH[2], S, Tv, R, Pv, V, nullptr,
-                                                                 nullptr);
+                if (f)
+                    k_update_xrp<float, true><<<fg, kBlock, 0, stream>>>(N, sp, X, PH[2], SH[2], S, Tv, R, Pv, V,
+                                                                           nullptr, nullptr, RH);
+                else
+                    k_update_xrp<float><<<fg, kBlock, 0, stream>>>(N, sp, X, PH[2], SH[2], S, Tv, R, Pv, V,
+                                                                     nullptr, nullptr);
             }));
         }
         HIPCHK(hipGetLastError());
@@ -1450,15 +1458,19 @@ int windhip_solver::build_mg()
 }
 
 // Launch `count` iterations, from a cached hipGraph when use_graph.
-int windhip_solver::launch_batch(int precision, int count, bool use_graph)
+// first: the batch opens an inner solve of the line or multigrid mode (its
+// first iteration reads r and p from r_hat, x from zero)
+int windhip_solver::launch_batch(int precision, int count, bool use_graph, bool first)
 {
+    first = first && (precision == 2 || precision == 3);
     auto enqueue = [&]() {
         if (precision == 0) return enqueue_iterations<double>(count);
-        if (precision == 2 || precision == 3) return enqueue_iterations_line(count, precision == 3);
+        if (precision == 2 || precision == 3) return enqueue_iterations_line(count, precision == 3, first);
         return enqueue_iterations<float>(count);
     };
     if (!use_graph || profiling) return enqueue();
-    const long key = ((long)count << 8) | ((long)nsweeps << 4) | ((long)precision << 1) | (probed ? 1 : 0);
+    const long key = ((long)count << 9) | ((long)first << 8) | ((long)nsweeps << 4) | ((long)precision << 1) |
+                     (probed ? 1 : 0);
     auto it = graphs.find(key);
     if (it == graphs.end()) {
         hipGraph_t graph;
@@ -1890,22 +1902,24 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
                 k_state_reset<<<1, 1, 0, stream>>>(st + 1, cap, 0.0, 0.0, tol_j);
                 CHK(timed(PK_FG_SETUP, [&] {
                     k_fg_setup<<<fb, kThreads, 0, stream>>>(N, st + 1, fg_part, vsq, attempt == 0, fg_v[j],
-                                                            line ? nullptr : dinv32,
-                                                            x32, r32, rh32, p32, ph32[0]);
+                                                            line ? nullptr : dinv32, line ? nullptr : x32,
+                                                            r32, rh32, p32, ph32[0]);
                 }));
                 HIPCHK(hipGetLastError());
                 int done_iters = 0;
                 if (ahead && attempt == 0) {
                     done_iters = std::min(o.iters_per_sync, std::max(1, cap));
-                    CHK(launch_batch(3, done_iters, o.use_graph != 0));
+                    CHK(launch_batch(3, done_iters, o.use_graph != 0, true));
                     CHK(enqueue_step());
-                    stepped = st_host[1].done != 0;
+                    // an inner solve that ended before its first iteration left
+                    // z_j unwritten: its step is made again from z_j = 0 (below)
+                    stepped = st_host[1].done != 0 && st_host[1].iter > 0;
                 } else {
                     CHK(read_state(1));
                 }
                 while (!st_host[1].done) {
                     const int n = std::min(o.iters_per_sync, std::max(1, cap - done_iters));
-                    CHK(launch_batch(multigrid ? 3 : (line ? 2 : 1), n, o.use_graph != 0));
+                    CHK(launch_batch(multigrid ? 3 : (line ? 2 : 1), n, o.use_graph != 0, done_iters == 0));
                     CHK(read_state(1));
                     done_iters += n;
                     if (done_iters > cap + o.iters_per_sync) break;
@@ -1925,7 +1939,12 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
             total_inner += spent;
             ++steps;
             }
-            if (!stepped) CHK(enqueue_step());
+            if (!stepped) {
+                // the line and multigrid set-ups leave z_j (x32) to the first
+                // iteration: a solve that ended before it has z_j = 0
+                if (line && st_host[1].iter == 0) HIPCHK(hipMemsetAsync(x32, 0, N * sizeof(float), stream));
+                CHK(enqueue_step());
+            }
             // Hessenberg column j, previous Givens rotations, new rotation
             for (int i = 0; i <= j; ++i) Hm[(size_t)i * m + j] = col[i];
             const double hn = sqrt(std::max(col[j + 1], 0.0));

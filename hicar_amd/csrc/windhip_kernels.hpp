@@ -987,29 +987,40 @@ __global__ __launch_bounds__(kThreads) void k_sweep_omega(Op op, Geo g, KState *
 // x += alpha p_hat + omega s_hat (vec_axpy2, :755); r = s - omega t (:771);
 // p = r + beta (p - omega v) (vec_p_update, :783); next p_hat(sweep 1) = D^-1 p
 // (skipped when dinv is null: the line-preconditioned iteration has no use for it).
-template <typename T>
+// FIRST: the first iteration of a solve from x = 0 whose p is still r_hat:
+// x is not read (0 + ...), p is read from p_in (the same values, the same
+// arithmetic) and written to p.
+template <typename T, bool FIRST = false>
 __global__ __launch_bounds__(kBlock) void k_update_xrp(long N, const KState *st, T *__restrict__ x,
                                                        const T *__restrict__ ph, const T *__restrict__ sh,
                                                        const T *__restrict__ s, const T *__restrict__ t,
                                                        T *__restrict__ r, T *__restrict__ p,
                                                        const T *__restrict__ v, const T *__restrict__ dinv,
-                                                       T *__restrict__ ph_next)
+                                                       T *__restrict__ ph_next, const T *__restrict__ p_in = nullptr)
 {
     const int mode = st->fmode;
     if (mode == 0) return;
     const T a = T(st->alpha), om = T(st->omega), be = T(st->beta), nom = T(-st->omega);
     const long tid = (long)blockIdx.x * kBlock + threadIdx.x, stride = (long)gridDim.x * kBlock;
+    const T *pr = FIRST ? p_in : p;
     long n0 = 0;
-    if (mode >= 3 && aligned16(x, ph, sh, s, t, r, p, v, dinv, ph_next)) {
+    if (mode >= 3 && aligned16(x, ph, sh, s, t, r, p, pr, v, dinv, ph_next)) {
         // the full update, 16 bytes per lane and array (the same per-element
         // arithmetic as the loop below)
         using V = typename Vec16<T>::type;
         constexpr int W = Vec16<T>::n;
         n0 = N / W * W;
         for (long q = tid; q < N / W; q += stride) {
-            const V xv = reinterpret_cast<const V *>(x)[q], phv = reinterpret_cast<const V *>(ph)[q],
+            V xv;
+            if constexpr (FIRST) {
+#pragma unroll
+                for (int e = 0; e < W; ++e) xv[e] = T(0);
+            } else {
+                xv = reinterpret_cast<const V *>(x)[q];
+            }
+            const V phv = reinterpret_cast<const V *>(ph)[q],
                     shv = reinterpret_cast<const V *>(sh)[q], sv = reinterpret_cast<const V *>(s)[q],
-                    tv = reinterpret_cast<const V *>(t)[q], pv0 = reinterpret_cast<const V *>(p)[q],
+                    tv = reinterpret_cast<const V *>(t)[q], pv0 = reinterpret_cast<const V *>(pr)[q],
                     vv = reinterpret_cast<const V *>(v)[q];
             V xo, ro, po;
 #pragma unroll
@@ -1031,12 +1042,12 @@ __global__ __launch_bounds__(kBlock) void k_update_xrp(long N, const KState *st,
         }
     }
     for (long n = n0 + tid; n < N; n += stride) {
-        x[n] = (x[n] + a * ph[n]) + om * sh[n];
+        x[n] = ((FIRST ? T(0) : x[n]) + a * ph[n]) + om * sh[n];
         if (mode >= 2) {
             const T rv = T(1) * s[n] + nom * t[n];
             r[n] = rv;
             if (mode >= 3) {
-                const T pv = rv + be * (p[n] - om * v[n]);
+                const T pv = rv + be * (pr[n] - om * v[n]);
                 p[n] = pv;
                 if (dinv) ph_next[n] = dinv[n] * pv;
             }
@@ -1955,7 +1966,9 @@ __global__ __launch_bounds__(kThreads) void k_fg_resid_zero(long N, KState *st, 
 // Start of one preconditioner application: scale the basis vector v by
 // 1/sqrt(*vsq) in place (FP64) when normalise is set, and set up the FP32
 // BiCGStab for A z = v from z = 0 (bicgstab_solve set-up,
-// wind_iterative.F90:586-640, with r0 = v).
+// wind_iterative.F90:586-640, with r0 = v). x32 null: only r_hat is written;
+// the first iteration of the line and multigrid inner solves reads r and p
+// from it and does not read x (k_update_xrp FIRST).
 __global__ __launch_bounds__(kThreads) void k_fg_setup(long N, KState *st_in, double *partials,
                                                        const double *vsq, int normalise, double *__restrict__ v,
                                                        const float *__restrict__ dinv32, float *__restrict__ x32,
@@ -1968,8 +1981,11 @@ __global__ __launch_bounds__(kThreads) void k_fg_setup(long N, KState *st_in, do
         const double vv = v[n] * sc;
         if (normalise) v[n] = vv;
         const float rf = (float)vv;
-        x32[n] = 0.0f;
-        r32[n] = rf; rh32[n] = rf; p32[n] = rf;
+        rh32[n] = rf;
+        if (x32) {  // null: the line and multigrid inner solves start from r_hat alone
+            x32[n] = 0.0f;
+            r32[n] = rf; p32[n] = rf;
+        }
         if (dinv32) ph32[n] = dinv32[n] * rf;  // the first Jacobi sweep of p (no dinv32: unused)
         acc[0] = acc[0] + (double)rf * (double)rf;
     }
