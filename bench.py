@@ -101,7 +101,7 @@ def kernel_bytes(kind: str, nx: int, ny: int, nz: int, coef_bytes: int = 0) -> f
         return N * 16 + N * 4
     if kind == "true_resid":
         return 2 * N * 8 + 6 * 8 * cells + 4 * 8 * P
-    if kind == "fg_setup":    # v (FP64 in/out) -> x32, r32, rh32, p32, ph32 and D32^-1
+    if kind == "fg_setup":    # v (FP64 in/out) -> x32, r32, rh32, p32, ph32 and D32^-1 (the Jacobi inner solve)
         return N * (16 + 4 * 6)
     if kind == "fg_matvec":   # z (FP64 widened), v0, coefficients -> w
         return 3 * N * 8 + 6 * 8 * cells + 4 * 8 * P
@@ -515,6 +515,11 @@ def main():
                                                          "line_solve_f32", "smooth_f32",
                                                          "resid_restrict_f32") else 0
             nb = kernel_bytes(kind, case.nx, case.ny, case.nz, cb)
+            if kind == "fg_setup" and arm.startswith(("line", "mg")):
+                # the line and multigrid set-up writes r_hat alone (the first inner
+                # iteration reads r and p from it): v (FP64 in/out) -> rh32
+                N = (case.nx + 2) * (case.ny + 2) * (case.nz + 2)
+                nb = N * (16 + 4)
             if kind == "fg_gram_schmidt":
                 nb = gram_schmidt_bytes(case.nx, case.ny, case.nz, int(calls))
             if kind == "mg_coarse_f32" and coarse is not None:
