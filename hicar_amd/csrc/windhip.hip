@@ -131,11 +131,6 @@ struct windhip_solver {
     double *x, *r, *rh, *p, *ph[3], *v, *s, *sh[3], *t, *b;
     // Krylov FP32 (inner solve of the refinement)
     float *x32 = nullptr, *r32, *rh32, *p32, *ph32[3], *v32, *s32, *sh32[3], *t32;
-    // the line and multigrid FGMRES: the next direction's r_hat, written by the
-    // last Gram-Schmidt pass while the current one may still be in use, and its
-    // r_hat.r_hat (swapped in at the next direction)
-    float *rh32_alt = nullptr;
-    double *fg_rho = nullptr;
     // FGMRES-IR (mixed precision): basis, FP32 directions, Hessenberg columns
     static constexpr int kMaxRestart = 40;
     static constexpr int kRestart = 32;  // FGMRES cycle length (directions kept per cycle)
@@ -477,23 +472,20 @@ struct windhip_solver {
     // the two classical Gram-Schmidt passes of FGMRES step j, over at most 8
     // basis vectors per launch (registers): the products with v_1..v_j (h_0
     // comes with the matvec), then the update with ||w||^2
-    void fg_dots(const FgBasis &vb, const FgNorms &nm, int j, const double *w, double *h, int fb)
+    void fg_dots(const FgBasis &vb, int j, const double *w, double *h, int fb)
     {
         for (int i0 = 1; i0 <= j; i0 += 8)
             with_count8(std::min(8, j + 1 - i0), [&](auto c) {
-                k_fg_dots<decltype(c)::value><<<fb, kThreads, 0, stream>>>(N, st, fg_part, w, vb, nm, i0, h);
+                k_fg_dots<decltype(c)::value><<<fb, kThreads, 0, stream>>>(N, st, fg_part, w, vb, i0, h);
             });
     }
-    // r32_next (null: not wanted): the FP32 copy of the orthogonalised w, with
-    // its r32.r32 in *rho
-    void fg_orth(const FgBasis &vb, const FgNorms &nm, int j, double *w, double *h, int fb, float *r32_next,
-                 double *rho)
+    void fg_orth(const FgBasis &vb, int j, double *w, double *h, int fb)
     {
         for (int i0 = 0; i0 <= j; i0 += 8) {
             const int cnt = std::min(8, j + 1 - i0);
             with_count8(cnt, [&](auto c) {
-                k_fg_orth<decltype(c)::value><<<fb, kThreads, 0, stream>>>(N, st, fg_part, w, vb, nm, i0, j,
-                                                                           i0 + cnt > j, h, r32_next, rho);
+                k_fg_orth<decltype(c)::value><<<fb, kThreads, 0, stream>>>(N, st, fg_part, w, vb, i0, j,
+                                                                           i0 + cnt > j, h);
             });
         }
     }
@@ -1477,10 +1469,8 @@ int windhip_solver::launch_batch(int precision, int count, bool use_graph, bool 
         return enqueue_iterations<float>(count);
     };
     if (!use_graph || profiling) return enqueue();
-    // the two r_hat buffers of the line and multigrid FGMRES swap between directions
-    const bool rh_swapped = rh32_alt && rh32 > rh32_alt;
-    const long key = ((long)count << 10) | ((long)rh_swapped << 9) | ((long)first << 8) | ((long)nsweeps << 4) |
-                     ((long)precision << 1) | (probed ? 1 : 0);
+    const long key = ((long)count << 9) | ((long)first << 8) | ((long)nsweeps << 4) | ((long)precision << 1) |
+                     (probed ? 1 : 0);
     auto it = graphs.find(key);
     if (it == graphs.end()) {
         hipGraph_t graph;
@@ -1820,15 +1810,10 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
     const double inner_tol = (o.inner_tol > 0 && o.inner_tol < 1) ? o.inner_tol : (multigrid ? 0.3 : 0.5);
     const int m = std::max(1, std::min(o.max_outer, kRestart));
     CHK(alloc_fgmres(m));
-    if (line && !rh32_alt) {
-        CHK(alloc(&rh32_alt, N));
-        CHK(alloc(&fg_rho, 1));
-    }
     const int H = m + 2;  // stride of a Hessenberg column in fg_h
     const int fb = (int)std::min<long>(kFlatBlocks, (N + kThreads - 1) / kThreads);
     k_state_reset<<<1, 1, 0, stream>>>(st, 0, o.tol_rel, o.tol_abs, 0.0);
     double *beta2 = fg_h + (long)m * H;  // ||r||^2 of the cycle's residual
-    const FgNorms nm{beta2, fg_h, H};
     auto resid = [&]() -> int {
         CHK(with_op<double>([&](auto op) {
             return timed(PK_IR_RESID, [&] {
@@ -1856,9 +1841,7 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
     info->res0 = st_host[0].outer_rnorm;
     info->bnorm = st_host[0].outer_bnorm;
     int total_inner = 0, steps = 0, status = 0;
-    // sc[q]: 1 / ||u_q|| of the cycle's (unnormalised) basis vectors, as the
-    // kernels form it (FgNorms)
-    std::vector<double> Hm((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), y(m), col(H), sc(m + 1);
+    std::vector<double> Hm((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), y(m), col(H);
     // With the multigrid preconditioner an inner solve usually ends within its
     // first batch of iterations. Its direction's FGMRES step (below) is then
     // enqueued right behind that batch, and one synchronisation reads both the
@@ -1870,7 +1853,6 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
     while (!st_host[0].outer_done && steps < o.max_outer && total_inner < o.max_iters) {
         std::fill(gv.begin(), gv.end(), 0.0);
         gv[0] = st_host[0].outer_rnorm;
-        sc[0] = 1.0 / gv[0];
         int k = 0;
         bool breakdown = false;
         for (int j = 0; j < m && steps < o.max_outer && total_inner < o.max_iters; ++j) {
@@ -1885,16 +1867,14 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
                 CHK(with_op<double>([&](auto op) {
                     return timed(PK_FG_MATVEC, [&] {
                         k_fg_matvec<decltype(op)><<<chunk_blocks(g), kColThreads, 0, stream>>>(
-                            op, g, st, partials, x32, fg_z[j], fg_v[0], fg_v[j + 1], hc, nm, j);
+                            op, g, st, partials, x32, fg_z[j], fg_v[0], fg_v[j + 1], hc);
                     });
                 }));
                 FgBasis vb{};
                 for (int i = 0; i <= j; ++i) vb.v[i] = fg_v[i];
                 CHK(timed(PK_FG_GS, [&] {
-                    if (j > 0) fg_dots(vb, nm, j, fg_v[j + 1], hc, fb);
-                    // the line and multigrid modes take the next direction's
-                    // r_hat from this pass (k_fg_inner_init)
-                    fg_orth(vb, nm, j, fg_v[j + 1], hc, fb, line ? rh32_alt : nullptr, fg_rho);
+                    if (j > 0) fg_dots(vb, j, fg_v[j + 1], hc, fb);
+                    fg_orth(vb, j, fg_v[j + 1], hc, fb);
                 }));
                 HIPCHK(hipGetLastError());
                 HIPCHK(hipMemcpyAsync(col.data(), hc, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, stream));
@@ -1904,10 +1884,10 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
             };
             bool stepped = false;  // col holds this direction's step
             {
-            // z_j ~ A^-1 u_j in FP32 (u_j = v_j / s_j, the basis kept
-            // unnormalised). An inner solve whose FP32 recurrence overflowed is
-            // run again from z = 0 with half the iterations it had reached,
-            // keeping a finite iterate.
+            // z_j ~ A^-1 v_j in FP32 (v_j normalised in place first). An inner
+            // solve whose FP32 recurrence overflowed is run again from z = 0 with
+            // half the iterations it had reached, keeping a finite iterate.
+            const double *vsq = j == 0 ? beta2 : fg_h + (long)(j - 1) * H + j;
             // inexact-Krylov relaxation: once the residual estimate is within
             // a few times the target, the inner solve of this direction only
             // needs to reduce its rhs by about target / estimate (with a
@@ -1919,18 +1899,12 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
             for (int attempt = 0;; ++attempt) {
                 // every inner solve starts from a fresh state with this solve's
                 // cap (o.inner_max_iters, halved on an overflow rerun)
-                if (line && j > 0 && attempt == 0) {
-                    // r_hat of u_j and its norm came with step j - 1's last pass
-                    std::swap(rh32, rh32_alt);
-                    k_fg_inner_init<<<1, 1, 0, stream>>>(st + 1, cap, tol_j, fg_rho);
-                } else {
-                    k_state_reset<<<1, 1, 0, stream>>>(st + 1, cap, 0.0, 0.0, tol_j);
-                    CHK(timed(PK_FG_SETUP, [&] {
-                        k_fg_setup<<<fb, kThreads, 0, stream>>>(N, st + 1, fg_part, fg_v[j],
-                                                                line ? nullptr : dinv32, line ? nullptr : x32,
-                                                                r32, rh32, p32, ph32[0]);
-                    }));
-                }
+                k_state_reset<<<1, 1, 0, stream>>>(st + 1, cap, 0.0, 0.0, tol_j);
+                CHK(timed(PK_FG_SETUP, [&] {
+                    k_fg_setup<<<fb, kThreads, 0, stream>>>(N, st + 1, fg_part, vsq, attempt == 0, fg_v[j],
+                                                            line ? nullptr : dinv32, line ? nullptr : x32,
+                                                            r32, rh32, p32, ph32[0]);
+                }));
                 HIPCHK(hipGetLastError());
                 int done_iters = 0;
                 if (ahead && attempt == 0) {
@@ -1974,7 +1948,6 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
             // Hessenberg column j, previous Givens rotations, new rotation
             for (int i = 0; i <= j; ++i) Hm[(size_t)i * m + j] = col[i];
             const double hn = sqrt(std::max(col[j + 1], 0.0));
-            sc[j + 1] = 1.0 / hn;
             for (int i = 0; i < j; ++i) {
                 const double a = Hm[(size_t)i * m + j], c2 = Hm[(size_t)(i + 1) * m + j];
                 Hm[(size_t)i * m + j] = cs[i] * a + sn[i] * c2;
@@ -2003,8 +1976,6 @@ int windhip_solver::run_mixed(const windhip_solve_opts &o, windhip_solve_info *i
             for (int q = i + 1; q < k; ++q) acc -= Hm[(size_t)i * m + q] * y[q];
             y[i] = Hm[(size_t)i * m + i] != 0 ? acc / Hm[(size_t)i * m + i] : 0.0;
         }
-        // the stored directions solve for u_q = v_q / s_q: lambda += sum (y_q s_q) z_q
-        for (int q = 0; q < k; ++q) y[q] *= sc[q];
         HIPCHK(hipMemcpyAsync(fg_y, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, stream));
         CHK(timed(PK_IR_ACC, [&] {
             k_fg_update<<<flat_grid(N), kBlock, 0, stream>>>(N, x, (const float *const *)fg_zs, fg_y, k);

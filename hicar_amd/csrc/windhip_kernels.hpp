@@ -1597,41 +1597,19 @@ __global__ __launch_bounds__(kColThreads) void k_fg_resid(Op op, Geo g, KState *
     });
 }
 
-// The Arnoldi basis vectors one classical Gram-Schmidt pass reads.
-constexpr int kFgMaxBasis = 33;
-struct FgBasis {
-    const double *v[kFgMaxBasis];
-};
-
-// The Arnoldi basis is kept unnormalised: u_i = v_i / s_i. Its squared norms
-// are where the FGMRES already keeps them: ||u_0||^2 (the cycle's residual) in
-// n2_0, ||u_i||^2 (i >= 1, the orthogonalised w of step i - 1) in Hessenberg
-// column i - 1, row i (column stride H).
-struct FgNorms {
-    const double *n2_0, *h;
-    int H;
-    __device__ __forceinline__ double scale(int i) const
-    {
-        return 1.0 / sqrt(i == 0 ? *n2_0 : h[(long)(i - 1) * H + i]);
-    }
-};
-
-// w = s_j A z (FP64, z the FP32 inner solution widened on load) and h[0] =
-// <w, v_0>; z is kept in zk (FP32) for the final update (scaled by s_j there).
+// w = A z (FP64, z the FP32 inner solution widened on load) and h[0] =
+// <w, v0>; z is kept in zk (FP32) for the final update.
 template <class Op>
 __global__ __launch_bounds__(kColThreads) void k_fg_matvec(Op op, Geo g, KState *st, double *partials,
                                                            const float *__restrict__ z, float *__restrict__ zk,
                                                            const double *__restrict__ v0, double *__restrict__ w,
-                                                           double *h, FgNorms nm, int j)
+                                                           double *h)
 {
     const ChunkPos c = chunk_pos(g);
-    // the inner solve's z solves A z = u_j: the direction of v_j = s_j u_j is s_j z
-    const double sj = nm.scale(j);
     struct Epi {
         const double *v0;
         double *w;
         float *zk;
-        double sj;
         double acc[1];
         bool on;
         rsrc_t rv, rw, rz;
@@ -1641,15 +1619,13 @@ __global__ __launch_bounds__(kColThreads) void k_fg_matvec(Op op, Geo g, KState 
         __device__ void apply(int, unsigned so, unsigned vo, double ax, double xc, const Pre &p)
         {
             if (!on) return;
-            const double wv = sj * ax;
-            bstore(wv, rw, vo, so);
+            bstore(ax, rw, vo, so);
             bstore((float)xc, rz, vo >> 1, so >> 1);  // exact: xc is the FP32 value widened
-            acc[0] = acc[0] + wv * p.v;
+            acc[0] = acc[0] + ax * p.v;
         }
-    } epi{v0, w, zk, sj, {0.0}, c.t.valid};
+    } epi{v0, w, zk, {0.0}, c.t.valid};
     march_pipe(op, g, c.t, z, c.k0, c.k1, epi);
-    reduce_finalize<1, kColWaves>(epi.acc, partials, &st->cnt[3], c.L, c.nb,
-                                  [&](const double *tot) { h[0] = tot[0] * nm.scale(0); });  // <w, v_0>
+    reduce_finalize<1, kColWaves>(epi.acc, partials, &st->cnt[3], c.L, c.nb, [&](const double *tot) { h[0] = tot[0]; });
 }
 
 // out = y + w L^-1 in (y null: out = w L^-1 in), the Thomas algorithm per column
@@ -1987,62 +1963,45 @@ __global__ __launch_bounds__(kThreads) void k_fg_resid_zero(long N, KState *st, 
     });
 }
 
-// The FP32 BiCGStab's state for A z = u from z = 0 (bicgstab_solve set-up,
-// wind_iterative.F90:586-640, with r0 = u), from rr = r32.r32 (FP64 sum of
-// the FP32 copy of u).
-__device__ __forceinline__ void fg_inner_state(KState *st_in, double rr)
-{
-    st_in->rho = rr;
-    st_in->rnorm = sqrt(rr);
-    st_in->res0 = st_in->rnorm;
-    st_in->bnorm = st_in->rnorm;
-    st_in->target = fmax(1.0e-300, st_in->inner_tol * st_in->rnorm);
-    st_in->iter = 0;
-    st_in->omega = 1.0;
-    st_in->alpha = 1.0;
-    st_in->fmode = 0;
-    if (st_in->rnorm <= st_in->target) { st_in->status = 0; st_in->done = 1; }
-    else { st_in->status = 1; st_in->done = 0; }
-}
-
-// Start of one preconditioner application on the (unnormalised) basis vector
-// u: its FP32 copy as r_hat (and, for the Jacobi inner solve, r, p, the first
-// Jacobi sweep of p and z = 0; x32 null: only r_hat, the first iteration of
-// the line and multigrid inner solves reads r and p from it and does not read
-// x, k_update_xrp FIRST), and the inner state.
+// Start of one preconditioner application: scale the basis vector v by
+// 1/sqrt(*vsq) in place (FP64) when normalise is set, and set up the FP32
+// BiCGStab for A z = v from z = 0 (bicgstab_solve set-up,
+// wind_iterative.F90:586-640, with r0 = v). x32 null: only r_hat is written;
+// the first iteration of the line and multigrid inner solves reads r and p
+// from it and does not read x (k_update_xrp FIRST).
 __global__ __launch_bounds__(kThreads) void k_fg_setup(long N, KState *st_in, double *partials,
-                                                       const double *__restrict__ v, const float *__restrict__ dinv32,
-                                                       float *__restrict__ x32, float *__restrict__ r32,
-                                                       float *__restrict__ rh32, float *__restrict__ p32,
-                                                       float *__restrict__ ph32)
+                                                       const double *vsq, int normalise, double *__restrict__ v,
+                                                       const float *__restrict__ dinv32, float *__restrict__ x32,
+                                                       float *__restrict__ r32, float *__restrict__ rh32,
+                                                       float *__restrict__ p32, float *__restrict__ ph32)
 {
+    const double sc = normalise ? 1.0 / sqrt(*vsq) : 1.0;
     double acc[1] = {0.0};
     for (long n = (long)blockIdx.x * kThreads + threadIdx.x; n < N; n += (long)gridDim.x * kThreads) {
-        const float rf = (float)v[n];
+        const double vv = v[n] * sc;
+        if (normalise) v[n] = vv;
+        const float rf = (float)vv;
         rh32[n] = rf;
-        if (x32) {
+        if (x32) {  // null: the line and multigrid inner solves start from r_hat alone
             x32[n] = 0.0f;
             r32[n] = rf; p32[n] = rf;
         }
         if (dinv32) ph32[n] = dinv32[n] * rf;  // the first Jacobi sweep of p (no dinv32: unused)
         acc[0] = acc[0] + (double)rf * (double)rf;
     }
-    reduce_finalize<1>(acc, partials, &st_in->cnt[3], blockIdx.x, gridDim.x,
-                       [&](const double *tot) { fg_inner_state(st_in, tot[0]); });
-}
-
-// The inner state of a direction whose r_hat the last Gram-Schmidt pass wrote
-// (k_fg_orth): the reset of k_state_reset, then the set-up of k_fg_setup.
-__global__ void k_fg_inner_init(KState *st_in, int max_iters, double inner_tol, const double *rho)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        KState z = {};
-        z.max_iters = max_iters;
-        z.inner_tol = inner_tol;
-        z.status = 1;
-        *st_in = z;
-        fg_inner_state(st_in, *rho);
-    }
+    reduce_finalize<1>(acc, partials, &st_in->cnt[3], blockIdx.x, gridDim.x, [&](const double *tot) {
+        st_in->rho = tot[0];
+        st_in->rnorm = sqrt(tot[0]);
+        st_in->res0 = st_in->rnorm;
+        st_in->bnorm = st_in->rnorm;
+        st_in->target = fmax(1.0e-300, st_in->inner_tol * st_in->rnorm);
+        st_in->iter = 0;
+        st_in->omega = 1.0;
+        st_in->alpha = 1.0;
+        st_in->fmode = 0;
+        if (st_in->rnorm <= st_in->target) { st_in->status = 0; st_in->done = 1; }
+        else { st_in->status = 1; st_in->done = 0; }
+    });
 }
 
 // out = a x + b y (out may alias x or y)
@@ -2071,15 +2030,18 @@ __global__ __launch_bounds__(kThreads) void k_fg_axpy_dot(long N, KState *st, do
                        [&](const double *tot) { h[i + 1] = tot[0]; });
 }
 
+// The Arnoldi basis vectors one classical Gram-Schmidt pass reads.
+constexpr int kFgMaxBasis = 33;
+struct FgBasis {
+    const double *v[kFgMaxBasis];
+};
 
 // Classical Gram-Schmidt, first pass: h[i0 + q] = <w, v_{i0+q}> for the C
 // basis vectors q < C, all from one read of w (C a compile-time count, so
 // every load is unconditional and a row's loads are in flight together).
-// The basis is stored unnormalised (FgNorms).
 template <int C>
 __global__ __launch_bounds__(kThreads) void k_fg_dots(long N, KState *st, double *partials,
-                                                      const double *__restrict__ w, FgBasis vb, FgNorms nm, int i0,
-                                                      double *h)
+                                                      const double *__restrict__ w, FgBasis vb, int i0, double *h)
 {
     const double *v[C];
 #pragma unroll
@@ -2094,46 +2056,35 @@ __global__ __launch_bounds__(kThreads) void k_fg_dots(long N, KState *st, double
     }
     reduce_finalize<C>(acc, partials, &st->cnt[3], blockIdx.x, gridDim.x, [&](const double *tot) {
 #pragma unroll
-        for (int q = 0; q < C; ++q) h[i0 + q] = tot[q] * nm.scale(i0 + q);  // <w, v_i> = s_i <w, u_i>
+        for (int q = 0; q < C; ++q) h[i0 + q] = tot[q];
     });
 }
 
 // Classical Gram-Schmidt, second pass, over the C basis vectors from i0:
-// w -= sum_q h[i0 + q] v_{i0+q} (in that order; v_i = s_i u_i). The pass that
-// ends at v_j (last) also forms h[j+1] = ||w||^2 and, for the next direction
-// (u_{j+1} = w, kept unnormalised), its FP32 copy r32 and sum r32.r32 in
-// *rho: the inner solve's set-up (k_fg_inner_init).
+// w -= sum_q h[i0 + q] v_{i0+q} (in that order); the pass that ends at v_j
+// (last) also forms h[j+1] = ||w||^2.
 template <int C>
 __global__ __launch_bounds__(kThreads) void k_fg_orth(long N, KState *st, double *partials, double *__restrict__ w,
-                                                      FgBasis vb, FgNorms nm, int i0, int j, int last, double *h,
-                                                      float *__restrict__ r32, double *rho)
+                                                      FgBasis vb, int i0, int j, int last, double *h)
 {
     const double *v[C];
     double c[C];
 #pragma unroll
     for (int q = 0; q < C; ++q) {
         v[q] = vb.v[i0 + q];
-        c[q] = h[i0 + q] * nm.scale(i0 + q);
+        c[q] = h[i0 + q];
     }
-    const bool copy = last && r32 != nullptr;  // r32 null: no copy (rho unused)
-    double acc[2] = {0.0, 0.0};
+    double acc[1] = {0.0};
     for (long n = (long)blockIdx.x * kThreads + threadIdx.x; n < N; n += (long)gridDim.x * kThreads) {
         double wv = w[n];
 #pragma unroll
         for (int q = 0; q < C; ++q) wv = wv - c[q] * v[q][n];
         w[n] = wv;
         acc[0] = acc[0] + wv * wv;
-        if (copy) {
-            const float rf = (float)wv;
-            r32[n] = rf;
-            acc[1] = acc[1] + (double)rf * (double)rf;
-        }
     }
     if (last)
-        reduce_finalize<2>(acc, partials, &st->cnt[3], blockIdx.x, gridDim.x, [&](const double *tot) {
-            h[j + 1] = tot[0];
-            if (copy) *rho = tot[1];
-        });
+        reduce_finalize<1>(acc, partials, &st->cnt[3], blockIdx.x, gridDim.x,
+                           [&](const double *tot) { h[j + 1] = tot[0]; });
 }
 
 // f(std::integral_constant<int, c>) for a count 1 <= c <= 8.
